@@ -1,0 +1,879 @@
+/* tpgp_engine: C17 host engine over the tpgp_hip device library (see include/tpgp_engine.h).
+ *
+ * Host responsibilities, per rotation of engine/src/tpgp_rev.c:535-771: argument validation in
+ * the reference's order and precedence, the trigonometric factors (libm, or MPFR-256 enclosures
+ * in validated mode), and the ledger update.  Everything that touches the terms -- branching,
+ * merging equal keys, truncation, the per-rotation sums, the FID single-site scan -- runs on the
+ * GPU through tpgp_hip.  There is no CPU implementation of the propagation in this library.
+ *
+ * Build: -std=c17 -fno-fast-math -ffp-contract=off, optionally -DTPGP_HAVE_MPFR.
+ */
+#include "../../include/tpgp_engine.h"
+#include "../../include/tpgp_hip.h"
+
+#include <fenv.h>
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef TPGP_HAVE_MPFR
+#include <mpfr.h>
+#endif
+
+/* ---- directed rounding (tpgp_rev.c:25-63) ------------------------------------------------- */
+static double upadd(double a, double b) {
+    if (a == 0) return b;
+    if (b == 0) return a;
+    return nextafter(a + b, INFINITY);
+}
+static double upmul(double a, double b) {
+    if (a == 0 || b == 0) return 0;
+    return nextafter(a * b, INFINITY);
+}
+static double upper_l2(const double *v, size_t n) {
+#ifdef TPGP_HAVE_MPFR
+    mpfr_t acc, t;
+    mpfr_init2(acc, 192);
+    mpfr_init2(t, 192);
+    mpfr_set_zero(acc, 0);
+    for (size_t k = 0; k < n; k++) {
+        mpfr_set_d(t, v[k], MPFR_RNDN);
+        mpfr_sqr(t, t, MPFR_RNDU);
+        mpfr_add(acc, acc, t, MPFR_RNDU);
+    }
+    mpfr_sqrt(acc, acc, MPFR_RNDU);
+    double out = mpfr_get_d(acc, MPFR_RNDU);
+    mpfr_clear(acc);
+    mpfr_clear(t);
+    return out;
+#else
+    double acc = 0;
+    for (size_t k = 0; k < n; k++) acc = upadd(acc, upmul(v[k], v[k]));
+    return acc == 0 ? 0 : nextafter(sqrt(acc), INFINITY);
+#endif
+}
+/* Upper bound of sqrt(S) from an upward-rounded S: one step above the rounded root. */
+static double upper_sqrt(double s) { return s == 0 ? 0 : nextafter(sqrt(s), INFINITY); }
+
+/* ---- statuses, capabilities, options (tpgp_rev.c:117-148) -------------------------------- */
+const char *tpgp_status_string(tpgp_status s) {
+    static const char *names[] = {"ok",
+                                  "invalid argument",
+                                  "allocation failed",
+                                  "active-term cap exceeded",
+                                  "nonfinite arithmetic or unsupported floating-point environment",
+                                  "backend unavailable",
+                                  "coefficient-removal budget exceeded"};
+    return s >= TPGP_OK && s <= TPGP_EBUDGET ? names[s] : "unknown status";
+}
+int tpgp_has_mpfr(void) {
+#ifdef TPGP_HAVE_MPFR
+    return 1;
+#else
+    return 0;
+#endif
+}
+int tpgp_has_openmp(void) { return 0; }
+tpgp_options tpgp_default_options(void) {
+    tpgp_options o = {0, 0, 1, TPGP_HIP, 0, 0, NULL, NULL};
+    return o;
+}
+void tpgp_ledger_init(tpgp_ledger *l, int validated) {
+    memset(l, 0, sizeof(*l));
+    l->arithmetic_enclosed = validated && tpgp_has_mpfr();
+}
+
+/* ---- host map container (tpgp_rev.c:64-116, 149-259) -------------------------------------- */
+static int mulz(size_t a, size_t b, size_t *o) {
+    if (b && a > SIZE_MAX / b) return 0;
+    *o = a * b;
+    return 1;
+}
+static int key_cmp(const uint64_t *a, const uint64_t *b, size_t W) {
+    for (size_t w = W; w-- > 0;)
+        if (a[w] != b[w]) return a[w] < b[w] ? -1 : 1;
+    return 0;
+}
+static int key_padding_ok(const tpgp_map *m, const uint64_t *key) {
+    size_t used = m->n_spins % 32;
+    return !used || !(key[m->n_words - 1] >> (2 * used));
+}
+static int map_ok(const tpgp_map *m) {
+    size_t nk, bytes;
+    return m && m->n_spins && m->n_spins <= SIZE_MAX - 31 && m->n_words == (m->n_spins + 31) / 32 &&
+           m->len <= m->capacity && mulz(m->capacity, m->n_words, &nk) &&
+           mulz(nk, sizeof(uint64_t), &bytes) && m->capacity <= SIZE_MAX / sizeof(double) &&
+           (!m->capacity || (m->keys && m->values));
+}
+static tpgp_status map_reserve(tpgp_map *m, size_t need) {
+    if (need <= m->capacity) return TPGP_OK;
+    size_t cap = m->capacity ? m->capacity : 8;
+    while (cap < need) cap = cap > SIZE_MAX / 2 ? need : 2 * cap;
+    size_t nk, bytes;
+    if (!mulz(cap, m->n_words, &nk) || !mulz(nk, sizeof(uint64_t), &bytes) ||
+        cap > SIZE_MAX / sizeof(double))
+        return TPGP_ENOMEM;
+    uint64_t *k = malloc(bytes);
+    double *v = malloc(cap * sizeof(double));
+    if (!k || !v) {
+        free(k);
+        free(v);
+        return TPGP_ENOMEM;
+    }
+    if (m->len) {
+        memcpy(k, m->keys, m->len * m->n_words * sizeof(uint64_t));
+        memcpy(v, m->values, m->len * sizeof(double));
+    }
+    free(m->keys);
+    free(m->values);
+    m->keys = k;
+    m->values = v;
+    m->capacity = cap;
+    return TPGP_OK;
+}
+tpgp_status tpgp_map_init(tpgp_map *m, size_t n) {
+    if (!m || !n || n > SIZE_MAX - 31) return TPGP_EINVAL;
+    memset(m, 0, sizeof(*m));
+    m->n_spins = n;
+    m->n_words = (n + 31) / 32;
+    return TPGP_OK;
+}
+void tpgp_map_free(tpgp_map *m) {
+    if (!m) return;
+    free(m->keys);
+    free(m->values);
+    memset(m, 0, sizeof(*m));
+}
+tpgp_status tpgp_map_clone(tpgp_map *dst, const tpgp_map *src) {
+    if (!dst || !map_ok(src)) return TPGP_EINVAL;
+    if (dst == src) return TPGP_OK;
+    tpgp_map t;
+    tpgp_status s = tpgp_map_init(&t, src->n_spins);
+    if (s) return s;
+    if ((s = map_reserve(&t, src->len))) {
+        tpgp_map_free(&t);
+        return s;
+    }
+    t.len = src->len;
+    if (src->len) {
+        memcpy(t.keys, src->keys, src->len * src->n_words * sizeof(uint64_t));
+        memcpy(t.values, src->values, src->len * sizeof(double));
+    }
+    tpgp_map_free(dst);
+    *dst = t;
+    return TPGP_OK;
+}
+unsigned tpgp_key_get(const uint64_t *key, size_t site) {
+    return (unsigned)(key[site / 32] >> (2 * (site % 32))) & 3u;
+}
+void tpgp_key_set(uint64_t *key, size_t site, unsigned p) {
+    size_t w = site / 32, sh = 2 * (site % 32);
+    key[w] = (key[w] & ~((uint64_t)3 << sh)) | ((uint64_t)(p & 3u) << sh);
+}
+static size_t lower_bound_key(const tpgp_map *m, const uint64_t *key, int *found) {
+    size_t lo = 0, hi = m->len;
+    while (lo < hi) {
+        size_t mid = lo + (hi - lo) / 2;
+        if (key_cmp(m->keys + mid * m->n_words, key, m->n_words) < 0)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    *found = lo < m->len && key_cmp(m->keys + lo * m->n_words, key, m->n_words) == 0;
+    return lo;
+}
+tpgp_status tpgp_map_add(tpgp_map *m, const uint64_t *key, double v) {
+    if (!map_ok(m) || !key || !isfinite(v) || !key_padding_ok(m, key)) return TPGP_EINVAL;
+    if (v == 0) return TPGP_OK;
+    const size_t W = m->n_words;
+    int found;
+    size_t at = lower_bound_key(m, key, &found);
+    if (found) {
+        double sum = m->values[at] + v;
+        if (!isfinite(sum)) return TPGP_ENUMERIC;
+        if (sum != 0) {
+            m->values[at] = sum;
+            return TPGP_OK;
+        }
+        memmove(m->keys + at * W, m->keys + (at + 1) * W, (m->len - at - 1) * W * 8);
+        memmove(m->values + at, m->values + at + 1, (m->len - at - 1) * 8);
+        m->len--;
+        return TPGP_OK;
+    }
+    uint64_t saved[64];
+    uint64_t *copy = W <= 64 ? saved : malloc(W * 8);
+    if (!copy) return TPGP_ENOMEM;
+    memcpy(copy, key, W * 8);
+    tpgp_status s = map_reserve(m, m->len + 1);
+    if (!s) {
+        memmove(m->keys + (at + 1) * W, m->keys + at * W, (m->len - at) * W * 8);
+        memmove(m->values + at + 1, m->values + at, (m->len - at) * 8);
+        memcpy(m->keys + at * W, copy, W * 8);
+        m->values[at] = v;
+        m->len++;
+    }
+    if (copy != saved) free(copy);
+    return s;
+}
+tpgp_status tpgp_map_add_single(tpgp_map *m, size_t site, unsigned p, double v) {
+    if (!m || site >= m->n_spins || p > 3) return TPGP_EINVAL;
+    uint64_t *key = calloc(m->n_words, sizeof(uint64_t));
+    if (!key) return TPGP_ENOMEM;
+    tpgp_key_set(key, site, p);
+    tpgp_status s = tpgp_map_add(m, key, v);
+    free(key);
+    return s;
+}
+double tpgp_map_get(const tpgp_map *m, const uint64_t *key) {
+    int found;
+    size_t at = lower_bound_key(m, key, &found);
+    return found ? m->values[at] : 0;
+}
+double tpgp_map_l2(const tpgp_map *m) {
+    long double s = 0;
+    for (size_t k = 0; k < m->len; k++) s += (long double)m->values[k] * m->values[k];
+    return (double)sqrtl(s);
+}
+static void weight_and_symbol(const uint64_t *key, size_t W, unsigned *weight, unsigned *sym) {
+    *weight = 0;
+    *sym = 0;
+    for (size_t w = 0; w < W; w++)
+        for (uint64_t q = key[w]; q; q >>= 2)
+            if (q & 3u) {
+                ++*weight;
+                *sym = (unsigned)(q & 3u);
+            }
+}
+/* Compensated long-double sum of values in the given order (tpgp_rev.c:311-338). */
+static double neumaier(const double *v, size_t n) {
+    long double sum = 0, comp = 0;
+    for (size_t k = 0; k < n; k++) {
+        long double x = v[k], t = sum + x;
+        comp += fabsl(sum) >= fabsl(x) ? (sum - t) + x : (x - t) + sum;
+        sum = t;
+    }
+    return (double)(sum + comp);
+}
+/* Weight-1 values of symbol p in map (= site) order. */
+static size_t single_values(const tpgp_map *m, unsigned p, double *out) {
+    size_t n = 0;
+    for (size_t k = 0; k < m->len; k++) {
+        unsigned weight, sym;
+        weight_and_symbol(m->keys + k * m->n_words, m->n_words, &weight, &sym);
+        if (weight == 1 && sym == p) out[n++] = m->values[k];
+    }
+    return n;
+}
+double tpgp_map_single_sum(const tpgp_map *m, unsigned p) {
+    if (p < 1 || p > 3) return NAN;
+    double *v = malloc((m->n_spins + 1) * sizeof(double));
+    if (!v) return NAN;
+    double r = neumaier(v, single_values(m, p, v));
+    free(v);
+    return r;
+}
+/* MPFR-192 readout of values[0..n) / normalizer (tpgp_rev.c:339-388). */
+static tpgp_status readout_values(const double *v, size_t n, size_t normalizer, double *midpoint,
+                                  double *radius) {
+#ifdef TPGP_HAVE_MPFR
+    mpfr_t lo, hi, mid, x, den, r;
+    mpfr_inits2(192, lo, hi, mid, x, den, r, (mpfr_ptr)0);
+    mpfr_set_zero(lo, 0);
+    mpfr_set_zero(hi, 0);
+    mpfr_set_zero(mid, 0);
+    mpfr_set_uj(den, (uintmax_t)normalizer, MPFR_RNDN);
+    for (size_t k = 0; k < n; k++) {
+        mpfr_set_d(x, v[k], MPFR_RNDN);
+        mpfr_add(lo, lo, x, MPFR_RNDD);
+        mpfr_add(hi, hi, x, MPFR_RNDU);
+        mpfr_add(mid, mid, x, MPFR_RNDN);
+    }
+    mpfr_div(lo, lo, den, MPFR_RNDD);
+    mpfr_div(hi, hi, den, MPFR_RNDU);
+    mpfr_div(mid, mid, den, MPFR_RNDN);
+    double centre = mpfr_get_d(mid, MPFR_RNDN);
+    mpfr_set_d(mid, centre, MPFR_RNDN);
+    mpfr_sub(r, mid, lo, MPFR_RNDU);
+    mpfr_sub(x, hi, mid, MPFR_RNDU);
+    mpfr_max(r, r, x, MPFR_RNDU);
+    double rad = mpfr_get_d(r, MPFR_RNDU);
+    mpfr_clears(lo, hi, mid, x, den, r, (mpfr_ptr)0);
+    if (!isfinite(centre) || !isfinite(rad)) return TPGP_ENUMERIC;
+    *midpoint = centre;
+    *radius = rad;
+    return TPGP_OK;
+#else
+    (void)v;
+    (void)n;
+    (void)normalizer;
+    (void)midpoint;
+    (void)radius;
+    return TPGP_EBACKEND;
+#endif
+}
+tpgp_status tpgp_readout(const tpgp_map *m, unsigned p, size_t normalizer, double *midpoint,
+                         double *radius) {
+    if (!map_ok(m) || !normalizer || p < 1 || p > 3 || !midpoint || !radius) return TPGP_EINVAL;
+    double *v = malloc((m->n_spins + 1) * sizeof(double));
+    if (!v) return TPGP_ENOMEM;
+    tpgp_status s = readout_values(v, single_values(m, p, v), normalizer, midpoint, radius);
+    free(v);
+    return s;
+}
+unsigned tpgp_pauli_phase(unsigned q, unsigned p) {
+    if (q > 3 || p > 3) return 4;
+    if (!q || !p || q == p) return 0;
+    return ((p + 3 - q) % 3 == 1) ? 1u : 3u;
+}
+static int anticomm(unsigned q, unsigned p) { return q && p && q != p; }
+
+/* ---- trigonometric factors (tpgp_rev.c:484-534) ------------------------------------------- */
+static tpgp_status trig(double h, double dur, int validated, double *c, double *s, double *rc,
+                        double *rs) {
+    *rc = *rs = 0;
+#ifdef TPGP_HAVE_MPFR
+    if (validated) {
+        mpfr_t x, lo, hi, mid, a, b;
+        mpfr_inits2(256, x, lo, hi, mid, a, b, (mpfr_ptr)0);
+        mpfr_set_d(x, h, MPFR_RNDN);
+        mpfr_set_d(a, dur, MPFR_RNDN);
+        mpfr_mul(x, x, a, MPFR_RNDN);
+        mpfr_mul_2ui(x, x, 1, MPFR_RNDN);
+        for (int k = 0; k < 2; k++) {
+            int (*f)(mpfr_ptr, mpfr_srcptr, mpfr_rnd_t) = k ? mpfr_sin : mpfr_cos;
+            f(lo, x, MPFR_RNDD);
+            f(hi, x, MPFR_RNDU);
+            f(mid, x, MPFR_RNDN);
+            double centre = mpfr_get_d(mid, MPFR_RNDN);
+            mpfr_set_d(mid, centre, MPFR_RNDN);
+            mpfr_sub(a, mid, lo, MPFR_RNDU);
+            mpfr_sub(b, hi, mid, MPFR_RNDU);
+            mpfr_max(a, a, b, MPFR_RNDU);
+            double rad = mpfr_get_d(a, MPFR_RNDU);
+            if (k) {
+                *s = centre;
+                *rs = rad;
+            } else {
+                *c = centre;
+                *rc = rad;
+            }
+        }
+        mpfr_clears(x, lo, hi, mid, a, b, (mpfr_ptr)0);
+        return TPGP_OK;
+    }
+#else
+    if (validated) return TPGP_EBACKEND;
+#endif
+    double x = 2 * h * dur;
+    if (!isfinite(x)) return TPGP_ENUMERIC;
+    *c = cos(x);
+    *s = sin(x);
+    return TPGP_OK;
+}
+
+/* ---- rotation plans and the ledger fold ---------------------------------------------------- */
+typedef struct {
+    tpgp_status pre;   /* host-side status of this rotation (validation, trig) */
+    int noop;          /* h == 0 or duration == 0: nothing enqueued, nothing counted */
+    size_t i, j;
+    unsigned qi, qj;
+    double c, s, rc, rs;
+} rot_plan;
+
+static int ledger_ok(const tpgp_ledger *l) {
+    return isfinite(l->dropped_l1) && l->dropped_l1 >= 0 && isfinite(l->dropped_l2) &&
+           l->dropped_l2 >= 0 && isfinite(l->dropped_l2_squared) && l->dropped_l2_squared >= 0 &&
+           isfinite(l->arithmetic_l2) && l->arithmetic_l2 >= 0;
+}
+/* Host checks of rotate_product in the reference's order (tpgp_rev.c:538-561), excluding the
+ * ledger (checked during the fold, since it evolves) and the map-length no-op. */
+static void plan_rotation(rot_plan *p, size_t n_spins, size_t i, size_t j, unsigned qi,
+                          unsigned qj, double h, double dur, const tpgp_options *o) {
+    memset(p, 0, sizeof(*p));
+    p->i = i;
+    p->j = j;
+    p->qi = qi;
+    p->qj = qj;
+    if (!o || i >= n_spins || j >= n_spins || i == j || qi > 3 || qj > 3 || !isfinite(h) ||
+        !isfinite(dur) || !isfinite(o->threshold) || o->threshold < 0 ||
+        !isfinite(o->max_drop_l2) || o->max_drop_l2 < 0 || o->threads == 0) {
+        p->pre = TPGP_EINVAL;
+        return;
+    }
+    if (o->validated) {
+        volatile double tiny = DBL_MIN, half = .5;
+        if (FLT_RADIX != 2 || DBL_MANT_DIG != 53 || fegetround() != FE_TONEAREST ||
+            tiny * half == 0) {
+            p->pre = TPGP_ENUMERIC;
+            return;
+        }
+    }
+    if (o->backend != TPGP_HIP) {
+        p->pre = TPGP_EBACKEND;
+        return;
+    }
+    p->pre = trig(h, dur, o->validated, &p->c, &p->s, &p->rc, &p->rs);
+    if (p->pre) return;
+    p->noop = h == 0 || dur == 0;
+}
+
+enum { FOLD_CAPACITY = 100 };
+
+static int cmp_entry(const void *a, const void *b) {
+    const tpgp_hip_log_entry *x = a, *y = b;
+    if (x->rotation != y->rotation) return x->rotation < y->rotation ? -1 : 1;
+    if (x->kind != y->kind) return x->kind < y->kind ? -1 : 1;
+    if (x->key[1] != y->key[1]) return x->key[1] < y->key[1] ? -1 : 1;
+    if (x->key[0] != y->key[0]) return x->key[0] < y->key[0] ? -1 : 1;
+    return 0;
+}
+
+/* Apply one executed rotation's record to the trial ledger (tpgp_rev.c:730-753).  entries are
+ * this rotation's logged drops (kind 0) and radii (kind 1) in ascending reference key order. */
+static int fold_record(const tpgp_hip_record *r, const tpgp_hip_log_entry *ent, size_t n_ent,
+                       int exact, const tpgp_options *o, tpgp_ledger *trial, double *scratch) {
+    if (r->overflow) return FOLD_CAPACITY;
+    if (r->nonfinite) return TPGP_ENUMERIC;
+    if (o->max_terms && r->len_out > o->max_terms) return TPGP_ECAP;
+    double l1, sq, l2, ar;
+    if (exact) {
+        size_t nd = 0, nr = 0;
+        l1 = sq = 0;
+        for (size_t k = 0; k < n_ent; k++)
+            if (ent[k].kind == 0) {
+                l1 = upadd(l1, ent[k].value);
+                sq = upadd(sq, upmul(ent[k].aux, ent[k].aux));
+                scratch[nd++] = ent[k].value;
+            }
+        l2 = upper_l2(scratch, nd);
+        for (size_t k = 0; k < n_ent; k++)
+            if (ent[k].kind == 1) scratch[nr++] = ent[k].value;
+        ar = upper_l2(scratch, nr);
+        if (nd != r->dropped || nr != r->radii) return TPGP_EBACKEND;
+    } else {
+        l1 = r->l1_up;
+        sq = r->sq_up;
+        l2 = upper_sqrt(r->sq_up);
+        ar = upper_sqrt(r->rad_sq_up);
+    }
+    tpgp_ledger next = *trial;
+    next.dropped_l1 = upadd(next.dropped_l1, l1);
+    next.dropped_l2 = upadd(next.dropped_l2, l2);
+    next.dropped_l2_squared = upadd(next.dropped_l2_squared, sq);
+    next.arithmetic_l2 = upadd(next.arithmetic_l2, ar);
+    next.rotations++;
+    if (r->dropped) next.removal_events++;
+    if (r->len_out > next.peak_terms) next.peak_terms = (size_t)r->len_out;
+    next.arithmetic_enclosed = next.arithmetic_enclosed && o->validated && tpgp_has_mpfr();
+    if (!isfinite(next.dropped_l2) || !isfinite(next.arithmetic_l2)) return TPGP_ENUMERIC;
+    if (o->max_drop_l2 && next.dropped_l2 > o->max_drop_l2) return TPGP_EBUDGET;
+    *trial = next;
+    return TPGP_OK;
+}
+
+/* ---- a resident single-map session used by every entry point ----------------------------- */
+typedef struct {
+    tpgp_hip_context *ctx;
+    int own_ctx;
+    tpgp_hip_batch *batch;
+    size_t cap;
+    tpgp_hip_record *rec;
+    size_t max_rec;
+    tpgp_hip_log_entry *log;
+    size_t log_cap;
+    double *scratch;
+    size_t scratch_cap;
+} session;
+
+static void session_close(session *S) {
+    tpgp_hip_batch_destroy(S->batch);
+    if (S->own_ctx) tpgp_hip_destroy(S->ctx);
+    free(S->rec);
+    free(S->log);
+    free(S->scratch);
+    memset(S, 0, sizeof(*S));
+}
+static tpgp_status map_status(int hs) {
+    return hs == TPGP_HIP_ENOMEM ? TPGP_ENOMEM : hs == TPGP_HIP_EINVAL ? TPGP_EINVAL
+                                                                         : TPGP_EBACKEND;
+}
+static tpgp_status session_open(session *S, const tpgp_map *m, const tpgp_options *o,
+                                size_t cap_hint, size_t max_rec) {
+    memset(S, 0, sizeof(*S));
+    char err[512];
+    S->ctx = o->hip_context;
+    if (!S->ctx) {
+        S->ctx = tpgp_hip_create(0, err, sizeof(err));
+        if (!S->ctx) return TPGP_EBACKEND;
+        S->own_ctx = 1;
+    }
+    S->cap = cap_hint;
+    size_t n = m->n_spins;
+    S->batch = tpgp_hip_batch_create(S->ctx, m->n_words, 1, &n, &S->cap, err, sizeof(err));
+    if (!S->batch) {
+        session_close(S);
+        return TPGP_ENOMEM;
+    }
+    S->max_rec = max_rec;
+    S->rec = calloc(max_rec ? max_rec : 1, sizeof(*S->rec));
+    if (!S->rec) {
+        session_close(S);
+        return TPGP_ENOMEM;
+    }
+    int hs = tpgp_hip_batch_upload(S->batch, 0, m->keys, m->values, m->len, err, sizeof(err));
+    if (hs) {
+        session_close(S);
+        return map_status(hs);
+    }
+    return TPGP_OK;
+}
+static tpgp_status session_grow(session *S, size_t need) {
+    char err[512];
+    size_t cap = S->cap * 2 > need ? S->cap * 2 : need;
+    int hs = tpgp_hip_batch_reserve(S->batch, 0, cap, err, sizeof(err));
+    if (hs) return map_status(hs);
+    S->cap = cap;
+    return TPGP_OK;
+}
+static tpgp_status session_download(session *S, tpgp_map *m) {
+    char err[512];
+    size_t len = 0;
+    int hs = tpgp_hip_batch_download(S->batch, 0, NULL, NULL, 0, &len, err, sizeof(err));
+    if (hs) return map_status(hs);
+    tpgp_map t;
+    tpgp_status s = tpgp_map_init(&t, m->n_spins);
+    if (s) return s;
+    if ((s = map_reserve(&t, len))) {
+        tpgp_map_free(&t);
+        return s;
+    }
+    hs = tpgp_hip_batch_download(S->batch, 0, t.keys, t.values, len, &t.len, err, sizeof(err));
+    if (hs) {
+        tpgp_map_free(&t);
+        return map_status(hs);
+    }
+    tpgp_map_free(m);
+    *m = t;
+    return TPGP_OK;
+}
+
+/* Run planned rotations [0, n) on the session's working state; fold them into *trial in order.
+ * Returns the first failing status (the reference stops at its first failing rotation). */
+static tpgp_status run_plans(session *S, const rot_plan *plans, size_t n, int exact,
+                             const tpgp_options *o, tpgp_ledger *trial, uint32_t seq0,
+                             int *capacity_needed, size_t *need) {
+    char err[512];
+    *capacity_needed = 0;
+    /* the device runs the valid prefix; a host-invalid rotation ends the sequence */
+    size_t limit = n;
+    for (size_t k = 0; k < n; k++)
+        if (plans[k].pre) {
+            limit = k;
+            break;
+        }
+    size_t enq = 0;
+    for (size_t k = 0; k < limit; k++) {
+        if (plans[k].noop) continue;
+        tpgp_hip_rotation r;
+        memset(&r, 0, sizeof(r));
+        r.active = 1;
+        r.i = plans[k].i;
+        r.j = plans[k].j;
+        r.qi = plans[k].qi;
+        r.qj = plans[k].qj;
+        r.c = plans[k].c;
+        r.s = plans[k].s;
+        r.rc = plans[k].rc;
+        r.rs = plans[k].rs;
+        r.threshold = o->threshold;
+        r.validated = o->validated;
+        r.log = exact;
+        int hs = tpgp_hip_batch_rotate(S->batch, &r, seq0 + (uint32_t)k, err, sizeof(err));
+        if (hs) return map_status(hs);
+        enq++;
+    }
+    size_t nrec = 0, nlog = 0;
+    if (exact && !S->log) {
+        S->log_cap = 1u << 20;
+        S->log = malloc(S->log_cap * sizeof(*S->log));
+        if (!S->log) return TPGP_ENOMEM;
+    }
+    int hs = tpgp_hip_batch_sync(S->batch, S->rec, S->max_rec, &nrec, NULL, 0, NULL, NULL,
+                                 exact ? S->log : NULL, exact ? S->log_cap : 0, &nlog, err,
+                                 sizeof(err));
+    if (hs) return hs == TPGP_HIP_ECAPACITY ? TPGP_EBACKEND : map_status(hs);
+    if (nrec != enq) return TPGP_EBACKEND;
+    if (exact && nlog) qsort(S->log, nlog, sizeof(*S->log), cmp_entry);
+    if (S->scratch_cap < nlog + 1) {
+        free(S->scratch);
+        S->scratch_cap = nlog + 1;
+        S->scratch = malloc(S->scratch_cap * sizeof(double));
+        if (!S->scratch) return TPGP_ENOMEM;
+    }
+    size_t r_at = 0, e_at = 0;
+    for (size_t k = 0; k < n; k++) {
+        const rot_plan *p = plans + k;
+        if (!ledger_ok(trial)) return TPGP_EINVAL; /* tpgp_rev.c:543-547 */
+        if (p->pre) return p->pre;
+        if (p->noop) continue;
+        const tpgp_hip_record *r = S->rec + r_at++;
+        size_t e0 = e_at;
+        while (e_at < nlog && S->log[e_at].rotation == r->rotation) e_at++;
+        if (!r->executed) continue; /* empty map: no-op (tpgp_rev.c:562-563) */
+        int fs = fold_record(r, S->log + e0, e_at - e0, exact, o, trial, S->scratch);
+        if (fs == FOLD_CAPACITY) {
+            *capacity_needed = 1;
+            *need = (size_t)r->len_out;
+            return TPGP_OK;
+        }
+        if (fs) return (tpgp_status)fs;
+    }
+    return TPGP_OK;
+}
+
+/* Transactional sequence of rotations on a host map (tpgp_rotate / tpgp_strang_step). */
+static tpgp_status rotate_sequence(tpgp_map *m, const rot_plan *plans, size_t n,
+                                   const tpgp_options *o, tpgp_ledger *l) {
+    if (!map_ok(m) || !o || !l) return TPGP_EINVAL;
+    int any = 0;
+    for (size_t k = 0; k < n && !plans[k].pre; k++) any |= !plans[k].noop;
+    if (!any || m->len == 0) {
+        /* nothing reaches the device (every rotation is a no-op until the first host-side
+         * failure): replay the per-rotation checks in order */
+        for (size_t k = 0; k < n; k++) {
+            if (!ledger_ok(l)) return TPGP_EINVAL;
+            if (plans[k].pre) return plans[k].pre;
+        }
+        return TPGP_OK;
+    }
+    for (size_t k = 0; k < n; k++)
+        if (!plans[k].pre && !plans[k].noop && m->len > SIZE_MAX / 2) return TPGP_ENOMEM;
+    session S;
+    size_t cap = m->len * 2 + 1024;
+    tpgp_status s = session_open(&S, m, o, cap, n + 1);
+    if (s) return s;
+    for (;;) {
+        tpgp_ledger trial = *l;
+        int grow = 0;
+        size_t need = 0;
+        s = run_plans(&S, plans, n, 1, o, &trial, 0, &grow, &need);
+        char err[512];
+        if (!s && grow) {
+            tpgp_hip_batch_rollback(S.batch, err, sizeof(err));
+            if ((s = session_grow(&S, need + need / 2 + 1024))) break;
+            continue;
+        }
+        if (s) {
+            tpgp_hip_batch_rollback(S.batch, err, sizeof(err));
+            break;
+        }
+        if (tpgp_hip_batch_commit(S.batch, NULL, err, sizeof(err))) {
+            s = TPGP_EBACKEND;
+            break;
+        }
+        if ((s = session_download(&S, m))) break;
+        *l = trial;
+        break;
+    }
+    session_close(&S);
+    return s;
+}
+
+tpgp_status tpgp_rotate(tpgp_map *m, size_t i, size_t j, unsigned qi, unsigned qj, double theta,
+                        const tpgp_options *o, tpgp_ledger *l) {
+    if (!map_ok(m) || !o || !l) return TPGP_EINVAL;
+    rot_plan p;
+    plan_rotation(&p, m->n_spins, i, j, qi, qj, theta, 1.0, o);
+    return rotate_sequence(m, &p, 1, o, l);
+}
+
+tpgp_status tpgp_strang_step(tpgp_map *m, const tpgp_term *t, size_t n, double dt,
+                             const tpgp_options *o, tpgp_ledger *l) {
+    if (!m || (!t && n) || !o || !l || !isfinite(dt)) return TPGP_EINVAL;
+    double half = dt * 0.5;
+    if (half * 2 != dt) return TPGP_ENUMERIC; /* tpgp_rev.c:780-784 */
+    if (!map_ok(m)) return TPGP_EINVAL;
+    rot_plan *plans = malloc((2 * n + 1) * sizeof(*plans));
+    if (!plans) return TPGP_ENOMEM;
+    for (size_t k = 0; k < 2 * n; k++) {
+        const tpgp_term *term = k < n ? t + k : t + (2 * n - 1 - k);
+        plan_rotation(plans + k, m->n_spins, term->i, term->j, term->qi, term->qj, term->h, half,
+                      o);
+    }
+    tpgp_status s = rotate_sequence(m, plans, 2 * n, o, l);
+    free(plans);
+    return s;
+}
+
+/* ---- resident evolution ------------------------------------------------------------------- */
+tpgp_status tpgp_hip_evolve(tpgp_map *m, const tpgp_term *t, size_t n, double dt, size_t steps,
+                            const tpgp_options *o, const tpgp_evolve_options *eo, tpgp_ledger *l,
+                            double *fid, double *fid_radius, size_t *active, size_t *accepted) {
+    if (accepted) *accepted = 0;
+    if (!map_ok(m) || (!t && n) || !o || !l || !eo || !isfinite(dt) || eo->pauli < 1 ||
+        eo->pauli > 3 || !eo->normalizer || m->n_spins > 64)
+        return TPGP_EINVAL;
+    if (o->backend != TPGP_HIP) return TPGP_EBACKEND;
+    double half = dt * 0.5;
+    if (half * 2 != dt) return TPGP_ENUMERIC;
+    rot_plan *plans = malloc((2 * n + 1) * sizeof(*plans));
+    if (!plans) return TPGP_ENOMEM;
+    for (size_t k = 0; k < 2 * n; k++) {
+        const tpgp_term *term = k < n ? t + k : t + (2 * n - 1 - k);
+        plan_rotation(plans + k, m->n_spins, term->i, term->j, term->qi, term->qj, term->h, half,
+                      o);
+    }
+    size_t cap = eo->initial_capacity ? eo->initial_capacity : 4 * m->len + (1u << 16);
+    session S;
+    tpgp_status s = session_open(&S, m, o, cap, 2 * n + 2);
+    if (s) {
+        free(plans);
+        return s;
+    }
+    double rd[2 * 64];
+    char err[512];
+    size_t nrd = 0;
+    size_t step = 0;
+    /* readout of the committed state before step `step` */
+    int hs;
+    for (;;) {
+        hs = tpgp_hip_batch_readout(S.batch, eo->pauli, NULL, err, sizeof(err));
+        if (hs) {
+            s = map_status(hs);
+            break;
+        }
+        uint64_t len = 0;
+        size_t nrec = 0;
+        hs = tpgp_hip_batch_sync(S.batch, S.rec, S.max_rec, &nrec, rd, 2, &nrd, &len, NULL, 0,
+                                 NULL, err, sizeof(err));
+        if (hs) {
+            s = map_status(hs);
+            break;
+        }
+        double mid = 0, rad = NAN;
+        if (tpgp_has_mpfr()) {
+            s = readout_values(rd, m->n_spins, eo->normalizer, &mid, &rad);
+            if (s) break;
+        } else {
+            mid = neumaier(rd, m->n_spins) / (double)eo->normalizer;
+        }
+        if (fid) fid[step] = mid;
+        if (fid_radius) fid_radius[step] = rad;
+        if (active) active[step] = (size_t)len;
+        if (step == steps) break;
+        /* one Strang step; retried from the committed state if a slot overflows */
+        tpgp_ledger trial;
+        for (;;) {
+            trial = *l;
+            int grow = 0;
+            size_t need = 0;
+            s = run_plans(&S, plans, 2 * n, eo->exact_ledger, o, &trial, 0, &grow, &need);
+            if (!s && grow) {
+                tpgp_hip_batch_rollback(S.batch, err, sizeof(err));
+                if ((s = session_grow(&S, need + need / 2 + 1024))) break;
+                continue;
+            }
+            break;
+        }
+        if (s) {
+            tpgp_hip_batch_rollback(S.batch, err, sizeof(err));
+            break;
+        }
+        if (tpgp_hip_batch_commit(S.batch, NULL, err, sizeof(err))) {
+            s = TPGP_EBACKEND;
+            break;
+        }
+        *l = trial;
+        step++;
+        if (accepted) *accepted = step;
+    }
+    tpgp_status ds = session_download(&S, m);
+    if (!s) s = ds;
+    session_close(&S);
+    free(plans);
+    return s;
+}
+
+/* ---- Trotter bounds and allowance helpers (tpgp_rev.c:858-961; residual_rev.c:195-201) ----- */
+static int terms_anticommute(const tpgp_term *a, const tpgp_term *b) {
+    int parity = 0;
+    if (a->i == b->i) parity ^= anticomm(a->qi, b->qi);
+    if (a->i == b->j) parity ^= anticomm(a->qi, b->qj);
+    if (a->j == b->i) parity ^= anticomm(a->qj, b->qi);
+    if (a->j == b->j) parity ^= anticomm(a->qj, b->qj);
+    return parity;
+}
+double tpgp_strang_bound_taylor(const tpgp_term *t, size_t n, double dt) {
+    if ((!t && n) || !isfinite(dt)) return NAN;
+    double lambda = 0;
+    for (size_t k = 0; k < n; k++) {
+        if (!isfinite(t[k].h) || t[k].i == t[k].j || t[k].qi > 3 || t[k].qj > 3) return NAN;
+        lambda = upadd(lambda, fabs(t[k].h));
+    }
+    double x = upmul(lambda, fabs(dt));
+    return upmul(upmul(upmul(x, x), x), nextafter(2.0 / 3.0, INFINITY));
+}
+double tpgp_strang_bound(const tpgp_term *t, size_t n, double dt) {
+    double general = tpgp_strang_bound_taylor(t, n, dt);
+    if (isnan(general)) return NAN;
+    /* dense adjacency bitmap: fine for the term counts of these models */
+    unsigned char *adj = n ? calloc(n * n, 1) : NULL;
+    if (n && !adj) return general;
+    for (size_t a = 0; a < n; a++)
+        for (size_t b = a + 1; b < n; b++)
+            if (t[a].h && t[b].h && terms_anticommute(t + a, t + b)) adj[a * n + b] = adj[b * n + a] = 1;
+    double C = 0;
+    for (size_t a = 0; a < n; a++) {
+        double ha = fabs(t[a].h), inner = 0;
+        for (size_t c = a + 1; c < n; c++)
+            if (adj[a * n + c]) inner = upadd(inner, fabs(t[c].h));
+        C = upadd(C, upmul(upmul(upmul(ha, ha), inner), nextafter(1.0 / 6.0, INFINITY)));
+        for (size_t b = a + 1; b < n; b++) {
+            if (!adj[a * n + b]) continue;
+            double sum = 0;
+            /* terms anticommuting with exactly one of a, b, in ascending index order */
+            for (size_t c = a + 1; c < n; c++)
+                if (adj[a * n + c] != adj[b * n + c]) sum = upadd(sum, fabs(t[c].h));
+            C = upadd(C, upmul(upmul(upmul(ha, fabs(t[b].h)), sum), nextafter(1.0 / 3.0, INFINITY)));
+        }
+    }
+    free(adj);
+    double T = fabs(dt), nested = upmul(upmul(upmul(upmul(2, C), T), T), T);
+    return fmin(general, nested);
+}
+double tpgp_bound_add_up(double a, double b) {
+    if (isnan(a) || isnan(b) || a < 0 || b < 0) return NAN;
+#ifdef TPGP_HAVE_MPFR
+    if (isinf(a) || isinf(b)) return INFINITY;
+    mpfr_t x, y;
+    mpfr_inits2(256, x, y, (mpfr_ptr)0);
+    mpfr_set_d(x, a, MPFR_RNDN);
+    mpfr_set_d(y, b, MPFR_RNDN);
+    mpfr_add(x, x, y, MPFR_RNDU);
+    double z = mpfr_get_d(x, MPFR_RNDU);
+    mpfr_clears(x, y, (mpfr_ptr)0);
+    return z;
+#else
+    return upadd(a, b);
+#endif
+}
+double tpgp_bound_fid_up(double a, size_t n) {
+#ifdef TPGP_HAVE_MPFR
+    if (!n || a < 0 || isnan(a)) return NAN;
+    if (isinf(a)) return INFINITY;
+    mpfr_t x, y;
+    mpfr_inits2(256, x, y, (mpfr_ptr)0);
+    mpfr_set_d(x, a, MPFR_RNDN);
+    mpfr_set_uj(y, (uintmax_t)n, MPFR_RNDN);
+    mpfr_sqrt(y, y, MPFR_RNDD);
+    mpfr_div(x, x, y, MPFR_RNDU);
+    double z = mpfr_get_d(x, MPFR_RNDU);
+    mpfr_clears(x, y, (mpfr_ptr)0);
+    return z;
+#else
+    (void)a;
+    (void)n;
+    return NAN;
+#endif
+}

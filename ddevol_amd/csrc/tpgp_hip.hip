@@ -1,0 +1,2049 @@
+// tpgp_hip: MI355X (gfx950) Pauli-propagation branch engine.
+//
+// Replaces the reference's CUDA branch backend (engine/src/cuda_rev.cu) and moves the rest of
+// the per-rotation work of engine/src/tpgp_rev.c:535-771 (stable sort, merge, truncation, ledger
+// inputs) and the FID scan (tpgp_rev.c:311-388) onto the device, with the maps kept resident.
+//
+// Arithmetic: binary64 round-to-nearest only, no contraction (-ffp-contract=off), FP64
+// denormals preserved.  No trigonometric function is evaluated on the device.
+//
+// Device layout of a resident map.  A map is one sorted array of (key, value) pairs in a slot of
+// an HBM arena.  Keys are stored in a *permuted digit layout*: digit position p (bits 2p..2p+1 of
+// the 64*W-bit key, word p/32) holds the 2-bit symbol of site sigma[p].  Keys are sorted as
+// unsigned integers.  Before a rotation on sites (i, j) the layout is arranged so that i and j
+// occupy the two most significant live digits (positions N-1 and N-2).  Then the 16 symbol
+// classes (p_top, p_second) are 16 contiguous regions, a rotation pairs region c with region
+// c ^ q, and because XOR on the top digits preserves order inside a region, every output region
+// is a 2-way merge of two sorted runs.  Changing the active pair is a stable 16-bucket partition
+// that moves the new pair's digits to the top.  Coefficients of one output key come from at most
+// two raw branches (the key's own a-slot and its partner's d-slot), so the merged value is one
+// IEEE addition whose result does not depend on order: outputs are bit-identical to the
+// reference's stable merge sort.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/tpgp_hip.h"
+
+// ------------------------------------------------------------------------------------------
+// Scalar device helpers (same semantics as tpgp_rev.c:25-42 and cuda_rev.cu:50-76)
+// ------------------------------------------------------------------------------------------
+namespace {
+
+constexpr int kBlock = 256;        // 4 waves of 64
+constexpr int kMergeItems = 4;     // merge-path elements per thread
+constexpr int kMergeTile = kBlock * kMergeItems;
+constexpr int kPartRounds = 16;    // partition: 16 rounds of 256 elements
+constexpr int kPartTile = kBlock * kPartRounds;
+
+__device__ __forceinline__ uint64_t dbits(double x) { return __builtin_bit_cast(uint64_t, x); }
+__device__ __forceinline__ double dfrom(uint64_t b) { return __builtin_bit_cast(double, b); }
+
+// Adjacent binary64 value toward +inf by bit arithmetic (zero maps to the smallest subnormal).
+__device__ __forceinline__ double next_up(double x) {
+    if (x != x || x == __builtin_inf()) return x;
+    if (x == 0.0) return dfrom(1ull);
+    uint64_t b = dbits(x);
+    return dfrom(x > 0.0 ? b + 1 : b - 1);
+}
+__device__ __forceinline__ double next_down(double x) { return -next_up(-x); }
+__device__ __forceinline__ double upadd(double a, double b) {
+    if (a == 0.0) return b;
+    if (b == 0.0) return a;
+    return next_up(a + b);
+}
+__device__ __forceinline__ double upmul(double a, double b) {
+    if (a == 0.0 || b == 0.0) return 0.0;
+    return next_up(a * b);
+}
+__device__ __forceinline__ double round_radius(double x) {
+    if (!__builtin_isfinite(x)) return __builtin_inf();
+    double above = __builtin_fabs(next_up(x) - x);
+    double below = __builtin_fabs(x - next_down(x));
+    return __builtin_fmax(above, below);
+}
+__device__ __forceinline__ bool anticomm(unsigned q, unsigned p) { return q && p && q != p; }
+// Exponent of i in QP (tpgp_rev.c:414-418): cyclic X->Y->Z gives +i (1), reverse -i (3).
+__device__ __forceinline__ unsigned pauli_phase(unsigned q, unsigned p) {
+    if (!q || !p || q == p) return 0;
+    return ((p + 3 - q) % 3 == 1) ? 1u : 3u;
+}
+
+// ------------------------------------------------------------------------------------------
+// Keys of W 64-bit words (W=1: N<=32, W=2: N<=64), word W-1 most significant.
+// ------------------------------------------------------------------------------------------
+template <int W> struct Key { uint64_t w[W]; };
+
+template <int W> __device__ __forceinline__ Key<W> kload(const uint64_t *p) {
+    Key<W> k;
+#pragma unroll
+    for (int i = 0; i < W; ++i) k.w[i] = p[i];
+    return k;
+}
+template <int W> __device__ __forceinline__ void kstore(uint64_t *p, const Key<W> &k) {
+#pragma unroll
+    for (int i = 0; i < W; ++i) p[i] = k.w[i];
+}
+template <int W> __device__ __forceinline__ bool klt(const Key<W> &a, const Key<W> &b) {
+#pragma unroll
+    for (int i = W - 1; i > 0; --i)
+        if (a.w[i] != b.w[i]) return a.w[i] < b.w[i];
+    return a.w[0] < b.w[0];
+}
+template <int W> __device__ __forceinline__ bool keq(const Key<W> &a, const Key<W> &b) {
+    bool e = true;
+#pragma unroll
+    for (int i = 0; i < W; ++i) e = e && a.w[i] == b.w[i];
+    return e;
+}
+template <int W> __device__ __forceinline__ Key<W> kxor(const Key<W> &a, const Key<W> &b) {
+    Key<W> r;
+#pragma unroll
+    for (int i = 0; i < W; ++i) r.w[i] = a.w[i] ^ b.w[i];
+    return r;
+}
+template <int W> __device__ __forceinline__ bool kzero(const Key<W> &a) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) o |= a.w[i];
+    return o == 0;
+}
+// Word selection by comparison rather than a dynamic array index (keeps keys in registers).
+template <int W> __device__ __forceinline__ unsigned kdigit(const Key<W> &a, unsigned p) {
+    uint64_t word = a.w[0];
+    if constexpr (W == 2) word = (p >> 5) ? a.w[1] : a.w[0];
+    return (unsigned)(word >> (2 * (p & 31))) & 3u;
+}
+template <int W> __device__ __forceinline__ Key<W> kdigit_mask(unsigned p, unsigned d) {
+    Key<W> r;
+    uint64_t bits = (uint64_t)d << (2 * (p & 31));
+    if constexpr (W == 1) {
+        r.w[0] = bits;
+    } else {
+        r.w[0] = (p >> 5) ? 0 : bits;
+        r.w[1] = (p >> 5) ? bits : 0;
+    }
+    return r;
+}
+
+// Move the digits at positions pa, pb to positions n-1 (pa's) and n-2 (pb's); the other live
+// digits keep their relative order, packed below.  n >= 2, pa != pb, both < n.
+__device__ __forceinline__ uint64_t remove_digit64(uint64_t k, unsigned p) {
+    uint64_t low = p ? (k & ((1ull << (2 * p)) - 1)) : 0;
+    uint64_t high = (k >> 2) & ~(p ? ((1ull << (2 * p)) - 1) : 0ull);
+    return low | high;
+}
+__device__ __forceinline__ unsigned __int128 remove_digit128(unsigned __int128 k, unsigned p) {
+    unsigned __int128 mask = p ? ((((unsigned __int128)1) << (2 * p)) - 1) : 0;
+    return (k & mask) | ((k >> 2) & ~mask);
+}
+template <int W>
+__device__ __forceinline__ Key<W> move_to_top(const Key<W> &k, unsigned pa, unsigned pb,
+                                              unsigned n) {
+    unsigned da = kdigit<W>(k, pa), db = kdigit<W>(k, pb);
+    unsigned hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
+    Key<W> r;
+    if constexpr (W == 1) {
+        uint64_t x = remove_digit64(remove_digit64(k.w[0], hi), lo);
+        x |= (uint64_t)da << (2 * (n - 1));
+        x |= (uint64_t)db << (2 * (n - 2));
+        r.w[0] = x;
+    } else {
+        unsigned __int128 x = ((unsigned __int128)k.w[1] << 64) | k.w[0];
+        x = remove_digit128(remove_digit128(x, hi), lo);
+        x |= (unsigned __int128)da << (2 * (n - 1));
+        x |= (unsigned __int128)db << (2 * (n - 2));
+        r.w[0] = (uint64_t)x;
+        r.w[1] = (uint64_t)(x >> 64);
+    }
+    return r;
+}
+
+// ------------------------------------------------------------------------------------------
+// Block-level helpers (256 threads = 4 waves of 64)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned lane_id() { return threadIdx.x & 63u; }
+__device__ __forceinline__ unsigned wave_id() { return threadIdx.x >> 6; }
+
+// Exclusive block scan of one unsigned per thread; returns the exclusive prefix, total in *tot.
+__device__ unsigned block_exclusive_scan(unsigned v, unsigned *scratch /*>= 4*/, unsigned *tot) {
+    unsigned x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned y = __shfl_up(x, d, 64);
+        if ((int)lane_id() >= d) x += y;
+    }
+    if (lane_id() == 63) scratch[wave_id()] = x;
+    __syncthreads();
+    unsigned base = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        unsigned s = scratch[w];
+        if ((unsigned)w < wave_id()) base += s;
+        all += s;
+    }
+    __syncthreads();
+    *tot = all;
+    return base + x - v;
+}
+
+// ------------------------------------------------------------------------------------------
+// Branch kernel (the tpgp_cuda_branches replacement).  Output slot order 2k (a) / 2k+1 (d).
+// flag bit 1: nonzero key padding; bit 2: nonfinite input, product or radius.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_branches(
+    const uint64_t *__restrict__ keys, const double *__restrict__ values, size_t count,
+    size_t words, size_t spins, size_t i, size_t j, unsigned qi, unsigned qj, double c, double s,
+    double rc, double rs, int enclose, uint64_t *__restrict__ out_keys,
+    double *__restrict__ out_values, double *__restrict__ out_radii, unsigned *flag) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const unsigned rem = (unsigned)(spins % 32);
+    unsigned local = 0;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
+        const uint64_t *src = keys + k * words;
+        uint64_t *a = out_keys + 2 * k * words, *d = a + words;
+        if (rem && (src[words - 1] >> (2 * rem))) local |= 1u;
+        for (size_t w = 0; w < words; ++w) a[w] = d[w] = src[w];
+        unsigned pi = (unsigned)(src[i >> 5] >> (2 * (i & 31))) & 3u;
+        unsigned pj = (unsigned)(src[j >> 5] >> (2 * (j & 31))) & 3u;
+        double v = values[k];
+        if (!__builtin_isfinite(v)) local |= 2u;
+        out_radii[2 * k] = 0.0;
+        out_radii[2 * k + 1] = 0.0;
+        if (anticomm(qi, pi) == anticomm(qj, pj)) {
+            out_values[2 * k] = v;
+            out_values[2 * k + 1] = 0.0;
+            continue;
+        }
+        unsigned ph = (pauli_phase(qi, pi) + pauli_phase(qj, pj)) & 3u;
+        d[i >> 5] ^= (uint64_t)qi << (2 * (i & 31));
+        d[j >> 5] ^= (uint64_t)qj << (2 * (j & 31));
+        double x = v * c, y = v * s;
+        out_values[2 * k] = x;
+        out_values[2 * k + 1] = ph == 1 ? y : -y;  // exact sign flip, as sign*y
+        double ra = 0.0, rd = 0.0;
+        if (enclose) {
+            ra = upadd(upmul(__builtin_fabs(v), rc), round_radius(x));
+            rd = upadd(upmul(__builtin_fabs(v), rs), round_radius(y));
+            out_radii[2 * k] = ra;
+            out_radii[2 * k + 1] = rd;
+        }
+        if (!__builtin_isfinite(x) || !__builtin_isfinite(y) || !__builtin_isfinite(ra) ||
+            !__builtin_isfinite(rd))
+            local |= 2u;
+    }
+    if (local) atomicOr(flag, local);
+}
+
+// ------------------------------------------------------------------------------------------
+// Resident batch: device descriptors
+// ------------------------------------------------------------------------------------------
+struct MapSlot {          // per map, fixed
+    uint64_t off;         // element offset of the slot in every arena
+    uint64_t cap;         // slot capacity (elements)
+    uint32_t n_spins;
+    uint32_t pad;
+};
+struct PassMap {          // per map, per pass (staged from the host)
+    uint32_t active;
+    uint32_t in_buf, out_buf;
+    uint32_t pa, pb;      // partition: positions moved to the top
+    uint32_t qt, qs;      // rotation: generator symbols on the top / second digit
+    uint32_t flags;       // bit0 validated, bit1 log
+    uint32_t rec_slot;    // record index (rotation) or readout index
+    uint32_t rot_seq;
+    double c, s, rc, rs, threshold;
+    uint32_t pos[64];     // readout: digit position of every site
+};
+struct TileLedger {
+    uint64_t dropped, radii;
+    double l1, sq, rsq;
+    uint32_t nonfinite, pad;
+};
+struct LogDev {
+    uint64_t key[2];
+    double value, aux;
+    uint32_t map, rotation, kind, pad;
+};
+
+struct Arena {
+    uint64_t *keys;
+    double *vals;
+};
+
+struct BatchDev {
+    Arena arena[3];
+    const MapSlot *slot;      // [M]
+    uint64_t *len;            // [M*3]
+    uint64_t *regions;        // [M*3*17]
+    uint32_t n_maps;
+    uint32_t max_rec, max_rd;
+    // scratch
+    uint64_t *tstart;         // partition [M+1] / rotation [M*16+1]
+    uint32_t *tile_hist;      // [maxT*16]
+    uint64_t *tile_base;      // [maxT*16]
+    uint64_t *map_hist;       // [M*16]
+    uint64_t *splits;         // [maxT+1]
+    uint64_t *status;         // [maxT]
+    uint64_t *class_out;      // [M*16]
+    TileLedger *tile_led;     // [maxT]
+    uint32_t *counter;        // [4]
+    uint32_t *overflow;       // [M]
+    tpgp_hip_record *rec;     // [M*max_rec]
+    double *rd;               // [M*max_rd*64]
+    LogDev *log;              // [log_cap]
+    uint64_t *log_count;      // [1]
+    uint64_t log_cap;
+};
+
+__device__ __forceinline__ uint32_t find_bucket(const uint64_t *start, uint32_t n, uint64_t g) {
+    // largest b in [0, n) with start[b] <= g  (start is nondecreasing, start[0] == 0)
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (start[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------------------------------------
+// Partition (change of active pair): plan -> count -> scan -> scatter
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap *pm) {
+    __shared__ unsigned scratch[8];
+    const uint32_t M = b.n_maps;
+    for (uint32_t t = threadIdx.x; t < M * 16; t += blockDim.x) b.map_hist[t] = 0;
+    // tiles per map, exclusive scan over maps in chunks of 256
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < M; base += kBlock) {
+        uint32_t m = base + threadIdx.x;
+        unsigned n = 0;
+        if (m < M && pm[m].active) {
+            uint64_t len = b.len[m * 3 + pm[m].in_buf];
+            n = (unsigned)((len + kPartTile - 1) / kPartTile);
+        }
+        unsigned tot;
+        unsigned ex = block_exclusive_scan(n, scratch, &tot);
+        if (m < M) b.tstart[m] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) b.tstart[M] = carry;
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap *pm) {
+    __shared__ unsigned hist[16];
+    const uint32_t M = b.n_maps;
+    const uint64_t ntiles = b.tstart[M];
+    for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
+        if (threadIdx.x < 16) hist[threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t m = find_bucket(b.tstart, M, g);
+        const PassMap &P = pm[m];
+        uint64_t len = b.len[m * 3 + P.in_buf];
+        uint64_t e0 = (g - b.tstart[m]) * kPartTile;
+        uint64_t e1 = e0 + kPartTile < len ? e0 + kPartTile : len;
+        const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[m].off * W;
+        for (uint64_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+            Key<W> k = kload<W>(keys + e * W);
+            unsigned cls = (kdigit<W>(k, P.pa) << 2) | kdigit<W>(k, P.pb);
+            atomicAdd(&hist[cls], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 16) {
+            b.tile_hist[g * 16 + threadIdx.x] = hist[threadIdx.x];
+            if (hist[threadIdx.x])
+                atomicAdd((unsigned long long *)&b.map_hist[m * 16 + threadIdx.x],
+                          (unsigned long long)hist[threadIdx.x]);
+        }
+        __syncthreads();
+    }
+}
+
+// One block per (map, class): exclusive scan of that class's tile counts over the map's tiles.
+__global__ __launch_bounds__(kBlock) void k_part_scan(BatchDev b, const PassMap *pm) {
+    __shared__ unsigned scratch[8];
+    const uint32_t m = blockIdx.x >> 4, cls = blockIdx.x & 15;
+    const PassMap &P = pm[m];
+    if (!P.active) return;
+    uint64_t class_base = 0;
+    for (unsigned c = 0; c < cls; ++c) class_base += b.map_hist[m * 16 + c];
+    const uint64_t t0 = b.tstart[m], t1 = b.tstart[m + 1];
+    uint64_t carry = class_base;
+    for (uint64_t base = t0; base < t1; base += kBlock) {
+        uint64_t g = base + threadIdx.x;
+        unsigned v = g < t1 ? b.tile_hist[g * 16 + cls] : 0;
+        unsigned tot;
+        unsigned ex = block_exclusive_scan(v, scratch, &tot);
+        if (g < t1) b.tile_base[g * 16 + cls] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        uint64_t *reg = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
+        reg[cls] = class_base;
+        if (cls == 15) {
+            reg[16] = carry;
+            b.len[m * 3 + P.out_buf] = carry;
+        }
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassMap *pm) {
+    __shared__ unsigned wave_cnt[4][16];
+    __shared__ uint64_t run[16];
+    const uint32_t M = b.n_maps;
+    const uint64_t ntiles = b.tstart[M];
+    for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
+        uint32_t m = find_bucket(b.tstart, M, g);
+        const PassMap &P = pm[m];
+        const uint32_t n = b.slot[m].n_spins;
+        uint64_t len = b.len[m * 3 + P.in_buf];
+        uint64_t e0 = (g - b.tstart[m]) * kPartTile;
+        uint64_t e1 = e0 + kPartTile < len ? e0 + kPartTile : len;
+        const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[m].off * W;
+        const double *vals = b.arena[P.in_buf].vals + b.slot[m].off;
+        uint64_t *okeys = b.arena[P.out_buf].keys + b.slot[m].off * W;
+        double *ovals = b.arena[P.out_buf].vals + b.slot[m].off;
+        if (threadIdx.x < 16) run[threadIdx.x] = b.tile_base[g * 16 + threadIdx.x];
+        __syncthreads();
+        for (uint64_t r0 = e0; r0 < e1; r0 += kBlock) {
+            uint64_t e = r0 + threadIdx.x;
+            bool live = e < e1;
+            Key<W> k;
+            double v = 0.0;
+            unsigned cls = 16;
+            if (live) {
+                k = kload<W>(keys + e * W);
+                v = vals[e];
+                cls = (kdigit<W>(k, P.pa) << 2) | kdigit<W>(k, P.pb);
+            }
+            // rank among equal classes in lower lanes of this wave
+            unsigned below = 0;
+            const uint64_t lanemask = (1ull << lane_id()) - 1ull;
+            for (unsigned c = 0; c < 16; ++c) {
+                uint64_t bal = __ballot(cls == c);
+                if (cls == c) below = (unsigned)__popcll(bal & lanemask);
+                if (lane_id() == 0) wave_cnt[wave_id()][c] = (unsigned)__popcll(bal);
+            }
+            __syncthreads();
+            if (live) {
+                uint64_t dst = run[cls] + below;
+                for (unsigned w = 0; w < wave_id(); ++w) dst += wave_cnt[w][cls];
+                Key<W> nk = move_to_top<W>(k, P.pa, P.pb, n);
+                kstore<W>(okeys + dst * W, nk);
+                ovals[dst] = v;
+            }
+            __syncthreads();
+            if (threadIdx.x < 16) {
+                unsigned add = 0;
+                for (unsigned w = 0; w < 4; ++w) add += wave_cnt[w][threadIdx.x];
+                run[threadIdx.x] += add;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Class regions of a canonical map whose top two digits are positions n-1, n-2 (after upload).
+template <int W>
+__global__ void k_regions(BatchDev b, uint32_t m, uint32_t buf) {
+    const uint64_t len = b.len[m * 3 + buf];
+    const uint32_t n = b.slot[m].n_spins;
+    const uint64_t *keys = b.arena[buf].keys + b.slot[m].off * W;
+    unsigned c = threadIdx.x;
+    if (c > 16) return;
+    uint64_t lo = 0, hi = len;  // first index whose class >= c
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        Key<W> k = kload<W>(keys + mid * W);
+        unsigned cls = (kdigit<W>(k, n - 1) << 2) | kdigit<W>(k, n - 2);
+        if (cls < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    b.regions[((uint64_t)m * 3 + buf) * 17 + c] = c == 16 ? len : lo;
+}
+
+// ------------------------------------------------------------------------------------------
+// Rotation: plan -> splits -> merge -> finish
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool class_anti(unsigned c, unsigned qt, unsigned qs) {
+    return anticomm(qt, c >> 2) != anticomm(qs, c & 3u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_rot_plan(BatchDev b, const PassMap *pm) {
+    __shared__ unsigned scratch[8];
+    const uint32_t M = b.n_maps, n = M * 16;
+    if (threadIdx.x == 0) b.counter[0] = 0;
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) b.class_out[t] = 0;
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < n; base += kBlock) {
+        uint32_t t = base + threadIdx.x;
+        unsigned tiles = 0;
+        if (t < n) {
+            uint32_t m = t >> 4, c = t & 15;
+            const PassMap &P = pm[m];
+            if (P.active) {
+                const uint64_t *reg = b.regions + ((uint64_t)m * 3 + P.in_buf) * 17;
+                uint64_t na = reg[c + 1] - reg[c];
+                uint64_t nd = 0;
+                if (class_anti(c, P.qt, P.qs)) {
+                    unsigned cq = c ^ ((P.qt << 2) | P.qs);
+                    nd = reg[cq + 1] - reg[cq];
+                }
+                tiles = (unsigned)((na + nd + kMergeTile - 1) / kMergeTile);
+            }
+        }
+        unsigned tot;
+        unsigned ex = block_exclusive_scan(tiles, scratch, &tot);
+        if (t < n) b.tstart[t] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) b.tstart[n] = carry;
+}
+
+struct MergeCtx {
+    uint32_t m, c, cq;
+    bool anti;
+    uint64_t A0, nA, D0, nD, k;
+};
+
+__device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm, uint64_t g,
+                                       MergeCtx &x) {
+    uint32_t t = find_bucket(b.tstart, b.n_maps * 16, g);
+    x.m = t >> 4;
+    x.c = t & 15;
+    const PassMap &P = pm[x.m];
+    const uint64_t *reg = b.regions + ((uint64_t)x.m * 3 + P.in_buf) * 17;
+    x.A0 = reg[x.c];
+    x.nA = reg[x.c + 1] - reg[x.c];
+    x.anti = class_anti(x.c, P.qt, P.qs);
+    x.cq = x.c ^ ((P.qt << 2) | P.qs);
+    x.D0 = reg[x.cq];
+    x.nD = x.anti ? reg[x.cq + 1] - reg[x.cq] : 0;
+    x.k = g - b.tstart[t];
+}
+
+// One wave per tile: merge-path split (number of A elements before diagonal d0), found with a
+// 64-ary search.  P(x) = A[x] > T(D[d0-x-1]); the split is the first x with P true.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap *pm) {
+    const uint64_t ntiles = b.tstart[b.n_maps * 16];
+    const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
+    for (uint64_t g = (uint64_t)blockIdx.x * (kBlock / 64) + wave_id(); g < ntiles; g += waves) {
+        MergeCtx x;
+        locate(b, pm, g, x);
+        const PassMap &P = pm[x.m];
+        const uint64_t d0 = x.k * kMergeTile;
+        if (!x.anti) {
+            if (lane_id() == 0) b.splits[g] = d0;
+            continue;
+        }
+        const uint32_t n = b.slot[x.m].n_spins;
+        const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
+        const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
+        uint64_t lo = d0 > x.nD ? d0 - x.nD : 0, hi = d0 < x.nA ? d0 : x.nA;
+        auto pred = [&](uint64_t xi) -> bool {
+            Key<W> a = kload<W>(keys + (x.A0 + xi) * W);
+            Key<W> d = kxor<W>(kload<W>(keys + (x.D0 + d0 - xi - 1) * W), q);
+            return klt<W>(d, a);
+        };
+        while (hi > lo) {
+            uint64_t span = hi - lo;
+            if (span <= 64) {
+                uint64_t xi = lo + lane_id();
+                bool p = xi < hi ? pred(xi) : true;
+                uint64_t bal = __ballot(p);
+                lo = lo + (uint64_t)(__ffsll((unsigned long long)bal) - 1);
+                hi = lo;
+                break;
+            }
+            uint64_t xi = lo + ((span * lane_id()) >> 6);
+            bool p = pred(xi);
+            uint64_t bal = __ballot(p);
+            if (bal == 0) {
+                lo = lo + ((span * 63) >> 6) + 1;
+            } else {
+                int f = __ffsll((unsigned long long)bal) - 1;
+                if (f == 0) {
+                    hi = lo;
+                    break;
+                }
+                uint64_t xf = lo + ((span * (uint64_t)f) >> 6);
+                uint64_t xp = lo + ((span * (uint64_t)(f - 1)) >> 6);
+                lo = xp + 1;
+                hi = xf;
+            }
+        }
+        if (lane_id() == 0) b.splits[g] = lo;
+    }
+}
+
+// Look-back status word: [epoch:24][flag:2][value:38]
+constexpr uint64_t kFlagAgg = 1, kFlagInc = 2;
+__device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint64_t flag, uint64_t v) {
+    return ((uint64_t)epoch << 40) | (flag << 38) | (v & ((1ull << 38) - 1));
+}
+__device__ __forceinline__ void lb_publish(uint64_t *p, uint64_t w) {
+    __hip_atomic_store(p, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t lb_wait(uint64_t *p, uint32_t epoch) {
+    for (;;) {
+        uint64_t w = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if ((uint32_t)(w >> 40) == epoch && ((w >> 38) & 3u)) return w;
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+template <int W> struct MergeShared {
+    uint64_t keys[(kMergeTile + 2) * W];  // inputs; after the merge, the compacted outputs
+    double vals[kMergeTile + 2];
+    uint64_t okeys[kMergeTile * W];       // outputs of thread t at [t*items, t*items + nout)
+    double ovals[kMergeTile];
+    unsigned scratch[8];
+    uint64_t tile;
+    uint64_t prefix;
+    double red[3][kBlock / 64];
+    uint64_t redn[2][kBlock / 64];
+    unsigned nonfinite;
+};
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_rot_merge(BatchDev b, const PassMap *pm,
+                                                     uint32_t epoch) {
+    __shared__ MergeShared<W> sh;
+    const uint64_t ntiles = b.tstart[b.n_maps * 16];
+    for (;;) {
+        if (threadIdx.x == 0) sh.tile = atomicAdd(&b.counter[0], 1u);
+        __syncthreads();
+        const uint64_t g = sh.tile;
+        __syncthreads();
+        if (g >= ntiles) return;
+        MergeCtx x;
+        locate(b, pm, g, x);
+        const PassMap &P = pm[x.m];
+        const uint32_t n = b.slot[x.m].n_spins;
+        const uint64_t total = x.nA + x.nD;
+        const uint64_t d0 = x.k * kMergeTile;
+        const uint64_t d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
+        const uint64_t tclass_end = b.tstart[x.m * 16 + x.c + 1];
+        const uint64_t a0 = b.splits[g];
+        const uint64_t a1 = (g + 1 < tclass_end) ? b.splits[g + 1] : x.nA;
+        // Guard: a split outside its range would overrun the tile buffers; report instead.
+        if (a0 > d0 || a1 > d1 || a1 < a0 || a0 > x.nA || a1 > x.nA || d0 - a0 > x.nD ||
+            d1 - a1 > x.nD || d1 - a1 < d0 - a0 || d1 - d0 > (uint64_t)kMergeTile) {
+            if (threadIdx.x == 0) {
+                atomicOr(&b.counter[1], 1u);
+                lb_publish(&b.status[g], lb_pack(epoch, b.tstart[x.m * 16] == g ? kFlagInc
+                                                                                : kFlagAgg, 0));
+            }
+            __syncthreads();
+            continue;
+        }
+        const uint64_t b0 = d0 - a0, b1 = d1 - a1;
+        const uint32_t na = (uint32_t)(a1 - a0), nd = (uint32_t)(b1 - b0);
+        const uint64_t *ikeys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
+        const double *ivals = b.arena[P.in_buf].vals + b.slot[x.m].off;
+        const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
+        // LDS: A at [0, na), D at [na, na+nd), D lookahead at na+nd, A predecessor at TILE+1
+        for (uint32_t t = threadIdx.x; t < na; t += kBlock) {
+            Key<W> k = kload<W>(ikeys + (x.A0 + a0 + t) * W);
+            kstore<W>(sh.keys + t * W, k);
+            sh.vals[t] = ivals[x.A0 + a0 + t];
+        }
+        for (uint32_t t = threadIdx.x; t < nd; t += kBlock) {
+            Key<W> k = kxor<W>(kload<W>(ikeys + (x.D0 + b0 + t) * W), q);
+            kstore<W>(sh.keys + (na + t) * W, k);
+            sh.vals[na + t] = ivals[x.D0 + b0 + t];
+        }
+        const bool has_next = x.anti && b1 < x.nD;
+        const bool has_prev = x.anti && a0 > 0;
+        if (threadIdx.x == 0) {
+            if (has_next) {
+                Key<W> k = kxor<W>(kload<W>(ikeys + (x.D0 + b1) * W), q);
+                kstore<W>(sh.keys + (na + nd) * W, k);
+                sh.vals[na + nd] = ivals[x.D0 + b1];
+            }
+            if (has_prev) kstore<W>(sh.keys + (kMergeTile + 1) * W,
+                                    kload<W>(ikeys + (x.A0 + a0 - 1) * W));
+            sh.nonfinite = 0;
+        }
+        __syncthreads();
+
+        // ---- per-thread merge of kMergeItems merge-path positions
+        const uint32_t nloc = (uint32_t)(d1 - d0);
+        const uint32_t dt = threadIdx.x * kMergeItems;
+        const double cc = P.c, ss = P.s, rc = P.rc, rs = P.rs, thr = P.threshold;
+        const bool validated = P.flags & 1u, want_log = (P.flags & 2u) != 0;
+        const unsigned ph = (pauli_phase(P.qt, x.cq >> 2) + pauli_phase(P.qs, x.cq & 3u)) & 3u;
+        const bool dneg = ph != 1;
+        const uint32_t nd_look = nd + (has_next ? 1u : 0u);
+        const uint32_t obase = threadIdx.x * kMergeItems;
+        unsigned nout = 0;
+        uint64_t n_drop = 0, n_rad = 0;
+        double l1 = 0.0, sq = 0.0, rsq = 0.0;
+        bool bad = false;
+        auto Akey = [&](uint32_t i) { return kload<W>(sh.keys + i * W); };
+        auto Dkey = [&](uint32_t j) { return kload<W>(sh.keys + (na + j) * W); };
+        auto emit = [&](const Key<W> &key, bool hasA, double va, bool hasD, double vd) {
+            double v, r = 0.0;
+            if (!x.anti) {
+                v = va;
+            } else {
+                double xa = 0.0, yd = 0.0, ra = 0.0, rd = 0.0;
+                if (hasA) {
+                    xa = va * cc;
+                    if (validated) ra = upadd(upmul(__builtin_fabs(va), rc), round_radius(xa));
+                }
+                if (hasD) {
+                    double y = vd * ss;
+                    yd = dneg ? -y : y;
+                    if (validated) rd = upadd(upmul(__builtin_fabs(vd), rs), round_radius(y));
+                    if (!__builtin_isfinite(y)) bad = true;
+                }
+                if (!__builtin_isfinite(xa) || !__builtin_isfinite(ra) ||
+                    !__builtin_isfinite(rd))
+                    bad = true;
+                if (hasA && hasD) {
+                    v = xa + yd;
+                    r = upadd(ra, rd);
+                    if (validated && xa != 0.0 && yd != 0.0) r = upadd(r, round_radius(v));
+                } else if (hasA) {
+                    v = xa;
+                    r = ra;
+                } else {
+                    v = yd;
+                    r = rd;
+                }
+            }
+            if (!__builtin_isfinite(v) || !__builtin_isfinite(r)) bad = true;
+            if (r != 0.0) {
+                n_rad++;
+                rsq = upadd(rsq, upmul(r, r));
+                if (want_log) {
+                    unsigned long long at = atomicAdd((unsigned long long *)b.log_count, 1ull);
+                    if (at < b.log_cap) {
+                        LogDev &L = b.log[at];
+                        L.key[0] = key.w[0];
+                        L.key[1] = W > 1 ? key.w[W - 1] : 0;
+                        L.value = r;
+                        L.aux = 0.0;
+                        L.map = x.m;
+                        L.rotation = P.rot_seq;
+                        L.kind = 1;
+                    }
+                }
+            }
+            if (v == 0.0) return;
+            if (!kzero<W>(key) && __builtin_fabs(v) <= thr) {
+                n_drop++;
+                l1 = upadd(l1, __builtin_fabs(v));
+                sq = upadd(sq, upmul(v, v));
+                if (want_log) {
+                    unsigned long long at = atomicAdd((unsigned long long *)b.log_count, 1ull);
+                    if (at < b.log_cap) {
+                        LogDev &L = b.log[at];
+                        L.key[0] = key.w[0];
+                        L.key[1] = W > 1 ? key.w[W - 1] : 0;
+                        L.value = __builtin_fabs(v);
+                        L.aux = v;
+                        L.map = x.m;
+                        L.rotation = P.rot_seq;
+                        L.kind = 0;
+                    }
+                }
+                return;
+            }
+            kstore<W>(sh.okeys + (obase + nout) * W, key);
+            sh.ovals[obase + nout] = v;
+            nout++;
+        };
+        if (dt < nloc) {
+            const uint32_t limit = (nloc - dt) < (uint32_t)kMergeItems ? (nloc - dt)
+                                                                        : (uint32_t)kMergeItems;
+            // local split: first i in [max(0,dt-nd), min(dt,na)] with A[i] > D[dt-i-1]
+            uint32_t lo = dt > nd ? dt - nd : 0, hi = dt < na ? dt : na;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (klt<W>(Dkey(dt - mid - 1), Akey(mid)))
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            uint32_t i = lo, j = dt - lo, used = 0;
+            // a D element equal to the A element just before this split was consumed with it
+            if (j < nd) {
+                bool prev_ok = i > 0 || has_prev;
+                if (prev_ok) {
+                    Key<W> pa = i > 0 ? Akey(i - 1) : kload<W>(sh.keys + (kMergeTile + 1) * W);
+                    if (keq<W>(pa, Dkey(j))) {
+                        ++j;
+                        ++used;
+                    }
+                }
+            }
+            while (used < limit) {
+                bool takeA;
+                if (i >= na)
+                    takeA = false;
+                else if (j >= nd)
+                    takeA = true;
+                else
+                    takeA = !klt<W>(Dkey(j), Akey(i));  // A[i] <= D[j]
+                if (takeA) {
+                    Key<W> ka = Akey(i);
+                    if (j < nd_look && keq<W>(ka, Dkey(j))) {
+                        emit(ka, true, sh.vals[i], true, sh.vals[na + j]);
+                        ++i;
+                        ++j;
+                        used += 2;
+                    } else {
+                        emit(ka, true, sh.vals[i], false, 0.0);
+                        ++i;
+                        ++used;
+                    }
+                } else {
+                    emit(Dkey(j), false, 0.0, true, sh.vals[na + j]);
+                    ++j;
+                    ++used;
+                }
+            }
+        }
+        // ---- tile aggregate, look-back within the map's tile chain
+        unsigned tot;
+        unsigned myoff = block_exclusive_scan(nout, sh.scratch, &tot);
+        if (threadIdx.x == 0) {
+            const uint64_t first = b.tstart[x.m * 16];
+            if (g == first) {
+                lb_publish(&b.status[g], lb_pack(epoch, kFlagInc, tot));
+                sh.prefix = 0;
+            } else {
+                lb_publish(&b.status[g], lb_pack(epoch, kFlagAgg, tot));
+                uint64_t acc = 0;
+                for (uint64_t p = g - 1;; --p) {
+                    uint64_t w = lb_wait(&b.status[p], epoch);
+                    acc += w & ((1ull << 38) - 1);
+                    if (((w >> 38) & 3u) == kFlagInc) break;
+                }
+                lb_publish(&b.status[g], lb_pack(epoch, kFlagInc, acc + tot));
+                sh.prefix = acc;
+            }
+        }
+        if (bad) atomicOr(&sh.nonfinite, 1u);
+        __syncthreads();
+        const uint64_t out_pos = sh.prefix;
+        const bool fits = out_pos + tot <= b.slot[x.m].cap;
+        // compact this thread's outputs into the (consumed) input area, then write coalesced
+        if (fits) {
+            for (unsigned t = 0; t < nout; ++t) {
+                kstore<W>(sh.keys + (myoff + t) * W, kload<W>(sh.okeys + (obase + t) * W));
+                sh.vals[myoff + t] = sh.ovals[obase + t];
+            }
+        }
+        __syncthreads();
+        if (fits) {
+            uint64_t *okeys = b.arena[P.out_buf].keys + (b.slot[x.m].off + out_pos) * W;
+            double *ovals = b.arena[P.out_buf].vals + b.slot[x.m].off + out_pos;
+            for (unsigned t = threadIdx.x; t < tot; t += kBlock) {
+                kstore<W>(okeys + (uint64_t)t * W, kload<W>(sh.keys + t * W));
+                ovals[t] = sh.vals[t];
+            }
+        } else if (threadIdx.x == 0) {
+            atomicOr(&b.overflow[x.m], 1u);
+        }
+        // ---- ledger partials: fixed-shape tree with upward rounding (deterministic)
+        for (int d = 1; d < 64; d <<= 1) {
+            double o1 = __shfl_down(l1, d, 64), o2 = __shfl_down(sq, d, 64),
+                   o3 = __shfl_down(rsq, d, 64);
+            uint64_t c1 = __shfl_down(n_drop, d, 64), c2 = __shfl_down(n_rad, d, 64);
+            if (lane_id() + d < 64) {
+                l1 = upadd(l1, o1);
+                sq = upadd(sq, o2);
+                rsq = upadd(rsq, o3);
+                n_drop += c1;
+                n_rad += c2;
+            }
+        }
+        if (lane_id() == 0) {
+            sh.red[0][wave_id()] = l1;
+            sh.red[1][wave_id()] = sq;
+            sh.red[2][wave_id()] = rsq;
+            sh.redn[0][wave_id()] = n_drop;
+            sh.redn[1][wave_id()] = n_rad;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            TileLedger tl;
+            tl.l1 = sh.red[0][0];
+            tl.sq = sh.red[1][0];
+            tl.rsq = sh.red[2][0];
+            tl.dropped = sh.redn[0][0];
+            tl.radii = sh.redn[1][0];
+            for (int w = 1; w < kBlock / 64; ++w) {
+                tl.l1 = upadd(tl.l1, sh.red[0][w]);
+                tl.sq = upadd(tl.sq, sh.red[1][w]);
+                tl.rsq = upadd(tl.rsq, sh.red[2][w]);
+                tl.dropped += sh.redn[0][w];
+                tl.radii += sh.redn[1][w];
+            }
+            tl.nonfinite = sh.nonfinite;
+            tl.pad = 0;
+            b.tile_led[g] = tl;
+            atomicAdd((unsigned long long *)&b.class_out[x.m * 16 + x.c],
+                      (unsigned long long)tot);
+        }
+        __syncthreads();
+    }
+}
+
+// One block per map: output regions, length, ledger fold in tile order, record.
+__global__ __launch_bounds__(kBlock) void k_rot_finish(BatchDev b, const PassMap *pm) {
+    const uint32_t m = blockIdx.x;
+    const PassMap &P = pm[m];
+    if (!P.active) return;
+    __shared__ double red[3][kBlock];
+    __shared__ uint64_t redn[2][kBlock];
+    __shared__ unsigned bad;
+    const uint64_t t0 = b.tstart[m * 16], t1 = b.tstart[m * 16 + 16];
+    double l1 = 0.0, sq = 0.0, rsq = 0.0;
+    uint64_t nd = 0, nr = 0;
+    unsigned nf = 0;
+    // each thread folds a contiguous run of tiles, then a fixed tree across threads
+    const uint64_t nt = t1 - t0;
+    const uint64_t per = (nt + kBlock - 1) / kBlock;
+    for (uint64_t t = t0 + threadIdx.x * per; t < t0 + (threadIdx.x + 1) * per && t < t1; ++t) {
+        const TileLedger &L = b.tile_led[t];
+        l1 = upadd(l1, L.l1);
+        sq = upadd(sq, L.sq);
+        rsq = upadd(rsq, L.rsq);
+        nd += L.dropped;
+        nr += L.radii;
+        nf |= L.nonfinite;
+    }
+    red[0][threadIdx.x] = l1;
+    red[1][threadIdx.x] = sq;
+    red[2][threadIdx.x] = rsq;
+    redn[0][threadIdx.x] = nd;
+    redn[1][threadIdx.x] = nr;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    if (nf) atomicOr(&bad, 1u);
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] = upadd(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = upadd(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+            red[2][threadIdx.x] = upadd(red[2][threadIdx.x], red[2][threadIdx.x + s]);
+            redn[0][threadIdx.x] += redn[0][threadIdx.x + s];
+            redn[1][threadIdx.x] += redn[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint64_t *rin = b.regions + ((uint64_t)m * 3 + P.in_buf) * 17;
+        uint64_t *rout = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
+        uint64_t acc = 0;
+        for (int c = 0; c < 16; ++c) acc += b.class_out[m * 16 + c];
+        // A map that outgrew its slot is emptied: later passes must never read past the slot.
+        // Its record carries the overflow; the host rolls the map back and grows the slot.
+        const bool over = b.overflow[m] || acc > b.slot[m].cap;
+        uint64_t at = 0;
+        for (int c = 0; c < 16; ++c) {
+            rout[c] = over ? 0 : at;
+            at += b.class_out[m * 16 + c];
+        }
+        rout[16] = over ? 0 : acc;
+        const uint64_t len_in = rin[16];
+        b.len[m * 3 + P.out_buf] = over ? 0 : acc;
+        if (over) b.overflow[m] = 1;
+        tpgp_hip_record r;
+        r.len_in = len_in;
+        r.len_out = acc;
+        r.dropped = redn[0][0];
+        r.l1_up = red[0][0];
+        r.sq_up = red[1][0];
+        r.rad_sq_up = red[2][0];
+        r.radii = redn[1][0];
+        r.executed = len_in > 0;
+        r.nonfinite = bad;
+        r.overflow = over ? 1u : 0u;
+        r.rotation = P.rot_seq;
+        b.rec[(uint64_t)m * b.max_rec + P.rec_slot] = r;
+    }
+}
+
+// One wave per map: binary search of the n single-site keys (symbol `pauli` at site s).
+template <int W>
+__global__ void k_readout(BatchDev b, const PassMap *pm, unsigned pauli) {
+    const uint32_t m = blockIdx.x;
+    const PassMap &P = pm[m];
+    if (!P.active) return;
+    const uint32_t n = b.slot[m].n_spins, s = threadIdx.x;
+    if (s >= 64) return;
+    double out = 0.0;
+    if (s < n) {
+        const uint64_t len = b.len[m * 3 + P.in_buf];
+        const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[m].off * W;
+        const double *vals = b.arena[P.in_buf].vals + b.slot[m].off;
+        Key<W> target = kdigit_mask<W>(P.pos[s], pauli);
+        uint64_t lo = 0, hi = len;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi) >> 1;
+            if (klt<W>(kload<W>(keys + mid * W), target))
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (lo < len && keq<W>(kload<W>(keys + lo * W), target)) out = vals[lo];
+    }
+    b.rd[((uint64_t)m * b.max_rd + P.rec_slot) * 64 + s] = out;
+}
+
+__global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x)
+        p[t] = v;
+}
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x)
+        p[t] = v;
+}
+__global__ void k_set_len(uint64_t *len, uint64_t idx, uint64_t v) { len[idx] = v; }
+
+}  // namespace
+
+// ==========================================================================================
+// Host side
+// ==========================================================================================
+struct tpgp_hip_context {
+    int device;
+    int cus;
+    hipStream_t stream;
+    void *storage;
+    size_t capacity;
+    std::mutex mutex;
+    char name[256];
+};
+
+namespace {
+void msg(char *e, size_t n, const char *fmt, ...) {
+    if (!e || !n) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(e, n, fmt, ap);
+    va_end(ap);
+}
+tpgp_hip_status hip_fail(hipError_t code, const char *what, char *e, size_t n) {
+    msg(e, n, "%s: %s", what, hipGetErrorString(code));
+    return code == hipErrorOutOfMemory ? TPGP_HIP_ENOMEM : TPGP_HIP_EDEVICE;
+}
+bool mulz(size_t a, size_t b, size_t *o) {
+    if (b && a > SIZE_MAX / b) return false;
+    *o = a * b;
+    return true;
+}
+bool addz(size_t a, size_t b, size_t *o) {
+    if (a > SIZE_MAX - b) return false;
+    *o = a + b;
+    return true;
+}
+bool overlap(const void *a, size_t as, const void *b, size_t bs) {
+    uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+    if (as > UINTPTR_MAX - x || bs > UINTPTR_MAX - y) return true;
+    return x < y + bs && y < x + as;
+}
+struct DeviceGuard {  // save/restore the caller's current device
+    int previous = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&previous) == hipSuccess && hipSetDevice(dev) == hipSuccess) ok = true;
+    }
+    ~DeviceGuard() {
+        if (previous >= 0) (void)hipSetDevice(previous);
+    }
+};
+}  // namespace
+
+extern "C" int tpgp_hip_compiled(void) { return 1; }
+
+extern "C" tpgp_hip_context *tpgp_hip_create(int index, char *error, size_t cap) {
+    msg(error, cap, "");
+    int count = 0, previous = 0;
+    hipError_t code = hipGetDeviceCount(&count);
+    if (code != hipSuccess) {
+        hip_fail(code, "HIP device discovery", error, cap);
+        return nullptr;
+    }
+    if (index < 0 || index >= count) {
+        msg(error, cap, "Requested HIP device index is unavailable.");
+        return nullptr;
+    }
+    if ((code = hipGetDevice(&previous)) != hipSuccess) {
+        hip_fail(code, "HIP current-device query", error, cap);
+        return nullptr;
+    }
+    if ((code = hipSetDevice(index)) != hipSuccess) {
+        hip_fail(code, "HIP device selection", error, cap);
+        return nullptr;
+    }
+    hipDeviceProp_t prop;
+    code = hipGetDeviceProperties(&prop, index);
+    if (code != hipSuccess) {
+        hip_fail(code, "HIP device properties", error, cap);
+        (void)hipSetDevice(previous);
+        return nullptr;
+    }
+    tpgp_hip_context *ctx = new (std::nothrow) tpgp_hip_context();
+    if (!ctx) {
+        msg(error, cap, "Could not allocate the HIP host context.");
+        (void)hipSetDevice(previous);
+        return nullptr;
+    }
+    ctx->device = index;
+    ctx->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
+    ctx->storage = nullptr;
+    ctx->capacity = 0;
+    snprintf(ctx->name, sizeof(ctx->name), "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
+             ctx->cus);
+    code = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (code != hipSuccess) {
+        hip_fail(code, "HIP stream creation", error, cap);
+        delete ctx;
+        (void)hipSetDevice(previous);
+        return nullptr;
+    }
+    if ((code = hipSetDevice(previous)) != hipSuccess) {
+        hip_fail(code, "HIP previous-device restoration", error, cap);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return nullptr;
+    }
+    return ctx;
+}
+
+extern "C" void tpgp_hip_destroy(tpgp_hip_context *ctx) {
+    if (!ctx) return;
+    {
+        DeviceGuard guard(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->storage) (void)hipFree(ctx->storage);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+extern "C" const char *tpgp_hip_device_name(const tpgp_hip_context *ctx) {
+    return ctx ? ctx->name : "unavailable";
+}
+
+extern "C" tpgp_hip_status tpgp_hip_branches(
+    tpgp_hip_context *ctx, const uint64_t *keys, const double *values, size_t count,
+    size_t words, size_t spins, size_t i, size_t j, unsigned qi, unsigned qj, double c, double s,
+    double rc, double rs, int enclose, uint64_t *out_keys, double *out_values, double *out_radii,
+    char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!ctx || spins < 2 || words != spins / 32 + (spins % 32 != 0) || i >= spins ||
+        j >= spins || i == j || qi > 3 || qj > 3 || !std::isfinite(c) || !std::isfinite(s) ||
+        !std::isfinite(rc) || !std::isfinite(rs) || rc < 0 || rs < 0 ||
+        (enclose != 0 && enclose != 1)) {
+        msg(error, cap,
+            "Invalid HIP branch dimensions, sites, generator, or trigonometric enclosure.");
+        return TPGP_HIP_EINVAL;
+    }
+    if (!count) return TPGP_HIP_OK;
+    size_t key_bytes, value_bytes, okey_bytes, oval_bytes, total, t;
+    if (!keys || !values || !out_keys || !out_values || !out_radii ||
+        !mulz(count, words, &t) || !mulz(t, sizeof(uint64_t), &key_bytes) ||
+        !mulz(count, sizeof(double), &value_bytes) || !mulz(key_bytes, 2, &okey_bytes) ||
+        !mulz(value_bytes, 2, &oval_bytes) || !mulz(key_bytes, 3, &total) ||
+        !mulz(value_bytes, 5, &t) || !addz(total, t, &total) || !addz(total, 256, &total)) {
+        msg(error, cap, "Null HIP array or packed-branch allocation size overflow.");
+        return TPGP_HIP_EINVAL;
+    }
+    const void *arrays[] = {keys, values, out_keys, out_values, out_radii};
+    const size_t lengths[] = {key_bytes, value_bytes, okey_bytes, oval_bytes, oval_bytes};
+    for (int a = 0; a < 5; ++a)
+        for (int b2 = a + 1; b2 < 5; ++b2)
+            if (overlap(arrays[a], lengths[a], arrays[b2], lengths[b2])) {
+                msg(error, cap, "HIP input/output arrays must be disjoint.");
+                return TPGP_HIP_EINVAL;
+            }
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    DeviceGuard guard(ctx->device);
+    if (!guard.ok) {
+        msg(error, cap, "HIP device selection failed.");
+        return TPGP_HIP_EDEVICE;
+    }
+    tpgp_hip_status result = TPGP_HIP_OK;
+    hipError_t code;
+    if (total > ctx->capacity) {
+        // release first: avoids a transient double allocation as a trajectory grows
+        if (ctx->storage) {
+            code = hipFree(ctx->storage);
+            ctx->storage = nullptr;
+            ctx->capacity = 0;
+            if (code != hipSuccess) return hip_fail(code, "HIP scratch release", error, cap);
+        }
+        code = hipMalloc(&ctx->storage, total);
+        if (code != hipSuccess) {
+            ctx->storage = nullptr;
+            return hip_fail(code, "HIP persistent scratch allocation", error, cap);
+        }
+        ctx->capacity = total;
+    }
+    unsigned char *base = (unsigned char *)ctx->storage;
+    uint64_t *dk = (uint64_t *)base;
+    double *dv = (double *)(base + key_bytes);
+    uint64_t *dok = (uint64_t *)(base + key_bytes + value_bytes);
+    double *dov = (double *)(base + key_bytes + value_bytes + okey_bytes);
+    double *dor = dov + 2 * count;
+    unsigned *dflag = (unsigned *)(dor + 2 * count);
+    unsigned flag = 0;
+#define HIPCALL(call, what)                                                   \
+    do {                                                                      \
+        if (result == TPGP_HIP_OK) {                                          \
+            code = (call);                                                    \
+            if (code != hipSuccess) result = hip_fail(code, what, error, cap); \
+        }                                                                     \
+    } while (0)
+    HIPCALL(hipMemcpyAsync(dk, keys, key_bytes, hipMemcpyHostToDevice, ctx->stream),
+            "HIP input-key transfer");
+    HIPCALL(hipMemcpyAsync(dv, values, value_bytes, hipMemcpyHostToDevice, ctx->stream),
+            "HIP input-coefficient transfer");
+    HIPCALL(hipMemsetAsync(dflag, 0, sizeof(unsigned), ctx->stream), "HIP status reset");
+    if (result == TPGP_HIP_OK) {
+        size_t want = (count + kBlock - 1) / kBlock;
+        size_t maxb = (size_t)ctx->cus * 16;
+        unsigned blocks = (unsigned)(want < maxb ? want : maxb);
+        hipLaunchKernelGGL(k_branches, dim3(blocks), dim3(kBlock), 0, ctx->stream, dk, dv, count,
+                           words, spins, i, j, qi, qj, c, s, rc, rs, enclose, dok, dov, dor,
+                           dflag);
+        HIPCALL(hipGetLastError(), "HIP branch-kernel launch");
+    }
+    HIPCALL(hipMemcpyAsync(&flag, dflag, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream),
+            "HIP status transfer");
+    HIPCALL(hipStreamSynchronize(ctx->stream), "HIP branch-kernel execution");
+    if (result == TPGP_HIP_OK && flag) {
+        result = (flag & 1u) ? TPGP_HIP_EINVAL : TPGP_HIP_ENUMERIC;
+        msg(error, cap, (flag & 1u) ? "HIP input keys contain nonzero padding bits."
+                                    : "HIP branch arithmetic or coefficients are nonfinite.");
+    }
+    HIPCALL(hipMemcpyAsync(out_keys, dok, okey_bytes, hipMemcpyDeviceToHost, ctx->stream),
+            "HIP output-key transfer");
+    HIPCALL(hipMemcpyAsync(out_values, dov, oval_bytes, hipMemcpyDeviceToHost, ctx->stream),
+            "HIP output-coefficient transfer");
+    HIPCALL(hipMemcpyAsync(out_radii, dor, oval_bytes, hipMemcpyDeviceToHost, ctx->stream),
+            "HIP output-radius transfer");
+    HIPCALL(hipStreamSynchronize(ctx->stream), "HIP output completion");
+#undef HIPCALL
+    // drain anything queued before an earlier failure; the caller may release its arrays
+    if (result != TPGP_HIP_OK) (void)hipStreamSynchronize(ctx->stream);
+    return result;
+}
+
+// ==========================================================================================
+// Resident batch
+// ==========================================================================================
+namespace {
+struct HostMap {
+    size_t n_spins = 0;
+    uint64_t off = 0, cap = 0;
+    int kc = 0, kcur = 0;
+    std::vector<uint8_t> sig_c, sig;  // sigma[p] = site at digit position p
+    uint32_t nrec = 0, nrd = 0;
+    uint64_t committed_len = 0;
+};
+struct LoggedLayout {
+    uint32_t map, rotation;
+    std::vector<uint8_t> sigma;
+};
+}  // namespace
+
+struct tpgp_hip_batch {
+    tpgp_hip_context *ctx = nullptr;
+    int W = 1;
+    uint32_t M = 0;
+    std::vector<HostMap> maps;
+    uint64_t total_cap = 0;
+    uint64_t max_tiles = 0;
+    BatchDev dev{};
+    MapSlot *d_slot = nullptr;
+    // staging ring for per-pass descriptors
+    PassMap *h_stage = nullptr;
+    PassMap *d_stage = nullptr;
+    size_t stage_cap = 0, stage_used = 0;
+    uint32_t epoch = 0;
+    unsigned grid = 1024;
+    uint32_t max_rec = 0, max_rd = 0;
+    std::vector<LoggedLayout> logged;
+    std::vector<void *> allocations;
+    bool poisoned = false;
+};
+
+namespace {
+
+template <typename T> hipError_t dalloc(tpgp_hip_batch *b, T **p, size_t count) {
+    size_t bytes = count * sizeof(T);
+    if (!bytes) bytes = sizeof(T);
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e == hipSuccess) {
+        b->allocations.push_back(q);
+        *p = (T *)q;
+    }
+    return e;
+}
+
+void free_all(tpgp_hip_batch *b) {
+    for (void *p : b->allocations) (void)hipFree(p);
+    b->allocations.clear();
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
+    b->h_stage = nullptr;
+}
+
+// Fit all device buffers to the current slot layout (maps' off/cap already assigned).
+int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
+    hipError_t e;
+    const uint32_t M = b->M;
+    const uint64_t T = b->total_cap;
+    b->max_tiles = (2 * T) / kMergeTile + 16ull * M + (T / kPartTile) + M + 64;
+    BatchDev &d = b->dev;
+    d = BatchDev{};
+    d.n_maps = M;
+    d.max_rec = b->max_rec;
+    d.max_rd = b->max_rd;
+#define ALLOC(ptr, n)                                                                    \
+    do {                                                                                 \
+        if ((e = dalloc(b, &(ptr), (n))) != hipSuccess) {                                \
+            hip_fail(e, "HIP resident-batch allocation", error, cap);                    \
+            return e == hipErrorOutOfMemory ? TPGP_HIP_ENOMEM : TPGP_HIP_EDEVICE;        \
+        }                                                                                \
+    } while (0)
+    for (int k = 0; k < 3; ++k) {
+        ALLOC(d.arena[k].keys, T * b->W + 4);
+        ALLOC(d.arena[k].vals, T + 4);
+    }
+    ALLOC(b->d_slot, M);
+    d.slot = b->d_slot;
+    ALLOC(d.len, (size_t)M * 3);
+    ALLOC(d.regions, (size_t)M * 3 * 17);
+    ALLOC(d.tstart, (size_t)M * 16 + 2);
+    ALLOC(d.tile_hist, b->max_tiles * 16);
+    ALLOC(d.tile_base, b->max_tiles * 16);
+    ALLOC(d.map_hist, (size_t)M * 16);
+    ALLOC(d.splits, b->max_tiles + 1);
+    ALLOC(d.status, b->max_tiles);
+    ALLOC(d.class_out, (size_t)M * 16);
+    ALLOC(d.tile_led, b->max_tiles);
+    ALLOC(d.counter, 4);
+    ALLOC(d.overflow, M);
+    ALLOC(d.rec, (size_t)M * b->max_rec);
+    ALLOC(d.rd, (size_t)M * b->max_rd * 64);
+    d.log_cap = 1u << 20;
+    ALLOC(d.log, d.log_cap);
+    ALLOC(d.log_count, 1);
+    ALLOC(b->d_stage, b->stage_cap);
+#undef ALLOC
+    e = hipHostMalloc((void **)&b->h_stage, b->stage_cap * sizeof(PassMap), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        b->h_stage = nullptr;
+        hip_fail(e, "HIP pinned staging allocation", error, cap);
+        return TPGP_HIP_ENOMEM;
+    }
+    hipStream_t st = b->ctx->stream;
+    std::vector<MapSlot> slots(M);
+    for (uint32_t m = 0; m < M; ++m)
+        slots[m] = MapSlot{b->maps[m].off, b->maps[m].cap, (uint32_t)b->maps[m].n_spins, 0};
+    if ((e = hipMemcpyAsync(b->d_slot, slots.data(), M * sizeof(MapSlot), hipMemcpyHostToDevice,
+                            st)) != hipSuccess)
+        return hip_fail(e, "HIP slot upload", error, cap);
+    hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, st, d.status, b->max_tiles, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, st, d.len, (uint64_t)M * 3, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, st, d.regions, (uint64_t)M * 3 * 17,
+                       0ull);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, d.overflow, (uint64_t)M, 0u);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, d.log_count, 1ull, 0ull);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, d.counter, 4ull, 0u);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess)
+        return hip_fail(e, "HIP batch initialisation", error, cap);
+    b->epoch = 0;
+    b->stage_used = 0;
+    return TPGP_HIP_OK;
+}
+
+// Reserve n descriptors in the staging ring; synchronises when the ring is full.
+PassMap *stage(tpgp_hip_batch *b, size_t n, PassMap **dev) {
+    if (b->stage_used + n > b->stage_cap) {
+        (void)hipStreamSynchronize(b->ctx->stream);
+        b->stage_used = 0;
+    }
+    PassMap *h = b->h_stage + b->stage_used;
+    *dev = b->d_stage + b->stage_used;
+    std::memset((void *)h, 0, n * sizeof(PassMap));
+    b->stage_used += n;
+    return h;
+}
+hipError_t push_stage(tpgp_hip_batch *b, PassMap *h, PassMap *d, size_t n) {
+    return hipMemcpyAsync(d, h, n * sizeof(PassMap), hipMemcpyHostToDevice, b->ctx->stream);
+}
+
+int next_buffer(const HostMap &hm) {
+    for (int k = 0; k < 3; ++k)
+        if (k != hm.kc && k != hm.kcur) return k;
+    return (hm.kc + 1) % 3;
+}
+
+uint32_t next_epoch(tpgp_hip_batch *b) {
+    if (++b->epoch >= (1u << 24)) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, b->ctx->stream, b->dev.status,
+                           b->max_tiles, 0ull);
+        b->epoch = 1;
+    }
+    return b->epoch;
+}
+
+// Enqueue a partition pass for the maps with want[m] (positions pa/pb given per map).
+hipError_t enqueue_partition(tpgp_hip_batch *b, const std::vector<int> &want,
+                             const std::vector<uint32_t> &pa, const std::vector<uint32_t> &pb) {
+    PassMap *d;
+    PassMap *h = stage(b, b->M, &d);
+    for (uint32_t m = 0; m < b->M; ++m) {
+        if (!want[m]) continue;
+        HostMap &hm = b->maps[m];
+        h[m].active = 1;
+        h[m].in_buf = (uint32_t)hm.kcur;
+        h[m].out_buf = (uint32_t)next_buffer(hm);
+        h[m].pa = pa[m];
+        h[m].pb = pb[m];
+    }
+    hipError_t e = push_stage(b, h, d, b->M);
+    if (e != hipSuccess) return e;
+    hipStream_t st = b->ctx->stream;
+    hipLaunchKernelGGL(k_part_plan, dim3(1), dim3(kBlock), 0, st, b->dev, (const PassMap *)d);
+    if (b->W == 1) {
+        hipLaunchKernelGGL(k_part_count<1>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
+                           (const PassMap *)d);
+    } else {
+        hipLaunchKernelGGL(k_part_count<2>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
+                           (const PassMap *)d);
+    }
+    hipLaunchKernelGGL(k_part_scan, dim3(b->M * 16), dim3(kBlock), 0, st, b->dev,
+                       (const PassMap *)d);
+    if (b->W == 1) {
+        hipLaunchKernelGGL(k_part_scatter<1>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
+                           (const PassMap *)d);
+    } else {
+        hipLaunchKernelGGL(k_part_scatter<2>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
+                           (const PassMap *)d);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (uint32_t m = 0; m < b->M; ++m) {
+        if (!want[m]) continue;
+        HostMap &hm = b->maps[m];
+        hm.kcur = (int)h[m].out_buf;
+        // sigma: move sites at pa, pb to the top, keep the others' order
+        size_t n = hm.n_spins;
+        uint8_t sa = hm.sig[pa[m]], sb = hm.sig[pb[m]];
+        std::vector<uint8_t> rest;
+        rest.reserve(n);
+        for (size_t p = 0; p < n; ++p)
+            if (p != pa[m] && p != pb[m]) rest.push_back(hm.sig[p]);
+        rest.push_back(sb);
+        rest.push_back(sa);
+        hm.sig = rest;
+    }
+    return hipSuccess;
+}
+
+size_t position_of(const HostMap &hm, size_t site) {
+    for (size_t p = 0; p < hm.n_spins; ++p)
+        if (hm.sig[p] == site) return p;
+    return SIZE_MAX;
+}
+
+// Reference-layout key from a stored key (digit p holds site sigma[p]).
+void to_reference(const std::vector<uint8_t> &sigma, const uint64_t *stored, uint64_t *ref,
+                  int W) {
+    ref[0] = 0;
+    if (W > 1) ref[1] = 0;
+    for (size_t p = 0; p < sigma.size(); ++p) {
+        unsigned d = (unsigned)(stored[p >> 5] >> (2 * (p & 31))) & 3u;
+        size_t s = sigma[p];
+        ref[s >> 5] |= (uint64_t)d << (2 * (s & 31));
+    }
+}
+
+}  // namespace
+
+extern "C" tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *ctx, size_t n_words,
+                                                 size_t n_maps, const size_t *n_spins,
+                                                 const size_t *capacities, char *error,
+                                                 size_t cap) {
+    msg(error, cap, "");
+    if (!ctx || (n_words != 1 && n_words != 2) || !n_maps || n_maps > (1u << 20) || !n_spins ||
+        !capacities) {
+        msg(error, cap, "Invalid resident-batch arguments.");
+        return nullptr;
+    }
+    tpgp_hip_batch *b = new (std::nothrow) tpgp_hip_batch();
+    if (!b) {
+        msg(error, cap, "Could not allocate the batch host state.");
+        return nullptr;
+    }
+    b->ctx = ctx;
+    b->W = (int)n_words;
+    b->M = (uint32_t)n_maps;
+    b->maps.resize(n_maps);
+    uint64_t off = 0;
+    for (size_t m = 0; m < n_maps; ++m) {
+        size_t n = n_spins[m];
+        if (n < 2 || n > 32 * n_words || (n_words == 2 && n <= 32) || capacities[m] == 0 ||
+            capacities[m] > (1ull << 37)) {
+            msg(error, cap, "Invalid spin count or capacity for map %zu.", m);
+            delete b;
+            return nullptr;
+        }
+        HostMap &hm = b->maps[m];
+        hm.n_spins = n;
+        hm.cap = capacities[m];
+        hm.off = off;
+        off += (capacities[m] + 63) & ~63ull;
+        hm.sig.resize(n);
+        for (size_t p = 0; p < n; ++p) hm.sig[p] = (uint8_t)p;
+        hm.sig_c = hm.sig;
+    }
+    b->total_cap = off;
+    b->max_rec = n_maps <= 16 ? 4096u : (uint32_t)std::max<size_t>(1024, 65536 / n_maps);
+    b->max_rd = n_maps <= 16 ? 1024u : 64u;
+    b->stage_cap = std::max<size_t>(n_maps * 64, 1u << 16);
+    b->grid = (unsigned)ctx->cus * 8;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    DeviceGuard guard(ctx->device);
+    if (!guard.ok || allocate_device(b, error, cap) != TPGP_HIP_OK) {
+        if (!guard.ok) msg(error, cap, "HIP device selection failed.");
+        free_all(b);
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+extern "C" void tpgp_hip_batch_destroy(tpgp_hip_batch *b) {
+    if (!b) return;
+    {
+        std::lock_guard<std::mutex> lock(b->ctx->mutex);
+        DeviceGuard guard(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+        free_all(b);
+    }
+    delete b;
+}
+
+extern "C" void *tpgp_hip_batch_stream(const tpgp_hip_batch *b) {
+    return b ? (void *)b->ctx->stream : nullptr;
+}
+
+extern "C" int tpgp_hip_batch_info(const tpgp_hip_batch *b, size_t map, size_t *len,
+                                   size_t *capacity) {
+    if (!b || map >= b->M) return TPGP_HIP_EINVAL;
+    if (len) *len = b->maps[map].committed_len;
+    if (capacity) *capacity = b->maps[map].cap;
+    return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_upload(tpgp_hip_batch *b, size_t map, const uint64_t *keys,
+                                     const double *values, size_t len, char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!b || map >= b->M || (len && (!keys || !values))) {
+        msg(error, cap, "Invalid upload arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    HostMap &hm = b->maps[map];
+    if (len > hm.cap) {
+        msg(error, cap, "Map %zu: %zu terms exceed the slot capacity %llu.", map, len,
+            (unsigned long long)hm.cap);
+        return TPGP_HIP_ECAPACITY;
+    }
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    hipStream_t st = b->ctx->stream;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "HIP upload", error, cap);
+    // discard pending work of this map, install the new state as committed
+    hm.kcur = hm.kc;
+    for (size_t p = 0; p < hm.n_spins; ++p) hm.sig[p] = (uint8_t)p;
+    hm.sig_c = hm.sig;
+    const int k = hm.kc;
+    if (len) {
+        e = hipMemcpyAsync(b->dev.arena[k].keys + hm.off * b->W, keys, len * b->W * 8,
+                           hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->dev.arena[k].vals + hm.off, values, len * 8,
+                               hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return hip_fail(e, "HIP upload transfer", error, cap);
+    }
+    hipLaunchKernelGGL(k_set_len, dim3(1), dim3(1), 0, st, b->dev.len, (uint64_t)map * 3 + k,
+                       (uint64_t)len);
+    if (b->W == 1)
+        hipLaunchKernelGGL(k_regions<1>, dim3(1), dim3(64), 0, st, b->dev, (uint32_t)map,
+                           (uint32_t)k);
+    else
+        hipLaunchKernelGGL(k_regions<2>, dim3(1), dim3(64), 0, st, b->dev, (uint32_t)map,
+                           (uint32_t)k);
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "HIP upload", error, cap);
+    hm.committed_len = len;
+    hm.nrec = hm.nrd = 0;
+    return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_rotate(tpgp_hip_batch *b, const tpgp_hip_rotation *req,
+                                     uint32_t rotation, char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!b || !req) {
+        msg(error, cap, "Invalid rotation arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    const uint32_t M = b->M;
+    for (uint32_t m = 0; m < M; ++m) {
+        const tpgp_hip_rotation &r = req[m];
+        if (!r.active) continue;
+        const HostMap &hm = b->maps[m];
+        if (r.i >= hm.n_spins || r.j >= hm.n_spins || r.i == r.j || r.qi > 3 || r.qj > 3 ||
+            !std::isfinite(r.c) || !std::isfinite(r.s) || !std::isfinite(r.rc) ||
+            !std::isfinite(r.rs) || !std::isfinite(r.threshold) || r.threshold < 0) {
+            msg(error, cap, "Invalid rotation request for map %u.", m);
+            return TPGP_HIP_EINVAL;
+        }
+        if (hm.nrec >= b->max_rec) {
+            msg(error, cap, "Too many rotations since the last sync (map %u).", m);
+            return TPGP_HIP_EINVAL;
+        }
+    }
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    hipStream_t st = b->ctx->stream;
+    // 1) bring each active map's pair (i, j) to the top two digits
+    std::vector<int> want(M, 0);
+    std::vector<uint32_t> pa(M, 0), pb(M, 0);
+    bool any = false;
+    for (uint32_t m = 0; m < M; ++m) {
+        if (!req[m].active) continue;
+        HostMap &hm = b->maps[m];
+        size_t n = hm.n_spins;
+        uint8_t top = hm.sig[n - 1], sec = hm.sig[n - 2];
+        bool ok = (top == req[m].i && sec == req[m].j) || (top == req[m].j && sec == req[m].i);
+        if (!ok) {
+            want[m] = 1;
+            any = true;
+            pa[m] = (uint32_t)position_of(hm, req[m].i);
+            pb[m] = (uint32_t)position_of(hm, req[m].j);
+        }
+    }
+    hipError_t e;
+    if (any && (e = enqueue_partition(b, want, pa, pb)) != hipSuccess)
+        return hip_fail(e, "HIP partition launch", error, cap);
+    // 2) rotation pass
+    PassMap *d;
+    PassMap *h = stage(b, M, &d);
+    for (uint32_t m = 0; m < M; ++m) {
+        const tpgp_hip_rotation &r = req[m];
+        if (!r.active) continue;
+        HostMap &hm = b->maps[m];
+        size_t n = hm.n_spins;
+        bool i_top = hm.sig[n - 1] == r.i;
+        h[m].active = 1;
+        h[m].in_buf = (uint32_t)hm.kcur;
+        h[m].out_buf = (uint32_t)next_buffer(hm);
+        h[m].qt = i_top ? r.qi : r.qj;
+        h[m].qs = i_top ? r.qj : r.qi;
+        h[m].flags = (r.validated ? 1u : 0u) | (r.log ? 2u : 0u);
+        h[m].rec_slot = hm.nrec;
+        h[m].rot_seq = rotation;
+        h[m].c = r.c;
+        h[m].s = r.s;
+        h[m].rc = r.validated ? r.rc : 0.0;
+        h[m].rs = r.validated ? r.rs : 0.0;
+        h[m].threshold = r.threshold;
+    }
+    if ((e = push_stage(b, h, d, M)) != hipSuccess)
+        return hip_fail(e, "HIP rotation staging", error, cap);
+    uint32_t epoch = next_epoch(b);
+    hipLaunchKernelGGL(k_rot_plan, dim3(1), dim3(kBlock), 0, st, b->dev, (const PassMap *)d);
+    if (b->W == 1) {
+        hipLaunchKernelGGL(k_rot_splits<1>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
+                           (const PassMap *)d);
+        hipLaunchKernelGGL(k_rot_merge<1>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
+                           (const PassMap *)d, epoch);
+    } else {
+        hipLaunchKernelGGL(k_rot_splits<2>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
+                           (const PassMap *)d);
+        hipLaunchKernelGGL(k_rot_merge<2>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
+                           (const PassMap *)d, epoch);
+    }
+    hipLaunchKernelGGL(k_rot_finish, dim3(M), dim3(kBlock), 0, st, b->dev, (const PassMap *)d);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return hip_fail(e, "HIP rotation launch", error, cap);
+    for (uint32_t m = 0; m < M; ++m) {
+        if (!req[m].active) continue;
+        HostMap &hm = b->maps[m];
+        hm.kcur = (int)h[m].out_buf;
+        hm.nrec++;
+        if (req[m].log) b->logged.push_back(LoggedLayout{m, rotation, hm.sig});
+    }
+    return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_readout(tpgp_hip_batch *b, unsigned pauli, const int *active,
+                                      char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!b || pauli < 1 || pauli > 3) {
+        msg(error, cap, "Invalid readout arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    for (uint32_t m = 0; m < b->M; ++m)
+        if ((!active || active[m]) && b->maps[m].nrd >= b->max_rd) {
+            msg(error, cap, "Too many readouts since the last sync (map %u).", m);
+            return TPGP_HIP_EINVAL;
+        }
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    PassMap *d;
+    PassMap *h = stage(b, b->M, &d);
+    for (uint32_t m = 0; m < b->M; ++m) {
+        if (active && !active[m]) continue;
+        HostMap &hm = b->maps[m];
+        h[m].active = 1;
+        h[m].in_buf = (uint32_t)hm.kcur;
+        h[m].rec_slot = hm.nrd++;
+        for (size_t p = 0; p < hm.n_spins; ++p) h[m].pos[hm.sig[p]] = (uint32_t)p;
+    }
+    hipError_t e = push_stage(b, h, d, b->M);
+    if (e != hipSuccess) return hip_fail(e, "HIP readout staging", error, cap);
+    if (b->W == 1)
+        hipLaunchKernelGGL(k_readout<1>, dim3(b->M), dim3(64), 0, b->ctx->stream, b->dev,
+                           (const PassMap *)d, pauli);
+    else
+        hipLaunchKernelGGL(k_readout<2>, dim3(b->M), dim3(64), 0, b->ctx->stream, b->dev,
+                           (const PassMap *)d, pauli);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return hip_fail(e, "HIP readout launch", error, cap);
+    return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_sync(tpgp_hip_batch *b, tpgp_hip_record *records,
+                                   size_t max_records, size_t *n_records, double *readouts,
+                                   size_t max_readouts, size_t *n_readouts, uint64_t *lengths,
+                                   tpgp_hip_log_entry *log, size_t log_capacity, size_t *n_log,
+                                   char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!b) return TPGP_HIP_EINVAL;
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    hipStream_t st = b->ctx->stream;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "HIP resident execution", error, cap);
+    const uint32_t M = b->M;
+    uint32_t nrec = 0, nrd = 0;
+    for (uint32_t m = 0; m < M; ++m) {
+        nrec = std::max(nrec, b->maps[m].nrec);
+        nrd = std::max(nrd, b->maps[m].nrd);
+    }
+    if (n_records) *n_records = nrec;
+    if (n_readouts) *n_readouts = nrd;
+    if (records && nrec) {
+        if (max_records < nrec) {
+            msg(error, cap, "Record buffer too small (%u needed).", nrec);
+            return TPGP_HIP_EINVAL;
+        }
+        std::vector<tpgp_hip_record> tmp((size_t)M * b->max_rec);
+        e = hipMemcpy(tmp.data(), b->dev.rec, tmp.size() * sizeof(tpgp_hip_record),
+                      hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "HIP record transfer", error, cap);
+        for (uint32_t m = 0; m < M; ++m)
+            for (uint32_t r = 0; r < b->maps[m].nrec; ++r)
+                records[(size_t)m * max_records + r] = tmp[(size_t)m * b->max_rec + r];
+    }
+    if (readouts && nrd) {
+        if (max_readouts < nrd) {
+            msg(error, cap, "Readout buffer too small (%u needed).", nrd);
+            return TPGP_HIP_EINVAL;
+        }
+        std::vector<double> tmp((size_t)M * b->max_rd * 64);
+        e = hipMemcpy(tmp.data(), b->dev.rd, tmp.size() * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "HIP readout transfer", error, cap);
+        for (uint32_t m = 0; m < M; ++m)
+            for (uint32_t q = 0; q < b->maps[m].nrd; ++q)
+                std::memcpy(readouts + ((size_t)m * max_readouts + q) * 64,
+                            tmp.data() + ((size_t)m * b->max_rd + q) * 64, 64 * sizeof(double));
+    }
+    if (lengths) {
+        std::vector<uint64_t> len((size_t)M * 3);
+        e = hipMemcpy(len.data(), b->dev.len, len.size() * sizeof(uint64_t),
+                      hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "HIP length transfer", error, cap);
+        for (uint32_t m = 0; m < M; ++m) lengths[m] = len[(size_t)m * 3 + b->maps[m].kcur];
+    }
+    uint32_t internal = 0;
+    e = hipMemcpy(&internal, b->dev.counter + 1, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "HIP status transfer", error, cap);
+    if (internal) {
+        msg(error, cap, "Internal merge-path inconsistency detected on the device.");
+        return TPGP_HIP_EDEVICE;
+    }
+    uint64_t nlog = 0;
+    e = hipMemcpy(&nlog, b->dev.log_count, sizeof(uint64_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "HIP log-count transfer", error, cap);
+    int status = TPGP_HIP_OK;
+    if (n_log) *n_log = 0;
+    if (nlog) {
+        if (nlog > b->dev.log_cap) {
+            msg(error, cap, "Ledger log overflow (%llu entries).", (unsigned long long)nlog);
+            status = TPGP_HIP_ECAPACITY;
+        } else if (log) {
+            std::vector<LogDev> tmp(nlog);
+            e = hipMemcpy(tmp.data(), b->dev.log, nlog * sizeof(LogDev), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return hip_fail(e, "HIP log transfer", error, cap);
+            size_t out = 0;
+            for (uint64_t t = 0; t < nlog; ++t) {
+                const LogDev &L = tmp[t];
+                const std::vector<uint8_t> *sig = nullptr;
+                for (const LoggedLayout &ll : b->logged)
+                    if (ll.map == L.map && ll.rotation == L.rotation) sig = &ll.sigma;
+                if (!sig) continue;
+                if (out >= log_capacity) {
+                    msg(error, cap, "Ledger log buffer too small (%llu entries).",
+                        (unsigned long long)nlog);
+                    status = TPGP_HIP_ECAPACITY;
+                    break;
+                }
+                tpgp_hip_log_entry &o = log[out++];
+                uint64_t stored[2] = {L.key[0], L.key[1]};
+                uint64_t sw[2] = {stored[0], b->W > 1 ? stored[1] : 0};
+                to_reference(*sig, sw, o.key, b->W);
+                if (b->W == 1) o.key[1] = 0;
+                o.value = L.value;
+                o.aux = L.aux;
+                o.map = L.map;
+                o.rotation = L.rotation;
+                o.kind = L.kind;
+                o.pad = 0;
+            }
+            if (n_log) *n_log = out;
+        }
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, b->dev.log_count, 1ull, 0ull);
+    }
+    b->logged.clear();
+    for (uint32_t m = 0; m < M; ++m) b->maps[m].nrec = b->maps[m].nrd = 0;
+    b->stage_used = 0;
+    return status;
+}
+
+extern "C" int tpgp_hip_batch_commit(tpgp_hip_batch *b, const int *mask, char *error,
+                                     size_t cap) {
+    msg(error, cap, "");
+    if (!b) return TPGP_HIP_EINVAL;
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    hipError_t e = hipStreamSynchronize(b->ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "HIP commit", error, cap);
+    std::vector<uint64_t> len((size_t)b->M * 3);
+    e = hipMemcpy(len.data(), b->dev.len, len.size() * sizeof(uint64_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "HIP commit", error, cap);
+    for (uint32_t m = 0; m < b->M; ++m) {
+        HostMap &hm = b->maps[m];
+        if (!mask || mask[m]) {
+            hm.kc = hm.kcur;
+            hm.sig_c = hm.sig;
+            hm.committed_len = len[(size_t)m * 3 + hm.kc];
+        } else {
+            hm.kcur = hm.kc;
+            hm.sig = hm.sig_c;
+        }
+    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, b->ctx->stream, b->dev.overflow,
+                       (uint64_t)b->M, 0u);
+    return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_rollback(tpgp_hip_batch *b, char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!b) return TPGP_HIP_EINVAL;
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    for (auto &hm : b->maps) {
+        hm.kcur = hm.kc;
+        hm.sig = hm.sig_c;
+        hm.nrec = hm.nrd = 0;
+    }
+    b->logged.clear();
+    b->stage_used = 0;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, b->ctx->stream, b->dev.overflow,
+                       (uint64_t)b->M, 0u);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, b->ctx->stream, b->dev.log_count, 1ull,
+                       0ull);
+    hipError_t e = hipStreamSynchronize(b->ctx->stream);
+    return e == hipSuccess ? TPGP_HIP_OK : hip_fail(e, "HIP rollback", error, cap);
+}
+
+extern "C" int tpgp_hip_batch_download(tpgp_hip_batch *b, size_t map, uint64_t *keys,
+                                       double *values, size_t capacity, size_t *len, char *error,
+                                       size_t cap) {
+    msg(error, cap, "");
+    if (!b || map >= b->M || !len) {
+        msg(error, cap, "Invalid download arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    HostMap &hm = b->maps[map];
+    *len = hm.committed_len;
+    if (!keys && !values && capacity == 0) return TPGP_HIP_OK;
+    if (capacity < hm.committed_len || (hm.committed_len && (!keys || !values))) {
+        msg(error, cap, "Download buffers hold %zu terms, %llu needed.", capacity,
+            (unsigned long long)hm.committed_len);
+        return TPGP_HIP_EINVAL;
+    }
+    // Reorder a scratch copy of the committed state into the reference layout with a
+    // least-significant-pair-first sequence of stable partitions: pairs (1,0), (3,2), ...,
+    // finishing with (n-1, n-2); each pass moves its pair above all others.
+    if (hm.kcur != hm.kc || hm.nrec || hm.nrd) {
+        msg(error, cap, "Map %zu has uncommitted work; sync and commit or roll back first.", map);
+        return TPGP_HIP_EINVAL;
+    }
+    {
+        std::lock_guard<std::mutex> lock(b->ctx->mutex);
+        DeviceGuard guard(b->ctx->device);
+        hipError_t e = hipStreamSynchronize(b->ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "HIP download", error, cap);
+        const size_t n = hm.n_spins;
+        std::vector<int> want(b->M, 0);
+        want[map] = 1;
+        std::vector<uint32_t> pa(b->M, 0), pb(b->M, 0);
+        std::vector<std::pair<size_t, size_t>> pairs;
+        for (size_t lo = 0; lo + 1 < n; lo += 2) pairs.push_back({lo + 1, lo});
+        if (n % 2) pairs.push_back({n - 1, n - 2});
+        bool identity = true;
+        for (size_t p = 0; p < n; ++p) identity = identity && hm.sig[p] == p;
+        if (!identity) {
+            for (auto &pr : pairs) {
+                pa[map] = (uint32_t)position_of(hm, pr.first);
+                pb[map] = (uint32_t)position_of(hm, pr.second);
+                if ((e = enqueue_partition(b, want, pa, pb)) != hipSuccess)
+                    return hip_fail(e, "HIP download reorder", error, cap);
+            }
+        }
+        const int k = hm.kcur;
+        if (hm.committed_len) {
+            e = hipMemcpyAsync(keys, b->dev.arena[k].keys + hm.off * b->W,
+                               hm.committed_len * b->W * 8, hipMemcpyDeviceToHost,
+                               b->ctx->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(values, b->dev.arena[k].vals + hm.off, hm.committed_len * 8,
+                                   hipMemcpyDeviceToHost, b->ctx->stream);
+            if (e != hipSuccess) return hip_fail(e, "HIP download transfer", error, cap);
+        }
+        e = hipStreamSynchronize(b->ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "HIP download", error, cap);
+        // the reordered copy becomes the committed state (same map, reference layout)
+        hm.kc = hm.kcur;
+        hm.sig_c = hm.sig;
+        b->stage_used = 0;
+    }
+    return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_reserve(tpgp_hip_batch *b, size_t map, size_t capacity,
+                                      char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!b || map >= b->M || capacity > (1ull << 37)) {
+        msg(error, cap, "Invalid reserve arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    if (capacity <= b->maps[map].cap) return TPGP_HIP_OK;
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    hipStream_t st = b->ctx->stream;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "HIP reserve", error, cap);
+    // keep the committed states (and lengths/regions) of every map
+    const uint32_t M = b->M;
+    const int W = b->W;
+    std::vector<uint64_t> len((size_t)M * 3);
+    if ((e = hipMemcpy(len.data(), b->dev.len, len.size() * 8, hipMemcpyDeviceToHost)) !=
+        hipSuccess)
+        return hip_fail(e, "HIP reserve", error, cap);
+    std::vector<std::vector<uint64_t>> hk(M);
+    std::vector<std::vector<double>> hv(M);
+    for (uint32_t m = 0; m < M; ++m) {
+        HostMap &hm = b->maps[m];
+        uint64_t n = len[(size_t)m * 3 + hm.kc];
+        hk[m].resize(n * W);
+        hv[m].resize(n);
+        if (n) {
+            if ((e = hipMemcpy(hk[m].data(), b->dev.arena[hm.kc].keys + hm.off * W, n * W * 8,
+                               hipMemcpyDeviceToHost)) != hipSuccess ||
+                (e = hipMemcpy(hv[m].data(), b->dev.arena[hm.kc].vals + hm.off, n * 8,
+                               hipMemcpyDeviceToHost)) != hipSuccess)
+                return hip_fail(e, "HIP reserve", error, cap);
+        }
+    }
+    free_all(b);
+    uint64_t off = 0;
+    for (uint32_t m = 0; m < M; ++m) {
+        HostMap &hm = b->maps[m];
+        if (m == map) hm.cap = capacity;
+        hm.off = off;
+        off += (hm.cap + 63) & ~63ull;
+    }
+    b->total_cap = off;
+    int s = allocate_device(b, error, cap);
+    if (s != TPGP_HIP_OK) {
+        b->poisoned = true;
+        return s;
+    }
+    // committed states go to buffer 0 of the new arenas, with their lengths and layouts
+    std::vector<uint64_t> nlen((size_t)M * 3, 0);
+    for (uint32_t m = 0; m < M; ++m) {
+        HostMap &hm = b->maps[m];
+        hm.kc = hm.kcur = 0;
+        hm.sig = hm.sig_c;
+        hm.nrec = hm.nrd = 0;
+        uint64_t n = hv[m].size();
+        nlen[(size_t)m * 3] = n;
+        if (n) {
+            if ((e = hipMemcpy(b->dev.arena[0].keys + hm.off * W, hk[m].data(), n * W * 8,
+                               hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(b->dev.arena[0].vals + hm.off, hv[m].data(), n * 8,
+                               hipMemcpyHostToDevice)) != hipSuccess)
+                return hip_fail(e, "HIP reserve", error, cap);
+        }
+    }
+    b->logged.clear();
+    if ((e = hipMemcpy(b->dev.len, nlen.data(), nlen.size() * 8, hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return hip_fail(e, "HIP reserve", error, cap);
+    for (uint32_t m = 0; m < M; ++m) {
+        // every layout keeps its active pair on the top two digits: regions follow from keys
+        if (W == 1)
+            hipLaunchKernelGGL(k_regions<1>, dim3(1), dim3(64), 0, st, b->dev, m, 0u);
+        else
+            hipLaunchKernelGGL(k_regions<2>, dim3(1), dim3(64), 0, st, b->dev, m, 0u);
+    }
+    e = hipStreamSynchronize(st);
+    return e == hipSuccess ? TPGP_HIP_OK : hip_fail(e, "HIP reserve", error, cap);
+}

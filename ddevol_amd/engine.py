@@ -1,0 +1,155 @@
+"""Python surface of the engine: the reference core's operations, backed by the native libraries.
+
+Mirrors engine/include/tpgp_rev.h (maps, tpgp_rotate, tpgp_strang_step, readout) and
+engine/include/tpgp_cuda_rev.h (context + branches) with the same argument meaning and statuses.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native as N
+
+
+class Context:
+    """A device context (tpgp_hip_create / tpgp_hip_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = N.hip()
+        err = C.create_string_buffer(512)
+        self.ptr = self.lib.tpgp_hip_create(device, err, 512)
+        if not self.ptr:
+            raise RuntimeError(f"HIP device unavailable: {err.value.decode()}")
+
+    @property
+    def name(self) -> str:
+        return self.lib.tpgp_hip_device_name(self.ptr).decode()
+
+    def close(self):
+        if self.ptr:
+            self.lib.tpgp_hip_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def branches(self, keys, values, n_words, n_spins, i, j, qi, qj, c, s, rc, rs, enclose):
+        """tpgp_hip_branches on host arrays; returns (status, out_keys, out_values, out_radii, msg)."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1)
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        count = len(values)
+        ok = np.zeros(2 * count * n_words, np.uint64)
+        ov = np.zeros(2 * count, np.float64)
+        orad = np.zeros(2 * count, np.float64)
+        err = C.create_string_buffer(512)
+        st = self.lib.tpgp_hip_branches(self.ptr, N.as_u64p(keys), N.as_dp(values), count,
+                                        n_words, n_spins, i, j, qi, qj, c, s, rc, rs, enclose,
+                                        N.as_u64p(ok), N.as_dp(ov), N.as_dp(orad), err, 512)
+        return st, ok.reshape(2 * count, n_words) if count else ok, ov, orad, err.value.decode()
+
+
+class Map:
+    """A host tpgp_map (sorted unique keys, reference layout)."""
+
+    def __init__(self, n_spins: int):
+        self.eng = N.engine()
+        self.m = N.TpgpMap()
+        st = self.eng.tpgp_map_init(C.byref(self.m), n_spins)
+        if st:
+            raise ValueError(f"tpgp_map_init: status {st}")
+
+    def __del__(self):
+        try:
+            self.eng.tpgp_map_free(C.byref(self.m))
+        except Exception:
+            pass
+
+    @property
+    def n_spins(self):
+        return self.m.n_spins
+
+    @property
+    def n_words(self):
+        return self.m.n_words
+
+    def __len__(self):
+        return self.m.len
+
+    def add_single(self, site, pauli, value):
+        return self.eng.tpgp_map_add_single(C.byref(self.m), site, pauli, value)
+
+    def add(self, key_words, value):
+        k = np.ascontiguousarray(key_words, dtype=np.uint64)
+        return self.eng.tpgp_map_add(C.byref(self.m), N.as_u64p(k), value)
+
+    @classmethod
+    def from_arrays(cls, n_spins, keys, values):
+        mp = cls(n_spins)
+        keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(len(values), -1)
+        for k in range(len(values)):
+            st = mp.add(keys[k], float(values[k]))
+            if st:
+                raise ValueError(f"tpgp_map_add: status {st}")
+        return mp
+
+    def arrays(self):
+        n, w = self.m.len, self.m.n_words
+        if n == 0:
+            return np.zeros((0, w), np.uint64), np.zeros(0)
+        keys = np.ctypeslib.as_array(self.m.keys, shape=(n * w,)).reshape(n, w).copy()
+        vals = np.ctypeslib.as_array(self.m.values, shape=(n,)).copy()
+        return keys, vals
+
+    def single_sum(self, pauli=1):
+        return self.eng.tpgp_map_single_sum(C.byref(self.m), pauli)
+
+    def readout(self, pauli, normalizer):
+        mid, rad = C.c_double(), C.c_double()
+        st = self.eng.tpgp_readout(C.byref(self.m), pauli, normalizer, C.byref(mid),
+                                   C.byref(rad))
+        return st, mid.value, rad.value
+
+
+def options(threshold=0.0, max_terms=0, validated=0, max_drop_l2=0.0, context=None,
+            backend=N.TPGP_HIP, threads=1):
+    o = N.TpgpOptions()
+    o.threshold, o.max_terms, o.threads, o.backend = threshold, max_terms, threads, backend
+    o.validated, o.max_drop_l2 = validated, max_drop_l2
+    o.hip_context = context.ptr if context is not None else None
+    return o
+
+
+def ledger(validated=0):
+    led = N.TpgpLedger()
+    N.engine().tpgp_ledger_init(C.byref(led), validated)
+    return led
+
+
+def rotate(mp: Map, i, j, qi, qj, theta, opts, led):
+    return N.engine().tpgp_rotate(C.byref(mp.m), i, j, qi, qj, theta, C.byref(opts),
+                                  C.byref(led))
+
+
+def strang_step(mp: Map, terms_arr, count, dt, opts, led):
+    return N.engine().tpgp_strang_step(C.byref(mp.m), terms_arr, count, dt, C.byref(opts),
+                                       C.byref(led))
+
+
+def evolve(mp: Map, terms, dt, steps, opts, led, exact_ledger=False, pauli=1, normalizer=None,
+           initial_capacity=0):
+    """Resident evolution (tpgp_hip_evolve); returns (status, fid, fid_radius, active, accepted)."""
+    ta = N.terms_array(terms)
+    eo = N.EvolveOptions(1 if exact_ledger else 0, pauli,
+                         normalizer if normalizer else mp.n_spins, initial_capacity)
+    fid = np.full(steps + 1, np.nan)
+    rad = np.full(steps + 1, np.nan)
+    act = np.zeros(steps + 1, np.uint64)
+    acc = C.c_size_t()
+    st = N.engine().tpgp_hip_evolve(C.byref(mp.m), ta, len(terms), dt, steps, C.byref(opts),
+                                    C.byref(eo), C.byref(led), N.as_dp(fid), N.as_dp(rad),
+                                    act.ctypes.data_as(C.POINTER(C.c_size_t)), C.byref(acc))
+    return st, fid, rad, act, acc.value

@@ -1,0 +1,202 @@
+"""ctypes bindings to the native libraries (include/tpgp_hip.h, include/tpgp_engine.h).
+
+Loading fails loudly when a library is missing: the engine has no Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "lib")
+HIP_SO = os.path.join(LIB, "libtpgp_hip.so")
+ENGINE_SO = os.path.join(LIB, "libtpgp_engine.so")
+
+# statuses of include/tpgp_hip.h and include/tpgp_engine.h
+HIP_OK, HIP_EINVAL, HIP_ENOMEM, HIP_EDEVICE, HIP_ENUMERIC = range(5)
+HIP_ECAPACITY = 16
+OK, EINVAL, ENOMEM, ECAP, ENUMERIC, EBACKEND, EBUDGET = range(7)
+TPGP_HIP = 4
+
+_hip = None
+_eng = None
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise RuntimeError(f"native library {path} is not built; run python -m ddevol_amd.build "
+                           "(there is no CPU fallback)")
+    return C.CDLL(path)
+
+
+class TpgpMap(C.Structure):  # tpgp_rev.h:25-29 layout
+    _fields_ = [("n_spins", C.c_size_t), ("n_words", C.c_size_t), ("len", C.c_size_t),
+                ("capacity", C.c_size_t), ("keys", C.POINTER(C.c_uint64)),
+                ("values", C.POINTER(C.c_double))]
+
+
+class TpgpTerm(C.Structure):  # tpgp_rev.h:30-35
+    _fields_ = [("i", C.c_size_t), ("j", C.c_size_t), ("qi", C.c_uint), ("qj", C.c_uint),
+                ("h", C.c_double), ("layer", C.c_size_t)]
+
+
+class TpgpOptions(C.Structure):  # tpgp_rev.h:36-45 layout, GPU slot
+    _fields_ = [("threshold", C.c_double), ("max_terms", C.c_size_t), ("threads", C.c_uint),
+                ("backend", C.c_int), ("validated", C.c_int), ("max_drop_l2", C.c_double),
+                ("metal_context", C.c_void_p), ("hip_context", C.c_void_p)]
+
+
+class TpgpLedger(C.Structure):  # tpgp_rev.h:46-51
+    _fields_ = [("dropped_l1", C.c_double), ("dropped_l2", C.c_double),
+                ("dropped_l2_squared", C.c_double), ("arithmetic_l2", C.c_double),
+                ("rotations", C.c_size_t), ("removal_events", C.c_size_t),
+                ("peak_terms", C.c_size_t), ("arithmetic_enclosed", C.c_int)]
+
+    def bits(self):
+        f = np.array([self.dropped_l1, self.dropped_l2, self.dropped_l2_squared,
+                      self.arithmetic_l2], dtype=np.float64).view(np.uint64).tolist()
+        return tuple(f) + (self.rotations, self.removal_events, self.peak_terms,
+                           self.arithmetic_enclosed)
+
+
+class EvolveOptions(C.Structure):
+    _fields_ = [("exact_ledger", C.c_int), ("pauli", C.c_uint), ("normalizer", C.c_size_t),
+                ("initial_capacity", C.c_size_t)]
+
+
+class HipRotation(C.Structure):
+    _fields_ = [("active", C.c_int), ("qi", C.c_uint), ("qj", C.c_uint), ("validated", C.c_int),
+                ("log", C.c_int), ("i", C.c_size_t), ("j", C.c_size_t), ("c", C.c_double),
+                ("s", C.c_double), ("rc", C.c_double), ("rs", C.c_double),
+                ("threshold", C.c_double)]
+
+
+class HipRecord(C.Structure):
+    _fields_ = [("len_in", C.c_uint64), ("len_out", C.c_uint64), ("dropped", C.c_uint64),
+                ("l1_up", C.c_double), ("sq_up", C.c_double), ("rad_sq_up", C.c_double),
+                ("radii", C.c_uint64), ("executed", C.c_uint32), ("nonfinite", C.c_uint32),
+                ("overflow", C.c_uint32), ("rotation", C.c_uint32)]
+
+
+class HipLogEntry(C.Structure):
+    _fields_ = [("key", C.c_uint64 * 2), ("value", C.c_double), ("aux", C.c_double),
+                ("map", C.c_uint32), ("rotation", C.c_uint32), ("kind", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+HIP_SYMBOLS = ["tpgp_hip_compiled", "tpgp_hip_create", "tpgp_hip_destroy",
+               "tpgp_hip_device_name", "tpgp_hip_branches", "tpgp_hip_batch_create",
+               "tpgp_hip_batch_destroy", "tpgp_hip_batch_upload", "tpgp_hip_batch_download",
+               "tpgp_hip_batch_rotate", "tpgp_hip_batch_readout", "tpgp_hip_batch_sync",
+               "tpgp_hip_batch_commit", "tpgp_hip_batch_rollback", "tpgp_hip_batch_reserve",
+               "tpgp_hip_batch_info", "tpgp_hip_batch_stream"]
+ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
+                  "tpgp_default_options", "tpgp_ledger_init", "tpgp_map_init", "tpgp_map_free",
+                  "tpgp_map_clone", "tpgp_key_get", "tpgp_key_set", "tpgp_map_add",
+                  "tpgp_map_add_single", "tpgp_map_get", "tpgp_map_l2", "tpgp_map_single_sum",
+                  "tpgp_readout", "tpgp_pauli_phase", "tpgp_rotate", "tpgp_strang_step",
+                  "tpgp_strang_bound", "tpgp_strang_bound_taylor", "tpgp_bound_add_up",
+                  "tpgp_bound_fid_up", "tpgp_hip_evolve"]
+
+
+def hip():
+    """The device library with typed entry points."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    L = _load(HIP_SO)
+    p_u64, p_d, p_c = C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.c_char_p
+    sz = C.c_size_t
+
+    def f(name, res, *args):
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = list(args)
+
+    f("tpgp_hip_compiled", C.c_int)
+    f("tpgp_hip_create", C.c_void_p, C.c_int, p_c, sz)
+    f("tpgp_hip_destroy", None, C.c_void_p)
+    f("tpgp_hip_device_name", C.c_char_p, C.c_void_p)
+    f("tpgp_hip_branches", C.c_int, C.c_void_p, p_u64, p_d, sz, sz, sz, sz, sz, C.c_uint,
+      C.c_uint, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, p_u64, p_d, p_d, p_c,
+      sz)
+    f("tpgp_hip_batch_create", C.c_void_p, C.c_void_p, sz, sz, C.POINTER(sz), C.POINTER(sz),
+      p_c, sz)
+    f("tpgp_hip_batch_destroy", None, C.c_void_p)
+    f("tpgp_hip_batch_upload", C.c_int, C.c_void_p, sz, p_u64, p_d, sz, p_c, sz)
+    f("tpgp_hip_batch_download", C.c_int, C.c_void_p, sz, p_u64, p_d, sz, C.POINTER(sz), p_c, sz)
+    f("tpgp_hip_batch_rotate", C.c_int, C.c_void_p, C.POINTER(HipRotation), C.c_uint32, p_c, sz)
+    f("tpgp_hip_batch_readout", C.c_int, C.c_void_p, C.c_uint, C.POINTER(C.c_int), p_c, sz)
+    f("tpgp_hip_batch_sync", C.c_int, C.c_void_p, C.POINTER(HipRecord), sz, C.POINTER(sz), p_d,
+      sz, C.POINTER(sz), p_u64, C.POINTER(HipLogEntry), sz, C.POINTER(sz), p_c, sz)
+    f("tpgp_hip_batch_commit", C.c_int, C.c_void_p, C.POINTER(C.c_int), p_c, sz)
+    f("tpgp_hip_batch_rollback", C.c_int, C.c_void_p, p_c, sz)
+    f("tpgp_hip_batch_reserve", C.c_int, C.c_void_p, sz, sz, p_c, sz)
+    f("tpgp_hip_batch_info", C.c_int, C.c_void_p, sz, C.POINTER(sz), C.POINTER(sz))
+    f("tpgp_hip_batch_stream", C.c_void_p, C.c_void_p)
+    _hip = L
+    return L
+
+
+def engine():
+    """The C17 host engine (reference-shaped API) with typed entry points."""
+    global _eng
+    if _eng is not None:
+        return _eng
+    hip()  # resolve the dependency from the same directory first
+    L = _load(ENGINE_SO)
+    mp = C.POINTER(TpgpMap)
+    op = C.POINTER(TpgpOptions)
+    lp = C.POINTER(TpgpLedger)
+    sz = C.c_size_t
+
+    def f(name, res, *args):
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = list(args)
+
+    f("tpgp_status_string", C.c_char_p, C.c_int)
+    f("tpgp_has_mpfr", C.c_int)
+    f("tpgp_has_openmp", C.c_int)
+    f("tpgp_default_options", TpgpOptions)
+    f("tpgp_ledger_init", None, lp, C.c_int)
+    f("tpgp_map_init", C.c_int, mp, sz)
+    f("tpgp_map_free", None, mp)
+    f("tpgp_map_clone", C.c_int, mp, mp)
+    f("tpgp_key_get", C.c_uint, C.POINTER(C.c_uint64), sz)
+    f("tpgp_key_set", None, C.POINTER(C.c_uint64), sz, C.c_uint)
+    f("tpgp_map_add", C.c_int, mp, C.POINTER(C.c_uint64), C.c_double)
+    f("tpgp_map_add_single", C.c_int, mp, sz, C.c_uint, C.c_double)
+    f("tpgp_map_get", C.c_double, mp, C.POINTER(C.c_uint64))
+    f("tpgp_map_l2", C.c_double, mp)
+    f("tpgp_map_single_sum", C.c_double, mp, C.c_uint)
+    f("tpgp_readout", C.c_int, mp, C.c_uint, sz, C.POINTER(C.c_double), C.POINTER(C.c_double))
+    f("tpgp_pauli_phase", C.c_uint, C.c_uint, C.c_uint)
+    f("tpgp_rotate", C.c_int, mp, sz, sz, C.c_uint, C.c_uint, C.c_double, op, lp)
+    f("tpgp_strang_step", C.c_int, mp, C.POINTER(TpgpTerm), sz, C.c_double, op, lp)
+    f("tpgp_strang_bound", C.c_double, C.POINTER(TpgpTerm), sz, C.c_double)
+    f("tpgp_strang_bound_taylor", C.c_double, C.POINTER(TpgpTerm), sz, C.c_double)
+    f("tpgp_bound_add_up", C.c_double, C.c_double, C.c_double)
+    f("tpgp_bound_fid_up", C.c_double, C.c_double, sz)
+    f("tpgp_hip_evolve", C.c_int, mp, C.POINTER(TpgpTerm), sz, C.c_double, sz, op,
+      C.POINTER(EvolveOptions), lp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+      C.POINTER(sz), C.POINTER(sz))
+    _eng = L
+    return L
+
+
+def as_u64p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def as_dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def terms_array(terms):
+    arr = (TpgpTerm * max(len(terms), 1))()
+    for k, t in enumerate(terms):
+        arr[k] = TpgpTerm(t[0], t[1], t[2], t[3], t[4], k)
+    return arr

@@ -1,0 +1,119 @@
+/* tpgp_engine -- C17 host engine over the tpgp_hip device library.
+ *
+ * Keeps the reference core's API (engine/include/tpgp_rev.h:9-96) with the same names, types,
+ * struct layouts, statuses and transactional behaviour, for the functions on the propagation
+ * path.  The only backend is the GPU: options.backend must be TPGP_HIP, which occupies the
+ * reference's TPGP_CUDA slot (tpgp_rev.h:23), and options.hip_context occupies its cuda_context
+ * slot (tpgp_rev.h:44).  Other backends return TPGP_EBACKEND: there is no CPU fallback.
+ */
+#ifndef TPGP_ENGINE_H
+#define TPGP_ENGINE_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* tpgp_rev.h:9-17 */
+typedef enum {
+    TPGP_OK = 0,
+    TPGP_EINVAL,
+    TPGP_ENOMEM,
+    TPGP_ECAP,
+    TPGP_ENUMERIC,
+    TPGP_EBACKEND,
+    TPGP_EBUDGET
+} tpgp_status;
+/* tpgp_rev.h:18-24: only the GPU slot is implemented (value 4, the reference's TPGP_CUDA). */
+typedef enum { TPGP_HIP = 4 } tpgp_backend;
+/* tpgp_rev.h:25-29 */
+typedef struct {
+    size_t n_spins, n_words, len, capacity;
+    uint64_t *keys;
+    double *values;
+} tpgp_map;
+/* tpgp_rev.h:30-35 */
+typedef struct {
+    size_t i, j;
+    unsigned qi, qj;
+    double h;
+    size_t layer;
+} tpgp_term;
+/* tpgp_rev.h:36-45 (same layout; metal_context is unused) */
+typedef struct {
+    double threshold;
+    size_t max_terms;
+    unsigned threads;
+    int backend;
+    int validated;
+    double max_drop_l2;
+    void *metal_context;
+    void *hip_context; /* optional reusable tpgp_hip_context, caller-owned */
+} tpgp_options;
+/* tpgp_rev.h:46-51 */
+typedef struct {
+    double dropped_l1, dropped_l2, dropped_l2_squared;
+    double arithmetic_l2;
+    size_t rotations, removal_events, peak_terms;
+    int arithmetic_enclosed;
+} tpgp_ledger;
+
+/* tpgp_rev.h:53-57 */
+const char *tpgp_status_string(tpgp_status status);
+int tpgp_has_mpfr(void);
+int tpgp_has_openmp(void);
+tpgp_options tpgp_default_options(void);
+void tpgp_ledger_init(tpgp_ledger *ledger, int validated);
+/* tpgp_rev.h:58-73: host containers of the sorted map (zero-initialise before init). */
+tpgp_status tpgp_map_init(tpgp_map *map, size_t n_spins);
+void tpgp_map_free(tpgp_map *map);
+tpgp_status tpgp_map_clone(tpgp_map *dst, const tpgp_map *src);
+unsigned tpgp_key_get(const uint64_t *key, size_t site);
+void tpgp_key_set(uint64_t *key, size_t site, unsigned pauli);
+tpgp_status tpgp_map_add(tpgp_map *map, const uint64_t *key, double value);
+tpgp_status tpgp_map_add_single(tpgp_map *map, size_t site, unsigned pauli, double value);
+double tpgp_map_get(const tpgp_map *map, const uint64_t *key);
+double tpgp_map_l2(const tpgp_map *map);
+double tpgp_map_single_sum(const tpgp_map *map, unsigned pauli);
+/* tpgp_rev.h:74-78: MPFR-enclosed readout (EBACKEND without MPFR). */
+tpgp_status tpgp_readout(const tpgp_map *map, unsigned pauli, size_t normalizer,
+                         double *midpoint, double *radius);
+/* tpgp_rev.h:81-82 */
+unsigned tpgp_pauli_phase(unsigned q, unsigned p);
+/* tpgp_rev.h:83-87: transactional rotations on the GPU; ledgers are bit-identical to the
+ * reference CPU engine (dropped values are folded on the host in ascending key order). */
+tpgp_status tpgp_rotate(tpgp_map *map, size_t i, size_t j, unsigned qi, unsigned qj,
+                        double theta, const tpgp_options *options, tpgp_ledger *ledger);
+tpgp_status tpgp_strang_step(tpgp_map *map, const tpgp_term *terms, size_t count,
+                             double duration, const tpgp_options *options, tpgp_ledger *ledger);
+/* tpgp_rev.h:91-96 (harness bounds, evaluated on the host) */
+double tpgp_strang_bound(const tpgp_term *terms, size_t count, double duration);
+double tpgp_strang_bound_taylor(const tpgp_term *terms, size_t count, double duration);
+/* engine/include/residual_rev.h (tpgp_bound_add_up / tpgp_bound_fid_up, used for allowances) */
+double tpgp_bound_add_up(double a, double b);
+double tpgp_bound_fid_up(double a, size_t n);
+
+/* ---- resident evolution: the step loop of engine/src/cli_rev.c:575-670 on one GPU ---------
+ * Uploads `map` once, runs `steps` Strang steps with the map resident in HBM, and reads the FID
+ * F(k) = sum_s coeff(P_s)/normalizer before every step k = 0..steps (cli_rev.c:576-581).
+ * fid[k] is the readout midpoint (MPFR readout when available, else the compensated
+ * long-double sum / normalizer); fid_radius[k] its enclosure radius (NaN without MPFR);
+ * active[k] the map length.  Arrays hold steps+1 entries; any may be NULL.
+ * exact_ledger: 1 = bit-identical ledger (host fold of logged drops, slower);
+ *               0 = device ledger: rigorous upward-rounded sums in a fixed tree order.
+ * On failure the map and ledger keep the last accepted step; *accepted counts steps done. */
+typedef struct {
+    int exact_ledger;
+    unsigned pauli;       /* readout symbol (1 = X) */
+    size_t normalizer;    /* readout divisor (n_spins for the global initial state) */
+    size_t initial_capacity; /* 0: automatic */
+} tpgp_evolve_options;
+tpgp_status tpgp_hip_evolve(tpgp_map *map, const tpgp_term *terms, size_t count, double dt,
+                            size_t steps, const tpgp_options *options,
+                            const tpgp_evolve_options *evolve, tpgp_ledger *ledger, double *fid,
+                            double *fid_radius, size_t *active, size_t *accepted);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,155 @@
+/* tpgp_hip -- MI355X (gfx950) Pauli-propagation branch engine, C ABI.
+ *
+ * Part 1 replaces the reference's CUDA branch backend one-for-one:
+ *   engine/include/tpgp_cuda_rev.h:20-48 (tpgp_cuda_compiled/create/destroy/device_name/branches),
+ *   implemented there by engine/src/cuda_rev.cu:117-272.  Same arguments, same byte-identical
+ *   output slot order (2k, 2k+1), same statuses, same error-buffer convention, no CPU fallback.
+ *
+ * Part 2 is the resident extension: a batch of Pauli maps kept in HBM across rotations.  It
+ * replaces the per-rotation host work of engine/src/tpgp_rev.c:535-771 (branch, stable sort,
+ * merge, truncation, ledger inputs) and the FID scan of tpgp_rev.c:311-388 with device kernels;
+ * keys/coefficients go in, merged and truncated maps plus FID samples come out.
+ *
+ * All pointers are host pointers; sizes are element counts.  No torch types cross this boundary.
+ */
+#ifndef TPGP_HIP_H
+#define TPGP_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ============================ Part 1: branch backend ==================================== */
+
+typedef struct tpgp_hip_context tpgp_hip_context;
+/* Values match tpgp_cuda_status (engine/include/tpgp_cuda_rev.h:11-17). */
+typedef enum {
+    TPGP_HIP_OK = 0,
+    TPGP_HIP_EINVAL,
+    TPGP_HIP_ENOMEM,
+    TPGP_HIP_EDEVICE,
+    TPGP_HIP_ENUMERIC
+} tpgp_hip_status;
+
+/* Replaces tpgp_cuda_compiled (tpgp_cuda_rev.h:21): build capability, not device presence. */
+int tpgp_hip_compiled(void);
+/* Replaces tpgp_cuda_create (tpgp_cuda_rev.h:22; cuda_rev.cu:118-157).  Device indices follow
+ * HIP_VISIBLE_DEVICES.  NULL plus a message on failure. */
+tpgp_hip_context *tpgp_hip_create(int device_index, char *error, size_t error_capacity);
+/* Replaces tpgp_cuda_destroy (tpgp_cuda_rev.h:23; cuda_rev.cu:158-169).  NULL-safe. */
+void tpgp_hip_destroy(tpgp_hip_context *context);
+/* Replaces tpgp_cuda_device_name (tpgp_cuda_rev.h:24): "unavailable" for NULL. */
+const char *tpgp_hip_device_name(const tpgp_hip_context *context);
+
+/* Replaces tpgp_cuda_branches (tpgp_cuda_rev.h:43-49; cuda_rev.cu:174-272).
+ * For each input term k: output slots 2k and 2k+1.  Commuting P gives (P,v,0),(P,0,0);
+ * anticommuting P gives (P, RN(v*c), radius_c) and (key(-iQP), sign*RN(v*s), radius_s).
+ * Host-provided c, s, rc, rs; the GPU evaluates no trigonometric function.  Arrays are disjoint
+ * host arrays: count*n_words keys, count values, 2*count*n_words output keys, 2*count output
+ * values and radii.  EINVAL: bad dimensions/sites/generator/factors, NULL arrays, size overflow,
+ * overlapping arrays, nonzero key padding.  ENUMERIC: nonfinite input, product or radius. */
+tpgp_hip_status tpgp_hip_branches(tpgp_hip_context *context, const uint64_t *keys,
+                                  const double *values, size_t count, size_t n_words,
+                                  size_t n_spins, size_t i, size_t j, unsigned qi, unsigned qj,
+                                  double c, double s, double rc, double rs, int enclose,
+                                  uint64_t *output_keys, double *output_values,
+                                  double *output_radii, char *error, size_t error_capacity);
+
+/* ============================ Part 2: resident batch ==================================== */
+
+/* A batch holds n_maps independent Pauli maps (one per FID: a powder orientation, a coupling
+ * draw, a served request) with a common key width n_words (1 for N<=32, 2 for N<=64).  Each
+ * map lives in a fixed slot of three HBM arenas: the committed state plus two working buffers.
+ * Rotations enqueue device work without host synchronisation; sync() collects per-rotation
+ * records; commit()/rollback() make multi-rotation updates transactional per map. */
+typedef struct tpgp_hip_batch tpgp_hip_batch;
+
+/* Extended statuses of the resident API (beyond tpgp_hip_status). */
+enum { TPGP_HIP_ECAPACITY = 16 /* a map outgrew its slot; roll back and grow */ };
+
+/* Per-map rotation request.  The term (i, j, qi, qj) acts on sites i != j < n_spins of that map;
+ * c, s (and rc, rs when validated) are the host's trigonometric factors for 2*h*duration
+ * (engine/src/tpgp_rev.c:484-534).  threshold: drop non-identity merged |v| <= threshold
+ * (tpgp_rev.c:721).  log: also record dropped values and merged radii (for bit-exact ledgers). */
+typedef struct {
+    int active;
+    unsigned qi, qj;
+    int validated;
+    int log;
+    size_t i, j;
+    double c, s, rc, rs;
+    double threshold;
+} tpgp_hip_rotation;
+
+/* Per-map, per-rotation record filled by sync() (inputs of the ledger of tpgp_rev.c:730-757). */
+typedef struct {
+    uint64_t len_in, len_out;  /* map length before/after the rotation */
+    uint64_t dropped;          /* number of truncated (ledger) terms */
+    double l1_up;              /* upward-rounded sum of |dropped| (deterministic tree order) */
+    double sq_up;              /* upward-rounded sum of dropped^2 */
+    double rad_sq_up;          /* upward-rounded sum of merged radius^2 (validated mode) */
+    uint64_t radii;            /* number of nonzero merged radii */
+    uint32_t executed;         /* 0: the rotation was a no-op (empty map) */
+    uint32_t nonfinite;        /* a raw value, merged value or radius was not finite */
+    uint32_t overflow;         /* output exceeded the map's slot capacity */
+    uint32_t rotation;         /* caller's rotation sequence number */
+} tpgp_hip_record;
+
+/* One dropped coefficient or merged radius of a logged rotation.  key is in the reference
+ * layout (site s at bits 2(s%32) of word s/32); kind 0 = dropped |v| (value = |v|, and the
+ * signed v in aux), kind 1 = nonzero merged radius. */
+typedef struct {
+    uint64_t key[2];
+    double value;
+    double aux;
+    uint32_t map, rotation, kind, pad;
+} tpgp_hip_log_entry;
+
+tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *context, size_t n_words, size_t n_maps,
+                                      const size_t *n_spins, const size_t *capacities,
+                                      char *error, size_t error_capacity);
+void tpgp_hip_batch_destroy(tpgp_hip_batch *batch);
+/* Replace map `map`'s committed state by a canonical sorted map (reference key order and layout,
+ * zero padding, finite nonzero values).  Synchronous. */
+int tpgp_hip_batch_upload(tpgp_hip_batch *batch, size_t map, const uint64_t *keys,
+                          const double *values, size_t len, char *error, size_t error_capacity);
+/* Copy map `map`'s committed state out in reference order.  *len receives the length; arrays
+ * must hold `capacity` entries (a NULL array with capacity 0 only queries the length). */
+int tpgp_hip_batch_download(tpgp_hip_batch *batch, size_t map, uint64_t *keys, double *values,
+                            size_t capacity, size_t *len, char *error, size_t error_capacity);
+/* Enqueue one rotation on every map whose request is active (requests[n_maps]).  rotation is a
+ * caller sequence number echoed in the records.  Asynchronous. */
+int tpgp_hip_batch_rotate(tpgp_hip_batch *batch, const tpgp_hip_rotation *requests,
+                          uint32_t rotation, char *error, size_t error_capacity);
+/* Enqueue a readout of the n_spins single-site coefficients of symbol `pauli` (1..3) for every
+ * map with active[m] != 0 (NULL = all), in the current working state.  Asynchronous. */
+int tpgp_hip_batch_readout(tpgp_hip_batch *batch, unsigned pauli, const int *active,
+                           char *error, size_t error_capacity);
+/* Wait for enqueued work.  Returns the number of records per map since the previous sync in
+ * *n_records (records[m*max_records + r], r < n_records), readouts in readouts[m*max_readouts*64
+ * + q*64 + s] with *n_readouts, and working lengths in lengths[m].  Any output pointer may be
+ * NULL.  Logged entries are returned through log (up to log_capacity) with *n_log. */
+int tpgp_hip_batch_sync(tpgp_hip_batch *batch, tpgp_hip_record *records, size_t max_records,
+                        size_t *n_records, double *readouts, size_t max_readouts,
+                        size_t *n_readouts, uint64_t *lengths, tpgp_hip_log_entry *log,
+                        size_t log_capacity, size_t *n_log, char *error, size_t error_capacity);
+/* After sync: make the working state of maps with mask[m] != 0 (NULL = all) their committed
+ * state; roll back maps with mask[m] == 0. */
+int tpgp_hip_batch_commit(tpgp_hip_batch *batch, const int *mask, char *error,
+                          size_t error_capacity);
+/* Discard all work since the last commit (maps return to their committed state). */
+int tpgp_hip_batch_rollback(tpgp_hip_batch *batch, char *error, size_t error_capacity);
+/* Grow map `map`'s slot to at least `capacity` terms, keeping its committed state. */
+int tpgp_hip_batch_reserve(tpgp_hip_batch *batch, size_t map, size_t capacity, char *error,
+                           size_t error_capacity);
+/* Committed length and slot capacity of a map. */
+int tpgp_hip_batch_info(const tpgp_hip_batch *batch, size_t map, size_t *committed_len,
+                        size_t *capacity);
+/* Stream the batch enqueues on (a hipStream_t), for callers that time or capture it. */
+void *tpgp_hip_batch_stream(const tpgp_hip_batch *batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
