@@ -56,6 +56,13 @@ static double upper_l2(const double *v, size_t n) {
 /* Upper bound of sqrt(S) from an upward-rounded S: one step above the rounded root. */
 static double upper_sqrt(double s) { return s == 0 ? 0 : nextafter(sqrt(s), INFINITY); }
 
+/* Message of the last device-side failure in this thread (empty when none). */
+static _Thread_local char last_error[512];
+const char *tpgp_hip_last_error(void) { return last_error; }
+static void note(const char *where, const char *what) {
+    snprintf(last_error, sizeof(last_error), "%s: %s", where, what);
+}
+
 /* ---- statuses, capabilities, options (tpgp_rev.c:117-148) -------------------------------- */
 const char *tpgp_status_string(tpgp_status s) {
     static const char *names[] = {"ok",
@@ -505,13 +512,17 @@ static tpgp_status session_open(session *S, const tpgp_map *m, const tpgp_option
     S->ctx = o->hip_context;
     if (!S->ctx) {
         S->ctx = tpgp_hip_create(0, err, sizeof(err));
-        if (!S->ctx) return TPGP_EBACKEND;
+        if (!S->ctx) {
+            note("context", err);
+            return TPGP_EBACKEND;
+        }
         S->own_ctx = 1;
     }
     S->cap = cap_hint;
     size_t n = m->n_spins;
     S->batch = tpgp_hip_batch_create(S->ctx, m->n_words, 1, &n, &S->cap, err, sizeof(err));
     if (!S->batch) {
+        note("batch", err);
         session_close(S);
         return TPGP_ENOMEM;
     }
@@ -523,6 +534,7 @@ static tpgp_status session_open(session *S, const tpgp_map *m, const tpgp_option
     }
     int hs = tpgp_hip_batch_upload(S->batch, 0, m->keys, m->values, m->len, err, sizeof(err));
     if (hs) {
+        note("upload", err);
         session_close(S);
         return map_status(hs);
     }
@@ -532,7 +544,10 @@ static tpgp_status session_grow(session *S, size_t need) {
     char err[512];
     size_t cap = S->cap * 2 > need ? S->cap * 2 : need;
     int hs = tpgp_hip_batch_reserve(S->batch, 0, cap, err, sizeof(err));
-    if (hs) return map_status(hs);
+    if (hs) {
+        note("reserve", err);
+        return map_status(hs);
+    }
     S->cap = cap;
     return TPGP_OK;
 }
@@ -550,6 +565,7 @@ static tpgp_status session_download(session *S, tpgp_map *m) {
     }
     hs = tpgp_hip_batch_download(S->batch, 0, t.keys, t.values, len, &t.len, err, sizeof(err));
     if (hs) {
+        note("download", err);
         tpgp_map_free(&t);
         return map_status(hs);
     }
@@ -590,7 +606,10 @@ static tpgp_status run_plans(session *S, const rot_plan *plans, size_t n, int ex
         r.validated = o->validated;
         r.log = exact;
         int hs = tpgp_hip_batch_rotate(S->batch, &r, seq0 + (uint32_t)k, err, sizeof(err));
-        if (hs) return map_status(hs);
+        if (hs) {
+            note("rotate", err);
+            return map_status(hs);
+        }
         enq++;
     }
     size_t nrec = 0, nlog = 0;
@@ -602,8 +621,14 @@ static tpgp_status run_plans(session *S, const rot_plan *plans, size_t n, int ex
     int hs = tpgp_hip_batch_sync(S->batch, S->rec, S->max_rec, &nrec, NULL, 0, NULL, NULL,
                                  exact ? S->log : NULL, exact ? S->log_cap : 0, &nlog, err,
                                  sizeof(err));
-    if (hs) return hs == TPGP_HIP_ECAPACITY ? TPGP_EBACKEND : map_status(hs);
-    if (nrec != enq) return TPGP_EBACKEND;
+    if (hs) {
+        note("sync", err);
+        return hs == TPGP_HIP_ECAPACITY ? TPGP_EBACKEND : map_status(hs);
+    }
+    if (nrec != enq) {
+        note("sync", "record count mismatch");
+        return TPGP_EBACKEND;
+    }
     if (exact && nlog) qsort(S->log, nlog, sizeof(*S->log), cmp_entry);
     if (S->scratch_cap < nlog + 1) {
         free(S->scratch);
@@ -627,7 +652,10 @@ static tpgp_status run_plans(session *S, const rot_plan *plans, size_t n, int ex
             *need = (size_t)r->len_out;
             return TPGP_OK;
         }
-        if (fs) return (tpgp_status)fs;
+        if (fs) {
+            if (fs == TPGP_EBACKEND) note("ledger", "logged entries disagree with the record");
+            return (tpgp_status)fs;
+        }
     }
     return TPGP_OK;
 }
@@ -740,6 +768,7 @@ tpgp_status tpgp_hip_evolve(tpgp_map *m, const tpgp_term *t, size_t n, double dt
     for (;;) {
         hs = tpgp_hip_batch_readout(S.batch, eo->pauli, NULL, err, sizeof(err));
         if (hs) {
+            note("readout", err);
             s = map_status(hs);
             break;
         }
@@ -748,6 +777,7 @@ tpgp_status tpgp_hip_evolve(tpgp_map *m, const tpgp_term *t, size_t n, double dt
         hs = tpgp_hip_batch_sync(S.batch, S.rec, S.max_rec, &nrec, rd, 2, &nrd, &len, NULL, 0,
                                  NULL, err, sizeof(err));
         if (hs) {
+            note("readout sync", err);
             s = map_status(hs);
             break;
         }
@@ -781,6 +811,7 @@ tpgp_status tpgp_hip_evolve(tpgp_map *m, const tpgp_term *t, size_t n, double dt
             break;
         }
         if (tpgp_hip_batch_commit(S.batch, NULL, err, sizeof(err))) {
+            note("commit", err);
             s = TPGP_EBACKEND;
             break;
         }

@@ -303,6 +303,7 @@ struct BatchDev {
     LogDev *log;              // [log_cap]
     uint64_t *log_count;      // [1]
     uint64_t log_cap;
+    uint64_t *debug;          // [16] validation report (debug[0] != 0: violation found)
 };
 
 __device__ __forceinline__ uint32_t find_bucket(const uint64_t *start, uint32_t n, uint64_t g) {
@@ -564,7 +565,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
         };
         while (hi > lo) {
             uint64_t span = hi - lo;
-            if (span <= 64) {
+            if (span < 64) {  // lane `span` maps to hi (P(hi) is true): the ballot is never 0
                 uint64_t xi = lo + lane_id();
                 bool p = xi < hi ? pred(xi) : true;
                 uint64_t bal = __ballot(p);
@@ -1012,6 +1013,50 @@ __global__ void k_readout(BatchDev b, const PassMap *pm, unsigned pauli) {
     b.rd[((uint64_t)m * b.max_rd + P.rec_slot) * 64 + s] = out;
 }
 
+// Validation (TPGP_HIP_VALIDATE=1): every active map's working buffer must be strictly sorted
+// and its class regions must match the classes of its keys.  The first violation is recorded.
+template <int W>
+__global__ void k_validate(BatchDev b, const PassMap *pm, uint32_t tag) {
+    const uint32_t M = b.n_maps;
+    for (uint32_t m = 0; m < M; ++m) {
+        const PassMap &P = pm[m];
+        if (!P.active) continue;
+        const uint32_t buf = P.out_buf, n = b.slot[m].n_spins;
+        const uint64_t len = b.len[m * 3 + buf];
+        const uint64_t *reg = b.regions + ((uint64_t)m * 3 + buf) * 17;
+        const uint64_t *keys = b.arena[buf].keys + b.slot[m].off * W;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            bool bad = reg[0] != 0 || reg[16] != len;
+            for (int c = 0; c < 16; ++c) bad = bad || reg[c] > reg[c + 1];
+            if (bad && atomicCAS((unsigned long long *)&b.debug[0], 0ull,
+                                 (unsigned long long)tag) == 0ull) {
+                b.debug[1] = m;
+                b.debug[2] = ~0ull;
+                b.debug[3] = len;
+                b.debug[4] = reg[16];
+            }
+        }
+        for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < len;
+             e += (uint64_t)gridDim.x * blockDim.x) {
+            Key<W> k = kload<W>(keys + e * W);
+            unsigned cls = (kdigit<W>(k, n - 1) << 2) | kdigit<W>(k, n - 2);
+            bool bad = e < reg[cls] || e >= reg[cls + 1];
+            if (e > 0) bad = bad || !klt<W>(kload<W>(keys + (e - 1) * W), k);
+            if (bad && atomicCAS((unsigned long long *)&b.debug[0], 0ull,
+                                 (unsigned long long)tag) == 0ull) {
+                b.debug[1] = m;
+                b.debug[2] = e;
+                b.debug[3] = len;
+                b.debug[4] = cls;
+                b.debug[5] = reg[cls];
+                b.debug[6] = reg[cls + 1];
+                b.debug[7] = k.w[0];
+                b.debug[8] = e > 0 ? keys[(e - 1) * W] : 0;
+            }
+        }
+    }
+}
+
 __global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
          t += (uint64_t)gridDim.x * blockDim.x)
@@ -1292,6 +1337,8 @@ struct tpgp_hip_batch {
     std::vector<LoggedLayout> logged;
     std::vector<void *> allocations;
     bool poisoned = false;
+    bool validate = false;  // TPGP_HIP_VALIDATE=1: check order and regions after every pass
+    uint32_t pass_count = 0;
 };
 
 namespace {
@@ -1356,6 +1403,7 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     d.log_cap = 1u << 20;
     ALLOC(d.log, d.log_cap);
     ALLOC(d.log_count, 1);
+    ALLOC(d.debug, 16);
     ALLOC(b->d_stage, b->stage_cap);
 #undef ALLOC
     e = hipHostMalloc((void **)&b->h_stage, b->stage_cap * sizeof(PassMap), hipHostMallocDefault);
@@ -1378,6 +1426,7 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, d.overflow, (uint64_t)M, 0u);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, d.log_count, 1ull, 0ull);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, d.counter, 4ull, 0u);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, d.debug, 16ull, 0ull);
     if ((e = hipStreamSynchronize(st)) != hipSuccess)
         return hip_fail(e, "HIP batch initialisation", error, cap);
     b->epoch = 0;
@@ -1449,6 +1498,15 @@ hipError_t enqueue_partition(tpgp_hip_batch *b, const std::vector<int> &want,
     } else {
         hipLaunchKernelGGL(k_part_scatter<2>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
                            (const PassMap *)d);
+    }
+    if (b->validate) {
+        uint32_t tag = (++b->pass_count << 1) | 0u;
+        if (b->W == 1)
+            hipLaunchKernelGGL(k_validate<1>, dim3(256), dim3(kBlock), 0, st, b->dev,
+                               (const PassMap *)d, tag);
+        else
+            hipLaunchKernelGGL(k_validate<2>, dim3(256), dim3(kBlock), 0, st, b->dev,
+                               (const PassMap *)d, tag);
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1532,6 +1590,8 @@ extern "C" tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *ctx, size_t n
     b->max_rd = n_maps <= 16 ? 1024u : 64u;
     b->stage_cap = std::max<size_t>(n_maps * 64, 1u << 16);
     b->grid = (unsigned)ctx->cus * 8;
+    const char *v = getenv("TPGP_HIP_VALIDATE");
+    b->validate = v && *v && *v != '0';
     std::lock_guard<std::mutex> lock(ctx->mutex);
     DeviceGuard guard(ctx->device);
     if (!guard.ok || allocate_device(b, error, cap) != TPGP_HIP_OK) {
@@ -1697,6 +1757,15 @@ extern "C" int tpgp_hip_batch_rotate(tpgp_hip_batch *b, const tpgp_hip_rotation 
                            (const PassMap *)d, epoch);
     }
     hipLaunchKernelGGL(k_rot_finish, dim3(M), dim3(kBlock), 0, st, b->dev, (const PassMap *)d);
+    if (b->validate) {
+        uint32_t tag = (++b->pass_count << 1) | 1u;
+        if (b->W == 1)
+            hipLaunchKernelGGL(k_validate<1>, dim3(256), dim3(kBlock), 0, st, b->dev,
+                               (const PassMap *)d, tag);
+        else
+            hipLaunchKernelGGL(k_validate<2>, dim3(256), dim3(kBlock), 0, st, b->dev,
+                               (const PassMap *)d, tag);
+    }
     if ((e = hipGetLastError()) != hipSuccess)
         return hip_fail(e, "HIP rotation launch", error, cap);
     for (uint32_t m = 0; m < M; ++m) {
@@ -1802,6 +1871,22 @@ extern "C" int tpgp_hip_batch_sync(tpgp_hip_batch *b, tpgp_hip_record *records,
     uint32_t internal = 0;
     e = hipMemcpy(&internal, b->dev.counter + 1, sizeof(uint32_t), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "HIP status transfer", error, cap);
+    if (b->validate) {
+        uint64_t dbg[16];
+        e = hipMemcpy(dbg, b->dev.debug, sizeof(dbg), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "HIP validation transfer", error, cap);
+        if (dbg[0]) {
+            msg(error, cap,
+                "Validation failed after pass %llu (%s): map %llu index %lld len %llu class %llu "
+                "region [%llu,%llu) key %#llx prev %#llx",
+                (unsigned long long)(dbg[0] >> 1), (dbg[0] & 1) ? "rotation" : "partition",
+                (unsigned long long)dbg[1], (long long)dbg[2], (unsigned long long)dbg[3],
+                (unsigned long long)dbg[4], (unsigned long long)dbg[5],
+                (unsigned long long)dbg[6], (unsigned long long)dbg[7],
+                (unsigned long long)dbg[8]);
+            return TPGP_HIP_EDEVICE;
+        }
+    }
     if (internal) {
         msg(error, cap, "Internal merge-path inconsistency detected on the device.");
         return TPGP_HIP_EDEVICE;

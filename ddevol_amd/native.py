@@ -98,7 +98,7 @@ ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_map_add_single", "tpgp_map_get", "tpgp_map_l2", "tpgp_map_single_sum",
                   "tpgp_readout", "tpgp_pauli_phase", "tpgp_rotate", "tpgp_strang_step",
                   "tpgp_strang_bound", "tpgp_strang_bound_taylor", "tpgp_bound_add_up",
-                  "tpgp_bound_fid_up", "tpgp_hip_evolve"]
+                  "tpgp_bound_fid_up", "tpgp_hip_evolve", "tpgp_hip_last_error"]
 
 
 def hip():
@@ -183,6 +183,7 @@ def engine():
     f("tpgp_hip_evolve", C.c_int, mp, C.POINTER(TpgpTerm), sz, C.c_double, sz, op,
       C.POINTER(EvolveOptions), lp, C.POINTER(C.c_double), C.POINTER(C.c_double),
       C.POINTER(sz), C.POINTER(sz))
+    f("tpgp_hip_last_error", C.c_char_p)
     _eng = L
     return L
 

@@ -113,6 +113,9 @@ tpgp_status tpgp_hip_evolve(tpgp_map *map, const tpgp_term *terms, size_t count,
                             const tpgp_evolve_options *evolve, tpgp_ledger *ledger, double *fid,
                             double *fid_radius, size_t *active, size_t *accepted);
 
+/* Message of the last device-side failure on this thread ("" when none). */
+const char *tpgp_hip_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,7 @@ def random_map(rng, n, count, max_weight=5, tiny_fraction=0.2):
     keys.add((0,) * words)
     while len(keys) < count:
         w = int(rng.integers(1, max_weight + 1))
-        sites = rng.choice(n, size=min(w, n), replace=False)
+        sites = [int(x) for x in rng.choice(n, size=min(w, n), replace=False)]
         k = [0] * words
         for s in sites:
             k[s // 32] |= int(rng.integers(1, 4)) << (2 * (s % 32))
@@ -193,3 +193,20 @@ def test_evolve_ring12(hip_ctx, oracle, exact):
         for a, b in [(ol.dropped_l1, gl.dropped_l1), (ol.dropped_l2, gl.dropped_l2),
                      (ol.dropped_l2_squared, gl.dropped_l2_squared)]:
             assert b >= a * (1 - 1e-12) and abs(b - a) <= 1e-9 * max(a, 1e-300)
+
+
+@pytest.mark.parametrize("n,count", [(16, 60000), (40, 30000)])
+def test_rotate_large_multi_tile(hip_ctx, oracle, n, count):
+    """Classes spanning many merge tiles and partition tiles (regression: tile splits)."""
+    rng = np.random.default_rng(77 + n)
+    keys, vals = random_map(rng, n, count, max_weight=7)
+    for (i, j, qi, qj) in [(0, 1, 1, 1), (n - 1, 3, 2, 2), (7, n // 2, 3, 3), (2, n - 2, 1, 0)]:
+        om, gm = both_maps(oracle, n, keys, vals)
+        oo = oracle.options(threshold=1e-5)
+        go = E.options(threshold=1e-5, context=hip_ctx)
+        ol, gl = oracle.ledger(0), E.ledger(0)
+        assert om.rotate(i, j, qi, qj, .377, oo, ol) == 0
+        assert E.rotate(gm, i, j, qi, qj, .377, go, gl) == 0
+        assert_same_map(om, gm)
+        assert ol.bits() == gl.bits()
+
