@@ -574,53 +574,67 @@ static tpgp_status session_download(session *S, tpgp_map *m) {
     return TPGP_OK;
 }
 
-/* Run planned rotations [0, n) on the session's working state; fold them into *trial in order.
- * Returns the first failing status (the reference stops at its first failing rotation). */
-static tpgp_status run_plans(session *S, const rot_plan *plans, size_t n, int exact,
-                             const tpgp_options *o, tpgp_ledger *trial, uint32_t seq0,
-                             int *capacity_needed, size_t *need) {
+/* Enqueue planned rotations [0, n) (the valid prefix, skipping no-ops); returns enqueued count
+ * in *enq. */
+static tpgp_status enqueue_plans(session *S, const rot_plan *plans, size_t n, int exact,
+                                 const tpgp_options *o, uint32_t seq0, size_t *enq) {
+    char err[512];
+    *enq = 0;
+    tpgp_hip_rotation *req = malloc((n + 1) * sizeof(*req));
+    uint32_t *seq = malloc((n + 1) * sizeof(*seq));
+    if (!req || !seq) {
+        free(req);
+        free(seq);
+        return TPGP_ENOMEM;
+    }
+    size_t cnt = 0;
+    for (size_t k = 0; k < n; k++) {
+        if (plans[k].pre) break;
+        if (plans[k].noop) continue;
+        tpgp_hip_rotation *r = req + cnt;
+        memset(r, 0, sizeof(*r));
+        r->active = 1;
+        r->i = plans[k].i;
+        r->j = plans[k].j;
+        r->qi = plans[k].qi;
+        r->qj = plans[k].qj;
+        r->c = plans[k].c;
+        r->s = plans[k].s;
+        r->rc = plans[k].rc;
+        r->rs = plans[k].rs;
+        r->threshold = o->threshold;
+        r->validated = o->validated;
+        r->log = exact;
+        seq[cnt++] = seq0 + (uint32_t)k;
+    }
+    int hs = cnt ? tpgp_hip_batch_rotate_many(S->batch, req, seq, cnt, err, sizeof(err)) : 0;
+    free(req);
+    free(seq);
+    if (hs) {
+        note("rotate", err);
+        return map_status(hs);
+    }
+    *enq = cnt;
+    return TPGP_OK;
+}
+
+/* Wait for the enqueued rotations (and an optional readout), then fold their records into
+ * *trial in rotation order.  The reference stops at its first failing rotation; so does this. */
+static tpgp_status sync_fold(session *S, const rot_plan *plans, size_t n, size_t enq, int exact,
+                             const tpgp_options *o, tpgp_ledger *trial, double *readout,
+                             uint64_t *len, int *capacity_needed, size_t *need,
+                             uint64_t *updates, uint64_t *in_out, uint64_t *executed) {
     char err[512];
     *capacity_needed = 0;
-    /* the device runs the valid prefix; a host-invalid rotation ends the sequence */
-    size_t limit = n;
-    for (size_t k = 0; k < n; k++)
-        if (plans[k].pre) {
-            limit = k;
-            break;
-        }
-    size_t enq = 0;
-    for (size_t k = 0; k < limit; k++) {
-        if (plans[k].noop) continue;
-        tpgp_hip_rotation r;
-        memset(&r, 0, sizeof(r));
-        r.active = 1;
-        r.i = plans[k].i;
-        r.j = plans[k].j;
-        r.qi = plans[k].qi;
-        r.qj = plans[k].qj;
-        r.c = plans[k].c;
-        r.s = plans[k].s;
-        r.rc = plans[k].rc;
-        r.rs = plans[k].rs;
-        r.threshold = o->threshold;
-        r.validated = o->validated;
-        r.log = exact;
-        int hs = tpgp_hip_batch_rotate(S->batch, &r, seq0 + (uint32_t)k, err, sizeof(err));
-        if (hs) {
-            note("rotate", err);
-            return map_status(hs);
-        }
-        enq++;
-    }
-    size_t nrec = 0, nlog = 0;
+    size_t nrec = 0, nlog = 0, nrd = 0;
     if (exact && !S->log) {
         S->log_cap = 1u << 20;
         S->log = malloc(S->log_cap * sizeof(*S->log));
         if (!S->log) return TPGP_ENOMEM;
     }
-    int hs = tpgp_hip_batch_sync(S->batch, S->rec, S->max_rec, &nrec, NULL, 0, NULL, NULL,
-                                 exact ? S->log : NULL, exact ? S->log_cap : 0, &nlog, err,
-                                 sizeof(err));
+    int hs = tpgp_hip_batch_sync(S->batch, S->rec, S->max_rec, &nrec, readout, readout ? 2 : 0,
+                                 &nrd, len, exact ? S->log : NULL, exact ? S->log_cap : 0, &nlog,
+                                 err, sizeof(err));
     if (hs) {
         note("sync", err);
         return hs == TPGP_HIP_ECAPACITY ? TPGP_EBACKEND : map_status(hs);
@@ -637,6 +651,7 @@ static tpgp_status run_plans(session *S, const rot_plan *plans, size_t n, int ex
         if (!S->scratch) return TPGP_ENOMEM;
     }
     size_t r_at = 0, e_at = 0;
+    uint64_t up = 0, io = 0, ex = 0;
     for (size_t k = 0; k < n; k++) {
         const rot_plan *p = plans + k;
         if (!ledger_ok(trial)) return TPGP_EINVAL; /* tpgp_rev.c:543-547 */
@@ -656,8 +671,25 @@ static tpgp_status run_plans(session *S, const rot_plan *plans, size_t n, int ex
             if (fs == TPGP_EBACKEND) note("ledger", "logged entries disagree with the record");
             return (tpgp_status)fs;
         }
+        up += r->len_in;
+        io += r->len_in + r->len_out;
+        ex++;
     }
+    if (updates) *updates += up;
+    if (in_out) *in_out += io;
+    if (executed) *executed += ex;
     return TPGP_OK;
+}
+
+/* Run planned rotations [0, n) on the session's working state; fold them into *trial. */
+static tpgp_status run_plans(session *S, const rot_plan *plans, size_t n, int exact,
+                             const tpgp_options *o, tpgp_ledger *trial, uint32_t seq0,
+                             int *capacity_needed, size_t *need) {
+    size_t enq = 0;
+    tpgp_status s = enqueue_plans(S, plans, n, exact, o, seq0, &enq);
+    if (s) return s;
+    return sync_fold(S, plans, n, enq, exact, o, trial, NULL, NULL, capacity_needed, need, NULL,
+                     NULL, NULL);
 }
 
 /* Transactional sequence of rotations on a host map (tpgp_rotate / tpgp_strang_step). */
@@ -735,95 +767,163 @@ tpgp_status tpgp_strang_step(tpgp_map *m, const tpgp_term *t, size_t n, double d
 }
 
 /* ---- resident evolution ------------------------------------------------------------------- */
-tpgp_status tpgp_hip_evolve(tpgp_map *m, const tpgp_term *t, size_t n, double dt, size_t steps,
-                            const tpgp_options *o, const tpgp_evolve_options *eo, tpgp_ledger *l,
-                            double *fid, double *fid_radius, size_t *active, size_t *accepted) {
-    if (accepted) *accepted = 0;
-    if (!map_ok(m) || (!t && n) || !o || !l || !eo || !isfinite(dt) || eo->pauli < 1 ||
+struct tpgp_evolution {
+    session S;
+    rot_plan *plans;
+    size_t nplans;
+    tpgp_options o;
+    tpgp_evolve_options eo;
+    size_t n_spins;
+    tpgp_evolution_counters c;
+};
+
+tpgp_status tpgp_evolution_open(tpgp_evolution **out, const tpgp_map *m, const tpgp_term *t,
+                                size_t n, double dt, const tpgp_options *o,
+                                const tpgp_evolve_options *eo) {
+    if (!out) return TPGP_EINVAL;
+    *out = NULL;
+    if (!map_ok(m) || (!t && n) || !o || !eo || !isfinite(dt) || eo->pauli < 1 ||
         eo->pauli > 3 || !eo->normalizer || m->n_spins > 64)
         return TPGP_EINVAL;
     if (o->backend != TPGP_HIP) return TPGP_EBACKEND;
     double half = dt * 0.5;
-    if (half * 2 != dt) return TPGP_ENUMERIC;
-    rot_plan *plans = malloc((2 * n + 1) * sizeof(*plans));
-    if (!plans) return TPGP_ENOMEM;
+    if (half * 2 != dt) return TPGP_ENUMERIC; /* tpgp_rev.c:780-784 */
+    tpgp_evolution *ev = calloc(1, sizeof(*ev));
+    if (!ev) return TPGP_ENOMEM;
+    ev->nplans = 2 * n;
+    ev->plans = malloc((2 * n + 1) * sizeof(*ev->plans));
+    if (!ev->plans) {
+        free(ev);
+        return TPGP_ENOMEM;
+    }
     for (size_t k = 0; k < 2 * n; k++) {
         const tpgp_term *term = k < n ? t + k : t + (2 * n - 1 - k);
-        plan_rotation(plans + k, m->n_spins, term->i, term->j, term->qi, term->qj, term->h, half,
-                      o);
+        plan_rotation(ev->plans + k, m->n_spins, term->i, term->j, term->qi, term->qj, term->h,
+                      half, o);
     }
+    ev->o = *o;
+    ev->eo = *eo;
+    ev->n_spins = m->n_spins;
     size_t cap = eo->initial_capacity ? eo->initial_capacity : 4 * m->len + (1u << 16);
-    session S;
-    tpgp_status s = session_open(&S, m, o, cap, 2 * n + 2);
+    tpgp_status s = session_open(&ev->S, m, o, cap, 2 * n + 2);
     if (s) {
-        free(plans);
+        free(ev->plans);
+        free(ev);
         return s;
     }
-    double rd[2 * 64];
+    *out = ev;
+    return TPGP_OK;
+}
+
+/* `steps` Strang steps with a readout before each and after the last: one host sync per step
+ * (the readout of the committed state rides with the step's rotations). */
+tpgp_status tpgp_evolution_run(tpgp_evolution *ev, size_t steps, tpgp_ledger *l, double *fid,
+                               double *fid_radius, size_t *active, size_t *accepted) {
+    if (accepted) *accepted = 0;
+    if (!ev || !l) return TPGP_EINVAL;
+    session *S = &ev->S;
+    const tpgp_options *o = &ev->o;
+    const int exact = ev->eo.exact_ledger;
     char err[512];
-    size_t nrd = 0;
-    size_t step = 0;
-    /* readout of the committed state before step `step` */
-    int hs;
-    for (;;) {
-        hs = tpgp_hip_batch_readout(S.batch, eo->pauli, NULL, err, sizeof(err));
-        if (hs) {
-            note("readout", err);
-            s = map_status(hs);
-            break;
-        }
-        uint64_t len = 0;
-        size_t nrec = 0;
-        hs = tpgp_hip_batch_sync(S.batch, S.rec, S.max_rec, &nrec, rd, 2, &nrd, &len, NULL, 0,
-                                 NULL, err, sizeof(err));
-        if (hs) {
-            note("readout sync", err);
-            s = map_status(hs);
-            break;
-        }
-        double mid = 0, rad = NAN;
-        if (tpgp_has_mpfr()) {
-            s = readout_values(rd, m->n_spins, eo->normalizer, &mid, &rad);
-            if (s) break;
-        } else {
-            mid = neumaier(rd, m->n_spins) / (double)eo->normalizer;
-        }
-        if (fid) fid[step] = mid;
-        if (fid_radius) fid_radius[step] = rad;
-        if (active) active[step] = (size_t)len;
-        if (step == steps) break;
-        /* one Strang step; retried from the committed state if a slot overflows */
-        tpgp_ledger trial;
+    double rd[2 * 64];
+    tpgp_status s = TPGP_OK;
+    for (size_t step = 0;; step++) {
+        const int more = step < steps;
+        tpgp_ledger trial = *l;
+        int grow = 0;
+        size_t need = 0;
+        uint64_t len = 0, up = 0, io = 0, ex = 0;
         for (;;) {
+            int hs = tpgp_hip_batch_readout(S->batch, ev->eo.pauli, NULL, err, sizeof(err));
+            if (hs) {
+                note("readout", err);
+                return map_status(hs);
+            }
+            size_t enq = 0;
+            if (more && (s = enqueue_plans(S, ev->plans, ev->nplans, exact, o, 0, &enq))) break;
             trial = *l;
-            int grow = 0;
-            size_t need = 0;
-            s = run_plans(&S, plans, 2 * n, eo->exact_ledger, o, &trial, 0, &grow, &need);
+            up = io = ex = 0;
+            s = sync_fold(S, ev->plans, more ? ev->nplans : 0, enq, exact, o, &trial, rd, &len,
+                          &grow, &need, &up, &io, &ex);
             if (!s && grow) {
-                tpgp_hip_batch_rollback(S.batch, err, sizeof(err));
-                if ((s = session_grow(&S, need + need / 2 + 1024))) break;
+                tpgp_hip_batch_rollback(S->batch, err, sizeof(err));
+                if ((s = session_grow(S, need + need / 2 + 1024))) break;
                 continue;
             }
             break;
         }
-        if (s) {
-            tpgp_hip_batch_rollback(S.batch, err, sizeof(err));
-            break;
+        /* readout of the state before this step (it was taken before the rotations ran) */
+        if (!s || !more) {
+            double mid = 0, rad = NAN;
+            tpgp_status rs = TPGP_OK;
+            if (tpgp_has_mpfr())
+                rs = readout_values(rd, ev->n_spins, ev->eo.normalizer, &mid, &rad);
+            else
+                mid = neumaier(rd, ev->n_spins) / (double)ev->eo.normalizer;
+            if (rs && !s) s = rs;
+            if (fid) fid[step] = mid;
+            if (fid_radius) fid_radius[step] = rad;
+            if (active) active[step] = (size_t)len;
         }
-        if (tpgp_hip_batch_commit(S.batch, NULL, err, sizeof(err))) {
+        if (s) {
+            tpgp_hip_batch_rollback(S->batch, err, sizeof(err));
+            return s;
+        }
+        if (!more) {
+            /* the final readout also committed nothing new: length is the committed length */
+            return TPGP_OK;
+        }
+        if (tpgp_hip_batch_commit(S->batch, NULL, err, sizeof(err))) {
             note("commit", err);
-            s = TPGP_EBACKEND;
-            break;
+            return TPGP_EBACKEND;
         }
         *l = trial;
-        step++;
-        if (accepted) *accepted = step;
+        ev->c.steps++;
+        ev->c.rotations += ex;
+        ev->c.updates += up;
+        ev->c.in_out += io;
+        if (accepted) *accepted = step + 1;
     }
-    tpgp_status ds = session_download(&S, m);
-    if (!s) s = ds;
-    session_close(&S);
-    free(plans);
+}
+
+tpgp_status tpgp_evolution_close(tpgp_evolution *ev, tpgp_map *m) {
+    if (!ev) return TPGP_EINVAL;
+    tpgp_status s = TPGP_OK;
+    if (m) {
+        if (m->n_spins != ev->n_spins) s = TPGP_EINVAL;
+        else s = session_download(&ev->S, m);
+    }
+    session_close(&ev->S);
+    free(ev->plans);
+    free(ev);
     return s;
+}
+
+void tpgp_evolution_counters_get(const tpgp_evolution *ev, tpgp_evolution_counters *c) {
+    if (ev && c) *c = ev->c;
+}
+int tpgp_evolution_profile(tpgp_evolution *ev, int enable) {
+    return ev ? tpgp_hip_batch_profile(ev->S.batch, enable) : TPGP_HIP_EINVAL;
+}
+int tpgp_evolution_profile_read(const tpgp_evolution *ev, unsigned kind, double *ms,
+                                uint64_t *launches) {
+    return ev ? tpgp_hip_batch_profile_read(ev->S.batch, kind, ms, launches) : TPGP_HIP_EINVAL;
+}
+size_t tpgp_evolution_key_words(const tpgp_evolution *ev) {
+    return ev ? (ev->n_spins + 31) / 32 : 0;
+}
+
+tpgp_status tpgp_hip_evolve(tpgp_map *m, const tpgp_term *t, size_t n, double dt, size_t steps,
+                            const tpgp_options *o, const tpgp_evolve_options *eo, tpgp_ledger *l,
+                            double *fid, double *fid_radius, size_t *active, size_t *accepted) {
+    if (accepted) *accepted = 0;
+    if (!l) return TPGP_EINVAL;
+    tpgp_evolution *ev = NULL;
+    tpgp_status s = tpgp_evolution_open(&ev, m, t, n, dt, o, eo);
+    if (s) return s;
+    s = tpgp_evolution_run(ev, steps, l, fid, fid_radius, active, accepted);
+    tpgp_status cs = tpgp_evolution_close(ev, m);
+    return s ? s : cs;
 }
 
 /* ---- Trotter bounds and allowance helpers (tpgp_rev.c:858-961; residual_rev.c:195-201) ----- */

@@ -293,7 +293,6 @@ struct BatchDev {
     uint64_t *tile_base;      // [maxT*16]
     uint64_t *map_hist;       // [M*16]
     uint64_t *splits;         // [maxT+1]
-    uint64_t *status;         // [maxT]
     uint64_t *class_out;      // [M*16]
     TileLedger *tile_led;     // [maxT]
     uint32_t *counter;        // [4]
@@ -539,8 +538,227 @@ __device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm, uin
     x.k = g - b.tstart[t];
 }
 
-// One wave per tile: merge-path split (number of A elements before diagonal d0), found with a
-// 64-ary search.  P(x) = A[x] > T(D[d0-x-1]); the split is the first x with P true.
+// Merge-path split, computed by one wave: the number of A elements among the first d elements
+// of merge(A, T(D)) (ties: A first).  P(x) = A[x] > T(D[d-x-1]) is monotone; the split is the
+// first x with P true, found with 64-ary search rounds.  All lanes return the same value.
+template <int W>
+__device__ uint64_t split_search(const uint64_t *keys, const MergeCtx &x, const Key<W> &q,
+                                 uint64_t d) {
+    uint64_t lo = d > x.nD ? d - x.nD : 0, hi = d < x.nA ? d : x.nA;
+    auto pred = [&](uint64_t xi) -> bool {
+        Key<W> a = kload<W>(keys + (x.A0 + xi) * W);
+        Key<W> dk = kxor<W>(kload<W>(keys + (x.D0 + d - xi - 1) * W), q);
+        return klt<W>(dk, a);
+    };
+    while (hi > lo) {
+        uint64_t span = hi - lo;
+        if (span < 64) {  // lane `span` maps to hi (P(hi) is true): the ballot is never 0
+            uint64_t xi = lo + lane_id();
+            bool p = xi < hi ? pred(xi) : true;
+            uint64_t bal = __ballot(p);
+            lo = lo + (uint64_t)(__ffsll((unsigned long long)bal) - 1);
+            break;
+        }
+        uint64_t xi = lo + ((span * lane_id()) >> 6);
+        uint64_t bal = __ballot(pred(xi));
+        if (bal == 0) {
+            lo = lo + ((span * 63) >> 6) + 1;
+        } else {
+            int f = __ffsll((unsigned long long)bal) - 1;
+            if (f == 0) break;  // P(lo) true: the split is lo
+            uint64_t xf = lo + ((span * (uint64_t)f) >> 6);
+            uint64_t xp = lo + ((span * (uint64_t)(f - 1)) >> 6);
+            lo = xp + 1;
+            hi = xf;
+        }
+    }
+    return lo;
+}
+
+// Tile inputs in LDS: A at [0, na), T(D) at [na, na+nd), the D element after the tile at na+nd
+// (pairs with the tile's last A element), the A element before the tile at kMergeTile+1 (its
+// pair, if any, was emitted by the previous tile).
+template <int W> struct TileIn {
+    uint64_t keys[(kMergeTile + 2) * W];
+    double vals[kMergeTile + 2];
+};
+struct TileMisc {
+    unsigned scratch[8];
+    uint64_t a0, a1;
+    double red[3][kBlock / 64];
+    uint64_t redn[2][kBlock / 64];
+    unsigned nonfinite;
+};
+
+struct TileGeom {
+    uint64_t d0, d1, a0, a1, b0, b1;
+    uint32_t na, nd;
+    bool has_next, has_prev;
+};
+
+// Range checks of a tile's splits; a violation means an internal inconsistency (reported).
+__device__ __forceinline__ bool tile_ok(const MergeCtx &x, const TileGeom &t) {
+    return !(t.a0 > t.d0 || t.a1 > t.d1 || t.a1 < t.a0 || t.a0 > x.nA || t.a1 > x.nA ||
+             t.d0 - t.a0 > x.nD || t.d1 - t.a1 > x.nD || t.d1 - t.a1 < t.d0 - t.a0 ||
+             t.d1 - t.d0 > (uint64_t)kMergeTile);
+}
+
+template <int W>
+__device__ void load_tile(const BatchDev &b, const PassMap &P, const MergeCtx &x,
+                          const Key<W> &q, TileGeom &t, TileIn<W> &in) {
+    t.b0 = t.d0 - t.a0;
+    t.b1 = t.d1 - t.a1;
+    t.na = (uint32_t)(t.a1 - t.a0);
+    t.nd = (uint32_t)(t.b1 - t.b0);
+    t.has_next = x.anti && t.b1 < x.nD;
+    t.has_prev = x.anti && t.a0 > 0;
+    const uint64_t *ikeys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
+    const double *ivals = b.arena[P.in_buf].vals + b.slot[x.m].off;
+    for (uint32_t i = threadIdx.x; i < t.na; i += kBlock) {
+        kstore<W>(in.keys + i * W, kload<W>(ikeys + (x.A0 + t.a0 + i) * W));
+        in.vals[i] = ivals[x.A0 + t.a0 + i];
+    }
+    for (uint32_t i = threadIdx.x; i < t.nd; i += kBlock) {
+        kstore<W>(in.keys + (t.na + i) * W, kxor<W>(kload<W>(ikeys + (x.D0 + t.b0 + i) * W), q));
+        in.vals[t.na + i] = ivals[x.D0 + t.b0 + i];
+    }
+    if (threadIdx.x == 0) {
+        if (t.has_next) {
+            kstore<W>(in.keys + (t.na + t.nd) * W, kxor<W>(kload<W>(ikeys + (x.D0 + t.b1) * W), q));
+            in.vals[t.na + t.nd] = ivals[x.D0 + t.b1];
+        }
+        if (t.has_prev)
+            kstore<W>(in.keys + (kMergeTile + 1) * W, kload<W>(ikeys + (x.A0 + t.a0 - 1) * W));
+    }
+}
+
+// One output key of the rotation: value and merged radius from its a-slot (own key) and/or its
+// d-slot (partner), exactly as the reference merges two raw branches (tpgp_rev.c:458-464,
+// 695-706); the merged value is one IEEE addition, independent of order.
+struct Coef {
+    double c, s, rc, rs, thr;
+    bool anti, validated, dneg;
+};
+__device__ __forceinline__ void merged(const Coef &k, bool hasA, double va, bool hasD, double vd,
+                                       double &v, double &r, bool &bad) {
+    r = 0.0;
+    if (!k.anti) {
+        v = va;
+        return;
+    }
+    double xa = 0.0, yd = 0.0, ra = 0.0, rd = 0.0;
+    if (hasA) {
+        xa = va * k.c;
+        if (k.validated) ra = upadd(upmul(__builtin_fabs(va), k.rc), round_radius(xa));
+    }
+    if (hasD) {
+        double y = vd * k.s;
+        yd = k.dneg ? -y : y;
+        if (k.validated) rd = upadd(upmul(__builtin_fabs(vd), k.rs), round_radius(y));
+        if (!__builtin_isfinite(y)) bad = true;
+    }
+    if (!__builtin_isfinite(xa) || !__builtin_isfinite(ra) || !__builtin_isfinite(rd)) bad = true;
+    if (hasA && hasD) {
+        v = xa + yd;
+        r = upadd(ra, rd);
+        if (k.validated && xa != 0.0 && yd != 0.0) r = upadd(r, round_radius(v));
+    } else if (hasA) {
+        v = xa;
+        r = ra;
+    } else {
+        v = yd;
+        r = rd;
+    }
+    if (!__builtin_isfinite(v) || !__builtin_isfinite(r)) bad = true;
+}
+
+// Walk this thread's kMergeItems merge-path positions of the tile, calling f(key, hasA, va,
+// hasD, vd) once per output key in ascending order.
+template <int W, typename F>
+__device__ __forceinline__ void walk_tile(const TileIn<W> &in, const TileGeom &t, F &&f) {
+    const uint32_t nloc = (uint32_t)(t.d1 - t.d0);
+    const uint32_t dt = threadIdx.x * kMergeItems;
+    if (dt >= nloc) return;
+    const uint32_t na = t.na, nd = t.nd, nd_look = nd + (t.has_next ? 1u : 0u);
+    auto Akey = [&](uint32_t i) { return kload<W>(in.keys + i * W); };
+    auto Dkey = [&](uint32_t j) { return kload<W>(in.keys + (na + j) * W); };
+    const uint32_t limit = (nloc - dt) < (uint32_t)kMergeItems ? (nloc - dt) : (uint32_t)kMergeItems;
+    uint32_t lo = dt > nd ? dt - nd : 0, hi = dt < na ? dt : na;
+    while (lo < hi) {  // local split: first i with A[i] > D[dt-i-1]
+        uint32_t mid = (lo + hi) >> 1;
+        if (klt<W>(Dkey(dt - mid - 1), Akey(mid)))
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    uint32_t i = lo, j = dt - lo, used = 0;
+    // a D element equal to the A element just before this split was consumed with it
+    if (j < nd && (i > 0 || t.has_prev)) {
+        Key<W> pa = i > 0 ? Akey(i - 1) : kload<W>(in.keys + (kMergeTile + 1) * W);
+        if (keq<W>(pa, Dkey(j))) {
+            ++j;
+            ++used;
+        }
+    }
+    while (used < limit) {
+        bool takeA;
+        if (i >= na)
+            takeA = false;
+        else if (j >= nd)
+            takeA = true;
+        else
+            takeA = !klt<W>(Dkey(j), Akey(i));  // A[i] <= D[j]
+        if (takeA) {
+            Key<W> ka = Akey(i);
+            if (j < nd_look && keq<W>(ka, Dkey(j))) {
+                f(ka, true, in.vals[i], true, in.vals[na + j]);
+                ++i;
+                ++j;
+                used += 2;
+            } else {
+                f(ka, true, in.vals[i], false, 0.0);
+                ++i;
+                ++used;
+            }
+        } else {
+            f(Dkey(j), false, 0.0, true, in.vals[na + j]);
+            ++j;
+            ++used;
+        }
+    }
+}
+
+__device__ __forceinline__ Coef coef_of(const PassMap &P, const MergeCtx &x) {
+    Coef k;
+    k.c = P.c;
+    k.s = P.s;
+    k.rc = P.rc;
+    k.rs = P.rs;
+    k.thr = P.threshold;
+    k.anti = x.anti;
+    k.validated = P.flags & 1u;
+    const unsigned ph = (pauli_phase(P.qt, x.cq >> 2) + pauli_phase(P.qs, x.cq & 3u)) & 3u;
+    k.dneg = ph != 1;
+    return k;
+}
+
+__device__ __forceinline__ void log_entry(const BatchDev &b, uint64_t k0, uint64_t k1, double v,
+                                          double aux, uint32_t m, uint32_t rot, uint32_t kind) {
+    unsigned long long at = atomicAdd((unsigned long long *)b.log_count, 1ull);
+    if (at < b.log_cap) {
+        LogDev &L = b.log[at];
+        L.key[0] = k0;
+        L.key[1] = k1;
+        L.value = v;
+        L.aux = aux;
+        L.map = m;
+        L.rotation = rot;
+        L.kind = kind;
+    }
+}
+
+// Pass 0 of a rotation: one wave per tile computes the tile's merge-path start split.  All
+// tiles search at once, so the dependent global loads of the searches overlap.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap *pm) {
     const uint64_t ntiles = b.tstart[b.n_maps * 16];
@@ -548,325 +766,81 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
     for (uint64_t g = (uint64_t)blockIdx.x * (kBlock / 64) + wave_id(); g < ntiles; g += waves) {
         MergeCtx x;
         locate(b, pm, g, x);
-        const PassMap &P = pm[x.m];
         const uint64_t d0 = x.k * kMergeTile;
-        if (!x.anti) {
-            if (lane_id() == 0) b.splits[g] = d0;
-            continue;
+        uint64_t a0 = d0;
+        if (x.anti) {
+            const PassMap &P = pm[x.m];
+            const uint32_t n = b.slot[x.m].n_spins;
+            const uint64_t *ikeys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
+            const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
+            a0 = split_search<W>(ikeys, x, q, d0);
         }
-        const uint32_t n = b.slot[x.m].n_spins;
-        const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
-        const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
-        uint64_t lo = d0 > x.nD ? d0 - x.nD : 0, hi = d0 < x.nA ? d0 : x.nA;
-        auto pred = [&](uint64_t xi) -> bool {
-            Key<W> a = kload<W>(keys + (x.A0 + xi) * W);
-            Key<W> d = kxor<W>(kload<W>(keys + (x.D0 + d0 - xi - 1) * W), q);
-            return klt<W>(d, a);
-        };
-        while (hi > lo) {
-            uint64_t span = hi - lo;
-            if (span < 64) {  // lane `span` maps to hi (P(hi) is true): the ballot is never 0
-                uint64_t xi = lo + lane_id();
-                bool p = xi < hi ? pred(xi) : true;
-                uint64_t bal = __ballot(p);
-                lo = lo + (uint64_t)(__ffsll((unsigned long long)bal) - 1);
-                hi = lo;
-                break;
-            }
-            uint64_t xi = lo + ((span * lane_id()) >> 6);
-            bool p = pred(xi);
-            uint64_t bal = __ballot(p);
-            if (bal == 0) {
-                lo = lo + ((span * 63) >> 6) + 1;
-            } else {
-                int f = __ffsll((unsigned long long)bal) - 1;
-                if (f == 0) {
-                    hi = lo;
-                    break;
-                }
-                uint64_t xf = lo + ((span * (uint64_t)f) >> 6);
-                uint64_t xp = lo + ((span * (uint64_t)(f - 1)) >> 6);
-                lo = xp + 1;
-                hi = xf;
-            }
-        }
-        if (lane_id() == 0) b.splits[g] = lo;
+        if (lane_id() == 0) b.splits[g] = a0;
     }
 }
 
-// Look-back status word: [epoch:24][flag:2][value:38]
-constexpr uint64_t kFlagAgg = 1, kFlagInc = 2;
-__device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint64_t flag, uint64_t v) {
-    return ((uint64_t)epoch << 40) | (flag << 38) | (v & ((1ull << 38) - 1));
-}
-__device__ __forceinline__ void lb_publish(uint64_t *p, uint64_t w) {
-    __hip_atomic_store(p, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint64_t lb_wait(uint64_t *p, uint32_t epoch) {
-    for (;;) {
-        uint64_t w = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if ((uint32_t)(w >> 40) == epoch && ((w >> 38) & 3u)) return w;
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
-
-template <int W> struct MergeShared {
-    uint64_t keys[(kMergeTile + 2) * W];  // inputs; after the merge, the compacted outputs
-    double vals[kMergeTile + 2];
-    uint64_t okeys[kMergeTile * W];       // outputs of thread t at [t*items, t*items + nout)
-    double ovals[kMergeTile];
-    unsigned scratch[8];
-    uint64_t tile;
-    uint64_t prefix;
-    double red[3][kBlock / 64];
-    uint64_t redn[2][kBlock / 64];
-    unsigned nonfinite;
-};
-
+// Pass 1 of a rotation: per tile, the merge-path splits, the number of kept outputs, the
+// ledger partial sums (deterministic upward-rounded tree) and, when asked, the log.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_merge(BatchDev b, const PassMap *pm,
-                                                     uint32_t epoch) {
-    __shared__ MergeShared<W> sh;
+__global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap *pm) {
+    __shared__ TileIn<W> in;
+    __shared__ TileMisc misc;
     const uint64_t ntiles = b.tstart[b.n_maps * 16];
-    for (;;) {
-        if (threadIdx.x == 0) sh.tile = atomicAdd(&b.counter[0], 1u);
-        __syncthreads();
-        const uint64_t g = sh.tile;
-        __syncthreads();
-        if (g >= ntiles) return;
+    for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
         MergeCtx x;
         locate(b, pm, g, x);
         const PassMap &P = pm[x.m];
         const uint32_t n = b.slot[x.m].n_spins;
+        const uint64_t *ikeys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
+        const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
+        TileGeom t;
         const uint64_t total = x.nA + x.nD;
-        const uint64_t d0 = x.k * kMergeTile;
-        const uint64_t d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
-        const uint64_t tclass_end = b.tstart[x.m * 16 + x.c + 1];
-        const uint64_t a0 = b.splits[g];
-        const uint64_t a1 = (g + 1 < tclass_end) ? b.splits[g + 1] : x.nA;
-        // Guard: a split outside its range would overrun the tile buffers; report instead.
-        if (a0 > d0 || a1 > d1 || a1 < a0 || a0 > x.nA || a1 > x.nA || d0 - a0 > x.nD ||
-            d1 - a1 > x.nD || d1 - a1 < d0 - a0 || d1 - d0 > (uint64_t)kMergeTile) {
+        t.d0 = x.k * kMergeTile;
+        t.d1 = t.d0 + kMergeTile < total ? t.d0 + kMergeTile : total;
+        (void)ikeys;
+        t.a0 = b.splits[g];
+        t.a1 = (g + 1 < b.tstart[x.m * 16 + x.c + 1]) ? b.splits[g + 1] : x.nA;
+        if (threadIdx.x == 0) misc.nonfinite = 0;
+        if (!tile_ok(x, t)) {
             if (threadIdx.x == 0) {
                 atomicOr(&b.counter[1], 1u);
-                lb_publish(&b.status[g], lb_pack(epoch, b.tstart[x.m * 16] == g ? kFlagInc
-                                                                                : kFlagAgg, 0));
+                b.tile_hist[g] = 0;
+                b.tile_led[g] = TileLedger{};
             }
             __syncthreads();
             continue;
         }
-        const uint64_t b0 = d0 - a0, b1 = d1 - a1;
-        const uint32_t na = (uint32_t)(a1 - a0), nd = (uint32_t)(b1 - b0);
-        const uint64_t *ikeys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
-        const double *ivals = b.arena[P.in_buf].vals + b.slot[x.m].off;
-        const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
-        // LDS: A at [0, na), D at [na, na+nd), D lookahead at na+nd, A predecessor at TILE+1
-        for (uint32_t t = threadIdx.x; t < na; t += kBlock) {
-            Key<W> k = kload<W>(ikeys + (x.A0 + a0 + t) * W);
-            kstore<W>(sh.keys + t * W, k);
-            sh.vals[t] = ivals[x.A0 + a0 + t];
-        }
-        for (uint32_t t = threadIdx.x; t < nd; t += kBlock) {
-            Key<W> k = kxor<W>(kload<W>(ikeys + (x.D0 + b0 + t) * W), q);
-            kstore<W>(sh.keys + (na + t) * W, k);
-            sh.vals[na + t] = ivals[x.D0 + b0 + t];
-        }
-        const bool has_next = x.anti && b1 < x.nD;
-        const bool has_prev = x.anti && a0 > 0;
-        if (threadIdx.x == 0) {
-            if (has_next) {
-                Key<W> k = kxor<W>(kload<W>(ikeys + (x.D0 + b1) * W), q);
-                kstore<W>(sh.keys + (na + nd) * W, k);
-                sh.vals[na + nd] = ivals[x.D0 + b1];
-            }
-            if (has_prev) kstore<W>(sh.keys + (kMergeTile + 1) * W,
-                                    kload<W>(ikeys + (x.A0 + a0 - 1) * W));
-            sh.nonfinite = 0;
-        }
+        load_tile<W>(b, P, x, q, t, in);
         __syncthreads();
-
-        // ---- per-thread merge of kMergeItems merge-path positions
-        const uint32_t nloc = (uint32_t)(d1 - d0);
-        const uint32_t dt = threadIdx.x * kMergeItems;
-        const double cc = P.c, ss = P.s, rc = P.rc, rs = P.rs, thr = P.threshold;
-        const bool validated = P.flags & 1u, want_log = (P.flags & 2u) != 0;
-        const unsigned ph = (pauli_phase(P.qt, x.cq >> 2) + pauli_phase(P.qs, x.cq & 3u)) & 3u;
-        const bool dneg = ph != 1;
-        const uint32_t nd_look = nd + (has_next ? 1u : 0u);
-        const uint32_t obase = threadIdx.x * kMergeItems;
+        const Coef k = coef_of(P, x);
+        const bool want_log = (P.flags & 2u) != 0;
         unsigned nout = 0;
         uint64_t n_drop = 0, n_rad = 0;
         double l1 = 0.0, sq = 0.0, rsq = 0.0;
         bool bad = false;
-        auto Akey = [&](uint32_t i) { return kload<W>(sh.keys + i * W); };
-        auto Dkey = [&](uint32_t j) { return kload<W>(sh.keys + (na + j) * W); };
-        auto emit = [&](const Key<W> &key, bool hasA, double va, bool hasD, double vd) {
-            double v, r = 0.0;
-            if (!x.anti) {
-                v = va;
-            } else {
-                double xa = 0.0, yd = 0.0, ra = 0.0, rd = 0.0;
-                if (hasA) {
-                    xa = va * cc;
-                    if (validated) ra = upadd(upmul(__builtin_fabs(va), rc), round_radius(xa));
-                }
-                if (hasD) {
-                    double y = vd * ss;
-                    yd = dneg ? -y : y;
-                    if (validated) rd = upadd(upmul(__builtin_fabs(vd), rs), round_radius(y));
-                    if (!__builtin_isfinite(y)) bad = true;
-                }
-                if (!__builtin_isfinite(xa) || !__builtin_isfinite(ra) ||
-                    !__builtin_isfinite(rd))
-                    bad = true;
-                if (hasA && hasD) {
-                    v = xa + yd;
-                    r = upadd(ra, rd);
-                    if (validated && xa != 0.0 && yd != 0.0) r = upadd(r, round_radius(v));
-                } else if (hasA) {
-                    v = xa;
-                    r = ra;
-                } else {
-                    v = yd;
-                    r = rd;
-                }
-            }
-            if (!__builtin_isfinite(v) || !__builtin_isfinite(r)) bad = true;
+        walk_tile<W>(in, t, [&](const Key<W> &key, bool hasA, double va, bool hasD, double vd) {
+            double v, r;
+            merged(k, hasA, va, hasD, vd, v, r, bad);
+            const uint64_t k0 = key.w[0], k1 = W > 1 ? key.w[W - 1] : 0;
             if (r != 0.0) {
                 n_rad++;
                 rsq = upadd(rsq, upmul(r, r));
-                if (want_log) {
-                    unsigned long long at = atomicAdd((unsigned long long *)b.log_count, 1ull);
-                    if (at < b.log_cap) {
-                        LogDev &L = b.log[at];
-                        L.key[0] = key.w[0];
-                        L.key[1] = W > 1 ? key.w[W - 1] : 0;
-                        L.value = r;
-                        L.aux = 0.0;
-                        L.map = x.m;
-                        L.rotation = P.rot_seq;
-                        L.kind = 1;
-                    }
-                }
+                if (want_log) log_entry(b, k0, k1, r, 0.0, x.m, P.rot_seq, 1);
             }
-            if (v == 0.0) return;
-            if (!kzero<W>(key) && __builtin_fabs(v) <= thr) {
+            if (v == 0.0) return;  // exact cancellation: dropped silently
+            if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) {
                 n_drop++;
                 l1 = upadd(l1, __builtin_fabs(v));
                 sq = upadd(sq, upmul(v, v));
-                if (want_log) {
-                    unsigned long long at = atomicAdd((unsigned long long *)b.log_count, 1ull);
-                    if (at < b.log_cap) {
-                        LogDev &L = b.log[at];
-                        L.key[0] = key.w[0];
-                        L.key[1] = W > 1 ? key.w[W - 1] : 0;
-                        L.value = __builtin_fabs(v);
-                        L.aux = v;
-                        L.map = x.m;
-                        L.rotation = P.rot_seq;
-                        L.kind = 0;
-                    }
-                }
+                if (want_log) log_entry(b, k0, k1, __builtin_fabs(v), v, x.m, P.rot_seq, 0);
                 return;
             }
-            kstore<W>(sh.okeys + (obase + nout) * W, key);
-            sh.ovals[obase + nout] = v;
             nout++;
-        };
-        if (dt < nloc) {
-            const uint32_t limit = (nloc - dt) < (uint32_t)kMergeItems ? (nloc - dt)
-                                                                        : (uint32_t)kMergeItems;
-            // local split: first i in [max(0,dt-nd), min(dt,na)] with A[i] > D[dt-i-1]
-            uint32_t lo = dt > nd ? dt - nd : 0, hi = dt < na ? dt : na;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (klt<W>(Dkey(dt - mid - 1), Akey(mid)))
-                    hi = mid;
-                else
-                    lo = mid + 1;
-            }
-            uint32_t i = lo, j = dt - lo, used = 0;
-            // a D element equal to the A element just before this split was consumed with it
-            if (j < nd) {
-                bool prev_ok = i > 0 || has_prev;
-                if (prev_ok) {
-                    Key<W> pa = i > 0 ? Akey(i - 1) : kload<W>(sh.keys + (kMergeTile + 1) * W);
-                    if (keq<W>(pa, Dkey(j))) {
-                        ++j;
-                        ++used;
-                    }
-                }
-            }
-            while (used < limit) {
-                bool takeA;
-                if (i >= na)
-                    takeA = false;
-                else if (j >= nd)
-                    takeA = true;
-                else
-                    takeA = !klt<W>(Dkey(j), Akey(i));  // A[i] <= D[j]
-                if (takeA) {
-                    Key<W> ka = Akey(i);
-                    if (j < nd_look && keq<W>(ka, Dkey(j))) {
-                        emit(ka, true, sh.vals[i], true, sh.vals[na + j]);
-                        ++i;
-                        ++j;
-                        used += 2;
-                    } else {
-                        emit(ka, true, sh.vals[i], false, 0.0);
-                        ++i;
-                        ++used;
-                    }
-                } else {
-                    emit(Dkey(j), false, 0.0, true, sh.vals[na + j]);
-                    ++j;
-                    ++used;
-                }
-            }
-        }
-        // ---- tile aggregate, look-back within the map's tile chain
+        });
         unsigned tot;
-        unsigned myoff = block_exclusive_scan(nout, sh.scratch, &tot);
-        if (threadIdx.x == 0) {
-            const uint64_t first = b.tstart[x.m * 16];
-            if (g == first) {
-                lb_publish(&b.status[g], lb_pack(epoch, kFlagInc, tot));
-                sh.prefix = 0;
-            } else {
-                lb_publish(&b.status[g], lb_pack(epoch, kFlagAgg, tot));
-                uint64_t acc = 0;
-                for (uint64_t p = g - 1;; --p) {
-                    uint64_t w = lb_wait(&b.status[p], epoch);
-                    acc += w & ((1ull << 38) - 1);
-                    if (((w >> 38) & 3u) == kFlagInc) break;
-                }
-                lb_publish(&b.status[g], lb_pack(epoch, kFlagInc, acc + tot));
-                sh.prefix = acc;
-            }
-        }
-        if (bad) atomicOr(&sh.nonfinite, 1u);
-        __syncthreads();
-        const uint64_t out_pos = sh.prefix;
-        const bool fits = out_pos + tot <= b.slot[x.m].cap;
-        // compact this thread's outputs into the (consumed) input area, then write coalesced
-        if (fits) {
-            for (unsigned t = 0; t < nout; ++t) {
-                kstore<W>(sh.keys + (myoff + t) * W, kload<W>(sh.okeys + (obase + t) * W));
-                sh.vals[myoff + t] = sh.ovals[obase + t];
-            }
-        }
-        __syncthreads();
-        if (fits) {
-            uint64_t *okeys = b.arena[P.out_buf].keys + (b.slot[x.m].off + out_pos) * W;
-            double *ovals = b.arena[P.out_buf].vals + b.slot[x.m].off + out_pos;
-            for (unsigned t = threadIdx.x; t < tot; t += kBlock) {
-                kstore<W>(okeys + (uint64_t)t * W, kload<W>(sh.keys + t * W));
-                ovals[t] = sh.vals[t];
-            }
-        } else if (threadIdx.x == 0) {
-            atomicOr(&b.overflow[x.m], 1u);
-        }
-        // ---- ledger partials: fixed-shape tree with upward rounding (deterministic)
+        (void)block_exclusive_scan(nout, misc.scratch, &tot);
+        if (bad) atomicOr(&misc.nonfinite, 1u);
+        // ledger partials: fixed-shape tree with upward rounding (deterministic)
         for (int d = 1; d < 64; d <<= 1) {
             double o1 = __shfl_down(l1, d, 64), o2 = __shfl_down(sq, d, 64),
                    o3 = __shfl_down(rsq, d, 64);
@@ -880,50 +854,73 @@ __global__ __launch_bounds__(kBlock) void k_rot_merge(BatchDev b, const PassMap 
             }
         }
         if (lane_id() == 0) {
-            sh.red[0][wave_id()] = l1;
-            sh.red[1][wave_id()] = sq;
-            sh.red[2][wave_id()] = rsq;
-            sh.redn[0][wave_id()] = n_drop;
-            sh.redn[1][wave_id()] = n_rad;
+            misc.red[0][wave_id()] = l1;
+            misc.red[1][wave_id()] = sq;
+            misc.red[2][wave_id()] = rsq;
+            misc.redn[0][wave_id()] = n_drop;
+            misc.redn[1][wave_id()] = n_rad;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             TileLedger tl;
-            tl.l1 = sh.red[0][0];
-            tl.sq = sh.red[1][0];
-            tl.rsq = sh.red[2][0];
-            tl.dropped = sh.redn[0][0];
-            tl.radii = sh.redn[1][0];
+            tl.l1 = misc.red[0][0];
+            tl.sq = misc.red[1][0];
+            tl.rsq = misc.red[2][0];
+            tl.dropped = misc.redn[0][0];
+            tl.radii = misc.redn[1][0];
             for (int w = 1; w < kBlock / 64; ++w) {
-                tl.l1 = upadd(tl.l1, sh.red[0][w]);
-                tl.sq = upadd(tl.sq, sh.red[1][w]);
-                tl.rsq = upadd(tl.rsq, sh.red[2][w]);
-                tl.dropped += sh.redn[0][w];
-                tl.radii += sh.redn[1][w];
+                tl.l1 = upadd(tl.l1, misc.red[0][w]);
+                tl.sq = upadd(tl.sq, misc.red[1][w]);
+                tl.rsq = upadd(tl.rsq, misc.red[2][w]);
+                tl.dropped += misc.redn[0][w];
+                tl.radii += misc.redn[1][w];
             }
-            tl.nonfinite = sh.nonfinite;
+            tl.nonfinite = misc.nonfinite;
             tl.pad = 0;
             b.tile_led[g] = tl;
-            atomicAdd((unsigned long long *)&b.class_out[x.m * 16 + x.c],
-                      (unsigned long long)tot);
+            b.tile_hist[g] = tot;
+            atomicAdd((unsigned long long *)&b.class_out[x.m * 16 + x.c], (unsigned long long)tot);
         }
         __syncthreads();
     }
 }
 
-// One block per map: output regions, length, ledger fold in tile order, record.
-__global__ __launch_bounds__(kBlock) void k_rot_finish(BatchDev b, const PassMap *pm) {
+// One block per map: output offsets of the tiles (exclusive scan of their counts in class
+// order), output regions and length, the ledger fold in tile order, and the record.
+__global__ __launch_bounds__(kBlock) void k_rot_scan(BatchDev b, const PassMap *pm) {
     const uint32_t m = blockIdx.x;
     const PassMap &P = pm[m];
     if (!P.active) return;
+    __shared__ unsigned scratch[8];
     __shared__ double red[3][kBlock];
     __shared__ uint64_t redn[2][kBlock];
     __shared__ unsigned bad;
     const uint64_t t0 = b.tstart[m * 16], t1 = b.tstart[m * 16 + 16];
+    // each thread owns a contiguous run of tiles: sum it, scan the run totals, then write the
+    // running offsets (one pass of independent loads instead of a chunked loop)
+    const uint64_t run = (t1 - t0 + kBlock - 1) / kBlock;
+    const uint64_t r0 = t0 + threadIdx.x * run, r1 = r0 + run < t1 ? r0 + run : t1;
+    uint64_t mine = 0;
+    for (uint64_t g = r0; g < r1; ++g) mine += b.tile_hist[g];
+    __shared__ uint64_t runs[kBlock];
+    runs[threadIdx.x] = mine;
+    __syncthreads();
+    for (int d = 1; d < kBlock; d <<= 1) {  // inclusive Hillis-Steele scan of the run totals
+        uint64_t v = (int)threadIdx.x >= d ? runs[threadIdx.x - d] : 0;
+        __syncthreads();
+        runs[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint64_t at = runs[threadIdx.x] - mine;
+    const uint64_t carry = runs[kBlock - 1];
+    for (uint64_t g = r0; g < r1; ++g) {
+        b.tile_base[g] = at;
+        at += b.tile_hist[g];
+    }
+    (void)scratch;
     double l1 = 0.0, sq = 0.0, rsq = 0.0;
     uint64_t nd = 0, nr = 0;
     unsigned nf = 0;
-    // each thread folds a contiguous run of tiles, then a fixed tree across threads
     const uint64_t nt = t1 - t0;
     const uint64_t per = (nt + kBlock - 1) / kBlock;
     for (uint64_t t = t0 + threadIdx.x * per; t < t0 + (threadIdx.x + 1) * per && t < t1; ++t) {
@@ -943,23 +940,22 @@ __global__ __launch_bounds__(kBlock) void k_rot_finish(BatchDev b, const PassMap
     if (threadIdx.x == 0) bad = 0;
     __syncthreads();
     if (nf) atomicOr(&bad, 1u);
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            red[0][threadIdx.x] = upadd(red[0][threadIdx.x], red[0][threadIdx.x + s]);
-            red[1][threadIdx.x] = upadd(red[1][threadIdx.x], red[1][threadIdx.x + s]);
-            red[2][threadIdx.x] = upadd(red[2][threadIdx.x], red[2][threadIdx.x + s]);
-            redn[0][threadIdx.x] += redn[0][threadIdx.x + s];
-            redn[1][threadIdx.x] += redn[1][threadIdx.x + s];
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            red[0][threadIdx.x] = upadd(red[0][threadIdx.x], red[0][threadIdx.x + st]);
+            red[1][threadIdx.x] = upadd(red[1][threadIdx.x], red[1][threadIdx.x + st]);
+            red[2][threadIdx.x] = upadd(red[2][threadIdx.x], red[2][threadIdx.x + st]);
+            redn[0][threadIdx.x] += redn[0][threadIdx.x + st];
+            redn[1][threadIdx.x] += redn[1][threadIdx.x + st];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         const uint64_t *rin = b.regions + ((uint64_t)m * 3 + P.in_buf) * 17;
         uint64_t *rout = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
-        uint64_t acc = 0;
-        for (int c = 0; c < 16; ++c) acc += b.class_out[m * 16 + c];
-        // A map that outgrew its slot is emptied: later passes must never read past the slot.
-        // Its record carries the overflow; the host rolls the map back and grows the slot.
+        const uint64_t acc = carry;
+        // A map that outgrew its slot is emptied: later passes never read past the slot; its
+        // record carries the overflow and the host rolls the map back and grows the slot.
         const bool over = b.overflow[m] || acc > b.slot[m].cap;
         uint64_t at = 0;
         for (int c = 0; c < 16; ++c) {
@@ -983,6 +979,64 @@ __global__ __launch_bounds__(kBlock) void k_rot_finish(BatchDev b, const PassMap
         r.overflow = over ? 1u : 0u;
         r.rotation = P.rot_seq;
         b.rec[(uint64_t)m * b.max_rec + P.rec_slot] = r;
+    }
+}
+
+// Pass 2 of a rotation: per tile, merge again and write the kept outputs at the tile's offset
+// (compacted in LDS first, so global writes are contiguous).
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap *pm) {
+    __shared__ TileIn<W> in;
+    __shared__ uint64_t okeys[kMergeTile * W];
+    __shared__ double ovals[kMergeTile];
+    __shared__ unsigned scratch[8];
+    const uint64_t ntiles = b.tstart[b.n_maps * 16];
+    for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
+        MergeCtx x;
+        locate(b, pm, g, x);
+        const PassMap &P = pm[x.m];
+        const uint32_t n = b.slot[x.m].n_spins;
+        const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
+        TileGeom t;
+        const uint64_t total = x.nA + x.nD;
+        t.d0 = x.k * kMergeTile;
+        t.d1 = t.d0 + kMergeTile < total ? t.d0 + kMergeTile : total;
+        t.a0 = b.splits[g];
+        t.a1 = (g + 1 < b.tstart[x.m * 16 + x.c + 1]) ? b.splits[g + 1] : x.nA;
+        const uint64_t out_pos = b.tile_base[g];
+        const uint64_t count = b.tile_hist[g];
+        // overflowed maps and inconsistent tiles write nothing (both are reported at sync)
+        if (!tile_ok(x, t) || b.overflow[x.m] || out_pos + count > b.slot[x.m].cap) continue;
+        load_tile<W>(b, P, x, q, t, in);
+        __syncthreads();
+        const Coef k = coef_of(P, x);
+        const uint32_t obase = threadIdx.x * kMergeItems;
+        unsigned nout = 0;
+        bool bad = false;
+        walk_tile<W>(in, t, [&](const Key<W> &key, bool hasA, double va, bool hasD, double vd) {
+            double v, r;
+            merged(k, hasA, va, hasD, vd, v, r, bad);
+            if (v == 0.0) return;
+            if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) return;
+            kstore<W>(okeys + (obase + nout) * W, key);
+            ovals[obase + nout] = v;
+            nout++;
+        });
+        unsigned tot;
+        unsigned myoff = block_exclusive_scan(nout, scratch, &tot);
+        // compact this thread's outputs into the (consumed) input area, then write contiguously
+        for (unsigned i = 0; i < nout; ++i) {
+            kstore<W>(in.keys + (myoff + i) * W, kload<W>(okeys + (obase + i) * W));
+            in.vals[myoff + i] = ovals[obase + i];
+        }
+        __syncthreads();
+        uint64_t *gk = b.arena[P.out_buf].keys + (b.slot[x.m].off + out_pos) * W;
+        double *gv = b.arena[P.out_buf].vals + b.slot[x.m].off + out_pos;
+        for (unsigned i = threadIdx.x; i < tot; i += kBlock) {
+            kstore<W>(gk + (uint64_t)i * W, kload<W>(in.keys + i * W));
+            gv[i] = in.vals[i];
+        }
+        __syncthreads();
     }
 }
 
@@ -1311,6 +1365,7 @@ struct HostMap {
     std::vector<uint8_t> sig_c, sig;  // sigma[p] = site at digit position p
     uint32_t nrec = 0, nrd = 0;
     uint64_t committed_len = 0;
+    uint64_t est_len = 0;  // last known working length (sizes launch grids)
 };
 struct LoggedLayout {
     uint32_t map, rotation;
@@ -1331,14 +1386,24 @@ struct tpgp_hip_batch {
     PassMap *h_stage = nullptr;
     PassMap *d_stage = nullptr;
     size_t stage_cap = 0, stage_used = 0;
-    uint32_t epoch = 0;
     unsigned grid = 1024;
     uint32_t max_rec = 0, max_rd = 0;
     std::vector<LoggedLayout> logged;
     std::vector<void *> allocations;
     bool poisoned = false;
     bool validate = false;  // TPGP_HIP_VALIDATE=1: check order and regions after every pass
+    uint64_t est_tiles = 1;  // merge tiles of the last planned rotation (grid sizing only)
     uint32_t pass_count = 0;
+    // kernel timing with HIP events on the batch stream (kinds: TPGP_HIP_PROFILE_*)
+    bool profiling = false;
+    std::vector<hipEvent_t> event_pool;
+    struct Pending {
+        unsigned kind;
+        hipEvent_t start, stop;
+    };
+    std::vector<Pending> pending;
+    double prof_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t prof_count[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -1355,7 +1420,11 @@ template <typename T> hipError_t dalloc(tpgp_hip_batch *b, T **p, size_t count) 
     return e;
 }
 
+void collect_profile(tpgp_hip_batch *b);
 void free_all(tpgp_hip_batch *b) {
+    collect_profile(b);
+    for (hipEvent_t e : b->event_pool) (void)hipEventDestroy(e);
+    b->event_pool.clear();
     for (void *p : b->allocations) (void)hipFree(p);
     b->allocations.clear();
     if (b->h_stage) (void)hipHostFree(b->h_stage);
@@ -1393,7 +1462,6 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     ALLOC(d.tile_base, b->max_tiles * 16);
     ALLOC(d.map_hist, (size_t)M * 16);
     ALLOC(d.splits, b->max_tiles + 1);
-    ALLOC(d.status, b->max_tiles);
     ALLOC(d.class_out, (size_t)M * 16);
     ALLOC(d.tile_led, b->max_tiles);
     ALLOC(d.counter, 4);
@@ -1419,7 +1487,6 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     if ((e = hipMemcpyAsync(b->d_slot, slots.data(), M * sizeof(MapSlot), hipMemcpyHostToDevice,
                             st)) != hipSuccess)
         return hip_fail(e, "HIP slot upload", error, cap);
-    hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, st, d.status, b->max_tiles, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, st, d.len, (uint64_t)M * 3, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, st, d.regions, (uint64_t)M * 3 * 17,
                        0ull);
@@ -1429,7 +1496,6 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, d.debug, 16ull, 0ull);
     if ((e = hipStreamSynchronize(st)) != hipSuccess)
         return hip_fail(e, "HIP batch initialisation", error, cap);
-    b->epoch = 0;
     b->stage_used = 0;
     return TPGP_HIP_OK;
 }
@@ -1456,20 +1522,78 @@ int next_buffer(const HostMap &hm) {
     return (hm.kc + 1) % 3;
 }
 
-uint32_t next_epoch(tpgp_hip_batch *b) {
-    if (++b->epoch >= (1u << 24)) {
-        hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, b->ctx->stream, b->dev.status,
-                           b->max_tiles, 0ull);
-        b->epoch = 1;
+// Grid sizes from the last known map lengths: small maps launch few workgroups; a stale
+// estimate only changes the grid-stride loop count, never the result.
+unsigned grid_for(const tpgp_hip_batch *b, uint64_t tiles) {
+    uint64_t cap = (uint64_t)b->ctx->cus * 8;
+    if (tiles < 1) tiles = 1;
+    return (unsigned)(tiles < cap ? tiles : cap);
+}
+unsigned partition_grid(const tpgp_hip_batch *b, const std::vector<int> &want) {
+    uint64_t t = 0;
+    for (uint32_t m = 0; m < b->M; ++m)
+        if (want[m]) t += b->maps[m].est_len / kPartTile + 1;
+    return grid_for(b, t);
+}
+unsigned rotation_grid(tpgp_hip_batch *b, const tpgp_hip_rotation *req) {
+    uint64_t t = 0;
+    for (uint32_t m = 0; m < b->M; ++m)
+        if (req[m].active) t += (3 * b->maps[m].est_len / 2) / kMergeTile + 16;
+    b->est_tiles = t;
+    return grid_for(b, t);
+}
+
+hipEvent_t pooled_event(tpgp_hip_batch *b) {
+    if (!b->event_pool.empty()) {
+        hipEvent_t e = b->event_pool.back();
+        b->event_pool.pop_back();
+        return e;
     }
-    return b->epoch;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+// Bracket the next launches of one kind with events when profiling is on.
+struct ProfileScope {
+    tpgp_hip_batch *b;
+    unsigned kind;
+    hipEvent_t start = nullptr;
+    ProfileScope(tpgp_hip_batch *bb, unsigned k) : b(bb), kind(k) {
+        if (!b->profiling) return;
+        start = pooled_event(b);
+        if (start) (void)hipEventRecord(start, b->ctx->stream);
+    }
+    ~ProfileScope() {
+        if (!start) return;
+        hipEvent_t stop = pooled_event(b);
+        if (!stop) {
+            b->event_pool.push_back(start);
+            return;
+        }
+        (void)hipEventRecord(stop, b->ctx->stream);
+        b->pending.push_back({kind, start, stop});
+    }
+};
+// After the stream has drained: accumulate elapsed times and recycle the events.
+void collect_profile(tpgp_hip_batch *b) {
+    for (auto &p : b->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+            b->prof_ms[p.kind] += ms;
+            b->prof_count[p.kind] += 1;
+        }
+        b->event_pool.push_back(p.start);
+        b->event_pool.push_back(p.stop);
+    }
+    b->pending.clear();
 }
 
 // Enqueue a partition pass for the maps with want[m] (positions pa/pb given per map).
-hipError_t enqueue_partition(tpgp_hip_batch *b, const std::vector<int> &want,
-                             const std::vector<uint32_t> &pa, const std::vector<uint32_t> &pb) {
-    PassMap *d;
-    PassMap *h = stage(b, b->M, &d);
+// Partition descriptors for the maps with want[m] (positions pa/pb moved to the top), and the
+// host-side state they imply: the working buffer and the digit layout of each map.
+unsigned plan_partition(tpgp_hip_batch *b, const std::vector<int> &want,
+                        const std::vector<uint32_t> &pa, const std::vector<uint32_t> &pb,
+                        PassMap *h) {
     for (uint32_t m = 0; m < b->M; ++m) {
         if (!want[m]) continue;
         HostMap &hm = b->maps[m];
@@ -1478,41 +1602,6 @@ hipError_t enqueue_partition(tpgp_hip_batch *b, const std::vector<int> &want,
         h[m].out_buf = (uint32_t)next_buffer(hm);
         h[m].pa = pa[m];
         h[m].pb = pb[m];
-    }
-    hipError_t e = push_stage(b, h, d, b->M);
-    if (e != hipSuccess) return e;
-    hipStream_t st = b->ctx->stream;
-    hipLaunchKernelGGL(k_part_plan, dim3(1), dim3(kBlock), 0, st, b->dev, (const PassMap *)d);
-    if (b->W == 1) {
-        hipLaunchKernelGGL(k_part_count<1>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
-                           (const PassMap *)d);
-    } else {
-        hipLaunchKernelGGL(k_part_count<2>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
-                           (const PassMap *)d);
-    }
-    hipLaunchKernelGGL(k_part_scan, dim3(b->M * 16), dim3(kBlock), 0, st, b->dev,
-                       (const PassMap *)d);
-    if (b->W == 1) {
-        hipLaunchKernelGGL(k_part_scatter<1>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
-                           (const PassMap *)d);
-    } else {
-        hipLaunchKernelGGL(k_part_scatter<2>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
-                           (const PassMap *)d);
-    }
-    if (b->validate) {
-        uint32_t tag = (++b->pass_count << 1) | 0u;
-        if (b->W == 1)
-            hipLaunchKernelGGL(k_validate<1>, dim3(256), dim3(kBlock), 0, st, b->dev,
-                               (const PassMap *)d, tag);
-        else
-            hipLaunchKernelGGL(k_validate<2>, dim3(256), dim3(kBlock), 0, st, b->dev,
-                               (const PassMap *)d, tag);
-    }
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    for (uint32_t m = 0; m < b->M; ++m) {
-        if (!want[m]) continue;
-        HostMap &hm = b->maps[m];
         hm.kcur = (int)h[m].out_buf;
         // sigma: move sites at pa, pb to the top, keep the others' order
         size_t n = hm.n_spins;
@@ -1525,7 +1614,46 @@ hipError_t enqueue_partition(tpgp_hip_batch *b, const std::vector<int> &want,
         rest.push_back(sa);
         hm.sig = rest;
     }
-    return hipSuccess;
+    return partition_grid(b, want);
+}
+
+void launch_partition(tpgp_hip_batch *b, const PassMap *d, unsigned pgrid) {
+    hipStream_t st = b->ctx->stream;
+    ProfileScope whole(b, TPGP_HIP_PROFILE_PARTITION);
+    hipLaunchKernelGGL(k_part_plan, dim3(1), dim3(kBlock), 0, st, b->dev, d);
+    {
+        ProfileScope ph(b, TPGP_HIP_PROFILE_HISTOGRAM);
+        if (b->W == 1)
+            hipLaunchKernelGGL(k_part_count<1>, dim3(pgrid), dim3(kBlock), 0, st, b->dev, d);
+        else
+            hipLaunchKernelGGL(k_part_count<2>, dim3(pgrid), dim3(kBlock), 0, st, b->dev, d);
+    }
+    hipLaunchKernelGGL(k_part_scan, dim3(b->M * 16), dim3(kBlock), 0, st, b->dev, d);
+    {
+        ProfileScope ps(b, TPGP_HIP_PROFILE_SCATTER);
+        if (b->W == 1)
+            hipLaunchKernelGGL(k_part_scatter<1>, dim3(pgrid), dim3(kBlock), 0, st, b->dev, d);
+        else
+            hipLaunchKernelGGL(k_part_scatter<2>, dim3(pgrid), dim3(kBlock), 0, st, b->dev, d);
+    }
+    if (b->validate) {
+        uint32_t tag = (++b->pass_count << 1) | 0u;
+        if (b->W == 1)
+            hipLaunchKernelGGL(k_validate<1>, dim3(256), dim3(kBlock), 0, st, b->dev, d, tag);
+        else
+            hipLaunchKernelGGL(k_validate<2>, dim3(256), dim3(kBlock), 0, st, b->dev, d, tag);
+    }
+}
+
+hipError_t enqueue_partition(tpgp_hip_batch *b, const std::vector<int> &want,
+                             const std::vector<uint32_t> &pa, const std::vector<uint32_t> &pb) {
+    PassMap *d;
+    PassMap *h = stage(b, b->M, &d);
+    unsigned pgrid = plan_partition(b, want, pa, pb, h);
+    hipError_t e = push_stage(b, h, d, b->M);
+    if (e != hipSuccess) return e;
+    launch_partition(b, d, pgrid);
+    return hipGetLastError();
 }
 
 size_t position_of(const HostMap &hm, size_t site) {
@@ -1668,60 +1796,16 @@ extern "C" int tpgp_hip_batch_upload(tpgp_hip_batch *b, size_t map, const uint64
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "HIP upload", error, cap);
     hm.committed_len = len;
+    hm.est_len = len;
     hm.nrec = hm.nrd = 0;
     return TPGP_HIP_OK;
 }
 
-extern "C" int tpgp_hip_batch_rotate(tpgp_hip_batch *b, const tpgp_hip_rotation *req,
-                                     uint32_t rotation, char *error, size_t cap) {
-    msg(error, cap, "");
-    if (!b || !req) {
-        msg(error, cap, "Invalid rotation arguments.");
-        return TPGP_HIP_EINVAL;
-    }
-    const uint32_t M = b->M;
-    for (uint32_t m = 0; m < M; ++m) {
-        const tpgp_hip_rotation &r = req[m];
-        if (!r.active) continue;
-        const HostMap &hm = b->maps[m];
-        if (r.i >= hm.n_spins || r.j >= hm.n_spins || r.i == r.j || r.qi > 3 || r.qj > 3 ||
-            !std::isfinite(r.c) || !std::isfinite(r.s) || !std::isfinite(r.rc) ||
-            !std::isfinite(r.rs) || !std::isfinite(r.threshold) || r.threshold < 0) {
-            msg(error, cap, "Invalid rotation request for map %u.", m);
-            return TPGP_HIP_EINVAL;
-        }
-        if (hm.nrec >= b->max_rec) {
-            msg(error, cap, "Too many rotations since the last sync (map %u).", m);
-            return TPGP_HIP_EINVAL;
-        }
-    }
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
-    DeviceGuard guard(b->ctx->device);
-    hipStream_t st = b->ctx->stream;
-    // 1) bring each active map's pair (i, j) to the top two digits
-    std::vector<int> want(M, 0);
-    std::vector<uint32_t> pa(M, 0), pb(M, 0);
-    bool any = false;
-    for (uint32_t m = 0; m < M; ++m) {
-        if (!req[m].active) continue;
-        HostMap &hm = b->maps[m];
-        size_t n = hm.n_spins;
-        uint8_t top = hm.sig[n - 1], sec = hm.sig[n - 2];
-        bool ok = (top == req[m].i && sec == req[m].j) || (top == req[m].j && sec == req[m].i);
-        if (!ok) {
-            want[m] = 1;
-            any = true;
-            pa[m] = (uint32_t)position_of(hm, req[m].i);
-            pb[m] = (uint32_t)position_of(hm, req[m].j);
-        }
-    }
-    hipError_t e;
-    if (any && (e = enqueue_partition(b, want, pa, pb)) != hipSuccess)
-        return hip_fail(e, "HIP partition launch", error, cap);
-    // 2) rotation pass
-    PassMap *d;
-    PassMap *h = stage(b, M, &d);
-    for (uint32_t m = 0; m < M; ++m) {
+namespace {
+// Rotation descriptors for the active requests, and the host-side state they imply.
+unsigned plan_rotation(tpgp_hip_batch *b, const tpgp_hip_rotation *req, uint32_t rotation,
+                       PassMap *h) {
+    for (uint32_t m = 0; m < b->M; ++m) {
         const tpgp_hip_rotation &r = req[m];
         if (!r.active) continue;
         HostMap &hm = b->maps[m];
@@ -1740,42 +1824,144 @@ extern "C" int tpgp_hip_batch_rotate(tpgp_hip_batch *b, const tpgp_hip_rotation 
         h[m].rc = r.validated ? r.rc : 0.0;
         h[m].rs = r.validated ? r.rs : 0.0;
         h[m].threshold = r.threshold;
+        hm.kcur = (int)h[m].out_buf;
+        hm.nrec++;
+        if (r.log) b->logged.push_back(LoggedLayout{m, rotation, hm.sig});
     }
-    if ((e = push_stage(b, h, d, M)) != hipSuccess)
-        return hip_fail(e, "HIP rotation staging", error, cap);
-    uint32_t epoch = next_epoch(b);
-    hipLaunchKernelGGL(k_rot_plan, dim3(1), dim3(kBlock), 0, st, b->dev, (const PassMap *)d);
-    if (b->W == 1) {
-        hipLaunchKernelGGL(k_rot_splits<1>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
-                           (const PassMap *)d);
-        hipLaunchKernelGGL(k_rot_merge<1>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
-                           (const PassMap *)d, epoch);
-    } else {
-        hipLaunchKernelGGL(k_rot_splits<2>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
-                           (const PassMap *)d);
-        hipLaunchKernelGGL(k_rot_merge<2>, dim3(b->grid), dim3(kBlock), 0, st, b->dev,
-                           (const PassMap *)d, epoch);
+    return rotation_grid(b, req);
+}
+
+void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid) {
+    hipStream_t st = b->ctx->stream;
+    {
+        ProfileScope whole(b, TPGP_HIP_PROFILE_ROTATION);
+        {
+            ProfileScope pp(b, TPGP_HIP_PROFILE_PLAN);
+            hipLaunchKernelGGL(k_rot_plan, dim3(1), dim3(kBlock), 0, st, b->dev, d);
+        }
+        {
+            ProfileScope pc(b, TPGP_HIP_PROFILE_SPLITS);
+            // one wave per tile, all in flight: up to 32 waves per CU
+            const uint64_t want = (b->est_tiles + 3) / 4, cap = (uint64_t)b->ctx->cus * 8;
+            const unsigned sgrid = (unsigned)(want < 1 ? 1 : (want < cap ? want : cap));
+            if (b->W == 1)
+                hipLaunchKernelGGL(k_rot_splits<1>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d);
+            else
+                hipLaunchKernelGGL(k_rot_splits<2>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d);
+        }
+        {
+            ProfileScope pc(b, TPGP_HIP_PROFILE_COUNT);
+            if (b->W == 1)
+                hipLaunchKernelGGL(k_rot_count<1>, dim3(rgrid), dim3(kBlock), 0, st, b->dev, d);
+            else
+                hipLaunchKernelGGL(k_rot_count<2>, dim3(rgrid), dim3(kBlock), 0, st, b->dev, d);
+        }
+        {
+            ProfileScope psn(b, TPGP_HIP_PROFILE_SCAN);
+            hipLaunchKernelGGL(k_rot_scan, dim3(b->M), dim3(kBlock), 0, st, b->dev, d);
+        }
+        {
+            ProfileScope pm(b, TPGP_HIP_PROFILE_MERGE);
+            if (b->W == 1)
+                hipLaunchKernelGGL(k_rot_write<1>, dim3(rgrid), dim3(kBlock), 0, st, b->dev, d);
+            else
+                hipLaunchKernelGGL(k_rot_write<2>, dim3(rgrid), dim3(kBlock), 0, st, b->dev, d);
+        }
     }
-    hipLaunchKernelGGL(k_rot_finish, dim3(M), dim3(kBlock), 0, st, b->dev, (const PassMap *)d);
     if (b->validate) {
         uint32_t tag = (++b->pass_count << 1) | 1u;
         if (b->W == 1)
-            hipLaunchKernelGGL(k_validate<1>, dim3(256), dim3(kBlock), 0, st, b->dev,
-                               (const PassMap *)d, tag);
+            hipLaunchKernelGGL(k_validate<1>, dim3(256), dim3(kBlock), 0, st, b->dev, d, tag);
         else
-            hipLaunchKernelGGL(k_validate<2>, dim3(256), dim3(kBlock), 0, st, b->dev,
-                               (const PassMap *)d, tag);
+            hipLaunchKernelGGL(k_validate<2>, dim3(256), dim3(kBlock), 0, st, b->dev, d, tag);
+    }
+}
+
+}  // namespace
+
+extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rotation *req,
+                                          const uint32_t *rotations, size_t n_rot, char *error,
+                                          size_t cap) {
+    msg(error, cap, "");
+    if (!b || (!req && n_rot) || (!rotations && n_rot)) {
+        msg(error, cap, "Invalid rotation arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    const uint32_t M = b->M;
+    if (2 * n_rot * M > b->stage_cap) {
+        msg(error, cap, "Too many rotations in one call (%zu x %u maps).", n_rot, M);
+        return TPGP_HIP_EINVAL;
+    }
+    std::vector<uint32_t> count(M, 0);
+    for (size_t r = 0; r < n_rot; ++r)
+        for (uint32_t m = 0; m < M; ++m) {
+            const tpgp_hip_rotation &q = req[r * M + m];
+            if (!q.active) continue;
+            const HostMap &hm = b->maps[m];
+            if (q.i >= hm.n_spins || q.j >= hm.n_spins || q.i == q.j || q.qi > 3 || q.qj > 3 ||
+                !std::isfinite(q.c) || !std::isfinite(q.s) || !std::isfinite(q.rc) ||
+                !std::isfinite(q.rs) || !std::isfinite(q.threshold) || q.threshold < 0) {
+                msg(error, cap, "Invalid rotation request for map %u.", m);
+                return TPGP_HIP_EINVAL;
+            }
+            if (hm.nrec + ++count[m] > b->max_rec) {
+                msg(error, cap, "Too many rotations since the last sync (map %u).", m);
+                return TPGP_HIP_EINVAL;
+            }
+        }
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    PassMap *d0;
+    PassMap *h0 = stage(b, 2 * n_rot * M, &d0);
+    struct Launch {
+        bool rotation;
+        size_t at;
+        unsigned grid;
+    };
+    std::vector<Launch> launches;
+    launches.reserve(2 * n_rot);
+    size_t used = 0;
+    std::vector<int> want(M);
+    std::vector<uint32_t> pa(M), pb(M);
+    for (size_t r = 0; r < n_rot; ++r) {
+        const tpgp_hip_rotation *q = req + r * M;
+        // bring each active map's pair (i, j) to the top two digits
+        bool any = false;
+        for (uint32_t m = 0; m < M; ++m) {
+            want[m] = 0;
+            if (!q[m].active) continue;
+            HostMap &hm = b->maps[m];
+            size_t n = hm.n_spins;
+            uint8_t top = hm.sig[n - 1], sec = hm.sig[n - 2];
+            if ((top == q[m].i && sec == q[m].j) || (top == q[m].j && sec == q[m].i)) continue;
+            want[m] = 1;
+            any = true;
+            pa[m] = (uint32_t)position_of(hm, q[m].i);
+            pb[m] = (uint32_t)position_of(hm, q[m].j);
+        }
+        if (any) {
+            launches.push_back({false, used, plan_partition(b, want, pa, pb, h0 + used)});
+            used += M;
+        }
+        launches.push_back({true, used, plan_rotation(b, q, rotations[r], h0 + used)});
+        used += M;
+    }
+    hipError_t e = push_stage(b, h0, d0, used);
+    if (e != hipSuccess) return hip_fail(e, "HIP descriptor staging", error, cap);
+    for (const Launch &L : launches) {
+        if (L.rotation)
+            launch_rotation(b, d0 + L.at, L.grid);
+        else
+            launch_partition(b, d0 + L.at, L.grid);
     }
     if ((e = hipGetLastError()) != hipSuccess)
         return hip_fail(e, "HIP rotation launch", error, cap);
-    for (uint32_t m = 0; m < M; ++m) {
-        if (!req[m].active) continue;
-        HostMap &hm = b->maps[m];
-        hm.kcur = (int)h[m].out_buf;
-        hm.nrec++;
-        if (req[m].log) b->logged.push_back(LoggedLayout{m, rotation, hm.sig});
-    }
     return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_rotate(tpgp_hip_batch *b, const tpgp_hip_rotation *req,
+                                     uint32_t rotation, char *error, size_t cap) {
+    return tpgp_hip_batch_rotate_many(b, req, &rotation, 1, error, cap);
 }
 
 extern "C" int tpgp_hip_batch_readout(tpgp_hip_batch *b, unsigned pauli, const int *active,
@@ -1827,6 +2013,7 @@ extern "C" int tpgp_hip_batch_sync(tpgp_hip_batch *b, tpgp_hip_record *records,
     hipStream_t st = b->ctx->stream;
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "HIP resident execution", error, cap);
+    collect_profile(b);
     const uint32_t M = b->M;
     uint32_t nrec = 0, nrd = 0;
     for (uint32_t m = 0; m < M; ++m) {
@@ -1861,12 +2048,16 @@ extern "C" int tpgp_hip_batch_sync(tpgp_hip_batch *b, tpgp_hip_record *records,
                 std::memcpy(readouts + ((size_t)m * max_readouts + q) * 64,
                             tmp.data() + ((size_t)m * b->max_rd + q) * 64, 64 * sizeof(double));
     }
-    if (lengths) {
+    {
         std::vector<uint64_t> len((size_t)M * 3);
         e = hipMemcpy(len.data(), b->dev.len, len.size() * sizeof(uint64_t),
                       hipMemcpyDeviceToHost);
         if (e != hipSuccess) return hip_fail(e, "HIP length transfer", error, cap);
-        for (uint32_t m = 0; m < M; ++m) lengths[m] = len[(size_t)m * 3 + b->maps[m].kcur];
+        for (uint32_t m = 0; m < M; ++m) {
+            const uint64_t l = len[(size_t)m * 3 + b->maps[m].kcur];
+            b->maps[m].est_len = std::max<uint64_t>(l, b->maps[m].est_len / 2);
+            if (lengths) lengths[m] = l;
+        }
     }
     uint32_t internal = 0;
     e = hipMemcpy(&internal, b->dev.counter + 1, sizeof(uint32_t), hipMemcpyDeviceToHost);
@@ -2109,6 +2300,7 @@ extern "C" int tpgp_hip_batch_reserve(tpgp_hip_batch *b, size_t map, size_t capa
         hm.sig = hm.sig_c;
         hm.nrec = hm.nrd = 0;
         uint64_t n = hv[m].size();
+        hm.est_len = n;
         nlen[(size_t)m * 3] = n;
         if (n) {
             if ((e = hipMemcpy(b->dev.arena[0].keys + hm.off * W, hk[m].data(), n * W * 8,
@@ -2131,4 +2323,26 @@ extern "C" int tpgp_hip_batch_reserve(tpgp_hip_batch *b, size_t map, size_t capa
     }
     e = hipStreamSynchronize(st);
     return e == hipSuccess ? TPGP_HIP_OK : hip_fail(e, "HIP reserve", error, cap);
+}
+
+extern "C" int tpgp_hip_batch_profile(tpgp_hip_batch *b, int enable) {
+    if (!b) return TPGP_HIP_EINVAL;
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    collect_profile(b);
+    b->profiling = enable != 0;
+    for (int k = 0; k < 9; ++k) {
+        b->prof_ms[k] = 0;
+        b->prof_count[k] = 0;
+    }
+    return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_profile_read(const tpgp_hip_batch *b, unsigned kind,
+                                           double *milliseconds, uint64_t *launches) {
+    if (!b || kind > 8) return TPGP_HIP_EINVAL;
+    if (milliseconds) *milliseconds = b->prof_ms[kind];
+    if (launches) *launches = b->prof_count[kind];
+    return TPGP_HIP_OK;
 }

@@ -96,6 +96,26 @@ class Map:
                 raise ValueError(f"tpgp_map_add: status {st}")
         return mp
 
+    @classmethod
+    def from_sorted(cls, n_spins, keys, values):
+        """Install arrays that are already canonical (sorted unique, nonzero) without re-adding."""
+        mp = cls(n_spins)
+        keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1)
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        libc = C.CDLL(None)
+        libc.malloc.restype = C.c_void_p
+        libc.malloc.argtypes = [C.c_size_t]
+        kp = libc.malloc(max(keys.nbytes, 8))
+        vp = libc.malloc(max(values.nbytes, 8))
+        if not kp or not vp:
+            raise MemoryError("host map allocation failed")
+        C.memmove(kp, keys.ctypes.data, keys.nbytes)
+        C.memmove(vp, values.ctypes.data, values.nbytes)
+        mp.m.keys = C.cast(kp, C.POINTER(C.c_uint64))
+        mp.m.values = C.cast(vp, C.POINTER(C.c_double))
+        mp.m.len = mp.m.capacity = len(values)
+        return mp
+
     def arrays(self):
         n, w = self.m.len, self.m.n_words
         if n == 0:
@@ -153,3 +173,58 @@ def evolve(mp: Map, terms, dt, steps, opts, led, exact_ledger=False, pauli=1, no
                                     C.byref(eo), C.byref(led), N.as_dp(fid), N.as_dp(rad),
                                     act.ctypes.data_as(C.POINTER(C.c_size_t)), C.byref(acc))
     return st, fid, rad, act, acc.value
+
+
+class Evolution:
+    """Resident evolution session (tpgp_evolution_open/run/close): the map stays in HBM."""
+
+    def __init__(self, mp: Map, terms, dt, opts, exact_ledger=False, pauli=1, normalizer=None,
+                 initial_capacity=0):
+        self.eng = N.engine()
+        self.n_spins = mp.n_spins
+        self._terms = N.terms_array(terms)
+        self._opts = opts
+        self.eo = N.EvolveOptions(1 if exact_ledger else 0, pauli,
+                                  normalizer if normalizer else mp.n_spins, initial_capacity)
+        self.ptr = C.c_void_p()
+        st = self.eng.tpgp_evolution_open(C.byref(self.ptr), C.byref(mp.m), self._terms,
+                                          len(terms), dt, C.byref(opts), C.byref(self.eo))
+        if st:
+            raise RuntimeError(f"tpgp_evolution_open: status {st}: "
+                               f"{self.eng.tpgp_hip_last_error().decode()}")
+
+    def run(self, steps, led):
+        fid = np.full(steps + 1, np.nan)
+        rad = np.full(steps + 1, np.nan)
+        act = np.zeros(steps + 1, np.uint64)
+        acc = C.c_size_t()
+        st = self.eng.tpgp_evolution_run(self.ptr, steps, C.byref(led), N.as_dp(fid),
+                                         N.as_dp(rad), act.ctypes.data_as(C.POINTER(C.c_size_t)),
+                                         C.byref(acc))
+        return st, fid, rad, act, acc.value
+
+    def counters(self):
+        c = N.EvolutionCounters()
+        self.eng.tpgp_evolution_counters_get(self.ptr, C.byref(c))
+        return c
+
+    def profile(self, enable=True):
+        return self.eng.tpgp_evolution_profile(self.ptr, 1 if enable else 0)
+
+    def profile_read(self, kind):
+        ms, n = C.c_double(), C.c_uint64()
+        self.eng.tpgp_evolution_profile_read(self.ptr, kind, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def close(self, mp: Map | None = None):
+        if self.ptr:
+            st = self.eng.tpgp_evolution_close(self.ptr, C.byref(mp.m) if mp is not None else None)
+            self.ptr = None
+            return st
+        return 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -66,6 +66,17 @@ class EvolveOptions(C.Structure):
                 ("initial_capacity", C.c_size_t)]
 
 
+class EvolutionCounters(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("rotations", C.c_uint64), ("updates", C.c_uint64),
+                ("in_out", C.c_uint64)]
+
+
+(PROFILE_MERGE, PROFILE_SCATTER, PROFILE_ROTATION, PROFILE_PARTITION, PROFILE_COUNT,
+ PROFILE_SCAN, PROFILE_PLAN, PROFILE_HISTOGRAM, PROFILE_SPLITS) = range(9)
+PROFILE_NAMES = ["k_rot_write", "k_part_scatter", "rotation_pass", "partition_pass",
+                 "k_rot_count", "k_rot_scan", "k_rot_plan", "k_part_count", "k_rot_splits"]
+
+
 class HipRotation(C.Structure):
     _fields_ = [("active", C.c_int), ("qi", C.c_uint), ("qj", C.c_uint), ("validated", C.c_int),
                 ("log", C.c_int), ("i", C.c_size_t), ("j", C.c_size_t), ("c", C.c_double),
@@ -89,16 +100,20 @@ class HipLogEntry(C.Structure):
 HIP_SYMBOLS = ["tpgp_hip_compiled", "tpgp_hip_create", "tpgp_hip_destroy",
                "tpgp_hip_device_name", "tpgp_hip_branches", "tpgp_hip_batch_create",
                "tpgp_hip_batch_destroy", "tpgp_hip_batch_upload", "tpgp_hip_batch_download",
-               "tpgp_hip_batch_rotate", "tpgp_hip_batch_readout", "tpgp_hip_batch_sync",
+               "tpgp_hip_batch_rotate", "tpgp_hip_batch_rotate_many", "tpgp_hip_batch_readout", "tpgp_hip_batch_sync",
                "tpgp_hip_batch_commit", "tpgp_hip_batch_rollback", "tpgp_hip_batch_reserve",
-               "tpgp_hip_batch_info", "tpgp_hip_batch_stream"]
+               "tpgp_hip_batch_info", "tpgp_hip_batch_stream", "tpgp_hip_batch_profile",
+               "tpgp_hip_batch_profile_read"]
 ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_default_options", "tpgp_ledger_init", "tpgp_map_init", "tpgp_map_free",
                   "tpgp_map_clone", "tpgp_key_get", "tpgp_key_set", "tpgp_map_add",
                   "tpgp_map_add_single", "tpgp_map_get", "tpgp_map_l2", "tpgp_map_single_sum",
                   "tpgp_readout", "tpgp_pauli_phase", "tpgp_rotate", "tpgp_strang_step",
                   "tpgp_strang_bound", "tpgp_strang_bound_taylor", "tpgp_bound_add_up",
-                  "tpgp_bound_fid_up", "tpgp_hip_evolve", "tpgp_hip_last_error"]
+                  "tpgp_bound_fid_up", "tpgp_hip_evolve", "tpgp_hip_last_error",
+                  "tpgp_evolution_open", "tpgp_evolution_run", "tpgp_evolution_close",
+                  "tpgp_evolution_counters_get", "tpgp_evolution_profile",
+                  "tpgp_evolution_profile_read", "tpgp_evolution_key_words"]
 
 
 def hip():
@@ -128,6 +143,8 @@ def hip():
     f("tpgp_hip_batch_upload", C.c_int, C.c_void_p, sz, p_u64, p_d, sz, p_c, sz)
     f("tpgp_hip_batch_download", C.c_int, C.c_void_p, sz, p_u64, p_d, sz, C.POINTER(sz), p_c, sz)
     f("tpgp_hip_batch_rotate", C.c_int, C.c_void_p, C.POINTER(HipRotation), C.c_uint32, p_c, sz)
+    f("tpgp_hip_batch_rotate_many", C.c_int, C.c_void_p, C.POINTER(HipRotation),
+      C.POINTER(C.c_uint32), sz, p_c, sz)
     f("tpgp_hip_batch_readout", C.c_int, C.c_void_p, C.c_uint, C.POINTER(C.c_int), p_c, sz)
     f("tpgp_hip_batch_sync", C.c_int, C.c_void_p, C.POINTER(HipRecord), sz, C.POINTER(sz), p_d,
       sz, C.POINTER(sz), p_u64, C.POINTER(HipLogEntry), sz, C.POINTER(sz), p_c, sz)
@@ -136,6 +153,9 @@ def hip():
     f("tpgp_hip_batch_reserve", C.c_int, C.c_void_p, sz, sz, p_c, sz)
     f("tpgp_hip_batch_info", C.c_int, C.c_void_p, sz, C.POINTER(sz), C.POINTER(sz))
     f("tpgp_hip_batch_stream", C.c_void_p, C.c_void_p)
+    f("tpgp_hip_batch_profile", C.c_int, C.c_void_p, C.c_int)
+    f("tpgp_hip_batch_profile_read", C.c_int, C.c_void_p, C.c_uint, C.POINTER(C.c_double),
+      C.POINTER(C.c_uint64))
     _hip = L
     return L
 
@@ -184,6 +204,15 @@ def engine():
       C.POINTER(EvolveOptions), lp, C.POINTER(C.c_double), C.POINTER(C.c_double),
       C.POINTER(sz), C.POINTER(sz))
     f("tpgp_hip_last_error", C.c_char_p)
+    pd, psz = C.POINTER(C.c_double), C.POINTER(sz)
+    f("tpgp_evolution_open", C.c_int, C.POINTER(C.c_void_p), mp, C.POINTER(TpgpTerm), sz,
+      C.c_double, op, C.POINTER(EvolveOptions))
+    f("tpgp_evolution_run", C.c_int, C.c_void_p, sz, lp, pd, pd, psz, psz)
+    f("tpgp_evolution_close", C.c_int, C.c_void_p, mp)
+    f("tpgp_evolution_counters_get", None, C.c_void_p, C.POINTER(EvolutionCounters))
+    f("tpgp_evolution_profile", C.c_int, C.c_void_p, C.c_int)
+    f("tpgp_evolution_profile_read", C.c_int, C.c_void_p, C.c_uint, pd, C.POINTER(C.c_uint64))
+    f("tpgp_evolution_key_words", sz, C.c_void_p)
     _eng = L
     return L
 

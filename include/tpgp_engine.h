@@ -113,6 +113,29 @@ tpgp_status tpgp_hip_evolve(tpgp_map *map, const tpgp_term *terms, size_t count,
                             const tpgp_evolve_options *evolve, tpgp_ledger *ledger, double *fid,
                             double *fid_radius, size_t *active, size_t *accepted);
 
+/* The same loop as an incremental session: open uploads the map once; run may be called
+ * repeatedly (each call reads the FID before each of its steps and after its last); close
+ * downloads the final map into `map` (or discards it for NULL).  Counters: executed rotations,
+ * branch-updates (sum of input lengths of executed rotations) and sum of input+output lengths. */
+typedef struct tpgp_evolution tpgp_evolution;
+typedef struct {
+    uint64_t steps, rotations, updates, in_out;
+} tpgp_evolution_counters;
+tpgp_status tpgp_evolution_open(tpgp_evolution **evolution, const tpgp_map *map,
+                                const tpgp_term *terms, size_t count, double dt,
+                                const tpgp_options *options, const tpgp_evolve_options *evolve);
+tpgp_status tpgp_evolution_run(tpgp_evolution *evolution, size_t steps, tpgp_ledger *ledger,
+                               double *fid, double *fid_radius, size_t *active,
+                               size_t *accepted);
+tpgp_status tpgp_evolution_close(tpgp_evolution *evolution, tpgp_map *map);
+void tpgp_evolution_counters_get(const tpgp_evolution *evolution,
+                                 tpgp_evolution_counters *counters);
+/* HIP-event timing of the device kernels (kinds as in tpgp_hip.h TPGP_HIP_PROFILE_*). */
+int tpgp_evolution_profile(tpgp_evolution *evolution, int enable);
+int tpgp_evolution_profile_read(const tpgp_evolution *evolution, unsigned kind,
+                                double *milliseconds, uint64_t *launches);
+size_t tpgp_evolution_key_words(const tpgp_evolution *evolution);
+
 /* Message of the last device-side failure on this thread ("" when none). */
 const char *tpgp_hip_last_error(void);
 

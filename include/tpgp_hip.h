@@ -122,6 +122,11 @@ int tpgp_hip_batch_download(tpgp_hip_batch *batch, size_t map, uint64_t *keys, d
  * caller sequence number echoed in the records.  Asynchronous. */
 int tpgp_hip_batch_rotate(tpgp_hip_batch *batch, const tpgp_hip_rotation *requests,
                           uint32_t rotation, char *error, size_t error_capacity);
+/* Enqueue n_rot rotations in order (requests[r*n_maps + m], sequence numbers rotations[r]) with
+ * one descriptor upload for the whole sequence.  Same semantics as n_rot rotate() calls. */
+int tpgp_hip_batch_rotate_many(tpgp_hip_batch *batch, const tpgp_hip_rotation *requests,
+                               const uint32_t *rotations, size_t n_rot, char *error,
+                               size_t error_capacity);
 /* Enqueue a readout of the n_spins single-site coefficients of symbol `pauli` (1..3) for every
  * map with active[m] != 0 (NULL = all), in the current working state.  Asynchronous. */
 int tpgp_hip_batch_readout(tpgp_hip_batch *batch, unsigned pauli, const int *active,
@@ -148,6 +153,23 @@ int tpgp_hip_batch_info(const tpgp_hip_batch *batch, size_t map, size_t *committ
                         size_t *capacity);
 /* Stream the batch enqueues on (a hipStream_t), for callers that time or capture it. */
 void *tpgp_hip_batch_stream(const tpgp_hip_batch *batch);
+/* Kernel timing with HIP events recorded on the batch stream around every launch of a kind:
+ * the merge kernel, the partition scatter kernel, whole rotation passes, whole partition
+ * passes.  profile(1) resets and enables; times accumulate at each sync. */
+enum {
+    TPGP_HIP_PROFILE_MERGE = 0,     /* rotation write pass: merge and write the outputs */
+    TPGP_HIP_PROFILE_SCATTER = 1,   /* partition scatter */
+    TPGP_HIP_PROFILE_ROTATION = 2,  /* whole rotation pass */
+    TPGP_HIP_PROFILE_PARTITION = 3, /* whole partition pass */
+    TPGP_HIP_PROFILE_COUNT = 4,     /* rotation count pass (splits, counts, ledger sums) */
+    TPGP_HIP_PROFILE_SCAN = 5,      /* rotation offsets, regions, ledger fold */
+    TPGP_HIP_PROFILE_PLAN = 6,      /* rotation tile plan */
+    TPGP_HIP_PROFILE_HISTOGRAM = 7, /* partition histogram */
+    TPGP_HIP_PROFILE_SPLITS = 8     /* rotation merge-path splits */
+};
+int tpgp_hip_batch_profile(tpgp_hip_batch *batch, int enable);
+int tpgp_hip_batch_profile_read(const tpgp_hip_batch *batch, unsigned kind,
+                                double *milliseconds, uint64_t *launches);
 
 #ifdef __cplusplus
 }
