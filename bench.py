@@ -12,10 +12,12 @@ region is the next K steps, bracketed by device synchronisation, with the map al
 in HBM.  One branch-update = one input term processed by one executed rotation (the
 reference's ledger.rotations counts the same rotations, engine/src/tpgp_rev.c:740).
 
-roofline: the dominant kernel is the rotation merge (k_rot_merge).  Its algorithmic bytes per
-launch are (8*W + 8) * (len_in + len_out) for W key words (16 B per term for N <= 32): the input
-map read once and the output map written once.  Its duration is measured live with HIP events
-recorded on the batch stream around every launch in the timed region.
+roofline: the dominant kernel is whichever of the rotation count pass (k_rot_count) and the
+rotation write pass (k_rot_write) takes more time in the timed region.  Algorithmic bytes per
+launch, for W key words (16 B per term for N <= 32): k_rot_write (8W + 8)(len_in + len_out) --
+the input map read once and the output map written once; k_rot_count (8W + 8) len_in.  Their
+durations are measured live with HIP events recorded on the batch stream around every launch
+in the timed region (only those two kinds, unless --breakdown asks for all).
 
 cpu_baseline: the reference CPU engine (engine/src/tpgp_rev.c compiled from its own sources into
 oracle/_ref) -- or, where that build is absent, the oracle's C restatement -- runs the same
@@ -100,7 +102,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=600)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--breakdown", action="store_true",
+                    help="time every kernel kind with HIP events (adds a little per-launch cost)")
     ap.add_argument("--save-state", help="after the warmup, save the map to this .npz and exit")
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="measured HBM bytes per launch of the dominant kernel (rocprofv3 "
+                         "FETCH_SIZE + WRITE_SIZE, calibrated), reported as roofline.traffic")
     ap.add_argument("--load-state", help="start the timed region from a state saved by "
                                          "--save-state (its warmup count is used)")
     args = ap.parse_args()
@@ -157,7 +164,9 @@ def main():
     # timed region: K steps on the resident map
     ev = E.Evolution(mp, terms, dt, opts, initial_capacity=4 * len(mp) + (1 << 20))
     ev.run(0, led)  # one readout: allocation and first-touch outside the timed region
-    ev.profile(True)
+    timed_kinds = [N.PROFILE_MERGE, N.PROFILE_COUNT]
+    mask = N.PROFILE_ALL if args.breakdown else sum(1 << k for k in timed_kinds)
+    ev.profile(mask)
     c0 = ev.counters()
     t0 = time.perf_counter()
     st, fid, rad, act, acc = ev.run(args.steps, led)
@@ -166,8 +175,8 @@ def main():
         raise RuntimeError(f"timed run failed: status {st}: "
                            f"{N.engine().tpgp_hip_last_error().decode()}")
     c1 = ev.counters()
-    prof = {N.PROFILE_NAMES[k]: ev.profile_read(k) for k in range(len(N.PROFILE_NAMES))}
-    merge_ms, merge_n = prof["k_rot_write"]
+    kinds = range(len(N.PROFILE_NAMES)) if args.breakdown else timed_kinds
+    prof = {N.PROFILE_NAMES[k]: ev.profile_read(k) for k in kinds}
     ev.close()
     elapsed = t1 - t0
     updates = c1.updates - c0.updates
@@ -176,8 +185,14 @@ def main():
     key_bytes = 8 * ((n + 31) // 32)
     term_bytes = key_bytes + 8
     value = updates / elapsed
-    merge_bytes = term_bytes * in_out
-    achieved = merge_bytes / (merge_ms * 1e-3) / 1e9 if merge_ms > 0 else None
+    # dominant kernel: the timed kind with the largest total time.  Algorithmic bytes per
+    # launch: k_rot_write reads every input term once and writes every output term once;
+    # k_rot_count reads every input term once.
+    alg_bytes = {"k_rot_write": term_bytes * in_out, "k_rot_count": term_bytes * updates}
+    dom = max(alg_bytes, key=lambda k: prof[k][0])
+    dom_ms, dom_n = prof[dom]
+    dom_bytes = alg_bytes[dom]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
     log(f"timed {args.steps} steps: {elapsed:.3f} s, {rotations} rotations, {updates} updates, "
         f"terms {start_terms} -> {int(act[-1])}")
     for name, (ms, cnt) in prof.items():
@@ -213,10 +228,13 @@ def main():
                    "rotations": rotations, "branch_updates": updates,
                    "fid_at_end": float(fid[-1])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
-                     "kernel": "k_rot_merge", "launches": merge_n,
-                     "avg_us": merge_ms / max(merge_n, 1) * 1e3,
-                     "algorithmic_bytes_per_launch": merge_bytes / max(merge_n, 1)},
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                     "traffic": args.traffic,
+                     "kernel": dom, "launches": dom_n,
+                     "avg_us": dom_ms / max(dom_n, 1) * 1e3,
+                     "algorithmic_bytes_per_launch": dom_bytes / max(dom_n, 1),
+                     "kernels": {k: {"launches": n_, "avg_us": ms / max(n_, 1) * 1e3}
+                                 for k, (ms, n_) in prof.items()}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

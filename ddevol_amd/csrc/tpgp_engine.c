@@ -902,8 +902,8 @@ tpgp_status tpgp_evolution_close(tpgp_evolution *ev, tpgp_map *m) {
 void tpgp_evolution_counters_get(const tpgp_evolution *ev, tpgp_evolution_counters *c) {
     if (ev && c) *c = ev->c;
 }
-int tpgp_evolution_profile(tpgp_evolution *ev, int enable) {
-    return ev ? tpgp_hip_batch_profile(ev->S.batch, enable) : TPGP_HIP_EINVAL;
+int tpgp_evolution_profile(tpgp_evolution *ev, unsigned mask) {
+    return ev ? tpgp_hip_batch_profile(ev->S.batch, mask) : TPGP_HIP_EINVAL;
 }
 int tpgp_evolution_profile_read(const tpgp_evolution *ev, unsigned kind, double *ms,
                                 uint64_t *launches) {

@@ -42,6 +42,7 @@ namespace {
 constexpr int kBlock = 256;        // 4 waves of 64
 constexpr int kMergeItems = 4;     // merge-path elements per thread
 constexpr int kMergeTile = kBlock * kMergeItems;
+constexpr int kChunkTiles = 4 * kBlock;  // rotation tiles per scan chunk
 constexpr int kPartRounds = 16;    // partition: 16 rounds of 256 elements
 constexpr int kPartTile = kBlock * kPartRounds;
 
@@ -293,8 +294,12 @@ struct BatchDev {
     uint64_t *tile_base;      // [maxT*16]
     uint64_t *map_hist;       // [M*16]
     uint64_t *splits;         // [maxT+1]
-    uint64_t *class_out;      // [M*16]
+    struct TileDesc *tdesc;   // [maxT] rotation tile descriptors (written by the split pass)
     TileLedger *tile_led;     // [maxT]
+    uint64_t *chunk_start;    // [M+1] first scan chunk of every map (rotation)
+    uint64_t *chunk_base;     // [maxC] output offset of every chunk within its map
+    uint32_t *chunk_sum;      // [maxC] kept outputs of every chunk
+    TileLedger *chunk_led;    // [maxC] ledger partials of every chunk
     uint32_t *counter;        // [4]
     uint32_t *overflow;       // [M]
     tpgp_hip_record *rec;     // [M*max_rec]
@@ -324,7 +329,6 @@ __device__ __forceinline__ uint32_t find_bucket(const uint64_t *start, uint32_t 
 __global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap *pm) {
     __shared__ unsigned scratch[8];
     const uint32_t M = b.n_maps;
-    for (uint32_t t = threadIdx.x; t < M * 16; t += blockDim.x) b.map_hist[t] = 0;
     // tiles per map, exclusive scan over maps in chunks of 256
     uint64_t carry = 0;
     for (uint32_t base = 0; base < M; base += kBlock) {
@@ -333,6 +337,11 @@ __global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap 
         if (m < M && pm[m].active) {
             uint64_t len = b.len[m * 3 + pm[m].in_buf];
             n = (unsigned)((len + kPartTile - 1) / kPartTile);
+            if (len == 0) {  // no tile will describe the (empty) output: do it here
+                uint64_t *reg = b.regions + ((uint64_t)m * 3 + pm[m].out_buf) * 17;
+                for (int c = 0; c < 17; ++c) reg[c] = 0;
+                b.len[m * 3 + pm[m].out_buf] = 0;
+            }
         }
         unsigned tot;
         unsigned ex = block_exclusive_scan(n, scratch, &tot);
@@ -342,6 +351,8 @@ __global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap 
     if (threadIdx.x == 0) b.tstart[M] = carry;
 }
 
+// Per tile: the histogram of the 16 new classes (digits pa, pb).  All of a thread's key loads
+// are issued before the first histogram update.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap *pm) {
     __shared__ unsigned hist[16];
@@ -349,39 +360,39 @@ __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap
     const uint64_t ntiles = b.tstart[M];
     for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
         if (threadIdx.x < 16) hist[threadIdx.x] = 0;
-        __syncthreads();
         uint32_t m = find_bucket(b.tstart, M, g);
         const PassMap &P = pm[m];
         uint64_t len = b.len[m * 3 + P.in_buf];
         uint64_t e0 = (g - b.tstart[m]) * kPartTile;
         uint64_t e1 = e0 + kPartTile < len ? e0 + kPartTile : len;
         const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[m].off * W;
-        for (uint64_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
-            Key<W> k = kload<W>(keys + e * W);
-            unsigned cls = (kdigit<W>(k, P.pa) << 2) | kdigit<W>(k, P.pb);
-            atomicAdd(&hist[cls], 1u);
+        Key<W> k[kPartRounds];
+#pragma unroll
+        for (int r = 0; r < kPartRounds; ++r) {
+            const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
+            if (e < e1) k[r] = kload<W>(keys + e * W);
         }
         __syncthreads();
-        if (threadIdx.x < 16) {
-            b.tile_hist[g * 16 + threadIdx.x] = hist[threadIdx.x];
-            if (hist[threadIdx.x])
-                atomicAdd((unsigned long long *)&b.map_hist[m * 16 + threadIdx.x],
-                          (unsigned long long)hist[threadIdx.x]);
+#pragma unroll
+        for (int r = 0; r < kPartRounds; ++r) {
+            const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
+            if (e < e1) atomicAdd(&hist[(kdigit<W>(k[r], P.pa) << 2) | kdigit<W>(k[r], P.pb)], 1u);
         }
+        __syncthreads();
+        if (threadIdx.x < 16) b.tile_hist[g * 16 + threadIdx.x] = hist[threadIdx.x];
         __syncthreads();
     }
 }
 
-// One block per (map, class): exclusive scan of that class's tile counts over the map's tiles.
+// One block per (map, class): exclusive scan of that class's tile counts over the map's tiles
+// (offsets within the class) and the class total.
 __global__ __launch_bounds__(kBlock) void k_part_scan(BatchDev b, const PassMap *pm) {
     __shared__ unsigned scratch[8];
     const uint32_t m = blockIdx.x >> 4, cls = blockIdx.x & 15;
     const PassMap &P = pm[m];
     if (!P.active) return;
-    uint64_t class_base = 0;
-    for (unsigned c = 0; c < cls; ++c) class_base += b.map_hist[m * 16 + c];
     const uint64_t t0 = b.tstart[m], t1 = b.tstart[m + 1];
-    uint64_t carry = class_base;
+    uint64_t carry = 0;
     for (uint64_t base = t0; base < t1; base += kBlock) {
         uint64_t g = base + threadIdx.x;
         unsigned v = g < t1 ? b.tile_hist[g * 16 + cls] : 0;
@@ -390,16 +401,11 @@ __global__ __launch_bounds__(kBlock) void k_part_scan(BatchDev b, const PassMap 
         if (g < t1) b.tile_base[g * 16 + cls] = carry + ex;
         carry += tot;
     }
-    if (threadIdx.x == 0) {
-        uint64_t *reg = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
-        reg[cls] = class_base;
-        if (cls == 15) {
-            reg[16] = carry;
-            b.len[m * 3 + P.out_buf] = carry;
-        }
-    }
+    if (threadIdx.x == 0) b.map_hist[m * 16 + cls] = carry;
 }
 
+// Per tile: move every element to its class's run (stable), with digits pa, pb moved to the
+// top of the key.  The map's first tile also writes the output regions and length.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassMap *pm) {
     __shared__ unsigned wave_cnt[4][16];
@@ -417,19 +423,36 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassM
         const double *vals = b.arena[P.in_buf].vals + b.slot[m].off;
         uint64_t *okeys = b.arena[P.out_buf].keys + b.slot[m].off * W;
         double *ovals = b.arena[P.out_buf].vals + b.slot[m].off;
-        if (threadIdx.x < 16) run[threadIdx.x] = b.tile_base[g * 16 + threadIdx.x];
-        __syncthreads();
-        for (uint64_t r0 = e0; r0 < e1; r0 += kBlock) {
-            uint64_t e = r0 + threadIdx.x;
-            bool live = e < e1;
-            Key<W> k;
-            double v = 0.0;
-            unsigned cls = 16;
-            if (live) {
-                k = kload<W>(keys + e * W);
-                v = vals[e];
-                cls = (kdigit<W>(k, P.pa) << 2) | kdigit<W>(k, P.pb);
+        Key<W> kk[kPartRounds];
+        double vv[kPartRounds];
+#pragma unroll
+        for (int r = 0; r < kPartRounds; ++r) {
+            const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
+            if (e < e1) {
+                kk[r] = kload<W>(keys + e * W);
+                vv[r] = vals[e];
             }
+        }
+        if (threadIdx.x < 16) {
+            uint64_t base = 0;
+            for (unsigned c = 0; c < threadIdx.x; ++c) base += b.map_hist[m * 16 + c];
+            run[threadIdx.x] = base + b.tile_base[g * 16 + threadIdx.x];
+            if (g == b.tstart[m]) {
+                uint64_t *reg = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
+                reg[threadIdx.x] = base;
+                if (threadIdx.x == 15) {
+                    const uint64_t total = base + b.map_hist[m * 16 + 15];
+                    reg[16] = total;
+                    b.len[m * 3 + P.out_buf] = total;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kPartRounds; ++r) {
+            const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
+            const bool live = e < e1;
+            const unsigned cls = live ? (kdigit<W>(kk[r], P.pa) << 2) | kdigit<W>(kk[r], P.pb) : 16u;
             // rank among equal classes in lower lanes of this wave
             unsigned below = 0;
             const uint64_t lanemask = (1ull << lane_id()) - 1ull;
@@ -442,9 +465,8 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassM
             if (live) {
                 uint64_t dst = run[cls] + below;
                 for (unsigned w = 0; w < wave_id(); ++w) dst += wave_cnt[w][cls];
-                Key<W> nk = move_to_top<W>(k, P.pa, P.pb, n);
-                kstore<W>(okeys + dst * W, nk);
-                ovals[dst] = v;
+                kstore<W>(okeys + dst * W, move_to_top<W>(kk[r], P.pa, P.pb, n));
+                ovals[dst] = vv[r];
             }
             __syncthreads();
             if (threadIdx.x < 16) {
@@ -489,7 +511,6 @@ __global__ __launch_bounds__(kBlock) void k_rot_plan(BatchDev b, const PassMap *
     __shared__ unsigned scratch[8];
     const uint32_t M = b.n_maps, n = M * 16;
     if (threadIdx.x == 0) b.counter[0] = 0;
-    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) b.class_out[t] = 0;
     uint64_t carry = 0;
     for (uint32_t base = 0; base < n; base += kBlock) {
         uint32_t t = base + threadIdx.x;
@@ -514,12 +535,34 @@ __global__ __launch_bounds__(kBlock) void k_rot_plan(BatchDev b, const PassMap *
         carry += tot;
     }
     if (threadIdx.x == 0) b.tstart[n] = carry;
+    __syncthreads();
+    // scan chunks of every map: ceil(tiles / kChunkTiles), exclusive prefix over maps
+    uint64_t ccarry = 0;
+    for (uint32_t base = 0; base < M; base += kBlock) {
+        const uint32_t m = base + threadIdx.x;
+        unsigned chunks = 0;
+        if (m < M) {
+            const uint64_t tiles = b.tstart[m * 16 + 16] - b.tstart[m * 16];
+            chunks = (unsigned)((tiles + kChunkTiles - 1) / kChunkTiles);
+        }
+        unsigned tot;
+        unsigned ex = block_exclusive_scan(chunks, scratch, &tot);
+        if (m < M) b.chunk_start[m] = ccarry + ex;
+        ccarry += tot;
+    }
+    if (threadIdx.x == 0) b.chunk_start[M] = ccarry;
 }
 
 struct MergeCtx {
     uint32_t m, c, cq;
     bool anti;
     uint64_t A0, nA, D0, nD, k;
+};
+// Where a rotation tile lies: its map and class, the class regions it merges, its index in the
+// class; flags bit0: anticommuting class, bit1: last tile of its class.
+struct TileDesc {
+    uint64_t A0, nA, D0, nD, k;
+    uint32_t m, c, cq, flags;
 };
 
 __device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm, uint64_t g,
@@ -536,43 +579,6 @@ __device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm, uin
     x.D0 = reg[x.cq];
     x.nD = x.anti ? reg[x.cq + 1] - reg[x.cq] : 0;
     x.k = g - b.tstart[t];
-}
-
-// Merge-path split, computed by one wave: the number of A elements among the first d elements
-// of merge(A, T(D)) (ties: A first).  P(x) = A[x] > T(D[d-x-1]) is monotone; the split is the
-// first x with P true, found with 64-ary search rounds.  All lanes return the same value.
-template <int W>
-__device__ uint64_t split_search(const uint64_t *keys, const MergeCtx &x, const Key<W> &q,
-                                 uint64_t d) {
-    uint64_t lo = d > x.nD ? d - x.nD : 0, hi = d < x.nA ? d : x.nA;
-    auto pred = [&](uint64_t xi) -> bool {
-        Key<W> a = kload<W>(keys + (x.A0 + xi) * W);
-        Key<W> dk = kxor<W>(kload<W>(keys + (x.D0 + d - xi - 1) * W), q);
-        return klt<W>(dk, a);
-    };
-    while (hi > lo) {
-        uint64_t span = hi - lo;
-        if (span < 64) {  // lane `span` maps to hi (P(hi) is true): the ballot is never 0
-            uint64_t xi = lo + lane_id();
-            bool p = xi < hi ? pred(xi) : true;
-            uint64_t bal = __ballot(p);
-            lo = lo + (uint64_t)(__ffsll((unsigned long long)bal) - 1);
-            break;
-        }
-        uint64_t xi = lo + ((span * lane_id()) >> 6);
-        uint64_t bal = __ballot(pred(xi));
-        if (bal == 0) {
-            lo = lo + ((span * 63) >> 6) + 1;
-        } else {
-            int f = __ffsll((unsigned long long)bal) - 1;
-            if (f == 0) break;  // P(lo) true: the split is lo
-            uint64_t xf = lo + ((span * (uint64_t)f) >> 6);
-            uint64_t xp = lo + ((span * (uint64_t)(f - 1)) >> 6);
-            lo = xp + 1;
-            hi = xf;
-        }
-    }
-    return lo;
 }
 
 // Tile inputs in LDS: A at [0, na), T(D) at [na, na+nd), the D element after the tile at na+nd
@@ -603,6 +609,9 @@ __device__ __forceinline__ bool tile_ok(const MergeCtx &x, const TileGeom &t) {
              t.d1 - t.d0 > (uint64_t)kMergeTile);
 }
 
+// Stage a tile's inputs in LDS.  Every thread issues all of its global loads (A elements, then
+// T(D) elements, in one index space) before its first LDS store, so the tile costs one memory
+// latency rather than one per loop trip.
 template <int W>
 __device__ void load_tile(const BatchDev &b, const PassMap &P, const MergeCtx &x,
                           const Key<W> &q, TileGeom &t, TileIn<W> &in) {
@@ -614,21 +623,43 @@ __device__ void load_tile(const BatchDev &b, const PassMap &P, const MergeCtx &x
     t.has_prev = x.anti && t.a0 > 0;
     const uint64_t *ikeys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
     const double *ivals = b.arena[P.in_buf].vals + b.slot[x.m].off;
-    for (uint32_t i = threadIdx.x; i < t.na; i += kBlock) {
-        kstore<W>(in.keys + i * W, kload<W>(ikeys + (x.A0 + t.a0 + i) * W));
-        in.vals[i] = ivals[x.A0 + t.a0 + i];
+    constexpr int R = kMergeTile / kBlock;
+    const uint32_t na = t.na, tot = t.na + t.nd;  // tot <= kMergeTile (checked by tile_ok)
+    const uint64_t abase = x.A0 + t.a0, dbase = x.D0 + t.b0 - na;
+    Key<W> kk[R];
+    double vv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t e = threadIdx.x + r * kBlock;
+        if (e < tot) {
+            const uint64_t src = e < na ? abase + e : dbase + e;
+            kk[r] = kload<W>(ikeys + src * W);
+            vv[r] = ivals[src];
+        }
     }
-    for (uint32_t i = threadIdx.x; i < t.nd; i += kBlock) {
-        kstore<W>(in.keys + (t.na + i) * W, kxor<W>(kload<W>(ikeys + (x.D0 + t.b0 + i) * W), q));
-        in.vals[t.na + i] = ivals[x.D0 + t.b0 + i];
+    Key<W> knext, kprev;
+    double vnext = 0.0;
+    if (threadIdx.x == 0) {
+        if (t.has_next) {
+            knext = kload<W>(ikeys + (x.D0 + t.b1) * W);
+            vnext = ivals[x.D0 + t.b1];
+        }
+        if (t.has_prev) kprev = kload<W>(ikeys + (x.A0 + t.a0 - 1) * W);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t e = threadIdx.x + r * kBlock;
+        if (e < tot) {
+            kstore<W>(in.keys + e * W, e < na ? kk[r] : kxor<W>(kk[r], q));
+            in.vals[e] = vv[r];
+        }
     }
     if (threadIdx.x == 0) {
         if (t.has_next) {
-            kstore<W>(in.keys + (t.na + t.nd) * W, kxor<W>(kload<W>(ikeys + (x.D0 + t.b1) * W), q));
-            in.vals[t.na + t.nd] = ivals[x.D0 + t.b1];
+            kstore<W>(in.keys + tot * W, kxor<W>(knext, q));
+            in.vals[tot] = vnext;
         }
-        if (t.has_prev)
-            kstore<W>(in.keys + (kMergeTile + 1) * W, kload<W>(ikeys + (x.A0 + t.a0 - 1) * W));
+        if (t.has_prev) kstore<W>(in.keys + (kMergeTile + 1) * W, kprev);
     }
 }
 
@@ -757,13 +788,57 @@ __device__ __forceinline__ void log_entry(const BatchDev &b, uint64_t k0, uint64
     }
 }
 
-// Pass 0 of a rotation: one wave per tile computes the tile's merge-path start split.  All
-// tiles search at once, so the dependent global loads of the searches overlap.
+// Merge-path split by 8 cooperating lanes (lanes 8*grp .. 8*grp+7 of the wave, `sub` = lane
+// within the group): the number of A elements among the first d elements of merge(A, T(D))
+// (ties: A first).  P(x) = A[x] > T(D[d-x-1]) is monotone; the split is the first x in
+// [lo, hi] with P(x) true (P(hi) counts as true), found with 8-ary rounds.  All 8 lanes return
+// the same value.  Lanes of one group stay in lockstep; groups may take different numbers of
+// rounds (the ballot of a round only holds the lanes still searching).
+template <int W>
+__device__ uint64_t split_search8(const uint64_t *keys, const MergeCtx &x, const Key<W> &q,
+                                  uint64_t d, unsigned grp, unsigned sub) {
+    uint64_t lo = d > x.nD ? d - x.nD : 0, hi = d < x.nA ? d : x.nA;
+    auto pred = [&](uint64_t xi) -> bool {
+        Key<W> a = kload<W>(keys + (x.A0 + xi) * W);
+        Key<W> dk = kxor<W>(kload<W>(keys + (x.D0 + d - xi - 1) * W), q);
+        return klt<W>(dk, a);
+    };
+    while (hi > lo) {
+        const uint64_t span = hi - lo;
+        if (span < 8) {  // lane `span` maps to hi (P(hi) is true): the group's byte is never 0
+            const uint64_t xi = lo + sub;
+            const bool p = xi < hi ? pred(xi) : true;
+            const unsigned byte = (unsigned)(__ballot(p) >> (8 * grp)) & 0xffu;
+            lo = lo + (uint64_t)(__ffs(byte) - 1);
+            break;
+        }
+        const uint64_t xi = lo + ((span * sub) >> 3);
+        const unsigned byte = (unsigned)(__ballot(pred(xi)) >> (8 * grp)) & 0xffu;
+        if (byte == 0) {
+            lo = lo + ((span * 7) >> 3) + 1;
+        } else {
+            const int f = __ffs(byte) - 1;
+            if (f == 0) break;  // P(lo) true: the split is lo
+            const uint64_t xf = lo + ((span * (uint64_t)f) >> 3);
+            const uint64_t xp = lo + ((span * (uint64_t)(f - 1)) >> 3);
+            lo = xp + 1;
+            hi = xf;
+        }
+    }
+    return lo;
+}
+
+// Pass 0 of a rotation: every tile's descriptor and merge-path start split, 8 lanes per tile
+// (32 tiles per workgroup, all searching at once so the dependent loads overlap).
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap *pm) {
     const uint64_t ntiles = b.tstart[b.n_maps * 16];
-    const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
-    for (uint64_t g = (uint64_t)blockIdx.x * (kBlock / 64) + wave_id(); g < ntiles; g += waves) {
+    const unsigned grp = lane_id() >> 3, sub = lane_id() & 7u;
+    const uint64_t per_block = kBlock / 8;
+    for (uint64_t base = (uint64_t)blockIdx.x * per_block; base < ntiles;
+         base += (uint64_t)gridDim.x * per_block) {
+        const uint64_t g = base + threadIdx.x / 8;
+        if (g >= ntiles) continue;  // whole 8-lane groups leave together
         MergeCtx x;
         locate(b, pm, g, x);
         const uint64_t d0 = x.k * kMergeTile;
@@ -773,10 +848,41 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
             const uint32_t n = b.slot[x.m].n_spins;
             const uint64_t *ikeys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
             const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
-            a0 = split_search<W>(ikeys, x, q, d0);
+            a0 = split_search8<W>(ikeys, x, q, d0, grp, sub);
         }
-        if (lane_id() == 0) b.splits[g] = a0;
+        if (sub == 0) {
+            b.splits[g] = a0;
+            TileDesc t;
+            t.A0 = x.A0;
+            t.nA = x.nA;
+            t.D0 = x.D0;
+            t.nD = x.nD;
+            t.k = x.k;
+            t.m = x.m;
+            t.c = x.c;
+            t.cq = x.cq;
+            const bool last = g + 1 >= b.tstart[x.m * 16 + x.c + 1];
+            t.flags = (x.anti ? 1u : 0u) | (last ? 2u : 0u);
+            b.tdesc[g] = t;
+        }
     }
+}
+
+// A tile's context and splits from its descriptor (uniform loads: no bucket search).
+__device__ __forceinline__ void tile_from_desc(const BatchDev &b, uint64_t g, MergeCtx &x,
+                                               uint64_t &a0, uint64_t &a1) {
+    const TileDesc t = b.tdesc[g];
+    x.A0 = t.A0;
+    x.nA = t.nA;
+    x.D0 = t.D0;
+    x.nD = t.nD;
+    x.k = t.k;
+    x.m = t.m;
+    x.c = t.c;
+    x.cq = t.cq;
+    x.anti = (t.flags & 1u) != 0;
+    a0 = b.splits[g];
+    a1 = (t.flags & 2u) ? t.nA : b.splits[g + 1];
 }
 
 // Pass 1 of a rotation: per tile, the merge-path splits, the number of kept outputs, the
@@ -788,18 +894,14 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap 
     const uint64_t ntiles = b.tstart[b.n_maps * 16];
     for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
         MergeCtx x;
-        locate(b, pm, g, x);
+        TileGeom t;
+        tile_from_desc(b, g, x, t.a0, t.a1);
         const PassMap &P = pm[x.m];
         const uint32_t n = b.slot[x.m].n_spins;
-        const uint64_t *ikeys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
         const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
-        TileGeom t;
         const uint64_t total = x.nA + x.nD;
         t.d0 = x.k * kMergeTile;
         t.d1 = t.d0 + kMergeTile < total ? t.d0 + kMergeTile : total;
-        (void)ikeys;
-        t.a0 = b.splits[g];
-        t.a1 = (g + 1 < b.tstart[x.m * 16 + x.c + 1]) ? b.splits[g + 1] : x.nA;
         if (threadIdx.x == 0) misc.nonfinite = 0;
         if (!tile_ok(x, t)) {
             if (threadIdx.x == 0) {
@@ -879,77 +981,114 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap 
             tl.pad = 0;
             b.tile_led[g] = tl;
             b.tile_hist[g] = tot;
-            atomicAdd((unsigned long long *)&b.class_out[x.m * 16 + x.c], (unsigned long long)tot);
         }
         __syncthreads();
     }
 }
 
-// One block per map: output offsets of the tiles (exclusive scan of their counts in class
-// order), output regions and length, the ledger fold in tile order, and the record.
-__global__ __launch_bounds__(kBlock) void k_rot_scan(BatchDev b, const PassMap *pm) {
-    const uint32_t m = blockIdx.x;
-    const PassMap &P = pm[m];
-    if (!P.active) return;
-    __shared__ unsigned scratch[8];
-    __shared__ double red[3][kBlock];
-    __shared__ uint64_t redn[2][kBlock];
-    __shared__ unsigned bad;
-    const uint64_t t0 = b.tstart[m * 16], t1 = b.tstart[m * 16 + 16];
-    // each thread owns a contiguous run of tiles: sum it, scan the run totals, then write the
-    // running offsets (one pass of independent loads instead of a chunked loop)
-    const uint64_t run = (t1 - t0 + kBlock - 1) / kBlock;
-    const uint64_t r0 = t0 + threadIdx.x * run, r1 = r0 + run < t1 ? r0 + run : t1;
-    uint64_t mine = 0;
-    for (uint64_t g = r0; g < r1; ++g) mine += b.tile_hist[g];
-    __shared__ uint64_t runs[kBlock];
-    runs[threadIdx.x] = mine;
+__device__ __forceinline__ void ledger_fold(TileLedger &a, const TileLedger &x) {
+    a.l1 = upadd(a.l1, x.l1);
+    a.sq = upadd(a.sq, x.sq);
+    a.rsq = upadd(a.rsq, x.rsq);
+    a.dropped += x.dropped;
+    a.radii += x.radii;
+    a.nonfinite |= x.nonfinite;
+}
+// Fixed-shape block reduction of per-thread ledgers (deterministic for a given input).
+__device__ TileLedger block_ledger(TileLedger mine, TileLedger *sh /*[kBlock]*/) {
+    sh[threadIdx.x] = mine;
     __syncthreads();
-    for (int d = 1; d < kBlock; d <<= 1) {  // inclusive Hillis-Steele scan of the run totals
-        uint64_t v = (int)threadIdx.x >= d ? runs[threadIdx.x - d] : 0;
-        __syncthreads();
-        runs[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint64_t at = runs[threadIdx.x] - mine;
-    const uint64_t carry = runs[kBlock - 1];
-    for (uint64_t g = r0; g < r1; ++g) {
-        b.tile_base[g] = at;
-        at += b.tile_hist[g];
-    }
-    (void)scratch;
-    double l1 = 0.0, sq = 0.0, rsq = 0.0;
-    uint64_t nd = 0, nr = 0;
-    unsigned nf = 0;
-    const uint64_t nt = t1 - t0;
-    const uint64_t per = (nt + kBlock - 1) / kBlock;
-    for (uint64_t t = t0 + threadIdx.x * per; t < t0 + (threadIdx.x + 1) * per && t < t1; ++t) {
-        const TileLedger &L = b.tile_led[t];
-        l1 = upadd(l1, L.l1);
-        sq = upadd(sq, L.sq);
-        rsq = upadd(rsq, L.rsq);
-        nd += L.dropped;
-        nr += L.radii;
-        nf |= L.nonfinite;
-    }
-    red[0][threadIdx.x] = l1;
-    red[1][threadIdx.x] = sq;
-    red[2][threadIdx.x] = rsq;
-    redn[0][threadIdx.x] = nd;
-    redn[1][threadIdx.x] = nr;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
-    if (nf) atomicOr(&bad, 1u);
     for (int st = kBlock / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) {
-            red[0][threadIdx.x] = upadd(red[0][threadIdx.x], red[0][threadIdx.x + st]);
-            red[1][threadIdx.x] = upadd(red[1][threadIdx.x], red[1][threadIdx.x + st]);
-            red[2][threadIdx.x] = upadd(red[2][threadIdx.x], red[2][threadIdx.x + st]);
-            redn[0][threadIdx.x] += redn[0][threadIdx.x + st];
-            redn[1][threadIdx.x] += redn[1][threadIdx.x + st];
+            TileLedger a = sh[threadIdx.x];
+            ledger_fold(a, sh[threadIdx.x + st]);
+            sh[threadIdx.x] = a;
         }
         __syncthreads();
     }
+    TileLedger r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Scan level 1, one block per chunk of kChunkTiles tiles of one map: tile offsets within the
+// chunk (exclusive scan of the kept counts), the chunk total and the chunk's ledger partials.
+__global__ __launch_bounds__(kBlock) void k_rot_scan1(BatchDev b, const PassMap *pm) {
+    __shared__ unsigned scratch[8];
+    __shared__ TileLedger sh[kBlock];
+    const uint32_t M = b.n_maps;
+    const uint64_t nchunks = b.chunk_start[M];
+    for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const uint32_t m = find_bucket(b.chunk_start, M, ch);
+        const uint64_t t1 = b.tstart[m * 16 + 16];
+        const uint64_t g0 = b.tstart[m * 16] + (ch - b.chunk_start[m]) * kChunkTiles;
+        constexpr int per = kChunkTiles / kBlock;
+        const uint64_t mine0 = g0 + (uint64_t)threadIdx.x * per;
+        unsigned cnt[per];
+        unsigned sum = 0;
+#pragma unroll
+        for (int i = 0; i < per; ++i) {
+            cnt[i] = mine0 + i < t1 ? b.tile_hist[mine0 + i] : 0u;
+            sum += cnt[i];
+        }
+        unsigned tot;
+        unsigned at = block_exclusive_scan(sum, scratch, &tot);
+        TileLedger L{};
+#pragma unroll
+        for (int i = 0; i < per; ++i) {
+            if (mine0 + i < t1) {
+                b.tile_base[mine0 + i] = at;
+                at += cnt[i];
+                ledger_fold(L, b.tile_led[mine0 + i]);
+            }
+        }
+        TileLedger C = block_ledger(L, sh);
+        if (threadIdx.x == 0) {
+            b.chunk_sum[ch] = tot;
+            b.chunk_led[ch] = C;
+        }
+    }
+    (void)pm;
+}
+
+// Scan level 2, one block per map: chunk offsets, output regions and length, the ledger fold
+// in chunk order, and the record.
+__global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap *pm) {
+    const uint32_t m = blockIdx.x;
+    const PassMap &P = pm[m];
+    if (!P.active) return;
+    __shared__ uint64_t runs[kBlock];
+    __shared__ TileLedger sh[kBlock];
+    const uint64_t c0 = b.chunk_start[m], c1 = b.chunk_start[m + 1];
+    uint64_t carry = 0;
+    TileLedger L{};
+    for (uint64_t base = c0; base < c1; base += kBlock) {
+        const uint64_t ch = base + threadIdx.x;
+        const uint64_t v = ch < c1 ? b.chunk_sum[ch] : 0;
+        if (ch < c1) ledger_fold(L, b.chunk_led[ch]);
+        runs[threadIdx.x] = v;
+        __syncthreads();
+        for (int d = 1; d < kBlock; d <<= 1) {  // inclusive Hillis-Steele scan
+            uint64_t y = (int)threadIdx.x >= d ? runs[threadIdx.x - d] : 0;
+            __syncthreads();
+            runs[threadIdx.x] += y;
+            __syncthreads();
+        }
+        if (ch < c1) b.chunk_base[ch] = carry + runs[threadIdx.x] - v;
+        carry += runs[kBlock - 1];
+        __syncthreads();
+    }
+    const TileLedger T = block_ledger(L, sh);  // (its barriers publish chunk_base too)
+    // output class c starts where its first tile writes; an empty class starts where the
+    // next nonempty class does (or at the end)
+    __shared__ uint64_t cstart[16];
+    if (threadIdx.x < 16) {
+        const uint64_t tm0 = b.tstart[m * 16];
+        const uint64_t t = b.tstart[m * 16 + threadIdx.x], te = b.tstart[m * 16 + threadIdx.x + 1];
+        cstart[threadIdx.x] =
+            t < te ? b.chunk_base[c0 + (t - tm0) / kChunkTiles] + b.tile_base[t] : ~0ull;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         const uint64_t *rin = b.regions + ((uint64_t)m * 3 + P.in_buf) * 17;
         uint64_t *rout = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
@@ -957,10 +1096,10 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan(BatchDev b, const PassMap *
         // A map that outgrew its slot is emptied: later passes never read past the slot; its
         // record carries the overflow and the host rolls the map back and grows the slot.
         const bool over = b.overflow[m] || acc > b.slot[m].cap;
-        uint64_t at = 0;
-        for (int c = 0; c < 16; ++c) {
-            rout[c] = over ? 0 : at;
-            at += b.class_out[m * 16 + c];
+        uint64_t next = acc;
+        for (int c = 15; c >= 0; --c) {
+            if (cstart[c] != ~0ull) next = cstart[c];
+            rout[c] = over ? 0 : next;
         }
         rout[16] = over ? 0 : acc;
         const uint64_t len_in = rin[16];
@@ -969,13 +1108,13 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan(BatchDev b, const PassMap *
         tpgp_hip_record r;
         r.len_in = len_in;
         r.len_out = acc;
-        r.dropped = redn[0][0];
-        r.l1_up = red[0][0];
-        r.sq_up = red[1][0];
-        r.rad_sq_up = red[2][0];
-        r.radii = redn[1][0];
+        r.dropped = T.dropped;
+        r.l1_up = T.l1;
+        r.sq_up = T.sq;
+        r.rad_sq_up = T.rsq;
+        r.radii = T.radii;
         r.executed = len_in > 0;
-        r.nonfinite = bad;
+        r.nonfinite = T.nonfinite;
         r.overflow = over ? 1u : 0u;
         r.rotation = P.rot_seq;
         b.rec[(uint64_t)m * b.max_rec + P.rec_slot] = r;
@@ -993,17 +1132,17 @@ __global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap 
     const uint64_t ntiles = b.tstart[b.n_maps * 16];
     for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
         MergeCtx x;
-        locate(b, pm, g, x);
+        TileGeom t;
+        tile_from_desc(b, g, x, t.a0, t.a1);
         const PassMap &P = pm[x.m];
         const uint32_t n = b.slot[x.m].n_spins;
         const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
-        TileGeom t;
         const uint64_t total = x.nA + x.nD;
         t.d0 = x.k * kMergeTile;
         t.d1 = t.d0 + kMergeTile < total ? t.d0 + kMergeTile : total;
-        t.a0 = b.splits[g];
-        t.a1 = (g + 1 < b.tstart[x.m * 16 + x.c + 1]) ? b.splits[g + 1] : x.nA;
-        const uint64_t out_pos = b.tile_base[g];
+        const uint64_t tm0 = b.tstart[x.m * 16];
+        const uint64_t out_pos =
+            b.chunk_base[b.chunk_start[x.m] + (g - tm0) / kChunkTiles] + b.tile_base[g];
         const uint64_t count = b.tile_hist[g];
         // overflowed maps and inconsistent tiles write nothing (both are reported at sync)
         if (!tile_ok(x, t) || b.overflow[x.m] || out_pos + count > b.slot[x.m].cap) continue;
@@ -1395,7 +1534,7 @@ struct tpgp_hip_batch {
     uint64_t est_tiles = 1;  // merge tiles of the last planned rotation (grid sizing only)
     uint32_t pass_count = 0;
     // kernel timing with HIP events on the batch stream (kinds: TPGP_HIP_PROFILE_*)
-    bool profiling = false;
+    unsigned profiling = 0;  // bit k: bracket launches of profile kind k with events
     std::vector<hipEvent_t> event_pool;
     struct Pending {
         unsigned kind;
@@ -1462,8 +1601,12 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     ALLOC(d.tile_base, b->max_tiles * 16);
     ALLOC(d.map_hist, (size_t)M * 16);
     ALLOC(d.splits, b->max_tiles + 1);
-    ALLOC(d.class_out, (size_t)M * 16);
+    ALLOC(d.tdesc, b->max_tiles);
     ALLOC(d.tile_led, b->max_tiles);
+    ALLOC(d.chunk_start, (size_t)M + 1);
+    ALLOC(d.chunk_base, b->max_tiles / kChunkTiles + M + 1);
+    ALLOC(d.chunk_sum, b->max_tiles / kChunkTiles + M + 1);
+    ALLOC(d.chunk_led, b->max_tiles / kChunkTiles + M + 1);
     ALLOC(d.counter, 4);
     ALLOC(d.overflow, M);
     ALLOC(d.rec, (size_t)M * b->max_rec);
@@ -1522,25 +1665,24 @@ int next_buffer(const HostMap &hm) {
     return (hm.kc + 1) % 3;
 }
 
-// Grid sizes from the last known map lengths: small maps launch few workgroups; a stale
-// estimate only changes the grid-stride loop count, never the result.
-unsigned grid_for(const tpgp_hip_batch *b, uint64_t tiles) {
-    uint64_t cap = (uint64_t)b->ctx->cus * 8;
-    if (tiles < 1) tiles = 1;
-    return (unsigned)(tiles < cap ? tiles : cap);
+// Tile kernels run one tile per workgroup: the grid covers the estimated tile count with a
+// margin (the kernels' grid-stride loops absorb an underestimate; surplus groups exit).
+unsigned tile_grid(uint64_t tiles) {
+    const uint64_t g = tiles + tiles / 16 + 64;
+    return (unsigned)(g < (1u << 22) ? g : (1u << 22));
 }
 unsigned partition_grid(const tpgp_hip_batch *b, const std::vector<int> &want) {
     uint64_t t = 0;
     for (uint32_t m = 0; m < b->M; ++m)
         if (want[m]) t += b->maps[m].est_len / kPartTile + 1;
-    return grid_for(b, t);
+    return tile_grid(t);
 }
 unsigned rotation_grid(tpgp_hip_batch *b, const tpgp_hip_rotation *req) {
     uint64_t t = 0;
     for (uint32_t m = 0; m < b->M; ++m)
         if (req[m].active) t += (3 * b->maps[m].est_len / 2) / kMergeTile + 16;
     b->est_tiles = t;
-    return grid_for(b, t);
+    return tile_grid(t);
 }
 
 hipEvent_t pooled_event(tpgp_hip_batch *b) {
@@ -1559,7 +1701,7 @@ struct ProfileScope {
     unsigned kind;
     hipEvent_t start = nullptr;
     ProfileScope(tpgp_hip_batch *bb, unsigned k) : b(bb), kind(k) {
-        if (!b->profiling) return;
+        if (!(b->profiling >> kind & 1u)) return;
         start = pooled_event(b);
         if (start) (void)hipEventRecord(start, b->ctx->stream);
     }
@@ -1841,9 +1983,7 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid) {
         }
         {
             ProfileScope pc(b, TPGP_HIP_PROFILE_SPLITS);
-            // one wave per tile, all in flight: up to 32 waves per CU
-            const uint64_t want = (b->est_tiles + 3) / 4, cap = (uint64_t)b->ctx->cus * 8;
-            const unsigned sgrid = (unsigned)(want < 1 ? 1 : (want < cap ? want : cap));
+            const unsigned sgrid = tile_grid(b->est_tiles / 32 + 1);  // 8 lanes per tile
             if (b->W == 1)
                 hipLaunchKernelGGL(k_rot_splits<1>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d);
             else
@@ -1858,7 +1998,10 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid) {
         }
         {
             ProfileScope psn(b, TPGP_HIP_PROFILE_SCAN);
-            hipLaunchKernelGGL(k_rot_scan, dim3(b->M), dim3(kBlock), 0, st, b->dev, d);
+            const uint64_t want = b->est_tiles / kChunkTiles + b->M + 1;
+            const unsigned cgrid = (unsigned)(want < (1u << 20) ? want : (1u << 20));
+            hipLaunchKernelGGL(k_rot_scan1, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d);
+            hipLaunchKernelGGL(k_rot_scan2, dim3(b->M), dim3(kBlock), 0, st, b->dev, d);
         }
         {
             ProfileScope pm(b, TPGP_HIP_PROFILE_MERGE);
@@ -2325,13 +2468,13 @@ extern "C" int tpgp_hip_batch_reserve(tpgp_hip_batch *b, size_t map, size_t capa
     return e == hipSuccess ? TPGP_HIP_OK : hip_fail(e, "HIP reserve", error, cap);
 }
 
-extern "C" int tpgp_hip_batch_profile(tpgp_hip_batch *b, int enable) {
+extern "C" int tpgp_hip_batch_profile(tpgp_hip_batch *b, unsigned mask) {
     if (!b) return TPGP_HIP_EINVAL;
     std::lock_guard<std::mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     collect_profile(b);
-    b->profiling = enable != 0;
+    b->profiling = mask & TPGP_HIP_PROFILE_ALL;
     for (int k = 0; k < 9; ++k) {
         b->prof_ms[k] = 0;
         b->prof_count[k] = 0;

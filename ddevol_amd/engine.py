@@ -208,8 +208,9 @@ class Evolution:
         self.eng.tpgp_evolution_counters_get(self.ptr, C.byref(c))
         return c
 
-    def profile(self, enable=True):
-        return self.eng.tpgp_evolution_profile(self.ptr, 1 if enable else 0)
+    def profile(self, mask=N.PROFILE_ALL):
+        """Enable HIP-event timing for the kinds in `mask` (bit k = kind k; 0 disables)."""
+        return self.eng.tpgp_evolution_profile(self.ptr, int(mask))
 
     def profile_read(self, kind):
         ms, n = C.c_double(), C.c_uint64()

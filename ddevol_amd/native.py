@@ -73,6 +73,7 @@ class EvolutionCounters(C.Structure):
 
 (PROFILE_MERGE, PROFILE_SCATTER, PROFILE_ROTATION, PROFILE_PARTITION, PROFILE_COUNT,
  PROFILE_SCAN, PROFILE_PLAN, PROFILE_HISTOGRAM, PROFILE_SPLITS) = range(9)
+PROFILE_ALL = 0x1FF
 PROFILE_NAMES = ["k_rot_write", "k_part_scatter", "rotation_pass", "partition_pass",
                  "k_rot_count", "k_rot_scan", "k_rot_plan", "k_part_count", "k_rot_splits"]
 
@@ -153,7 +154,7 @@ def hip():
     f("tpgp_hip_batch_reserve", C.c_int, C.c_void_p, sz, sz, p_c, sz)
     f("tpgp_hip_batch_info", C.c_int, C.c_void_p, sz, C.POINTER(sz), C.POINTER(sz))
     f("tpgp_hip_batch_stream", C.c_void_p, C.c_void_p)
-    f("tpgp_hip_batch_profile", C.c_int, C.c_void_p, C.c_int)
+    f("tpgp_hip_batch_profile", C.c_int, C.c_void_p, C.c_uint)
     f("tpgp_hip_batch_profile_read", C.c_int, C.c_void_p, C.c_uint, C.POINTER(C.c_double),
       C.POINTER(C.c_uint64))
     _hip = L
@@ -210,7 +211,7 @@ def engine():
     f("tpgp_evolution_run", C.c_int, C.c_void_p, sz, lp, pd, pd, psz, psz)
     f("tpgp_evolution_close", C.c_int, C.c_void_p, mp)
     f("tpgp_evolution_counters_get", None, C.c_void_p, C.POINTER(EvolutionCounters))
-    f("tpgp_evolution_profile", C.c_int, C.c_void_p, C.c_int)
+    f("tpgp_evolution_profile", C.c_int, C.c_void_p, C.c_uint)
     f("tpgp_evolution_profile_read", C.c_int, C.c_void_p, C.c_uint, pd, C.POINTER(C.c_uint64))
     f("tpgp_evolution_key_words", sz, C.c_void_p)
     _eng = L

@@ -131,7 +131,7 @@ tpgp_status tpgp_evolution_close(tpgp_evolution *evolution, tpgp_map *map);
 void tpgp_evolution_counters_get(const tpgp_evolution *evolution,
                                  tpgp_evolution_counters *counters);
 /* HIP-event timing of the device kernels (kinds as in tpgp_hip.h TPGP_HIP_PROFILE_*). */
-int tpgp_evolution_profile(tpgp_evolution *evolution, int enable);
+int tpgp_evolution_profile(tpgp_evolution *evolution, unsigned mask);
 int tpgp_evolution_profile_read(const tpgp_evolution *evolution, unsigned kind,
                                 double *milliseconds, uint64_t *launches);
 size_t tpgp_evolution_key_words(const tpgp_evolution *evolution);

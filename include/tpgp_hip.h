@@ -153,9 +153,10 @@ int tpgp_hip_batch_info(const tpgp_hip_batch *batch, size_t map, size_t *committ
                         size_t *capacity);
 /* Stream the batch enqueues on (a hipStream_t), for callers that time or capture it. */
 void *tpgp_hip_batch_stream(const tpgp_hip_batch *batch);
-/* Kernel timing with HIP events recorded on the batch stream around every launch of a kind:
- * the merge kernel, the partition scatter kernel, whole rotation passes, whole partition
- * passes.  profile(1) resets and enables; times accumulate at each sync. */
+/* Kernel timing with HIP events recorded on the batch stream around every launch of the kinds
+ * selected by `mask` (bit k = kind k; 0 disables, TPGP_HIP_PROFILE_ALL selects all).  The call
+ * resets the totals; times accumulate at each sync.  Events cost a little time per launch, so
+ * a throughput measurement selects only the kinds it reports. */
 enum {
     TPGP_HIP_PROFILE_MERGE = 0,     /* rotation write pass: merge and write the outputs */
     TPGP_HIP_PROFILE_SCATTER = 1,   /* partition scatter */
@@ -165,9 +166,10 @@ enum {
     TPGP_HIP_PROFILE_SCAN = 5,      /* rotation offsets, regions, ledger fold */
     TPGP_HIP_PROFILE_PLAN = 6,      /* rotation tile plan */
     TPGP_HIP_PROFILE_HISTOGRAM = 7, /* partition histogram */
-    TPGP_HIP_PROFILE_SPLITS = 8     /* rotation merge-path splits */
+    TPGP_HIP_PROFILE_SPLITS = 8,    /* rotation merge-path splits */
+    TPGP_HIP_PROFILE_ALL = 0x1ff
 };
-int tpgp_hip_batch_profile(tpgp_hip_batch *batch, int enable);
+int tpgp_hip_batch_profile(tpgp_hip_batch *batch, unsigned mask);
 int tpgp_hip_batch_profile_read(const tpgp_hip_batch *batch, unsigned kind,
                                 double *milliseconds, uint64_t *launches);
 

@@ -69,11 +69,12 @@ class RefOptions(C.Structure):  # engine/include/tpgp_rev.h:36-45
                 ("metal_context", C.c_void_p), ("cuda_context", C.c_void_p)]
 
 
-def build(ref: bool = False) -> None:
+def build(ref: bool = False, force: bool = False) -> None:
     """Compile the oracle (and the reference when its sources are present)."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    extra = ["-B"] if force else []
+    subprocess.run(["make", "-s", *extra, "-C", HERE], check=True)
     if ref:
-        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        subprocess.run(["make", "-s", *extra, "-C", HERE, "ref"], check=True)
 
 
 def terms_array(terms):
