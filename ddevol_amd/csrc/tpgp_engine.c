@@ -551,6 +551,28 @@ static tpgp_status session_grow(session *S, size_t need) {
     S->cap = cap;
     return TPGP_OK;
 }
+/* Logs of dropped values and radii: device and host buffers hold `need` entries and more. */
+static tpgp_status grow_logs(tpgp_hip_batch *batch, tpgp_hip_log_entry **log, size_t *log_cap,
+                             size_t need) {
+    char err[512];
+    size_t cap = need + need / 2 + 4096;
+    int hs = tpgp_hip_batch_reserve_log(batch, cap, err, sizeof(err));
+    if (hs) {
+        note("log reserve", err);
+        return map_status(hs);
+    }
+    tpgp_hip_log_entry *fresh = malloc(cap * sizeof(*fresh));
+    if (!fresh) return TPGP_ENOMEM;
+    free(*log);
+    *log = fresh;
+    *log_cap = cap;
+    return TPGP_OK;
+}
+/* A step that reported capacity_needed: 1 = the map outgrew its slot, 2 = logs overflowed. */
+static tpgp_status session_regrow(session *S, int kind, size_t need) {
+    if (kind == 2) return grow_logs(S->batch, &S->log, &S->log_cap, need);
+    return session_grow(S, need + need / 2 + 1024);
+}
 static tpgp_status session_download(session *S, tpgp_map *m) {
     char err[512];
     size_t len = 0;
@@ -635,9 +657,14 @@ static tpgp_status sync_fold(session *S, const rot_plan *plans, size_t n, size_t
     int hs = tpgp_hip_batch_sync(S->batch, S->rec, S->max_rec, &nrec, readout, readout ? 2 : 0,
                                  &nrd, len, exact ? S->log : NULL, exact ? S->log_cap : 0, &nlog,
                                  err, sizeof(err));
+    if (hs == TPGP_HIP_ECAPACITY && exact) { /* logs too small: grow them and run again */
+        *capacity_needed = 2;
+        *need = nlog;
+        return TPGP_OK;
+    }
     if (hs) {
         note("sync", err);
-        return hs == TPGP_HIP_ECAPACITY ? TPGP_EBACKEND : map_status(hs);
+        return map_status(hs);
     }
     if (nrec != enq) {
         note("sync", "record count mismatch");
@@ -721,7 +748,7 @@ static tpgp_status rotate_sequence(tpgp_map *m, const rot_plan *plans, size_t n,
         char err[512];
         if (!s && grow) {
             tpgp_hip_batch_rollback(S.batch, err, sizeof(err));
-            if ((s = session_grow(&S, need + need / 2 + 1024))) break;
+            if ((s = session_regrow(&S, grow, need))) break;
             continue;
         }
         if (s) {
@@ -833,7 +860,10 @@ tpgp_status tpgp_evolution_run(tpgp_evolution *ev, size_t steps, tpgp_ledger *l,
         int grow = 0;
         size_t need = 0;
         uint64_t len = 0, up = 0, io = 0, ex = 0;
+        size_t read_len = 0; /* length of the state read out (the committed state) */
+        tpgp_hip_batch_info(S->batch, 0, &read_len, NULL);
         for (;;) {
+            for (size_t k = 0; k < 2 * 64; k++) rd[k] = NAN; /* filled only by a good sync */
             int hs = tpgp_hip_batch_readout(S->batch, ev->eo.pauli, NULL, err, sizeof(err));
             if (hs) {
                 note("readout", err);
@@ -847,13 +877,14 @@ tpgp_status tpgp_evolution_run(tpgp_evolution *ev, size_t steps, tpgp_ledger *l,
                           &grow, &need, &up, &io, &ex);
             if (!s && grow) {
                 tpgp_hip_batch_rollback(S->batch, err, sizeof(err));
-                if ((s = session_grow(S, need + need / 2 + 1024))) break;
+                if ((s = session_regrow(S, grow, need))) break;
                 continue;
             }
             break;
         }
-        /* readout of the state before this step (it was taken before the rotations ran) */
-        if (!s || !more) {
+        /* readout of the state before this step (taken before the rotations ran; it is a
+         * valid sample even when the step itself fails, as in the reference's step loop) */
+        if (!isnan(rd[0])) {
             double mid = 0, rad = NAN;
             tpgp_status rs = TPGP_OK;
             if (tpgp_has_mpfr())
@@ -863,7 +894,7 @@ tpgp_status tpgp_evolution_run(tpgp_evolution *ev, size_t steps, tpgp_ledger *l,
             if (rs && !s) s = rs;
             if (fid) fid[step] = mid;
             if (fid_radius) fid_radius[step] = rad;
-            if (active) active[step] = (size_t)len;
+            if (active) active[step] = read_len;
         }
         if (s) {
             tpgp_hip_batch_rollback(S->batch, err, sizeof(err));
@@ -924,6 +955,488 @@ tpgp_status tpgp_hip_evolve(tpgp_map *m, const tpgp_term *t, size_t n, double dt
     s = tpgp_evolution_run(ev, steps, l, fid, fid_radius, active, accepted);
     tpgp_status cs = tpgp_evolution_close(ev, m);
     return s ? s : cs;
+}
+
+/* ---- fleet: many evolutions in one resident batch ----------------------------------------- */
+typedef struct {
+    tpgp_slot_state state;
+    tpgp_status status;
+    size_t n_spins, steps, done;
+    rot_plan *plans;
+    size_t nplans;
+    size_t *enq;      /* indices of the plans enqueued in the current step */
+    size_t nenq;
+    tpgp_options o;
+    tpgp_evolve_options eo;
+    tpgp_ledger ledger;
+    double *fid, *rad;
+    size_t *act;
+    size_t cap;
+} fleet_slot;
+
+struct tpgp_fleet {
+    tpgp_hip_context *ctx;
+    int own_ctx;
+    tpgp_hip_batch *batch;
+    size_t W, M;
+    fleet_slot *slot;
+    tpgp_hip_record *rec;
+    size_t max_rec;
+    double *rd;
+    uint64_t *len;
+    tpgp_hip_log_entry *log;
+    size_t log_cap;
+    double *scratch;
+    size_t scratch_cap;
+    tpgp_evolution_counters c;
+};
+
+static void slot_clear(fleet_slot *q) {
+    free(q->plans);
+    free(q->enq);
+    free(q->fid);
+    free(q->rad);
+    free(q->act);
+    size_t cap = q->cap;
+    memset(q, 0, sizeof(*q));
+    q->cap = cap;
+}
+
+void tpgp_fleet_close(tpgp_fleet *f) {
+    if (!f) return;
+    for (size_t m = 0; m < f->M; m++) slot_clear(&f->slot[m]);
+    tpgp_hip_batch_destroy(f->batch);
+    if (f->own_ctx) tpgp_hip_destroy(f->ctx);
+    free(f->slot);
+    free(f->rec);
+    free(f->rd);
+    free(f->len);
+    free(f->log);
+    free(f->scratch);
+    free(f);
+}
+
+tpgp_status tpgp_fleet_open(tpgp_fleet **out, void *context, size_t n_words, size_t n_slots,
+                            size_t slot_capacity) {
+    if (!out) return TPGP_EINVAL;
+    *out = NULL;
+    if ((n_words != 1 && n_words != 2) || !n_slots || n_slots > (1u << 16) || !slot_capacity)
+        return TPGP_EINVAL;
+    tpgp_fleet *f = calloc(1, sizeof(*f));
+    if (!f) return TPGP_ENOMEM;
+    char err[512];
+    f->ctx = context;
+    if (!f->ctx) {
+        f->ctx = tpgp_hip_create(0, err, sizeof(err));
+        if (!f->ctx) {
+            note("context", err);
+            free(f);
+            return TPGP_EBACKEND;
+        }
+        f->own_ctx = 1;
+    }
+    f->W = n_words;
+    f->M = n_slots;
+    f->slot = calloc(n_slots, sizeof(*f->slot));
+    size_t *spins = malloc(n_slots * sizeof(size_t));
+    size_t *caps = malloc(n_slots * sizeof(size_t));
+    if (!f->slot || !spins || !caps) {
+        free(spins);
+        free(caps);
+        tpgp_fleet_close(f);
+        return TPGP_ENOMEM;
+    }
+    for (size_t m = 0; m < n_slots; m++) {
+        spins[m] = n_words == 1 ? 2 : 33; /* placeholder until a job is admitted */
+        caps[m] = slot_capacity;
+        f->slot[m].cap = slot_capacity;
+    }
+    f->batch = tpgp_hip_batch_create(f->ctx, n_words, n_slots, spins, caps, err, sizeof(err));
+    free(spins);
+    free(caps);
+    if (!f->batch) {
+        note("batch", err);
+        tpgp_fleet_close(f);
+        return TPGP_ENOMEM;
+    }
+    f->max_rec = n_slots <= 16 ? 4096 : (65536 / n_slots > 1024 ? 65536 / n_slots : 1024);
+    f->rec = malloc(n_slots * f->max_rec * sizeof(*f->rec));
+    f->rd = malloc(n_slots * 64 * sizeof(double));
+    f->len = malloc(n_slots * sizeof(uint64_t));
+    if (!f->rec || !f->rd || !f->len) {
+        tpgp_fleet_close(f);
+        return TPGP_ENOMEM;
+    }
+    *out = f;
+    return TPGP_OK;
+}
+
+tpgp_status tpgp_fleet_admit(tpgp_fleet *f, size_t m, const tpgp_fid_job *job) {
+    if (!f || m >= f->M || !job || !job->initial) return TPGP_EINVAL;
+    fleet_slot *q = &f->slot[m];
+    const tpgp_map *init = job->initial;
+    const tpgp_options *o = &job->options;
+    const tpgp_evolve_options *eo = &job->evolve;
+    if (q->state != TPGP_SLOT_FREE) return TPGP_EINVAL;
+    if (!map_ok(init) || init->n_words != f->W || (!job->terms && job->count) ||
+        !isfinite(job->dt) || eo->pauli < 1 || eo->pauli > 3 || !eo->normalizer ||
+        2 * job->count > f->max_rec)
+        return TPGP_EINVAL;
+    if (o->backend != TPGP_HIP) return TPGP_EBACKEND;
+    double half = job->dt * 0.5;
+    if (half * 2 != job->dt) return TPGP_ENUMERIC; /* tpgp_rev.c:780-784 */
+    slot_clear(q);
+    q->nplans = 2 * job->count;
+    q->plans = malloc((q->nplans + 1) * sizeof(*q->plans));
+    q->enq = malloc((q->nplans + 1) * sizeof(*q->enq));
+    q->fid = malloc((job->steps + 1) * sizeof(double));
+    q->rad = malloc((job->steps + 1) * sizeof(double));
+    q->act = malloc((job->steps + 1) * sizeof(size_t));
+    if (!q->plans || !q->enq || !q->fid || !q->rad || !q->act) {
+        slot_clear(q);
+        return TPGP_ENOMEM;
+    }
+    for (size_t k = 0; k < q->nplans; k++) {
+        const tpgp_term *term = k < job->count ? job->terms + k : job->terms + (q->nplans - 1 - k);
+        plan_rotation(q->plans + k, init->n_spins, term->i, term->j, term->qi, term->qj, term->h,
+                      half, o);
+    }
+    if (init->len > q->cap) {
+        char err[512];
+        size_t cap = init->len * 2 + 1024;
+        int hs = tpgp_hip_batch_reserve(f->batch, m, cap, err, sizeof(err));
+        if (hs) {
+            note("reserve", err);
+            slot_clear(q);
+            return map_status(hs);
+        }
+        q->cap = cap;
+    }
+    char err[512];
+    int hs = tpgp_hip_batch_assign(f->batch, m, init->n_spins, init->keys, init->values,
+                                   init->len, err, sizeof(err));
+    if (hs) {
+        note("assign", err);
+        slot_clear(q);
+        return map_status(hs);
+    }
+    q->n_spins = init->n_spins;
+    q->steps = job->steps;
+    q->o = *o;
+    q->o.hip_context = f->ctx;
+    q->eo = *eo;
+    tpgp_ledger_init(&q->ledger, o->validated);
+    q->state = TPGP_SLOT_RUNNING;
+    return TPGP_OK;
+}
+
+/* Fold one slot's records of this step (plans in order, as sync_fold does for one map). */
+static tpgp_status fleet_fold(tpgp_fleet *f, size_t m, const tpgp_hip_log_entry *log,
+                              size_t nlog, tpgp_ledger *trial, int *grow, size_t *need,
+                              uint64_t *up, uint64_t *io, uint64_t *ex) {
+    fleet_slot *q = &f->slot[m];
+    const int exact = q->eo.exact_ledger;
+    size_t r_at = 0, e_at = 0;
+    *grow = 0;
+    for (size_t k = 0; k < q->nplans; k++) {
+        const rot_plan *p = q->plans + k;
+        if (!ledger_ok(trial)) return TPGP_EINVAL;
+        if (p->pre) return p->pre;
+        if (p->noop) continue;
+        const tpgp_hip_record *r = f->rec + m * f->max_rec + r_at++;
+        size_t e0 = e_at;
+        while (e_at < nlog && log[e_at].rotation == r->rotation) e_at++;
+        if (!r->executed) continue;
+        int fs = fold_record(r, log + e0, e_at - e0, exact, &q->o, trial, f->scratch);
+        if (fs == FOLD_CAPACITY) {
+            *grow = 1;
+            *need = (size_t)r->len_out;
+            return TPGP_OK;
+        }
+        if (fs) return (tpgp_status)fs;
+        *up += r->len_in;
+        *io += r->len_in + r->len_out;
+        (*ex)++;
+    }
+    return TPGP_OK;
+}
+
+static int cmp_entry_map(const void *a, const void *b) {
+    const tpgp_hip_log_entry *x = a, *y = b;
+    if (x->map != y->map) return x->map < y->map ? -1 : 1;
+    return cmp_entry(a, b);
+}
+
+tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
+    if (!f) return TPGP_EINVAL;
+    char err[512];
+    const size_t M = f->M;
+    int *part = calloc(M, sizeof(int));   /* slots taking part in this round */
+    int *mask = calloc(M, sizeof(int));
+    int *pmask = calloc(M, sizeof(int));
+    if (!part || !mask || !pmask) {
+        free(part);
+        free(mask);
+        free(pmask);
+        return TPGP_ENOMEM;
+    }
+    tpgp_status s = TPGP_OK;
+    int any = 0;
+    for (size_t m = 0; m < M; m++) any |= part[m] = f->slot[m].state == TPGP_SLOT_RUNNING;
+    while (any && !s) {
+        /* readouts of the committed state, one call per readout symbol */
+        for (unsigned pauli = 1; pauli <= 3 && !s; pauli++) {
+            int need_call = 0;
+            for (size_t m = 0; m < M; m++) {
+                pmask[m] = part[m] && f->slot[m].eo.pauli == pauli;
+                need_call |= pmask[m];
+            }
+            if (need_call && tpgp_hip_batch_readout(f->batch, pauli, pmask, err, sizeof(err))) {
+                note("readout", err);
+                s = TPGP_EBACKEND;
+            }
+        }
+        if (s) break;
+        /* this step's rotations: slot m's r-th enqueued rotation is request r */
+        size_t R = 0;
+        int exact_any = 0;
+        for (size_t m = 0; m < M; m++) {
+            fleet_slot *q = &f->slot[m];
+            q->nenq = 0;
+            if (!part[m] || q->done >= q->steps) continue;
+            for (size_t k = 0; k < q->nplans; k++) {
+                if (q->plans[k].pre) break;
+                if (!q->plans[k].noop) q->enq[q->nenq++] = k;
+            }
+            if (q->nenq > R) R = q->nenq;
+            exact_any |= q->eo.exact_ledger;
+        }
+        if (R) {
+            tpgp_hip_rotation *req = calloc(R * M, sizeof(*req));
+            uint32_t *seq = malloc(R * sizeof(*seq));
+            if (!req || !seq) {
+                free(req);
+                free(seq);
+                s = TPGP_ENOMEM;
+                break;
+            }
+            for (size_t r = 0; r < R; r++) seq[r] = (uint32_t)r;
+            for (size_t m = 0; m < M; m++) {
+                fleet_slot *q = &f->slot[m];
+                for (size_t r = 0; r < q->nenq; r++) {
+                    const rot_plan *p = q->plans + q->enq[r];
+                    tpgp_hip_rotation *t = req + r * M + m;
+                    t->active = 1;
+                    t->i = p->i;
+                    t->j = p->j;
+                    t->qi = p->qi;
+                    t->qj = p->qj;
+                    t->c = p->c;
+                    t->s = p->s;
+                    t->rc = p->rc;
+                    t->rs = p->rs;
+                    t->threshold = q->o.threshold;
+                    t->validated = q->o.validated;
+                    t->log = q->eo.exact_ledger;
+                }
+            }
+            int hs = tpgp_hip_batch_rotate_many(f->batch, req, seq, R, err, sizeof(err));
+            free(req);
+            free(seq);
+            if (hs) {
+                note("rotate", err);
+                s = map_status(hs);
+                break;
+            }
+        }
+        if (exact_any && !f->log) {
+            f->log_cap = 1u << 20;
+            f->log = malloc(f->log_cap * sizeof(*f->log));
+            if (!f->log) {
+                s = TPGP_ENOMEM;
+                break;
+            }
+        }
+        size_t nrec = 0, nrd = 0, nlog = 0;
+        int hs = tpgp_hip_batch_sync(f->batch, f->rec, f->max_rec, &nrec, f->rd, 1, &nrd, f->len,
+                                     exact_any ? f->log : NULL, exact_any ? f->log_cap : 0, &nlog,
+                                     err, sizeof(err));
+        if (hs == TPGP_HIP_ECAPACITY && exact_any) {
+            /* logs too small: discard this round, grow them, run the same slots again */
+            tpgp_hip_batch_rollback(f->batch, err, sizeof(err));
+            if ((s = grow_logs(f->batch, &f->log, &f->log_cap, nlog))) break;
+            continue;
+        }
+        if (hs) {
+            note("sync", err);
+            s = hs == TPGP_HIP_ECAPACITY ? TPGP_EBACKEND : map_status(hs);
+            break;
+        }
+        if (nlog) qsort(f->log, nlog, sizeof(*f->log), cmp_entry_map);
+        if (f->scratch_cap < nlog + 1) {
+            free(f->scratch);
+            f->scratch_cap = nlog + 1;
+            f->scratch = malloc(f->scratch_cap * sizeof(double));
+            if (!f->scratch) {
+                s = TPGP_ENOMEM;
+                break;
+            }
+        }
+        /* per slot: readout, fold, decision */
+        size_t l_at = 0;
+        int regrow = 0;
+        for (size_t m = 0; m < M; m++) {
+            mask[m] = 0;
+            size_t l0 = l_at;
+            while (l_at < nlog && f->log[l_at].map == m) l_at++;
+            if (!part[m]) continue;
+            part[m] = 0;
+            fleet_slot *q = &f->slot[m];
+            double mid = 0, rad = NAN;
+            tpgp_status rs = TPGP_OK;
+            const double *rd = f->rd + m * 64;
+            if (tpgp_has_mpfr())
+                rs = readout_values(rd, q->n_spins, q->eo.normalizer, &mid, &rad);
+            else
+                mid = neumaier(rd, q->n_spins) / (double)q->eo.normalizer;
+            size_t read_len = 0; /* the state read out is the committed one */
+            tpgp_hip_batch_info(f->batch, m, &read_len, NULL);
+            if (q->done >= q->steps) { /* final readout only */
+                q->fid[q->done] = mid;
+                q->rad[q->done] = rad;
+                q->act[q->done] = read_len;
+                q->status = rs;
+                q->state = rs ? TPGP_SLOT_FAILED : TPGP_SLOT_DONE;
+                continue;
+            }
+            tpgp_ledger trial = q->ledger;
+            int grow = 0;
+            size_t need = 0;
+            uint64_t up = 0, io = 0, ex = 0;
+            tpgp_status fs = fleet_fold(f, m, f->log + l0, l_at - l0, &trial, &grow, &need, &up,
+                                        &io, &ex);
+            if (!fs && grow) {
+                size_t cap = need + need / 2 + 1024;
+                if (cap < 2 * q->cap) cap = 2 * q->cap;
+                q->cap = cap;
+                part[m] = 1; /* rerun this slot's step after growing */
+                regrow = 1;
+                continue;
+            }
+            q->fid[q->done] = mid;
+            q->rad[q->done] = rad;
+            q->act[q->done] = read_len;
+            if (!fs && rs) fs = rs;
+            if (fs) {
+                q->status = fs;
+                q->state = TPGP_SLOT_FAILED;
+                continue;
+            }
+            mask[m] = 1;
+            q->ledger = trial;
+            q->done++;
+            f->c.steps++;
+            f->c.rotations += ex;
+            f->c.updates += up;
+            f->c.in_out += io;
+        }
+        if (tpgp_hip_batch_commit(f->batch, mask, err, sizeof(err))) {
+            note("commit", err);
+            s = TPGP_EBACKEND;
+            break;
+        }
+        any = 0;
+        if (regrow) {
+            for (size_t m = 0; m < M && !s; m++) {
+                if (!part[m]) continue;
+                int rh = tpgp_hip_batch_reserve(f->batch, m, f->slot[m].cap, err, sizeof(err));
+                if (rh) {
+                    note("reserve", err);
+                    f->slot[m].status = map_status(rh);
+                    f->slot[m].state = TPGP_SLOT_FAILED;
+                    part[m] = 0;
+                }
+                any |= part[m];
+            }
+        }
+    }
+    if (running) {
+        size_t n = 0;
+        for (size_t m = 0; m < M; m++) n += f->slot[m].state == TPGP_SLOT_RUNNING;
+        *running = n;
+    }
+    if (s) {
+        tpgp_hip_batch_rollback(f->batch, err, sizeof(err));
+    }
+    free(part);
+    free(mask);
+    free(pmask);
+    return s;
+}
+
+tpgp_status tpgp_fleet_slot(const tpgp_fleet *f, size_t m, tpgp_slot_state *state,
+                            size_t *steps_done, tpgp_status *status, tpgp_ledger *ledger) {
+    if (!f || m >= f->M) return TPGP_EINVAL;
+    const fleet_slot *q = &f->slot[m];
+    if (state) *state = q->state;
+    if (steps_done) *steps_done = q->done;
+    if (status) *status = q->status;
+    if (ledger) *ledger = q->ledger;
+    return TPGP_OK;
+}
+
+tpgp_status tpgp_fleet_fid(const tpgp_fleet *f, size_t m, double *fid, double *fid_radius,
+                           size_t *active, size_t capacity, size_t *n) {
+    if (!f || m >= f->M) return TPGP_EINVAL;
+    const fleet_slot *q = &f->slot[m];
+    size_t k = q->state == TPGP_SLOT_FREE ? 0 : q->done;
+    if (q->state == TPGP_SLOT_DONE || q->state == TPGP_SLOT_FAILED) k = q->done + 1;
+    if (k > capacity) k = capacity;
+    for (size_t t = 0; t < k; t++) {
+        if (fid) fid[t] = q->fid[t];
+        if (fid_radius) fid_radius[t] = q->rad[t];
+        if (active) active[t] = q->act[t];
+    }
+    if (n) *n = k;
+    return TPGP_OK;
+}
+
+tpgp_status tpgp_fleet_release(tpgp_fleet *f, size_t m, tpgp_map *final_map) {
+    if (!f || m >= f->M) return TPGP_EINVAL;
+    fleet_slot *q = &f->slot[m];
+    tpgp_status s = TPGP_OK;
+    if (final_map && q->state != TPGP_SLOT_FREE) {
+        char err[512];
+        size_t len = 0;
+        tpgp_map t;
+        int hs = tpgp_hip_batch_download(f->batch, m, NULL, NULL, 0, &len, err, sizeof(err));
+        if (hs) return map_status(hs);
+        if ((s = tpgp_map_init(&t, q->n_spins))) return s;
+        if ((s = map_reserve(&t, len))) {
+            tpgp_map_free(&t);
+            return s;
+        }
+        hs = tpgp_hip_batch_download(f->batch, m, t.keys, t.values, len, &t.len, err,
+                                     sizeof(err));
+        if (hs) {
+            note("download", err);
+            tpgp_map_free(&t);
+            return map_status(hs);
+        }
+        tpgp_map_free(final_map);
+        *final_map = t;
+    }
+    slot_clear(q);
+    return s;
+}
+
+void tpgp_fleet_counters_get(const tpgp_fleet *f, tpgp_evolution_counters *c) {
+    if (f && c) *c = f->c;
+}
+
+uint64_t tpgp_fleet_slot_bytes(const tpgp_fleet *f, size_t m) {
+    if (!f || m >= f->M) return 0;
+    return 3ull * f->slot[m].cap * (8 * f->W + 8);
 }
 
 /* ---- Trotter bounds and allowance helpers (tpgp_rev.c:858-961; residual_rev.c:195-201) ----- */

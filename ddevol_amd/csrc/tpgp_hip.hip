@@ -30,6 +30,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/tpgp_hip.h"
@@ -263,7 +264,10 @@ struct PassMap {          // per map, per pass (staged from the host)
     uint32_t rec_slot;    // record index (rotation) or readout index
     uint32_t rot_seq;
     double c, s, rc, rs, threshold;
-    uint32_t pos[64];     // readout: digit position of every site
+};
+struct ReadoutMap {       // per map, per readout (staged from the host)
+    uint32_t active, in_buf, rec_slot, pad;
+    uint32_t pos[64];     // digit position of every site
 };
 struct TileLedger {
     uint64_t dropped, radii;
@@ -1181,9 +1185,9 @@ __global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap 
 
 // One wave per map: binary search of the n single-site keys (symbol `pauli` at site s).
 template <int W>
-__global__ void k_readout(BatchDev b, const PassMap *pm, unsigned pauli) {
+__global__ void k_readout(BatchDev b, const ReadoutMap *pm, unsigned pauli) {
     const uint32_t m = blockIdx.x;
-    const PassMap &P = pm[m];
+    const ReadoutMap &P = pm[m];
     if (!P.active) return;
     const uint32_t n = b.slot[m].n_spins, s = threadIdx.x;
     if (s >= 64) return;
@@ -1521,10 +1525,16 @@ struct tpgp_hip_batch {
     uint64_t max_tiles = 0;
     BatchDev dev{};
     MapSlot *d_slot = nullptr;
-    // staging ring for per-pass descriptors
-    PassMap *h_stage = nullptr;
-    PassMap *d_stage = nullptr;
-    size_t stage_cap = 0, stage_used = 0;
+    // staging ring for per-pass descriptors (pinned host bytes mirrored on the device); a
+    // region is reused only after the event recorded behind its upload has completed
+    uint8_t *h_stage = nullptr;
+    uint8_t *d_stage = nullptr;
+    size_t stage_cap = 0, stage_used = 0;  // bytes
+    struct StageRegion {
+        size_t begin, end;
+        hipEvent_t ev;
+    };
+    std::vector<StageRegion> inflight;
     unsigned grid = 1024;
     uint32_t max_rec = 0, max_rd = 0;
     std::vector<LoggedLayout> logged;
@@ -1532,6 +1542,7 @@ struct tpgp_hip_batch {
     bool poisoned = false;
     bool validate = false;  // TPGP_HIP_VALIDATE=1: check order and regions after every pass
     uint64_t est_tiles = 1;  // merge tiles of the last planned rotation (grid sizing only)
+    uint64_t log_entries = 1u << 20;  // device log capacity (grown by reserve_log)
     uint32_t pass_count = 0;
     // kernel timing with HIP events on the batch stream (kinds: TPGP_HIP_PROFILE_*)
     unsigned profiling = 0;  // bit k: bracket launches of profile kind k with events
@@ -1562,6 +1573,9 @@ template <typename T> hipError_t dalloc(tpgp_hip_batch *b, T **p, size_t count) 
 void collect_profile(tpgp_hip_batch *b);
 void free_all(tpgp_hip_batch *b) {
     collect_profile(b);
+    for (auto &R : b->inflight) b->event_pool.push_back(R.ev);
+    b->inflight.clear();
+    b->stage_used = 0;
     for (hipEvent_t e : b->event_pool) (void)hipEventDestroy(e);
     b->event_pool.clear();
     for (void *p : b->allocations) (void)hipFree(p);
@@ -1611,13 +1625,13 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     ALLOC(d.overflow, M);
     ALLOC(d.rec, (size_t)M * b->max_rec);
     ALLOC(d.rd, (size_t)M * b->max_rd * 64);
-    d.log_cap = 1u << 20;
+    d.log_cap = b->log_entries;
     ALLOC(d.log, d.log_cap);
     ALLOC(d.log_count, 1);
     ALLOC(d.debug, 16);
     ALLOC(b->d_stage, b->stage_cap);
 #undef ALLOC
-    e = hipHostMalloc((void **)&b->h_stage, b->stage_cap * sizeof(PassMap), hipHostMallocDefault);
+    e = hipHostMalloc((void **)&b->h_stage, b->stage_cap, hipHostMallocDefault);
     if (e != hipSuccess) {
         b->h_stage = nullptr;
         hip_fail(e, "HIP pinned staging allocation", error, cap);
@@ -1643,20 +1657,47 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     return TPGP_HIP_OK;
 }
 
-// Reserve n descriptors in the staging ring; synchronises when the ring is full.
-PassMap *stage(tpgp_hip_batch *b, size_t n, PassMap **dev) {
-    if (b->stage_used + n > b->stage_cap) {
-        (void)hipStreamSynchronize(b->ctx->stream);
-        b->stage_used = 0;
+// Reserve room for n descriptors of type T in the staging ring (zeroed).  Waits only for the
+// uploads of earlier regions that the new one overlaps.
+void release_stage(tpgp_hip_batch *b);
+template <class T> T *stage(tpgp_hip_batch *b, size_t n, T **dev) {
+    const size_t bytes = (n * sizeof(T) + 255) & ~(size_t)255;
+    if (b->stage_used + bytes > b->stage_cap) b->stage_used = 0;
+    const size_t lo = b->stage_used, hi = lo + bytes;
+    size_t keep = 0;
+    for (size_t r = 0; r < b->inflight.size(); ++r) {
+        auto &R = b->inflight[r];
+        if (R.begin < hi && lo < R.end) {
+            (void)hipEventSynchronize(R.ev);
+            b->event_pool.push_back(R.ev);
+        } else {
+            b->inflight[keep++] = R;
+        }
     }
-    PassMap *h = b->h_stage + b->stage_used;
-    *dev = b->d_stage + b->stage_used;
-    std::memset((void *)h, 0, n * sizeof(PassMap));
-    b->stage_used += n;
+    b->inflight.resize(keep);
+    T *h = reinterpret_cast<T *>(b->h_stage + lo);
+    *dev = reinterpret_cast<T *>(b->d_stage + lo);
+    std::memset((void *)h, 0, bytes);
+    b->stage_used = hi;
     return h;
 }
-hipError_t push_stage(tpgp_hip_batch *b, PassMap *h, PassMap *d, size_t n) {
-    return hipMemcpyAsync(d, h, n * sizeof(PassMap), hipMemcpyHostToDevice, b->ctx->stream);
+hipEvent_t pooled_event(tpgp_hip_batch *b);
+template <class T> hipError_t push_stage(tpgp_hip_batch *b, T *h, T *d, size_t n) {
+    const size_t bytes = n * sizeof(T);
+    hipError_t e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, b->ctx->stream);
+    if (e != hipSuccess) return e;
+    hipEvent_t ev = pooled_event(b);
+    if (!ev) return hipStreamSynchronize(b->ctx->stream);  // no event: finish the upload now
+    const size_t lo = (size_t)(reinterpret_cast<uint8_t *>(h) - b->h_stage);
+    e = hipEventRecord(ev, b->ctx->stream);
+    b->inflight.push_back({lo, lo + ((bytes + 255) & ~(size_t)255), ev});
+    return e;
+}
+// After the stream has drained: every staged region is free again.
+void release_stage(tpgp_hip_batch *b) {
+    for (auto &R : b->inflight) b->event_pool.push_back(R.ev);
+    b->inflight.clear();
+    b->stage_used = 0;
 }
 
 int next_buffer(const HostMap &hm) {
@@ -1858,7 +1899,7 @@ extern "C" tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *ctx, size_t n
     b->total_cap = off;
     b->max_rec = n_maps <= 16 ? 4096u : (uint32_t)std::max<size_t>(1024, 65536 / n_maps);
     b->max_rd = n_maps <= 16 ? 1024u : 64u;
-    b->stage_cap = std::max<size_t>(n_maps * 64, 1u << 16);
+    b->stage_cap = std::max<size_t>(n_maps * 256, 1u << 16) * sizeof(PassMap);
     b->grid = (unsigned)ctx->cus * 8;
     const char *v = getenv("TPGP_HIP_VALIDATE");
     b->validate = v && *v && *v != '0';
@@ -1941,6 +1982,31 @@ extern "C" int tpgp_hip_batch_upload(tpgp_hip_batch *b, size_t map, const uint64
     hm.est_len = len;
     hm.nrec = hm.nrd = 0;
     return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_assign(tpgp_hip_batch *b, size_t map, size_t n_spins,
+                                     const uint64_t *keys, const double *values, size_t len,
+                                     char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!b || map >= b->M || n_spins < 2 || n_spins > 32u * (unsigned)b->W ||
+        (b->W == 2 && n_spins <= 32)) {
+        msg(error, cap, "Invalid assign arguments (map %zu, %zu spins).", map, n_spins);
+        return TPGP_HIP_EINVAL;
+    }
+    HostMap &hm = b->maps[map];
+    if (hm.n_spins != n_spins) {
+        std::lock_guard<std::mutex> lock(b->ctx->mutex);
+        DeviceGuard guard(b->ctx->device);
+        hipError_t e = hipStreamSynchronize(b->ctx->stream);
+        hm.n_spins = n_spins;
+        hm.sig.assign(n_spins, 0);
+        hm.sig_c.assign(n_spins, 0);
+        const MapSlot slot{hm.off, hm.cap, (uint32_t)n_spins, 0};
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_slot + map, &slot, sizeof(slot), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "HIP assign", error, cap);
+    }
+    return tpgp_hip_batch_upload(b, map, keys, values, len, error, cap);
 }
 
 namespace {
@@ -2031,10 +2097,6 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
         return TPGP_HIP_EINVAL;
     }
     const uint32_t M = b->M;
-    if (2 * n_rot * M > b->stage_cap) {
-        msg(error, cap, "Too many rotations in one call (%zu x %u maps).", n_rot, M);
-        return TPGP_HIP_EINVAL;
-    }
     std::vector<uint32_t> count(M, 0);
     for (size_t r = 0; r < n_rot; ++r)
         for (uint32_t m = 0; m < M; ++m) {
@@ -2054,19 +2116,25 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
         }
     std::lock_guard<std::mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
-    PassMap *d0;
-    PassMap *h0 = stage(b, 2 * n_rot * M, &d0);
+    // rotations go out in chunks whose descriptors fill at most a quarter of the staging ring,
+    // so the host stages one chunk while the device runs the previous ones
+    const size_t per_rot = 2 * (size_t)M * sizeof(PassMap);
+    const size_t chunk = std::max<size_t>(1, b->stage_cap / 4 / per_rot);
+    std::vector<int> want(M);
+    std::vector<uint32_t> pa(M), pb(M);
     struct Launch {
         bool rotation;
         size_t at;
         unsigned grid;
     };
     std::vector<Launch> launches;
-    launches.reserve(2 * n_rot);
+    for (size_t r0 = 0; r0 < n_rot; r0 += chunk) {
+    const size_t r1 = std::min(n_rot, r0 + chunk);
+    PassMap *d0;
+    PassMap *h0 = stage(b, 2 * (r1 - r0) * M, &d0);
+    launches.clear();
     size_t used = 0;
-    std::vector<int> want(M);
-    std::vector<uint32_t> pa(M), pb(M);
-    for (size_t r = 0; r < n_rot; ++r) {
+    for (size_t r = r0; r < r1; ++r) {
         const tpgp_hip_rotation *q = req + r * M;
         // bring each active map's pair (i, j) to the top two digits
         bool any = false;
@@ -2099,6 +2167,7 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
     }
     if ((e = hipGetLastError()) != hipSuccess)
         return hip_fail(e, "HIP rotation launch", error, cap);
+    }
     return TPGP_HIP_OK;
 }
 
@@ -2121,8 +2190,8 @@ extern "C" int tpgp_hip_batch_readout(tpgp_hip_batch *b, unsigned pauli, const i
         }
     std::lock_guard<std::mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
-    PassMap *d;
-    PassMap *h = stage(b, b->M, &d);
+    ReadoutMap *d;
+    ReadoutMap *h = stage(b, b->M, &d);
     for (uint32_t m = 0; m < b->M; ++m) {
         if (active && !active[m]) continue;
         HostMap &hm = b->maps[m];
@@ -2135,10 +2204,10 @@ extern "C" int tpgp_hip_batch_readout(tpgp_hip_batch *b, unsigned pauli, const i
     if (e != hipSuccess) return hip_fail(e, "HIP readout staging", error, cap);
     if (b->W == 1)
         hipLaunchKernelGGL(k_readout<1>, dim3(b->M), dim3(64), 0, b->ctx->stream, b->dev,
-                           (const PassMap *)d, pauli);
+                           (const ReadoutMap *)d, pauli);
     else
         hipLaunchKernelGGL(k_readout<2>, dim3(b->M), dim3(64), 0, b->ctx->stream, b->dev,
-                           (const PassMap *)d, pauli);
+                           (const ReadoutMap *)d, pauli);
     if ((e = hipGetLastError()) != hipSuccess)
         return hip_fail(e, "HIP readout launch", error, cap);
     return TPGP_HIP_OK;
@@ -2231,26 +2300,24 @@ extern "C" int tpgp_hip_batch_sync(tpgp_hip_batch *b, tpgp_hip_record *records,
     int status = TPGP_HIP_OK;
     if (n_log) *n_log = 0;
     if (nlog) {
-        if (nlog > b->dev.log_cap) {
+        if (nlog > b->dev.log_cap || (log && nlog > log_capacity)) {
+            // the caller rolls back, grows the logs (tpgp_hip_batch_reserve_log) and retries
             msg(error, cap, "Ledger log overflow (%llu entries).", (unsigned long long)nlog);
+            if (n_log) *n_log = nlog;
             status = TPGP_HIP_ECAPACITY;
         } else if (log) {
             std::vector<LogDev> tmp(nlog);
             e = hipMemcpy(tmp.data(), b->dev.log, nlog * sizeof(LogDev), hipMemcpyDeviceToHost);
             if (e != hipSuccess) return hip_fail(e, "HIP log transfer", error, cap);
+            std::unordered_map<uint64_t, const std::vector<uint8_t> *> layout;
+            for (const LoggedLayout &ll : b->logged)
+                layout[((uint64_t)ll.map << 32) | ll.rotation] = &ll.sigma;
             size_t out = 0;
             for (uint64_t t = 0; t < nlog; ++t) {
                 const LogDev &L = tmp[t];
-                const std::vector<uint8_t> *sig = nullptr;
-                for (const LoggedLayout &ll : b->logged)
-                    if (ll.map == L.map && ll.rotation == L.rotation) sig = &ll.sigma;
-                if (!sig) continue;
-                if (out >= log_capacity) {
-                    msg(error, cap, "Ledger log buffer too small (%llu entries).",
-                        (unsigned long long)nlog);
-                    status = TPGP_HIP_ECAPACITY;
-                    break;
-                }
+                auto it = layout.find(((uint64_t)L.map << 32) | L.rotation);
+                if (it == layout.end()) continue;
+                const std::vector<uint8_t> *sig = it->second;
                 tpgp_hip_log_entry &o = log[out++];
                 uint64_t stored[2] = {L.key[0], L.key[1]};
                 uint64_t sw[2] = {stored[0], b->W > 1 ? stored[1] : 0};
@@ -2269,7 +2336,7 @@ extern "C" int tpgp_hip_batch_sync(tpgp_hip_batch *b, tpgp_hip_record *records,
     }
     b->logged.clear();
     for (uint32_t m = 0; m < M; ++m) b->maps[m].nrec = b->maps[m].nrd = 0;
-    b->stage_used = 0;
+    release_stage(b);
     return status;
 }
 
@@ -2312,7 +2379,7 @@ extern "C" int tpgp_hip_batch_rollback(tpgp_hip_batch *b, char *error, size_t ca
         hm.nrec = hm.nrd = 0;
     }
     b->logged.clear();
-    b->stage_used = 0;
+    release_stage(b);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, b->ctx->stream, b->dev.overflow,
                        (uint64_t)b->M, 0u);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, b->ctx->stream, b->dev.log_count, 1ull,
@@ -2381,7 +2448,7 @@ extern "C" int tpgp_hip_batch_download(tpgp_hip_batch *b, size_t map, uint64_t *
         // the reordered copy becomes the committed state (same map, reference layout)
         hm.kc = hm.kcur;
         hm.sig_c = hm.sig;
-        b->stage_used = 0;
+        release_stage(b);
     }
     return TPGP_HIP_OK;
 }
@@ -2466,6 +2533,35 @@ extern "C" int tpgp_hip_batch_reserve(tpgp_hip_batch *b, size_t map, size_t capa
     }
     e = hipStreamSynchronize(st);
     return e == hipSuccess ? TPGP_HIP_OK : hip_fail(e, "HIP reserve", error, cap);
+}
+
+extern "C" int tpgp_hip_batch_reserve_log(tpgp_hip_batch *b, size_t entries, char *error,
+                                          size_t cap) {
+    msg(error, cap, "");
+    if (!b || entries > (1ull << 34)) {
+        msg(error, cap, "Invalid log reserve arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    if (entries <= b->dev.log_cap) return TPGP_HIP_OK;
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    hipError_t e = hipStreamSynchronize(b->ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "HIP log reserve", error, cap);
+    LogDev *fresh = nullptr;
+    if ((e = dalloc(b, &fresh, entries)) != hipSuccess) {
+        hip_fail(e, "HIP log reserve", error, cap);
+        return e == hipErrorOutOfMemory ? TPGP_HIP_ENOMEM : TPGP_HIP_EDEVICE;
+    }
+    // the old log is released with the batch's other allocations
+    auto it = std::find(b->allocations.begin(), b->allocations.end(), (void *)b->dev.log);
+    if (it != b->allocations.end()) {
+        (void)hipFree(*it);
+        b->allocations.erase(it);
+    }
+    b->dev.log = fresh;
+    b->dev.log_cap = entries;
+    b->log_entries = entries;
+    return TPGP_HIP_OK;
 }
 
 extern "C" int tpgp_hip_batch_profile(tpgp_hip_batch *b, unsigned mask) {

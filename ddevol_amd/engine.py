@@ -6,6 +6,7 @@ engine/include/tpgp_cuda_rev.h (context + branches) with the same argument meani
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -17,6 +18,7 @@ class Context:
 
     def __init__(self, device: int = 0):
         self.lib = N.hip()
+        self._children = weakref.WeakSet()  # sessions on this context, closed before it
         err = C.create_string_buffer(512)
         self.ptr = self.lib.tpgp_hip_create(device, err, 512)
         if not self.ptr:
@@ -28,6 +30,8 @@ class Context:
 
     def close(self):
         if self.ptr:
+            for child in list(self._children):
+                child.close()
             self.lib.tpgp_hip_destroy(self.ptr)
             self.ptr = None
 
@@ -179,11 +183,14 @@ class Evolution:
     """Resident evolution session (tpgp_evolution_open/run/close): the map stays in HBM."""
 
     def __init__(self, mp: Map, terms, dt, opts, exact_ledger=False, pauli=1, normalizer=None,
-                 initial_capacity=0):
+                 initial_capacity=0, context: Context | None = None):
         self.eng = N.engine()
         self.n_spins = mp.n_spins
         self._terms = N.terms_array(terms)
         self._opts = opts
+        self._ctx = context  # keeps the caller's Context (if given) alive as long as the session
+        if context is not None:
+            context._children.add(self)
         self.eo = N.EvolveOptions(1 if exact_ledger else 0, pauli,
                                   normalizer if normalizer else mp.n_spins, initial_capacity)
         self.ptr = C.c_void_p()
@@ -223,6 +230,83 @@ class Evolution:
             self.ptr = None
             return st
         return 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Fleet:
+    """Many independent FIDs in one resident batch (tpgp_fleet_*): batched powder orientations,
+    coupling draws, or served requests.  Each slot runs exactly the loop of ``evolve``."""
+
+    def __init__(self, n_words, n_slots, slot_capacity=1 << 16, context: Context | None = None):
+        self.eng = N.engine()
+        self.n_slots = n_slots
+        self.ptr = C.c_void_p()
+        self._ctx = context  # the context must outlive the fleet
+        if context is not None:
+            context._children.add(self)
+        st = self.eng.tpgp_fleet_open(C.byref(self.ptr), context.ptr if context else None,
+                                      n_words, n_slots, slot_capacity)
+        if st:
+            raise RuntimeError(f"tpgp_fleet_open: status {st}: "
+                               f"{self.eng.tpgp_hip_last_error().decode()}")
+
+    def admit(self, slot, initial: Map, terms, dt, steps, opts, exact_ledger=False, pauli=1,
+              normalizer=None):
+        ta = N.terms_array(terms)
+        job = N.FidJob()
+        job.initial = C.pointer(initial.m)
+        job.terms = C.cast(ta, C.POINTER(N.TpgpTerm))
+        job.count = len(terms)
+        job.dt = dt
+        job.steps = steps
+        job.options = opts
+        job.evolve = N.EvolveOptions(1 if exact_ledger else 0, pauli,
+                                     normalizer if normalizer else initial.n_spins, 0)
+        return self.eng.tpgp_fleet_admit(self.ptr, slot, C.byref(job))
+
+    def step(self):
+        running = C.c_size_t()
+        st = self.eng.tpgp_fleet_step(self.ptr, C.byref(running))
+        return st, running.value
+
+    def slot(self, slot):
+        state, status = C.c_int(), C.c_int()
+        done = C.c_size_t()
+        led = N.TpgpLedger()
+        self.eng.tpgp_fleet_slot(self.ptr, slot, C.byref(state), C.byref(done), C.byref(status),
+                                 C.byref(led))
+        return state.value, done.value, status.value, led
+
+    def fid(self, slot, capacity):
+        fid = np.full(capacity, np.nan)
+        rad = np.full(capacity, np.nan)
+        act = np.zeros(capacity, np.uint64)
+        n = C.c_size_t()
+        self.eng.tpgp_fleet_fid(self.ptr, slot, N.as_dp(fid), N.as_dp(rad),
+                                act.ctypes.data_as(C.POINTER(C.c_size_t)), capacity, C.byref(n))
+        return fid[:n.value], rad[:n.value], act[:n.value]
+
+    def release(self, slot, final: Map | None = None):
+        return self.eng.tpgp_fleet_release(self.ptr, slot,
+                                           C.byref(final.m) if final is not None else None)
+
+    def counters(self):
+        c = N.EvolutionCounters()
+        self.eng.tpgp_fleet_counters_get(self.ptr, C.byref(c))
+        return c
+
+    def slot_bytes(self, slot):
+        return self.eng.tpgp_fleet_slot_bytes(self.ptr, slot)
+
+    def close(self):
+        if self.ptr:
+            self.eng.tpgp_fleet_close(self.ptr)
+            self.ptr = None
 
     def __del__(self):
         try:

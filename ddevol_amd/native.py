@@ -66,6 +66,15 @@ class EvolveOptions(C.Structure):
                 ("initial_capacity", C.c_size_t)]
 
 
+class FidJob(C.Structure):  # tpgp_fid_job
+    _fields_ = [("initial", C.POINTER(TpgpMap)), ("terms", C.POINTER(TpgpTerm)),
+                ("count", C.c_size_t), ("dt", C.c_double), ("steps", C.c_size_t),
+                ("options", TpgpOptions), ("evolve", EvolveOptions)]
+
+
+SLOT_FREE, SLOT_RUNNING, SLOT_DONE, SLOT_FAILED = range(4)
+
+
 class EvolutionCounters(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("rotations", C.c_uint64), ("updates", C.c_uint64),
                 ("in_out", C.c_uint64)]
@@ -104,7 +113,8 @@ HIP_SYMBOLS = ["tpgp_hip_compiled", "tpgp_hip_create", "tpgp_hip_destroy",
                "tpgp_hip_batch_rotate", "tpgp_hip_batch_rotate_many", "tpgp_hip_batch_readout", "tpgp_hip_batch_sync",
                "tpgp_hip_batch_commit", "tpgp_hip_batch_rollback", "tpgp_hip_batch_reserve",
                "tpgp_hip_batch_info", "tpgp_hip_batch_stream", "tpgp_hip_batch_profile",
-               "tpgp_hip_batch_profile_read"]
+               "tpgp_hip_batch_profile_read", "tpgp_hip_batch_assign",
+               "tpgp_hip_batch_reserve_log"]
 ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_default_options", "tpgp_ledger_init", "tpgp_map_init", "tpgp_map_free",
                   "tpgp_map_clone", "tpgp_key_get", "tpgp_key_set", "tpgp_map_add",
@@ -114,7 +124,10 @@ ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_bound_fid_up", "tpgp_hip_evolve", "tpgp_hip_last_error",
                   "tpgp_evolution_open", "tpgp_evolution_run", "tpgp_evolution_close",
                   "tpgp_evolution_counters_get", "tpgp_evolution_profile",
-                  "tpgp_evolution_profile_read", "tpgp_evolution_key_words"]
+                  "tpgp_evolution_profile_read", "tpgp_evolution_key_words",
+                  "tpgp_fleet_open", "tpgp_fleet_close", "tpgp_fleet_admit", "tpgp_fleet_step",
+                  "tpgp_fleet_slot", "tpgp_fleet_fid", "tpgp_fleet_release",
+                  "tpgp_fleet_counters_get", "tpgp_fleet_slot_bytes"]
 
 
 def hip():
@@ -154,6 +167,8 @@ def hip():
     f("tpgp_hip_batch_reserve", C.c_int, C.c_void_p, sz, sz, p_c, sz)
     f("tpgp_hip_batch_info", C.c_int, C.c_void_p, sz, C.POINTER(sz), C.POINTER(sz))
     f("tpgp_hip_batch_stream", C.c_void_p, C.c_void_p)
+    f("tpgp_hip_batch_assign", C.c_int, C.c_void_p, sz, sz, p_u64, p_d, sz, p_c, sz)
+    f("tpgp_hip_batch_reserve_log", C.c_int, C.c_void_p, sz, p_c, sz)
     f("tpgp_hip_batch_profile", C.c_int, C.c_void_p, C.c_uint)
     f("tpgp_hip_batch_profile_read", C.c_int, C.c_void_p, C.c_uint, C.POINTER(C.c_double),
       C.POINTER(C.c_uint64))
@@ -214,6 +229,16 @@ def engine():
     f("tpgp_evolution_profile", C.c_int, C.c_void_p, C.c_uint)
     f("tpgp_evolution_profile_read", C.c_int, C.c_void_p, C.c_uint, pd, C.POINTER(C.c_uint64))
     f("tpgp_evolution_key_words", sz, C.c_void_p)
+    vp = C.c_void_p
+    f("tpgp_fleet_open", C.c_int, C.POINTER(vp), vp, sz, sz, sz)
+    f("tpgp_fleet_close", None, vp)
+    f("tpgp_fleet_admit", C.c_int, vp, sz, C.POINTER(FidJob))
+    f("tpgp_fleet_step", C.c_int, vp, psz)
+    f("tpgp_fleet_slot", C.c_int, vp, sz, C.POINTER(C.c_int), psz, C.POINTER(C.c_int), lp)
+    f("tpgp_fleet_fid", C.c_int, vp, sz, pd, pd, psz, sz, psz)
+    f("tpgp_fleet_release", C.c_int, vp, sz, mp)
+    f("tpgp_fleet_counters_get", None, vp, C.POINTER(EvolutionCounters))
+    f("tpgp_fleet_slot_bytes", C.c_uint64, vp, sz)
     _eng = L
     return L
 

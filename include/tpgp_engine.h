@@ -98,10 +98,11 @@ double tpgp_bound_fid_up(double a, size_t n);
  * F(k) = sum_s coeff(P_s)/normalizer before every step k = 0..steps (cli_rev.c:576-581).
  * fid[k] is the readout midpoint (MPFR readout when available, else the compensated
  * long-double sum / normalizer); fid_radius[k] its enclosure radius (NaN without MPFR);
- * active[k] the map length.  Arrays hold steps+1 entries; any may be NULL.
+ * active[k] the length of the sampled map.  Arrays hold steps+1 entries; any may be NULL.
  * exact_ledger: 1 = bit-identical ledger (host fold of logged drops, slower);
  *               0 = device ledger: rigorous upward-rounded sums in a fixed tree order.
- * On failure the map and ledger keep the last accepted step; *accepted counts steps done. */
+ * On failure the map and ledger keep the last accepted step; *accepted counts steps done and
+ * sample *accepted (the state the failing step started from) is still written. */
 typedef struct {
     int exact_ledger;
     unsigned pauli;       /* readout symbol (1 = X) */
@@ -135,6 +136,50 @@ int tpgp_evolution_profile(tpgp_evolution *evolution, unsigned mask);
 int tpgp_evolution_profile_read(const tpgp_evolution *evolution, unsigned kind,
                                 double *milliseconds, uint64_t *launches);
 size_t tpgp_evolution_key_words(const tpgp_evolution *evolution);
+
+/* ---- Batched and served FIDs ------------------------------------------------------------
+ * A fleet runs many independent FIDs (powder orientations, coupling draws, served requests)
+ * as the maps of one resident batch: every device pass works on all running slots at once,
+ * whatever their term counts.  Each slot runs exactly the loop of tpgp_hip_evolve -- same
+ * rotations, same readouts, same transactional steps, same statuses -- so a FID is bit for bit
+ * the same alone and in any mix.  All slots share the key width n_words (1: N <= 32, 2:
+ * 33 <= N <= 64).  Slots start with slot_capacity terms and grow on demand. */
+typedef struct tpgp_fleet tpgp_fleet;
+typedef struct {
+    const tpgp_map *initial;     /* canonical initial state (copied at admission) */
+    const tpgp_term *terms;      /* copied at admission */
+    size_t count;
+    double dt;
+    size_t steps;                /* Strang steps; steps + 1 FID samples */
+    tpgp_options options;        /* backend TPGP_HIP; hip_context ignored (the fleet's is used) */
+    tpgp_evolve_options evolve;  /* initial_capacity ignored */
+} tpgp_fid_job;
+typedef enum {
+    TPGP_SLOT_FREE = 0,
+    TPGP_SLOT_RUNNING,
+    TPGP_SLOT_DONE,
+    TPGP_SLOT_FAILED
+} tpgp_slot_state;
+/* context: caller-owned tpgp_hip_context (NULL: the fleet creates one on device 0). */
+tpgp_status tpgp_fleet_open(tpgp_fleet **fleet, void *context, size_t n_words, size_t n_slots,
+                            size_t slot_capacity);
+void tpgp_fleet_close(tpgp_fleet *fleet);
+/* Start a job in a FREE slot. */
+tpgp_status tpgp_fleet_admit(tpgp_fleet *fleet, size_t slot, const tpgp_fid_job *job);
+/* Advance every RUNNING slot by one Strang step (reading its FID first); a slot whose steps
+ * are done takes its last readout and becomes DONE; a failing step leaves the slot FAILED with
+ * the map and ledger of its last accepted step.  *running: RUNNING slots after the call. */
+tpgp_status tpgp_fleet_step(tpgp_fleet *fleet, size_t *running);
+tpgp_status tpgp_fleet_slot(const tpgp_fleet *fleet, size_t slot, tpgp_slot_state *state,
+                            size_t *steps_done, tpgp_status *status, tpgp_ledger *ledger);
+/* The slot's samples so far (capacity entries at most; returns the count in *n). */
+tpgp_status tpgp_fleet_fid(const tpgp_fleet *fleet, size_t slot, double *fid,
+                           double *fid_radius, size_t *active, size_t capacity, size_t *n);
+/* Free a slot; final_map (optional) receives its last accepted map. */
+tpgp_status tpgp_fleet_release(tpgp_fleet *fleet, size_t slot, tpgp_map *final_map);
+void tpgp_fleet_counters_get(const tpgp_fleet *fleet, tpgp_evolution_counters *counters);
+/* Bytes of device memory the fleet's slots hold (three term buffers per slot). */
+uint64_t tpgp_fleet_slot_bytes(const tpgp_fleet *fleet, size_t slot);
 
 /* Message of the last device-side failure on this thread ("" when none). */
 const char *tpgp_hip_last_error(void);

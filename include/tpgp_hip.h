@@ -114,6 +114,11 @@ void tpgp_hip_batch_destroy(tpgp_hip_batch *batch);
  * zero padding, finite nonzero values).  Synchronous. */
 int tpgp_hip_batch_upload(tpgp_hip_batch *batch, size_t map, const uint64_t *keys,
                           const double *values, size_t len, char *error, size_t error_capacity);
+/* Re-purpose slot `map` for a map of n_spins spins (same key width as the batch) and upload
+ * its canonical state as for tpgp_hip_batch_upload.  Pending work of that map is discarded. */
+int tpgp_hip_batch_assign(tpgp_hip_batch *batch, size_t map, size_t n_spins,
+                          const uint64_t *keys, const double *values, size_t len, char *error,
+                          size_t error_capacity);
 /* Copy map `map`'s committed state out in reference order.  *len receives the length; arrays
  * must hold `capacity` entries (a NULL array with capacity 0 only queries the length). */
 int tpgp_hip_batch_download(tpgp_hip_batch *batch, size_t map, uint64_t *keys, double *values,
@@ -148,6 +153,10 @@ int tpgp_hip_batch_rollback(tpgp_hip_batch *batch, char *error, size_t error_cap
 /* Grow map `map`'s slot to at least `capacity` terms, keeping its committed state. */
 int tpgp_hip_batch_reserve(tpgp_hip_batch *batch, size_t map, size_t capacity, char *error,
                            size_t error_capacity);
+/* Grow the device log of dropped values and radii to `entries` (sync reports an overflow as
+ * TPGP_HIP_ECAPACITY with the needed count in *n_log: roll back, reserve, run again). */
+int tpgp_hip_batch_reserve_log(tpgp_hip_batch *batch, size_t entries, char *error,
+                               size_t error_capacity);
 /* Committed length and slot capacity of a map. */
 int tpgp_hip_batch_info(const tpgp_hip_batch *batch, size_t map, size_t *committed_len,
                         size_t *capacity);
