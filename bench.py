@@ -95,9 +95,184 @@ def cpu_baseline(keys, values, n_spins, terms, dt, threshold, budget_s, threads)
                       f"{el:.1f} s, OpenMP branch generation on {threads} host threads"}
 
 
+def same_bits(a, b):
+    import numpy as np
+
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return a.shape == b.shape and bool(np.array_equal(a.view(np.uint64), b.view(np.uint64)))
+
+
+def bench_powder(args):
+    """Batched powder average: N=27 3x3x3 cluster, `orientations` field directions from the
+    seeded PCG, dt 0.01, threshold 1e-5, a window of `powder_steps` Strang steps per FID, all
+    orientations in one fleet; against the one-FID-at-a-time loop on the first `alone` of them
+    (whose FIDs must be bit-identical to the batched ones)."""
+    from ddevol_amd import engine as E
+    from ddevol_amd import hamiltonians as H
+
+    ctx = E.Context(0)
+    K, steps, n, dt, thr = args.orientations, args.powder_steps, 27, 0.01, 1e-5
+    orients = H.powder_orientations(K)
+    terms = [H.powder_terms(o) for o in orients]
+    fl = E.Fleet(1, K, 1 << 15, context=ctx)
+    for k in range(K):
+        init = E.Map(n)
+        for s in range(n):
+            init.add_single(s, 1, 1.0)
+        st = fl.admit(k, init, terms[k], dt, steps, E.options(threshold=thr, max_terms=10 ** 10))
+        assert st == 0, st
+    t0 = time.perf_counter()
+    rounds = 0
+    while True:
+        st, running = fl.step()
+        assert st == 0, N_last_error()
+        rounds += 1
+        if running == 0:
+            break
+    t_batch = time.perf_counter() - t0
+    c = fl.counters()
+    fids = [fl.fid(k, steps + 1)[0] for k in range(K)]
+    peak = max(int(fl.fid(k, steps + 1)[2].max()) for k in range(K))
+    fl.close()
+    log(f"powder batched: {K} FIDs x {steps} steps in {t_batch:.2f} s ({rounds} rounds), "
+        f"{c.updates} updates, largest map {peak} terms")
+    # one FID at a time (tpgp_hip_evolve per orientation)
+    A = min(args.alone, K)
+    t0 = time.perf_counter()
+    same = True
+    for k in range(A):
+        m = E.Map(n)
+        for s in range(n):
+            m.add_single(s, 1, 1.0)
+        led = E.ledger(0)
+        st, fid, rad, act, acc = E.evolve(m, terms[k], dt, steps,
+                                          E.options(threshold=thr, max_terms=10 ** 10,
+                                                    context=ctx), led)
+        assert st == 0 and acc == steps
+        same &= same_bits(fid, fids[k])
+    t_alone = time.perf_counter() - t0
+    batched, alone = K / t_batch, A / t_alone
+    log(f"powder one-at-a-time: {A} FIDs in {t_alone:.2f} s; identical FIDs: {same}")
+    return {
+        "metric": "batched powder FIDs/s (N=27 3x3x3 cluster, 20-step FID window)",
+        "value": batched, "unit": "FIDs/s", "n_gpus": 1, "higher_is_better": True,
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (orientations from InputPCG(20260711); initial state sum_s X_s)",
+        "config": {"workload": f"powder N=27, {K} orientations, dt 0.01, threshold 1e-5, "
+                               f"steps 0..{steps} per orientation, one fleet",
+                   "rounds": rounds, "branch_updates": c.updates,
+                   "updates_per_s": c.updates / t_batch, "largest_map_terms": peak},
+        "one_at_a_time": {"value": alone, "unit": "FIDs/s", "sample": f"first {A} orientations",
+                          "fids_identical": same},
+        "batched_over_one_at_a_time": batched / alone,
+    }
+
+
+def N_last_error():
+    from ddevol_amd import native as N
+
+    return N.engine().tpgp_hip_last_error().decode()
+
+
+def bench_serve(args):
+    """Serving stream with continuous batching (ddevol_amd.serve) against the one-FID-at-a-time
+    loop on the same requests; p50/p99 of completion - arrival, FIDs/s, identical FIDs."""
+    import numpy as np
+
+    from ddevol_amd import engine as E
+    from ddevol_amd import serve as S
+
+    ctx = E.Context(0)
+    lo, hi = args.serve_steps
+    reqs = S.poisson_stream(args.requests, args.rate, seed=7, steps_range=(lo, hi))
+    srv = S.Server(ctx, slots_per_width=args.slots, budget_bytes=int(args.budget_gb * 2 ** 30))
+    t0 = time.perf_counter()
+    srv.run(reqs)
+    t_srv = time.perf_counter() - t0
+    rounds = srv.rounds
+    srv.close()
+    lat = np.array([r.latency for r in reqs])
+    alone = [S.Request(r.rid, r.arrival, r.n, r.terms, r.threshold, r.steps, r.dt) for r in reqs]
+    t0 = time.perf_counter()
+    S.run_alone(ctx, alone)
+    t_alone = time.perf_counter() - t0
+    lat_a = np.array([r.latency for r in alone])
+    same = all(same_bits(a.fid, b.fid) for a, b in zip(reqs, alone))
+    log(f"served {len(reqs)} requests in {t_srv:.2f} s ({rounds} rounds); one at a time "
+        f"{t_alone:.2f} s; identical FIDs: {same}")
+    pct = lambda a, q: float(np.percentile(a, q))  # noqa: E731
+    return {
+        "metric": "served FIDs/s with p50/p99 latency (continuous batching)",
+        "value": len(reqs) / t_srv, "unit": "FIDs/s", "n_gpus": 1, "higher_is_better": True,
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Poisson stream, InputPCG(7))",
+        "config": {"workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, N in 8..48 "
+                               f"rings (NN U[0.5,1.5], NNN 1/8), threshold in "
+                               f"{{1e-4,1e-5,1e-6}}, steps U{{{lo}..{hi}}}, dt 0.01",
+                   "steps_note": "the config's U{100..2000} steps scaled down: at threshold "
+                                 "1e-6 an N=48 ring passes 4e7 terms by step 300",
+                   "hbm_budget_gb": args.budget_gb, "slots_per_key_width": args.slots,
+                   "rounds": rounds},
+        "latency_s": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": float(lat.max())},
+        "one_at_a_time": {"value": len(reqs) / t_alone, "unit": "FIDs/s",
+                          "latency_s": {"p50": pct(lat_a, 50), "p99": pct(lat_a, 99)},
+                          "fids_identical": same},
+        "batched_over_one_at_a_time": t_alone / t_srv,
+    }
+
+
+def bench_single(args, n, terms, dt, thr, label, workload):
+    """One large FID: untimed warmup steps, then timed steps on the resident map."""
+    from ddevol_amd import engine as E
+
+    ctx = E.Context(0)
+    mp = E.Map(n)
+    for s in range(n):
+        mp.add_single(s, 1, 1.0)
+    opts = E.options(threshold=thr, max_terms=10 ** 11, context=ctx)
+    led = E.ledger(0)
+    ev = E.Evolution(mp, terms, dt, opts, context=ctx)
+    if args.warmup:
+        st, fid, rad, act, acc = ev.run(args.warmup, led)
+        assert st == 0, N_last_error()
+    c0 = ev.counters()
+    t0 = time.perf_counter()
+    st, fid, rad, act, acc = ev.run(args.steps, led)
+    el = time.perf_counter() - t0
+    assert st == 0, N_last_error()
+    c1 = ev.counters()
+    ev.close()
+    upd = c1.updates - c0.updates
+    log(f"{label}: steps {args.warmup + 1}..{args.warmup + args.steps}, {el:.2f} s, "
+        f"terms {int(act[0])} -> {int(act[-1])}")
+    return {
+        "metric": f"FP64 Pauli branch-updates/s ({label})", "value": upd / el,
+        "unit": "branch-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / max(args.steps, 1) * 1e3, "higher_is_better": True,
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (initial state sum_s X_s)",
+        "config": {"workload": workload, "terms_at_start": int(act[0]),
+                   "terms_at_end": int(act[-1]), "fid": [float(x) for x in fid[[0, -1]]],
+                   "ledger_dropped_l2": led.dropped_l2},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="j1j2",
+                    choices=["j1j2", "powder", "serve", "large", "plain"],
+                    help="j1j2: the headline N=16 J1-J2 window (default); powder: batched "
+                         "orientations vs one at a time; serve: continuous batching with "
+                         "p50/p99; large: N=64 range-4 ring; plain: N=12 ring")
+    ap.add_argument("--orientations", type=int, default=512)
+    ap.add_argument("--powder-steps", type=int, default=20)
+    ap.add_argument("--alone", type=int, default=16,
+                    help="powder: orientations run one at a time for the comparison")
+    ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--rate", type=float, default=16.0)
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=[10, 200])
+    ap.add_argument("--slots", type=int, default=64)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=600)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -115,6 +290,22 @@ def main():
         print(json.dumps({"error": "this engine runs one FID per GPU: only --gpus 1 is supported; "
                                    "multi-GPU sharding is a design, not part of this build"}))
         sys.exit(2)
+    if args.workload != "j1j2":
+        from ddevol_amd import hamiltonians as H
+
+        if args.workload == "powder":
+            out = bench_powder(args)
+        elif args.workload == "serve":
+            out = bench_serve(args)
+        elif args.workload == "large":
+            out = bench_single(args, 64, H.range_ring(64, 4), 0.01, 1e-5,
+                               "N=64 range-4 ring",
+                               "N=64 ring, 1/r^3 couplings r=1..4, dt 0.01, threshold 1e-5")
+        else:
+            out = bench_single(args, 12, H.ring(12), 0.01, 1e-8, "N=12 ring",
+                               "N=12 NN ring, dt 0.01, threshold 1e-8")
+        print(json.dumps(out), flush=True)
+        return
 
     import numpy as np
 
