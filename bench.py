@@ -264,6 +264,9 @@ def main():
                     help="j1j2: the headline N=16 J1-J2 window (default); powder: batched "
                          "orientations vs one at a time; serve: continuous batching with "
                          "p50/p99; large: N=64 range-4 ring; plain: N=12 ring")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="calibration only: run 5 plain device-to-device copies of this many "
+                         "bytes (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     ap.add_argument("--orientations", type=int, default=512)
     ap.add_argument("--powder-steps", type=int, default=20)
     ap.add_argument("--alone", type=int, default=16,
@@ -290,6 +293,17 @@ def main():
         print(json.dumps({"error": "this engine runs one FID per GPU: only --gpus 1 is supported; "
                                    "multi-GPU sharding is a design, not part of this build"}))
         sys.exit(2)
+    if args.copy_bytes:
+        import torch
+
+        src = torch.ones(args.copy_bytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_bytes, "copies": 5}), flush=True)
+        return
     if args.workload != "j1j2":
         from ddevol_amd import hamiltonians as H
 

@@ -1183,6 +1183,302 @@ __global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap 
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Small maps: one workgroup per map runs a whole sequence of passes (partitions and rotations)
+// in one launch.  It walks the tiles of every class in order, so output offsets are a running
+// count: no split, count or scan kernels, and no launch between passes.
+// ------------------------------------------------------------------------------------------
+constexpr uint32_t kPassSmall = 4u;      // PassMap.flags: the map runs on the small-map path
+constexpr uint32_t kPassPartition = 8u;  // PassMap.flags: the descriptor is a partition
+
+// Merge-path split at diagonal d, searched by one wave over [lo, hi] (64-ary rounds; P(hi)
+// counts as true).  All lanes return the same value.
+template <int W>
+__device__ uint64_t split_wave(const uint64_t *keys, const MergeCtx &x, const Key<W> &q,
+                               uint64_t d, uint64_t lo, uint64_t hi) {
+    auto pred = [&](uint64_t xi) -> bool {
+        Key<W> a = kload<W>(keys + (x.A0 + xi) * W);
+        Key<W> dk = kxor<W>(kload<W>(keys + (x.D0 + d - xi - 1) * W), q);
+        return klt<W>(dk, a);
+    };
+    while (hi > lo) {
+        const uint64_t span = hi - lo;
+        if (span < 64) {  // lane `span` maps to hi (true): the ballot is never 0
+            const uint64_t xi = lo + lane_id();
+            const bool p = xi < hi ? pred(xi) : true;
+            lo = lo + (uint64_t)(__ffsll((unsigned long long)__ballot(p)) - 1);
+            break;
+        }
+        const uint64_t xi = lo + ((span * lane_id()) >> 6);
+        const uint64_t bal = __ballot(pred(xi));
+        if (bal == 0) {
+            lo = lo + ((span * 63) >> 6) + 1;
+        } else {
+            const int f = __ffsll((unsigned long long)bal) - 1;
+            if (f == 0) break;
+            const uint64_t xf = lo + ((span * (uint64_t)f) >> 6);
+            const uint64_t xp = lo + ((span * (uint64_t)(f - 1)) >> 6);
+            lo = xp + 1;
+            hi = xf;
+        }
+    }
+    return lo;
+}
+
+template <int W> struct SmallShared {
+    TileIn<W> in;
+    uint64_t okeys[kMergeTile * W];
+    double ovals[kMergeTile];
+    unsigned scratch[8];
+    uint64_t split;
+    uint64_t rout[17];
+    uint64_t run[16];
+    unsigned hist[16];
+    unsigned wave_cnt[4][16];
+    TileLedger red[kBlock / 64];
+    unsigned nonfinite;
+};
+
+template <int W>
+__device__ __forceinline__ void small_rotation(const BatchDev &b, const PassMap &P, uint32_t m,
+                               SmallShared<W> &sh) {
+    const uint32_t n = b.slot[m].n_spins;
+    const uint64_t off = b.slot[m].off, cap = b.slot[m].cap;
+    const uint64_t *ikeys = b.arena[P.in_buf].keys + off * W;
+    uint64_t *gk = b.arena[P.out_buf].keys + off * W;
+    double *gv = b.arena[P.out_buf].vals + off;
+    const uint64_t *rin = b.regions + ((uint64_t)m * 3 + P.in_buf) * 17;
+    uint64_t *rout = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
+    const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
+    const bool want_log = (P.flags & 2u) != 0;
+    bool over = b.overflow[m] != 0;
+    uint64_t out_at = 0;
+    double l1 = 0.0, sq = 0.0, rsq = 0.0;
+    uint64_t n_drop = 0, n_rad = 0;
+    bool bad = false;
+    if (threadIdx.x == 0) sh.nonfinite = 0;
+    for (uint32_t c = 0; c < 16; ++c) {
+        if (threadIdx.x == 0) sh.rout[c] = out_at;
+        MergeCtx x;
+        x.m = m;
+        x.c = c;
+        x.A0 = rin[c];
+        x.nA = rin[c + 1] - rin[c];
+        x.anti = class_anti(c, P.qt, P.qs);
+        x.cq = c ^ ((P.qt << 2) | P.qs);
+        x.D0 = rin[x.cq];
+        x.nD = x.anti ? rin[x.cq + 1] - rin[x.cq] : 0;
+        x.k = 0;
+        const Coef k = coef_of(P, x);
+        const uint64_t total = x.nA + x.nD;
+        uint64_t a0 = 0;
+        for (uint64_t d0 = 0; d0 < total; d0 += kMergeTile) {
+            TileGeom t;
+            t.d0 = d0;
+            t.d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
+            t.a0 = a0;
+            if (!x.anti) {
+                t.a1 = t.d1;
+            } else if (t.d1 == total) {
+                t.a1 = x.nA;
+            } else {
+                if (wave_id() == 0) {
+                    uint64_t lo = t.d1 > x.nD ? t.d1 - x.nD : 0;
+                    if (lo < a0) lo = a0;
+                    uint64_t hi = a0 + (t.d1 - t.d0);
+                    if (hi > x.nA) hi = x.nA;
+                    const uint64_t a1 = split_wave<W>(ikeys, x, q, t.d1, lo, hi);
+                    if (lane_id() == 0) sh.split = a1;
+                }
+                __syncthreads();
+                t.a1 = sh.split;
+            }
+            load_tile<W>(b, P, x, q, t, sh.in);
+            __syncthreads();
+            const uint32_t obase = threadIdx.x * kMergeItems;
+            unsigned nout = 0;
+            walk_tile<W>(sh.in, t, [&](const Key<W> &key, bool hasA, double va, bool hasD,
+                                       double vd) {
+                double v, r;
+                merged(k, hasA, va, hasD, vd, v, r, bad);
+                const uint64_t k0 = key.w[0], k1 = W > 1 ? key.w[W - 1] : 0;
+                if (r != 0.0) {
+                    n_rad++;
+                    rsq = upadd(rsq, upmul(r, r));
+                    if (want_log) log_entry(b, k0, k1, r, 0.0, m, P.rot_seq, 1);
+                }
+                if (v == 0.0) return;  // exact cancellation: dropped silently
+                if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) {
+                    n_drop++;
+                    l1 = upadd(l1, __builtin_fabs(v));
+                    sq = upadd(sq, upmul(v, v));
+                    if (want_log) log_entry(b, k0, k1, __builtin_fabs(v), v, m, P.rot_seq, 0);
+                    return;
+                }
+                kstore<W>(sh.okeys + (obase + nout) * W, key);
+                sh.ovals[obase + nout] = v;
+                nout++;
+            });
+            unsigned tot;
+            const unsigned myoff = block_exclusive_scan(nout, sh.scratch, &tot);
+            for (unsigned i = 0; i < nout; ++i) {
+                kstore<W>(sh.in.keys + (myoff + i) * W, kload<W>(sh.okeys + (obase + i) * W));
+                sh.in.vals[myoff + i] = sh.ovals[obase + i];
+            }
+            __syncthreads();
+            if (!over && out_at + tot > cap) over = true;
+            if (!over) {
+                for (unsigned i = threadIdx.x; i < tot; i += kBlock) {
+                    kstore<W>(gk + (out_at + i) * W, kload<W>(sh.in.keys + i * W));
+                    gv[out_at + i] = sh.in.vals[i];
+                }
+            }
+            out_at += tot;
+            a0 = t.a1;
+            __syncthreads();
+        }
+    }
+    if (bad) atomicOr(&sh.nonfinite, 1u);
+    // ledger partials: per-thread chains in tile order, then a fixed tree over the block
+    for (int d = 1; d < 64; d <<= 1) {
+        const double o1 = __shfl_down(l1, d, 64), o2 = __shfl_down(sq, d, 64),
+                     o3 = __shfl_down(rsq, d, 64);
+        const uint64_t c1 = __shfl_down(n_drop, d, 64), c2 = __shfl_down(n_rad, d, 64);
+        if (lane_id() + d < 64) {
+            l1 = upadd(l1, o1);
+            sq = upadd(sq, o2);
+            rsq = upadd(rsq, o3);
+            n_drop += c1;
+            n_rad += c2;
+        }
+    }
+    if (lane_id() == 0) {
+        TileLedger L;
+        L.l1 = l1;
+        L.sq = sq;
+        L.rsq = rsq;
+        L.dropped = n_drop;
+        L.radii = n_rad;
+        L.nonfinite = 0;
+        L.pad = 0;
+        sh.red[wave_id()] = L;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        TileLedger T = sh.red[0];
+        for (int w = 1; w < kBlock / 64; ++w) ledger_fold(T, sh.red[w]);
+        for (int c = 0; c < 16; ++c) rout[c] = over ? 0 : sh.rout[c];
+        rout[16] = over ? 0 : out_at;
+        b.len[m * 3 + P.out_buf] = over ? 0 : out_at;
+        if (over) b.overflow[m] = 1;
+        tpgp_hip_record r;
+        r.len_in = rin[16];
+        r.len_out = out_at;
+        r.dropped = T.dropped;
+        r.l1_up = T.l1;
+        r.sq_up = T.sq;
+        r.rad_sq_up = T.rsq;
+        r.radii = T.radii;
+        r.executed = rin[16] > 0;
+        r.nonfinite = sh.nonfinite;
+        r.overflow = over ? 1u : 0u;
+        r.rotation = P.rot_seq;
+        b.rec[(uint64_t)m * b.max_rec + P.rec_slot] = r;
+    }
+    __syncthreads();
+}
+
+template <int W>
+__device__ __forceinline__ void small_partition(const BatchDev &b, const PassMap &P, uint32_t m,
+                                SmallShared<W> &sh) {
+    const uint32_t n = b.slot[m].n_spins;
+    const uint64_t off = b.slot[m].off;
+    const uint64_t len = b.len[m * 3 + P.in_buf];
+    const uint64_t *keys = b.arena[P.in_buf].keys + off * W;
+    const double *vals = b.arena[P.in_buf].vals + off;
+    uint64_t *okeys = b.arena[P.out_buf].keys + off * W;
+    double *ovals = b.arena[P.out_buf].vals + off;
+    if (threadIdx.x < 16) sh.hist[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint64_t e0 = 0; e0 < len; e0 += kPartTile) {  // histogram of the new classes
+        Key<W> k[kPartRounds];
+#pragma unroll
+        for (int r = 0; r < kPartRounds; ++r) {
+            const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
+            if (e < len) k[r] = kload<W>(keys + e * W);
+        }
+#pragma unroll
+        for (int r = 0; r < kPartRounds; ++r) {
+            const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
+            if (e < len) atomicAdd(&sh.hist[(kdigit<W>(k[r], P.pa) << 2) | kdigit<W>(k[r], P.pb)], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t *reg = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
+        uint64_t at = 0;
+        for (int c = 0; c < 16; ++c) {
+            reg[c] = at;
+            sh.run[c] = at;
+            at += sh.hist[c];
+        }
+        reg[16] = at;
+        b.len[m * 3 + P.out_buf] = at;
+    }
+    __syncthreads();
+    for (uint64_t e0 = 0; e0 < len; e0 += kBlock) {  // stable scatter, one round at a time
+        const uint64_t e = e0 + threadIdx.x;
+        const bool live = e < len;
+        Key<W> kk;
+        double vv = 0.0;
+        unsigned cls = 16;
+        if (live) {
+            kk = kload<W>(keys + e * W);
+            vv = vals[e];
+            cls = (kdigit<W>(kk, P.pa) << 2) | kdigit<W>(kk, P.pb);
+        }
+        unsigned below = 0;
+        const uint64_t lanemask = (1ull << lane_id()) - 1ull;
+        for (unsigned c = 0; c < 16; ++c) {
+            const uint64_t bal = __ballot(cls == c);
+            if (cls == c) below = (unsigned)__popcll(bal & lanemask);
+            if (lane_id() == 0) sh.wave_cnt[wave_id()][c] = (unsigned)__popcll(bal);
+        }
+        __syncthreads();
+        if (live) {
+            uint64_t dst = sh.run[cls] + below;
+            for (unsigned w = 0; w < wave_id(); ++w) dst += sh.wave_cnt[w][cls];
+            kstore<W>(okeys + dst * W, move_to_top<W>(kk, P.pa, P.pb, n));
+            ovals[dst] = vv;
+        }
+        __syncthreads();
+        if (threadIdx.x < 16) {
+            unsigned add = 0;
+            for (unsigned w = 0; w < 4; ++w) add += sh.wave_cnt[w][threadIdx.x];
+            sh.run[threadIdx.x] += add;
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per small map (maps[blockIdx.x]) runs n_ops passes: descriptor k of map m is
+// ops[k * M + m]; descriptors without kPassSmall belong to other maps' paths.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_small(BatchDev b, const PassMap *ops, uint32_t n_ops,
+                                                  const uint32_t *maps, uint32_t n_small) {
+    __shared__ SmallShared<W> sh;
+    if (blockIdx.x >= n_small) return;
+    const uint32_t m = maps[blockIdx.x];
+    for (uint32_t k = 0; k < n_ops; ++k) {
+        const PassMap &P = ops[(uint64_t)k * b.n_maps + m];
+        if (!(P.flags & kPassSmall)) continue;
+        if (P.flags & kPassPartition)
+            small_partition<W>(b, P, m, sh);
+        else
+            small_rotation<W>(b, P, m, sh);
+    }
+}
+
 // One wave per map: binary search of the n single-site keys (symbol `pauli` at site s).
 template <int W>
 __global__ void k_readout(BatchDev b, const ReadoutMap *pm, unsigned pauli) {
@@ -1543,6 +1839,9 @@ struct tpgp_hip_batch {
     bool validate = false;  // TPGP_HIP_VALIDATE=1: check order and regions after every pass
     uint64_t est_tiles = 1;  // merge tiles of the last planned rotation (grid sizing only)
     uint64_t log_entries = 1u << 20;  // device log capacity (grown by reserve_log)
+    // maps at most this long run on the small-map path (one persistent workgroup per map)
+    uint64_t small_limit = 0;
+    std::vector<uint8_t> small;  // per map, decided at each rotate_many call
     uint32_t pass_count = 0;
     // kernel timing with HIP events on the batch stream (kinds: TPGP_HIP_PROFILE_*)
     unsigned profiling = 0;  // bit k: bracket launches of profile kind k with events
@@ -1552,8 +1851,8 @@ struct tpgp_hip_batch {
         hipEvent_t start, stop;
     };
     std::vector<Pending> pending;
-    double prof_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    uint64_t prof_count[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double prof_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t prof_count[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -1715,13 +2014,13 @@ unsigned tile_grid(uint64_t tiles) {
 unsigned partition_grid(const tpgp_hip_batch *b, const std::vector<int> &want) {
     uint64_t t = 0;
     for (uint32_t m = 0; m < b->M; ++m)
-        if (want[m]) t += b->maps[m].est_len / kPartTile + 1;
+        if (want[m] && !b->small[m]) t += b->maps[m].est_len / kPartTile + 1;
     return tile_grid(t);
 }
 unsigned rotation_grid(tpgp_hip_batch *b, const tpgp_hip_rotation *req) {
     uint64_t t = 0;
     for (uint32_t m = 0; m < b->M; ++m)
-        if (req[m].active) t += (3 * b->maps[m].est_len / 2) / kMergeTile + 16;
+        if (req[m].active && !b->small[m]) t += (3 * b->maps[m].est_len / 2) / kMergeTile + 16;
     b->est_tiles = t;
     return tile_grid(t);
 }
@@ -1903,6 +2202,12 @@ extern "C" tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *ctx, size_t n
     b->grid = (unsigned)ctx->cus * 8;
     const char *v = getenv("TPGP_HIP_VALIDATE");
     b->validate = v && *v && *v != '0';
+    // small-map path: worth it while one workgroup per map beats ~10 launches per rotation;
+    // with many maps the workgroups fill the GPU, so larger maps qualify
+    const char *sl = getenv("TPGP_HIP_SMALL_LIMIT");
+    b->small_limit = sl && *sl ? strtoull(sl, nullptr, 10) : (n_maps == 1 ? 16384u : 262144u);
+    if (b->validate) b->small_limit = 0;  // validation checks the multi-kernel passes
+    b->small.assign(n_maps, 0);
     std::lock_guard<std::mutex> lock(ctx->mutex);
     DeviceGuard guard(ctx->device);
     if (!guard.ok || allocate_device(b, error, cap) != TPGP_HIP_OK) {
@@ -2128,12 +2433,26 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
         unsigned grid;
     };
     std::vector<Launch> launches;
+    // path of every map for this call (by its last known length)
+    std::vector<uint32_t> small_ids;
+    for (uint32_t m = 0; m < M; ++m) {
+        b->small[m] = b->maps[m].est_len <= b->small_limit;
+        if (b->small[m]) small_ids.push_back(m);
+    }
+    uint32_t *d_small = nullptr;
+    if (!small_ids.empty()) {
+        uint32_t *h_small = stage(b, small_ids.size(), &d_small);
+        std::memcpy(h_small, small_ids.data(), small_ids.size() * sizeof(uint32_t));
+        hipError_t e = push_stage(b, h_small, d_small, small_ids.size());
+        if (e != hipSuccess) return hip_fail(e, "HIP descriptor staging", error, cap);
+    }
     for (size_t r0 = 0; r0 < n_rot; r0 += chunk) {
     const size_t r1 = std::min(n_rot, r0 + chunk);
     PassMap *d0;
     PassMap *h0 = stage(b, 2 * (r1 - r0) * M, &d0);
     launches.clear();
     size_t used = 0;
+    bool any_small = false;
     for (size_t r = r0; r < r1; ++r) {
         const tpgp_hip_rotation *q = req + r * M;
         // bring each active map's pair (i, j) to the top two digits
@@ -2157,13 +2476,39 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
         launches.push_back({true, used, plan_rotation(b, q, rotations[r], h0 + used)});
         used += M;
     }
+    // maps on the small-map path leave the multi-kernel passes to the persistent kernel
+    std::vector<uint8_t> pass_large(used / M, 0);
+    for (size_t k = 0; k < used / M; ++k)
+        for (uint32_t m = 0; m < M; ++m) {
+            PassMap &P = h0[k * M + m];
+            if (!P.active) continue;
+            if (b->small[m]) {
+                P.active = 0;
+                P.flags |= kPassSmall | (launches[k].rotation ? 0u : kPassPartition);
+                any_small = true;
+            } else {
+                pass_large[k] = 1;
+            }
+        }
     hipError_t e = push_stage(b, h0, d0, used);
     if (e != hipSuccess) return hip_fail(e, "HIP descriptor staging", error, cap);
-    for (const Launch &L : launches) {
+    for (size_t k = 0; k < launches.size(); ++k) {
+        const Launch &L = launches[k];
+        if (!pass_large[k]) continue;
         if (L.rotation)
             launch_rotation(b, d0 + L.at, L.grid);
         else
             launch_partition(b, d0 + L.at, L.grid);
+    }
+    if (any_small) {
+        ProfileScope ps(b, TPGP_HIP_PROFILE_SMALL);
+        const unsigned ns = (unsigned)small_ids.size();
+        if (b->W == 1)
+            hipLaunchKernelGGL(k_small<1>, dim3(ns), dim3(kBlock), 0, b->ctx->stream, b->dev,
+                               (const PassMap *)d0, (uint32_t)(used / M), d_small, ns);
+        else
+            hipLaunchKernelGGL(k_small<2>, dim3(ns), dim3(kBlock), 0, b->ctx->stream, b->dev,
+                               (const PassMap *)d0, (uint32_t)(used / M), d_small, ns);
     }
     if ((e = hipGetLastError()) != hipSuccess)
         return hip_fail(e, "HIP rotation launch", error, cap);
@@ -2571,7 +2916,7 @@ extern "C" int tpgp_hip_batch_profile(tpgp_hip_batch *b, unsigned mask) {
     (void)hipStreamSynchronize(b->ctx->stream);
     collect_profile(b);
     b->profiling = mask & TPGP_HIP_PROFILE_ALL;
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < 10; ++k) {
         b->prof_ms[k] = 0;
         b->prof_count[k] = 0;
     }
@@ -2580,7 +2925,7 @@ extern "C" int tpgp_hip_batch_profile(tpgp_hip_batch *b, unsigned mask) {
 
 extern "C" int tpgp_hip_batch_profile_read(const tpgp_hip_batch *b, unsigned kind,
                                            double *milliseconds, uint64_t *launches) {
-    if (!b || kind > 8) return TPGP_HIP_EINVAL;
+    if (!b || kind > 9) return TPGP_HIP_EINVAL;
     if (milliseconds) *milliseconds = b->prof_ms[kind];
     if (launches) *launches = b->prof_count[kind];
     return TPGP_HIP_OK;

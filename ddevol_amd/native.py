@@ -81,10 +81,11 @@ class EvolutionCounters(C.Structure):
 
 
 (PROFILE_MERGE, PROFILE_SCATTER, PROFILE_ROTATION, PROFILE_PARTITION, PROFILE_COUNT,
- PROFILE_SCAN, PROFILE_PLAN, PROFILE_HISTOGRAM, PROFILE_SPLITS) = range(9)
-PROFILE_ALL = 0x1FF
+ PROFILE_SCAN, PROFILE_PLAN, PROFILE_HISTOGRAM, PROFILE_SPLITS, PROFILE_SMALL) = range(10)
+PROFILE_ALL = 0x3FF
 PROFILE_NAMES = ["k_rot_write", "k_part_scatter", "rotation_pass", "partition_pass",
-                 "k_rot_count", "k_rot_scan", "k_rot_plan", "k_part_count", "k_rot_splits"]
+                 "k_rot_count", "k_rot_scan", "k_rot_plan", "k_part_count", "k_rot_splits",
+                 "k_small"]
 
 
 class HipRotation(C.Structure):

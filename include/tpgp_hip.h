@@ -176,7 +176,8 @@ enum {
     TPGP_HIP_PROFILE_PLAN = 6,      /* rotation tile plan */
     TPGP_HIP_PROFILE_HISTOGRAM = 7, /* partition histogram */
     TPGP_HIP_PROFILE_SPLITS = 8,    /* rotation merge-path splits */
-    TPGP_HIP_PROFILE_ALL = 0x1ff
+    TPGP_HIP_PROFILE_SMALL = 9,     /* small-map persistent kernel (one chunk of passes) */
+    TPGP_HIP_PROFILE_ALL = 0x3ff
 };
 int tpgp_hip_batch_profile(tpgp_hip_batch *batch, unsigned mask);
 int tpgp_hip_batch_profile_read(const tpgp_hip_batch *batch, unsigned kind,

@@ -76,3 +76,15 @@ def test_golden_configs_gpu(hip_ctx, golden, name):
     m, fids, rads, act, leds = gpu_traj(hip_ctx, rec["n"], config_terms(name), fromhex(rec["dt"]),
                                         rec["steps"], fromhex(rec["threshold"]))
     check(rec, m, fids, rads, act, leds)
+
+
+@pytest.mark.parametrize("limit", ["0", "100000000"])
+@pytest.mark.parametrize("name", ["ring12", "j1j2_n16", "cluster27", "range64"])
+def test_golden_both_paths(hip_ctx, golden, name, limit, monkeypatch):
+    """The same golden trajectories with every map forced onto the multi-kernel path (limit 0)
+    and onto the small-map persistent path (limit 1e8)."""
+    monkeypatch.setenv("TPGP_HIP_SMALL_LIMIT", limit)
+    rec = golden[name]
+    m, fids, rads, act, leds = gpu_traj(hip_ctx, rec["n"], config_terms(name), fromhex(rec["dt"]),
+                                        rec["steps"], fromhex(rec["threshold"]))
+    check(rec, m, fids, rads, act, leds)
