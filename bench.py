@@ -187,14 +187,14 @@ def bench_serve(args):
     reqs = S.poisson_stream(args.requests, args.rate, seed=7, steps_range=(lo, hi))
     srv = S.Server(ctx, slots_per_width=args.slots, budget_bytes=int(args.budget_gb * 2 ** 30))
     t0 = time.perf_counter()
-    srv.run(reqs)
+    srv.run(reqs, log=log)
     t_srv = time.perf_counter() - t0
     rounds = srv.rounds
     srv.close()
     lat = np.array([r.latency for r in reqs])
     alone = [S.Request(r.rid, r.arrival, r.n, r.terms, r.threshold, r.steps, r.dt) for r in reqs]
     t0 = time.perf_counter()
-    S.run_alone(ctx, alone)
+    S.run_alone(ctx, alone, log=log)
     t_alone = time.perf_counter() - t0
     lat_a = np.array([r.latency for r in alone])
     same = all(same_bits(a.fid, b.fid) for a, b in zip(reqs, alone))

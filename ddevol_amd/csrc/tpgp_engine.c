@@ -862,6 +862,9 @@ tpgp_status tpgp_evolution_run(tpgp_evolution *ev, size_t steps, tpgp_ledger *l,
         uint64_t len = 0, up = 0, io = 0, ex = 0;
         size_t read_len = 0; /* length of the state read out (the committed state) */
         tpgp_hip_batch_info(S->batch, 0, &read_len, NULL);
+        if (more && read_len + read_len / 2 + 4096 > S->cap &&
+            (s = session_grow(S, 2 * read_len + 4096))) /* room ahead of need */
+            return s;
         for (;;) {
             for (size_t k = 0; k < 2 * 64; k++) rd[k] = NAN; /* filled only by a good sync */
             int hs = tpgp_hip_batch_readout(S->batch, ev->eo.pauli, NULL, err, sizeof(err));
@@ -1183,6 +1186,33 @@ tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
     tpgp_status s = TPGP_OK;
     int any = 0;
     for (size_t m = 0; m < M; m++) any |= part[m] = f->slot[m].state == TPGP_SLOT_RUNNING;
+    /* grow slots ahead of need (one re-layout for all of them): a slot whose committed map
+     * fills half of it gets room for three times that */
+    if (any) {
+        size_t *caps = calloc(M, sizeof(size_t));
+        int grow_any = 0;
+        if (!caps) s = TPGP_ENOMEM;
+        for (size_t m = 0; m < M && caps; m++) {
+            size_t len = 0;
+            if (!part[m] || tpgp_hip_batch_info(f->batch, m, &len, NULL)) continue;
+            if (2 * len + 4096 > f->slot[m].cap) {
+                size_t cap = 3 * len + 4096;
+                if (cap < 2 * f->slot[m].cap) cap = 2 * f->slot[m].cap;
+                caps[m] = cap;
+                grow_any = 1;
+            }
+        }
+        if (grow_any) {
+            if (tpgp_hip_batch_reserve_many(f->batch, caps, err, sizeof(err))) {
+                note("reserve", err);
+                s = TPGP_ENOMEM;
+            } else {
+                for (size_t m = 0; m < M; m++)
+                    if (caps[m]) f->slot[m].cap = caps[m];
+            }
+        }
+        free(caps);
+    }
     while (any && !s) {
         /* readouts of the committed state, one call per readout symbol */
         for (unsigned pauli = 1; pauli <= 3 && !s; pauli++) {
@@ -1346,10 +1376,18 @@ tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
             break;
         }
         any = 0;
-        if (regrow) {
-            for (size_t m = 0; m < M && !s; m++) {
+        if (regrow) { /* every overflowing slot grows in one re-layout, then runs again */
+            size_t *caps = calloc(M, sizeof(size_t));
+            if (!caps) {
+                s = TPGP_ENOMEM;
+                break;
+            }
+            for (size_t m = 0; m < M; m++)
+                if (part[m]) caps[m] = f->slot[m].cap;
+            int rh = tpgp_hip_batch_reserve_many(f->batch, caps, err, sizeof(err));
+            free(caps);
+            for (size_t m = 0; m < M; m++) {
                 if (!part[m]) continue;
-                int rh = tpgp_hip_batch_reserve(f->batch, m, f->slot[m].cap, err, sizeof(err));
                 if (rh) {
                     note("reserve", err);
                     f->slot[m].status = map_status(rh);

@@ -330,6 +330,68 @@ __device__ __forceinline__ uint32_t find_bucket(const uint64_t *start, uint32_t 
 // ------------------------------------------------------------------------------------------
 // Partition (change of active pair): plan -> count -> scan -> scatter
 // ------------------------------------------------------------------------------------------
+// Stable scatter of one partition tile (elements [e0, e1), at most kPartTile): every element
+// goes to run[class] + (number of elements of its class before it in the tile), with digits
+// pa, pb moved to the top of its key; run[] (LDS) then advances by the tile's class counts.
+// Ranks come from per-round, per-wave ballots; three barriers per tile.
+struct ScatterShared {
+    unsigned cnt[kPartRounds][kBlock / 64][16];
+    uint64_t base[kPartRounds][kBlock / 64][16];
+    uint64_t run[16];
+};
+template <int W>
+__device__ __forceinline__ void scatter_tile(const uint64_t *keys, const double *vals,
+                                             uint64_t e0, uint64_t e1, uint32_t pa, uint32_t pb,
+                                             uint32_t n, uint64_t *okeys, double *ovals,
+                                             ScatterShared &sh) {
+    Key<W> kk[kPartRounds];
+    double vv[kPartRounds];
+    unsigned tag[kPartRounds];  // class (bits 0-4, 16 = none) | rank in wave (bits 8-14)
+#pragma unroll
+    for (int r = 0; r < kPartRounds; ++r) {
+        const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
+        if (e < e1) {
+            kk[r] = kload<W>(keys + e * W);
+            vv[r] = vals[e];
+        }
+    }
+    const uint64_t lanemask = (1ull << lane_id()) - 1ull;
+#pragma unroll
+    for (int r = 0; r < kPartRounds; ++r) {
+        const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
+        const unsigned cls = e < e1 ? (kdigit<W>(kk[r], pa) << 2) | kdigit<W>(kk[r], pb) : 16u;
+        unsigned below = 0;
+        for (unsigned c = 0; c < 16; ++c) {
+            const uint64_t bal = __ballot(cls == c);
+            if (cls == c) below = (unsigned)__popcll(bal & lanemask);
+            if (lane_id() == 0) sh.cnt[r][wave_id()][c] = (unsigned)__popcll(bal);
+        }
+        tag[r] = cls | (below << 8);
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {  // class c: offsets of every (round, wave) group, in element order
+        const unsigned c = threadIdx.x;
+        uint64_t at = sh.run[c];
+        for (int r = 0; r < kPartRounds; ++r)
+            for (int w = 0; w < kBlock / 64; ++w) {
+                sh.base[r][w][c] = at;
+                at += sh.cnt[r][w][c];
+            }
+        sh.run[c] = at;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kPartRounds; ++r) {
+        const unsigned cls = tag[r] & 31u;
+        if (cls < 16) {
+            const uint64_t dst = sh.base[r][wave_id()][cls] + (tag[r] >> 8);
+            kstore<W>(okeys + dst * W, move_to_top<W>(kk[r], pa, pb, n));
+            ovals[dst] = vv[r];
+        }
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap *pm) {
     __shared__ unsigned scratch[8];
     const uint32_t M = b.n_maps;
@@ -410,6 +472,10 @@ __global__ __launch_bounds__(kBlock) void k_part_scan(BatchDev b, const PassMap 
 
 // Per tile: move every element to its class's run (stable), with digits pa, pb moved to the
 // top of the key.  The map's first tile also writes the output regions and length.
+// Per tile: move every element to its class's run (stable), with digits pa, pb moved to the
+// top of the key; one round of 256 elements at a time (ranks from per-wave ballots), which keeps
+// register use low for the many-tile launches.  The map's first tile also writes the output
+// regions and length.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassMap *pm) {
     __shared__ unsigned wave_cnt[4][16];
@@ -457,7 +523,6 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassM
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
             const bool live = e < e1;
             const unsigned cls = live ? (kdigit<W>(kk[r], P.pa) << 2) | kdigit<W>(kk[r], P.pb) : 16u;
-            // rank among equal classes in lower lanes of this wave
             unsigned below = 0;
             const uint64_t lanemask = (1ull << lane_id()) - 1ull;
             for (unsigned c = 0; c < 16; ++c) {
@@ -1232,11 +1297,10 @@ template <int W> struct SmallShared {
     unsigned scratch[8];
     uint64_t split;
     uint64_t rout[17];
-    uint64_t run[16];
     unsigned hist[16];
-    unsigned wave_cnt[4][16];
     TileLedger red[kBlock / 64];
     unsigned nonfinite;
+    ScatterShared sc;
 };
 
 template <int W>
@@ -1292,6 +1356,11 @@ __device__ __forceinline__ void small_rotation(const BatchDev &b, const PassMap 
                 }
                 __syncthreads();
                 t.a1 = sh.split;
+            }
+            if (!tile_ok(x, t)) {  // internal inconsistency: reported at sync, nothing read
+                if (threadIdx.x == 0) atomicOr(&b.counter[1], 1u);
+                over = true;
+                break;
             }
             load_tile<W>(b, P, x, q, t, sh.in);
             __syncthreads();
@@ -1390,7 +1459,7 @@ __device__ __forceinline__ void small_rotation(const BatchDev &b, const PassMap 
 
 template <int W>
 __device__ __forceinline__ void small_partition(const BatchDev &b, const PassMap &P, uint32_t m,
-                                SmallShared<W> &sh) {
+                                                SmallShared<W> &sh) {
     const uint32_t n = b.slot[m].n_spins;
     const uint64_t off = b.slot[m].off;
     const uint64_t len = b.len[m * 3 + P.in_buf];
@@ -1419,45 +1488,16 @@ __device__ __forceinline__ void small_partition(const BatchDev &b, const PassMap
         uint64_t at = 0;
         for (int c = 0; c < 16; ++c) {
             reg[c] = at;
-            sh.run[c] = at;
+            sh.sc.run[c] = at;
             at += sh.hist[c];
         }
         reg[16] = at;
         b.len[m * 3 + P.out_buf] = at;
     }
     __syncthreads();
-    for (uint64_t e0 = 0; e0 < len; e0 += kBlock) {  // stable scatter, one round at a time
-        const uint64_t e = e0 + threadIdx.x;
-        const bool live = e < len;
-        Key<W> kk;
-        double vv = 0.0;
-        unsigned cls = 16;
-        if (live) {
-            kk = kload<W>(keys + e * W);
-            vv = vals[e];
-            cls = (kdigit<W>(kk, P.pa) << 2) | kdigit<W>(kk, P.pb);
-        }
-        unsigned below = 0;
-        const uint64_t lanemask = (1ull << lane_id()) - 1ull;
-        for (unsigned c = 0; c < 16; ++c) {
-            const uint64_t bal = __ballot(cls == c);
-            if (cls == c) below = (unsigned)__popcll(bal & lanemask);
-            if (lane_id() == 0) sh.wave_cnt[wave_id()][c] = (unsigned)__popcll(bal);
-        }
-        __syncthreads();
-        if (live) {
-            uint64_t dst = sh.run[cls] + below;
-            for (unsigned w = 0; w < wave_id(); ++w) dst += sh.wave_cnt[w][cls];
-            kstore<W>(okeys + dst * W, move_to_top<W>(kk, P.pa, P.pb, n));
-            ovals[dst] = vv;
-        }
-        __syncthreads();
-        if (threadIdx.x < 16) {
-            unsigned add = 0;
-            for (unsigned w = 0; w < 4; ++w) add += sh.wave_cnt[w][threadIdx.x];
-            sh.run[threadIdx.x] += add;
-        }
-        __syncthreads();
+    for (uint64_t e0 = 0; e0 < len; e0 += kPartTile) {  // stable scatter, tile by tile
+        const uint64_t e1 = e0 + kPartTile < len ? e0 + kPartTile : len;
+        scatter_tile<W>(keys, vals, e0, e1, P.pa, P.pb, n, okeys, ovals, sh.sc);
     }
 }
 
@@ -1839,7 +1879,12 @@ struct tpgp_hip_batch {
     bool validate = false;  // TPGP_HIP_VALIDATE=1: check order and regions after every pass
     uint64_t est_tiles = 1;  // merge tiles of the last planned rotation (grid sizing only)
     uint64_t log_entries = 1u << 20;  // device log capacity (grown by reserve_log)
-    // maps at most this long run on the small-map path (one persistent workgroup per map)
+    // path choice (per rotate_many call): modelled costs of one rotation, or a fixed bound
+    double cost_launch_us = 45.0;  // multi-kernel chain of one rotation (launches, latency)
+    double cost_multi_ns = 0.03;   // multi-kernel streaming cost per term
+    double cost_small_ns = 5.0;    // one persistent workgroup, per term of its map (W=1;
+                                   // twice that for two-word keys)
+    bool small_fixed = false;      // TPGP_HIP_SMALL_LIMIT given: length bound instead
     uint64_t small_limit = 0;
     std::vector<uint8_t> small;  // per map, decided at each rotate_many call
     uint32_t pass_count = 0;
@@ -2003,6 +2048,52 @@ int next_buffer(const HostMap &hm) {
     for (int k = 0; k < 3; ++k)
         if (k != hm.kc && k != hm.kcur) return k;
     return (hm.kc + 1) % 3;
+}
+
+// Which maps run on the small-map path.  Per rotation, the multi-kernel passes cost a fixed
+// launch chain plus a per-term streaming cost over their maps; the persistent kernel costs the
+// per-term time of one workgroup on its largest map (its workgroups run side by side, in waves
+// of `resident` groups).  Maps are taken onto the small path in increasing length while that
+// lowers the modelled time.  TPGP_HIP_SMALL_LIMIT fixes the cut instead (a length bound).
+void choose_paths(tpgp_hip_batch *b, const tpgp_hip_rotation *req, size_t n_rot) {
+    const uint32_t M = b->M;
+    std::vector<uint32_t> idx;
+    idx.reserve(M);
+    for (uint32_t m = 0; m < M; ++m) {
+        b->small[m] = 0;
+        bool active = false;
+        for (size_t r = 0; r < n_rot && !active; ++r) active = req[r * M + m].active != 0;
+        if (active) idx.push_back(m);
+    }
+    if (b->validate) return;
+    if (b->small_fixed) {
+        for (uint32_t m : idx) b->small[m] = b->maps[m].est_len <= b->small_limit;
+        return;
+    }
+    std::sort(idx.begin(), idx.end(),
+              [&](uint32_t x, uint32_t y) { return b->maps[x].est_len < b->maps[y].est_len; });
+    double rest = 0;  // terms left on the multi-kernel path
+    for (uint32_t m : idx) rest += (double)b->maps[m].est_len;
+    const double resident = 4.0 * b->ctx->cus;
+    auto cost = [&](size_t k, double large_terms) {  // first k maps small
+        double t = 0;
+        if (k < idx.size()) t += b->cost_launch_us + b->cost_multi_ns * large_terms * 1e-3;
+        if (k > 0)
+            t += b->cost_small_ns * (b->W == 2 ? 2.0 : 1.0) *
+                 (double)b->maps[idx[k - 1]].est_len * 1e-3 * std::ceil((double)k / resident);
+        return t;
+    };
+    size_t best = 0;
+    double best_t = cost(0, rest), large = rest;
+    for (size_t k = 1; k <= idx.size(); ++k) {
+        large -= (double)b->maps[idx[k - 1]].est_len;
+        const double t = cost(k, large);
+        if (t < best_t) {
+            best_t = t;
+            best = k;
+        }
+    }
+    for (size_t k = 0; k < best; ++k) b->small[idx[k]] = 1;
 }
 
 // Tile kernels run one tile per workgroup: the grid covers the estimated tile count with a
@@ -2205,8 +2296,8 @@ extern "C" tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *ctx, size_t n
     // small-map path: worth it while one workgroup per map beats ~10 launches per rotation;
     // with many maps the workgroups fill the GPU, so larger maps qualify
     const char *sl = getenv("TPGP_HIP_SMALL_LIMIT");
-    b->small_limit = sl && *sl ? strtoull(sl, nullptr, 10) : (n_maps == 1 ? 16384u : 262144u);
-    if (b->validate) b->small_limit = 0;  // validation checks the multi-kernel passes
+    b->small_fixed = sl && *sl;
+    b->small_limit = b->small_fixed ? strtoull(sl, nullptr, 10) : 0;
     b->small.assign(n_maps, 0);
     std::lock_guard<std::mutex> lock(ctx->mutex);
     DeviceGuard guard(ctx->device);
@@ -2434,18 +2525,10 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
     };
     std::vector<Launch> launches;
     // path of every map for this call (by its last known length)
+    choose_paths(b, req, n_rot);
     std::vector<uint32_t> small_ids;
-    for (uint32_t m = 0; m < M; ++m) {
-        b->small[m] = b->maps[m].est_len <= b->small_limit;
+    for (uint32_t m = 0; m < M; ++m)
         if (b->small[m]) small_ids.push_back(m);
-    }
-    uint32_t *d_small = nullptr;
-    if (!small_ids.empty()) {
-        uint32_t *h_small = stage(b, small_ids.size(), &d_small);
-        std::memcpy(h_small, small_ids.data(), small_ids.size() * sizeof(uint32_t));
-        hipError_t e = push_stage(b, h_small, d_small, small_ids.size());
-        if (e != hipSuccess) return hip_fail(e, "HIP descriptor staging", error, cap);
-    }
     for (size_t r0 = 0; r0 < n_rot; r0 += chunk) {
     const size_t r1 = std::min(n_rot, r0 + chunk);
     PassMap *d0;
@@ -2501,6 +2584,13 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
             launch_partition(b, d0 + L.at, L.grid);
     }
     if (any_small) {
+        // the map list goes with this chunk's descriptors: a ring region is only guaranteed
+        // until later uploads reuse it, so every launch reads a copy staged for it
+        uint32_t *d_small = nullptr;
+        uint32_t *h_small = stage(b, small_ids.size(), &d_small);
+        std::memcpy(h_small, small_ids.data(), small_ids.size() * sizeof(uint32_t));
+        if ((e = push_stage(b, h_small, d_small, small_ids.size())) != hipSuccess)
+            return hip_fail(e, "HIP descriptor staging", error, cap);
         ProfileScope ps(b, TPGP_HIP_PROFILE_SMALL);
         const unsigned ns = (unsigned)small_ids.size();
         if (b->W == 1)
@@ -2798,6 +2888,130 @@ extern "C" int tpgp_hip_batch_download(tpgp_hip_batch *b, size_t map, uint64_t *
     return TPGP_HIP_OK;
 }
 
+namespace {
+// Re-lay the arenas with new slot capacities, keeping every map's committed state (moved to
+// buffer 0 with its length, regions and digit layout).  The committed states wait in one
+// temporary device buffer (device-to-device copies); only if that allocation fails do they
+// take the way through host memory.
+int relayout(tpgp_hip_batch *b, const std::vector<uint64_t> &new_cap, char *error, size_t cap) {
+    hipStream_t st = b->ctx->stream;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "HIP reserve", error, cap);
+    const uint32_t M = b->M;
+    const int W = b->W;
+    std::vector<uint64_t> len((size_t)M * 3), reg((size_t)M * 3 * 17);
+    if ((e = hipMemcpy(len.data(), b->dev.len, len.size() * 8, hipMemcpyDeviceToHost)) !=
+            hipSuccess ||
+        (e = hipMemcpy(reg.data(), b->dev.regions, reg.size() * 8, hipMemcpyDeviceToHost)) !=
+            hipSuccess)
+        return hip_fail(e, "HIP reserve", error, cap);
+    std::vector<uint64_t> n(M), at(M);
+    uint64_t total = 0;
+    for (uint32_t m = 0; m < M; ++m) {
+        n[m] = len[(size_t)m * 3 + b->maps[m].kc];
+        at[m] = total;
+        total += n[m];
+    }
+    uint64_t *tk = nullptr;
+    double *tv = nullptr;
+    bool on_device = hipMalloc((void **)&tk, (total * W + 1) * 8) == hipSuccess &&
+                     hipMalloc((void **)&tv, (total + 1) * 8) == hipSuccess;
+    std::vector<uint64_t> hk;
+    std::vector<double> hv;
+    if (!on_device) {
+        (void)hipGetLastError();
+        if (tk) (void)hipFree(tk);
+        tk = nullptr;
+        hk.resize(total * W);
+        hv.resize(total);
+    }
+    for (uint32_t m = 0; m < M && e == hipSuccess; ++m) {
+        const HostMap &hm = b->maps[m];
+        if (!n[m]) continue;
+        uint64_t *dk = on_device ? tk + at[m] * W : hk.data() + at[m] * W;
+        double *dv = on_device ? tv + at[m] : hv.data() + at[m];
+        const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        e = hipMemcpyAsync(dk, b->dev.arena[hm.kc].keys + hm.off * W, n[m] * W * 8, kind, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dv, b->dev.arena[hm.kc].vals + hm.off, n[m] * 8, kind, st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        if (tk) (void)hipFree(tk);
+        if (tv) (void)hipFree(tv);
+        return hip_fail(e, "HIP reserve", error, cap);
+    }
+    free_all(b);
+    uint64_t off = 0;
+    for (uint32_t m = 0; m < M; ++m) {
+        HostMap &hm = b->maps[m];
+        if (new_cap[m] > hm.cap) hm.cap = new_cap[m];
+        hm.off = off;
+        off += (hm.cap + 63) & ~63ull;
+    }
+    b->total_cap = off;
+    int s = allocate_device(b, error, cap);
+    if (s != TPGP_HIP_OK) {
+        if (tk) (void)hipFree(tk);
+        if (tv) (void)hipFree(tv);
+        b->poisoned = true;
+        return s;
+    }
+    std::vector<uint64_t> nlen((size_t)M * 3, 0), nreg((size_t)M * 3 * 17, 0);
+    for (uint32_t m = 0; m < M && e == hipSuccess; ++m) {
+        HostMap &hm = b->maps[m];
+        nlen[(size_t)m * 3] = n[m];
+        for (int c = 0; c < 17; ++c)
+            nreg[(size_t)m * 3 * 17 + c] = reg[((size_t)m * 3 + hm.kc) * 17 + c];
+        if (n[m]) {
+            const uint64_t *sk = on_device ? tk + at[m] * W : hk.data() + at[m] * W;
+            const double *sv = on_device ? tv + at[m] : hv.data() + at[m];
+            const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+            e = hipMemcpyAsync(b->dev.arena[0].keys + hm.off * W, sk, n[m] * W * 8, kind, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(b->dev.arena[0].vals + hm.off, sv, n[m] * 8, kind, st);
+        }
+        hm.kc = hm.kcur = 0;
+        hm.sig = hm.sig_c;
+        hm.nrec = hm.nrd = 0;
+        hm.est_len = n[m];
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->dev.len, nlen.data(), nlen.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->dev.regions, nreg.data(), nreg.size() * 8, hipMemcpyHostToDevice,
+                           st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (tk) (void)hipFree(tk);
+    if (tv) (void)hipFree(tv);
+    b->logged.clear();
+    return e == hipSuccess ? TPGP_HIP_OK : hip_fail(e, "HIP reserve", error, cap);
+}
+}  // namespace
+
+extern "C" int tpgp_hip_batch_reserve_many(tpgp_hip_batch *b, const size_t *capacities,
+                                           char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!b || !capacities) {
+        msg(error, cap, "Invalid reserve arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    bool grow = false;
+    std::vector<uint64_t> want(b->M);
+    for (uint32_t m = 0; m < b->M; ++m) {
+        if (capacities[m] > (1ull << 37)) {
+            msg(error, cap, "Invalid reserve capacity for map %u.", m);
+            return TPGP_HIP_EINVAL;
+        }
+        want[m] = capacities[m];
+        grow |= capacities[m] > b->maps[m].cap;
+    }
+    if (!grow) return TPGP_HIP_OK;
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    return relayout(b, want, error, cap);
+}
+
 extern "C" int tpgp_hip_batch_reserve(tpgp_hip_batch *b, size_t map, size_t capacity,
                                       char *error, size_t cap) {
     msg(error, cap, "");
@@ -2806,78 +3020,9 @@ extern "C" int tpgp_hip_batch_reserve(tpgp_hip_batch *b, size_t map, size_t capa
         return TPGP_HIP_EINVAL;
     }
     if (capacity <= b->maps[map].cap) return TPGP_HIP_OK;
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
-    DeviceGuard guard(b->ctx->device);
-    hipStream_t st = b->ctx->stream;
-    hipError_t e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(e, "HIP reserve", error, cap);
-    // keep the committed states (and lengths/regions) of every map
-    const uint32_t M = b->M;
-    const int W = b->W;
-    std::vector<uint64_t> len((size_t)M * 3);
-    if ((e = hipMemcpy(len.data(), b->dev.len, len.size() * 8, hipMemcpyDeviceToHost)) !=
-        hipSuccess)
-        return hip_fail(e, "HIP reserve", error, cap);
-    std::vector<std::vector<uint64_t>> hk(M);
-    std::vector<std::vector<double>> hv(M);
-    for (uint32_t m = 0; m < M; ++m) {
-        HostMap &hm = b->maps[m];
-        uint64_t n = len[(size_t)m * 3 + hm.kc];
-        hk[m].resize(n * W);
-        hv[m].resize(n);
-        if (n) {
-            if ((e = hipMemcpy(hk[m].data(), b->dev.arena[hm.kc].keys + hm.off * W, n * W * 8,
-                               hipMemcpyDeviceToHost)) != hipSuccess ||
-                (e = hipMemcpy(hv[m].data(), b->dev.arena[hm.kc].vals + hm.off, n * 8,
-                               hipMemcpyDeviceToHost)) != hipSuccess)
-                return hip_fail(e, "HIP reserve", error, cap);
-        }
-    }
-    free_all(b);
-    uint64_t off = 0;
-    for (uint32_t m = 0; m < M; ++m) {
-        HostMap &hm = b->maps[m];
-        if (m == map) hm.cap = capacity;
-        hm.off = off;
-        off += (hm.cap + 63) & ~63ull;
-    }
-    b->total_cap = off;
-    int s = allocate_device(b, error, cap);
-    if (s != TPGP_HIP_OK) {
-        b->poisoned = true;
-        return s;
-    }
-    // committed states go to buffer 0 of the new arenas, with their lengths and layouts
-    std::vector<uint64_t> nlen((size_t)M * 3, 0);
-    for (uint32_t m = 0; m < M; ++m) {
-        HostMap &hm = b->maps[m];
-        hm.kc = hm.kcur = 0;
-        hm.sig = hm.sig_c;
-        hm.nrec = hm.nrd = 0;
-        uint64_t n = hv[m].size();
-        hm.est_len = n;
-        nlen[(size_t)m * 3] = n;
-        if (n) {
-            if ((e = hipMemcpy(b->dev.arena[0].keys + hm.off * W, hk[m].data(), n * W * 8,
-                               hipMemcpyHostToDevice)) != hipSuccess ||
-                (e = hipMemcpy(b->dev.arena[0].vals + hm.off, hv[m].data(), n * 8,
-                               hipMemcpyHostToDevice)) != hipSuccess)
-                return hip_fail(e, "HIP reserve", error, cap);
-        }
-    }
-    b->logged.clear();
-    if ((e = hipMemcpy(b->dev.len, nlen.data(), nlen.size() * 8, hipMemcpyHostToDevice)) !=
-        hipSuccess)
-        return hip_fail(e, "HIP reserve", error, cap);
-    for (uint32_t m = 0; m < M; ++m) {
-        // every layout keeps its active pair on the top two digits: regions follow from keys
-        if (W == 1)
-            hipLaunchKernelGGL(k_regions<1>, dim3(1), dim3(64), 0, st, b->dev, m, 0u);
-        else
-            hipLaunchKernelGGL(k_regions<2>, dim3(1), dim3(64), 0, st, b->dev, m, 0u);
-    }
-    e = hipStreamSynchronize(st);
-    return e == hipSuccess ? TPGP_HIP_OK : hip_fail(e, "HIP reserve", error, cap);
+    std::vector<size_t> caps(b->M, 0);
+    caps[map] = capacity;
+    return tpgp_hip_batch_reserve_many(b, caps.data(), error, cap);
 }
 
 extern "C" int tpgp_hip_batch_reserve_log(tpgp_hip_batch *b, size_t entries, char *error,

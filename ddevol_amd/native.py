@@ -115,7 +115,7 @@ HIP_SYMBOLS = ["tpgp_hip_compiled", "tpgp_hip_create", "tpgp_hip_destroy",
                "tpgp_hip_batch_commit", "tpgp_hip_batch_rollback", "tpgp_hip_batch_reserve",
                "tpgp_hip_batch_info", "tpgp_hip_batch_stream", "tpgp_hip_batch_profile",
                "tpgp_hip_batch_profile_read", "tpgp_hip_batch_assign",
-               "tpgp_hip_batch_reserve_log"]
+               "tpgp_hip_batch_reserve_log", "tpgp_hip_batch_reserve_many"]
 ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_default_options", "tpgp_ledger_init", "tpgp_map_init", "tpgp_map_free",
                   "tpgp_map_clone", "tpgp_key_get", "tpgp_key_set", "tpgp_map_add",
@@ -170,6 +170,7 @@ def hip():
     f("tpgp_hip_batch_stream", C.c_void_p, C.c_void_p)
     f("tpgp_hip_batch_assign", C.c_int, C.c_void_p, sz, sz, p_u64, p_d, sz, p_c, sz)
     f("tpgp_hip_batch_reserve_log", C.c_int, C.c_void_p, sz, p_c, sz)
+    f("tpgp_hip_batch_reserve_many", C.c_int, C.c_void_p, C.POINTER(sz), p_c, sz)
     f("tpgp_hip_batch_profile", C.c_int, C.c_void_p, C.c_uint)
     f("tpgp_hip_batch_profile_read", C.c_int, C.c_void_p, C.c_uint, C.POINTER(C.c_double),
       C.POINTER(C.c_uint64))

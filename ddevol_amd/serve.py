@@ -162,7 +162,7 @@ class Server:
             fl.close()
 
 
-def run_alone(context: E.Context, requests, clock=time.perf_counter):
+def run_alone(context: E.Context, requests, clock=time.perf_counter, log=None):
     """The one-FID-at-a-time loop: each request runs by itself (tpgp_hip_evolve) in arrival
     order, starting no earlier than its arrival; returns the requests with results filled in."""
     t0 = clock()
@@ -179,4 +179,7 @@ def run_alone(context: E.Context, requests, clock=time.perf_counter):
         r.fid = fid[:acc + 1]
         r.active = act[:acc + 1]
         r.finished = clock() - t0
+        if log:
+            log(f"alone: request {r.rid} (N={r.n}, {r.steps} steps, thr {r.threshold:g}) done "
+                f"at {r.finished:.2f} s")
     return requests

@@ -153,6 +153,11 @@ int tpgp_hip_batch_rollback(tpgp_hip_batch *batch, char *error, size_t error_cap
 /* Grow map `map`'s slot to at least `capacity` terms, keeping its committed state. */
 int tpgp_hip_batch_reserve(tpgp_hip_batch *batch, size_t map, size_t capacity, char *error,
                            size_t error_capacity);
+/* Grow several slots at once (capacities[m] terms; 0 or less than the current capacity keeps
+ * the slot): one re-layout of the arenas for all of them.  Committed states are kept; work
+ * since the last commit is discarded. */
+int tpgp_hip_batch_reserve_many(tpgp_hip_batch *batch, const size_t *capacities, char *error,
+                                size_t error_capacity);
 /* Grow the device log of dropped values and radii to `entries` (sync reports an overflow as
  * TPGP_HIP_ECAPACITY with the needed count in *n_log: roll back, reserve, run again). */
 int tpgp_hip_batch_reserve_log(tpgp_hip_batch *batch, size_t entries, char *error,

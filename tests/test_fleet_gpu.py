@@ -159,3 +159,22 @@ def test_fleet_growth_and_failure_isolation(hip_ctx):
     for k in range(3):
         check_job(fl, k, refs[k], n)
     fl.close()
+
+
+@pytest.mark.parametrize("limit", ["0", "100000000"])
+def test_fleet_descriptor_ring_wraps(hip_ctx, limit, monkeypatch):
+    """64 powder orientations: one step's 948 rotations span several descriptor chunks, so the
+    staging ring wraps within a call (regression: every chunk must carry its own map list).
+    Both device paths; every FID equal to its alone run."""
+    monkeypatch.setenv("TPGP_HIP_SMALL_LIMIT", limit)
+    K, n, dt, thr, steps = 64, 27, 0.01, 1e-5, 2
+    orients = H.powder_orientations(K)
+    fl = E.Fleet(1, K, 1 << 12, context=hip_ctx)
+    for k, o in enumerate(orients):
+        assert fl.admit(k, sum_x(n), H.powder_terms(o), dt, steps,
+                        E.options(threshold=thr, max_terms=10 ** 8)) == 0
+    run_all(fl)
+    for k in (0, 17, 63):
+        ref = alone(hip_ctx, n, H.powder_terms(orients[k]), dt, steps, thr, False)
+        check_job(fl, k, ref, n, exact=False)
+    fl.close()
