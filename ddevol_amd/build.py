@@ -3,6 +3,8 @@
 * ``ddevol_amd/lib/libtpgp_hip.so``    -- HIP kernels + C ABI (hipcc, gfx950)
 * ``ddevol_amd/lib/libtpgp_engine.so`` -- C17 host engine (gcc), linked to the above, with MPFR
   when its headers are present (as the reference's TPGP_HAVE_MPFR build).
+* ``ddevol_amd/lib/tpgp_rev``           -- the FID run driver (command line of the reference's
+  ``tpgp_rev``) on the engine above.
 
 Run ``python -m ddevol_amd.build`` (or ``__graft_entry__.build()``).  Every call recompiles.
 """
@@ -70,10 +72,17 @@ def build(verbose: bool = False) -> None:
         cmd += ["-DTPGP_HAVE_MPFR=1", f"-I{pre}/include"]
         libs = [f"-L{pre}/lib", f"-Wl,-rpath,{pre}/lib", "-lmpfr", "-lgmp"]
     tmp = eng_so + ".tmp"
-    cmd += ["-o", tmp, os.path.join(CSRC, "tpgp_engine.c"), f"-L{LIB}", "-ltpgp_hip",
+    cmd += ["-o", tmp, os.path.join(CSRC, "tpgp_engine.c"),
+            os.path.join(CSRC, "tpgp_checkpoint.c"), f"-L{LIB}", "-ltpgp_hip",
             "-Wl,-rpath,$ORIGIN", *libs, "-lm"]
     run(cmd)
     os.replace(tmp, eng_so)
+    cli = os.path.join(LIB, "tpgp_rev")
+    tmp = cli + ".tmp"
+    run(["gcc", *[f for f in CC_FLAGS if f not in ("-shared", "-fPIC")], "-o", tmp,
+         os.path.join(CSRC, "tpgp_cli.c"), f"-L{LIB}", "-ltpgp_engine", "-ltpgp_hip",
+         "-Wl,-rpath,$ORIGIN", "-Wl,--allow-shlib-undefined", *libs, "-lm"])
+    os.replace(tmp, cli)
 
 
 if __name__ == "__main__":

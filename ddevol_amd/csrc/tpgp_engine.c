@@ -933,6 +933,24 @@ tpgp_status tpgp_evolution_close(tpgp_evolution *ev, tpgp_map *m) {
     return s;
 }
 
+tpgp_status tpgp_evolution_snapshot(tpgp_evolution *ev, tpgp_map *m) {
+    if (!ev || !m || m->n_spins != ev->n_spins) return TPGP_EINVAL;
+    return session_download(&ev->S, m);
+}
+
+tpgp_status tpgp_evolution_l2(tpgp_evolution *ev, double *l2) {
+    if (!ev || !l2) return TPGP_EINVAL;
+    char err[512];
+    double pair[2];
+    const int hs = tpgp_hip_batch_sum_squares(ev->S.batch, 0, pair, err, sizeof(err));
+    if (hs) {
+        note("sum of squares", err);
+        return map_status(hs);
+    }
+    *l2 = (double)sqrtl((long double)pair[0] + (long double)pair[1]);
+    return TPGP_OK;
+}
+
 void tpgp_evolution_counters_get(const tpgp_evolution *ev, tpgp_evolution_counters *c) {
     if (ev && c) *c = ev->c;
 }

@@ -44,6 +44,7 @@ constexpr int kBlock = 256;        // 4 waves of 64
 constexpr int kMergeItems = 4;     // merge-path elements per thread
 constexpr int kMergeTile = kBlock * kMergeItems;
 constexpr int kChunkTiles = 4 * kBlock;  // rotation tiles per scan chunk
+constexpr uint32_t kSumBlocks = 512;     // blocks of the sum-of-squares reduction
 // PassMap.flags bit: the map's commuting classes cannot lose a key to this rotation's
 // truncation (every coefficient there already exceeds the threshold), so they pass unchanged
 constexpr uint32_t kPassCommutingKept = 16u;
@@ -332,6 +333,7 @@ struct BatchDev {
     uint64_t *log_count;      // [1]
     uint64_t log_cap;
     uint64_t *debug;          // [16] validation report (debug[0] != 0: violation found)
+    double *sumsq;            // [kSumBlocks * 2] per-block double-double partial sums
 };
 
 __device__ __forceinline__ uint32_t find_bucket(const uint64_t *start, uint32_t n, uint64_t g) {
@@ -2456,6 +2458,45 @@ __global__ void k_readout(BatchDev b, const ReadoutMap *pm, unsigned pauli) {
     b.rd[((uint64_t)m * b.max_rd + P.rec_slot) * 64 + s] = out;
 }
 
+// Sum of squares of one map's values as an unevaluated double-double (hi, lo) per block.  Every
+// square splits exactly (x*x = p + e, e = fma(x, x, -p)) and partial sums combine with
+// error-free two-sums, so the total is within a relative ~len * 2^-104 of the exact sum: the
+// map's l2 norm comes out correctly rounded (but for ties) whatever the summation order.
+__host__ __device__ __forceinline__ void dd_add(double &hi, double &lo, double b_hi,
+                                                double b_lo) {
+    const double s = hi + b_hi;
+    const double bb = s - hi;
+    double err = (hi - (s - bb)) + (b_hi - bb);
+    err += lo + b_lo;
+    hi = s + err;
+    lo = err - (hi - s);
+}
+__global__ void __launch_bounds__(256) k_sumsq(const double *vals, uint64_t len, double *out) {
+    double hi = 0.0, lo = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < len; k += stride) {
+        const double x = vals[k];
+        const double p = x * x;
+        dd_add(hi, lo, p, fma(x, x, -p));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double oh = __shfl_down(hi, off, 64), ol = __shfl_down(lo, off, 64);
+        dd_add(hi, lo, oh, ol);
+    }
+    __shared__ double part[8];
+    const unsigned wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[2 * wave] = hi;
+        part[2 * wave + 1] = lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned w = 1; w < (blockDim.x >> 6); ++w) dd_add(hi, lo, part[2 * w], part[2 * w + 1]);
+        out[2 * blockIdx.x] = hi;
+        out[2 * blockIdx.x + 1] = lo;
+    }
+}
+
 // Validation (TPGP_HIP_VALIDATE=1): every active map's working buffer must be strictly sorted
 // and its class regions must match the classes of its keys.  The first violation is recorded.
 template <int W>
@@ -2903,6 +2944,7 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     ALLOC(d.log, d.log_cap);
     ALLOC(d.log_count, 1);
     ALLOC(d.debug, 16);
+    ALLOC(d.sumsq, 2 * kSumBlocks);
     ALLOC(b->d_stage, b->stage_cap);
 #undef ALLOC
     e = hipHostMalloc((void **)&b->h_stage, b->stage_cap, hipHostMallocDefault);
@@ -3792,6 +3834,41 @@ extern "C" int tpgp_hip_batch_readout(tpgp_hip_batch *b, unsigned pauli, const i
                            (const ReadoutMap *)d, pauli);
     if ((e = hipGetLastError()) != hipSuccess)
         return hip_fail(e, "HIP readout launch", error, cap);
+    return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_sum_squares(tpgp_hip_batch *b, size_t map, double out[2],
+                                          char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!b || map >= b->M || !out) {
+        msg(error, cap, "Invalid sum-of-squares arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    HostMap &hm = b->maps[map];
+    if (hm.kcur != hm.kc || hm.nrec || hm.nrd) {
+        msg(error, cap, "Map %zu has uncommitted work; sync and commit or roll back first.", map);
+        return TPGP_HIP_EINVAL;
+    }
+    out[0] = out[1] = 0.0;
+    const uint64_t len = hm.committed_len;
+    if (!len) return TPGP_HIP_OK;
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    const double *vals = b->dev.arena[hm.kc].vals + hm.off;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(kSumBlocks, (len + 255) / 256);
+    hipLaunchKernelGGL(k_sumsq, dim3(grid), dim3(256), 0, b->ctx->stream, vals, len,
+                       b->dev.sumsq);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "HIP sum-of-squares launch", error, cap);
+    double part[2 * kSumBlocks];
+    e = hipMemcpyAsync(part, b->dev.sumsq, 2 * grid * sizeof(double), hipMemcpyDeviceToHost,
+                       b->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "HIP sum-of-squares transfer", error, cap);
+    double hi = 0.0, lo = 0.0;
+    for (uint32_t k = 0; k < grid; ++k) dd_add(hi, lo, part[2 * k], part[2 * k + 1]);
+    out[0] = hi;
+    out[1] = lo;
     return TPGP_HIP_OK;
 }
 

@@ -224,6 +224,24 @@ class Evolution:
         self.eng.tpgp_evolution_profile_read(self.ptr, kind, C.byref(ms), C.byref(n))
         return ms.value, n.value
 
+    def snapshot(self) -> Map:
+        """The last accepted map, copied out without closing the session."""
+        mp = Map(self.n_spins)
+        st = self.eng.tpgp_evolution_snapshot(self.ptr, C.byref(mp.m))
+        if st:
+            raise RuntimeError(f"tpgp_evolution_snapshot: status {st}: "
+                               f"{self.eng.tpgp_hip_last_error().decode()}")
+        return mp
+
+    def l2(self) -> float:
+        """l2 norm of the last accepted map (device reduction)."""
+        out = C.c_double()
+        st = self.eng.tpgp_evolution_l2(self.ptr, C.byref(out))
+        if st:
+            raise RuntimeError(f"tpgp_evolution_l2: status {st}: "
+                               f"{self.eng.tpgp_hip_last_error().decode()}")
+        return out.value
+
     def close(self, mp: Map | None = None):
         if self.ptr:
             st = self.eng.tpgp_evolution_close(self.ptr, C.byref(mp.m) if mp is not None else None)
@@ -234,6 +252,90 @@ class Evolution:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class Checkpoint:
+    """A TPGPRVCP version-2 restart record (tpgp_checkpoint_*; checkpoint_rev.h:7-40).
+
+    ``load`` returns a record that owns its arrays (freed with the object); ``of`` builds one
+    that borrows the caller's map and terms, for saving."""
+
+    def __init__(self):
+        self.eng = N.engine()
+        self.c = N.Checkpoint()
+        self._owned = False
+        self._keep = None
+
+    @classmethod
+    def load(cls, path):
+        ck = cls()
+        err = C.create_string_buffer(512)
+        st = ck.eng.tpgp_checkpoint_load(str(path).encode(), C.byref(ck.c), err, 512)
+        if st:
+            raise ValueError(f"checkpoint rejected: {err.value.decode()} (status {st})")
+        ck._owned = True
+        return ck
+
+    @classmethod
+    def of(cls, mp: Map, terms, opts, led, dt, step, total_steps, model="ring", j2=0.0,
+           linear_size=0, initial_global=True, pf_bound=0.0, residual_depth=-1,
+           residual_bound=float("nan"), max_fid_bound=0.0, resumed_from_step=0,
+           prior_total_steps=0, prior_max_terms=0, stop_reason=0, last_status=0):
+        ck = cls()
+        c = ck.c
+        ta = N.terms_array(terms) if not isinstance(terms, C.Array) else terms
+        ck._keep = (mp, ta)
+        c.map = mp.m
+        c.terms = C.cast(ta, C.POINTER(N.TpgpTerm))
+        c.term_count = len(terms)
+        c.options = opts
+        c.options.hip_context = None
+        c.ledger = led
+        c.dt, c.pf_bound, c.residual_bound, c.max_fid_bound = dt, pf_bound, residual_bound, \
+            max_fid_bound
+        c.step, c.total_steps = step, total_steps
+        c.residual_depth, c.initial_global = residual_depth, 1 if initial_global else 0
+        c.model = model.encode()
+        c.j2, c.linear_size = j2, linear_size
+        c.resumed_from_step, c.prior_total_steps = resumed_from_step, prior_total_steps
+        c.prior_max_terms, c.stop_reason, c.last_status = prior_max_terms, stop_reason, last_status
+        return ck
+
+    def save(self, path):
+        err = C.create_string_buffer(512)
+        st = self.eng.tpgp_checkpoint_save(str(path).encode(), C.byref(self.c), err, 512)
+        if st:
+            raise ValueError(f"checkpoint save failed: {err.value.decode()} (status {st})")
+
+    def same_configuration(self, other: "Checkpoint"):
+        return bool(self.eng.tpgp_checkpoint_same_configuration(C.byref(self.c),
+                                                                C.byref(other.c)))
+
+    def map(self) -> Map:
+        """A copy of the saved map."""
+        mp = Map(self.c.map.n_spins)
+        st = self.eng.tpgp_map_clone(C.byref(mp.m), C.byref(self.c.map))
+        if st:
+            raise MemoryError(f"tpgp_map_clone: status {st}")
+        return mp
+
+    def terms(self):
+        """The saved Hamiltonian as (i, j, qi, qj, h) tuples, in order."""
+        t = self.c.terms
+        return [(t[k].i, t[k].j, t[k].qi, t[k].qj, t[k].h) for k in range(self.c.term_count)]
+
+    def __getattr__(self, name):
+        if name in ("eng", "c", "_owned", "_keep"):
+            raise AttributeError(name)
+        return getattr(self.c, name)
+
+    def __del__(self):
+        try:
+            if self._owned:
+                self.eng.tpgp_checkpoint_free(C.byref(self.c))
+                self._owned = False
         except Exception:
             pass
 

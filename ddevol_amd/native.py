@@ -108,6 +108,18 @@ class HipLogEntry(C.Structure):
                 ("pad", C.c_uint32)]
 
 
+class Checkpoint(C.Structure):  # checkpoint_rev.h:7-23 layout
+    _fields_ = [("map", TpgpMap), ("terms", C.POINTER(TpgpTerm)), ("term_count", C.c_size_t),
+                ("options", TpgpOptions), ("ledger", TpgpLedger), ("dt", C.c_double),
+                ("pf_bound", C.c_double), ("residual_bound", C.c_double),
+                ("max_fid_bound", C.c_double), ("step", C.c_size_t),
+                ("total_steps", C.c_size_t), ("residual_depth", C.c_int),
+                ("initial_global", C.c_int), ("model", C.c_char * 32), ("j2", C.c_double),
+                ("linear_size", C.c_size_t), ("resumed_from_step", C.c_size_t),
+                ("prior_total_steps", C.c_size_t), ("prior_max_terms", C.c_size_t),
+                ("stop_reason", C.c_int), ("last_status", C.c_int)]
+
+
 HIP_SYMBOLS = ["tpgp_hip_compiled", "tpgp_hip_create", "tpgp_hip_destroy",
                "tpgp_hip_device_name", "tpgp_hip_branches", "tpgp_hip_batch_create",
                "tpgp_hip_batch_destroy", "tpgp_hip_batch_upload", "tpgp_hip_batch_download",
@@ -115,7 +127,8 @@ HIP_SYMBOLS = ["tpgp_hip_compiled", "tpgp_hip_create", "tpgp_hip_destroy",
                "tpgp_hip_batch_commit", "tpgp_hip_batch_rollback", "tpgp_hip_batch_reserve",
                "tpgp_hip_batch_info", "tpgp_hip_batch_stream", "tpgp_hip_batch_profile",
                "tpgp_hip_batch_profile_read", "tpgp_hip_batch_assign",
-               "tpgp_hip_batch_reserve_log", "tpgp_hip_batch_reserve_many"]
+               "tpgp_hip_batch_reserve_log", "tpgp_hip_batch_reserve_many",
+               "tpgp_hip_batch_sum_squares"]
 ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_default_options", "tpgp_ledger_init", "tpgp_map_init", "tpgp_map_free",
                   "tpgp_map_clone", "tpgp_key_get", "tpgp_key_set", "tpgp_map_add",
@@ -128,7 +141,10 @@ ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_evolution_profile_read", "tpgp_evolution_key_words",
                   "tpgp_fleet_open", "tpgp_fleet_close", "tpgp_fleet_admit", "tpgp_fleet_step",
                   "tpgp_fleet_slot", "tpgp_fleet_fid", "tpgp_fleet_release",
-                  "tpgp_fleet_counters_get", "tpgp_fleet_slot_bytes"]
+                  "tpgp_fleet_counters_get", "tpgp_fleet_slot_bytes",
+                  "tpgp_checkpoint_free", "tpgp_checkpoint_save", "tpgp_checkpoint_load",
+                  "tpgp_checkpoint_same_configuration", "tpgp_payload_sha256",
+                  "tpgp_evolution_snapshot", "tpgp_evolution_l2"]
 
 
 def hip():
@@ -171,6 +187,7 @@ def hip():
     f("tpgp_hip_batch_assign", C.c_int, C.c_void_p, sz, sz, p_u64, p_d, sz, p_c, sz)
     f("tpgp_hip_batch_reserve_log", C.c_int, C.c_void_p, sz, p_c, sz)
     f("tpgp_hip_batch_reserve_many", C.c_int, C.c_void_p, C.POINTER(sz), p_c, sz)
+    f("tpgp_hip_batch_sum_squares", C.c_int, C.c_void_p, sz, p_d, p_c, sz)
     f("tpgp_hip_batch_profile", C.c_int, C.c_void_p, C.c_uint)
     f("tpgp_hip_batch_profile_read", C.c_int, C.c_void_p, C.c_uint, C.POINTER(C.c_double),
       C.POINTER(C.c_uint64))
@@ -241,6 +258,14 @@ def engine():
     f("tpgp_fleet_release", C.c_int, vp, sz, mp)
     f("tpgp_fleet_counters_get", None, vp, C.POINTER(EvolutionCounters))
     f("tpgp_fleet_slot_bytes", C.c_uint64, vp, sz)
+    cp = C.POINTER(Checkpoint)
+    f("tpgp_checkpoint_free", None, cp)
+    f("tpgp_checkpoint_save", C.c_int, C.c_char_p, cp, C.c_char_p, sz)
+    f("tpgp_checkpoint_load", C.c_int, C.c_char_p, cp, C.c_char_p, sz)
+    f("tpgp_checkpoint_same_configuration", C.c_int, cp, cp)
+    f("tpgp_payload_sha256", None, C.c_void_p, sz, C.POINTER(C.c_ubyte))
+    f("tpgp_evolution_snapshot", C.c_int, vp, mp)
+    f("tpgp_evolution_l2", C.c_int, vp, C.POINTER(C.c_double))
     _eng = L
     return L
 

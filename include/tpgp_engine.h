@@ -181,6 +181,43 @@ void tpgp_fleet_counters_get(const tpgp_fleet *fleet, tpgp_evolution_counters *c
 /* Bytes of device memory the fleet's slots hold (three term buffers per slot). */
 uint64_t tpgp_fleet_slot_bytes(const tpgp_fleet *fleet, size_t slot);
 
+/* ---- Checkpoint / restart (the reference's TPGPRVCP version-2 record) --------------------
+ * Same record as engine/include/checkpoint_rev.h:7-40 (field order, little-endian encoding,
+ * SHA-256 of the payload appended, exclusive temporary file + fsync + atomic rename + parent
+ * directory fsync, companion .manifest.json), so a run checkpointed here resumes in the
+ * reference's CLI and the other way round.  Context pointers are never serialised. */
+typedef struct {
+    tpgp_map map;
+    tpgp_term *terms;
+    size_t term_count;
+    tpgp_options options;
+    tpgp_ledger ledger;
+    double dt, pf_bound, residual_bound, max_fid_bound;
+    size_t step, total_steps;
+    int residual_depth; /* -1: not requested (residual_bound is then NaN) */
+    int initial_global;
+    char model[32];
+    double j2;
+    size_t linear_size;
+    size_t resumed_from_step, prior_total_steps, prior_max_terms;
+    int stop_reason; /* 0 running, 1 complete, 2 walltime, 3 signal, 4 resource, 5 numerical,
+                        6 accuracy */
+    tpgp_status last_status;
+} tpgp_checkpoint;
+void tpgp_checkpoint_free(tpgp_checkpoint *checkpoint);
+tpgp_status tpgp_checkpoint_save(const char *path, const tpgp_checkpoint *checkpoint, char *error,
+                                 size_t error_capacity);
+tpgp_status tpgp_checkpoint_load(const char *path, tpgp_checkpoint *destination, char *error,
+                                 size_t error_capacity);
+int tpgp_checkpoint_same_configuration(const tpgp_checkpoint *a, const tpgp_checkpoint *b);
+/* SHA-256 of a buffer (the checksum the record carries). */
+void tpgp_payload_sha256(const void *data, size_t bytes, unsigned char digest[32]);
+/* Copy a resident session's last accepted map out (reference layout) without closing it. */
+tpgp_status tpgp_evolution_snapshot(tpgp_evolution *evolution, tpgp_map *map);
+/* l2 norm of the last accepted map, computed on the device: sqrt of the sum of squares, as
+ * tpgp_map_l2 (tpgp_rev.c:266-271) with an accumulation that is exact to ~2^-104 relative. */
+tpgp_status tpgp_evolution_l2(tpgp_evolution *evolution, double *l2);
+
 /* Message of the last device-side failure on this thread ("" when none). */
 const char *tpgp_hip_last_error(void);
 

@@ -136,6 +136,11 @@ int tpgp_hip_batch_rotate_many(tpgp_hip_batch *batch, const tpgp_hip_rotation *r
  * map with active[m] != 0 (NULL = all), in the current working state.  Asynchronous. */
 int tpgp_hip_batch_readout(tpgp_hip_batch *batch, unsigned pauli, const int *active,
                            char *error, size_t error_capacity);
+/* Sum of squares of map `map`'s committed values as an unevaluated pair out[0] + out[1]
+ * (error-free double-double accumulation, relative error ~len * 2^-104).  Synchronous; the map
+ * must have no uncommitted work.  Feeds the l2 norm of tpgp_map_l2 (tpgp_rev.c:266-271). */
+int tpgp_hip_batch_sum_squares(tpgp_hip_batch *batch, size_t map, double out[2], char *error,
+                               size_t error_capacity);
 /* Wait for enqueued work.  Returns the number of records per map since the previous sync in
  * *n_records (records[m*max_records + r], r < n_records), readouts in readouts[m*max_readouts*64
  * + q*64 + s] with *n_readouts, and working lengths in lengths[m].  Any output pointer may be
