@@ -37,6 +37,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public specification
+# HBM bytes per launch of k_rot_write over the default window (warmup 600, 40 timed steps),
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings (factors 2.00 and
+# 1.00): profiles/k_rot_write_j1j2_n16_window_pmc.md.  Used for roofline.traffic when the run
+# is that window and no --traffic is given; other windows report null.
+MEASURED_TRAFFIC = {("k_rot_write", 600, 40): 669.6e6}
 
 
 def log(*a):
@@ -434,7 +439,8 @@ def main():
                    "fid_at_end": float(fid[-1])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": args.traffic,
+                     "traffic": args.traffic if args.traffic is not None
+                     else MEASURED_TRAFFIC.get((dom, args.warmup, args.steps)),
                      "kernel": dom, "launches": dom_n,
                      "avg_us": dom_ms / max(dom_n, 1) * 1e3,
                      "algorithmic_bytes_per_launch": dom_bytes / max(dom_n, 1),
