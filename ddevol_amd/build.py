@@ -62,8 +62,22 @@ def build(verbose: bool = False) -> None:
     os.makedirs(LIB, exist_ok=True)
     hip_so = os.path.join(LIB, "libtpgp_hip.so")
     tmp = hip_so + ".tmp"
-    run([hipcc(), *HIPCC_FLAGS, "-o", tmp, os.path.join(CSRC, "tpgp_hip.hip")])
+    # the device units compile in parallel, then link into one library
+    units = ["tpgp_hip", "tpgp_generator"]
+    objs = [os.path.join(LIB, u + ".o") for u in units]
+    flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    procs = [(subprocess.Popen([hipcc(), *flags, "-c", "-o", o, os.path.join(CSRC, u + ".hip")],
+                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True), u)
+             for u, o in zip(units, objs)]
+    for p, u in procs:
+        _, err = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write("\n".join(l for l in err.splitlines() if "packed-fp32-ops" not in l))
+            raise RuntimeError(f"build failed: hipcc {u}.hip")
+    run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, hip_so)
+    for o in objs:
+        os.remove(o)
     eng_so = os.path.join(LIB, "libtpgp_engine.so")
     cmd = ["gcc", *CC_FLAGS]
     pre = mpfr_prefix()
@@ -74,7 +88,7 @@ def build(verbose: bool = False) -> None:
     tmp = eng_so + ".tmp"
     cmd += ["-o", tmp, os.path.join(CSRC, "tpgp_engine.c"),
             os.path.join(CSRC, "tpgp_checkpoint.c"), f"-L{LIB}", "-ltpgp_hip",
-            "-Wl,-rpath,$ORIGIN", *libs, "-lm"]
+            "-Wl,-rpath,$ORIGIN", *libs, "-lm", "-lpthread"]
     run(cmd)
     os.replace(tmp, eng_so)
     cli = os.path.join(LIB, "tpgp_rev")

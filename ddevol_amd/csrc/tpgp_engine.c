@@ -12,6 +12,7 @@
 #include "../../include/tpgp_hip.h"
 
 #include <fenv.h>
+#include <pthread.h>
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
@@ -931,6 +932,58 @@ tpgp_status tpgp_evolution_close(tpgp_evolution *ev, tpgp_map *m) {
     free(ev->plans);
     free(ev);
     return s;
+}
+
+/* The device context of calls without options (tpgp_liouvillian): created on first use on
+ * device 0 and kept for the life of the process. */
+static pthread_once_t default_once = PTHREAD_ONCE_INIT;
+static tpgp_hip_context *default_ctx;
+static char default_err[512];
+static void default_create(void) {
+    default_ctx = tpgp_hip_create(0, default_err, sizeof(default_err));
+}
+
+tpgp_status tpgp_liouvillian(tpgp_map *out, const tpgp_map *in, const tpgp_term *terms,
+                             size_t n) {
+    if (!out || !in || (!terms && n) || !in->n_spins || in->n_spins > 64 ||
+        in->n_words != (in->n_spins + 31) / 32 || (in->len && (!in->keys || !in->values)))
+        return TPGP_EINVAL;
+    pthread_once(&default_once, default_create);
+    if (!default_ctx) {
+        note("context", default_err);
+        return TPGP_EBACKEND;
+    }
+    tpgp_map t;
+    tpgp_status s = tpgp_map_init(&t, in->n_spins);
+    if (s) return s;
+    /* tpgp_hip_term has tpgp_term's layout */
+    const tpgp_hip_term *ht = (const tpgp_hip_term *)terms;
+    size_t cap = 4 * in->len + 1024, got = 0;
+    char err[512];
+    for (int attempt = 0; attempt < 2; attempt++) {
+        if ((s = map_reserve(&t, cap))) {
+            tpgp_map_free(&t);
+            return s;
+        }
+        const int hs = tpgp_hip_liouvillian(default_ctx, in->keys, in->values, in->len,
+                                            in->n_words, in->n_spins, ht, n, t.keys, t.values,
+                                            t.capacity, &got, err, sizeof(err));
+        if (hs == TPGP_HIP_ECAPACITY && attempt == 0) {
+            cap = got;
+            continue;
+        }
+        if (hs) {
+            note("liouvillian", err);
+            tpgp_map_free(&t);
+            return hs == TPGP_HIP_ENUMERIC ? TPGP_ENUMERIC : map_status(hs);
+        }
+        t.len = got;
+        tpgp_map_free(out);
+        *out = t;
+        return TPGP_OK;
+    }
+    tpgp_map_free(&t);
+    return TPGP_EBACKEND;
 }
 
 tpgp_status tpgp_evolution_snapshot(tpgp_evolution *ev, tpgp_map *m) {

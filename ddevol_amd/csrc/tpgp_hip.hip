@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "../../include/tpgp_hip.h"
+#include "tpgp_hip_context.h"
 
 // ------------------------------------------------------------------------------------------
 // Scalar device helpers (same semantics as tpgp_rev.c:25-42 and cuda_rev.cu:50-76)
@@ -2558,28 +2559,7 @@ __global__ void k_set_len(uint64_t *len, uint64_t idx, uint64_t v) { len[idx] = 
 // ==========================================================================================
 // Host side
 // ==========================================================================================
-struct tpgp_hip_context {
-    int device;
-    int cus;
-    hipStream_t stream;
-    void *storage;
-    size_t capacity;
-    std::mutex mutex;
-    char name[256];
-};
-
 namespace {
-void msg(char *e, size_t n, const char *fmt, ...) {
-    if (!e || !n) return;
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(e, n, fmt, ap);
-    va_end(ap);
-}
-tpgp_hip_status hip_fail(hipError_t code, const char *what, char *e, size_t n) {
-    msg(e, n, "%s: %s", what, hipGetErrorString(code));
-    return code == hipErrorOutOfMemory ? TPGP_HIP_ENOMEM : TPGP_HIP_EDEVICE;
-}
 bool mulz(size_t a, size_t b, size_t *o) {
     if (b && a > SIZE_MAX / b) return false;
     *o = a * b;
@@ -2595,16 +2575,6 @@ bool overlap(const void *a, size_t as, const void *b, size_t bs) {
     if (as > UINTPTR_MAX - x || bs > UINTPTR_MAX - y) return true;
     return x < y + bs && y < x + as;
 }
-struct DeviceGuard {  // save/restore the caller's current device
-    int previous = -1;
-    bool ok = false;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&previous) == hipSuccess && hipSetDevice(dev) == hipSuccess) ok = true;
-    }
-    ~DeviceGuard() {
-        if (previous >= 0) (void)hipSetDevice(previous);
-    }
-};
 }  // namespace
 
 extern "C" int tpgp_hip_compiled(void) { return 1; }

@@ -1,0 +1,48 @@
+// Internal to libtpgp_hip: the device context and the host-side helpers shared by its
+// translation units (the resident engine in tpgp_hip.hip, the generator scan in
+// tpgp_generator.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <mutex>
+
+#include "../../include/tpgp_hip.h"
+
+// One device, its stream and its persistent scratch; calls on a context are serialised by the
+// mutex (as the reference's CUDA context, cuda_rev.cu:22).
+struct tpgp_hip_context {
+    int device;
+    int cus;
+    hipStream_t stream;
+    void *storage;
+    size_t capacity;
+    std::mutex mutex;
+    char name[256];
+};
+
+namespace {
+// printf into a caller's error buffer (NULL or zero capacity: nothing)
+inline void msg(char *e, size_t n, const char *fmt, ...) {
+    if (!e || !n) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(e, n, fmt, ap);
+    va_end(ap);
+}
+inline tpgp_hip_status hip_fail(hipError_t code, const char *what, char *e, size_t n) {
+    msg(e, n, "%s: %s", what, hipGetErrorString(code));
+    return code == hipErrorOutOfMemory ? TPGP_HIP_ENOMEM : TPGP_HIP_EDEVICE;
+}
+struct DeviceGuard {  // save/restore the caller's current device
+    int previous = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&previous) == hipSuccess && hipSetDevice(dev) == hipSuccess) ok = true;
+    }
+    ~DeviceGuard() {
+        if (previous >= 0) (void)hipSetDevice(previous);
+    }
+};
+}  // namespace

@@ -163,6 +163,14 @@ def strang_step(mp: Map, terms_arr, count, dt, opts, led):
                                        C.byref(led))
 
 
+def liouvillian(mp: Map, terms):
+    """-i[H, A] on the GPU (tpgp_liouvillian); returns (status, Map)."""
+    out = Map(mp.n_spins)
+    ta = N.terms_array(terms)
+    st = N.engine().tpgp_liouvillian(C.byref(out.m), C.byref(mp.m), ta, len(terms))
+    return st, out
+
+
 def evolve(mp: Map, terms, dt, steps, opts, led, exact_ledger=False, pauli=1, normalizer=None,
            initial_capacity=0):
     """Resident evolution (tpgp_hip_evolve); returns (status, fid, fid_radius, active, accepted)."""

@@ -128,7 +128,7 @@ HIP_SYMBOLS = ["tpgp_hip_compiled", "tpgp_hip_create", "tpgp_hip_destroy",
                "tpgp_hip_batch_info", "tpgp_hip_batch_stream", "tpgp_hip_batch_profile",
                "tpgp_hip_batch_profile_read", "tpgp_hip_batch_assign",
                "tpgp_hip_batch_reserve_log", "tpgp_hip_batch_reserve_many",
-               "tpgp_hip_batch_sum_squares"]
+               "tpgp_hip_batch_sum_squares", "tpgp_hip_liouvillian"]
 ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_default_options", "tpgp_ledger_init", "tpgp_map_init", "tpgp_map_free",
                   "tpgp_map_clone", "tpgp_key_get", "tpgp_key_set", "tpgp_map_add",
@@ -144,7 +144,7 @@ ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_fleet_counters_get", "tpgp_fleet_slot_bytes",
                   "tpgp_checkpoint_free", "tpgp_checkpoint_save", "tpgp_checkpoint_load",
                   "tpgp_checkpoint_same_configuration", "tpgp_payload_sha256",
-                  "tpgp_evolution_snapshot", "tpgp_evolution_l2"]
+                  "tpgp_evolution_snapshot", "tpgp_evolution_l2", "tpgp_liouvillian"]
 
 
 def hip():
@@ -188,6 +188,8 @@ def hip():
     f("tpgp_hip_batch_reserve_log", C.c_int, C.c_void_p, sz, p_c, sz)
     f("tpgp_hip_batch_reserve_many", C.c_int, C.c_void_p, C.POINTER(sz), p_c, sz)
     f("tpgp_hip_batch_sum_squares", C.c_int, C.c_void_p, sz, p_d, p_c, sz)
+    f("tpgp_hip_liouvillian", C.c_int, C.c_void_p, p_u64, p_d, sz, sz, sz, C.POINTER(TpgpTerm),
+      sz, p_u64, p_d, sz, C.POINTER(sz), p_c, sz)
     f("tpgp_hip_batch_profile", C.c_int, C.c_void_p, C.c_uint)
     f("tpgp_hip_batch_profile_read", C.c_int, C.c_void_p, C.c_uint, C.POINTER(C.c_double),
       C.POINTER(C.c_uint64))
@@ -266,6 +268,7 @@ def engine():
     f("tpgp_payload_sha256", None, C.c_void_p, sz, C.POINTER(C.c_ubyte))
     f("tpgp_evolution_snapshot", C.c_int, vp, mp)
     f("tpgp_evolution_l2", C.c_int, vp, C.POINTER(C.c_double))
+    f("tpgp_liouvillian", C.c_int, mp, mp, C.POINTER(TpgpTerm), sz)
     _eng = L
     return L
 

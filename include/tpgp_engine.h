@@ -181,6 +181,12 @@ void tpgp_fleet_counters_get(const tpgp_fleet *fleet, tpgp_evolution_counters *c
 /* Bytes of device memory the fleet's slots hold (three term buffers per slot). */
 uint64_t tpgp_fleet_slot_bytes(const tpgp_fleet *fleet, size_t slot);
 
+/* -i[H, A] in the Pauli basis, computed on the GPU (default context on device 0), bit for bit
+ * as the reference's tpgp_liouvillian (tpgp_rev.h:89, tpgp_rev.c:810-857); replaces *out on
+ * success, leaves it unchanged on failure.  At most 64 spins. */
+tpgp_status tpgp_liouvillian(tpgp_map *out, const tpgp_map *in, const tpgp_term *terms,
+                             size_t count);
+
 /* ---- Checkpoint / restart (the reference's TPGPRVCP version-2 record) --------------------
  * Same record as engine/include/checkpoint_rev.h:7-40 (field order, little-endian encoding,
  * SHA-256 of the payload appended, exclusive temporary file + fsync + atomic rename + parent

@@ -136,6 +136,26 @@ int tpgp_hip_batch_rotate_many(tpgp_hip_batch *batch, const tpgp_hip_rotation *r
  * map with active[m] != 0 (NULL = all), in the current working state.  Asynchronous. */
 int tpgp_hip_batch_readout(tpgp_hip_batch *batch, unsigned pauli, const int *active,
                            char *error, size_t error_capacity);
+/* ---- Part 3: the Liouvillian on the device ----------------------------------------------
+ * A Hamiltonian term, same layout as tpgp_term (tpgp_rev.h:30-35). */
+typedef struct {
+    size_t i, j;
+    unsigned qi, qj;
+    double h;
+    size_t layer;
+} tpgp_hip_term;
+/* -i[H, A] of the map (keys, values, count) with H = sum_t h_t Q_t, bit for bit as
+ * tpgp_liouvillian (tpgp_rev.c:810-857): sorted unique keys with nonzero values into
+ * out_keys / out_values and the length into *out_count.  n_spins <= 64.  A result longer than
+ * out_capacity returns TPGP_HIP_ECAPACITY with the length in *out_count and writes nothing.
+ * ENUMERIC: a non-finite contribution or partial sum; EINVAL: an invalid term (whichever the
+ * reference meets first in term order).  Synchronous; device memory is allocated per call. */
+int tpgp_hip_liouvillian(tpgp_hip_context *context, const uint64_t *keys, const double *values,
+                         size_t count, size_t n_words, size_t n_spins,
+                         const tpgp_hip_term *terms, size_t n_terms, uint64_t *out_keys,
+                         double *out_values, size_t out_capacity, size_t *out_count,
+                         char *error, size_t error_capacity);
+
 /* Sum of squares of map `map`'s committed values as an unevaluated pair out[0] + out[1]
  * (error-free double-double accumulation, relative error ~len * 2^-104).  Synchronous; the map
  * must have no uncommitted work.  Feeds the l2 norm of tpgp_map_l2 (tpgp_rev.c:266-271). */

@@ -125,6 +125,7 @@ class CpuEngine:
         fn("strang_bound", C.c_double, C.POINTER(Term), C.c_size_t, C.c_double)
         fn("bound_add_up", C.c_double, C.c_double, C.c_double)
         fn("bound_fid_up", C.c_double, C.c_double, C.c_size_t)
+        fn("liouvillian", C.c_int, mp, mp, C.POINTER(Term), C.c_size_t)
         if kind == "oracle":
             fn("map_new", mp, C.c_size_t)
             fn("map_delete", None, mp)
@@ -243,3 +244,9 @@ class CpuMap:
 
     def l2(self):
         return self.eng.map_l2(self.ptr)
+
+    def liouvillian(self, terms_arr, count) -> tuple:
+        """-i[H, A]: (status, result map)."""
+        out = self.eng.new_map(self.n_spins)
+        st = self.eng.liouvillian(out.ptr, self.ptr, terms_arr, count)
+        return st, out

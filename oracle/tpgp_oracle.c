@@ -791,3 +791,107 @@ double orc_bound_fid_up(double a, size_t n) {
     return NAN;
 #endif
 }
+
+/* ---- Liouvillian -i[H, A]: tpgp_rev.c:810-857 ---------------------------------------------
+ * Every term q (sites i, j) and key P that anticommute contribute v = (2 h x)(+-1) to the key
+ * q xor P (sign + when the phase of qP is 1), x the coefficient of P.  Restated as: emit all
+ * contributions in term-major, key-minor order, stable-sort them by key, and sum each key's run
+ * left to right -- the same additions in the same order as the reference's sequential inserts
+ * (a run whose sum is exactly zero leaves no key; an intermediate zero restarts from zero
+ * exactly as a removal followed by a fresh insert does).  Terms are processed up to the first
+ * invalid one; any non-finite contribution or partial sum among them gives ENUMERIC, else the
+ * invalid term gives EINVAL -- the reference's status order. */
+typedef struct {
+    const uint64_t *key; /* into the contribution key array */
+    size_t order;
+} liou_ref;
+static size_t liou_words;
+static int liou_cmp(const void *a, const void *b) {
+    const liou_ref *x = a, *y = b;
+    int c = key_order(x->key, y->key, liou_words);
+    if (c) return c;
+    return (x->order > y->order) - (x->order < y->order);
+}
+int orc_liouvillian(orc_map *out, const orc_map *in, const orc_term *terms, size_t n) {
+    if (!out || !in || (!terms && n) || !map_ok(in)) return ORC_EINVAL;
+    const size_t W = in->n_words, N = in->n_spins;
+    size_t valid = 0;
+    while (valid < n) {
+        const orc_term *q = terms + valid;
+        if (q->i >= N || q->j >= N || q->i == q->j || q->qi > 3 || q->qj > 3 || !isfinite(q->h))
+            break;
+        valid++;
+    }
+    size_t count = 0;
+    for (size_t t = 0; t < valid; t++)
+        for (size_t k = 0; k < in->len; k++) {
+            const uint64_t *p = in->keys + k * W;
+            count += anticommutes(terms[t].qi, symbol_at(p, terms[t].i)) !=
+                     anticommutes(terms[t].qj, symbol_at(p, terms[t].j));
+        }
+    uint64_t *ck = malloc((count ? count : 1) * W * sizeof(uint64_t));
+    double *cv = malloc((count ? count : 1) * sizeof(double));
+    liou_ref *ref = malloc((count ? count : 1) * sizeof(liou_ref));
+    if (!ck || !cv || !ref) {
+        free(ck);
+        free(cv);
+        free(ref);
+        return ORC_ENOMEM;
+    }
+    int numeric_failure = 0;
+    size_t at = 0;
+    for (size_t t = 0; t < valid; t++) {
+        const orc_term *q = terms + t;
+        for (size_t k = 0; k < in->len; k++) {
+            const uint64_t *p = in->keys + k * W;
+            const unsigned a = symbol_at(p, q->i), b = symbol_at(p, q->j);
+            if (anticommutes(q->qi, a) == anticommutes(q->qj, b)) continue;
+            const unsigned phase = (orc_pauli_phase(q->qi, a) + orc_pauli_phase(q->qj, b)) & 3u;
+            const double v = 2 * q->h * in->values[k] * (phase == 1 ? 1 : -1);
+            if (!isfinite(v)) numeric_failure = 1;
+            uint64_t *dst = ck + at * W;
+            memcpy(dst, p, W * sizeof(uint64_t));
+            dst[q->i / 32] ^= (uint64_t)q->qi << (2 * (q->i % 32));
+            dst[q->j / 32] ^= (uint64_t)q->qj << (2 * (q->j % 32));
+            cv[at] = v;
+            ref[at].key = dst;
+            ref[at].order = at;
+            at++;
+        }
+    }
+    liou_words = W;
+    qsort(ref, count, sizeof(liou_ref), liou_cmp);
+    orc_map *tmp = orc_map_new(N);
+    int s = tmp ? ORC_OK : ORC_ENOMEM;
+    for (size_t r = 0; !s && !numeric_failure && r < count;) {
+        size_t e = r + 1;
+        while (e < count && key_order(ref[e].key, ref[r].key, W) == 0) e++;
+        double sum = 0;
+        for (size_t x = r; x < e; x++) {
+            sum = sum + cv[ref[x].order];
+            if (!isfinite(sum)) numeric_failure = 1;
+        }
+        if (!numeric_failure && sum != 0) {
+            s = grow(tmp, tmp->len + 1);
+            if (!s) {
+                memcpy(tmp->keys + tmp->len * W, ref[r].key, W * sizeof(uint64_t));
+                tmp->values[tmp->len++] = sum;
+            }
+        }
+        r = e;
+    }
+    free(ck);
+    free(cv);
+    free(ref);
+    if (!s && numeric_failure) s = ORC_ENUMERIC;
+    if (!s && valid < n) s = ORC_EINVAL;
+    if (s) {
+        orc_map_delete(tmp);
+        return s;
+    }
+    free(out->keys);
+    free(out->values);
+    *out = *tmp;
+    free(tmp);
+    return ORC_OK;
+}

@@ -82,6 +82,8 @@ double orc_strang_bound(const orc_term *terms, size_t count, double dt);
 double orc_bound_add_up(double a, double b);
 double orc_bound_fid_up(double a, size_t n);
 /* Directed-rounding helpers exposed so tests can check them against their definitions. */
+/* -i[H, A] in the Pauli basis (tpgp_rev.c:810-857); replaces *out on success. */
+int orc_liouvillian(orc_map *out, const orc_map *in, const orc_term *terms, size_t count);
 double orc_upadd(double a, double b);
 double orc_upmul(double a, double b);
 double orc_round_radius(double x);
