@@ -12,8 +12,8 @@ script runs it on small FID runs and records what ddevol_amd/lib/tpgp_rev must r
   (restart chains, failure and accuracy stops);
 * j1j2_n16.edges: the coupling file of the J1-J2 N=16 case (J2_i from InputPCG(7)).
 
-Cases: every model (ring, j1j2 validated, ising, xy, cubic L=3, edges), the local initial
-state, a drop-budget failure, a restart with a longer horizon, a cap failure resumed with a
+Cases: every model (ring, j1j2 validated, ising, xy, cubic L=3, edges), two-word keys (ring
+N=40, validated J1-J2 N=36), the local initial state, a drop-budget failure, a restart with a longer horizon, a cap failure resumed with a
 larger cap, an accuracy stop, and a walltime pause resumed to completion.
 
 Run in the build container (where the reference sources exist):
@@ -50,6 +50,9 @@ CASES = {
     "edges16": [["--model", "edges", "--n", "16", "--edges", "{edges}", "--dt", "0.005",
                  "--time", "0.05", "--threshold", "1e-5"]],
     "local8": [["--n", "8", "--initial", "local", "--time", "0.05", "--threshold", "1e-6"]],
+    "ring40": [["--n", "40", "--time", "0.03", "--threshold", "1e-4"]],
+    "j1j2_36_validated": [["--model", "j1j2", "--n", "36", "--time", "0.02", "--threshold",
+                           "1e-4", "--validated"]],
     "budget": [["--n", "10", "--time", "0.1", "--threshold", "1e-5", "--drop-budget", "2e-3",
                 "--checkpoint", "{ck}"]],
     "restart": [["--n", "10", "--time", "0.06", "--threshold", "1e-5", "--checkpoint", "{ck}",
