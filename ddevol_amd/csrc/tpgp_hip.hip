@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap 
 // Per tile: the histogram of the 16 new classes (digits pa, pb).  All of a thread's key loads
 // are issued before the first histogram update.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap *pm) {
+__global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ unsigned hist[16];
     const uint32_t M = b.n_maps;
     const uint64_t ntiles = b.tstart[M];
@@ -455,17 +455,18 @@ __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap
         uint64_t e0 = (g - b.tstart[m]) * kPartTile;
         uint64_t e1 = e0 + kPartTile < len ? e0 + kPartTile : len;
         const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[m].off * W;
+        const uint32_t pa = P.pa, pb = P.pb;  // read once, not after every LDS update
         Key<W> k[kPartRounds];
 #pragma unroll
-        for (int r = 0; r < kPartRounds; ++r) {
+        for (int r = 0; r < kPartRounds; ++r) {  // branch-free: past the end, reload the last
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
-            if (e < e1) k[r] = kload<W>(keys + e * W);
+            k[r] = kload<W>(keys + (e < e1 ? e : e1 - 1) * W);
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < kPartRounds; ++r) {
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
-            if (e < e1) atomicAdd(&hist[(kdigit<W>(k[r], P.pa) << 2) | kdigit<W>(k[r], P.pb)], 1u);
+            if (e < e1) atomicAdd(&hist[(kdigit<W>(k[r], pa) << 2) | kdigit<W>(k[r], pb)], 1u);
         }
         __syncthreads();
         if (threadIdx.x < 16) b.tile_hist[g * 16 + threadIdx.x] = hist[threadIdx.x];
@@ -500,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_part_scan(BatchDev b, const PassMap 
 // register use low for the many-tile launches.  The map's first tile also writes the output
 // regions and length.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassMap *pm) {
+__global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ unsigned wave_cnt[4][16];
     __shared__ uint64_t run[16];
     const uint32_t M = b.n_maps;
@@ -516,15 +517,16 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassM
         const double *vals = b.arena[P.in_buf].vals + b.slot[m].off;
         uint64_t *okeys = b.arena[P.out_buf].keys + b.slot[m].off * W;
         double *ovals = b.arena[P.out_buf].vals + b.slot[m].off;
+        // read once: the rounds' global stores could alias the descriptor for the compiler
+        const uint32_t pa = P.pa, pb = P.pb;
         Key<W> kk[kPartRounds];
         double vv[kPartRounds];
 #pragma unroll
-        for (int r = 0; r < kPartRounds; ++r) {
+        for (int r = 0; r < kPartRounds; ++r) {  // branch-free: past the end, reload the last
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
-            if (e < e1) {
-                kk[r] = kload<W>(keys + e * W);
-                vv[r] = vals[e];
-            }
+            const uint64_t ec = e < e1 ? e : e1 - 1;
+            kk[r] = kload<W>(keys + ec * W);
+            vv[r] = vals[ec];
         }
         if (threadIdx.x < 16) {
             uint64_t base = 0;
@@ -545,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassM
         for (int r = 0; r < kPartRounds; ++r) {
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
             const bool live = e < e1;
-            const unsigned cls = live ? (kdigit<W>(kk[r], P.pa) << 2) | kdigit<W>(kk[r], P.pb) : 16u;
+            const unsigned cls = live ? (kdigit<W>(kk[r], pa) << 2) | kdigit<W>(kk[r], pb) : 16u;
             unsigned below = 0;
             const uint64_t lanemask = (1ull << lane_id()) - 1ull;
             for (unsigned c = 0; c < 16; ++c) {
@@ -557,7 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassM
             if (live) {
                 uint64_t dst = run[cls] + below;
                 for (unsigned w = 0; w < wave_id(); ++w) dst += wave_cnt[w][cls];
-                kstore<W>(okeys + dst * W, move_to_top<W>(kk[r], P.pa, P.pb, n));
+                kstore<W>(okeys + dst * W, move_to_top<W>(kk[r], pa, pb, n));
                 ovals[dst] = vv[r];
             }
             __syncthreads();
@@ -721,13 +723,11 @@ __device__ void load_tile(const BatchDev &b, const PassMap &P, const MergeCtx &x
     Key<W> kk[R];
     double vv[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const uint32_t e = threadIdx.x + r * kBlock;
-        if (e < tot) {
-            const uint64_t src = e < na ? abase + e : dbase + e;
-            kk[r] = kload<W>(ikeys + src * W);
-            vv[r] = ivals[src];
-        }
+    for (int r = 0; r < R; ++r) {  // branch-free (tot >= 1): past the end, reload the last
+        const uint32_t e0 = threadIdx.x + r * kBlock, e = e0 < tot ? e0 : tot - 1;
+        const uint64_t src = e < na ? abase + e : dbase + e;
+        kk[r] = kload<W>(ikeys + src * W);
+        vv[r] = ivals[src];
     }
     Key<W> knext, kprev;
     double vnext = 0.0;
@@ -923,7 +923,7 @@ __device__ uint64_t split_search8(const uint64_t *keys, const MergeCtx &x, const
 // Pass 0 of a rotation: every tile's descriptor and merge-path start split, 8 lanes per tile
 // (32 tiles per workgroup, all searching at once so the dependent loads overlap).
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap *pm) {
+__global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap *__restrict__ pm) {
     const uint64_t ntiles = b.tstart[b.n_maps * 16];
     const unsigned grp = lane_id() >> 3, sub = lane_id() & 7u;
     const uint64_t per_block = kBlock / 8;
@@ -980,7 +980,7 @@ __device__ __forceinline__ void tile_from_desc(const BatchDev &b, uint64_t g, Me
 // Pass 1 of a rotation: per tile, the merge-path splits, the number of kept outputs, the
 // ledger partial sums (deterministic upward-rounded tree) and, when asked, the log.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap *pm) {
+__global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ TileIn<W> in;
     __shared__ TileMisc misc;
     const uint64_t ntiles = b.tstart[b.n_maps * 16];
@@ -1223,7 +1223,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
 // Pass 2 of a rotation: per tile, merge again and write the kept outputs at the tile's offset
 // (compacted in LDS first, so global writes are contiguous).
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap *pm) {
+__global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ TileIn<W> in;
     __shared__ uint64_t okeys[kMergeTile * W];
     __shared__ double ovals[kMergeTile];
@@ -1254,12 +1254,11 @@ __global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap 
             Key<W> kk[R];
             double vv[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const uint64_t i = threadIdx.x + (uint64_t)r * kBlock;
-                if (i < count) {
-                    kk[r] = kload<W>(sk + i * W);
-                    vv[r] = sv[i];
-                }
+            for (int r = 0; r < R; ++r) {  // branch-free (count >= 1)
+                const uint64_t i0 = threadIdx.x + (uint64_t)r * kBlock;
+                const uint64_t i = i0 < count ? i0 : count - 1;
+                kk[r] = kload<W>(sk + i * W);
+                vv[r] = sv[i];
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
