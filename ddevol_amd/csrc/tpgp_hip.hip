@@ -495,21 +495,24 @@ __global__ __launch_bounds__(kBlock) void k_part_scan(BatchDev b, const PassMap 
 }
 
 // Per tile: move every element to its class's run (stable), with digits pa, pb moved to the
-// top of the key.  The map's first tile also writes the output regions and length.
-// Per tile: move every element to its class's run (stable), with digits pa, pb moved to the
 // top of the key; one round of 256 elements at a time (ranks from per-wave ballots), which keeps
-// register use low for the many-tile launches.  The map's first tile also writes the output
+// register use low for the many-tile launches.  The per-round counts and runs are double
+// buffered, so one barrier per round suffices.  The map's first tile also writes the output
 // regions and length.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassMap *__restrict__ pm) {
-    __shared__ unsigned wave_cnt[4][16];
-    __shared__ uint64_t run[16];
+__global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b,
+                                                         const PassMap *__restrict__ pm) {
+    __shared__ unsigned wave_cnt[2][kBlock / 64][16];
+    __shared__ uint64_t run[2][16];
+    __shared__ uint64_t class_total[16];
     const uint32_t M = b.n_maps;
     const uint64_t ntiles = b.tstart[M];
     for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
         uint32_t m = find_bucket(b.tstart, M, g);
         const PassMap &P = pm[m];
         const uint32_t n = b.slot[m].n_spins;
+        // read once: the rounds' global stores could alias the descriptor for the compiler
+        const uint32_t pa = P.pa, pb = P.pb;
         uint64_t len = b.len[m * 3 + P.in_buf];
         uint64_t e0 = (g - b.tstart[m]) * kPartTile;
         uint64_t e1 = e0 + kPartTile < len ? e0 + kPartTile : len;
@@ -517,8 +520,12 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassM
         const double *vals = b.arena[P.in_buf].vals + b.slot[m].off;
         uint64_t *okeys = b.arena[P.out_buf].keys + b.slot[m].off * W;
         double *ovals = b.arena[P.out_buf].vals + b.slot[m].off;
-        // read once: the rounds' global stores could alias the descriptor for the compiler
-        const uint32_t pa = P.pa, pb = P.pb;
+        // the class totals and this tile's offsets first, so their loads fly with the tile's
+        uint64_t tile_off = 0;
+        if (threadIdx.x < 16) {
+            class_total[threadIdx.x] = b.map_hist[m * 16 + threadIdx.x];
+            tile_off = b.tile_base[g * 16 + threadIdx.x];
+        }
         Key<W> kk[kPartRounds];
         double vv[kPartRounds];
 #pragma unroll
@@ -528,48 +535,50 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b, const PassM
             kk[r] = kload<W>(keys + ec * W);
             vv[r] = vals[ec];
         }
+        __syncthreads();
         if (threadIdx.x < 16) {
             uint64_t base = 0;
-            for (unsigned c = 0; c < threadIdx.x; ++c) base += b.map_hist[m * 16 + c];
-            run[threadIdx.x] = base + b.tile_base[g * 16 + threadIdx.x];
+            for (unsigned c = 0; c < threadIdx.x; ++c) base += class_total[c];
+            run[0][threadIdx.x] = base + tile_off;
             if (g == b.tstart[m]) {
                 uint64_t *reg = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
                 reg[threadIdx.x] = base;
                 if (threadIdx.x == 15) {
-                    const uint64_t total = base + b.map_hist[m * 16 + 15];
+                    const uint64_t total = base + class_total[15];
                     reg[16] = total;
                     b.len[m * 3 + P.out_buf] = total;
                 }
             }
         }
-        __syncthreads();
+        const uint64_t lanemask = (1ull << lane_id()) - 1ull;
 #pragma unroll
         for (int r = 0; r < kPartRounds; ++r) {
+            const int cur = r & 1;
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
             const bool live = e < e1;
             const unsigned cls = live ? (kdigit<W>(kk[r], pa) << 2) | kdigit<W>(kk[r], pb) : 16u;
             unsigned below = 0;
-            const uint64_t lanemask = (1ull << lane_id()) - 1ull;
             for (unsigned c = 0; c < 16; ++c) {
                 uint64_t bal = __ballot(cls == c);
                 if (cls == c) below = (unsigned)__popcll(bal & lanemask);
-                if (lane_id() == 0) wave_cnt[wave_id()][c] = (unsigned)__popcll(bal);
+                if (lane_id() == 0) wave_cnt[cur][wave_id()][c] = (unsigned)__popcll(bal);
             }
+            // publishes this round's counts and the runs the previous round advanced; the other
+            // buffers were last read before this barrier's predecessor
             __syncthreads();
             if (live) {
-                uint64_t dst = run[cls] + below;
-                for (unsigned w = 0; w < wave_id(); ++w) dst += wave_cnt[w][cls];
+                uint64_t dst = run[cur][cls] + below;
+                for (unsigned w = 0; w < wave_id(); ++w) dst += wave_cnt[cur][w][cls];
                 kstore<W>(okeys + dst * W, move_to_top<W>(kk[r], pa, pb, n));
                 ovals[dst] = vv[r];
             }
-            __syncthreads();
             if (threadIdx.x < 16) {
                 unsigned add = 0;
-                for (unsigned w = 0; w < 4; ++w) add += wave_cnt[w][threadIdx.x];
-                run[threadIdx.x] += add;
+                for (unsigned w = 0; w < kBlock / 64; ++w) add += wave_cnt[cur][w][threadIdx.x];
+                run[cur ^ 1][threadIdx.x] = run[cur][threadIdx.x] + add;
             }
-            __syncthreads();
         }
+        __syncthreads();  // LDS is reused by the next tile
     }
 }
 
