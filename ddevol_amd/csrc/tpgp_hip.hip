@@ -986,6 +986,15 @@ __device__ __forceinline__ void tile_from_desc(const BatchDev &b, uint64_t g, Me
     a1 = (t.flags & 2u) ? t.nA : b.splits[g + 1];
 }
 
+// The coefficients of the partner class c ^ q of an anticommuting tile: its own entries are
+// the tile's D elements, its partner's the tile's A elements.
+__device__ __forceinline__ Coef partner_coef(const PassMap &P, const MergeCtx &x) {
+    Coef k = coef_of(P, x);
+    const unsigned ph = (pauli_phase(P.qt, x.c >> 2) + pauli_phase(P.qs, x.c & 3u)) & 3u;
+    k.dneg = ph != 1;
+    return k;
+}
+
 // Pass 1 of a rotation: per tile, the merge-path splits, the number of kept outputs, the
 // ledger partial sums (deterministic upward-rounded tree) and, when asked, the log.
 template <int W>
@@ -1229,8 +1238,13 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
     }
 }
 
-// Pass 2 of a rotation: per tile, merge again and write the kept outputs at the tile's offset
-// (compacted in LDS first, so global writes are contiguous).
+// Pass 2 of a rotation: per tile, merge again and write the kept outputs at the tile's offset.
+// Paired classes: an anticommuting class c and its partner c ^ q merge the same two sorted runs
+// (c's keys, and c ^ q's with the top digits xor-ed), so tile k of c and tile k of c ^ q cover
+// the same merge-path range and emit the same lower digits thread by thread.  The lower class's
+// workgroup stages the pair once and walks it once per class; the higher class's workgroup
+// returns at once.  Each thread's kept outputs (at most kMergeItems,
+// consecutive in the output) go from its LDS slots straight to their places.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ TileIn<W> in;
@@ -1242,23 +1256,27 @@ __global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap 
         MergeCtx x;
         TileGeom t;
         tile_from_desc(b, g, x, t.a0, t.a1);
+        if (x.anti && x.cq < x.c) continue;  // written with its partner's tile
         const PassMap &P = pm[x.m];
         const uint32_t n = b.slot[x.m].n_spins;
         const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
         const uint64_t total = x.nA + x.nD;
         t.d0 = x.k * kMergeTile;
         t.d1 = t.d0 + kMergeTile < total ? t.d0 + kMergeTile : total;
-        const uint64_t tm0 = b.tstart[x.m * 16];
-        const uint64_t out_pos =
-            b.chunk_base[b.chunk_start[x.m] + (g - tm0) / kChunkTiles] + b.tile_base[g];
+        const uint64_t tm0 = b.tstart[x.m * 16], c0 = b.chunk_start[x.m];
+        const uint64_t out_pos = b.chunk_base[c0 + (g - tm0) / kChunkTiles] + b.tile_base[g];
         const uint64_t count = b.tile_hist[g];
+        const uint64_t cap = b.slot[x.m].cap;
         // overflowed maps and inconsistent tiles write nothing (both are reported at sync)
-        if (!tile_ok(x, t) || b.overflow[x.m] || out_pos + count > b.slot[x.m].cap) continue;
+        if (!tile_ok(x, t) || b.overflow[x.m]) continue;
+        uint64_t *gkeys = b.arena[P.out_buf].keys + b.slot[x.m].off * W;
+        double *gvals = b.arena[P.out_buf].vals + b.slot[x.m].off;
         if (!x.anti && (P.flags & kPassCommutingKept)) {  // unchanged keys: a straight copy
+            if (out_pos + count > cap) continue;
             const uint64_t *sk = b.arena[P.in_buf].keys + (b.slot[x.m].off + x.A0 + t.d0) * W;
             const double *sv = b.arena[P.in_buf].vals + b.slot[x.m].off + x.A0 + t.d0;
-            uint64_t *gk = b.arena[P.out_buf].keys + (b.slot[x.m].off + out_pos) * W;
-            double *gv = b.arena[P.out_buf].vals + b.slot[x.m].off + out_pos;
+            uint64_t *gk = gkeys + out_pos * W;
+            double *gv = gvals + out_pos;
             constexpr int R = kMergeTile / kBlock;  // count <= kMergeTile: loads first, then stores
             Key<W> kk[R];
             double vv[R];
@@ -1279,34 +1297,45 @@ __global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap 
             }
             continue;
         }
-        load_tile<W>(b, P, x, q, t, in);
-        __syncthreads();
-        const Coef k = coef_of(P, x);
-        const uint32_t obase = threadIdx.x * kMergeItems;
-        unsigned nout = 0;
-        bool bad = false;
-        walk_tile<W>(in, t, [&](const Key<W> &key, bool hasA, double va, bool hasD, double vd) {
-            double v, r;
-            merged(k, hasA, va, hasD, vd, v, r, bad);
-            if (v == 0.0) return;
-            if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) return;
-            kstore<W>(okeys + (obase + nout) * W, key);
-            ovals[obase + nout] = v;
-            nout++;
-        });
-        unsigned tot;
-        unsigned myoff = block_exclusive_scan(nout, scratch, &tot);
-        // compact this thread's outputs into the (consumed) input area, then write contiguously
-        for (unsigned i = 0; i < nout; ++i) {
-            kstore<W>(in.keys + (myoff + i) * W, kload<W>(okeys + (obase + i) * W));
-            in.vals[myoff + i] = ovals[obase + i];
+        uint64_t out_pos2 = 0, count2 = 0;
+        if (x.anti) {
+            const uint64_t g2 = b.tstart[x.m * 16 + x.cq] + x.k;
+            out_pos2 = b.chunk_base[c0 + (g2 - tm0) / kChunkTiles] + b.tile_base[g2];
+            count2 = b.tile_hist[g2];
         }
-        __syncthreads();
-        uint64_t *gk = b.arena[P.out_buf].keys + (b.slot[x.m].off + out_pos) * W;
-        double *gv = b.arena[P.out_buf].vals + b.slot[x.m].off + out_pos;
-        for (unsigned i = threadIdx.x; i < tot; i += kBlock) {
-            kstore<W>(gk + (uint64_t)i * W, kload<W>(in.keys + i * W));
-            gv[i] = in.vals[i];
+        load_tile<W>(b, P, x, q, t, in);
+        const uint32_t obase = threadIdx.x * kMergeItems;
+        const int passes = x.anti ? 2 : 1;
+        for (int pass = 0; pass < passes; ++pass) {
+            const Coef k = pass ? partner_coef(P, x) : coef_of(P, x);
+            const uint64_t at = pass ? out_pos2 : out_pos;
+            const bool fits = (pass ? out_pos2 + count2 : out_pos + count) <= cap;
+            __syncthreads();  // staged inputs; the previous pass's slots are written out
+            unsigned nout = 0;
+            bool bad = false;
+            walk_tile<W>(in, t, [&](const Key<W> &key_a, bool hasA_a, double va_a, bool hasD_a,
+                                    double vd_a) {
+                const Key<W> key = pass ? kxor<W>(key_a, q) : key_a;
+                const bool hasA = pass ? hasD_a : hasA_a, hasD = pass ? hasA_a : hasD_a;
+                const double va = pass ? vd_a : va_a, vd = pass ? va_a : vd_a;
+                double v, r;
+                merged(k, hasA, va, hasD, vd, v, r, bad);
+                if (v == 0.0) return;
+                if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) return;
+                kstore<W>(okeys + (obase + nout) * W, key);
+                ovals[obase + nout] = v;
+                nout++;
+            });
+            unsigned tot;
+            const unsigned myoff = block_exclusive_scan(nout, scratch, &tot);
+            if (fits) {
+                uint64_t *gk = gkeys + (at + myoff) * W;
+                double *gv = gvals + at + myoff;
+                for (unsigned i = 0; i < nout; ++i) {
+                    kstore<W>(gk + (uint64_t)i * W, kload<W>(okeys + (obase + i) * W));
+                    gv[i] = ovals[obase + i];
+                }
+            }
         }
         __syncthreads();
     }
