@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public specification
 # from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings (factors 2.00 and
 # 1.00): profiles/k_rot_write_j1j2_n16_window_pmc.md.  Used for roofline.traffic when the run
 # is that window and no --traffic is given; other windows report null.
-MEASURED_TRAFFIC = {("k_rot_write", 600, 40): 669.6e6}
+MEASURED_TRAFFIC = {("k_rot_write", 600, 40): 551.0e6}
 
 
 def log(*a):
@@ -237,9 +237,13 @@ def bench_single(args, n, terms, dt, thr, label, workload):
     opts = E.options(threshold=thr, max_terms=10 ** 11, context=ctx)
     led = E.ledger(0)
     ev = E.Evolution(mp, terms, dt, opts, context=ctx)
-    if args.warmup:
-        st, fid, rad, act, acc = ev.run(args.warmup, led)
+    done = 0
+    while done < args.warmup:  # untimed, in chunks with a progress line each
+        chunk = min(25, args.warmup - done)
+        st, fid, rad, act, acc = ev.run(chunk, led)
         assert st == 0, N_last_error()
+        done += chunk
+        log(f"{label}: warmup step {done}: {int(act[-1])} terms")
     c0 = ev.counters()
     t0 = time.perf_counter()
     st, fid, rad, act, acc = ev.run(args.steps, led)
@@ -281,8 +285,10 @@ def main():
     ap.add_argument("--serve-steps", type=int, nargs=2, default=[10, 200])
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--budget-gb", type=float, default=64.0)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=600)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed Strang steps (default 40; 20 for --workload large)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps before them (default 600; 100 for --workload large)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--breakdown", action="store_true",
@@ -294,6 +300,11 @@ def main():
     ap.add_argument("--load-state", help="start the timed region from a state saved by "
                                          "--save-state (its warmup count is used)")
     args = ap.parse_args()
+    large = args.workload == "large"
+    if args.steps is None:
+        args.steps = 20 if large else 40
+    if args.warmup is None:
+        args.warmup = 100 if large else 600
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one FID per GPU: only --gpus 1 is supported; "
                                    "multi-GPU sharding is a design, not part of this build"}))

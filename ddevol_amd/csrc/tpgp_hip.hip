@@ -610,28 +610,29 @@ __device__ __forceinline__ bool class_anti(unsigned c, unsigned qt, unsigned qs)
     return anticomm(qt, c >> 2) != anticomm(qs, c & 3u);
 }
 
+// Merge tiles of (map, class) t = 16 m + c of a rotation: ceil((|c| + |c ^ q|) / kMergeTile)
+// for an anticommuting class, ceil(|c| / kMergeTile) otherwise.
+__device__ __forceinline__ unsigned class_tiles(const BatchDev &b, const PassMap *pm, uint32_t t) {
+    const uint32_t m = t >> 4, c = t & 15;
+    const PassMap &P = pm[m];
+    if (!P.active) return 0;
+    const uint64_t *reg = b.regions + ((uint64_t)m * 3 + P.in_buf) * 17;
+    const uint64_t na = reg[c + 1] - reg[c];
+    uint64_t nd = 0;
+    if (class_anti(c, P.qt, P.qs)) {
+        const unsigned cq = c ^ ((P.qt << 2) | P.qs);
+        nd = reg[cq + 1] - reg[cq];
+    }
+    return (unsigned)((na + nd + kMergeTile - 1) / kMergeTile);
+}
+
 __global__ __launch_bounds__(kBlock) void k_rot_plan(BatchDev b, const PassMap *pm) {
     __shared__ unsigned scratch[8];
     const uint32_t M = b.n_maps, n = M * 16;
-    if (threadIdx.x == 0) b.counter[0] = 0;
     uint64_t carry = 0;
     for (uint32_t base = 0; base < n; base += kBlock) {
         uint32_t t = base + threadIdx.x;
-        unsigned tiles = 0;
-        if (t < n) {
-            uint32_t m = t >> 4, c = t & 15;
-            const PassMap &P = pm[m];
-            if (P.active) {
-                const uint64_t *reg = b.regions + ((uint64_t)m * 3 + P.in_buf) * 17;
-                uint64_t na = reg[c + 1] - reg[c];
-                uint64_t nd = 0;
-                if (class_anti(c, P.qt, P.qs)) {
-                    unsigned cq = c ^ ((P.qt << 2) | P.qs);
-                    nd = reg[cq + 1] - reg[cq];
-                }
-                tiles = (unsigned)((na + nd + kMergeTile - 1) / kMergeTile);
-            }
-        }
+        const unsigned tiles = t < n ? class_tiles(b, pm, t) : 0u;
         unsigned tot;
         unsigned ex = block_exclusive_scan(tiles, scratch, &tot);
         if (t < n) b.tstart[t] = carry + ex;
@@ -668,9 +669,9 @@ struct TileDesc {
     uint32_t m, c, cq, flags;
 };
 
-__device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm, uint64_t g,
-                                       MergeCtx &x) {
-    uint32_t t = find_bucket(b.tstart, b.n_maps * 16, g);
+__device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm,
+                                       const uint64_t *tstart, uint64_t g, MergeCtx &x) {
+    uint32_t t = find_bucket(tstart, b.n_maps * 16, g);
     x.m = t >> 4;
     x.c = t & 15;
     const PassMap &P = pm[x.m];
@@ -681,7 +682,7 @@ __device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm, uin
     x.cq = x.c ^ ((P.qt << 2) | P.qs);
     x.D0 = reg[x.cq];
     x.nD = x.anti ? reg[x.cq + 1] - reg[x.cq] : 0;
-    x.k = g - b.tstart[t];
+    x.k = g - tstart[t];
 }
 
 // Tile inputs in LDS: A at [0, na), T(D) at [na, na+nd), the D element after the tile at na+nd
@@ -931,9 +932,40 @@ __device__ uint64_t split_search8(const uint64_t *keys, const MergeCtx &x, const
 
 // Pass 0 of a rotation: every tile's descriptor and merge-path start split, 8 lanes per tile
 // (32 tiles per workgroup, all searching at once so the dependent loads overlap).
+// With `plan` (at most kPlanMaps maps), every workgroup first derives the tile offsets of all
+// (map, class) pairs itself -- what k_rot_plan computes -- and workgroup 0 publishes them and
+// the scan chunks for the later passes: one launch less per rotation.
+constexpr uint32_t kPlanMaps = kBlock / 16;
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap *__restrict__ pm) {
-    const uint64_t ntiles = b.tstart[b.n_maps * 16];
+__global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap *__restrict__ pm,
+                                                       int plan) {
+    __shared__ uint64_t s_tstart[kBlock + 1];
+    __shared__ unsigned scratch[8];
+    const uint64_t *tstart = b.tstart;
+    if (plan) {
+        const uint32_t M = b.n_maps, n = M * 16;  // n <= kBlock
+        const unsigned tiles = threadIdx.x < n ? class_tiles(b, pm, threadIdx.x) : 0u;
+        unsigned tot;
+        const unsigned ex = block_exclusive_scan(tiles, scratch, &tot);
+        if (threadIdx.x < n) s_tstart[threadIdx.x] = ex;
+        if (threadIdx.x == 0) s_tstart[n] = tot;
+        __syncthreads();
+        if (blockIdx.x == 0) {
+            if (threadIdx.x < n) b.tstart[threadIdx.x] = s_tstart[threadIdx.x];
+            if (threadIdx.x == 0) b.tstart[n] = s_tstart[n];
+            const unsigned chunks =
+                threadIdx.x < M ? (unsigned)((s_tstart[threadIdx.x * 16 + 16] -
+                                              s_tstart[threadIdx.x * 16] + kChunkTiles - 1) /
+                                             kChunkTiles)
+                                : 0u;
+            unsigned ctot;
+            const unsigned cex = block_exclusive_scan(chunks, scratch, &ctot);
+            if (threadIdx.x < M) b.chunk_start[threadIdx.x] = cex;
+            if (threadIdx.x == 0) b.chunk_start[M] = ctot;
+        }
+        tstart = s_tstart;
+    }
+    const uint64_t ntiles = tstart[b.n_maps * 16];
     const unsigned grp = lane_id() >> 3, sub = lane_id() & 7u;
     const uint64_t per_block = kBlock / 8;
     for (uint64_t base = (uint64_t)blockIdx.x * per_block; base < ntiles;
@@ -941,7 +973,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
         const uint64_t g = base + threadIdx.x / 8;
         if (g >= ntiles) continue;  // whole 8-lane groups leave together
         MergeCtx x;
-        locate(b, pm, g, x);
+        locate(b, pm, tstart, g, x);
         const uint64_t d0 = x.k * kMergeTile;
         uint64_t a0 = d0;
         if (x.anti) {
@@ -962,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
             t.m = x.m;
             t.c = x.c;
             t.cq = x.cq;
-            const bool last = g + 1 >= b.tstart[x.m * 16 + x.c + 1];
+            const bool last = g + 1 >= tstart[x.m * 16 + x.c + 1];
             t.flags = (x.anti ? 1u : 0u) | (last ? 2u : 0u);
             b.tdesc[g] = t;
         }
@@ -3433,7 +3465,8 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid) {
     hipStream_t st = b->ctx->stream;
     {
         ProfileScope whole(b, TPGP_HIP_PROFILE_ROTATION);
-        {
+        const int fused_plan = b->M <= kPlanMaps;  // the split pass plans too
+        if (!fused_plan) {
             ProfileScope pp(b, TPGP_HIP_PROFILE_PLAN);
             hipLaunchKernelGGL(k_rot_plan, dim3(1), dim3(kBlock), 0, st, b->dev, d);
         }
@@ -3441,9 +3474,11 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid) {
             ProfileScope pc(b, TPGP_HIP_PROFILE_SPLITS);
             const unsigned sgrid = tile_grid(b->est_tiles / 32 + 1);  // 8 lanes per tile
             if (b->W == 1)
-                hipLaunchKernelGGL(k_rot_splits<1>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d);
+                hipLaunchKernelGGL(k_rot_splits<1>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d,
+                                   fused_plan);
             else
-                hipLaunchKernelGGL(k_rot_splits<2>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d);
+                hipLaunchKernelGGL(k_rot_splits<2>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d,
+                                   fused_plan);
         }
         {
             ProfileScope pc(b, TPGP_HIP_PROFILE_COUNT);
