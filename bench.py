@@ -226,6 +226,15 @@ def bench_serve(args):
     }
 
 
+def fid_allowance(E, terms, dt, led, n, steps):
+    """The run driver's FID allowance after `steps` Strang steps: the per-step Strang bound summed
+    upward, plus the ledger's (dropped + arithmetic) l2 over sqrt(N)."""
+    per_step, pf = E.strang_bound(terms, dt), 0.0
+    for _ in range(steps):
+        pf = E.N.engine().tpgp_bound_add_up(pf, per_step)
+    return E.fid_allowance(pf, led, n)
+
+
 def bench_single(args, n, terms, dt, thr, label, workload):
     """One large FID: untimed warmup steps, then timed steps on the resident map."""
     from ddevol_amd import engine as E
@@ -390,11 +399,20 @@ def main():
     ev.profile(mask)
     c0 = ev.counters()
     t0 = time.perf_counter()
-    st, fid, rad, act, acc = ev.run(args.steps, led)
+    done = 0
+    while done < args.steps:
+        # one call for a short window; a long one (a whole FID) in chunks of 25 steps, each
+        # ending with a progress line (one readout sync per chunk)
+        chunk = args.steps - done if args.steps <= 100 else min(25, args.steps - done)
+        st, fid, rad, act, acc = ev.run(chunk, led)
+        if st:
+            raise RuntimeError(f"timed run failed at step {args.warmup + done + acc}: status "
+                               f"{st}: {N.engine().tpgp_hip_last_error().decode()}")
+        done += acc
+        if done < args.steps:
+            log(f"timed step {args.warmup + done}: {int(act[-1])} terms, "
+                f"{time.perf_counter() - t0:.1f} s")
     t1 = time.perf_counter()
-    if st:
-        raise RuntimeError(f"timed run failed: status {st}: "
-                           f"{N.engine().tpgp_hip_last_error().decode()}")
     c1 = ev.counters()
     kinds = range(len(N.PROFILE_NAMES)) if args.breakdown else timed_kinds
     prof = {N.PROFILE_NAMES[k]: ev.profile_read(k) for k in kinds}
@@ -447,7 +465,10 @@ def main():
                    "timed_steps": f"{args.warmup + 1}..{args.warmup + args.steps}",
                    "terms_at_start": start_terms, "terms_at_end": int(act[-1]),
                    "rotations": rotations, "branch_updates": updates,
-                   "fid_at_end": float(fid[-1])},
+                   "fid_at_end": float(fid[-1]),
+                   # a loaded state carries no ledger of its warmup steps
+                   "fid_allowance_at_end": None if args.load_state else
+                   fid_allowance(E, terms, dt, led, n, args.warmup + args.steps)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,
                      "traffic": args.traffic if args.traffic is not None

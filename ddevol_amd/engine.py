@@ -6,6 +6,7 @@ engine/include/tpgp_cuda_rev.h (context + branches) with the same argument meani
 from __future__ import annotations
 
 import ctypes as C
+import math
 import weakref
 
 import numpy as np
@@ -161,6 +162,20 @@ def rotate(mp: Map, i, j, qi, qj, theta, opts, led):
 def strang_step(mp: Map, terms_arr, count, dt, opts, led):
     return N.engine().tpgp_strang_step(C.byref(mp.m), terms_arr, count, dt, C.byref(opts),
                                        C.byref(led))
+
+
+def strang_bound(terms, dt):
+    """Per-step Strang operator bound of the term table (tpgp_strang_bound, tpgp_rev.c:887-961)."""
+    return N.engine().tpgp_strang_bound(N.terms_array(terms), len(terms), dt)
+
+
+def fid_allowance(pf, led, n_spins):
+    """FID allowance of the global initial state sum_s X_s after the accumulated Strang bound
+    `pf`: pf + (dropped_l2 + arithmetic_l2) / sqrt(N), summed upward (the no-MPFR row of
+    cli_rev.c:584-602 with input_terms = N, as the run driver prints it)."""
+    e = N.engine()
+    loss = (led.dropped_l2 + led.arithmetic_l2) / math.sqrt(n_spins)
+    return e.tpgp_bound_add_up(pf, math.nextafter(loss, math.inf))
 
 
 def liouvillian(mp: Map, terms):
