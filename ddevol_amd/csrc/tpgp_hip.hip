@@ -1065,8 +1065,13 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap 
         __syncthreads();
         const Coef k = coef_of(P, x);
         const bool want_log = (P.flags & 2u) != 0;
-        unsigned nout = 0;
-        uint64_t n_drop = 0, n_rad = 0;
+        unsigned nout = 0, n_drop = 0, n_rad = 0;
+        // Ledger partials of the tile.  Every summand is >= 0 (|v|, and squares rounded up), so
+        // the sums use plain round-to-nearest additions in a fixed tree (deterministic) and are
+        // made upper bounds once per tile: at most kMergeItems - 1 = 3 additions per thread in
+        // sequence, 6 levels across the wave and 3 in sequence across the waves give a depth of
+        // d = 12, so the computed sum S' >= (1 - gamma_12) S with gamma_d = d u / (1 - d u),
+        // u = 2^-53, and next_up(S' (1 + 24 u)) >= S.
         double l1 = 0.0, sq = 0.0, rsq = 0.0;
         bool bad = false;
         walk_tile<W>(in, t, [&](const Key<W> &key, bool hasA, double va, bool hasD, double vd) {
@@ -1075,14 +1080,14 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap 
             const uint64_t k0 = key.w[0], k1 = W > 1 ? key.w[W - 1] : 0;
             if (r != 0.0) {
                 n_rad++;
-                rsq = upadd(rsq, upmul(r, r));
+                rsq += upmul(r, r);
                 if (want_log) log_entry(b, k0, k1, r, 0.0, x.m, P.rot_seq, 1);
             }
             if (v == 0.0) return;  // exact cancellation: dropped silently
             if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) {
                 n_drop++;
-                l1 = upadd(l1, __builtin_fabs(v));
-                sq = upadd(sq, upmul(v, v));
+                l1 += __builtin_fabs(v);
+                sq += upmul(v, v);
                 if (want_log) log_entry(b, k0, k1, __builtin_fabs(v), v, x.m, P.rot_seq, 0);
                 return;
             }
@@ -1091,41 +1096,52 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap 
         unsigned tot;
         (void)block_exclusive_scan(nout, misc.scratch, &tot);
         if (bad) atomicOr(&misc.nonfinite, 1u);
-        // ledger partials: fixed-shape tree with upward rounding (deterministic)
+        const bool radii = k.validated;  // uniform: without enclosures every radius is 0
         for (int d = 1; d < 64; d <<= 1) {
-            double o1 = __shfl_down(l1, d, 64), o2 = __shfl_down(sq, d, 64),
-                   o3 = __shfl_down(rsq, d, 64);
-            uint64_t c1 = __shfl_down(n_drop, d, 64), c2 = __shfl_down(n_rad, d, 64);
-            if (lane_id() + d < 64) {
-                l1 = upadd(l1, o1);
-                sq = upadd(sq, o2);
-                rsq = upadd(rsq, o3);
+            const double o1 = __shfl_down(l1, d, 64), o2 = __shfl_down(sq, d, 64);
+            const unsigned c1 = __shfl_down(n_drop, d, 64);
+            const bool in_wave = lane_id() + d < 64;
+            if (in_wave) {
+                l1 += o1;
+                sq += o2;
                 n_drop += c1;
-                n_rad += c2;
+            }
+            if (radii) {
+                const double o3 = __shfl_down(rsq, d, 64);
+                const unsigned c2 = __shfl_down(n_rad, d, 64);
+                if (in_wave) {
+                    rsq += o3;
+                    n_rad += c2;
+                }
             }
         }
         if (lane_id() == 0) {
             misc.red[0][wave_id()] = l1;
             misc.red[1][wave_id()] = sq;
-            misc.red[2][wave_id()] = rsq;
+            misc.red[2][wave_id()] = radii ? rsq : 0.0;
             misc.redn[0][wave_id()] = n_drop;
-            misc.redn[1][wave_id()] = n_rad;
+            misc.redn[1][wave_id()] = radii ? n_rad : 0u;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            TileLedger tl;
-            tl.l1 = misc.red[0][0];
-            tl.sq = misc.red[1][0];
-            tl.rsq = misc.red[2][0];
-            tl.dropped = misc.redn[0][0];
-            tl.radii = misc.redn[1][0];
+            constexpr int kDepth = (kMergeItems - 1) + 6 + (kBlock / 64 - 1);
+            static_assert(kDepth == 12, "the tree depth the inflation is derived for");
+            constexpr double kInflate = 1.0 + 2.0 * kDepth / 9007199254740992.0;  // exact
+            double s1 = misc.red[0][0], s2 = misc.red[1][0], s3 = misc.red[2][0];
+            uint64_t c1 = misc.redn[0][0], c2 = misc.redn[1][0];
             for (int w = 1; w < kBlock / 64; ++w) {
-                tl.l1 = upadd(tl.l1, misc.red[0][w]);
-                tl.sq = upadd(tl.sq, misc.red[1][w]);
-                tl.rsq = upadd(tl.rsq, misc.red[2][w]);
-                tl.dropped += misc.redn[0][w];
-                tl.radii += misc.redn[1][w];
+                s1 += misc.red[0][w];
+                s2 += misc.red[1][w];
+                s3 += misc.red[2][w];
+                c1 += misc.redn[0][w];
+                c2 += misc.redn[1][w];
             }
+            TileLedger tl;
+            tl.l1 = upmul(s1, kInflate);
+            tl.sq = upmul(s2, kInflate);
+            tl.rsq = upmul(s3, kInflate);
+            tl.dropped = c1;
+            tl.radii = c2;
             tl.nonfinite = misc.nonfinite;
             tl.pad = 0;
             b.tile_led[g] = tl;
@@ -1275,13 +1291,15 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
 // (c's keys, and c ^ q's with the top digits xor-ed), so tile k of c and tile k of c ^ q cover
 // the same merge-path range and emit the same lower digits thread by thread.  The lower class's
 // workgroup stages the pair once and walks it once per class; the higher class's workgroup
-// returns at once.  Each thread's kept outputs (at most kMergeItems,
-// consecutive in the output) go from its LDS slots straight to their places.
+// returns at once.  Each thread's kept outputs (at most kMergeItems, consecutive in the output)
+// stay in registers, placed by static-index selects, and go straight to their places: no LDS
+// output staging, so the block needs half the LDS and more workgroups fit on a CU.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
+// One-word keys: at most 96 VGPRs, so five waves per SIMD fit (unconstrained it takes 111 and
+// four); two-word keys would spill under that cap and are left to the compiler.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W == 1 ? 5 : 1, 8)))
+void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ TileIn<W> in;
-    __shared__ uint64_t okeys[kMergeTile * W];
-    __shared__ double ovals[kMergeTile];
     __shared__ unsigned scratch[8];
     const uint64_t ntiles = b.tstart[b.n_maps * 16];
     for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
@@ -1336,15 +1354,16 @@ __global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap 
             count2 = b.tile_hist[g2];
         }
         load_tile<W>(b, P, x, q, t, in);
-        const uint32_t obase = threadIdx.x * kMergeItems;
         const int passes = x.anti ? 2 : 1;
         for (int pass = 0; pass < passes; ++pass) {
             const Coef k = pass ? partner_coef(P, x) : coef_of(P, x);
             const uint64_t at = pass ? out_pos2 : out_pos;
             const bool fits = (pass ? out_pos2 + count2 : out_pos + count) <= cap;
-            __syncthreads();  // staged inputs; the previous pass's slots are written out
+            __syncthreads();  // staged inputs visible; the previous pass's scan is done
             unsigned nout = 0;
             bool bad = false;
+            Key<W> okey[kMergeItems];
+            double oval[kMergeItems];
             walk_tile<W>(in, t, [&](const Key<W> &key_a, bool hasA_a, double va_a, bool hasD_a,
                                     double vd_a) {
                 const Key<W> key = pass ? kxor<W>(key_a, q) : key_a;
@@ -1354,8 +1373,13 @@ __global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap 
                 merged(k, hasA, va, hasD, vd, v, r, bad);
                 if (v == 0.0) return;
                 if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) return;
-                kstore<W>(okeys + (obase + nout) * W, key);
-                ovals[obase + nout] = v;
+#pragma unroll
+                for (int o = 0; o < kMergeItems; ++o) {  // static indices: stays in registers
+                    if (nout == (unsigned)o) {
+                        okey[o] = key;
+                        oval[o] = v;
+                    }
+                }
                 nout++;
             });
             unsigned tot;
@@ -1363,9 +1387,12 @@ __global__ __launch_bounds__(kBlock) void k_rot_write(BatchDev b, const PassMap 
             if (fits) {
                 uint64_t *gk = gkeys + (at + myoff) * W;
                 double *gv = gvals + at + myoff;
-                for (unsigned i = 0; i < nout; ++i) {
-                    kstore<W>(gk + (uint64_t)i * W, kload<W>(okeys + (obase + i) * W));
-                    gv[i] = ovals[obase + i];
+#pragma unroll
+                for (int o = 0; o < kMergeItems; ++o) {
+                    if ((unsigned)o < nout) {
+                        kstore<W>(gk + (uint64_t)o * W, okey[o]);
+                        gv[o] = oval[o];
+                    }
                 }
             }
         }
