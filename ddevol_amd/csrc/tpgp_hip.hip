@@ -1290,13 +1290,11 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
 // Paired classes: an anticommuting class c and its partner c ^ q merge the same two sorted runs
 // (c's keys, and c ^ q's with the top digits xor-ed), so tile k of c and tile k of c ^ q cover
 // the same merge-path range and emit the same lower digits thread by thread.  The lower class's
-// workgroup stages the pair once and walks it once per class; the higher class's workgroup
-// returns at once.  Each thread's kept outputs (at most kMergeItems, consecutive in the output)
-// stay in registers, placed by static-index selects, and go straight to their places: no LDS
-// output staging, so the block needs half the LDS and more workgroups fit on a CU.
+// workgroup stages the pair once and walks it once for both classes; the higher class's
+// workgroup returns at once.  Each thread's kept outputs (at most kMergeItems per class,
+// consecutive in the output) stay in registers, placed by static-index selects, and go
+// straight to their places: no LDS output staging.
 template <int W>
-// One-word keys: at most 96 VGPRs, so five waves per SIMD fit (unconstrained it takes 111 and
-// four); two-word keys would spill under that cap and are left to the compiler.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W == 1 ? 5 : 1, 8)))
 void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ TileIn<W> in;
@@ -1354,45 +1352,63 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
             count2 = b.tile_hist[g2];
         }
         load_tile<W>(b, P, x, q, t, in);
-        const int passes = x.anti ? 2 : 1;
-        for (int pass = 0; pass < passes; ++pass) {
-            const Coef k = pass ? partner_coef(P, x) : coef_of(P, x);
-            const uint64_t at = pass ? out_pos2 : out_pos;
-            const bool fits = (pass ? out_pos2 + count2 : out_pos + count) <= cap;
-            __syncthreads();  // staged inputs visible; the previous pass's scan is done
-            unsigned nout = 0;
-            bool bad = false;
-            Key<W> okey[kMergeItems];
-            double oval[kMergeItems];
-            walk_tile<W>(in, t, [&](const Key<W> &key_a, bool hasA_a, double va_a, bool hasD_a,
-                                    double vd_a) {
-                const Key<W> key = pass ? kxor<W>(key_a, q) : key_a;
-                const bool hasA = pass ? hasD_a : hasA_a, hasD = pass ? hasA_a : hasD_a;
-                const double va = pass ? vd_a : va_a, vd = pass ? va_a : vd_a;
-                double v, r;
-                merged(k, hasA, va, hasD, vd, v, r, bad);
-                if (v == 0.0) return;
-                if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) return;
+        __syncthreads();
+        // One walk serves both classes of a pair: each merge position gives class c's output
+        // (key) and class c ^ q's (key ^ q, roles of the two slots swapped); the two kept
+        // counts share one block scan (packed in 16-bit halves: a tile keeps <= kMergeTile).
+        const bool anti = x.anti;
+        const Coef k0 = coef_of(P, x);
+        const Coef k1 = anti ? partner_coef(P, x) : k0;
+        Key<W> okey[kMergeItems];
+        double ov0[kMergeItems], ov1[kMergeItems];
+        unsigned keep0 = 0, keep1 = 0, ncall = 0;
+        bool bad = false;
+        walk_tile<W>(in, t, [&](const Key<W> &key, bool hasA, double va, bool hasD, double vd) {
+            double v, w = 0.0, r;
+            merged(k0, hasA, va, hasD, vd, v, r, bad);
+            const bool kept0 = v != 0.0 && (kzero<W>(key) || __builtin_fabs(v) > k0.thr);
+            bool kept1 = false;
+            if (anti) {
+                merged(k1, hasD, vd, hasA, va, w, r, bad);
+                kept1 = w != 0.0 && (kzero<W>(kxor<W>(key, q)) || __builtin_fabs(w) > k1.thr);
+            }
 #pragma unroll
-                for (int o = 0; o < kMergeItems; ++o) {  // static indices: stays in registers
-                    if (nout == (unsigned)o) {
-                        okey[o] = key;
-                        oval[o] = v;
-                    }
+            for (int o = 0; o < kMergeItems; ++o) {  // static indices: stays in registers
+                if (ncall == (unsigned)o) {
+                    okey[o] = key;
+                    ov0[o] = v;
+                    ov1[o] = w;
                 }
-                nout++;
-            });
-            unsigned tot;
-            const unsigned myoff = block_exclusive_scan(nout, scratch, &tot);
-            if (fits) {
-                uint64_t *gk = gkeys + (at + myoff) * W;
-                double *gv = gvals + at + myoff;
+            }
+            keep0 |= (unsigned)kept0 << ncall;
+            keep1 |= (unsigned)kept1 << ncall;
+            ncall++;
+        });
+        unsigned tot;
+        const unsigned myoff = block_exclusive_scan(
+            (unsigned)__builtin_popcount(keep0) | ((unsigned)__builtin_popcount(keep1) << 16),
+            scratch, &tot);
+        if (out_pos + count <= cap) {
+            uint64_t *gk = gkeys + (out_pos + (myoff & 0xffffu)) * W;
+            double *gv = gvals + out_pos + (myoff & 0xffffu);
 #pragma unroll
-                for (int o = 0; o < kMergeItems; ++o) {
-                    if ((unsigned)o < nout) {
-                        kstore<W>(gk + (uint64_t)o * W, okey[o]);
-                        gv[o] = oval[o];
-                    }
+            for (int o = 0; o < kMergeItems; ++o) {
+                if ((keep0 >> o) & 1u) {
+                    const unsigned at = (unsigned)__builtin_popcount(keep0 & ((1u << o) - 1u));
+                    kstore<W>(gk + (uint64_t)at * W, okey[o]);
+                    gv[at] = ov0[o];
+                }
+            }
+        }
+        if (anti && out_pos2 + count2 <= cap) {
+            uint64_t *gk = gkeys + (out_pos2 + (myoff >> 16)) * W;
+            double *gv = gvals + out_pos2 + (myoff >> 16);
+#pragma unroll
+            for (int o = 0; o < kMergeItems; ++o) {
+                if ((keep1 >> o) & 1u) {
+                    const unsigned at = (unsigned)__builtin_popcount(keep1 & ((1u << o) - 1u));
+                    kstore<W>(gk + (uint64_t)at * W, kxor<W>(okey[o], q));
+                    gv[at] = ov1[o];
                 }
             }
         }
