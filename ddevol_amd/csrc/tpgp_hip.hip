@@ -692,11 +692,10 @@ template <int W> struct TileIn {
     uint64_t keys[(kMergeTile + 2) * W];
     double vals[kMergeTile + 2];
 };
-struct TileMisc {
+struct CountMisc {
     unsigned scratch[8];
-    uint64_t a0, a1;
-    double red[3][kBlock / 64];
-    uint64_t redn[2][kBlock / 64];
+    double red[6][kBlock / 64];     // per wave: l1, sq, rsq of class c, then of its partner
+    unsigned redn[2][kBlock / 64];  // per wave: drops, radii (class c low 16 bits, partner high)
     unsigned nonfinite;
 };
 
@@ -1027,17 +1026,57 @@ __device__ __forceinline__ Coef partner_coef(const PassMap &P, const MergeCtx &x
     return k;
 }
 
+// Per-class tallies of one thread's outputs in the count pass.
+struct TileAcc {
+    unsigned nout, n_drop, n_rad;
+    double l1, sq, rsq;
+};
+// One output key of class `k`: kept, dropped into the ledger or cancelled (tpgp_rev.c:690-729),
+// with the exact-ledger log entries when asked.
+template <int W>
+__device__ __forceinline__ void account(const BatchDev &b, uint32_t m, uint32_t rot, bool want_log,
+                                        const Coef &k, const Key<W> &key, bool hasA, double va,
+                                        bool hasD, double vd, TileAcc &a, bool &bad) {
+    double v, r;
+    merged(k, hasA, va, hasD, vd, v, r, bad);
+    const uint64_t k0 = key.w[0], k1 = W > 1 ? key.w[W - 1] : 0;
+    if (r != 0.0) {
+        a.n_rad++;
+        a.rsq += upmul(r, r);
+        if (want_log) log_entry(b, k0, k1, r, 0.0, m, rot, 1);
+    }
+    if (v == 0.0) return;  // exact cancellation: dropped silently
+    if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) {
+        a.n_drop++;
+        a.l1 += __builtin_fabs(v);
+        a.sq += upmul(v, v);
+        if (want_log) log_entry(b, k0, k1, __builtin_fabs(v), v, m, rot, 0);
+        return;
+    }
+    a.nout++;
+}
+
 // Pass 1 of a rotation: per tile, the merge-path splits, the number of kept outputs, the
-// ledger partial sums (deterministic upward-rounded tree) and, when asked, the log.
+// ledger partial sums and, when asked, the log.  An anticommuting class c and its partner
+// c ^ q share their tiles (tile k of both covers the same merge-path range, as in the write
+// pass): the lower class's workgroup stages the pair once and one walk tallies both classes;
+// the partner's workgroup returns at once.
+//
+// Ledger partials.  Every summand is >= 0 (|v|, and squares rounded up), so the sums use plain
+// round-to-nearest additions in a fixed tree (deterministic) and are made upper bounds once per
+// tile: at most kMergeItems - 1 = 3 additions per thread in sequence, 6 levels across the wave
+// and 3 in sequence across the waves give a depth of d = 12, so the computed sum S' >=
+// (1 - gamma_12) S with gamma_d = d u / (1 - d u), u = 2^-53, and next_up(S' (1 + 24 u)) >= S.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ TileIn<W> in;
-    __shared__ TileMisc misc;
+    __shared__ CountMisc misc;
     const uint64_t ntiles = b.tstart[b.n_maps * 16];
     for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
         MergeCtx x;
         TileGeom t;
         tile_from_desc(b, g, x, t.a0, t.a1);
+        if (x.anti && x.cq < x.c) continue;  // tallied with its partner's tile
         const PassMap &P = pm[x.m];
         const uint32_t n = b.slot[x.m].n_spins;
         const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
@@ -1051,90 +1090,90 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap 
             }
             continue;
         }
+        const bool anti = x.anti;
+        const uint64_t g2 = anti ? b.tstart[x.m * 16 + x.cq] + x.k : g;
         if (threadIdx.x == 0) misc.nonfinite = 0;
         if (!tile_ok(x, t)) {
             if (threadIdx.x == 0) {
                 atomicOr(&b.counter[1], 1u);
                 b.tile_hist[g] = 0;
                 b.tile_led[g] = TileLedger{};
+                b.tile_hist[g2] = 0;
+                b.tile_led[g2] = TileLedger{};
             }
             __syncthreads();
             continue;
         }
         load_tile<W>(b, P, x, q, t, in);
         __syncthreads();
-        const Coef k = coef_of(P, x);
+        const Coef k0 = coef_of(P, x);
+        const Coef k1 = anti ? partner_coef(P, x) : k0;
         const bool want_log = (P.flags & 2u) != 0;
-        unsigned nout = 0, n_drop = 0, n_rad = 0;
-        // Ledger partials of the tile.  Every summand is >= 0 (|v|, and squares rounded up), so
-        // the sums use plain round-to-nearest additions in a fixed tree (deterministic) and are
-        // made upper bounds once per tile: at most kMergeItems - 1 = 3 additions per thread in
-        // sequence, 6 levels across the wave and 3 in sequence across the waves give a depth of
-        // d = 12, so the computed sum S' >= (1 - gamma_12) S with gamma_d = d u / (1 - d u),
-        // u = 2^-53, and next_up(S' (1 + 24 u)) >= S.
-        double l1 = 0.0, sq = 0.0, rsq = 0.0;
+        TileAcc a0{}, a1{};
         bool bad = false;
         walk_tile<W>(in, t, [&](const Key<W> &key, bool hasA, double va, bool hasD, double vd) {
-            double v, r;
-            merged(k, hasA, va, hasD, vd, v, r, bad);
-            const uint64_t k0 = key.w[0], k1 = W > 1 ? key.w[W - 1] : 0;
-            if (r != 0.0) {
-                n_rad++;
-                rsq += upmul(r, r);
-                if (want_log) log_entry(b, k0, k1, r, 0.0, x.m, P.rot_seq, 1);
-            }
-            if (v == 0.0) return;  // exact cancellation: dropped silently
-            if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) {
-                n_drop++;
-                l1 += __builtin_fabs(v);
-                sq += upmul(v, v);
-                if (want_log) log_entry(b, k0, k1, __builtin_fabs(v), v, x.m, P.rot_seq, 0);
-                return;
-            }
-            nout++;
+            account<W>(b, x.m, P.rot_seq, want_log, k0, key, hasA, va, hasD, vd, a0, bad);
+            if (anti)
+                account<W>(b, x.m, P.rot_seq, want_log, k1, kxor<W>(key, q), hasD, vd, hasA, va,
+                           a1, bad);
         });
-        unsigned tot;
-        (void)block_exclusive_scan(nout, misc.scratch, &tot);
+        unsigned tot;  // both classes' kept counts, packed (each <= kMergeTile)
+        (void)block_exclusive_scan(a0.nout | (a1.nout << 16), misc.scratch, &tot);
         if (bad) atomicOr(&misc.nonfinite, 1u);
-        const bool radii = k.validated;  // uniform: without enclosures every radius is 0
+        const bool radii = k0.validated;  // uniform: without enclosures every radius is 0
+        unsigned nd = a0.n_drop | (a1.n_drop << 16), nr = a0.n_rad | (a1.n_rad << 16);
         for (int d = 1; d < 64; d <<= 1) {
-            const double o1 = __shfl_down(l1, d, 64), o2 = __shfl_down(sq, d, 64);
-            const unsigned c1 = __shfl_down(n_drop, d, 64);
             const bool in_wave = lane_id() + d < 64;
+            const double o1 = __shfl_down(a0.l1, d, 64), o2 = __shfl_down(a0.sq, d, 64);
+            const unsigned c1 = __shfl_down(nd, d, 64);
             if (in_wave) {
-                l1 += o1;
-                sq += o2;
-                n_drop += c1;
+                a0.l1 += o1;
+                a0.sq += o2;
+                nd += c1;
+            }
+            if (anti) {
+                const double p1 = __shfl_down(a1.l1, d, 64), p2 = __shfl_down(a1.sq, d, 64);
+                if (in_wave) {
+                    a1.l1 += p1;
+                    a1.sq += p2;
+                }
             }
             if (radii) {
-                const double o3 = __shfl_down(rsq, d, 64);
-                const unsigned c2 = __shfl_down(n_rad, d, 64);
+                const double o3 = __shfl_down(a0.rsq, d, 64), p3 = __shfl_down(a1.rsq, d, 64);
+                const unsigned c2 = __shfl_down(nr, d, 64);
                 if (in_wave) {
-                    rsq += o3;
-                    n_rad += c2;
+                    a0.rsq += o3;
+                    a1.rsq += p3;
+                    nr += c2;
                 }
             }
         }
         if (lane_id() == 0) {
-            misc.red[0][wave_id()] = l1;
-            misc.red[1][wave_id()] = sq;
-            misc.red[2][wave_id()] = radii ? rsq : 0.0;
-            misc.redn[0][wave_id()] = n_drop;
-            misc.redn[1][wave_id()] = radii ? n_rad : 0u;
+            const unsigned w = wave_id();
+            misc.red[0][w] = a0.l1;
+            misc.red[1][w] = a0.sq;
+            misc.red[2][w] = radii ? a0.rsq : 0.0;
+            misc.red[3][w] = anti ? a1.l1 : 0.0;
+            misc.red[4][w] = anti ? a1.sq : 0.0;
+            misc.red[5][w] = anti && radii ? a1.rsq : 0.0;
+            misc.redn[0][w] = nd;
+            misc.redn[1][w] = radii ? nr : 0u;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < 2 && (threadIdx.x == 0 || anti)) {  // thread h writes class h's tile
             constexpr int kDepth = (kMergeItems - 1) + 6 + (kBlock / 64 - 1);
             static_assert(kDepth == 12, "the tree depth the inflation is derived for");
             constexpr double kInflate = 1.0 + 2.0 * kDepth / 9007199254740992.0;  // exact
-            double s1 = misc.red[0][0], s2 = misc.red[1][0], s3 = misc.red[2][0];
-            uint64_t c1 = misc.redn[0][0], c2 = misc.redn[1][0];
+            const unsigned h = threadIdx.x, sh = 16 * h;
+            double s1 = misc.red[3 * h][0], s2 = misc.red[3 * h + 1][0],
+                   s3 = misc.red[3 * h + 2][0];
+            unsigned c1 = (misc.redn[0][0] >> sh) & 0xffffu, c2 = (misc.redn[1][0] >> sh) & 0xffffu;
             for (int w = 1; w < kBlock / 64; ++w) {
-                s1 += misc.red[0][w];
-                s2 += misc.red[1][w];
-                s3 += misc.red[2][w];
-                c1 += misc.redn[0][w];
-                c2 += misc.redn[1][w];
+                s1 += misc.red[3 * h][w];
+                s2 += misc.red[3 * h + 1][w];
+                s3 += misc.red[3 * h + 2][w];
+                c1 += (misc.redn[0][w] >> sh) & 0xffffu;
+                c2 += (misc.redn[1][w] >> sh) & 0xffffu;
             }
             TileLedger tl;
             tl.l1 = upmul(s1, kInflate);
@@ -1144,8 +1183,8 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap 
             tl.radii = c2;
             tl.nonfinite = misc.nonfinite;
             tl.pad = 0;
-            b.tile_led[g] = tl;
-            b.tile_hist[g] = tot;
+            b.tile_led[h ? g2 : g] = tl;
+            b.tile_hist[h ? g2 : g] = (tot >> sh) & 0xffffu;
         }
         __syncthreads();
     }
