@@ -52,7 +52,7 @@ constexpr uint32_t kPassCommutingKept = 16u;
 constexpr int kFuseMax = 3;              // rotations fused into one pass (one bond: XX, YY, ZZ)
 constexpr int kSampleStride = 64;        // fused passes: one sample per 64 elements of a class
 constexpr int kSamplesPerTile = 12;      // fused tiles hold < (12 + 4) * 64 = 1024 elements
-constexpr int kPartRounds = 16;    // partition: 16 rounds of 256 elements
+constexpr int kPartRounds = 8;     // partition: 8 rounds of 256 elements
 constexpr int kPartTile = kBlock * kPartRounds;
 
 __device__ __forceinline__ uint64_t dbits(double x) { return __builtin_bit_cast(uint64_t, x); }
@@ -475,20 +475,31 @@ __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap
 }
 
 // One block per (map, class): exclusive scan of that class's tile counts over the map's tiles
-// (offsets within the class) and the class total.
+// (offsets within the class) and the class total.  Each thread takes kScanPer consecutive tiles,
+// so one block scan covers kBlock * kScanPer tiles (a map of 2e7 terms in two rounds).
 __global__ __launch_bounds__(kBlock) void k_part_scan(BatchDev b, const PassMap *pm) {
+    constexpr int kScanPer = 16;
     __shared__ unsigned scratch[8];
     const uint32_t m = blockIdx.x >> 4, cls = blockIdx.x & 15;
     const PassMap &P = pm[m];
     if (!P.active) return;
     const uint64_t t0 = b.tstart[m], t1 = b.tstart[m + 1];
     uint64_t carry = 0;
-    for (uint64_t base = t0; base < t1; base += kBlock) {
-        uint64_t g = base + threadIdx.x;
-        unsigned v = g < t1 ? b.tile_hist[g * 16 + cls] : 0;
+    for (uint64_t base = t0; base < t1; base += (uint64_t)kBlock * kScanPer) {
+        const uint64_t g0 = base + (uint64_t)threadIdx.x * kScanPer;
+        unsigned v[kScanPer], sum = 0;
+#pragma unroll
+        for (int i = 0; i < kScanPer; ++i) {
+            v[i] = g0 + i < t1 ? b.tile_hist[(g0 + i) * 16 + cls] : 0u;
+            sum += v[i];
+        }
         unsigned tot;
-        unsigned ex = block_exclusive_scan(v, scratch, &tot);
-        if (g < t1) b.tile_base[g * 16 + cls] = carry + ex;
+        unsigned ex = block_exclusive_scan(sum, scratch, &tot);
+#pragma unroll
+        for (int i = 0; i < kScanPer; ++i) {
+            if (g0 + i < t1) b.tile_base[(g0 + i) * 16 + cls] = carry + ex;
+            ex += v[i];
+        }
         carry += tot;
     }
     if (threadIdx.x == 0) b.map_hist[m * 16 + cls] = carry;
