@@ -440,36 +440,61 @@ __global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap 
     if (threadIdx.x == 0) b.tstart[M] = carry;
 }
 
-// Per tile: the histogram of the 16 new classes (digits pa, pb).  All of a thread's key loads
-// are issued before the first histogram update.
+// Per tile: the histogram of the 16 new classes (digits pa, pb).  Each thread counts its
+// kPartRounds keys (coalesced rounds of 256) in 4-bit counters packed in one register: no LDS
+// atomics, which the sorted keys would serialize (the new digits come in long runs, so a wave's
+// atomics hit one bin).  The counters are widened to 16-bit halves of 8 dwords, summed across
+// the wave by shuffles and across the waves in LDS.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap *__restrict__ pm) {
-    __shared__ unsigned hist[16];
+    static_assert(kPartRounds <= 15, "4-bit per-thread counters");
+    __shared__ unsigned whist[kBlock / 64][8];
     const uint32_t M = b.n_maps;
     const uint64_t ntiles = b.tstart[M];
     for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
-        if (threadIdx.x < 16) hist[threadIdx.x] = 0;
         uint32_t m = find_bucket(b.tstart, M, g);
         const PassMap &P = pm[m];
         uint64_t len = b.len[m * 3 + P.in_buf];
         uint64_t e0 = (g - b.tstart[m]) * kPartTile;
         uint64_t e1 = e0 + kPartTile < len ? e0 + kPartTile : len;
         const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[m].off * W;
-        const uint32_t pa = P.pa, pb = P.pb;  // read once, not after every LDS update
+        const uint32_t pa = P.pa, pb = P.pb;  // read once
         Key<W> k[kPartRounds];
 #pragma unroll
         for (int r = 0; r < kPartRounds; ++r) {  // branch-free: past the end, reload the last
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
             k[r] = kload<W>(keys + (e < e1 ? e : e1 - 1) * W);
         }
-        __syncthreads();
+        uint64_t cnt = 0;  // 16 bins x 4 bits
 #pragma unroll
         for (int r = 0; r < kPartRounds; ++r) {
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
-            if (e < e1) atomicAdd(&hist[(kdigit<W>(k[r], pa) << 2) | kdigit<W>(k[r], pb)], 1u);
+            const unsigned bin = (kdigit<W>(k[r], pa) << 2) | kdigit<W>(k[r], pb);
+            cnt += e < e1 ? 1ull << (4 * bin) : 0ull;
+        }
+        unsigned h[8];  // dword j: bin 2j in bits 0-15, bin 2j+1 in bits 16-31
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint64_t two = cnt >> (8 * j);
+            h[j] = (unsigned)(two & 0xfu) | ((unsigned)((two >> 4) & 0xfu) << 16);
+        }
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {  // <= 64 * kPartRounds per half: no carry
+#pragma unroll
+            for (int j = 0; j < 8; ++j) h[j] += __shfl_xor(h[j], d, 64);
+        }
+        if (lane_id() == 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) whist[wave_id()][j] = h[j];
         }
         __syncthreads();
-        if (threadIdx.x < 16) b.tile_hist[g * 16 + threadIdx.x] = hist[threadIdx.x];
+        if (threadIdx.x < 16) {
+            const unsigned j = threadIdx.x >> 1, sh = 16 * (threadIdx.x & 1);
+            unsigned c = 0;
+#pragma unroll
+            for (int w = 0; w < kBlock / 64; ++w) c += (whist[w][j] >> sh) & 0xffffu;
+            b.tile_hist[g * 16 + threadIdx.x] = c;
+        }
         __syncthreads();
     }
 }
