@@ -592,13 +592,18 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b,
             const int cur = r & 1;
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
             const bool live = e < e1;
-            const unsigned cls = live ? (kdigit<W>(kk[r], pa) << 2) | kdigit<W>(kk[r], pb) : 16u;
-            unsigned below = 0;
-            for (unsigned c = 0; c < 16; ++c) {
-                uint64_t bal = __ballot(cls == c);
-                if (cls == c) below = (unsigned)__popcll(bal & lanemask);
-                if (lane_id() == 0) wave_cnt[cur][wave_id()][c] = (unsigned)__popcll(bal);
+            const unsigned cls = live ? (kdigit<W>(kk[r], pa) << 2) | kdigit<W>(kk[r], pb) : 0u;
+            // the live lanes of my class: four bit ballots combined; my rank among them, and
+            // their count written by the first of them (the wave's other entries stay 0)
+            uint64_t same = __ballot(live);
+#pragma unroll
+            for (int bit = 0; bit < 4; ++bit) {
+                const uint64_t bb = __ballot((cls >> bit) & 1u);
+                same &= ((cls >> bit) & 1u) ? bb : ~bb;
             }
+            const unsigned below = (unsigned)__popcll(same & lanemask);
+            if (lane_id() < 16) wave_cnt[cur][wave_id()][lane_id()] = 0;
+            if (live && below == 0) wave_cnt[cur][wave_id()][cls] = (unsigned)__popcll(same);
             // publishes this round's counts and the runs the previous round advanced; the other
             // buffers were last read before this barrier's predecessor
             __syncthreads();
