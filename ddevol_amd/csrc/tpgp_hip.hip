@@ -1016,7 +1016,9 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
         locate(b, pm, tstart, g, x);
         const uint64_t d0 = x.k * kMergeTile;
         uint64_t a0 = d0;
-        if (x.anti) {
+        // the higher class of an anticommuting pair is counted and written with its partner's
+        // tiles and splits: its own are never read
+        if (x.anti && x.c < x.cq) {
             const PassMap &P = pm[x.m];
             const uint32_t n = b.slot[x.m].n_spins;
             const uint64_t *ikeys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
