@@ -307,6 +307,9 @@ struct BatchDev {
     uint64_t *map_hist;       // [M*16]
     uint64_t *splits;         // [maxT+1]
     struct TileDesc *tdesc;   // [maxT] rotation tile descriptors (written by the split pass)
+    uint64_t *work;           // [maxT] rotation tiles the count / write passes visit
+    uint64_t *wstart;         // [M*16+2] first work-list slot per (map, class); [M*16] count
+                              // tiles, [M*16+1] all work tiles
     TileLedger *tile_led;     // [maxT]
     uint64_t *chunk_start;    // [M+1] first scan chunk of every map (rotation)
     uint64_t *chunk_base;     // [maxC] output offset of every chunk within its map
@@ -667,6 +670,16 @@ __device__ __forceinline__ unsigned class_tiles(const BatchDev &b, const PassMap
     return (unsigned)((na + nd + kMergeTile - 1) / kMergeTile);
 }
 
+// What the passes do with the tiles of class c: 1 = merged by the count and write passes (the
+// lower class of an anticommuting pair, or a commuting class truncation may change); 2 = copied
+// by the write pass only (a commuting class truncation cannot change: the split pass records
+// its counts); 0 = nothing (the higher class of a pair is handled with its partner's tiles).
+// The work list holds the kind-1 tiles in (map, class) order, then the kind-2 tiles.
+__device__ __forceinline__ unsigned tile_kind(const PassMap &P, uint32_t c) {
+    if (class_anti(c, P.qt, P.qs)) return (c ^ ((P.qt << 2) | P.qs)) > c ? 1u : 0u;
+    return (P.flags & kPassCommutingKept) ? 2u : 1u;
+}
+
 __global__ __launch_bounds__(kBlock) void k_rot_plan(BatchDev b, const PassMap *pm) {
     __shared__ unsigned scratch[8];
     const uint32_t M = b.n_maps, n = M * 16;
@@ -680,6 +693,25 @@ __global__ __launch_bounds__(kBlock) void k_rot_plan(BatchDev b, const PassMap *
         carry += tot;
     }
     if (threadIdx.x == 0) b.tstart[n] = carry;
+    __syncthreads();
+    // work-list offsets: kind-1 tiles first, then kind-2
+    uint64_t wcarry = 0, n1 = 0;
+    for (unsigned want = 1; want <= 2; ++want) {
+        for (uint32_t base = 0; base < n; base += kBlock) {
+            const uint32_t t = base + threadIdx.x;
+            const unsigned kind = t < n ? tile_kind(pm[t >> 4], t & 15) : 0u;
+            const unsigned tiles = t < n && kind == want ? class_tiles(b, pm, t) : 0u;
+            unsigned tot;
+            const unsigned ex = block_exclusive_scan(tiles, scratch, &tot);
+            if (t < n && kind == want) b.wstart[t] = wcarry + ex;
+            wcarry += tot;
+        }
+        if (want == 1) n1 = wcarry;
+    }
+    if (threadIdx.x == 0) {
+        b.wstart[n] = n1;
+        b.wstart[n + 1] = wcarry;
+    }
     __syncthreads();
     // scan chunks of every map: ceil(tiles / kChunkTiles), exclusive prefix over maps
     uint64_t ccarry = 0;
@@ -980,19 +1012,31 @@ template <int W>
 __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap *__restrict__ pm,
                                                        int plan) {
     __shared__ uint64_t s_tstart[kBlock + 1];
+    __shared__ uint64_t s_wstart[kBlock];
     __shared__ unsigned scratch[8];
-    const uint64_t *tstart = b.tstart;
+    const uint64_t *tstart = b.tstart, *wstart = b.wstart;
     if (plan) {
         const uint32_t M = b.n_maps, n = M * 16;  // n <= kBlock
         const unsigned tiles = threadIdx.x < n ? class_tiles(b, pm, threadIdx.x) : 0u;
-        unsigned tot;
+        const unsigned kind = threadIdx.x < n ? tile_kind(pm[threadIdx.x >> 4], threadIdx.x & 15)
+                                              : 0u;
+        unsigned tot, t1, t2;
         const unsigned ex = block_exclusive_scan(tiles, scratch, &tot);
-        if (threadIdx.x < n) s_tstart[threadIdx.x] = ex;
+        const unsigned x1 = block_exclusive_scan(kind == 1 ? tiles : 0u, scratch, &t1);
+        const unsigned x2 = block_exclusive_scan(kind == 2 ? tiles : 0u, scratch, &t2);
+        if (threadIdx.x < n) {
+            s_tstart[threadIdx.x] = ex;
+            s_wstart[threadIdx.x] = kind == 2 ? t1 + x2 : x1;
+        }
         if (threadIdx.x == 0) s_tstart[n] = tot;
         __syncthreads();
         if (blockIdx.x == 0) {
             if (threadIdx.x < n) b.tstart[threadIdx.x] = s_tstart[threadIdx.x];
-            if (threadIdx.x == 0) b.tstart[n] = s_tstart[n];
+            if (threadIdx.x == 0) {
+                b.tstart[n] = s_tstart[n];
+                b.wstart[n] = t1;
+                b.wstart[n + 1] = t1 + t2;
+            }
             const unsigned chunks =
                 threadIdx.x < M ? (unsigned)((s_tstart[threadIdx.x * 16 + 16] -
                                               s_tstart[threadIdx.x * 16] + kChunkTiles - 1) /
@@ -1004,6 +1048,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
             if (threadIdx.x == 0) b.chunk_start[M] = ctot;
         }
         tstart = s_tstart;
+        wstart = s_wstart;
     }
     const uint64_t ntiles = tstart[b.n_maps * 16];
     const unsigned grp = lane_id() >> 3, sub = lane_id() & 7u;
@@ -1039,6 +1084,13 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
             const bool last = g + 1 >= tstart[x.m * 16 + x.c + 1];
             t.flags = (x.anti ? 1u : 0u) | (last ? 2u : 0u);
             b.tdesc[g] = t;
+            const unsigned kind = tile_kind(pm[x.m], x.c);
+            if (kind) b.work[wstart[x.m * 16 + x.c] + x.k] = g;
+            if (kind == 2) {  // unchanged keys: the count pass does not visit the tile
+                const uint64_t d1 = d0 + kMergeTile < x.nA ? d0 + kMergeTile : x.nA;
+                b.tile_hist[g] = (unsigned)(d1 - d0);
+                b.tile_led[g] = TileLedger{};
+            }
         }
     }
 }
@@ -1114,8 +1166,11 @@ template <int W>
 __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ TileIn<W> in;
     __shared__ CountMisc misc;
-    const uint64_t ntiles = b.tstart[b.n_maps * 16];
-    for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
+    const uint64_t nwork = b.wstart[b.n_maps * 16];  // kind-1 tiles
+    uint64_t gn = blockIdx.x < nwork ? b.work[blockIdx.x] : 0;
+    for (uint64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const uint64_t g = gn;
+        gn = w + gridDim.x < nwork ? b.work[w + gridDim.x] : 0;  // the next tile, early
         MergeCtx x;
         TileGeom t;
         tile_from_desc(b, g, x, t.a0, t.a1);
@@ -1381,8 +1436,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W == 1 ?
 void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ TileIn<W> in;
     __shared__ unsigned scratch[8];
-    const uint64_t ntiles = b.tstart[b.n_maps * 16];
-    for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
+    const uint64_t nwork = b.wstart[b.n_maps * 16 + 1];  // kind-1 and kind-2 tiles
+    uint64_t gn = blockIdx.x < nwork ? b.work[blockIdx.x] : 0;
+    for (uint64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const uint64_t g = gn;
+        gn = w + gridDim.x < nwork ? b.work[w + gridDim.x] : 0;  // the next tile, early
         MergeCtx x;
         TileGeom t;
         tile_from_desc(b, g, x, t.a0, t.a1);
@@ -3080,6 +3138,8 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     ALLOC(d.map_hist, (size_t)M * 16);
     ALLOC(d.splits, b->max_tiles + 1);
     ALLOC(d.tdesc, b->max_tiles);
+    ALLOC(d.work, b->max_tiles);
+    ALLOC(d.wstart, (size_t)M * 16 + 2);
     ALLOC(d.tile_led, b->max_tiles);
     d.max_samples = T / kSampleStride + 16ull * M + 64;
     d.max_ftiles = d.max_samples / kSamplesPerTile + 32ull * M + 64;
@@ -3607,10 +3667,13 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid) {
         }
         {
             ProfileScope pc(b, TPGP_HIP_PROFILE_COUNT);
+            // the count pass visits one tile per anticommuting pair and the commuting classes
+            // truncation may change: about a third of all tiles (more loop in the grid)
+            const unsigned cgrid = rgrid / 2 + 64;  // rgrid: about one workgroup per tile
             if (b->W == 1)
-                hipLaunchKernelGGL(k_rot_count<1>, dim3(rgrid), dim3(kBlock), 0, st, b->dev, d);
+                hipLaunchKernelGGL(k_rot_count<1>, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d);
             else
-                hipLaunchKernelGGL(k_rot_count<2>, dim3(rgrid), dim3(kBlock), 0, st, b->dev, d);
+                hipLaunchKernelGGL(k_rot_count<2>, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d);
         }
         {
             ProfileScope psn(b, TPGP_HIP_PROFILE_SCAN);
@@ -3621,10 +3684,12 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid) {
         }
         {
             ProfileScope pm(b, TPGP_HIP_PROFILE_MERGE);
+            // every tile but the higher class's of each anticommuting pair
+            const unsigned wgrid = rgrid / 3 * 2 + 64;
             if (b->W == 1)
-                hipLaunchKernelGGL(k_rot_write<1>, dim3(rgrid), dim3(kBlock), 0, st, b->dev, d);
+                hipLaunchKernelGGL(k_rot_write<1>, dim3(wgrid), dim3(kBlock), 0, st, b->dev, d);
             else
-                hipLaunchKernelGGL(k_rot_write<2>, dim3(rgrid), dim3(kBlock), 0, st, b->dev, d);
+                hipLaunchKernelGGL(k_rot_write<2>, dim3(wgrid), dim3(kBlock), 0, st, b->dev, d);
         }
     }
     if (b->validate) {
