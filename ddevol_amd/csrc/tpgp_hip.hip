@@ -44,6 +44,7 @@ namespace {
 constexpr int kBlock = 256;        // 4 waves of 64
 constexpr int kMergeItems = 4;     // merge-path elements per thread
 constexpr int kMergeTile = kBlock * kMergeItems;
+static_assert(kMergeTile <= 65535, "per-thread splits are stored in 16 bits");
 constexpr int kChunkTiles = 4 * kBlock;  // rotation tiles per scan chunk
 constexpr uint32_t kSumBlocks = 512;     // blocks of the sum-of-squares reduction
 // PassMap.flags bit: the map's commuting classes cannot lose a key to this rotation's
@@ -308,6 +309,7 @@ struct BatchDev {
     uint64_t *splits;         // [maxT+1]
     struct TileDesc *tdesc;   // [maxT] rotation tile descriptors (written by the split pass)
     uint64_t *work;           // [maxT] rotation tiles the count / write passes visit
+    uint16_t *tsplit;         // [maxT*kBlock] per-thread merge-path splits (count -> write)
     uint64_t *wstart;         // [M*16+2] first work-list slot per (map, class); [M*16] count
                               // tiles, [M*16+1] all work tiles
     TileLedger *tile_led;     // [maxT]
@@ -877,10 +879,30 @@ __device__ __forceinline__ void merged(const Coef &k, bool hasA, double va, bool
     if (!__builtin_isfinite(v) || !__builtin_isfinite(r)) bad = true;
 }
 
-// Walk this thread's kMergeItems merge-path positions of the tile, calling f(key, hasA, va,
-// hasD, vd) once per output key in ascending order.
+// This thread's merge-path split of the staged tile: the number of A elements among the first
+// kMergeItems * threadIdx.x positions of merge(A, T(D)) (0 past the tile's end).
+template <int W>
+__device__ __forceinline__ uint32_t local_split(const TileIn<W> &in, const TileGeom &t) {
+    const uint32_t nloc = (uint32_t)(t.d1 - t.d0);
+    const uint32_t dt = threadIdx.x * kMergeItems;
+    if (dt >= nloc) return 0;
+    const uint32_t na = t.na, nd = t.nd;
+    uint32_t lo = dt > nd ? dt - nd : 0, hi = dt < na ? dt : na;
+    while (lo < hi) {  // first i with A[i] > D[dt-i-1]
+        uint32_t mid = (lo + hi) >> 1;
+        if (klt<W>(kload<W>(in.keys + (na + dt - mid - 1) * W), kload<W>(in.keys + mid * W)))
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// Walk this thread's kMergeItems merge-path positions of the tile from its split `lo`, calling
+// f(key, hasA, va, hasD, vd) once per output key in ascending order.
 template <int W, typename F>
-__device__ __forceinline__ void walk_tile(const TileIn<W> &in, const TileGeom &t, F &&f) {
+__device__ __forceinline__ void walk_from(const TileIn<W> &in, const TileGeom &t, uint32_t lo,
+                                          F &&f) {
     const uint32_t nloc = (uint32_t)(t.d1 - t.d0);
     const uint32_t dt = threadIdx.x * kMergeItems;
     if (dt >= nloc) return;
@@ -888,14 +910,6 @@ __device__ __forceinline__ void walk_tile(const TileIn<W> &in, const TileGeom &t
     auto Akey = [&](uint32_t i) { return kload<W>(in.keys + i * W); };
     auto Dkey = [&](uint32_t j) { return kload<W>(in.keys + (na + j) * W); };
     const uint32_t limit = (nloc - dt) < (uint32_t)kMergeItems ? (nloc - dt) : (uint32_t)kMergeItems;
-    uint32_t lo = dt > nd ? dt - nd : 0, hi = dt < na ? dt : na;
-    while (lo < hi) {  // local split: first i with A[i] > D[dt-i-1]
-        uint32_t mid = (lo + hi) >> 1;
-        if (klt<W>(Dkey(dt - mid - 1), Akey(mid)))
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
     uint32_t i = lo, j = dt - lo, used = 0;
     // a D element equal to the A element just before this split was consumed with it
     if (j < nd && (i > 0 || t.has_prev)) {
@@ -931,6 +945,11 @@ __device__ __forceinline__ void walk_tile(const TileIn<W> &in, const TileGeom &t
             ++used;
         }
     }
+}
+
+template <int W, typename F>
+__device__ __forceinline__ void walk_tile(const TileIn<W> &in, const TileGeom &t, F &&f) {
+    walk_from<W>(in, t, local_split<W>(in, t), f);
 }
 
 __device__ __forceinline__ Coef coef_of(const PassMap &P, const MergeCtx &x) {
@@ -1209,7 +1228,10 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap 
         const bool want_log = (P.flags & 2u) != 0;
         TileAcc a0{}, a1{};
         bool bad = false;
-        walk_tile<W>(in, t, [&](const Key<W> &key, bool hasA, double va, bool hasD, double vd) {
+        const uint32_t lo = local_split<W>(in, t);
+        b.tsplit[g * kBlock + threadIdx.x] = (uint16_t)lo;  // the write pass walks from it too
+        walk_from<W>(in, t, lo, [&](const Key<W> &key, bool hasA, double va, bool hasD,
+                                    double vd) {
             account<W>(b, x.m, P.rot_seq, want_log, k0, key, hasA, va, hasD, vd, a0, bad);
             if (anti)
                 account<W>(b, x.m, P.rot_seq, want_log, k1, kxor<W>(key, q), hasD, vd, hasA, va,
@@ -1491,6 +1513,7 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
             out_pos2 = b.chunk_base[c0 + (g2 - tm0) / kChunkTiles] + b.tile_base[g2];
             count2 = b.tile_hist[g2];
         }
+        const uint32_t lo = b.tsplit[g * kBlock + threadIdx.x];  // the count pass's split
         load_tile<W>(b, P, x, q, t, in);
         __syncthreads();
         // One walk serves both classes of a pair: each merge position gives class c's output
@@ -1503,7 +1526,8 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
         double ov0[kMergeItems], ov1[kMergeItems];
         unsigned keep0 = 0, keep1 = 0, ncall = 0;
         bool bad = false;
-        walk_tile<W>(in, t, [&](const Key<W> &key, bool hasA, double va, bool hasD, double vd) {
+        walk_from<W>(in, t, lo, [&](const Key<W> &key, bool hasA, double va, bool hasD,
+                                    double vd) {
             double v, w = 0.0, r;
             merged(k0, hasA, va, hasD, vd, v, r, bad);
             const bool kept0 = v != 0.0 && (kzero<W>(key) || __builtin_fabs(v) > k0.thr);
@@ -3139,6 +3163,7 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     ALLOC(d.splits, b->max_tiles + 1);
     ALLOC(d.tdesc, b->max_tiles);
     ALLOC(d.work, b->max_tiles);
+    ALLOC(d.tsplit, b->max_tiles * kBlock);
     ALLOC(d.wstart, (size_t)M * 16 + 2);
     ALLOC(d.tile_led, b->max_tiles);
     d.max_samples = T / kSampleStride + 16ull * M + 64;
