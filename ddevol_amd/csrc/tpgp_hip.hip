@@ -899,7 +899,8 @@ __device__ __forceinline__ uint32_t local_split(const TileIn<W> &in, const TileG
 }
 
 // Walk this thread's kMergeItems merge-path positions of the tile from its split `lo`, calling
-// f(key, hasA, va, hasD, vd) once per output key in ascending order.
+// f(key, hasA, va, hasD, vd, i, j) once per output key in ascending order (i, j: the LDS
+// indices of its A and T(D) elements, meaningful where hasA / hasD).
 template <int W, typename F>
 __device__ __forceinline__ void walk_from(const TileIn<W> &in, const TileGeom &t, uint32_t lo,
                                           F &&f) {
@@ -930,17 +931,17 @@ __device__ __forceinline__ void walk_from(const TileIn<W> &in, const TileGeom &t
         if (takeA) {
             Key<W> ka = Akey(i);
             if (j < nd_look && keq<W>(ka, Dkey(j))) {
-                f(ka, true, in.vals[i], true, in.vals[na + j]);
+                f(ka, true, in.vals[i], true, in.vals[na + j], i, j);
                 ++i;
                 ++j;
                 used += 2;
             } else {
-                f(ka, true, in.vals[i], false, 0.0);
+                f(ka, true, in.vals[i], false, 0.0, i, j);
                 ++i;
                 ++used;
             }
         } else {
-            f(Dkey(j), false, 0.0, true, in.vals[na + j]);
+            f(Dkey(j), false, 0.0, true, in.vals[na + j], i, j);
             ++j;
             ++used;
         }
@@ -949,7 +950,9 @@ __device__ __forceinline__ void walk_from(const TileIn<W> &in, const TileGeom &t
 
 template <int W, typename F>
 __device__ __forceinline__ void walk_tile(const TileIn<W> &in, const TileGeom &t, F &&f) {
-    walk_from<W>(in, t, local_split<W>(in, t), f);
+    walk_from<W>(in, t, local_split<W>(in, t),
+                 [&](const Key<W> &key, bool hasA, double va, bool hasD, double vd, uint32_t,
+                     uint32_t) { f(key, hasA, va, hasD, vd); });
 }
 
 __device__ __forceinline__ Coef coef_of(const PassMap &P, const MergeCtx &x) {
@@ -1231,7 +1234,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap 
         const uint32_t lo = local_split<W>(in, t);
         b.tsplit[g * kBlock + threadIdx.x] = (uint16_t)lo;  // the write pass walks from it too
         walk_from<W>(in, t, lo, [&](const Key<W> &key, bool hasA, double va, bool hasD,
-                                    double vd) {
+                                    double vd, uint32_t, uint32_t) {
             account<W>(b, x.m, P.rot_seq, want_log, k0, key, hasA, va, hasD, vd, a0, bad);
             if (anti)
                 account<W>(b, x.m, P.rot_seq, want_log, k1, kxor<W>(key, q), hasD, vd, hasA, va,
@@ -1449,12 +1452,13 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
 // Paired classes: an anticommuting class c and its partner c ^ q merge the same two sorted runs
 // (c's keys, and c ^ q's with the top digits xor-ed), so tile k of c and tile k of c ^ q cover
 // the same merge-path range and emit the same lower digits thread by thread.  The lower class's
-// workgroup stages the pair once and walks it once for both classes; the higher class's
-// workgroup returns at once.  Each thread's kept outputs (at most kMergeItems per class,
-// consecutive in the output) stay in registers, placed by static-index selects, and go
-// straight to their places: no LDS output staging.
+// workgroup stages the pair once and walks it once for both classes; the higher class's tiles
+// are not visited.  Each thread's kept outputs (at most kMergeItems per class, consecutive in
+// the output) are held as LDS indices and go straight to their places: no LDS output staging.
+// One-word keys: at most 72 VGPRs, so seven waves per SIMD fit without spills (unconstrained
+// it takes 74 and six; eight would spill); two-word keys are left to the compiler.
 template <int W>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W == 1 ? 5 : 1, 8)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W == 1 ? 7 : 1, 8)))
 void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ TileIn<W> in;
     __shared__ unsigned scratch[8];
@@ -1522,12 +1526,15 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
         const bool anti = x.anti;
         const Coef k0 = coef_of(P, x);
         const Coef k1 = anti ? partner_coef(P, x) : k0;
-        Key<W> okey[kMergeItems];
-        double ov0[kMergeItems], ov1[kMergeItems];
+        // A kept output is held as its LDS indices and flags (i | j << 11 | hasA << 22 |
+        // hasD << 23: indices <= kMergeTile + 1) and rebuilt from the staged tile when stored:
+        // four registers per thread instead of four keys and eight values.
+        static_assert(kMergeTile + 1 < 2048, "11-bit LDS indices");
+        uint32_t code[kMergeItems];
         unsigned keep0 = 0, keep1 = 0, ncall = 0;
         bool bad = false;
         walk_from<W>(in, t, lo, [&](const Key<W> &key, bool hasA, double va, bool hasD,
-                                    double vd) {
+                                    double vd, uint32_t i, uint32_t j) {
             double v, w = 0.0, r;
             merged(k0, hasA, va, hasD, vd, v, r, bad);
             const bool kept0 = v != 0.0 && (kzero<W>(key) || __builtin_fabs(v) > k0.thr);
@@ -1536,13 +1543,10 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
                 merged(k1, hasD, vd, hasA, va, w, r, bad);
                 kept1 = w != 0.0 && (kzero<W>(kxor<W>(key, q)) || __builtin_fabs(w) > k1.thr);
             }
+            const uint32_t c = i | (j << 11) | ((uint32_t)hasA << 22) | ((uint32_t)hasD << 23);
 #pragma unroll
             for (int o = 0; o < kMergeItems; ++o) {  // static indices: stays in registers
-                if (ncall == (unsigned)o) {
-                    okey[o] = key;
-                    ov0[o] = v;
-                    ov1[o] = w;
-                }
+                if (ncall == (unsigned)o) code[o] = c;
             }
             keep0 |= (unsigned)kept0 << ncall;
             keep1 |= (unsigned)kept1 << ncall;
@@ -1552,28 +1556,32 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
         const unsigned myoff = block_exclusive_scan(
             (unsigned)__builtin_popcount(keep0) | ((unsigned)__builtin_popcount(keep1) << 16),
             scratch, &tot);
-        if (out_pos + count <= cap) {
-            uint64_t *gk = gkeys + (out_pos + (myoff & 0xffffu)) * W;
-            double *gv = gvals + out_pos + (myoff & 0xffffu);
+        const bool fits0 = out_pos + count <= cap, fits1 = anti && out_pos2 + count2 <= cap;
+        uint64_t *gk0 = gkeys + (out_pos + (myoff & 0xffffu)) * W;
+        double *gv0 = gvals + out_pos + (myoff & 0xffffu);
+        uint64_t *gk1 = gkeys + (out_pos2 + (myoff >> 16)) * W;
+        double *gv1 = gvals + out_pos2 + (myoff >> 16);
+        const uint32_t na = t.na;
 #pragma unroll
-            for (int o = 0; o < kMergeItems; ++o) {
-                if ((keep0 >> o) & 1u) {
-                    const unsigned at = (unsigned)__builtin_popcount(keep0 & ((1u << o) - 1u));
-                    kstore<W>(gk + (uint64_t)at * W, okey[o]);
-                    gv[at] = ov0[o];
-                }
+        for (int o = 0; o < kMergeItems; ++o) {
+            const bool s0 = fits0 && ((keep0 >> o) & 1u), s1 = fits1 && ((keep1 >> o) & 1u);
+            if (!(s0 || s1)) continue;
+            const uint32_t c = code[o], i = c & 0x7ffu, j = (c >> 11) & 0x7ffu;
+            const bool hasA = (c >> 22) & 1u, hasD = (c >> 23) & 1u;
+            const Key<W> key = kload<W>(in.keys + (hasA ? i : na + j) * W);
+            const double va = hasA ? in.vals[i] : 0.0, vd = hasD ? in.vals[na + j] : 0.0;
+            double v, r;
+            if (s0) {  // the walk's arithmetic again: the same value, bit for bit
+                merged(k0, hasA, va, hasD, vd, v, r, bad);
+                const unsigned at = (unsigned)__builtin_popcount(keep0 & ((1u << o) - 1u));
+                kstore<W>(gk0 + (uint64_t)at * W, key);
+                gv0[at] = v;
             }
-        }
-        if (anti && out_pos2 + count2 <= cap) {
-            uint64_t *gk = gkeys + (out_pos2 + (myoff >> 16)) * W;
-            double *gv = gvals + out_pos2 + (myoff >> 16);
-#pragma unroll
-            for (int o = 0; o < kMergeItems; ++o) {
-                if ((keep1 >> o) & 1u) {
-                    const unsigned at = (unsigned)__builtin_popcount(keep1 & ((1u << o) - 1u));
-                    kstore<W>(gk + (uint64_t)at * W, kxor<W>(okey[o], q));
-                    gv[at] = ov1[o];
-                }
+            if (s1) {
+                merged(k1, hasD, vd, hasA, va, v, r, bad);
+                const unsigned at = (unsigned)__builtin_popcount(keep1 & ((1u << o) - 1u));
+                kstore<W>(gk1 + (uint64_t)at * W, kxor<W>(key, q));
+                gv1[at] = v;
             }
         }
         __syncthreads();
