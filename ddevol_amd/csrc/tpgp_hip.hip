@@ -841,11 +841,13 @@ __device__ void load_tile(const BatchDev &b, const PassMap &P, const MergeCtx &x
 
 // One output key of the rotation: value and merged radius from its a-slot (own key) and/or its
 // d-slot (partner), exactly as the reference merges two raw branches (tpgp_rev.c:458-464,
-// 695-706); the merged value is one IEEE addition, independent of order.
+// 695-706); the merged value is one IEEE addition, independent of order.  Radii = false drops
+// the radius arithmetic at compile time (the value is the same).
 struct Coef {
     double c, s, rc, rs, thr;
     bool anti, validated, dneg;
 };
+template <bool Radii = true>
 __device__ __forceinline__ void merged(const Coef &k, bool hasA, double va, bool hasD, double vd,
                                        double &v, double &r, bool &bad) {
     r = 0.0;
@@ -856,19 +858,19 @@ __device__ __forceinline__ void merged(const Coef &k, bool hasA, double va, bool
     double xa = 0.0, yd = 0.0, ra = 0.0, rd = 0.0;
     if (hasA) {
         xa = va * k.c;
-        if (k.validated) ra = upadd(upmul(__builtin_fabs(va), k.rc), round_radius(xa));
+        if ((Radii && k.validated)) ra = upadd(upmul(__builtin_fabs(va), k.rc), round_radius(xa));
     }
     if (hasD) {
         double y = vd * k.s;
         yd = k.dneg ? -y : y;
-        if (k.validated) rd = upadd(upmul(__builtin_fabs(vd), k.rs), round_radius(y));
+        if ((Radii && k.validated)) rd = upadd(upmul(__builtin_fabs(vd), k.rs), round_radius(y));
         if (!__builtin_isfinite(y)) bad = true;
     }
     if (!__builtin_isfinite(xa) || !__builtin_isfinite(ra) || !__builtin_isfinite(rd)) bad = true;
     if (hasA && hasD) {
         v = xa + yd;
         r = upadd(ra, rd);
-        if (k.validated && xa != 0.0 && yd != 0.0) r = upadd(r, round_radius(v));
+        if ((Radii && k.validated) && xa != 0.0 && yd != 0.0) r = upadd(r, round_radius(v));
     } else if (hasA) {
         v = xa;
         r = ra;
@@ -1145,32 +1147,32 @@ __device__ __forceinline__ Coef partner_coef(const PassMap &P, const MergeCtx &x
 
 // Per-class tallies of one thread's outputs in the count pass.
 struct TileAcc {
-    unsigned nout, n_drop, n_rad;
+    unsigned cnt;  // kept (bits 0-7), dropped (8-15), nonzero radii (16-23): each <= 4
     double l1, sq, rsq;
 };
 // One output key of class `k`: kept, dropped into the ledger or cancelled (tpgp_rev.c:690-729),
 // with the exact-ledger log entries when asked.
-template <int W>
+template <int W, bool V>
 __device__ __forceinline__ void account(const BatchDev &b, uint32_t m, uint32_t rot, bool want_log,
                                         const Coef &k, const Key<W> &key, bool hasA, double va,
                                         bool hasD, double vd, TileAcc &a, bool &bad) {
     double v, r;
-    merged(k, hasA, va, hasD, vd, v, r, bad);
+    merged<V>(k, hasA, va, hasD, vd, v, r, bad);
     const uint64_t k0 = key.w[0], k1 = W > 1 ? key.w[W - 1] : 0;
-    if (r != 0.0) {
-        a.n_rad++;
+    if (V && r != 0.0) {
+        a.cnt += 1u << 16;
         a.rsq += upmul(r, r);
         if (want_log) log_entry(b, k0, k1, r, 0.0, m, rot, 1);
     }
     if (v == 0.0) return;  // exact cancellation: dropped silently
     if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) {
-        a.n_drop++;
+        a.cnt += 1u << 8;
         a.l1 += __builtin_fabs(v);
         a.sq += upmul(v, v);
         if (want_log) log_entry(b, k0, k1, __builtin_fabs(v), v, m, rot, 0);
         return;
     }
-    a.nout++;
+    a.cnt += 1u;
 }
 
 // Pass 1 of a rotation: per tile, the merge-path splits, the number of kept outputs, the
@@ -1184,8 +1186,11 @@ __device__ __forceinline__ void account(const BatchDev &b, uint32_t m, uint32_t 
 // tile: at most kMergeItems - 1 = 3 additions per thread in sequence, 6 levels across the wave
 // and 3 in sequence across the waves give a depth of d = 12, so the computed sum S' >=
 // (1 - gamma_12) S with gamma_d = d u / (1 - d u), u = 2^-53, and next_up(S' (1 + 24 u)) >= S.
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap *__restrict__ pm) {
+// V: some map of the launch is validated (radii and their ledger sums); without it the radius
+// arithmetic is compiled out and one-word keys fit 80 VGPRs: six waves per SIMD.
+template <int W, bool V>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W == 1 && !V ? 6 : 1, 8)))
+void k_rot_count(BatchDev b, const PassMap *__restrict__ pm) {
     __shared__ TileIn<W> in;
     __shared__ CountMisc misc;
     const uint64_t nwork = b.wstart[b.n_maps * 16];  // kind-1 tiles
@@ -1235,16 +1240,18 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(BatchDev b, const PassMap 
         b.tsplit[g * kBlock + threadIdx.x] = (uint16_t)lo;  // the write pass walks from it too
         walk_from<W>(in, t, lo, [&](const Key<W> &key, bool hasA, double va, bool hasD,
                                     double vd, uint32_t, uint32_t) {
-            account<W>(b, x.m, P.rot_seq, want_log, k0, key, hasA, va, hasD, vd, a0, bad);
+            account<W, V>(b, x.m, P.rot_seq, want_log, k0, key, hasA, va, hasD, vd, a0, bad);
             if (anti)
-                account<W>(b, x.m, P.rot_seq, want_log, k1, kxor<W>(key, q), hasD, vd, hasA, va,
-                           a1, bad);
+                account<W, V>(b, x.m, P.rot_seq, want_log, k1, kxor<W>(key, q), hasD, vd, hasA,
+                              va, a1, bad);
         });
         unsigned tot;  // both classes' kept counts, packed (each <= kMergeTile)
-        (void)block_exclusive_scan(a0.nout | (a1.nout << 16), misc.scratch, &tot);
+        (void)block_exclusive_scan((a0.cnt & 0xffu) | ((a1.cnt & 0xffu) << 16), misc.scratch,
+                                   &tot);
         if (bad) atomicOr(&misc.nonfinite, 1u);
-        const bool radii = k0.validated;  // uniform: without enclosures every radius is 0
-        unsigned nd = a0.n_drop | (a1.n_drop << 16), nr = a0.n_rad | (a1.n_rad << 16);
+        const bool radii = V && k0.validated;  // uniform: without enclosures every radius is 0
+        unsigned nd = ((a0.cnt >> 8) & 0xffu) | (((a1.cnt >> 8) & 0xffu) << 16),
+                 nr = ((a0.cnt >> 16) & 0xffu) | (((a1.cnt >> 16) & 0xffu) << 16);
         for (int d = 1; d < 64; d <<= 1) {
             const bool in_wave = lane_id() + d < 64;
             const double o1 = __shfl_down(a0.l1, d, 64), o2 = __shfl_down(a0.sq, d, 64);
@@ -1536,11 +1543,11 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
         walk_from<W>(in, t, lo, [&](const Key<W> &key, bool hasA, double va, bool hasD,
                                     double vd, uint32_t i, uint32_t j) {
             double v, w = 0.0, r;
-            merged(k0, hasA, va, hasD, vd, v, r, bad);
+            merged<false>(k0, hasA, va, hasD, vd, v, r, bad);
             const bool kept0 = v != 0.0 && (kzero<W>(key) || __builtin_fabs(v) > k0.thr);
             bool kept1 = false;
             if (anti) {
-                merged(k1, hasD, vd, hasA, va, w, r, bad);
+                merged<false>(k1, hasD, vd, hasA, va, w, r, bad);
                 kept1 = w != 0.0 && (kzero<W>(kxor<W>(key, q)) || __builtin_fabs(w) > k1.thr);
             }
             const uint32_t c = i | (j << 11) | ((uint32_t)hasA << 22) | ((uint32_t)hasD << 23);
@@ -1572,13 +1579,13 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
             const double va = hasA ? in.vals[i] : 0.0, vd = hasD ? in.vals[na + j] : 0.0;
             double v, r;
             if (s0) {  // the walk's arithmetic again: the same value, bit for bit
-                merged(k0, hasA, va, hasD, vd, v, r, bad);
+                merged<false>(k0, hasA, va, hasD, vd, v, r, bad);
                 const unsigned at = (unsigned)__builtin_popcount(keep0 & ((1u << o) - 1u));
                 kstore<W>(gk0 + (uint64_t)at * W, key);
                 gv0[at] = v;
             }
             if (s1) {
-                merged(k1, hasD, vd, hasA, va, v, r, bad);
+                merged<false>(k1, hasD, vd, hasA, va, v, r, bad);
                 const unsigned at = (unsigned)__builtin_popcount(keep1 & ((1u << o) - 1u));
                 kstore<W>(gk1 + (uint64_t)at * W, kxor<W>(key, q));
                 gv1[at] = v;
@@ -3679,7 +3686,7 @@ unsigned plan_rotation(tpgp_hip_batch *b, const tpgp_hip_rotation *req, uint32_t
     return rotation_grid(b, req);
 }
 
-void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid) {
+void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool validated) {
     hipStream_t st = b->ctx->stream;
     {
         ProfileScope whole(b, TPGP_HIP_PROFILE_ROTATION);
@@ -3703,10 +3710,18 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid) {
             // the count pass visits one tile per anticommuting pair and the commuting classes
             // truncation may change: about a third of all tiles (more loop in the grid)
             const unsigned cgrid = rgrid / 2 + 64;  // rgrid: about one workgroup per tile
-            if (b->W == 1)
-                hipLaunchKernelGGL(k_rot_count<1>, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d);
+            if (b->W == 1 && validated)
+                hipLaunchKernelGGL((k_rot_count<1, true>), dim3(cgrid), dim3(kBlock), 0, st,
+                                   b->dev, d);
+            else if (b->W == 1)
+                hipLaunchKernelGGL((k_rot_count<1, false>), dim3(cgrid), dim3(kBlock), 0, st,
+                                   b->dev, d);
+            else if (validated)
+                hipLaunchKernelGGL((k_rot_count<2, true>), dim3(cgrid), dim3(kBlock), 0, st,
+                                   b->dev, d);
             else
-                hipLaunchKernelGGL(k_rot_count<2>, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d);
+                hipLaunchKernelGGL((k_rot_count<2, false>), dim3(cgrid), dim3(kBlock), 0, st,
+                                   b->dev, d);
         }
         {
             ProfileScope psn(b, TPGP_HIP_PROFILE_SCAN);
@@ -3941,6 +3956,12 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
         int kind;  // 0 rotation slot, 1 partition slot, 2 fused run
         size_t at;
         unsigned grid;
+        bool validated;  // rotation: some map computes radii
+    };
+    auto any_validated = [M](const tpgp_hip_rotation *rq) {
+        for (uint32_t m = 0; m < M; ++m)
+            if (rq[m].active && rq[m].validated) return true;
+        return false;
     };
     std::vector<Launch> launches;
     std::vector<FusedMap> fused;
@@ -3974,7 +3995,7 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
             pb[m] = (uint32_t)position_of(hm, q[m].j);
         }
         if (any) {
-            launches.push_back({1, used, plan_partition(b, want, pa, pb, h0 + used)});
+            launches.push_back({1, used, plan_partition(b, want, pa, pb, h0 + used), false});
             used += M;
         }
         const size_t R = fuse ? run_length(b, req, r, r1) : 1;
@@ -3983,19 +4004,21 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
             const size_t at = fused.size();
             fused.resize(at + M);
             if (plan_fused(b, req, rotations, r, R, fused.data() + at))
-                launches.push_back({2, at, 0});
+                launches.push_back({2, at, 0, false});
             else
                 fused.resize(at);
             for (size_t k = 0; k < R; ++k) {
                 std::vector<tpgp_hip_rotation> qs(q + k * M, q + (k + 1) * M);
                 for (uint32_t m = 0; m < M; ++m)
                     if (!b->small[m]) qs[m].active = 0;
-                launches.push_back({0, used, plan_rotation(b, qs.data(), rotations[r + k],
-                                                           h0 + used)});
+                launches.push_back({0, used,
+                                    plan_rotation(b, qs.data(), rotations[r + k], h0 + used),
+                                    any_validated(qs.data())});
                 used += M;
             }
         } else {
-            launches.push_back({0, used, plan_rotation(b, q, rotations[r], h0 + used)});
+            launches.push_back({0, used, plan_rotation(b, q, rotations[r], h0 + used),
+                                any_validated(q)});
             used += M;
         }
         r += R;
@@ -4033,7 +4056,7 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
         }
         if (!pass_large[L.at / M]) continue;
         if (L.kind == 0)
-            launch_rotation(b, d0 + L.at, L.grid);
+            launch_rotation(b, d0 + L.at, L.grid, L.validated);
         else
             launch_partition(b, d0 + L.at, L.grid);
     }
