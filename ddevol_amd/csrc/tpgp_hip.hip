@@ -186,13 +186,41 @@ __device__ __forceinline__ unsigned lane_id() { return threadIdx.x & 63u; }
 __device__ __forceinline__ unsigned wave_id() { return threadIdx.x >> 6; }
 
 // Exclusive block scan of one unsigned per thread; returns the exclusive prefix, total in *tot.
+// Wave-wide inclusive scans by DPP lane moves (all 64 lanes active): row_shr 1, 2, 4, 8 scan
+// each row of 16 lanes (lanes before a row's start read 0), then row_bcast 15 and 31 add the
+// last lane of each row to the rows after it.  Lane i ends with the sum over lanes <= i, lane 63
+// with the total: a fixed addition tree in which every input passes through at most 6
+// additions.
+template <int CTRL, int ROWS> __device__ __forceinline__ unsigned dpp_u32(unsigned x) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xf, true);
+}
+template <int CTRL, int ROWS> __device__ __forceinline__ double dpp_f64(double x) {
+    const uint64_t b = dbits(x);
+    const unsigned lo = dpp_u32<CTRL, ROWS>((unsigned)b);
+    const unsigned hi = dpp_u32<CTRL, ROWS>((unsigned)(b >> 32));
+    return dfrom(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ unsigned wave_scan_u32(unsigned v) {
+    v += dpp_u32<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_u32<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_u32<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_u32<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_u32<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_u32<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+__device__ __forceinline__ double wave_scan_f64(double v) {
+    v += dpp_f64<0x111, 0xf>(v);
+    v += dpp_f64<0x112, 0xf>(v);
+    v += dpp_f64<0x114, 0xf>(v);
+    v += dpp_f64<0x118, 0xf>(v);
+    v += dpp_f64<0x142, 0xa>(v);
+    v += dpp_f64<0x143, 0xc>(v);
+    return v;
+}
+
 __device__ unsigned block_exclusive_scan(unsigned v, unsigned *scratch /*>= 4*/, unsigned *tot) {
-    unsigned x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        unsigned y = __shfl_up(x, d, 64);
-        if ((int)lane_id() >= d) x += y;
-    }
+    const unsigned x = wave_scan_u32(v);
     if (lane_id() == 63) scratch[wave_id()] = x;
     __syncthreads();
     unsigned base = 0, all = 0;
@@ -768,9 +796,8 @@ template <int W> struct TileIn {
     double vals[kMergeTile + 2];
 };
 struct CountMisc {
-    unsigned scratch[8];
     double red[6][kBlock / 64];     // per wave: l1, sq, rsq of class c, then of its partner
-    unsigned redn[2][kBlock / 64];  // per wave: drops, radii (class c low 16 bits, partner high)
+    unsigned redn[3][kBlock / 64];  // per wave: kept, drops, radii (class c in the low 16 bits)
     unsigned nonfinite;
 };
 
@@ -1245,40 +1272,26 @@ void k_rot_count(BatchDev b, const PassMap *__restrict__ pm) {
                 account<W, V>(b, x.m, P.rot_seq, want_log, k1, kxor<W>(key, q), hasD, vd, hasA,
                               va, a1, bad);
         });
-        unsigned tot;  // both classes' kept counts, packed (each <= kMergeTile)
-        (void)block_exclusive_scan((a0.cnt & 0xffu) | ((a1.cnt & 0xffu) << 16), misc.scratch,
-                                   &tot);
         if (bad) atomicOr(&misc.nonfinite, 1u);
+        // wave sums (lane 63) of both classes' tallies, counts packed in 16-bit halves (a tile
+        // keeps and drops <= kMergeTile per class)
         const bool radii = V && k0.validated;  // uniform: without enclosures every radius is 0
-        unsigned nd = ((a0.cnt >> 8) & 0xffu) | (((a1.cnt >> 8) & 0xffu) << 16),
-                 nr = ((a0.cnt >> 16) & 0xffu) | (((a1.cnt >> 16) & 0xffu) << 16);
-        for (int d = 1; d < 64; d <<= 1) {
-            const bool in_wave = lane_id() + d < 64;
-            const double o1 = __shfl_down(a0.l1, d, 64), o2 = __shfl_down(a0.sq, d, 64);
-            const unsigned c1 = __shfl_down(nd, d, 64);
-            if (in_wave) {
-                a0.l1 += o1;
-                a0.sq += o2;
-                nd += c1;
-            }
-            if (anti) {
-                const double p1 = __shfl_down(a1.l1, d, 64), p2 = __shfl_down(a1.sq, d, 64);
-                if (in_wave) {
-                    a1.l1 += p1;
-                    a1.sq += p2;
-                }
-            }
-            if (radii) {
-                const double o3 = __shfl_down(a0.rsq, d, 64), p3 = __shfl_down(a1.rsq, d, 64);
-                const unsigned c2 = __shfl_down(nr, d, 64);
-                if (in_wave) {
-                    a0.rsq += o3;
-                    a1.rsq += p3;
-                    nr += c2;
-                }
-            }
+        const unsigned nk = wave_scan_u32((a0.cnt & 0xffu) | ((a1.cnt & 0xffu) << 16));
+        const unsigned nd =
+            wave_scan_u32(((a0.cnt >> 8) & 0xffu) | (((a1.cnt >> 8) & 0xffu) << 16));
+        a0.l1 = wave_scan_f64(a0.l1);
+        a0.sq = wave_scan_f64(a0.sq);
+        if (anti) {
+            a1.l1 = wave_scan_f64(a1.l1);
+            a1.sq = wave_scan_f64(a1.sq);
         }
-        if (lane_id() == 0) {
+        unsigned nr = 0;
+        if (radii) {
+            nr = wave_scan_u32(((a0.cnt >> 16) & 0xffu) | (((a1.cnt >> 16) & 0xffu) << 16));
+            a0.rsq = wave_scan_f64(a0.rsq);
+            a1.rsq = wave_scan_f64(a1.rsq);
+        }
+        if (lane_id() == 63) {
             const unsigned w = wave_id();
             misc.red[0][w] = a0.l1;
             misc.red[1][w] = a0.sq;
@@ -1286,8 +1299,9 @@ void k_rot_count(BatchDev b, const PassMap *__restrict__ pm) {
             misc.red[3][w] = anti ? a1.l1 : 0.0;
             misc.red[4][w] = anti ? a1.sq : 0.0;
             misc.red[5][w] = anti && radii ? a1.rsq : 0.0;
-            misc.redn[0][w] = nd;
-            misc.redn[1][w] = radii ? nr : 0u;
+            misc.redn[0][w] = nk;
+            misc.redn[1][w] = nd;
+            misc.redn[2][w] = radii ? nr : 0u;
         }
         __syncthreads();
         if (threadIdx.x < 2 && (threadIdx.x == 0 || anti)) {  // thread h writes class h's tile
@@ -1297,13 +1311,15 @@ void k_rot_count(BatchDev b, const PassMap *__restrict__ pm) {
             const unsigned h = threadIdx.x, sh = 16 * h;
             double s1 = misc.red[3 * h][0], s2 = misc.red[3 * h + 1][0],
                    s3 = misc.red[3 * h + 2][0];
-            unsigned c1 = (misc.redn[0][0] >> sh) & 0xffffu, c2 = (misc.redn[1][0] >> sh) & 0xffffu;
+            unsigned c0 = (misc.redn[0][0] >> sh) & 0xffffu;
+            unsigned c1 = (misc.redn[1][0] >> sh) & 0xffffu, c2 = (misc.redn[2][0] >> sh) & 0xffffu;
             for (int w = 1; w < kBlock / 64; ++w) {
                 s1 += misc.red[3 * h][w];
                 s2 += misc.red[3 * h + 1][w];
                 s3 += misc.red[3 * h + 2][w];
-                c1 += (misc.redn[0][w] >> sh) & 0xffffu;
-                c2 += (misc.redn[1][w] >> sh) & 0xffffu;
+                c0 += (misc.redn[0][w] >> sh) & 0xffffu;
+                c1 += (misc.redn[1][w] >> sh) & 0xffffu;
+                c2 += (misc.redn[2][w] >> sh) & 0xffffu;
             }
             TileLedger tl;
             tl.l1 = upmul(s1, kInflate);
@@ -1314,7 +1330,7 @@ void k_rot_count(BatchDev b, const PassMap *__restrict__ pm) {
             tl.nonfinite = misc.nonfinite;
             tl.pad = 0;
             b.tile_led[h ? g2 : g] = tl;
-            b.tile_hist[h ? g2 : g] = (tot >> sh) & 0xffffu;
+            b.tile_hist[h ? g2 : g] = c0;
         }
         __syncthreads();
     }
