@@ -45,7 +45,7 @@ constexpr int kBlock = 256;        // 4 waves of 64
 constexpr int kMergeItems = 4;     // merge-path elements per thread
 constexpr int kMergeTile = kBlock * kMergeItems;
 static_assert(kMergeTile <= 65535, "per-thread splits are stored in 16 bits");
-constexpr int kChunkTiles = 4 * kBlock;  // rotation tiles per scan chunk
+constexpr int kChunkTiles = kBlock;  // rotation tiles per scan chunk (one per thread)
 constexpr uint32_t kSumBlocks = 512;     // blocks of the sum-of-squares reduction
 // PassMap.flags bit: the map's commuting classes cannot lose a key to this rotation's
 // truncation (every coefficient there already exceeds the threshold), so they pass unchanged
@@ -1344,19 +1344,23 @@ __device__ __forceinline__ void ledger_fold(TileLedger &a, const TileLedger &x) 
     a.radii += x.radii;
     a.nonfinite |= x.nonfinite;
 }
-// Fixed-shape block reduction of per-thread ledgers (deterministic for a given input).
-__device__ TileLedger block_ledger(TileLedger mine, TileLedger *sh /*[kBlock]*/) {
-    sh[threadIdx.x] = mine;
-    __syncthreads();
-    for (int st = kBlock / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) {
-            TileLedger a = sh[threadIdx.x];
-            ledger_fold(a, sh[threadIdx.x + st]);
-            sh[threadIdx.x] = a;
-        }
-        __syncthreads();
+// Fixed-shape block reduction of per-thread ledgers by wave shuffles and one fold across the
+// waves (deterministic for a given input); every thread gets the result.
+__device__ TileLedger block_ledger(TileLedger a, TileLedger *sh /*[kBlock / 64]*/) {
+    for (int d = 1; d < 64; d <<= 1) {
+        TileLedger o;
+        o.l1 = __shfl_down(a.l1, d, 64);
+        o.sq = __shfl_down(a.sq, d, 64);
+        o.rsq = __shfl_down(a.rsq, d, 64);
+        o.dropped = __shfl_down(a.dropped, d, 64);
+        o.radii = __shfl_down(a.radii, d, 64);
+        o.nonfinite = __shfl_down(a.nonfinite, d, 64);
+        if (lane_id() + d < 64) ledger_fold(a, o);
     }
+    if (lane_id() == 0) sh[wave_id()] = a;
+    __syncthreads();
     TileLedger r = sh[0];
+    for (int w = 1; w < kBlock / 64; ++w) ledger_fold(r, sh[w]);
     __syncthreads();
     return r;
 }
@@ -1365,7 +1369,7 @@ __device__ TileLedger block_ledger(TileLedger mine, TileLedger *sh /*[kBlock]*/)
 // chunk (exclusive scan of the kept counts), the chunk total and the chunk's ledger partials.
 __global__ __launch_bounds__(kBlock) void k_rot_scan1(BatchDev b, const PassMap *pm) {
     __shared__ unsigned scratch[8];
-    __shared__ TileLedger sh[kBlock];
+    __shared__ TileLedger sh[kBlock / 64];
     const uint32_t M = b.n_maps;
     const uint64_t nchunks = b.chunk_start[M];
     for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
@@ -1407,26 +1411,20 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
     const uint32_t m = blockIdx.x;
     const PassMap &P = pm[m];
     if (!P.active) return;
-    __shared__ uint64_t runs[kBlock];
-    __shared__ TileLedger sh[kBlock];
+    __shared__ unsigned scratch[8];
+    __shared__ TileLedger sh[kBlock / 64];
     const uint64_t c0 = b.chunk_start[m], c1 = b.chunk_start[m + 1];
     uint64_t carry = 0;
     TileLedger L{};
     for (uint64_t base = c0; base < c1; base += kBlock) {
         const uint64_t ch = base + threadIdx.x;
-        const uint64_t v = ch < c1 ? b.chunk_sum[ch] : 0;
+        // a chunk keeps < 2^20 terms, so kBlock of them fit 32 bits
+        const unsigned v = ch < c1 ? b.chunk_sum[ch] : 0u;
         if (ch < c1) ledger_fold(L, b.chunk_led[ch]);
-        runs[threadIdx.x] = v;
-        __syncthreads();
-        for (int d = 1; d < kBlock; d <<= 1) {  // inclusive Hillis-Steele scan
-            uint64_t y = (int)threadIdx.x >= d ? runs[threadIdx.x - d] : 0;
-            __syncthreads();
-            runs[threadIdx.x] += y;
-            __syncthreads();
-        }
-        if (ch < c1) b.chunk_base[ch] = carry + runs[threadIdx.x] - v;
-        carry += runs[kBlock - 1];
-        __syncthreads();
+        unsigned tot;
+        const unsigned ex = block_exclusive_scan(v, scratch, &tot);
+        if (ch < c1) b.chunk_base[ch] = carry + ex;
+        carry += tot;
     }
     const TileLedger T = block_ledger(L, sh);  // (its barriers publish chunk_base too)
     // output class c starts where its first tile writes; an empty class starts where the
@@ -2230,7 +2228,7 @@ __global__ __launch_bounds__(kBlock) void k_fuse_count(BatchDev b, const FusedMa
 // offsets within the chunk, chunk totals, lengths and ledger partials (tile order).
 __global__ __launch_bounds__(kBlock) void k_fuse_scan1(BatchDev b, const FusedMap *fm) {
     __shared__ unsigned scratch[8];
-    __shared__ TileLedger sh[kBlock];
+    __shared__ TileLedger sh[kBlock / 64];
     __shared__ uint64_t ash[kBlock];
     if (b.counter[2]) return;
     const uint32_t n16 = b.n_maps * 16;
@@ -2299,7 +2297,7 @@ __global__ __launch_bounds__(kBlock) void k_fuse_scan1(BatchDev b, const FusedMa
 // group's lengths and ledgers per rotation (chunk order).
 __global__ __launch_bounds__(kBlock) void k_fuse_scan2(BatchDev b, const FusedMap *fm) {
     __shared__ uint64_t runs[kBlock];
-    __shared__ TileLedger sh[kBlock];
+    __shared__ TileLedger sh[kBlock / 64];
     __shared__ uint64_t ash[kBlock];
     if (b.counter[2]) return;
     const uint32_t t = blockIdx.x, m = t >> 4, g = t & 15;
