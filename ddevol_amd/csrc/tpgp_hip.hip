@@ -477,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap 
 // kPartRounds keys (coalesced rounds of 256) in 4-bit counters packed in one register: no LDS
 // atomics, which the sorted keys would serialize (the new digits come in long runs, so a wave's
 // atomics hit one bin).  The counters are widened to 16-bit halves of 8 dwords, summed across
-// the wave by shuffles and across the waves in LDS.
+// the wave by DPP scans and across the waves in LDS.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap *__restrict__ pm) {
     static_assert(kPartRounds <= 15, "4-bit per-thread counters");
@@ -512,11 +512,8 @@ __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap
             h[j] = (unsigned)(two & 0xfu) | ((unsigned)((two >> 4) & 0xfu) << 16);
         }
 #pragma unroll
-        for (int d = 32; d > 0; d >>= 1) {  // <= 64 * kPartRounds per half: no carry
-#pragma unroll
-            for (int j = 0; j < 8; ++j) h[j] += __shfl_xor(h[j], d, 64);
-        }
-        if (lane_id() == 0) {
+        for (int j = 0; j < 8; ++j) h[j] = wave_scan_u32(h[j]);  // <= 64 * kPartRounds per half
+        if (lane_id() == 63) {
 #pragma unroll
             for (int j = 0; j < 8; ++j) whist[wave_id()][j] = h[j];
         }
