@@ -87,7 +87,8 @@ def build(verbose: bool = False) -> None:
         libs = [f"-L{pre}/lib", f"-Wl,-rpath,{pre}/lib", "-lmpfr", "-lgmp"]
     tmp = eng_so + ".tmp"
     cmd += ["-o", tmp, os.path.join(CSRC, "tpgp_engine.c"),
-            os.path.join(CSRC, "tpgp_checkpoint.c"), f"-L{LIB}", "-ltpgp_hip",
+            os.path.join(CSRC, "tpgp_checkpoint.c"), os.path.join(CSRC, "tpgp_residual.c"),
+            f"-L{LIB}", "-ltpgp_hip",
             "-Wl,-rpath,$ORIGIN", *libs, "-lm", "-lpthread"]
     run(cmd)
     os.replace(tmp, eng_so)

@@ -178,6 +178,17 @@ def fid_allowance(pf, led, n_spins):
     return e.tpgp_bound_add_up(pf, math.nextafter(loss, math.inf))
 
 
+def residual_bound(a: Map, b: Map, terms, dt, depth, reference=False):
+    """Independent endpoint bound of one step from a to b (tpgp_residual_bound, residual_rev.c:
+    181-194); returns (status, upper)."""
+    e = N.engine()
+    fn = e.tpgp_residual_bound_reference if reference else e.tpgp_residual_bound
+    up = C.c_double(float("nan"))
+    st = fn(C.byref(a.m), C.byref(b.m), N.terms_array(terms), len(terms), dt, depth,
+            C.byref(up))
+    return st, up.value
+
+
 def liouvillian(mp: Map, terms):
     """-i[H, A] on the GPU (tpgp_liouvillian); returns (status, Map)."""
     out = Map(mp.n_spins)

@@ -135,7 +135,8 @@ ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_map_add_single", "tpgp_map_get", "tpgp_map_l2", "tpgp_map_single_sum",
                   "tpgp_readout", "tpgp_pauli_phase", "tpgp_rotate", "tpgp_strang_step",
                   "tpgp_strang_bound", "tpgp_strang_bound_taylor", "tpgp_bound_add_up",
-                  "tpgp_bound_fid_up", "tpgp_hip_evolve", "tpgp_hip_last_error",
+                  "tpgp_bound_fid_up", "tpgp_residual_bound", "tpgp_residual_bound_reference",
+                  "tpgp_hip_evolve", "tpgp_hip_last_error",
                   "tpgp_evolution_open", "tpgp_evolution_run", "tpgp_evolution_close",
                   "tpgp_evolution_counters_get", "tpgp_evolution_profile",
                   "tpgp_evolution_profile_read", "tpgp_evolution_key_words",
@@ -237,6 +238,9 @@ def engine():
     f("tpgp_strang_bound_taylor", C.c_double, C.POINTER(TpgpTerm), sz, C.c_double)
     f("tpgp_bound_add_up", C.c_double, C.c_double, C.c_double)
     f("tpgp_bound_fid_up", C.c_double, C.c_double, sz)
+    for name in ("tpgp_residual_bound", "tpgp_residual_bound_reference"):
+        f(name, C.c_int, mp, mp, C.POINTER(TpgpTerm), sz, C.c_double, C.c_uint,
+          C.POINTER(C.c_double))
     f("tpgp_hip_evolve", C.c_int, mp, C.POINTER(TpgpTerm), sz, C.c_double, sz, op,
       C.POINTER(EvolveOptions), lp, C.POINTER(C.c_double), C.POINTER(C.c_double),
       C.POINTER(sz), C.POINTER(sz))

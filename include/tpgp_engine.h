@@ -92,6 +92,17 @@ double tpgp_strang_bound_taylor(const tpgp_term *terms, size_t count, double dur
 /* engine/include/residual_rev.h (tpgp_bound_add_up / tpgp_bound_fid_up, used for allowances) */
 double tpgp_bound_add_up(double a, double b);
 double tpgp_bound_fid_up(double a, size_t n);
+/* engine/include/residual_rev.h:1-27 (residual_rev.c:181-194): the independent endpoint bound
+ * of one Strang step from the map before (a) and after (b) it, in 256-bit MPFR interval
+ * arithmetic; the rounded-up binary64 result equals the reference's.  `_reference` is the
+ * diagnostic variant (every term scanned, no Gram enclosure).  EBACKEND without MPFR, EINVAL
+ * for malformed maps or terms, ENOMEM, ENUMERIC for a non-finite result; *upper unchanged on
+ * failure.  depth <= 10. */
+tpgp_status tpgp_residual_bound(const tpgp_map *a, const tpgp_map *b, const tpgp_term *terms,
+                                size_t count, double dt, unsigned depth, double *upper);
+tpgp_status tpgp_residual_bound_reference(const tpgp_map *a, const tpgp_map *b,
+                                          const tpgp_term *terms, size_t count, double dt,
+                                          unsigned depth, double *upper);
 
 /* ---- resident evolution: the step loop of engine/src/cli_rev.c:575-670 on one GPU ---------
  * Uploads `map` once, runs `steps` Strang steps with the map resident in HBM, and reads the FID

@@ -1,0 +1,157 @@
+"""The independent endpoint (residual) bound, tpgp_residual_bound (host, MPFR).
+
+1. Bit for bit against the reference library compiled from its own sources (oracle/_ref), both
+   variants (Gram enclosure, and every term scanned with direct norms), depths 0..4, one- and
+   two-word keys, on consecutive maps of Strang trajectories from the oracle.
+2. Against the mathematics: for N = 4 the generator L is a 256 x 256 matrix over Pauli strings;
+   the exact one-step image exp(dt L) a of the map a lies within the bound of the Strang step's
+   result b (the bound encloses the whole endpoint error of the step), for several dt.
+3. The same refusals as the reference (EINVAL for malformed input, depth > 10, dt <= 0).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from ddevol_amd import engine as E
+from ddevol_amd import hamiltonians as H
+from ddevol_amd import native as N
+from oracle import bindings as OB
+
+OK, EINVAL = 0, 1
+
+
+def has_mpfr():
+    return N.engine().tpgp_has_mpfr() == 1
+
+
+pytestmark = pytest.mark.skipif(not has_mpfr(), reason="built without MPFR")
+
+
+def trajectory(oracle, n, terms, dt, steps, thr):
+    m = oracle.new_map(n)
+    for s in range(n):
+        assert m.add_single(s, 1, 1.0) == 0
+    o = oracle.options(threshold=thr, max_terms=10 ** 8)
+    led = oracle.ledger(0)
+    ta = OB.terms_array(terms)
+    maps = []
+    for k in range(steps + 1):
+        keys, vals = m.arrays()
+        maps.append(E.Map.from_sorted(n, keys, vals))
+        if k < steps:
+            assert m.strang_step(ta, len(terms), dt, o, led) == 0
+    return maps
+
+
+@pytest.fixture(scope="module")
+def ref_lib():
+    if not os.path.exists(OB.REF_SO):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    lib = C.CDLL(OB.REF_SO)
+    for name in ("tpgp_residual_bound", "tpgp_residual_bound_reference"):
+        fn = getattr(lib, name)
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_double, C.c_uint,
+                       C.POINTER(C.c_double)]
+    return lib
+
+
+def ref_bound(lib, a, b, terms, dt, depth, reference):
+    fn = lib.tpgp_residual_bound_reference if reference else lib.tpgp_residual_bound
+    up = C.c_double(float("nan"))
+    ta = N.terms_array(terms)
+    st = fn(C.addressof(a.m), C.addressof(b.m), C.cast(ta, C.c_void_p), len(terms), dt, depth,
+            C.byref(up))
+    return st, up.value
+
+
+def bits(x):
+    return np.float64(x).view(np.uint64)
+
+
+@pytest.mark.parametrize("n,model,dt,thr", [(6, "j1j2", 0.05, 1e-6), (8, "ring", 0.02, 0.0),
+                                            (12, "ring", 0.01, 1e-8), (40, "serving", 0.01, 1e-4)])
+def test_bit_identical_to_reference(oracle, ref_lib, n, model, dt, thr):
+    terms = (H.j1j2(n, 0.3) if model == "j1j2" else H.ring(n) if model == "ring"
+             else H.serving_ring(n, [0.5 + 0.01 * i for i in range(n)]))
+    maps = trajectory(oracle, n, terms, dt, 2, thr)
+    for k in range(2):
+        for depth in (0, 1, 4):
+            for reference in (False, True):
+                st, up = E.residual_bound(maps[k], maps[k + 1], terms, dt, depth, reference)
+                rst, rup = ref_bound(ref_lib, maps[k], maps[k + 1], terms, dt, depth, reference)
+                assert st == rst == OK
+                assert bits(up) == bits(rup), (n, k, depth, reference, up, rup)
+
+
+def pauli_index_generator(n, terms):
+    """Dense L over the 4^n Pauli strings (index = key bits): L[P ^ q, P] += +-2h for every term
+    anticommuting with P, the sign from the phase of the product (I0 X1 Y2 Z3)."""
+    dim = 4 ** n
+    L = np.zeros((dim, dim))
+
+    def phase(q, p):
+        if not q or not p or q == p:
+            return 0
+        return 1 if (p + 3 - q) % 3 == 1 else 3
+
+    for key in range(dim):
+        for (i, j, qi, qj, h) in terms:
+            a, b = (key >> (2 * i)) & 3, (key >> (2 * j)) & 3
+            p = (phase(qi, a) + phase(qj, b)) & 3
+            if p & 1 and h != 0:
+                L[key ^ (qi << (2 * i)) ^ (qj << (2 * j)), key] += 2 * h if p == 1 else -2 * h
+    return L
+
+
+@pytest.mark.parametrize("dt", [0.02, 0.1, 0.3])
+def test_bound_encloses_exact_step(oracle, dt):
+    from scipy.linalg import expm
+
+    n = 4
+    terms = H.j1j2(n, 0.4)
+    maps = trajectory(oracle, n, terms, dt, 3, 0.0)
+    L = pauli_index_generator(n, terms)
+    U = expm(dt * L)
+    for k in range(3):
+        a = np.zeros(4 ** n)
+        b = np.zeros(4 ** n)
+        for vec, mp in ((a, maps[k]), (b, maps[k + 1])):
+            keys, vals = mp.arrays()
+            vec[keys[:, 0].astype(np.int64)] = vals
+        err = np.linalg.norm(b - U @ a)
+        for depth in (0, 2):
+            st, up = E.residual_bound(maps[k], maps[k + 1], terms, dt, depth)
+            assert st == OK
+            assert err <= up * (1 + 1e-12), (k, depth, err, up)
+        # and it is a real statement: within a modest factor of the true error
+        assert up < 1e3 * max(err, 1e-300)
+
+
+def test_refusals_match_reference(ref_lib):
+    n = 4
+    terms = H.ring(n)
+    a = E.Map(n)
+    for s in range(n):
+        a.add_single(s, 1, 1.0)
+    b = E.Map(n)
+    b.add_single(0, 1, 0.5)
+    other = E.Map(6)
+    other.add_single(0, 1, 1.0)
+    cases = [(a, b, terms, 0.01, 11), (a, b, terms, 0.0, 1), (a, b, terms, -0.01, 1),
+             (a, b, terms, float("nan"), 1), (a, other, terms, 0.01, 1),
+             (a, b, [(0, 0, 3, 3, 1.0)], 0.01, 1), (a, b, [(0, 5, 3, 3, 1.0)], 0.01, 1),
+             (a, b, [(0, 1, 4, 3, 1.0)], 0.01, 1), (a, b, [(0, 1, 3, 3, float("inf"))], 0.01, 1)]
+    for (x, y, t, dt, depth) in cases:
+        st, up = E.residual_bound(x, y, t, dt, depth)
+        rst, _ = ref_bound(ref_lib, x, y, t, dt, depth, False)
+        assert st == rst == EINVAL, (dt, depth, st, rst)
+        assert np.isnan(up)  # output untouched on failure
+    # no terms: the generator vanishes, the bound is the pure endpoint difference
+    st, up = E.residual_bound(a, b, [], 0.01, 0)
+    rst, rup = ref_bound(ref_lib, a, b, [], 0.01, 0, False)
+    assert st == rst == OK and bits(up) == bits(rup) and up > 0
