@@ -15,7 +15,9 @@
  *   --backend takes hip (or cuda, the slot the GPU occupies); the CPU and Metal backends do not
  *   exist here.  A restart continues on the GPU whatever backend its record names, and records
  *   written afterwards name the GPU.
- *   --residual-depth (the independent residual bound of residual_rev.c) is not available.
+ * --residual-depth D adds the independent endpoint bound of every step (tpgp_residual_bound,
+ * host MPFR) from the maps before and after it, which are then copied out once per step
+ * (cli_rev.c:640-667).
  */
 #define _POSIX_C_SOURCE 200809L
 #include "../../include/tpgp_engine.h"
@@ -307,7 +309,7 @@ typedef struct {
 
 static tpgp_status write_checkpoint(const run_info *ri, const tpgp_map *map,
                                     const tpgp_options *o, const tpgp_ledger *led, double pf,
-                                    size_t step, int reason, tpgp_status last) {
+                                    double residual, size_t step, int reason, tpgp_status last) {
     tpgp_checkpoint s;
     memset(&s, 0, sizeof(s));
     s.map = *map;
@@ -318,11 +320,11 @@ static tpgp_status write_checkpoint(const run_info *ri, const tpgp_map *map,
     s.ledger = *led;
     s.dt = ri->c->dt;
     s.pf_bound = pf;
-    s.residual_bound = NAN;
+    s.residual_bound = residual;
     s.max_fid_bound = ri->c->max_fid_bound;
     s.step = step;
     s.total_steps = ri->steps;
-    s.residual_depth = -1;
+    s.residual_depth = ri->c->residual_depth;
     s.initial_global = ri->global;
     snprintf(s.model, sizeof(s.model), "%s", ri->model);
     s.j2 = ri->c->j2;
@@ -365,11 +367,6 @@ int main(int argc, char **argv) {
     c.o.max_terms = 1000000;
     const int parsed = parse_args(argc, argv, &c);
     if (parsed) return parsed == 1 ? EXIT_DONE : EXIT_USAGE;
-    if (c.residual_depth >= 0) {
-        fprintf(stderr, "The independent residual bound (--residual-depth) is not available in "
-                        "this driver.\n");
-        return EXIT_USAGE;
-    }
     if (c.restart && !c.checkpoint) c.checkpoint = c.restart;
     int lock = -1;
     if (c.checkpoint && (lock = take_lock(c.checkpoint, c.restart)) < 0) return EXIT_USAGE;
@@ -379,7 +376,7 @@ int main(int argc, char **argv) {
     size_t prior_steps = 0, prior_cap = 0;
     int code = EXIT_USAGE;
     term_list terms = {0};
-    tpgp_map m = {0};
+    tpgp_map m = {0}, before = {0}, after = {0}; /* before/after: around each step (residual) */
     tpgp_evolution *ev = NULL;
     tpgp_hip_context *ctx = NULL;
     FILE *out = NULL;
@@ -411,6 +408,7 @@ int main(int argc, char **argv) {
         c.o.metal_context = NULL;
         if (c.given >> F_CAP & 1) c.o.max_terms = cap;
         if (!(c.given >> F_MAX_FID & 1)) c.max_fid_bound = rec.max_fid_bound;
+        c.residual_depth = rec.residual_depth;
     }
     if (((c.walltime > 0 || c.max_fid_bound > 0) && !c.checkpoint) ||
         (c.checkpoint && c.output && !strcmp(c.checkpoint, c.output))) {
@@ -442,7 +440,12 @@ int main(int argc, char **argv) {
         fprintf(stderr, "Validated mode requires MPFR at build time.\n");
         goto done;
     }
-    if (c.max_fid_bound > 0 && (!tpgp_has_mpfr() || !c.o.validated)) {
+    if (c.residual_depth >= 0 && (!tpgp_has_mpfr() || strcmp(c.initial, "global"))) {
+        fprintf(stderr, "Residual CLI bounds require MPFR and global initial sum Xi.\n");
+        goto done;
+    }
+    if (c.max_fid_bound > 0 &&
+        (!tpgp_has_mpfr() || (!c.o.validated && c.residual_depth < 0))) {
         fprintf(stderr, "An accuracy stop requires a complete MPFR-enclosed error bound.\n");
         goto done;
     }
@@ -512,6 +515,14 @@ int main(int argc, char **argv) {
     const double start = now_seconds(), per_step = tpgp_strang_bound(terms.data, terms.len, c.dt),
                  norm0 = sqrt((double)input_terms);
     double pf = c.restart ? rec.pf_bound : 0, last_save_time = start;
+    double residual = c.residual_depth < 0 ? NAN : c.restart ? rec.residual_bound : 0;
+    tpgp_ledger before_ledger = led;
+    int rolled_back = 0, have_after = 0;
+    if (c.residual_depth >= 0 &&
+        ((status = tpgp_map_init(&before, c.n)) || (status = tpgp_map_init(&after, c.n)))) {
+        code = EXIT_FAILED;
+        goto done;
+    }
     size_t accepted = ri.first_step, last_saved = SIZE_MAX;
     int paused = 0, accuracy = 0;
     signal(SIGINT, on_signal);
@@ -543,17 +554,22 @@ int main(int argc, char **argv) {
         if ((status = tpgp_evolution_l2(ev, &l2))) break;
         double pf_obs, total;
         row_bounds(pf, &led, c.n, input_terms, norm0, radius[0], &pf_obs, &total);
+        const double residual_fid =
+            c.residual_depth >= 0
+                ? tpgp_bound_add_up(tpgp_bound_fid_up(residual, c.n), radius[0])
+                : NAN;
         fprintf(out,
                 "%.17g,%.17g,%zu,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%d,"
                 "%.6f,%.3f\n",
                 step * c.dt, fid[0], active[0], l2, led.dropped_l1, led.dropped_l2,
-                led.dropped_l2_squared, led.arithmetic_l2, pf_obs, total, NAN, NAN, radius[0],
-                led.arithmetic_enclosed, now_seconds() - start, peak_rss_mb());
+                led.dropped_l2_squared, led.arithmetic_l2, pf_obs, total, residual, residual_fid,
+                radius[0], led.arithmetic_enclosed, now_seconds() - start, peak_rss_mb());
         if (fflush(out)) {
             status = TPGP_EINVAL;
             break;
         }
-        const double best = c.o.validated && led.arithmetic_enclosed ? total : INFINITY;
+        double best = c.o.validated && led.arithmetic_enclosed ? total : INFINITY;
+        if (c.residual_depth >= 0 && residual_fid < best) best = residual_fid;
         const int reason = c.max_fid_bound > 0 && best > c.max_fid_bound     ? STOP_ACCURACY
                            : step == steps                                 ? STOP_COMPLETE
                            : g_stop                                        ? STOP_SIGNAL
@@ -564,7 +580,8 @@ int main(int argc, char **argv) {
                          now_seconds() - last_save_time >= c.checkpoint_seconds);
         if (c.checkpoint && due && last_saved != step) {
             if ((status = tpgp_evolution_snapshot(ev, &m))) break;
-            if ((status = write_checkpoint(&ri, &m, &c.o, &led, pf, step, reason, TPGP_OK)))
+            if ((status = write_checkpoint(&ri, &m, &c.o, &led, pf, residual, step, reason,
+                                           TPGP_OK)))
                 break;
             last_saved = step;
             last_save_time = now_seconds();
@@ -574,12 +591,36 @@ int main(int argc, char **argv) {
             accuracy = reason == STOP_ACCURACY;
             break;
         }
-        const size_t before = active[0];
+        const size_t active_before = active[0];
+        if (c.residual_depth >= 0) {
+            before_ledger = led;
+            if (!have_after && (status = tpgp_evolution_snapshot(ev, &after))) break;
+            tpgp_map t = before; /* `before` = the map now (the last step's `after`) */
+            before = after;
+            after = t;
+            have_after = 0;
+        }
         status = tpgp_evolution_run(ev, 1, &led, fid, radius, active, &done_steps);
         if (status) {
             fprintf(stderr, "Step %zu rejected: %s; last accepted time %.17g, active=%zu.\n",
-                    step + 1, tpgp_status_string(status), step * c.dt, before);
+                    step + 1, tpgp_status_string(status), step * c.dt, active_before);
             break;
+        }
+        if (c.residual_depth >= 0) {
+            double increment = 0;
+            status = tpgp_evolution_snapshot(ev, &after);
+            have_after = !status;
+            if (!status)
+                status = tpgp_residual_bound(&before, &after, terms.data, terms.len, c.dt,
+                                             (unsigned)c.residual_depth, &increment);
+            if (status) {
+                fprintf(stderr, "Independent endpoint bound failed after step %zu: %s.\n",
+                        step + 1, tpgp_status_string(status));
+                led = before_ledger; /* the step is not accepted */
+                rolled_back = 1;
+                break;
+            }
+            residual = tpgp_bound_add_up(residual, increment);
         }
         fid[0] = fid[1];
         radius[0] = radius[1];
@@ -591,8 +632,11 @@ int main(int argc, char **argv) {
         const int reason = status == TPGP_ECAP || status == TPGP_ENOMEM || status == TPGP_EBUDGET
                                ? STOP_RESOURCE
                                : STOP_NUMERICAL;
-        if (!tpgp_evolution_snapshot(ev, &m))
-            (void)write_checkpoint(&ri, &m, &c.o, &led, pf, accepted, reason, status);
+        if (rolled_back) /* the map before the step whose bound failed */
+            (void)write_checkpoint(&ri, &before, &c.o, &led, pf, residual, accepted, reason,
+                                   status);
+        else if (!tpgp_evolution_snapshot(ev, &m))
+            (void)write_checkpoint(&ri, &m, &c.o, &led, pf, residual, accepted, reason, status);
     }
     fprintf(stderr,
             "status=%s accepted_rotations=%zu peak_terms=%zu elapsed=%.3fs peak_rss=%.3fMiB "
@@ -609,6 +653,8 @@ done:
     if (ctx) tpgp_hip_destroy(ctx);
     free(terms.data);
     tpgp_map_free(&m);
+    tpgp_map_free(&before);
+    tpgp_map_free(&after);
     tpgp_checkpoint_free(&rec);
     if (lock >= 0) close(lock);
     return code;

@@ -13,8 +13,10 @@ script runs it on small FID runs and records what ddevol_amd/lib/tpgp_rev must r
 * j1j2_n16.edges: the coupling file of the J1-J2 N=16 case (J2_i from InputPCG(7)).
 
 Cases: every model (ring, j1j2 validated, ising, xy, cubic L=3, edges), two-word keys (ring
-N=40, validated J1-J2 N=36), the local initial state, a drop-budget failure, a restart with a longer horizon, a cap failure resumed with a
-larger cap, an accuracy stop, and a walltime pause resumed to completion.
+N=40, validated J1-J2 N=36), the local initial state, a drop-budget failure, a restart with a
+longer horizon, a cap failure resumed with a larger cap, an accuracy stop, a walltime pause
+resumed to completion, and the independent endpoint bound (--residual-depth: plain, validated,
+a restart chain and an accuracy stop it drives).
 
 Run in the build container (where the reference sources exist):
     make -C oracle ref && python tests/golden/make_cli_golden.py
@@ -67,6 +69,18 @@ CASES = {
     "walltime": [["--model", "j1j2", "--n", "9", "--time", "0.05", "--threshold", "1e-6",
                   "--walltime", "1e-9", "--checkpoint", "{ck}"],
                  ["--restart", "{ck}"]],
+    # the independent endpoint bound (residual columns, records, accuracy stops)
+    "residual8": [["--n", "8", "--time", "0.05", "--threshold", "1e-6", "--residual-depth",
+                   "2"]],
+    "residual_j1j2_validated": [["--model", "j1j2", "--n", "8", "--time", "0.03", "--threshold",
+                                 "1e-6", "--validated", "--residual-depth", "1"]],
+    "residual_restart": [["--n", "8", "--time", "0.03", "--threshold", "1e-6",
+                          "--residual-depth", "1", "--checkpoint", "{ck}",
+                          "--checkpoint-every", "2"],
+                         ["--restart", "{ck}", "--time", "0.06"]],
+    "residual_accuracy": [["--n", "8", "--time", "0.1", "--threshold", "1e-6",
+                           "--residual-depth", "0", "--max-fid-bound", "1e-7",
+                           "--checkpoint", "{ck}"]],
 }
 
 

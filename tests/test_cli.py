@@ -15,7 +15,7 @@ pytestmark = pytest.mark.skipif(not os.path.exists(G.OURS), reason="driver not b
 def test_help_and_backend_refusals(tmp_path):
     assert G.run(G.OURS, ["--help"], tmp_path / "h.csv")[0] == 0
     for args in (["--backend", "serial"], ["--backend", "openmp"], ["--bogus", "1"],
-                 ["--n"], ["--residual-depth", "2"], ["--dt", "nan"], ["--cap", "-5"],
+                 ["--n"], ["--residual-depth", "11"], ["--dt", "nan"], ["--cap", "-5"],
                  ["--threads", "0"], ["--max-fid-bound", "0"]):
         code, _, rows, err = G.run(G.OURS, args, tmp_path / "x.csv")
         assert code == 2 and not rows, (args, code, err)
@@ -33,8 +33,10 @@ CASES = [
     (["--model", "edges", "--n", "16"], True),           # no coupling file
     (["--model", "triangle"], True),
     (["--initial", "somewhere"], True),
-    # an accuracy stop needs an enclosed bound (validated mode)
+    # an accuracy stop needs an enclosed bound (validated mode or the residual bound)
     (["--max-fid-bound", "1e-3", "--checkpoint", "{ck}"], True),
+    # the residual bound is for the global initial state only
+    (["--residual-depth", "1", "--initial", "local"], True),
 ]
 
 

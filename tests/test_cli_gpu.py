@@ -57,7 +57,8 @@ def test_driver_matches_reference(tmp_path, case):
             assert mine.read_bytes() == open(want, "rb").read(), (case, k)
 
 
-@pytest.mark.parametrize("case,first", [("restart", 0), ("cap", 0), ("walltime", 0)])
+@pytest.mark.parametrize("case,first", [("restart", 0), ("cap", 0), ("walltime", 0),
+                                        ("residual_restart", 0)])
 def test_driver_resumes_reference_records(tmp_path, case, first):
     """The reference's record after invocation `first`, resumed here, continues exactly as the
     reference's own resumed run."""
@@ -74,7 +75,7 @@ def test_driver_resumes_reference_records(tmp_path, case, first):
 
 
 @pytest.mark.skipif(not os.path.exists(G.REF_CLI), reason="reference driver not built")
-@pytest.mark.parametrize("case", ["restart", "cap", "walltime"])
+@pytest.mark.parametrize("case", ["restart", "cap", "walltime", "residual_restart"])
 def test_reference_resumes_our_records(tmp_path, case):
     """Our record after the first invocation, resumed by the reference's driver (on its CPU
     backend), continues exactly as the reference's own chain."""
