@@ -3104,7 +3104,6 @@ struct tpgp_hip_batch {
     std::vector<void *> allocations;
     bool poisoned = false;
     bool validate = false;  // TPGP_HIP_VALIDATE=1: check order and regions after every pass
-    uint64_t est_tiles = 1;  // merge tiles of the last planned rotation (grid sizing only)
     uint64_t log_entries = 1u << 20;  // device log capacity (grown by reserve_log)
     // path choice (per rotate_many call): modelled costs of one rotation, or a fixed bound
     double cost_launch_us = 45.0;  // multi-kernel chain of one rotation (launches, latency)
@@ -3359,7 +3358,6 @@ unsigned rotation_grid(tpgp_hip_batch *b, const tpgp_hip_rotation *req) {
     uint64_t t = 0;
     for (uint32_t m = 0; m < b->M; ++m)
         if (req[m].active && !b->small[m]) t += (3 * b->maps[m].est_len / 2) / kMergeTile + 16;
-    b->est_tiles = t;
     return tile_grid(t);
 }
 
@@ -3699,6 +3697,9 @@ unsigned plan_rotation(tpgp_hip_batch *b, const tpgp_hip_rotation *req, uint32_t
 
 void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool validated) {
     hipStream_t st = b->ctx->stream;
+    // this rotation's tile estimate, from its own grid (rgrid = tile_grid(tiles)); planning may
+    // run ahead of launching, so the batch's last estimate can belong to a later rotation
+    const uint64_t est_tiles = rgrid > 64 ? (uint64_t)(rgrid - 64) * 16 / 17 + 1 : 1;
     {
         ProfileScope whole(b, TPGP_HIP_PROFILE_ROTATION);
         const int fused_plan = b->M <= kPlanMaps;  // the split pass plans too
@@ -3708,7 +3709,7 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
         }
         {
             ProfileScope pc(b, TPGP_HIP_PROFILE_SPLITS);
-            const unsigned sgrid = tile_grid(b->est_tiles / 32 + 1);  // 8 lanes per tile
+            const unsigned sgrid = tile_grid(est_tiles / 32 + 1);  // 8 lanes per tile
             if (b->W == 1)
                 hipLaunchKernelGGL(k_rot_splits<1>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d,
                                    fused_plan);
@@ -3736,7 +3737,7 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
         }
         {
             ProfileScope psn(b, TPGP_HIP_PROFILE_SCAN);
-            const uint64_t want = b->est_tiles / kChunkTiles + b->M + 1;
+            const uint64_t want = est_tiles / kChunkTiles + b->M + 1;
             const unsigned cgrid = (unsigned)(want < (1u << 20) ? want : (1u << 20));
             hipLaunchKernelGGL(k_rot_scan1, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d);
             hipLaunchKernelGGL(k_rot_scan2, dim3(b->M), dim3(kBlock), 0, st, b->dev, d);
