@@ -761,12 +761,14 @@ struct MergeCtx {
     uint32_t m, c, cq;
     bool anti;
     uint64_t A0, nA, D0, nD, k;
+    uint64_t off;     // the map's slot offset in the arena
+    uint32_t in_buf;  // the arena buffer holding the rotation's input
 };
 // Where a rotation tile lies: its map and class, the class regions it merges, its index in the
 // class; flags bit0: anticommuting class, bit1: last tile of its class.
 struct TileDesc {
-    uint64_t A0, nA, D0, nD, k;
-    uint32_t m, c, cq, flags;
+    uint64_t A0, nA, D0, nD, k, off;
+    uint32_t m, c, cq, flags, in_buf, pad;
 };
 
 __device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm,
@@ -783,6 +785,8 @@ __device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm,
     x.D0 = reg[x.cq];
     x.nD = x.anti ? reg[x.cq + 1] - reg[x.cq] : 0;
     x.k = g - tstart[t];
+    x.off = b.slot[x.m].off;
+    x.in_buf = P.in_buf;
 }
 
 // Tile inputs in LDS: A at [0, na), T(D) at [na, na+nd), the D element after the tile at na+nd
@@ -823,8 +827,9 @@ __device__ void load_tile(const BatchDev &b, const PassMap &P, const MergeCtx &x
     t.nd = (uint32_t)(t.b1 - t.b0);
     t.has_next = x.anti && t.b1 < x.nD;
     t.has_prev = x.anti && t.a0 > 0;
-    const uint64_t *ikeys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
-    const double *ivals = b.arena[P.in_buf].vals + b.slot[x.m].off;
+    // addresses from the tile's own descriptor: no load of the map's descriptors on the way
+    const uint64_t *ikeys = b.arena[x.in_buf].keys + x.off * W;
+    const double *ivals = b.arena[x.in_buf].vals + x.off;
     constexpr int R = kMergeTile / kBlock;
     const uint32_t na = t.na, tot = t.na + t.nd;  // tot <= kMergeTile (checked by tile_ok)
     const uint64_t abase = x.A0 + t.a0, dbase = x.D0 + t.b0 - na;
@@ -1129,6 +1134,9 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
             t.m = x.m;
             t.c = x.c;
             t.cq = x.cq;
+            t.off = x.off;
+            t.in_buf = x.in_buf;
+            t.pad = 0;
             const bool last = g + 1 >= tstart[x.m * 16 + x.c + 1];
             t.flags = (x.anti ? 1u : 0u) | (last ? 2u : 0u);
             b.tdesc[g] = t;
@@ -1156,8 +1164,11 @@ __device__ __forceinline__ void tile_from_desc(const BatchDev &b, uint64_t g, Me
     x.c = t.c;
     x.cq = t.cq;
     x.anti = (t.flags & 1u) != 0;
+    x.off = t.off;
+    x.in_buf = t.in_buf;
     a0 = b.splits[g];
-    a1 = (t.flags & 2u) ? t.nA : b.splits[g + 1];
+    const uint64_t next = b.splits[g + 1];  // unconditional (splits has maxT + 1 entries): the
+    a1 = (t.flags & 2u) ? t.nA : next;      // load does not wait for the descriptor
 }
 
 // The coefficients of the partner class c ^ q of an anticommuting tile: its own entries are
@@ -1505,8 +1516,8 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
         double *gvals = b.arena[P.out_buf].vals + b.slot[x.m].off;
         if (!x.anti && (P.flags & kPassCommutingKept)) {  // unchanged keys: a straight copy
             if (out_pos + count > cap) continue;
-            const uint64_t *sk = b.arena[P.in_buf].keys + (b.slot[x.m].off + x.A0 + t.d0) * W;
-            const double *sv = b.arena[P.in_buf].vals + b.slot[x.m].off + x.A0 + t.d0;
+            const uint64_t *sk = b.arena[x.in_buf].keys + (x.off + x.A0 + t.d0) * W;
+            const double *sv = b.arena[x.in_buf].vals + x.off + x.A0 + t.d0;
             uint64_t *gk = gkeys + out_pos * W;
             double *gv = gvals + out_pos;
             constexpr int R = kMergeTile / kBlock;  // count <= kMergeTile: loads first, then stores
@@ -2548,6 +2559,8 @@ __device__ __forceinline__ void small_rotation(const BatchDev &b, const PassMap 
         x.D0 = rin[x.cq];
         x.nD = x.anti ? rin[x.cq + 1] - rin[x.cq] : 0;
         x.k = 0;
+        x.off = b.slot[m].off;
+        x.in_buf = P.in_buf;
         const Coef k = coef_of(P, x);
         const uint64_t total = x.nA + x.nD;
         uint64_t a0 = 0;
