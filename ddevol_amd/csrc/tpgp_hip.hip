@@ -1413,33 +1413,14 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan1(BatchDev b, const PassMap 
     (void)pm;
 }
 
-// Scan level 2, one block per map: chunk offsets, output regions and length, the ledger fold
-// in chunk order, and the record.
-__global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap *pm) {
-    const uint32_t m = blockIdx.x;
-    const PassMap &P = pm[m];
-    if (!P.active) return;
-    __shared__ unsigned scratch[8];
-    __shared__ TileLedger sh[kBlock / 64];
-    const uint64_t c0 = b.chunk_start[m], c1 = b.chunk_start[m + 1];
-    uint64_t carry = 0;
-    TileLedger L{};
-    for (uint64_t base = c0; base < c1; base += kBlock) {
-        const uint64_t ch = base + threadIdx.x;
-        // a chunk keeps < 2^20 terms, so kBlock of them fit 32 bits
-        const unsigned v = ch < c1 ? b.chunk_sum[ch] : 0u;
-        if (ch < c1) ledger_fold(L, b.chunk_led[ch]);
-        unsigned tot;
-        const unsigned ex = block_exclusive_scan(v, scratch, &tot);
-        if (ch < c1) b.chunk_base[ch] = carry + ex;
-        carry += tot;
-    }
-    const TileLedger T = block_ledger(L, sh);  // (its barriers publish chunk_base too)
+// The end of a map's rotation scan (whole block, chunk_base and tile_base of the map written and
+// published): output regions and length, overflow, and the record with the map's ledger T.
+__device__ void finish_map_scan(const BatchDev &b, const PassMap &P, uint32_t m, uint64_t acc,
+                                const TileLedger &T, uint64_t *cstart /*[16], LDS*/) {
     // output class c starts where its first tile writes; an empty class starts where the
     // next nonempty class does (or at the end)
-    __shared__ uint64_t cstart[16];
     if (threadIdx.x < 16) {
-        const uint64_t tm0 = b.tstart[m * 16];
+        const uint64_t tm0 = b.tstart[m * 16], c0 = b.chunk_start[m];
         const uint64_t t = b.tstart[m * 16 + threadIdx.x], te = b.tstart[m * 16 + threadIdx.x + 1];
         cstart[threadIdx.x] =
             t < te ? b.chunk_base[c0 + (t - tm0) / kChunkTiles] + b.tile_base[t] : ~0ull;
@@ -1448,7 +1429,6 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
     if (threadIdx.x == 0) {
         const uint64_t *rin = b.regions + ((uint64_t)m * 3 + P.in_buf) * 17;
         uint64_t *rout = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
-        const uint64_t acc = carry;
         // A map that outgrew its slot is emptied: later passes never read past the slot; its
         // record carries the overflow and the host rolls the map back and grows the slot.
         const bool over = b.overflow[m] || acc > b.slot[m].cap;
@@ -1474,6 +1454,64 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
         r.overflow = over ? 1u : 0u;
         r.rotation = P.rot_seq;
         b.rec[(uint64_t)m * b.max_rec + P.rec_slot] = r;
+    }
+    __syncthreads();
+}
+
+// Scan level 2, one block per map: chunk offsets, output regions and length, the ledger fold
+// in chunk order, and the record.
+__global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap *pm) {
+    const uint32_t m = blockIdx.x;
+    const PassMap &P = pm[m];
+    if (!P.active) return;
+    __shared__ unsigned scratch[8];
+    __shared__ TileLedger sh[kBlock / 64];
+    __shared__ uint64_t cstart[16];
+    const uint64_t c0 = b.chunk_start[m], c1 = b.chunk_start[m + 1];
+    uint64_t carry = 0;
+    TileLedger L{};
+    for (uint64_t base = c0; base < c1; base += kBlock) {
+        const uint64_t ch = base + threadIdx.x;
+        // a chunk keeps < 2^20 terms, so kBlock of them fit 32 bits
+        const unsigned v = ch < c1 ? b.chunk_sum[ch] : 0u;
+        if (ch < c1) ledger_fold(L, b.chunk_led[ch]);
+        unsigned tot;
+        const unsigned ex = block_exclusive_scan(v, scratch, &tot);
+        if (ch < c1) b.chunk_base[ch] = carry + ex;
+        carry += tot;
+    }
+    const TileLedger T = block_ledger(L, sh);  // (its barriers publish chunk_base too)
+    finish_map_scan(b, P, m, carry, T, cstart);
+}
+
+// The whole rotation scan of small maps in one launch, one block per map: its chunks one after
+// another (tile offsets within the chunk, chunk offsets), the ledger folded per thread across
+// the chunks, then the map's regions and record.  Correct for any size; used when the launch's
+// maps are small, where the two-level scan's second launch costs more than the serial chunks.
+__global__ __launch_bounds__(kBlock) void k_rot_scan_maps(BatchDev b, const PassMap *pm) {
+    __shared__ unsigned scratch[8];
+    __shared__ TileLedger sh[kBlock / 64];
+    __shared__ uint64_t cstart[16];
+    for (uint32_t m = blockIdx.x; m < b.n_maps; m += gridDim.x) {
+        const PassMap &P = pm[m];
+        if (!P.active) continue;
+        const uint64_t t0 = b.tstart[m * 16], t1 = b.tstart[m * 16 + 16];
+        uint64_t carry = 0, ch = b.chunk_start[m];
+        TileLedger L{};
+        for (uint64_t g0 = t0; g0 < t1; g0 += kChunkTiles, ++ch) {  // one tile per thread
+            const uint64_t g = g0 + threadIdx.x;
+            const unsigned cnt = g < t1 ? b.tile_hist[g] : 0u;
+            unsigned tot;
+            const unsigned ex = block_exclusive_scan(cnt, scratch, &tot);
+            if (g < t1) {
+                b.tile_base[g] = ex;
+                ledger_fold(L, b.tile_led[g]);
+            }
+            if (threadIdx.x == 0) b.chunk_base[ch] = carry;
+            carry += tot;
+        }
+        const TileLedger T = block_ledger(L, sh);  // (its barriers publish the offsets too)
+        finish_map_scan(b, P, m, carry, T, cstart);
     }
 }
 
@@ -3708,6 +3746,9 @@ unsigned plan_rotation(tpgp_hip_batch *b, const tpgp_hip_rotation *req, uint32_t
     return rotation_grid(b, req);
 }
 
+// Rotations whose maps average at most this many tiles scan in one launch (k_rot_scan_maps).
+constexpr uint64_t kSmallScanTiles = 4 * kChunkTiles;
+
 void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool validated) {
     hipStream_t st = b->ctx->stream;
     // this rotation's tile estimate, from its own grid (rgrid = tile_grid(tiles)); planning may
@@ -3752,8 +3793,12 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
             ProfileScope psn(b, TPGP_HIP_PROFILE_SCAN);
             const uint64_t want = est_tiles / kChunkTiles + b->M + 1;
             const unsigned cgrid = (unsigned)(want < (1u << 20) ? want : (1u << 20));
-            hipLaunchKernelGGL(k_rot_scan1, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d);
-            hipLaunchKernelGGL(k_rot_scan2, dim3(b->M), dim3(kBlock), 0, st, b->dev, d);
+            if (est_tiles <= (uint64_t)kSmallScanTiles * b->M) {
+                hipLaunchKernelGGL(k_rot_scan_maps, dim3(b->M), dim3(kBlock), 0, st, b->dev, d);
+            } else {
+                hipLaunchKernelGGL(k_rot_scan1, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d);
+                hipLaunchKernelGGL(k_rot_scan2, dim3(b->M), dim3(kBlock), 0, st, b->dev, d);
+            }
         }
         {
             ProfileScope pm(b, TPGP_HIP_PROFILE_MERGE);

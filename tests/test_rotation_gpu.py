@@ -20,6 +20,8 @@ pytestmark = pytest.mark.gpu
 def random_map(rng, n, count, max_weight=5, tiny_fraction=0.2):
     """Sorted unique random Pauli strings with low weight (so partners collide) plus identity."""
     words = (n + 31) // 32
+    if words == 1 and count > 100000:
+        return random_map_one_word(rng, n, count, max_weight, tiny_fraction)
     keys = set()
     keys.add((0,) * words)
     while len(keys) < count:
@@ -34,6 +36,23 @@ def random_map(rng, n, count, max_weight=5, tiny_fraction=0.2):
     tiny = rng.random(len(keys)) < tiny_fraction
     vals[tiny] *= 1e-4
     return np.array(keys, np.uint64).reshape(len(keys), words), vals
+
+
+def random_map_one_word(rng, n, count, max_weight, tiny_fraction):
+    """random_map for large one-word maps, drawn in vectorised batches: each candidate keeps
+    each site with probability max_weight / (2n), a random non-identity Pauli on kept sites."""
+    keys = np.zeros(1, np.uint64)
+    while len(keys) < count:
+        live = rng.random((count, n)) < max_weight / (2 * n)
+        ops = rng.integers(1, 4, size=(count, n), dtype=np.uint64) * live
+        cand = np.bitwise_or.reduce(ops << (np.uint64(2) * np.arange(n, dtype=np.uint64)), axis=1)
+        keys = np.unique(np.concatenate([keys, cand]))
+    keys = np.sort(rng.choice(keys[1:], size=count - 1, replace=False))
+    keys = np.concatenate([np.zeros(1, np.uint64), keys])
+    vals = rng.standard_normal(count)
+    tiny = rng.random(count) < tiny_fraction
+    vals[tiny] *= 1e-4
+    return keys.reshape(count, 1), vals
 
 
 def both_maps(oracle, n, keys, vals):
@@ -195,9 +214,10 @@ def test_evolve_ring12(hip_ctx, oracle, exact):
             assert b >= a * (1 - 1e-12) and abs(b - a) <= 1e-9 * max(a, 1e-300)
 
 
-@pytest.mark.parametrize("n,count", [(16, 60000), (40, 30000)])
+@pytest.mark.parametrize("n,count", [(16, 60000), (40, 30000), (16, 600000)])
 def test_rotate_large_multi_tile(hip_ctx, oracle, n, count):
-    """Classes spanning many merge tiles and partition tiles (regression: tile splits)."""
+    """Classes spanning many merge tiles and partition tiles (regression: tile splits).  At
+    600000 terms a map has several hundred tiles: the one-launch scan walks several chunks."""
     rng = np.random.default_rng(77 + n)
     keys, vals = random_map(rng, n, count, max_weight=7)
     for (i, j, qi, qj) in [(0, 1, 1, 1), (n - 1, 3, 2, 2), (7, n // 2, 3, 3), (2, n - 2, 1, 0)]:
