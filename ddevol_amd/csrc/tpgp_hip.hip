@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -53,8 +54,9 @@ constexpr uint32_t kPassCommutingKept = 16u;
 constexpr int kFuseMax = 3;              // rotations fused into one pass (one bond: XX, YY, ZZ)
 constexpr int kSampleStride = 64;        // fused passes: one sample per 64 elements of a class
 constexpr int kSamplesPerTile = 12;      // fused tiles hold < (12 + 4) * 64 = 1024 elements
-constexpr int kPartRounds = 8;     // partition: 8 rounds of 256 elements
+constexpr int kPartRounds = 8;     // partition: 8 rounds of 256 elements (the default)
 constexpr int kPartTile = kBlock * kPartRounds;
+constexpr int kPartRoundsMin = 4;  // smallest tunable partition tile: 4 rounds
 
 __device__ __forceinline__ uint64_t dbits(double x) { return __builtin_bit_cast(uint64_t, x); }
 __device__ __forceinline__ double dfrom(uint64_t b) { return __builtin_bit_cast(double, b); }
@@ -448,7 +450,8 @@ __device__ __forceinline__ void scatter_tile(const uint64_t *keys, const double 
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap *pm) {
+__global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap *pm,
+                                                      uint32_t tile) {
     __shared__ unsigned scratch[8];
     const uint32_t M = b.n_maps;
     // tiles per map, exclusive scan over maps in chunks of 256
@@ -458,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap 
         unsigned n = 0;
         if (m < M && pm[m].active) {
             uint64_t len = b.len[m * 3 + pm[m].in_buf];
-            n = (unsigned)((len + kPartTile - 1) / kPartTile);
+            n = (unsigned)((len + tile - 1) / tile);
             if (len == 0) {  // no tile will describe the (empty) output: do it here
                 uint64_t *reg = b.regions + ((uint64_t)m * 3 + pm[m].out_buf) * 17;
                 for (int c = 0; c < 17; ++c) reg[c] = 0;
@@ -473,14 +476,15 @@ __global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap 
     if (threadIdx.x == 0) b.tstart[M] = carry;
 }
 
-// Per tile: the histogram of the 16 new classes (digits pa, pb).  Each thread counts its
-// kPartRounds keys (coalesced rounds of 256) in 4-bit counters packed in one register: no LDS
+// Per tile of kBlock * R elements: the histogram of the 16 new classes (digits pa, pb).  Each
+// thread counts its R keys (coalesced rounds of 256) in 4-bit counters packed in one register: no LDS
 // atomics, which the sorted keys would serialize (the new digits come in long runs, so a wave's
 // atomics hit one bin).  The counters are widened to 16-bit halves of 8 dwords, summed across
 // the wave by DPP scans and across the waves in LDS.
-template <int W>
+template <int W, int R>
 __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap *__restrict__ pm) {
-    static_assert(kPartRounds <= 15, "4-bit per-thread counters");
+    static_assert(R <= 15, "4-bit per-thread counters");
+    constexpr uint64_t kTile = (uint64_t)kBlock * R;
     __shared__ unsigned whist[kBlock / 64][8];
     const uint32_t M = b.n_maps;
     const uint64_t ntiles = b.tstart[M];
@@ -488,19 +492,19 @@ __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap
         uint32_t m = find_bucket(b.tstart, M, g);
         const PassMap &P = pm[m];
         uint64_t len = b.len[m * 3 + P.in_buf];
-        uint64_t e0 = (g - b.tstart[m]) * kPartTile;
-        uint64_t e1 = e0 + kPartTile < len ? e0 + kPartTile : len;
+        uint64_t e0 = (g - b.tstart[m]) * kTile;
+        uint64_t e1 = e0 + kTile < len ? e0 + kTile : len;
         const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[m].off * W;
         const uint32_t pa = P.pa, pb = P.pb;  // read once
-        Key<W> k[kPartRounds];
+        Key<W> k[R];
 #pragma unroll
-        for (int r = 0; r < kPartRounds; ++r) {  // branch-free: past the end, reload the last
+        for (int r = 0; r < R; ++r) {  // branch-free: past the end, reload the last
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
             k[r] = kload<W>(keys + (e < e1 ? e : e1 - 1) * W);
         }
         uint64_t cnt = 0;  // 16 bins x 4 bits
 #pragma unroll
-        for (int r = 0; r < kPartRounds; ++r) {
+        for (int r = 0; r < R; ++r) {
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
             const unsigned bin = (kdigit<W>(k[r], pa) << 2) | kdigit<W>(k[r], pb);
             cnt += e < e1 ? 1ull << (4 * bin) : 0ull;
@@ -512,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap
             h[j] = (unsigned)(two & 0xfu) | ((unsigned)((two >> 4) & 0xfu) << 16);
         }
 #pragma unroll
-        for (int j = 0; j < 8; ++j) h[j] = wave_scan_u32(h[j]);  // <= 64 * kPartRounds per half
+        for (int j = 0; j < 8; ++j) h[j] = wave_scan_u32(h[j]);  // <= 64 * R per half
         if (lane_id() == 63) {
 #pragma unroll
             for (int j = 0; j < 8; ++j) whist[wave_id()][j] = h[j];
@@ -565,9 +569,10 @@ __global__ __launch_bounds__(kBlock) void k_part_scan(BatchDev b, const PassMap 
 // register use low for the many-tile launches.  The per-round counts and runs are double
 // buffered, so one barrier per round suffices.  The map's first tile also writes the output
 // regions and length.
-template <int W>
+template <int W, int R>
 __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b,
                                                          const PassMap *__restrict__ pm) {
+    constexpr uint64_t kTile = (uint64_t)kBlock * R;
     __shared__ unsigned wave_cnt[2][kBlock / 64][16];
     __shared__ uint64_t run[2][16];
     __shared__ uint64_t class_total[16];
@@ -580,8 +585,8 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b,
         // read once: the rounds' global stores could alias the descriptor for the compiler
         const uint32_t pa = P.pa, pb = P.pb;
         uint64_t len = b.len[m * 3 + P.in_buf];
-        uint64_t e0 = (g - b.tstart[m]) * kPartTile;
-        uint64_t e1 = e0 + kPartTile < len ? e0 + kPartTile : len;
+        uint64_t e0 = (g - b.tstart[m]) * kTile;
+        uint64_t e1 = e0 + kTile < len ? e0 + kTile : len;
         const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[m].off * W;
         const double *vals = b.arena[P.in_buf].vals + b.slot[m].off;
         uint64_t *okeys = b.arena[P.out_buf].keys + b.slot[m].off * W;
@@ -592,10 +597,10 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b,
             class_total[threadIdx.x] = b.map_hist[m * 16 + threadIdx.x];
             tile_off = b.tile_base[g * 16 + threadIdx.x];
         }
-        Key<W> kk[kPartRounds];
-        double vv[kPartRounds];
+        Key<W> kk[R];
+        double vv[R];
 #pragma unroll
-        for (int r = 0; r < kPartRounds; ++r) {  // branch-free: past the end, reload the last
+        for (int r = 0; r < R; ++r) {  // branch-free: past the end, reload the last
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
             const uint64_t ec = e < e1 ? e : e1 - 1;
             kk[r] = kload<W>(keys + ec * W);
@@ -618,7 +623,7 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b,
         }
         const uint64_t lanemask = (1ull << lane_id()) - 1ull;
 #pragma unroll
-        for (int r = 0; r < kPartRounds; ++r) {
+        for (int r = 0; r < R; ++r) {
             const int cur = r & 1;
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
             const bool live = e < e1;
@@ -2932,6 +2937,51 @@ bool overlap(const void *a, size_t as, const void *b, size_t bs) {
 
 extern "C" int tpgp_hip_compiled(void) { return 1; }
 
+namespace {
+bool tuning_value_ok(unsigned param, unsigned value) {
+    if (value == 0) return true;  // the default
+    if (param == TPGP_HIP_TUNE_PARTITION_ROUNDS) return value == 4 || value == 8 || value == 12;
+    return value <= 24;  // grids, in sixths of the tile estimate
+}
+const unsigned kTuneDefault[TPGP_HIP_TUNE_PARAMS] = {kPartRounds, 3, 4};
+
+// Lines "param bucket value" ('#' starts a comment); false with a message on the first bad line.
+bool load_tuning(tpgp_hip_context *ctx, const char *path, char *error, size_t cap) {
+    FILE *f = fopen(path, "r");
+    if (!f) {
+        msg(error, cap, "TPGP_HIP_TUNING: cannot open %s", path);
+        return false;
+    }
+    char line[256];
+    unsigned lineno = 0;
+    bool ok = true;
+    while (ok && fgets(line, sizeof line, f)) {
+        ++lineno;
+        char *hash = strchr(line, '#');
+        if (hash) *hash = 0;
+        unsigned p, bkt, v;
+        char extra;
+        const int got = sscanf(line, "%u %u %u %c", &p, &bkt, &v, &extra);
+        if (got == EOF || got <= 0) {
+            bool blank = true;
+            for (const char *c = line; *c; ++c)
+                if (!isspace((unsigned char)*c)) blank = false;
+            if (blank) continue;
+        }
+        if (got != 3 || p >= TPGP_HIP_TUNE_PARAMS || bkt >= TPGP_HIP_TUNE_BUCKETS ||
+            !tuning_value_ok(p, v)) {
+            msg(error, cap, "TPGP_HIP_TUNING: %s line %u is not a valid \"param bucket value\"",
+                path, lineno);
+            ok = false;
+        } else {
+            ctx->tune[p][bkt] = (unsigned char)v;
+        }
+    }
+    fclose(f);
+    return ok;
+}
+}  // namespace
+
 extern "C" tpgp_hip_context *tpgp_hip_create(int index, char *error, size_t cap) {
     msg(error, cap, "");
     int count = 0, previous = 0;
@@ -2984,6 +3034,11 @@ extern "C" tpgp_hip_context *tpgp_hip_create(int index, char *error, size_t cap)
         delete ctx;
         return nullptr;
     }
+    const char *tuning = getenv("TPGP_HIP_TUNING");
+    if (tuning && *tuning && !load_tuning(ctx, tuning, error, cap)) {
+        tpgp_hip_destroy(ctx);
+        return nullptr;
+    }
     return ctx;
 }
 
@@ -3000,6 +3055,26 @@ extern "C" void tpgp_hip_destroy(tpgp_hip_context *ctx) {
 
 extern "C" const char *tpgp_hip_device_name(const tpgp_hip_context *ctx) {
     return ctx ? ctx->name : "unavailable";
+}
+
+extern "C" int tpgp_hip_set_tuning(tpgp_hip_context *ctx, unsigned param, unsigned bucket,
+                                   unsigned value) {
+    if (!ctx || param >= TPGP_HIP_TUNE_PARAMS || bucket >= TPGP_HIP_TUNE_BUCKETS ||
+        !tuning_value_ok(param, value))
+        return TPGP_HIP_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    ctx->tune[param][bucket] = (unsigned char)value;
+    return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_get_tuning(tpgp_hip_context *ctx, unsigned param, unsigned bucket,
+                                   unsigned *value) {
+    if (!ctx || !value || param >= TPGP_HIP_TUNE_PARAMS || bucket >= TPGP_HIP_TUNE_BUCKETS)
+        return TPGP_HIP_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    const unsigned v = ctx->tune[param][bucket];
+    *value = v ? v : kTuneDefault[param];
+    return TPGP_HIP_OK;
 }
 
 extern "C" tpgp_hip_status tpgp_hip_branches(
@@ -3211,7 +3286,7 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     hipError_t e;
     const uint32_t M = b->M;
     const uint64_t T = b->total_cap;
-    b->max_tiles = (2 * T) / kMergeTile + 16ull * M + (T / kPartTile) + M + 64;
+    b->max_tiles = (2 * T) / kMergeTile + 16ull * M + (T / (kBlock * kPartRoundsMin)) + M + 64;
     BatchDev &d = b->dev;
     d = BatchDev{};
     d.n_maps = M;
@@ -3399,10 +3474,23 @@ unsigned tile_grid(uint64_t tiles) {
     const uint64_t g = tiles + tiles / 16 + 64;
     return (unsigned)(g < (1u << 22) ? g : (1u << 22));
 }
-unsigned partition_grid(const tpgp_hip_batch *b, const std::vector<int> &want) {
+// The tuned value of a launch-shape parameter for a pass over `terms` terms (0: the default).
+unsigned tuned(const tpgp_hip_context *ctx, unsigned param, uint64_t terms) {
+    unsigned bucket = terms ? 63u - (unsigned)__builtin_clzll(terms) : 0u;
+    if (bucket >= TPGP_HIP_TUNE_BUCKETS) bucket = TPGP_HIP_TUNE_BUCKETS - 1;
+    return ctx->tune[param][bucket];
+}
+// Grid and rounds per thread of a partition pass over the maps with want[m].
+unsigned partition_grid(const tpgp_hip_batch *b, const std::vector<int> &want, int *rounds) {
+    uint64_t terms = 0;
+    for (uint32_t m = 0; m < b->M; ++m)
+        if (want[m] && !b->small[m]) terms += b->maps[m].est_len;
+    const unsigned r = tuned(b->ctx, TPGP_HIP_TUNE_PARTITION_ROUNDS, terms);
+    *rounds = r ? (int)r : kPartRounds;
+    const uint64_t tile = (uint64_t)kBlock * *rounds;
     uint64_t t = 0;
     for (uint32_t m = 0; m < b->M; ++m)
-        if (want[m] && !b->small[m]) t += b->maps[m].est_len / kPartTile + 1;
+        if (want[m] && !b->small[m]) t += b->maps[m].est_len / tile + 1;
     return tile_grid(t);
 }
 unsigned rotation_grid(tpgp_hip_batch *b, const tpgp_hip_rotation *req) {
@@ -3462,7 +3550,7 @@ void collect_profile(tpgp_hip_batch *b) {
 // host-side state they imply: the working buffer and the digit layout of each map.
 unsigned plan_partition(tpgp_hip_batch *b, const std::vector<int> &want,
                         const std::vector<uint32_t> &pa, const std::vector<uint32_t> &pb,
-                        PassMap *h) {
+                        PassMap *h, int *rounds) {
     for (uint32_t m = 0; m < b->M; ++m) {
         if (!want[m]) continue;
         HostMap &hm = b->maps[m];
@@ -3483,27 +3571,36 @@ unsigned plan_partition(tpgp_hip_batch *b, const std::vector<int> &want,
         rest.push_back(sa);
         hm.sig = rest;
     }
-    return partition_grid(b, want);
+    return partition_grid(b, want, rounds);
 }
 
-void launch_partition(tpgp_hip_batch *b, const PassMap *d, unsigned pgrid) {
+template <int W, int R>
+void launch_partition_tiles(tpgp_hip_batch *b, const PassMap *d, unsigned pgrid) {
     hipStream_t st = b->ctx->stream;
-    ProfileScope whole(b, TPGP_HIP_PROFILE_PARTITION);
-    hipLaunchKernelGGL(k_part_plan, dim3(1), dim3(kBlock), 0, st, b->dev, d);
+    hipLaunchKernelGGL(k_part_plan, dim3(1), dim3(kBlock), 0, st, b->dev, d,
+                       (uint32_t)(kBlock * R));
     {
         ProfileScope ph(b, TPGP_HIP_PROFILE_HISTOGRAM);
-        if (b->W == 1)
-            hipLaunchKernelGGL(k_part_count<1>, dim3(pgrid), dim3(kBlock), 0, st, b->dev, d);
-        else
-            hipLaunchKernelGGL(k_part_count<2>, dim3(pgrid), dim3(kBlock), 0, st, b->dev, d);
+        hipLaunchKernelGGL((k_part_count<W, R>), dim3(pgrid), dim3(kBlock), 0, st, b->dev, d);
     }
     hipLaunchKernelGGL(k_part_scan, dim3(b->M * 16), dim3(kBlock), 0, st, b->dev, d);
     {
         ProfileScope ps(b, TPGP_HIP_PROFILE_SCATTER);
-        if (b->W == 1)
-            hipLaunchKernelGGL(k_part_scatter<1>, dim3(pgrid), dim3(kBlock), 0, st, b->dev, d);
-        else
-            hipLaunchKernelGGL(k_part_scatter<2>, dim3(pgrid), dim3(kBlock), 0, st, b->dev, d);
+        hipLaunchKernelGGL((k_part_scatter<W, R>), dim3(pgrid), dim3(kBlock), 0, st, b->dev, d);
+    }
+}
+
+void launch_partition(tpgp_hip_batch *b, const PassMap *d, unsigned pgrid, int rounds) {
+    hipStream_t st = b->ctx->stream;
+    ProfileScope whole(b, TPGP_HIP_PROFILE_PARTITION);
+    if (b->W == 1) {
+        if (rounds == 4) launch_partition_tiles<1, 4>(b, d, pgrid);
+        else if (rounds == 12) launch_partition_tiles<1, 12>(b, d, pgrid);
+        else launch_partition_tiles<1, kPartRounds>(b, d, pgrid);
+    } else {
+        if (rounds == 4) launch_partition_tiles<2, 4>(b, d, pgrid);
+        else if (rounds == 12) launch_partition_tiles<2, 12>(b, d, pgrid);
+        else launch_partition_tiles<2, kPartRounds>(b, d, pgrid);
     }
     if (b->validate) {
         uint32_t tag = (++b->pass_count << 1) | 0u;
@@ -3518,10 +3615,11 @@ hipError_t enqueue_partition(tpgp_hip_batch *b, const std::vector<int> &want,
                              const std::vector<uint32_t> &pa, const std::vector<uint32_t> &pb) {
     PassMap *d;
     PassMap *h = stage(b, b->M, &d);
-    unsigned pgrid = plan_partition(b, want, pa, pb, h);
+    int rounds = kPartRounds;
+    unsigned pgrid = plan_partition(b, want, pa, pb, h, &rounds);
     hipError_t e = push_stage(b, h, d, b->M);
     if (e != hipSuccess) return e;
-    launch_partition(b, d, pgrid);
+    launch_partition(b, d, pgrid, rounds);
     return hipGetLastError();
 }
 
@@ -3775,7 +3873,9 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
             ProfileScope pc(b, TPGP_HIP_PROFILE_COUNT);
             // the count pass visits one tile per anticommuting pair and the commuting classes
             // truncation may change: about a third of all tiles (more loop in the grid)
-            const unsigned cgrid = rgrid / 2 + 64;  // rgrid: about one workgroup per tile
+            const unsigned tc = tuned(b->ctx, TPGP_HIP_TUNE_COUNT_GRID, est_tiles * kMergeTile);
+            const unsigned cgrid = tc ? (unsigned)((uint64_t)rgrid * tc / 6) + 64
+                                      : rgrid / 2 + 64;  // rgrid: about one workgroup per tile
             if (b->W == 1 && validated)
                 hipLaunchKernelGGL((k_rot_count<1, true>), dim3(cgrid), dim3(kBlock), 0, st,
                                    b->dev, d);
@@ -3803,7 +3903,9 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
         {
             ProfileScope pm(b, TPGP_HIP_PROFILE_MERGE);
             // every tile but the higher class's of each anticommuting pair
-            const unsigned wgrid = rgrid / 3 * 2 + 64;
+            const unsigned tw = tuned(b->ctx, TPGP_HIP_TUNE_WRITE_GRID, est_tiles * kMergeTile);
+            const unsigned wgrid = tw ? (unsigned)((uint64_t)rgrid * tw / 6) + 64
+                                      : rgrid / 3 * 2 + 64;
             if (b->W == 1)
                 hipLaunchKernelGGL(k_rot_write<1>, dim3(wgrid), dim3(kBlock), 0, st, b->dev, d);
             else
@@ -4027,6 +4129,7 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
         size_t at;
         unsigned grid;
         bool validated;  // rotation: some map computes radii
+        int rounds;      // partition: elements per thread (tile = kBlock * rounds)
     };
     auto any_validated = [M](const tpgp_hip_rotation *rq) {
         for (uint32_t m = 0; m < M; ++m)
@@ -4065,7 +4168,9 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
             pb[m] = (uint32_t)position_of(hm, q[m].j);
         }
         if (any) {
-            launches.push_back({1, used, plan_partition(b, want, pa, pb, h0 + used), false});
+            int rounds = kPartRounds;
+            const unsigned g = plan_partition(b, want, pa, pb, h0 + used, &rounds);
+            launches.push_back({1, used, g, false, rounds});
             used += M;
         }
         const size_t R = fuse ? run_length(b, req, r, r1) : 1;
@@ -4074,7 +4179,7 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
             const size_t at = fused.size();
             fused.resize(at + M);
             if (plan_fused(b, req, rotations, r, R, fused.data() + at))
-                launches.push_back({2, at, 0, false});
+                launches.push_back({2, at, 0, false, 0});
             else
                 fused.resize(at);
             for (size_t k = 0; k < R; ++k) {
@@ -4083,12 +4188,12 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
                     if (!b->small[m]) qs[m].active = 0;
                 launches.push_back({0, used,
                                     plan_rotation(b, qs.data(), rotations[r + k], h0 + used),
-                                    any_validated(qs.data())});
+                                    any_validated(qs.data()), 0});
                 used += M;
             }
         } else {
             launches.push_back({0, used, plan_rotation(b, q, rotations[r], h0 + used),
-                                any_validated(q)});
+                                any_validated(q), 0});
             used += M;
         }
         r += R;
@@ -4128,7 +4233,7 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
         if (L.kind == 0)
             launch_rotation(b, d0 + L.at, L.grid, L.validated);
         else
-            launch_partition(b, d0 + L.at, L.grid);
+            launch_partition(b, d0 + L.at, L.grid, L.rounds);
     }
     if (any_small) {
         // the map list goes with this chunk's descriptors: a ring region is only guaranteed

@@ -20,6 +20,14 @@ HIP_ECAPACITY = 16
 OK, EINVAL, ENOMEM, ECAP, ENUMERIC, EBACKEND, EBUDGET = range(7)
 TPGP_HIP = 4
 
+# launch-shape tuning (tpgp_hip_set_tuning): parameters, size buckets (floor(log2 terms))
+TUNE_PARTITION_ROUNDS, TUNE_COUNT_GRID, TUNE_WRITE_GRID = range(3)
+TUNE_PARAMS, TUNE_BUCKETS = 3, 48
+TUNE_NAMES = ["partition_rounds", "count_grid", "write_grid"]
+# the table the autotuner measured on MI355X (ddevol_amd/autotune.py); contexts load it through
+# TPGP_HIP_TUNING unless the caller names another file (or an empty name: built-in defaults)
+TUNING_TABLE = os.path.join(HERE, "tuning_gfx950.txt")
+
 _hip = None
 _eng = None
 
@@ -128,7 +136,8 @@ HIP_SYMBOLS = ["tpgp_hip_compiled", "tpgp_hip_create", "tpgp_hip_destroy",
                "tpgp_hip_batch_info", "tpgp_hip_batch_stream", "tpgp_hip_batch_profile",
                "tpgp_hip_batch_profile_read", "tpgp_hip_batch_assign",
                "tpgp_hip_batch_reserve_log", "tpgp_hip_batch_reserve_many",
-               "tpgp_hip_batch_sum_squares", "tpgp_hip_liouvillian"]
+               "tpgp_hip_batch_sum_squares", "tpgp_hip_liouvillian", "tpgp_hip_set_tuning",
+               "tpgp_hip_get_tuning"]
 ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_default_options", "tpgp_ledger_init", "tpgp_map_init", "tpgp_map_free",
                   "tpgp_map_clone", "tpgp_key_get", "tpgp_key_set", "tpgp_map_add",
@@ -153,6 +162,8 @@ def hip():
     global _hip
     if _hip is not None:
         return _hip
+    if "TPGP_HIP_TUNING" not in os.environ and os.path.exists(TUNING_TABLE):
+        os.environ["TPGP_HIP_TUNING"] = TUNING_TABLE  # read by every tpgp_hip_create
     L = _load(HIP_SO)
     p_u64, p_d, p_c = C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.c_char_p
     sz = C.c_size_t
@@ -194,6 +205,8 @@ def hip():
     f("tpgp_hip_batch_profile", C.c_int, C.c_void_p, C.c_uint)
     f("tpgp_hip_batch_profile_read", C.c_int, C.c_void_p, C.c_uint, C.POINTER(C.c_double),
       C.POINTER(C.c_uint64))
+    f("tpgp_hip_set_tuning", C.c_int, C.c_void_p, C.c_uint, C.c_uint, C.c_uint)
+    f("tpgp_hip_get_tuning", C.c_int, C.c_void_p, C.c_uint, C.c_uint, C.POINTER(C.c_uint))
     _hip = L
     return L
 
