@@ -22,9 +22,31 @@ DEFAULTS = {N.TUNE_PARTITION_ROUNDS: 8, N.TUNE_COUNT_GRID: 3, N.TUNE_WRITE_GRID:
 
 @pytest.fixture(scope="module")
 def tuned_ctx():
-    ctx = E.Context(0)
+    """A context on the built-in defaults (the committed table left out)."""
+    old = os.environ.get("TPGP_HIP_TUNING")
+    os.environ["TPGP_HIP_TUNING"] = ""
+    try:
+        ctx = E.Context(0)
+    finally:
+        if old is None:
+            os.environ.pop("TPGP_HIP_TUNING", None)
+        else:
+            os.environ["TPGP_HIP_TUNING"] = old
     yield ctx
     ctx.close()
+
+
+def test_committed_table_loads():
+    """A default context carries the committed table's choices (bindings set TPGP_HIP_TUNING)."""
+    ctx = E.Context(0)
+    try:
+        with open(N.TUNING_TABLE) as f:
+            rows = [tuple(int(x) for x in ln.split("#")[0].split())
+                    for ln in f if ln.split("#")[0].strip()]
+        for p, b, v in rows:
+            assert ctx.get_tuning(p, b) == v
+    finally:
+        ctx.close()
 
 
 def set_all(ctx, param, value):
