@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public specification
 # from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings (factors 2.00 and
 # 1.00): profiles/k_rot_write_j1j2_n16_window_pmc.md.  Used for roofline.traffic when the run
 # is that window and no --traffic is given; other windows report null.
-MEASURED_TRAFFIC = {("k_rot_write", 600, 40): 555.6e6}
+MEASURED_TRAFFIC = {("k_rot_write", 600, 40): 555.8e6}
 
 
 def log(*a):
@@ -238,6 +238,7 @@ def fid_allowance(E, terms, dt, led, n, steps):
 def bench_single(args, n, terms, dt, thr, label, workload):
     """One large FID: untimed warmup steps, then timed steps on the resident map."""
     from ddevol_amd import engine as E
+    from ddevol_amd import native as N
 
     ctx = E.Context(0)
     mp = E.Map(n)
@@ -245,32 +246,72 @@ def bench_single(args, n, terms, dt, thr, label, workload):
         mp.add_single(s, 1, 1.0)
     opts = E.options(threshold=thr, max_terms=10 ** 11, context=ctx)
     led = E.ledger(0)
-    ev = E.Evolution(mp, terms, dt, opts, context=ctx)
     done = 0
-    while done < args.warmup:  # untimed, in chunks with a progress line each
-        chunk = min(25, args.warmup - done)
+    if args.load_state:  # the map after an earlier run's warmup (--save-state)
+        import numpy as np
+
+        st = np.load(args.load_state)
+        mp = E.Map.from_sorted(n, st["keys"], st["values"])
+        done = args.warmup = int(st["warmup"])
+        log(f"{label}: loaded state after {done} steps: {len(mp)} terms")
+    ev = E.Evolution(mp, terms, dt, opts, context=ctx,
+                     initial_capacity=4 * len(mp) + (1 << 20) if args.load_state else 0)
+    size = len(mp)
+    tw = time.perf_counter()
+    # untimed, in chunks with a progress line each; --until-terms: until the map reaches that
+    # many terms (--warmup then bounds the steps)
+    while done < args.warmup and not (args.until_terms and size >= args.until_terms):
+        chunk = min(5 if args.until_terms else 25, args.warmup - done)
         st, fid, rad, act, acc = ev.run(chunk, led)
         assert st == 0, N_last_error()
         done += chunk
-        log(f"{label}: warmup step {done}: {int(act[-1])} terms")
+        size = int(act[-1])
+        log(f"{label}: warmup step {done}: {size} terms")
+    warm_s = time.perf_counter() - tw
+    if args.save_state:
+        import numpy as np
+
+        assert ev.close(mp) == 0
+        keys, vals = mp.arrays()
+        np.savez(args.save_state, keys=keys, values=vals, warmup=done)
+        log(f"{label}: saved {len(vals)} terms after {done} steps to {args.save_state}")
+        return {"saved": args.save_state, "terms": len(vals), "warmup": done}
+    ev.run(0, led)
+    kinds = [N.PROFILE_MERGE, N.PROFILE_COUNT]
+    ev.profile(sum(1 << k for k in kinds))
     c0 = ev.counters()
     t0 = time.perf_counter()
     st, fid, rad, act, acc = ev.run(args.steps, led)
     el = time.perf_counter() - t0
     assert st == 0, N_last_error()
     c1 = ev.counters()
+    prof = {N.PROFILE_NAMES[k]: ev.profile_read(k) for k in kinds}
     ev.close()
     upd = c1.updates - c0.updates
-    log(f"{label}: steps {args.warmup + 1}..{args.warmup + args.steps}, {el:.2f} s, "
+    in_out = c1.in_out - c0.in_out
+    log(f"{label}: steps {done + 1}..{done + args.steps}, {el:.2f} s, "
         f"terms {int(act[0])} -> {int(act[-1])}")
+    term_bytes = 8 * ((n + 31) // 32) + 8
+    ms_w, n_w = prof["k_rot_write"]
+    roof = None
+    if n_w and ms_w > 0:
+        alg = term_bytes * in_out / n_w  # every input term read once, every output written once
+        avg_s = ms_w * 1e-3 / n_w
+        roof = {"bound": "hbm", "achieved": alg / avg_s / 1e9, "peak": 8000.0, "unit": "GB/s",
+                "frac": alg / avg_s / 8e12, "traffic": None, "kernel": "k_rot_write",
+                "launches": n_w, "avg_us": avg_s * 1e6, "algorithmic_bytes_per_launch": alg,
+                "kernels": {k: {"launches": v[1], "avg_us": v[0] * 1e3 / max(v[1], 1)}
+                            for k, v in prof.items()}}
     return {
         "metric": f"FP64 Pauli branch-updates/s ({label})", "value": upd / el,
-        "unit": "branch-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "unit": "branch-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": done,
         "ms_per_step": el / max(args.steps, 1) * 1e3, "higher_is_better": True,
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (initial state sum_s X_s)",
         "config": {"workload": workload, "terms_at_start": int(act[0]),
                    "terms_at_end": int(act[-1]), "fid": [float(x) for x in fid[[0, -1]]],
-                   "ledger_dropped_l2": led.dropped_l2},
+                   "ledger_dropped_l2": led.dropped_l2, "warmup_seconds": warm_s,
+                   "whole_run_seconds": warm_s + el},
+        "roofline": roof,
     }
 
 
@@ -297,7 +338,11 @@ def main():
     ap.add_argument("--steps", type=int, default=None,
                     help="timed Strang steps (default 40; 20 for --workload large)")
     ap.add_argument("--warmup", type=int, default=None,
-                    help="untimed steps before them (default 600; 100 for --workload large)")
+                    help="untimed steps before them (default 600; 480 for --workload large: "
+                         "the timed steps are the last 20 of its 500-step FID)")
+    ap.add_argument("--until-terms", type=int, default=0,
+                    help="large/plain: warm up until the map holds this many terms (--warmup "
+                         "then bounds the warmup steps)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--breakdown", action="store_true",
@@ -313,7 +358,7 @@ def main():
     if args.steps is None:
         args.steps = 20 if large else 40
     if args.warmup is None:
-        args.warmup = 100 if large else 600
+        args.warmup = 480 if large else 600
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one FID per GPU: only --gpus 1 is supported; "
                                    "multi-GPU sharding is a design, not part of this build"}))
