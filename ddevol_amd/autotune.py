@@ -1,10 +1,12 @@
 """Launch-shape autotuner: picks the partition's elements per thread (its tile size) and the
-grids of the rotation's count and write passes per size bucket, by timing the candidates on
-the device, and writes the table that contexts load (TPGP_HIP_TUNING, include/tpgp_hip.h).
+grids of the rotation's count and write passes per key width and size bucket, by timing the
+candidates on the device, and writes the table that contexts load (TPGP_HIP_TUNING,
+include/tpgp_hip.h).
 
 Every candidate gives the same keys, values and ledgers (the parameters only change how a pass
-is cut into workgroups), so the tuner compares time alone.  The workload is the N=16 J1-J2 FID
-trajectory (dt 0.005, threshold 1e-6, the Z-sum initial state): the map is snapshotted as it
+is cut into workgroups), so the tuner compares time alone.  The workloads are FID trajectories
+from the X-sum initial state: one-word keys on the N=16 J1-J2 ring (dt 0.005, threshold 1e-6),
+two-word keys on the N=64 range-4 ring (dt 0.01, threshold 1e-5).  Each map is snapshotted as it
 first reaches 2**k terms, and from each snapshot every candidate runs the same few Strang steps
 on a fresh session, timed with the HIP events of the pass the parameter governs (partition
 pass, count pass, write pass).  One parameter at a time, the others at the values already
@@ -19,16 +21,14 @@ import os
 import sys
 import time
 
-import numpy as np
-
 from . import engine as E
 from . import hamiltonians as H
 from . import native as N
 
 CANDIDATES = {
     N.TUNE_PARTITION_ROUNDS: [8, 4, 12],  # the default first
-    N.TUNE_COUNT_GRID: [3, 2, 4, 6],
-    N.TUNE_WRITE_GRID: [4, 3, 6, 8],
+    N.TUNE_COUNT_GRID: [3, 2, 6],
+    N.TUNE_WRITE_GRID: [4, 3, 8],
 }
 KIND = {
     N.TUNE_PARTITION_ROUNDS: N.PROFILE_PARTITION,
@@ -43,14 +43,16 @@ def log(msg):
     print(f"[autotune] {msg}", file=sys.stderr, flush=True)
 
 
-def j1j2_workload():
-    n = 16
-    return n, H.j1j2_n16_terms(n), 0.005, 1e-6
+def workload(words):
+    """(spins, terms, dt, threshold) of the trajectory that tunes keys of `words` words."""
+    if words == 1:
+        return 16, H.j1j2_n16_terms(16), 0.005, 1e-6
+    return 64, H.range_ring(64, 4), 0.01, 1e-5
 
 
-def snapshots(ctx, buckets, max_steps=2000):
-    """Maps of the N=16 J1-J2 trajectory as it first reaches 2**k terms, k in buckets."""
-    n, terms, dt, thr = j1j2_workload()
+def snapshots(ctx, words, buckets, max_steps=2000):
+    """Maps of the trajectory as it first reaches 2**k terms, k in buckets."""
+    n, terms, dt, thr = workload(words)
     mp = E.Map(n)
     for s in range(n):
         mp.add_single(s, 1, 1.0)
@@ -76,9 +78,9 @@ def snapshots(ctx, buckets, max_steps=2000):
     return out
 
 
-def time_trial(ctx, snap, steps, kind):
+def time_trial(ctx, words, snap, steps, kind):
     """Milliseconds of pass `kind` over `steps` Strang steps from the snapshot."""
-    n, terms, dt, thr = j1j2_workload()
+    n, terms, dt, thr = workload(words)
     keys, vals, _ = snap
     mp = E.Map.from_sorted(n, keys, vals)
     opts = E.options(threshold=thr, max_terms=10 ** 11, context=ctx)
@@ -95,28 +97,36 @@ def time_trial(ctx, snap, steps, kind):
     return ms, launches
 
 
-def set_all(ctx, param, value):
+def set_all(ctx, words, param, value):
     for b in range(N.TUNE_BUCKETS):
-        assert ctx.set_tuning(param, b, value) == 0
+        assert ctx.set_tuning(words, param, b, value) == 0
 
 
-def tune(ctx, buckets=range(14, 25), reps=2):
-    """{bucket: {param: value}} from timings on this context's device."""
+def trial_steps(words, k):
+    """Strang steps per trial: enough launches to time at 2**k terms (one J1-J2 step is 192
+    rotations, one N=64 ring step 1536)."""
+    if words == 1:
+        return int(min(20, max(2, (1 << 21) // (1 << k))))
+    return int(min(4, max(1, (1 << 19) // (1 << k))))
+
+
+def tune(ctx, words, buckets=range(14, 25), reps=2):
+    """{bucket: {param: value}} for keys of `words` words, from timings on this device."""
     t0 = time.perf_counter()
-    snaps = snapshots(ctx, buckets)
+    snaps = snapshots(ctx, words, buckets)
     table = {}
     for k in sorted(snaps):
-        steps = int(min(20, max(2, (1 << 21) // (1 << k))))
+        steps = trial_steps(words, k)
         chosen = dict(DEFAULT)
         for param, cands in CANDIDATES.items():
             for p, v in chosen.items():
-                set_all(ctx, p, v)
+                set_all(ctx, words, p, v)
             times = {}
             for v in cands:
-                set_all(ctx, param, v)
+                set_all(ctx, words, param, v)
                 best = None
                 for _ in range(reps):
-                    ms, launches = time_trial(ctx, snaps[k], steps, KIND[param])
+                    ms, launches = time_trial(ctx, words, snaps[k], steps, KIND[param])
                     best = ms if best is None else min(best, ms)
                 times[v] = best
             base = times[cands[0]]
@@ -124,30 +134,34 @@ def tune(ctx, buckets=range(14, 25), reps=2):
             if base <= 0 or times[win] > base * (1 - TIE):
                 win = cands[0]
             chosen[param] = win
-            log(f"bucket {k} ({len(snaps[k][1])} terms, {steps} steps) {N.TUNE_NAMES[param]}: "
+            log(f"W={words} bucket {k} ({len(snaps[k][1])} terms, {steps} steps) "
+                f"{N.TUNE_NAMES[param]}: "
                 + ", ".join(f"{v}: {t:.2f} ms" for v, t in times.items())
                 + f" -> {win}  [{time.perf_counter() - t0:.0f} s]")
         table[k] = chosen
     for p in CANDIDATES:
-        set_all(ctx, p, 0)
+        set_all(ctx, words, p, 0)
     return table
 
 
-def write_table(path, table, device):
-    """Lines "param bucket value" for the non-default choices; buckets below the smallest tuned
-    one take its values, those above the largest take the largest one's."""
-    ks = sorted(table)
+def write_table(path, tables, device):
+    """Lines "words param bucket value" for the non-default choices ({words: {bucket: {param:
+    value}}}); buckets below a width's smallest tuned one take its values, those above its
+    largest take the largest one's."""
     lines = [f"# launch-shape table measured by ddevol_amd/autotune.py on {device}",
-             "# workload: N=16 J1-J2 FID trajectory (dt 0.005, threshold 1e-6)",
-             "# param bucket value  (params: 0 partition rounds, 1 count grid, 2 write grid;",
+             "# workloads: 1 key word: N=16 J1-J2 FID trajectory (dt 0.005, threshold 1e-6);",
+             "#            2 key words: N=64 range-4 ring FID trajectory (dt 0.01, threshold 1e-5)",
+             "# words param bucket value  (params: 0 partition rounds, 1 count grid, 2 write grid;",
              "#  bucket = floor(log2(terms in the pass)); unlisted = built-in default)"]
-    for b in range(N.TUNE_BUCKETS):
-        src = min(ks, key=lambda k: abs(k - b)) if ks else None
-        if src is None:
+    for words in sorted(tables):
+        ks = sorted(tables[words])
+        if not ks:
             continue
-        for p, v in sorted(table[src].items()):
-            if v != DEFAULT[p]:
-                lines.append(f"{p} {b} {v}")
+        for b in range(N.TUNE_BUCKETS):
+            src = min(ks, key=lambda k: abs(k - b))
+            for p, v in sorted(tables[words][src].items()):
+                if v != DEFAULT[p]:
+                    lines.append(f"{words} {p} {b} {v}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -158,12 +172,14 @@ def main(argv=None):
     ap.add_argument("--min-bucket", type=int, default=14)
     ap.add_argument("--max-bucket", type=int, default=24)
     ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--words", type=int, nargs="+", default=[1, 2])
     args = ap.parse_args(argv)
     os.environ["TPGP_HIP_TUNING"] = ""  # measure from the built-in defaults
     ctx = E.Context(0)
     log(f"device: {ctx.name}")
-    table = tune(ctx, range(args.min_bucket, args.max_bucket + 1), args.reps)
-    write_table(args.out, table, ctx.name)
+    tables = {w: tune(ctx, w, range(args.min_bucket, args.max_bucket + 1), args.reps)
+              for w in args.words}
+    write_table(args.out, tables, ctx.name)
     log(f"wrote {args.out}")
     ctx.close()
 

@@ -2945,7 +2945,8 @@ bool tuning_value_ok(unsigned param, unsigned value) {
 }
 const unsigned kTuneDefault[TPGP_HIP_TUNE_PARAMS] = {kPartRounds, 3, 4};
 
-// Lines "param bucket value" ('#' starts a comment); false with a message on the first bad line.
+// Lines "words param bucket value" ('#' starts a comment); false with a message on the first bad
+// line.
 bool load_tuning(tpgp_hip_context *ctx, const char *path, char *error, size_t cap) {
     FILE *f = fopen(path, "r");
     if (!f) {
@@ -2959,22 +2960,23 @@ bool load_tuning(tpgp_hip_context *ctx, const char *path, char *error, size_t ca
         ++lineno;
         char *hash = strchr(line, '#');
         if (hash) *hash = 0;
-        unsigned p, bkt, v;
+        unsigned words, p, bkt, v;
         char extra;
-        const int got = sscanf(line, "%u %u %u %c", &p, &bkt, &v, &extra);
+        const int got = sscanf(line, "%u %u %u %u %c", &words, &p, &bkt, &v, &extra);
         if (got == EOF || got <= 0) {
             bool blank = true;
             for (const char *c = line; *c; ++c)
                 if (!isspace((unsigned char)*c)) blank = false;
             if (blank) continue;
         }
-        if (got != 3 || p >= TPGP_HIP_TUNE_PARAMS || bkt >= TPGP_HIP_TUNE_BUCKETS ||
-            !tuning_value_ok(p, v)) {
-            msg(error, cap, "TPGP_HIP_TUNING: %s line %u is not a valid \"param bucket value\"",
-                path, lineno);
+        if (got != 4 || words < 1 || words > 2 || p >= TPGP_HIP_TUNE_PARAMS ||
+            bkt >= TPGP_HIP_TUNE_BUCKETS || !tuning_value_ok(p, v)) {
+            msg(error, cap,
+                "TPGP_HIP_TUNING: %s line %u is not a valid \"words param bucket value\"", path,
+                lineno);
             ok = false;
         } else {
-            ctx->tune[p][bkt] = (unsigned char)v;
+            ctx->tune[words - 1][p][bkt] = (unsigned char)v;
         }
     }
     fclose(f);
@@ -3057,22 +3059,23 @@ extern "C" const char *tpgp_hip_device_name(const tpgp_hip_context *ctx) {
     return ctx ? ctx->name : "unavailable";
 }
 
-extern "C" int tpgp_hip_set_tuning(tpgp_hip_context *ctx, unsigned param, unsigned bucket,
-                                   unsigned value) {
-    if (!ctx || param >= TPGP_HIP_TUNE_PARAMS || bucket >= TPGP_HIP_TUNE_BUCKETS ||
-        !tuning_value_ok(param, value))
+extern "C" int tpgp_hip_set_tuning(tpgp_hip_context *ctx, unsigned words, unsigned param,
+                                   unsigned bucket, unsigned value) {
+    if (!ctx || words < 1 || words > 2 || param >= TPGP_HIP_TUNE_PARAMS ||
+        bucket >= TPGP_HIP_TUNE_BUCKETS || !tuning_value_ok(param, value))
         return TPGP_HIP_EINVAL;
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    ctx->tune[param][bucket] = (unsigned char)value;
+    ctx->tune[words - 1][param][bucket] = (unsigned char)value;
     return TPGP_HIP_OK;
 }
 
-extern "C" int tpgp_hip_get_tuning(tpgp_hip_context *ctx, unsigned param, unsigned bucket,
-                                   unsigned *value) {
-    if (!ctx || !value || param >= TPGP_HIP_TUNE_PARAMS || bucket >= TPGP_HIP_TUNE_BUCKETS)
+extern "C" int tpgp_hip_get_tuning(tpgp_hip_context *ctx, unsigned words, unsigned param,
+                                   unsigned bucket, unsigned *value) {
+    if (!ctx || !value || words < 1 || words > 2 || param >= TPGP_HIP_TUNE_PARAMS ||
+        bucket >= TPGP_HIP_TUNE_BUCKETS)
         return TPGP_HIP_EINVAL;
     std::lock_guard<std::mutex> lock(ctx->mutex);
-    const unsigned v = ctx->tune[param][bucket];
+    const unsigned v = ctx->tune[words - 1][param][bucket];
     *value = v ? v : kTuneDefault[param];
     return TPGP_HIP_OK;
 }
@@ -3474,18 +3477,19 @@ unsigned tile_grid(uint64_t tiles) {
     const uint64_t g = tiles + tiles / 16 + 64;
     return (unsigned)(g < (1u << 22) ? g : (1u << 22));
 }
-// The tuned value of a launch-shape parameter for a pass over `terms` terms (0: the default).
-unsigned tuned(const tpgp_hip_context *ctx, unsigned param, uint64_t terms) {
+// The tuned value of a launch-shape parameter for a pass over `terms` terms of a batch with W
+// key words (0: the default).
+unsigned tuned(const tpgp_hip_batch *b, unsigned param, uint64_t terms) {
     unsigned bucket = terms ? 63u - (unsigned)__builtin_clzll(terms) : 0u;
     if (bucket >= TPGP_HIP_TUNE_BUCKETS) bucket = TPGP_HIP_TUNE_BUCKETS - 1;
-    return ctx->tune[param][bucket];
+    return b->ctx->tune[b->W - 1][param][bucket];
 }
 // Grid and rounds per thread of a partition pass over the maps with want[m].
 unsigned partition_grid(const tpgp_hip_batch *b, const std::vector<int> &want, int *rounds) {
     uint64_t terms = 0;
     for (uint32_t m = 0; m < b->M; ++m)
         if (want[m] && !b->small[m]) terms += b->maps[m].est_len;
-    const unsigned r = tuned(b->ctx, TPGP_HIP_TUNE_PARTITION_ROUNDS, terms);
+    const unsigned r = tuned(b, TPGP_HIP_TUNE_PARTITION_ROUNDS, terms);
     *rounds = r ? (int)r : kPartRounds;
     const uint64_t tile = (uint64_t)kBlock * *rounds;
     uint64_t t = 0;
@@ -3873,7 +3877,7 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
             ProfileScope pc(b, TPGP_HIP_PROFILE_COUNT);
             // the count pass visits one tile per anticommuting pair and the commuting classes
             // truncation may change: about a third of all tiles (more loop in the grid)
-            const unsigned tc = tuned(b->ctx, TPGP_HIP_TUNE_COUNT_GRID, est_tiles * kMergeTile);
+            const unsigned tc = tuned(b, TPGP_HIP_TUNE_COUNT_GRID, est_tiles * kMergeTile);
             const unsigned cgrid = tc ? (unsigned)((uint64_t)rgrid * tc / 6) + 64
                                       : rgrid / 2 + 64;  // rgrid: about one workgroup per tile
             if (b->W == 1 && validated)
@@ -3903,7 +3907,7 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
         {
             ProfileScope pm(b, TPGP_HIP_PROFILE_MERGE);
             // every tile but the higher class's of each anticommuting pair
-            const unsigned tw = tuned(b->ctx, TPGP_HIP_TUNE_WRITE_GRID, est_tiles * kMergeTile);
+            const unsigned tw = tuned(b, TPGP_HIP_TUNE_WRITE_GRID, est_tiles * kMergeTile);
             const unsigned wgrid = tw ? (unsigned)((uint64_t)rgrid * tw / 6) + 64
                                       : rgrid / 3 * 2 + 64;
             if (b->W == 1)

@@ -20,8 +20,9 @@ struct tpgp_hip_context {
     size_t capacity;
     std::mutex mutex;
     char name[256];
-    // launch-shape choices (tpgp_hip_set_tuning): per parameter and size bucket, 0 = default
-    unsigned char tune[TPGP_HIP_TUNE_PARAMS][TPGP_HIP_TUNE_BUCKETS] = {};
+    // launch-shape choices (tpgp_hip_set_tuning): per key width (1 or 2 words), parameter and
+    // size bucket; 0 = the default
+    unsigned char tune[2][TPGP_HIP_TUNE_PARAMS][TPGP_HIP_TUNE_BUCKETS] = {};
 };
 
 namespace {

@@ -29,14 +29,15 @@ class Context:
     def name(self) -> str:
         return self.lib.tpgp_hip_device_name(self.ptr).decode()
 
-    def set_tuning(self, param: int, bucket: int, value: int) -> int:
-        """tpgp_hip_set_tuning: a launch shape for passes over about 2**bucket terms (0 resets)."""
-        return self.lib.tpgp_hip_set_tuning(self.ptr, param, bucket, value)
+    def set_tuning(self, words: int, param: int, bucket: int, value: int) -> int:
+        """tpgp_hip_set_tuning: a launch shape for passes over about 2**bucket terms of keys of
+        `words` words (0 resets)."""
+        return self.lib.tpgp_hip_set_tuning(self.ptr, words, param, bucket, value)
 
-    def get_tuning(self, param: int, bucket: int) -> int:
+    def get_tuning(self, words: int, param: int, bucket: int) -> int:
         """The value in force (the built-in default where none is set)."""
         v = C.c_uint()
-        st = self.lib.tpgp_hip_get_tuning(self.ptr, param, bucket, C.byref(v))
+        st = self.lib.tpgp_hip_get_tuning(self.ptr, words, param, bucket, C.byref(v))
         if st:
             raise ValueError(f"tpgp_hip_get_tuning: status {st}")
         return v.value

@@ -20,7 +20,8 @@ HIP_ECAPACITY = 16
 OK, EINVAL, ENOMEM, ECAP, ENUMERIC, EBACKEND, EBUDGET = range(7)
 TPGP_HIP = 4
 
-# launch-shape tuning (tpgp_hip_set_tuning): parameters, size buckets (floor(log2 terms))
+# launch-shape tuning (tpgp_hip_set_tuning): per key width (1 or 2 words), parameter and size
+# bucket (floor(log2 terms))
 TUNE_PARTITION_ROUNDS, TUNE_COUNT_GRID, TUNE_WRITE_GRID = range(3)
 TUNE_PARAMS, TUNE_BUCKETS = 3, 48
 TUNE_NAMES = ["partition_rounds", "count_grid", "write_grid"]
@@ -205,8 +206,9 @@ def hip():
     f("tpgp_hip_batch_profile", C.c_int, C.c_void_p, C.c_uint)
     f("tpgp_hip_batch_profile_read", C.c_int, C.c_void_p, C.c_uint, C.POINTER(C.c_double),
       C.POINTER(C.c_uint64))
-    f("tpgp_hip_set_tuning", C.c_int, C.c_void_p, C.c_uint, C.c_uint, C.c_uint)
-    f("tpgp_hip_get_tuning", C.c_int, C.c_void_p, C.c_uint, C.c_uint, C.POINTER(C.c_uint))
+    f("tpgp_hip_set_tuning", C.c_int, C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_uint)
+    f("tpgp_hip_get_tuning", C.c_int, C.c_void_p, C.c_uint, C.c_uint, C.c_uint,
+      C.POINTER(C.c_uint))
     _hip = L
     return L
 

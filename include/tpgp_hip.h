@@ -215,17 +215,17 @@ int tpgp_hip_batch_profile_read(const tpgp_hip_batch *batch, unsigned kind,
 
 /* Launch-shape tuning.  No counterpart in the reference (its CPU path has no launch shapes);
  * the project's autotuner (ddevol_amd/autotune.py) writes these.  Each parameter holds one value
- * per size bucket, bucket = floor(log2(terms the pass reads)) clamped to
- * [0, TPGP_HIP_TUNE_BUCKETS); value 0 restores the built-in default.  Every value gives the same
+ * per key width (`words` = 1 or 2 key words) and size bucket, bucket = floor(log2(terms the pass
+ * reads)) clamped to [0, TPGP_HIP_TUNE_BUCKETS); value 0 restores the built-in default.  Every value gives the same
  * keys, values and ledgers: the parameters only change how a pass is cut into workgroups.
  *   PARTITION_ROUNDS: elements per thread of the partition (the change of active pair) in
  *     coalesced rounds of 256: 4, 8 or 12, i.e. tiles of 1024, 2048 or 3072 terms; default 8.
  *   COUNT_GRID, WRITE_GRID: workgroups of the rotation's count and write passes in sixths of
  *     the estimated tile count (plus 64), 1..24; defaults 3 (a half) and 4 (two thirds).
  * tpgp_hip_create loads a table from the file named by the environment variable
- * TPGP_HIP_TUNING when it is set (lines "param bucket value", '#' starts a comment; an invalid
- * line fails the create).  A setting applies to the passes enqueued after it.  EINVAL for a
- * parameter, bucket or value out of range. */
+ * TPGP_HIP_TUNING when it is set (lines "words param bucket value", '#' starts a comment; an
+ * invalid line fails the create).  A setting applies to the passes enqueued after it.  EINVAL
+ * for a key width, parameter, bucket or value out of range. */
 enum {
     TPGP_HIP_TUNE_PARTITION_ROUNDS = 0,
     TPGP_HIP_TUNE_COUNT_GRID = 1,
@@ -233,11 +233,11 @@ enum {
     TPGP_HIP_TUNE_PARAMS = 3,
     TPGP_HIP_TUNE_BUCKETS = 48
 };
-int tpgp_hip_set_tuning(tpgp_hip_context *context, unsigned param, unsigned bucket,
-                        unsigned value);
+int tpgp_hip_set_tuning(tpgp_hip_context *context, unsigned words, unsigned param,
+                        unsigned bucket, unsigned value);
 /* The value in force (the default where none is set). */
-int tpgp_hip_get_tuning(tpgp_hip_context *context, unsigned param, unsigned bucket,
-                        unsigned *value);
+int tpgp_hip_get_tuning(tpgp_hip_context *context, unsigned words, unsigned param,
+                        unsigned bucket, unsigned *value);
 
 #ifdef __cplusplus
 }
