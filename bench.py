@@ -277,7 +277,8 @@ def bench_single(args, n, terms, dt, thr, label, workload):
         log(f"{label}: saved {len(vals)} terms after {done} steps to {args.save_state}")
         return {"saved": args.save_state, "terms": len(vals), "warmup": done}
     ev.run(0, led)
-    kinds = [N.PROFILE_MERGE, N.PROFILE_COUNT]
+    kinds = (list(range(len(N.PROFILE_NAMES))) if args.breakdown
+             else [N.PROFILE_MERGE, N.PROFILE_COUNT])
     ev.profile(sum(1 << k for k in kinds))
     c0 = ev.counters()
     t0 = time.perf_counter()
