@@ -1,0 +1,95 @@
+"""Large maps on the GPU: the N=16 J1-J2 FID and the N=64 range-4 ring grown to millions of
+terms (one- and two-word keys), where the oracle cannot replay the trajectory in test time.
+
+Checked at that size:
+1. canonical order: keys strictly increasing (highest word most significant), every
+   coefficient finite and nonzero;
+2. conservation: each rotation is orthogonal on the coefficient vector (c^2 + s^2 = 1 on every
+   branch pair), so sum(v^2) + dropped_l2_squared equals the initial sum of squares (N, from
+   sum_s X_s) up to rounding, whatever the trajectory;
+3. the device's sum of squares (tpgp_evolution_l2) agrees with the downloaded coefficients';
+4. parity at full size: the first bond's three rotations applied to the large map by the GPU
+   and by the oracle (engine/src/tpgp_rev.c:535-771) give the same keys, values and ledger bit
+   for bit.
+"""
+import math
+import time
+
+import numpy as np
+import pytest
+
+from ddevol_amd import engine as E
+from ddevol_amd import hamiltonians as H
+
+from test_rotation_gpu import assert_same_map
+
+pytestmark = pytest.mark.gpu
+
+# the J1-J2 map is grown to the size at which the headline window starts (1.4e7 terms, step
+# 600); the N=64 ring to 4e6 terms (its full run saturates near 2e7)
+CASES = {
+    "j1j2_n16": (16, lambda: H.j1j2_n16_terms(16), 0.005, 1e-6, 14_000_000),
+    "ring64_range4": (64, lambda: H.range_ring(64, 4), 0.01, 1e-5, 4_000_000),
+}
+
+
+def canonical_order(keys):
+    """Whether the keys are strictly increasing, highest word most significant."""
+    w = keys.shape[1]
+    order = np.lexsort(tuple(keys[:, k] for k in range(w)))
+    if not np.array_equal(order, np.arange(len(keys))):
+        return False
+    diff = np.zeros(len(keys) - 1, bool)
+    for k in range(w):
+        diff |= keys[1:, k] != keys[:-1, k]
+    return bool(diff.all())
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_large_map_properties_and_rotation_parity(hip_ctx, oracle, name):
+    n, make_terms, dt, thr, target = CASES[name]
+    terms = make_terms()
+    mp = E.Map(n)
+    for s in range(n):
+        mp.add_single(s, 1, 1.0)
+    opts = E.options(threshold=thr, max_terms=10 ** 10, context=hip_ctx)
+    led = E.ledger(0)
+    ev = E.Evolution(mp, terms, dt, opts, context=hip_ctx)
+    t0, steps, size = time.perf_counter(), 0, n
+    while size < target:
+        st, fid, rad, act, acc = ev.run(10, led)
+        assert st == 0
+        steps += 10
+        size = int(act[-1])
+        assert steps <= 700, f"{name}: {size} terms after {steps} steps"
+    print(f"{name}: {size} terms after {steps} steps, {time.perf_counter() - t0:.1f} s",
+          flush=True)
+    dev_l2 = ev.l2()
+    assert ev.close(mp) == 0
+    keys, vals = mp.arrays()
+    keys = keys.reshape(len(vals), -1)
+    assert len(vals) == size
+
+    # 1. canonical order, finite nonzero coefficients
+    assert canonical_order(keys)
+    assert np.isfinite(vals).all() and (vals != 0).all()
+    # 2. conservation of the sum of squares, up to rounding over every rotation so far
+    sq = math.fsum((vals * vals).tolist())
+    assert abs(sq + led.dropped_l2_squared - n) <= 1e-9 * n, (sq, led.dropped_l2_squared)
+    # 3. the device's sum of squares
+    assert abs(dev_l2 - math.sqrt(sq)) <= 1e-12 * math.sqrt(sq), (dev_l2, math.sqrt(sq))
+
+    # 4. the first bond's rotations on the large map, GPU against the oracle, bit for bit
+    for (i, j, qi, qj, h) in terms[:3]:
+        theta = 2.0 * h * dt
+        om = oracle.new_map(n)
+        assert om.assign(keys, vals) == 0
+        ol = oracle.ledger(0)
+        assert om.rotate(i, j, qi, qj, theta, oracle.options(threshold=thr), ol) == 0
+        gm = E.Map.from_sorted(n, keys, vals)
+        gl = E.ledger(0)
+        go = E.options(threshold=thr, context=hip_ctx)
+        assert E.rotate(gm, i, j, qi, qj, theta, go, gl) == 0
+        assert_same_map(om, gm)
+        assert ol.bits() == gl.bits()
+        print(f"{name}: rotation ({i},{j},{qi},{qj}) on {len(vals)} terms bit-exact", flush=True)
