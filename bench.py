@@ -341,6 +341,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=None,
                     help="untimed steps before them (default 600; 480 for --workload large: "
                          "the timed steps are the last 20 of its 500-step FID)")
+    ap.add_argument("--sample-every", type=int, default=4,
+                    help="headline: HIP events on every Nth call of the timed window (kernel "
+                         "durations for the roofline); 1 = every call")
     ap.add_argument("--until-terms", type=int, default=0,
                     help="large/plain: warm up until the map holds this many terms (--warmup "
                          "then bounds the warmup steps)")
@@ -441,27 +444,57 @@ def main():
     ev = E.Evolution(mp, terms, dt, opts, initial_capacity=4 * len(mp) + (1 << 20))
     ev.run(0, led)  # one readout: allocation and first-touch outside the timed region
     timed_kinds = [N.PROFILE_MERGE, N.PROFILE_COUNT]
-    mask = N.PROFILE_ALL if args.breakdown else sum(1 << k for k in timed_kinds)
-    ev.profile(mask)
+    kinds = list(range(len(N.PROFILE_NAMES))) if args.breakdown else timed_kinds
+    mask = sum(1 << k for k in kinds)
+    # The window runs as calls of a few steps (one readout sync each).  HIP events around every
+    # launch cost a few microseconds each, so the kernel durations are sampled: events are on
+    # for every `sample_every`-th call, and the algorithmic bytes of the roofline are counted
+    # over those same calls (engine counters), so achieved = bytes / time covers the same
+    # launches.  --breakdown and --sample-every 1 time every call.
+    call_steps = 5 if args.steps <= 100 else 25
+    sample_every = 1 if args.breakdown else max(1, args.sample_every)
+    n_calls = -(-args.steps // call_steps)
+
+    def sampled_call(c):
+        # mirrored pairs around the window's centre (calls 1 and 6 of 8 at every 4th), so the
+        # sampled launches see the window's mean map size while the map grows
+        if sample_every == 1 or n_calls < 2 * sample_every:  # short windows: every call
+            return True
+        k = c if 2 * c < n_calls else n_calls - 1 - c
+        return k % sample_every == 1
+
+    prof = {N.PROFILE_NAMES[k]: [0.0, 0] for k in kinds}
+    s_updates = s_in_out = s_steps = 0
     c0 = ev.counters()
     t0 = time.perf_counter()
-    done = 0
+    done = call = 0
     while done < args.steps:
-        # one call for a short window; a long one (a whole FID) in chunks of 25 steps, each
-        # ending with a progress line (one readout sync per chunk)
-        chunk = args.steps - done if args.steps <= 100 else min(25, args.steps - done)
+        chunk = min(call_steps, args.steps - done)
+        sampled = sampled_call(call)
+        ev.profile(mask if sampled else 0)  # resets the event totals
+        ca = ev.counters()
         st, fid, rad, act, acc = ev.run(chunk, led)
         if st:
             raise RuntimeError(f"timed run failed at step {args.warmup + done + acc}: status "
                                f"{st}: {N.engine().tpgp_hip_last_error().decode()}")
+        if sampled:
+            cb = ev.counters()
+            for k in kinds:
+                ms, cnt = ev.profile_read(k)
+                prof[N.PROFILE_NAMES[k]][0] += ms
+                prof[N.PROFILE_NAMES[k]][1] += cnt
+            s_updates += cb.updates - ca.updates
+            s_in_out += cb.in_out - ca.in_out
+            s_steps += acc
         done += acc
-        if done < args.steps:
+        call += 1
+        if args.steps > 100 and done < args.steps:
             log(f"timed step {args.warmup + done}: {int(act[-1])} terms, "
                 f"{time.perf_counter() - t0:.1f} s")
     t1 = time.perf_counter()
     c1 = ev.counters()
-    kinds = range(len(N.PROFILE_NAMES)) if args.breakdown else timed_kinds
-    prof = {N.PROFILE_NAMES[k]: ev.profile_read(k) for k in kinds}
+    prof = {k: tuple(v) for k, v in prof.items()}
+    ev.profile(0)
     ev.close()
     elapsed = t1 - t0
     updates = c1.updates - c0.updates
@@ -473,7 +506,7 @@ def main():
     # dominant kernel: the timed kind with the largest total time.  Algorithmic bytes per
     # launch: k_rot_write reads every input term once and writes every output term once;
     # k_rot_count reads every input term once.
-    alg_bytes = {"k_rot_write": term_bytes * in_out, "k_rot_count": term_bytes * updates}
+    alg_bytes = {"k_rot_write": term_bytes * s_in_out, "k_rot_count": term_bytes * s_updates}
     dom = max(alg_bytes, key=lambda k: prof[k][0])
     dom_ms, dom_n = prof[dom]
     dom_bytes = alg_bytes[dom]
@@ -510,7 +543,7 @@ def main():
                                "threshold 1e-6, binary64",
                    "timed_steps": f"{args.warmup + 1}..{args.warmup + args.steps}",
                    "terms_at_start": start_terms, "terms_at_end": int(act[-1]),
-                   "rotations": rotations, "branch_updates": updates,
+                   "rotations": rotations, "branch_updates": updates, "terms_in_out": in_out,
                    "fid_at_end": float(fid[-1]),
                    # a loaded state carries no ledger of its warmup steps
                    "fid_allowance_at_end": None if args.load_state else
@@ -520,6 +553,10 @@ def main():
                      "traffic": args.traffic if args.traffic is not None
                      else MEASURED_TRAFFIC.get((dom, args.warmup, args.steps)),
                      "kernel": dom, "launches": dom_n,
+                     "sample": f"HIP events on {s_steps} of the {args.steps} timed steps (calls "
+                               f"of {call_steps} steps, 1 in {sample_every}, in pairs mirrored "
+                               f"about the window centre); algorithmic bytes over the same "
+                               f"launches",
                      "avg_us": dom_ms / max(dom_n, 1) * 1e3,
                      "algorithmic_bytes_per_launch": dom_bytes / max(dom_n, 1),
                      "kernels": {k: {"launches": n_, "avg_us": ms / max(n_, 1) * 1e3}
