@@ -235,6 +235,69 @@ def fid_allowance(E, terms, dt, led, n, steps):
     return E.fid_allowance(pf, led, n)
 
 
+def timed_window(ev, led, steps, kinds, sample_every, progress=None):
+    """`steps` Strang steps on the resident map, timed end to end, as calls of a few steps (one
+    readout sync each).  HIP events around every launch cost a few microseconds each, so the
+    kernel durations are sampled: events are on for one call in `sample_every`, chosen in pairs
+    mirrored about the window's centre so the sample sees the window's mean map size while the
+    map grows (short windows time every call), and the algorithmic counts of the roofline are
+    taken over the same calls (engine counters)."""
+    from ddevol_amd import native as N
+
+    call_steps = 5 if steps <= 100 else 25
+    n_calls = -(-steps // call_steps)
+
+    def sampled_call(c):
+        if sample_every <= 1 or n_calls < 2 * sample_every:
+            return True
+        k = c if 2 * c < n_calls else n_calls - 1 - c
+        return k % sample_every == 1
+
+    mask = sum(1 << k for k in kinds)
+    prof = {N.PROFILE_NAMES[k]: [0.0, 0] for k in kinds}
+    w = {"s_updates": 0, "s_in_out": 0, "s_steps": 0, "call_steps": call_steps,
+         "sample_every": sample_every, "steps": steps}
+    c0 = ev.counters()
+    t0 = time.perf_counter()
+    done = call = 0
+    while done < steps:
+        chunk = min(call_steps, steps - done)
+        sampled = sampled_call(call)
+        ev.profile(mask if sampled else 0)  # resets the event totals
+        ca = ev.counters()
+        st, fid, rad, act, acc = ev.run(chunk, led)
+        if st:
+            raise RuntimeError(f"timed run failed {done + acc} steps into the window: status "
+                               f"{st}: {N.engine().tpgp_hip_last_error().decode()}")
+        if sampled:
+            cb = ev.counters()
+            for k in kinds:
+                ms, cnt = ev.profile_read(k)
+                prof[N.PROFILE_NAMES[k]][0] += ms
+                prof[N.PROFILE_NAMES[k]][1] += cnt
+            w["s_updates"] += cb.updates - ca.updates
+            w["s_in_out"] += cb.in_out - ca.in_out
+            w["s_steps"] += acc
+        if call == 0:
+            w["terms_first"], w["fid_first"] = int(act[0]), float(fid[0])
+        done += acc
+        call += 1
+        if progress and done < steps:
+            progress(done, int(act[-1]), time.perf_counter() - t0)
+    w["elapsed"] = time.perf_counter() - t0
+    c1 = ev.counters()
+    ev.profile(0)
+    w["prof"] = {k: tuple(v) for k, v in prof.items()}
+    w["updates"] = c1.updates - c0.updates
+    w["in_out"] = c1.in_out - c0.in_out
+    w["rotations"] = c1.rotations - c0.rotations
+    w["terms_last"], w["fid_last"] = int(act[-1]), float(fid[-1])
+    w["sample"] = (f"HIP events on {w['s_steps']} of the {steps} timed steps (calls of "
+                   f"{call_steps} steps, 1 in {sample_every}, in pairs mirrored about the window "
+                   f"centre); algorithmic bytes over the same launches")
+    return w
+
+
 def bench_single(args, n, terms, dt, thr, label, workload):
     """One large FID: untimed warmup steps, then timed steps on the resident map."""
     from ddevol_amd import engine as E
@@ -279,37 +342,31 @@ def bench_single(args, n, terms, dt, thr, label, workload):
     ev.run(0, led)
     kinds = (list(range(len(N.PROFILE_NAMES))) if args.breakdown
              else [N.PROFILE_MERGE, N.PROFILE_COUNT])
-    ev.profile(sum(1 << k for k in kinds))
-    c0 = ev.counters()
-    t0 = time.perf_counter()
-    st, fid, rad, act, acc = ev.run(args.steps, led)
-    el = time.perf_counter() - t0
-    assert st == 0, N_last_error()
-    c1 = ev.counters()
-    prof = {N.PROFILE_NAMES[k]: ev.profile_read(k) for k in kinds}
+    w = timed_window(ev, led, args.steps, kinds, 1 if args.breakdown else args.sample_every)
     ev.close()
-    upd = c1.updates - c0.updates
-    in_out = c1.in_out - c0.in_out
+    el, prof = w["elapsed"], w["prof"]
     log(f"{label}: steps {done + 1}..{done + args.steps}, {el:.2f} s, "
-        f"terms {int(act[0])} -> {int(act[-1])}")
+        f"terms {w['terms_first']} -> {w['terms_last']}")
     term_bytes = 8 * ((n + 31) // 32) + 8
     ms_w, n_w = prof["k_rot_write"]
     roof = None
     if n_w and ms_w > 0:
-        alg = term_bytes * in_out / n_w  # every input term read once, every output written once
+        # every input term read once, every output written once (sampled launches)
+        alg = term_bytes * w["s_in_out"] / n_w
         avg_s = ms_w * 1e-3 / n_w
         roof = {"bound": "hbm", "achieved": alg / avg_s / 1e9, "peak": 8000.0, "unit": "GB/s",
                 "frac": alg / avg_s / 8e12, "traffic": None, "kernel": "k_rot_write",
-                "launches": n_w, "avg_us": avg_s * 1e6, "algorithmic_bytes_per_launch": alg,
+                "launches": n_w, "sample": w["sample"], "avg_us": avg_s * 1e6,
+                "algorithmic_bytes_per_launch": alg,
                 "kernels": {k: {"launches": v[1], "avg_us": v[0] * 1e3 / max(v[1], 1)}
                             for k, v in prof.items()}}
     return {
-        "metric": f"FP64 Pauli branch-updates/s ({label})", "value": upd / el,
+        "metric": f"FP64 Pauli branch-updates/s ({label})", "value": w["updates"] / el,
         "unit": "branch-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": done,
         "ms_per_step": el / max(args.steps, 1) * 1e3, "higher_is_better": True,
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (initial state sum_s X_s)",
-        "config": {"workload": workload, "terms_at_start": int(act[0]),
-                   "terms_at_end": int(act[-1]), "fid": [float(x) for x in fid[[0, -1]]],
+        "config": {"workload": workload, "terms_at_start": w["terms_first"],
+                   "terms_at_end": w["terms_last"], "fid": [w["fid_first"], w["fid_last"]],
                    "ledger_dropped_l2": led.dropped_l2, "warmup_seconds": warm_s,
                    "whole_run_seconds": warm_s + el},
         "roofline": roof,
@@ -445,61 +502,17 @@ def main():
     ev.run(0, led)  # one readout: allocation and first-touch outside the timed region
     timed_kinds = [N.PROFILE_MERGE, N.PROFILE_COUNT]
     kinds = list(range(len(N.PROFILE_NAMES))) if args.breakdown else timed_kinds
-    mask = sum(1 << k for k in kinds)
-    # The window runs as calls of a few steps (one readout sync each).  HIP events around every
-    # launch cost a few microseconds each, so the kernel durations are sampled: events are on
-    # for every `sample_every`-th call, and the algorithmic bytes of the roofline are counted
-    # over those same calls (engine counters), so achieved = bytes / time covers the same
-    # launches.  --breakdown and --sample-every 1 time every call.
-    call_steps = 5 if args.steps <= 100 else 25
-    sample_every = 1 if args.breakdown else max(1, args.sample_every)
-    n_calls = -(-args.steps // call_steps)
 
-    def sampled_call(c):
-        # mirrored pairs around the window's centre (calls 1 and 6 of 8 at every 4th), so the
-        # sampled launches see the window's mean map size while the map grows
-        if sample_every == 1 or n_calls < 2 * sample_every:  # short windows: every call
-            return True
-        k = c if 2 * c < n_calls else n_calls - 1 - c
-        return k % sample_every == 1
+    def progress(done, terms_now, secs):  # a whole FID: a line per call of 25 steps
+        if args.steps > 100:
+            log(f"timed step {args.warmup + done}: {terms_now} terms, {secs:.1f} s")
 
-    prof = {N.PROFILE_NAMES[k]: [0.0, 0] for k in kinds}
-    s_updates = s_in_out = s_steps = 0
-    c0 = ev.counters()
-    t0 = time.perf_counter()
-    done = call = 0
-    while done < args.steps:
-        chunk = min(call_steps, args.steps - done)
-        sampled = sampled_call(call)
-        ev.profile(mask if sampled else 0)  # resets the event totals
-        ca = ev.counters()
-        st, fid, rad, act, acc = ev.run(chunk, led)
-        if st:
-            raise RuntimeError(f"timed run failed at step {args.warmup + done + acc}: status "
-                               f"{st}: {N.engine().tpgp_hip_last_error().decode()}")
-        if sampled:
-            cb = ev.counters()
-            for k in kinds:
-                ms, cnt = ev.profile_read(k)
-                prof[N.PROFILE_NAMES[k]][0] += ms
-                prof[N.PROFILE_NAMES[k]][1] += cnt
-            s_updates += cb.updates - ca.updates
-            s_in_out += cb.in_out - ca.in_out
-            s_steps += acc
-        done += acc
-        call += 1
-        if args.steps > 100 and done < args.steps:
-            log(f"timed step {args.warmup + done}: {int(act[-1])} terms, "
-                f"{time.perf_counter() - t0:.1f} s")
-    t1 = time.perf_counter()
-    c1 = ev.counters()
-    prof = {k: tuple(v) for k, v in prof.items()}
-    ev.profile(0)
+    w = timed_window(ev, led, args.steps, kinds, 1 if args.breakdown else args.sample_every,
+                     progress)
     ev.close()
-    elapsed = t1 - t0
-    updates = c1.updates - c0.updates
-    in_out = c1.in_out - c0.in_out
-    rotations = c1.rotations - c0.rotations
+    elapsed, prof = w["elapsed"], w["prof"]
+    updates, in_out, rotations = w["updates"], w["in_out"], w["rotations"]
+    s_updates, s_in_out = w["s_updates"], w["s_in_out"]
     key_bytes = 8 * ((n + 31) // 32)
     term_bytes = key_bytes + 8
     value = updates / elapsed
@@ -512,7 +525,7 @@ def main():
     dom_bytes = alg_bytes[dom]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
     log(f"timed {args.steps} steps: {elapsed:.3f} s, {rotations} rotations, {updates} updates, "
-        f"terms {start_terms} -> {int(act[-1])}")
+        f"terms {start_terms} -> {w['terms_last']}")
     for name, (ms, cnt) in prof.items():
         log(f"  {name:16s} {cnt:7d} launches {ms:9.1f} ms total {ms / max(cnt, 1) * 1e3:8.1f} us avg")
 
@@ -542,9 +555,9 @@ def main():
         "config": {"workload": "N=16 J1-J2 ring FID, J2 from InputPCG(7), dt=0.005, "
                                "threshold 1e-6, binary64",
                    "timed_steps": f"{args.warmup + 1}..{args.warmup + args.steps}",
-                   "terms_at_start": start_terms, "terms_at_end": int(act[-1]),
+                   "terms_at_start": start_terms, "terms_at_end": w["terms_last"],
                    "rotations": rotations, "branch_updates": updates, "terms_in_out": in_out,
-                   "fid_at_end": float(fid[-1]),
+                   "fid_at_end": w["fid_last"],
                    # a loaded state carries no ledger of its warmup steps
                    "fid_allowance_at_end": None if args.load_state else
                    fid_allowance(E, terms, dt, led, n, args.warmup + args.steps)},
@@ -553,10 +566,7 @@ def main():
                      "traffic": args.traffic if args.traffic is not None
                      else MEASURED_TRAFFIC.get((dom, args.warmup, args.steps)),
                      "kernel": dom, "launches": dom_n,
-                     "sample": f"HIP events on {s_steps} of the {args.steps} timed steps (calls "
-                               f"of {call_steps} steps, 1 in {sample_every}, in pairs mirrored "
-                               f"about the window centre); algorithmic bytes over the same "
-                               f"launches",
+                     "sample": w["sample"],
                      "avg_us": dom_ms / max(dom_n, 1) * 1e3,
                      "algorithmic_bytes_per_launch": dom_bytes / max(dom_n, 1),
                      "kernels": {k: {"launches": n_, "avg_us": ms / max(n_, 1) * 1e3}
