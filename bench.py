@@ -134,6 +134,9 @@ def bench_powder(args):
         rounds += 1
         if running == 0:
             break
+        if rounds % 10 == 0:  # a progress line per 10 fleet rounds (Strang steps)
+            log(f"powder batched: round {rounds}, {running} running, "
+                f"{time.perf_counter() - t0:.1f} s")
     t_batch = time.perf_counter() - t0
     c = fl.counters()
     fids = [fl.fid(k, steps + 1)[0] for k in range(K)]
@@ -159,7 +162,7 @@ def bench_powder(args):
     batched, alone = K / t_batch, A / t_alone
     log(f"powder one-at-a-time: {A} FIDs in {t_alone:.2f} s; identical FIDs: {same}")
     return {
-        "metric": "batched powder FIDs/s (N=27 3x3x3 cluster, 20-step FID window)",
+        "metric": f"batched powder FIDs/s (N=27 3x3x3 cluster, {steps}-step FID window)",
         "value": batched, "unit": "FIDs/s", "n_gpus": 1, "higher_is_better": True,
         "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (orientations from InputPCG(20260711); initial state sum_s X_s)",
