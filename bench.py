@@ -238,6 +238,17 @@ def fid_allowance(E, terms, dt, led, n, steps):
     return E.fid_allowance(pf, led, n)
 
 
+def sampled_calls(steps, call_steps, sample_every):
+    """Indices of the calls (of `call_steps` steps) that a window of `steps` steps times with HIP
+    events: one in `sample_every`, in pairs mirrored about the window's centre, so the sample's
+    mean position is the window's; windows shorter than two periods time every call."""
+    n_calls = -(-steps // call_steps)
+    if sample_every <= 1 or n_calls < 2 * sample_every:
+        return list(range(n_calls))
+    return [c for c in range(n_calls)
+            if (c if 2 * c < n_calls else n_calls - 1 - c) % sample_every == 1]
+
+
 def timed_window(ev, led, steps, kinds, sample_every, progress=None):
     """`steps` Strang steps on the resident map, timed end to end, as calls of a few steps (one
     readout sync each).  HIP events around every launch cost a few microseconds each, so the
@@ -248,13 +259,7 @@ def timed_window(ev, led, steps, kinds, sample_every, progress=None):
     from ddevol_amd import native as N
 
     call_steps = 5 if steps <= 100 else 25
-    n_calls = -(-steps // call_steps)
-
-    def sampled_call(c):
-        if sample_every <= 1 or n_calls < 2 * sample_every:
-            return True
-        k = c if 2 * c < n_calls else n_calls - 1 - c
-        return k % sample_every == 1
+    sampled = set(sampled_calls(steps, call_steps, sample_every))
 
     mask = sum(1 << k for k in kinds)
     prof = {N.PROFILE_NAMES[k]: [0.0, 0] for k in kinds}
@@ -265,14 +270,14 @@ def timed_window(ev, led, steps, kinds, sample_every, progress=None):
     done = call = 0
     while done < steps:
         chunk = min(call_steps, steps - done)
-        sampled = sampled_call(call)
-        ev.profile(mask if sampled else 0)  # resets the event totals
+        timed = call in sampled
+        ev.profile(mask if timed else 0)  # resets the event totals
         ca = ev.counters()
         st, fid, rad, act, acc = ev.run(chunk, led)
         if st:
             raise RuntimeError(f"timed run failed {done + acc} steps into the window: status "
                                f"{st}: {N.engine().tpgp_hip_last_error().decode()}")
-        if sampled:
+        if timed:
             cb = ev.counters()
             for k in kinds:
                 ms, cnt = ev.profile_read(k)

@@ -1,0 +1,48 @@
+"""bench.py on the CPU: the sampling of HIP-event timing over the timed window, and the refusal
+of more than one GPU (the engine runs one FID per GPU)."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+@pytest.mark.parametrize("steps,call_steps,every", [(40, 5, 4), (1000, 25, 4), (400, 25, 4),
+                                                    (80, 5, 4), (40, 5, 3), (48, 5, 4)])
+def test_sampled_calls_mirrored(steps, call_steps, every):
+    n = -(-steps // call_steps)
+    calls = bench.sampled_calls(steps, call_steps, every)
+    assert calls == sorted(set(calls)) and all(0 <= c < n for c in calls)
+    # one call in `every` (within one), and the sample's mean position is the window's centre
+    assert abs(len(calls) - n / every) <= 1
+    assert sum(calls) / len(calls) == pytest.approx((n - 1) / 2)
+    # mirrored: c is sampled exactly when n - 1 - c is
+    assert set(calls) == {n - 1 - c for c in calls}
+
+
+def test_sampled_calls_headline_window():
+    assert bench.sampled_calls(40, 5, 4) == [1, 6]
+
+
+@pytest.mark.parametrize("steps,call_steps", [(5, 5), (10, 5), (20, 5), (35, 5), (3, 5)])
+def test_short_windows_time_every_call(steps, call_steps):
+    n = -(-steps // call_steps)
+    assert bench.sampled_calls(steps, call_steps, 4) == list(range(n))
+
+
+def test_every_call_when_asked():
+    assert bench.sampled_calls(40, 5, 1) == list(range(8))
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one FID per GPU" in msg["error"]
