@@ -10,7 +10,8 @@ Checked at that size:
 3. the device's sum of squares (tpgp_evolution_l2) agrees with the downloaded coefficients';
 4. parity at full size: the first bond's three rotations applied to the large map by the GPU
    and by the oracle (engine/src/tpgp_rev.c:535-771) give the same keys, values and ledger bit
-   for bit.
+   for bit;
+5. the same in validated mode (radii through the merges, the arithmetic ledger).
 """
 import math
 import time
@@ -93,3 +94,20 @@ def test_large_map_properties_and_rotation_parity(hip_ctx, oracle, name):
         assert_same_map(om, gm)
         assert ol.bits() == gl.bits()
         print(f"{name}: rotation ({i},{j},{qi},{qj}) on {len(vals)} terms bit-exact", flush=True)
+
+    # 5. validated mode on the same large map: enclosure radii through the merges and the
+    #    arithmetic ledger (engine/src/tpgp_rev.c:484-534, 699-706), bit for bit as well
+    (i, j, qi, qj, h) = terms[0]
+    theta = 2.0 * h * dt
+    om = oracle.new_map(n)
+    assert om.assign(keys, vals) == 0
+    ol = oracle.ledger(1)
+    assert om.rotate(i, j, qi, qj, theta, oracle.options(threshold=thr, validated=1), ol) == 0
+    gm = E.Map.from_sorted(n, keys, vals)
+    gl = E.ledger(1)
+    go = E.options(threshold=thr, validated=1, context=hip_ctx)
+    assert E.rotate(gm, i, j, qi, qj, theta, go, gl) == 0
+    assert_same_map(om, gm)
+    assert ol.bits() == gl.bits()
+    assert gl.arithmetic_l2 > 0
+    print(f"{name}: validated rotation on {len(vals)} terms bit-exact", flush=True)
