@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <dlfcn.h>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -3036,7 +3037,28 @@ extern "C" tpgp_hip_context *tpgp_hip_create(int index, char *error, size_t cap)
         delete ctx;
         return nullptr;
     }
+    // the table: TPGP_HIP_TUNING names it (empty: the built-in defaults); unset, the library
+    // looks for tuning_gfx950.txt in the directory above its own (the package directory)
     const char *tuning = getenv("TPGP_HIP_TUNING");
+    char beside[4096] = {0};
+    if (!tuning) {
+        Dl_info info;
+        if (dladdr((void *)&tpgp_hip_create, &info) && info.dli_fname &&
+            strlen(info.dli_fname) < sizeof(beside) - 32) {
+            strcpy(beside, info.dli_fname);
+            char *slash = strrchr(beside, '/');
+            if (slash) *slash = 0;  // .../lib
+            slash = strrchr(beside, '/');
+            if (slash) {
+                strcpy(slash, "/tuning_gfx950.txt");
+                FILE *probe = fopen(beside, "r");
+                if (probe) {
+                    fclose(probe);
+                    tuning = beside;
+                }
+            }
+        }
+    }
     if (tuning && *tuning && !load_tuning(ctx, tuning, error, cap)) {
         tpgp_hip_destroy(ctx);
         return nullptr;

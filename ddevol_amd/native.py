@@ -25,8 +25,8 @@ TPGP_HIP = 4
 TUNE_PARTITION_ROUNDS, TUNE_COUNT_GRID, TUNE_WRITE_GRID = range(3)
 TUNE_PARAMS, TUNE_BUCKETS = 3, 48
 TUNE_NAMES = ["partition_rounds", "count_grid", "write_grid"]
-# the table the autotuner measured on MI355X (ddevol_amd/autotune.py); contexts load it through
-# TPGP_HIP_TUNING unless the caller names another file (or an empty name: built-in defaults)
+# the table the autotuner measured on MI355X (ddevol_amd/autotune.py); tpgp_hip_create loads it
+# from beside the library unless TPGP_HIP_TUNING names another file (empty: built-in defaults)
 TUNING_TABLE = os.path.join(HERE, "tuning_gfx950.txt")
 
 _hip = None
@@ -163,8 +163,6 @@ def hip():
     global _hip
     if _hip is not None:
         return _hip
-    if "TPGP_HIP_TUNING" not in os.environ and os.path.exists(TUNING_TABLE):
-        os.environ["TPGP_HIP_TUNING"] = TUNING_TABLE  # read by every tpgp_hip_create
     L = _load(HIP_SO)
     p_u64, p_d, p_c = C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.c_char_p
     sz = C.c_size_t

@@ -223,8 +223,9 @@ int tpgp_hip_batch_profile_read(const tpgp_hip_batch *batch, unsigned kind,
  *   COUNT_GRID, WRITE_GRID: workgroups of the rotation's count and write passes in sixths of
  *     the estimated tile count (plus 64), 1..24; defaults 3 (a half) and 4 (two thirds).
  * tpgp_hip_create loads a table from the file named by the environment variable
- * TPGP_HIP_TUNING when it is set (lines "words param bucket value", '#' starts a comment; an
- * invalid line fails the create).  A setting applies to the passes enqueued after it.  EINVAL
+ * TPGP_HIP_TUNING (empty: the built-in defaults), or, when it is unset, from
+ * tuning_gfx950.txt in the directory above the library's own (lines "words param bucket value",
+ * '#' starts a comment; an invalid line fails the create).  A setting applies to the passes enqueued after it.  EINVAL
  * for a key width, parameter, bucket or value out of range. */
 enum {
     TPGP_HIP_TUNE_PARTITION_ROUNDS = 0,

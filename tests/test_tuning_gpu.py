@@ -37,8 +37,14 @@ def tuned_ctx():
 
 
 def test_committed_table_loads():
-    """A default context carries the committed table's choices (bindings set TPGP_HIP_TUNING)."""
-    ctx = E.Context(0)
+    """With TPGP_HIP_TUNING unset, a context loads the committed table from beside the library
+    (tuning_gfx950.txt in the package directory, above lib/)."""
+    old = os.environ.pop("TPGP_HIP_TUNING", None)
+    try:
+        ctx = E.Context(0)
+    finally:
+        if old is not None:
+            os.environ["TPGP_HIP_TUNING"] = old
     try:
         with open(N.TUNING_TABLE) as f:
             rows = [tuple(int(x) for x in ln.split("#")[0].split())
