@@ -6,23 +6,27 @@ with u_i the first 16 uniforms of the reference's seeded InputPCG(7); secular di
 (96 per step order); observable sum_s X_s with coefficients 1; Strang steps dt = 0.005;
 truncation |c| <= 1e-6; binary64 throughout.  Synthetic by construction: no dataset.
 
-A "step" is one Strang step: 192 rotations over the resident map.  W untimed warmup steps grow
-the map from 16 terms (it reaches ~6e6 terms by step 500 and ~4e7 by step 750); the timed
-region is the next K steps, bracketed by device synchronisation, with the map already resident
-in HBM.  One branch-update = one input term processed by one executed rotation (the
-reference's ledger.rotations counts the same rotations, engine/src/tpgp_rev.c:740).
+A "step" is one Strang step: 192 rotations over the resident map.  The headline is quoted at
+the FID's operating point: untimed steps 1..600 grow the map from 16 terms to ~1.43e7 (8-9 s on
+MI355X), whatever --warmup says; --warmup W adds W further untimed steps there, and the timed
+region is the next K steps (601+W .. 600+W+K), bracketed by device synchronisation, with the map
+already resident in HBM.  --start-step moves the operating point (0 with --warmup 0 times a whole
+FID).  One branch-update = one input term processed by one executed rotation (the reference's
+ledger.rotations counts the same rotations, engine/src/tpgp_rev.c:740).
 
-roofline: the dominant kernel is whichever of the rotation count pass (k_rot_count) and the
-rotation write pass (k_rot_write) takes more time in the timed region.  Algorithmic bytes per
-launch, for W key words (16 B per term for N <= 32): k_rot_write (8W + 8)(len_in + len_out) --
-the input map read once and the output map written once; k_rot_count (8W + 8) len_in.  Their
-durations are measured live with HIP events recorded on the batch stream around every launch
-in the timed region (only those two kinds, unless --breakdown asks for all).
+roofline: the dominant kernel is the leaf kernel kind (rotation write, count, splits, scans,
+partition histogram and scatter, small-map kernel) with the largest total time over the sampled
+launches of the timed region.  Algorithmic bytes per launch, for W key words (16 B per term for
+N <= 32): k_rot_write (8W + 8)(len_in + len_out) -- the input map read once and the output map
+written once; k_rot_count (8W + 8) len_in; k_small (8W + 8)(len_in + len_out) per rotation it
+runs.  Durations are measured live with HIP events recorded on the batch stream around the
+launches of a sample of the timed calls.
 
 cpu_baseline: the reference CPU engine (engine/src/tpgp_rev.c compiled from its own sources into
-oracle/_ref) -- or, where that build is absent, the oracle's C restatement -- runs the same
-rotation sequence from the same map state (the state at the start of the timed region) with its
-OpenMP branch backend on the host cores, for a bounded ~20 s sample.
+oracle/_ref) -- or, where that build is absent, the oracle's C restatement -- runs the J1-J2
+rotation sequence, repeated, from the same map state (the state at the start of the timed
+region) with its OpenMP branch backend for --cpu-seconds (default 20 s) on every core of this
+job's CPU share, then again on one thread.  The CPU model and thread counts are reported.
 """
 from __future__ import annotations
 
@@ -48,10 +52,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(keys, values, n_spins, terms, dt, threshold, budget_s, threads):
-    """Time the reference (or oracle) CPU path on the same state; returns a dict."""
-    import numpy as np
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
 
+    return platform.processor() or platform.machine()
+
+
+def host_threads():
+    """Threads for the CPU baseline: every core this job may use.  On a shared host the job's
+    share is what OMP_NUM_THREADS grants (its CPU allotment); otherwise the cores in this
+    process's affinity mask."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return min(int(env), avail or int(env)), avail
+    return max(1, avail or 1), avail
+
+
+def cpu_baseline(keys, values, n_spins, terms, dt, threshold, budget_s, threads):
+    """Time the reference (or oracle) CPU path on the same state: the Strang rotation sequence,
+    repeated from the given map state (the evolution simply continues), until `budget_s` has
+    passed; returns a dict."""
     from oracle import bindings as OB
 
     kind = "reference" if os.path.exists(OB.REF_SO) else "port"
@@ -82,9 +111,8 @@ def cpu_baseline(keys, values, n_spins, terms, dt, threshold, budget_s, threads)
     updates = 0
     rots = 0
     t0 = time.perf_counter()
-    for (i, j, qi, qj, h) in seq:
-        if time.perf_counter() - t0 > budget_s:
-            break
+    while time.perf_counter() - t0 < budget_s:
+        i, j, qi, qj, h = seq[rots % len(seq)]
         n_in = len(m)
         # tpgp_rotate(theta=h*half) evaluates cos/sin of 2*(h*half)*1, which equals the step's
         # 2*h*half bit for bit (scaling by 2 is exact), so the work is the step's rotation
@@ -95,9 +123,8 @@ def cpu_baseline(keys, values, n_spins, terms, dt, threshold, budget_s, threads)
         rots += 1
     el = time.perf_counter() - t0
     return {"value": updates / el, "unit": "branch-updates/s", "cores": threads, "kind": kind,
-            "sample": f"{rots} rotations (the J1-J2 Strang sequence from its first term) on the "
-                      f"{len(values)}-term map at the start of the timed region, "
-                      f"{el:.1f} s, OpenMP branch generation on {threads} host threads"}
+            "rotations": rots, "seconds": el, "terms_at_start": len(values),
+            "terms_at_end": len(m)}
 
 
 def same_bits(a, b):
@@ -258,7 +285,7 @@ def timed_window(ev, led, steps, kinds, sample_every, progress=None):
     taken over the same calls (engine counters)."""
     from ddevol_amd import native as N
 
-    call_steps = 5 if steps <= 100 else 25
+    call_steps = 2 if steps <= 40 else 5 if steps <= 100 else 25
     sampled = set(sampled_calls(steps, call_steps, sample_every))
 
     mask = sum(1 << k for k in kinds)
@@ -323,16 +350,17 @@ def bench_single(args, n, terms, dt, thr, label, workload):
 
         st = np.load(args.load_state)
         mp = E.Map.from_sorted(n, st["keys"], st["values"])
-        done = args.warmup = int(st["warmup"])
+        done = int(st["warmup"])
         log(f"{label}: loaded state after {done} steps: {len(mp)} terms")
+    grow = (done if args.load_state else args.start_step) + args.warmup
     ev = E.Evolution(mp, terms, dt, opts, context=ctx,
                      initial_capacity=4 * len(mp) + (1 << 20) if args.load_state else 0)
     size = len(mp)
     tw = time.perf_counter()
     # untimed, in chunks with a progress line each; --until-terms: until the map reaches that
-    # many terms (--warmup then bounds the steps)
-    while done < args.warmup and not (args.until_terms and size >= args.until_terms):
-        chunk = min(5 if args.until_terms else 25, args.warmup - done)
+    # many terms (the start step then bounds the steps)
+    while done < grow and not (args.until_terms and size >= args.until_terms):
+        chunk = min(5 if args.until_terms else 25, grow - done)
         st, fid, rad, act, acc = ev.run(chunk, led)
         assert st == 0, N_last_error()
         done += chunk
@@ -401,11 +429,15 @@ def main():
     ap.add_argument("--serve-steps", type=int, nargs=2, default=[10, 200])
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--budget-gb", type=float, default=64.0)
-    ap.add_argument("--steps", type=int, default=None,
-                    help="timed Strang steps (default 40; 20 for --workload large)")
-    ap.add_argument("--warmup", type=int, default=None,
-                    help="untimed steps before them (default 600; 480 for --workload large: "
-                         "the timed steps are the last 20 of its 500-step FID)")
+    ap.add_argument("--steps", type=int, default=20, help="timed Strang steps")
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="untimed steps at the operating point, after the untimed growth to "
+                         "--start-step and before the timed steps")
+    ap.add_argument("--start-step", type=int, default=None,
+                    help="the operating point: untimed Strang steps that grow the map before "
+                         "--warmup (default 600, where the N=16 J1-J2 map holds ~1.4e7 terms; "
+                         "475 for --workload large, so that the timed steps are the last 20 of "
+                         "its 500-step FID; 0 with --warmup 0 times a whole FID)")
     ap.add_argument("--sample-every", type=int, default=4,
                     help="headline: HIP events on every Nth call of the timed window (kernel "
                          "durations for the roofline); 1 = every call")
@@ -422,12 +454,27 @@ def main():
                          "FETCH_SIZE + WRITE_SIZE, calibrated), reported as roofline.traffic")
     ap.add_argument("--load-state", help="start the timed region from a state saved by "
                                          "--save-state (its warmup count is used)")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print the timed window's kernel table from a rocprofv3 --kernel-trace "
+                         "output directory of a bench run (needs --bench-json) and exit")
+    ap.add_argument("--summarize-pmc", metavar="DIR", nargs="+",
+                    help="print the timed window's counter averages from rocprofv3 --pmc "
+                         "output directories of a bench run (needs --bench-json) and exit")
+    ap.add_argument("--copy-pmc", metavar="DIR", nargs="*",
+                    help="with --summarize-pmc: the same counters for a --copy-bytes run")
+    ap.add_argument("--bench-json", help="the JSON line the profiled bench run printed")
     args = ap.parse_args()
-    large = args.workload == "large"
-    if args.steps is None:
-        args.steps = 20 if large else 40
-    if args.warmup is None:
-        args.warmup = 480 if large else 600
+    if args.summarize_trace or args.summarize_pmc:
+        with open(args.bench_json) as fh:
+            line = json.loads([s for s in fh.read().splitlines() if s.startswith("{")][-1])
+        rot = line["config"]["rotations"]
+        if args.summarize_trace:
+            print(summarize_trace(args.summarize_trace, rot, line["steps"]))
+        else:
+            print(summarize_pmc(args.summarize_pmc, rot, args.copy_pmc))
+        return
+    if args.start_step is None and args.workload != "j1j2":
+        args.start_step = 475 if args.workload == "large" else HEADLINE_START_STEP
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one FID per GPU: only --gpus 1 is supported; "
                                    "multi-GPU sharding is a design, not part of this build"}))
@@ -460,6 +507,145 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
+    print(json.dumps(bench_headline(args)), flush=True)
+
+
+def roofline_of(prof, term_bytes, s_updates, s_in_out, sample):
+    """Roofline entry for the dominant kernel: the leaf kernel kind with the largest total time
+    over the sampled launches, among those with an algorithmic byte model.  Algorithmic bytes
+    per launch (B = 8W + 8 bytes per term): k_rot_write B(len_in + len_out) -- every input term
+    read once, every output written once; k_rot_count B len_in; k_small B(len_in + len_out) per
+    rotation it runs (only when it ran every sampled rotation, so the counters are its own)."""
+    models = {"k_rot_write": term_bytes * s_in_out, "k_rot_count": term_bytes * s_updates}
+    if prof.get("k_rot_write", (0, 0))[1] == 0 and prof.get("k_small", (0, 0))[1]:
+        models = {"k_small": term_bytes * s_in_out}
+    leaf = {k: v for k, v in prof.items() if k not in ("rotation_pass", "partition_pass")}
+    ranked = sorted(leaf, key=lambda k: leaf[k][0], reverse=True)
+    dom = next((k for k in ranked if k in models and leaf[k][1]), None)
+    kernels = {k: {"launches": c, "avg_us": ms / max(c, 1) * 1e3, "total_ms": ms}
+               for k, (ms, c) in leaf.items() if c}
+    if dom is None:
+        return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None, "traffic": None, "kernel": None, "sample": sample,
+                "kernels": kernels}
+    ms, cnt = leaf[dom]
+    alg = models[dom] / cnt
+    avg_s = ms * 1e-3 / cnt
+    achieved = alg / avg_s / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "launches": cnt,
+            "avg_us": avg_s * 1e6, "algorithmic_bytes_per_launch": alg,
+            "largest_kind_by_time": ranked[0] if ranked else None, "sample": sample,
+            "kernels": kernels}
+
+
+def _rocprof_csvs(path, suffix):
+    import glob
+
+    if os.path.isfile(path):
+        return [path]
+    return sorted(glob.glob(os.path.join(path, "**", f"*{suffix}"), recursive=True))
+
+
+def _short_name(name):
+    """'void k_rot_write<1>(BatchDev, ...)' -> 'k_rot_write<1>'."""
+    name = name.strip()
+    if name.startswith("void "):
+        name = name[5:]
+    depth = 0
+    for k, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return name[:k].strip()
+    return name
+
+
+def summarize_trace(path, rotations, steps, anchor="k_rot_write"):
+    """Markdown table of a rocprofv3 --kernel-trace over the timed window only.  The window is
+    found from the run itself: every rotation of the multi-kernel path launches the anchor kernel
+    once, so the window is everything dispatched after the anchor launch that precedes the last
+    `rotations` ones (the timed region is the run's last GPU work; the CPU baseline launches
+    nothing)."""
+    import csv
+
+    rows = []
+    for f in _rocprof_csvs(path, "kernel_trace.csv"):
+        with open(f, newline="") as fh:
+            rows += list(csv.DictReader(fh))
+    if not rows:
+        raise SystemExit(f"no kernel_trace.csv under {path}")
+    recs = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                   _short_name(r["Kernel_Name"])) for r in rows)
+    idx = [k for k, r in enumerate(recs) if r[2].split("<")[0] == anchor]
+    if len(idx) < rotations:
+        raise SystemExit(f"only {len(idx)} {anchor} launches in the trace, window has {rotations}")
+    first = idx[len(idx) - rotations]
+    prev_end = recs[idx[len(idx) - rotations - 1]][1] if len(idx) > rotations else -1
+    win = [r for r in recs[:first] if r[0] > prev_end] + recs[first:]
+    stats = {}
+    for s, e, name in win:
+        d = (e - s) * 1e-3  # us
+        st = stats.setdefault(name, [0, 0.0, float("inf"), 0.0])
+        st[0] += 1
+        st[1] += d
+        st[2] = min(st[2], d)
+        st[3] = max(st[3], d)
+    total = sum(v[1] for v in stats.values())
+    out = ["| kernel | calls | total ms | avg us | min us | max us | % |",
+           "|---|---|---|---|---|---|---|"]
+    for name, (n, tot, lo, hi) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        out.append(f"| {name} | {n} | {tot * 1e-3:.1f} | {tot / n:.1f} | {lo:.1f} | {hi:.1f} | "
+                   f"{100 * tot / total:.2f} |")
+    span = (win[-1][1] - win[0][0]) * 1e-6
+    out.append("")
+    out.append(f"Window: {len(win)} dispatches, {rotations} rotations over {steps} steps; "
+               f"device time {total * 1e-3:.1f} ms ({total * 1e-3 / steps:.2f} ms per step), "
+               f"first-to-last dispatch span {span:.1f} ms ({span / steps:.2f} ms per step).")
+    return "\n".join(out)
+
+
+def summarize_pmc(path, rotations, copy_path=None, kernels=("k_rot_write", "k_rot_count")):
+    """Markdown table of rocprofv3 --pmc runs (one counter per run): the per-dispatch average of
+    each counter over the last `rotations` launches of each kernel (the timed window), and, when
+    `copy_path` holds the same counters for a plain device copy, the copy's average."""
+    import csv
+
+    def load(p):
+        vals = {}  # (counter, kernel) -> [(dispatch, value)]
+        for f in _rocprof_csvs(p, "counter_collection.csv"):
+            with open(f, newline="") as fh:
+                for r in csv.DictReader(fh):
+                    key = (r["Counter_Name"], _short_name(r["Kernel_Name"]))
+                    vals.setdefault(key, []).append((int(r["Dispatch_Id"]),
+                                                     float(r["Counter_Value"])))
+        return vals
+
+    out = ["| kernel | counter | dispatches | average (KiB per dispatch) |", "|---|---|---|---|"]
+    for p in path:
+        for (cn, kn), v in sorted(load(p).items()):
+            if kn.split("<")[0] not in kernels:
+                continue
+            v = sorted(v)[-rotations:]
+            out.append(f"| {kn} | {cn} | {len(v)} | {sum(x for _, x in v) / len(v):.1f} |")
+    for p in copy_path or []:
+        for (cn, kn), v in sorted(load(p).items()):
+            out.append(f"| {kn} (copy) | {cn} | {len(v)} | {sum(x for _, x in v) / len(v):.1f} |")
+    return "\n".join(out)
+
+
+# The headline FID's operating point: Strang step 600 of the N=16 J1-J2 FID, where the map holds
+# ~1.43e7 terms (the survey's windowing: the CPU path cannot run the whole 1000-step FID).
+HEADLINE_START_STEP = 600
+
+
+def bench_headline(args):
+    """The N=16 J1-J2 FID at its operating point.  Untimed: steps 1..start (the map grows from 16
+    to ~1.4e7 terms), then --warmup further untimed steps there.  Timed: the next --steps steps
+    on the resident map.  The CPU reference path then runs from the same state (the map at the
+    start of the timed region) on all of this job's cores, and on one core."""
     import numpy as np
 
     from ddevol_amd import engine as E
@@ -475,79 +661,90 @@ def main():
         mp.add_single(s, 1, 1.0)
     opts = E.options(threshold=thr, context=ctx)
     led = E.ledger(0)
+    start = args.start_step if args.start_step is not None else HEADLINE_START_STEP
+    grow = start + args.warmup  # untimed steps before the timed region
 
+    t = time.perf_counter()
     if args.load_state:
         st = np.load(args.load_state)
         keys0, vals0 = st["keys"], st["values"]
-        args.warmup = int(st["warmup"])
+        start = done = int(st["warmup"])
         mp = E.Map.from_sorted(n, keys0, vals0)
-        log(f"loaded state after {args.warmup} steps: {len(mp)} terms")
+        grow = done + args.warmup
+        log(f"loaded state after {done} steps: {len(mp)} terms")
     else:
-        # warmup: grow the map to the timed region's starting state (untimed)
-        t = time.perf_counter()
-        ev = E.Evolution(mp, terms, dt, opts)
         done = 0
-        while done < args.warmup:
-            chunk = min(50, args.warmup - done)
+    if done < grow:
+        ev = E.Evolution(mp, terms, dt, opts)
+        while done < grow:
+            chunk = min(50, grow - done)
             st, fid, rad, act, acc = ev.run(chunk, led)
             if st:
                 raise RuntimeError(f"warmup failed at step {done + acc}: status {st}: "
                                    f"{N.engine().tpgp_hip_last_error().decode()}")
             done += acc
-            log(f"warmup step {done}: {int(act[-1])} terms, FID {fid[-1]:.9f}, "
+            log(f"untimed step {done}: {int(act[-1])} terms, FID {fid[-1]:.9f}, "
                 f"{time.perf_counter() - t:.1f} s")
-        st = ev.close(mp)
-        assert st == 0
-        keys0, vals0 = mp.arrays()
-        if args.save_state:
-            np.savez(args.save_state, keys=keys0, values=vals0, warmup=args.warmup)
-            log(f"saved {len(mp)} terms to {args.save_state}")
-            return
+        assert ev.close(mp) == 0
+    keys0, vals0 = mp.arrays()
+    grow_s = time.perf_counter() - t
+    if args.save_state:
+        np.savez(args.save_state, keys=keys0, values=vals0, warmup=done)
+        log(f"saved {len(mp)} terms to {args.save_state}")
+        return {"saved": args.save_state, "terms": len(mp), "warmup": done}
     start_terms = len(mp)
 
-    # timed region: K steps on the resident map
+    # timed region: --steps steps on the resident map (slots sized for the window's growth, so
+    # no re-layout falls inside it)
     ev = E.Evolution(mp, terms, dt, opts, initial_capacity=4 * len(mp) + (1 << 20))
     ev.run(0, led)  # one readout: allocation and first-touch outside the timed region
-    timed_kinds = [N.PROFILE_MERGE, N.PROFILE_COUNT]
-    kinds = list(range(len(N.PROFILE_NAMES))) if args.breakdown else timed_kinds
+    leaf = [N.PROFILE_MERGE, N.PROFILE_SCATTER, N.PROFILE_COUNT, N.PROFILE_SCAN,
+            N.PROFILE_PLAN, N.PROFILE_HISTOGRAM, N.PROFILE_SPLITS, N.PROFILE_SMALL]
 
-    def progress(done, terms_now, secs):  # a whole FID: a line per call of 25 steps
+    def progress(k, terms_now, secs):  # a whole FID: a line per call of 25 steps
         if args.steps > 100:
-            log(f"timed step {args.warmup + done}: {terms_now} terms, {secs:.1f} s")
+            log(f"timed step {done + k}: {terms_now} terms, {secs:.1f} s")
 
-    w = timed_window(ev, led, args.steps, kinds, 1 if args.breakdown else args.sample_every,
-                     progress)
+    w = timed_window(ev, led, args.steps, leaf, args.sample_every, progress)
     ev.close()
-    elapsed, prof = w["elapsed"], w["prof"]
+    elapsed = w["elapsed"]
     updates, in_out, rotations = w["updates"], w["in_out"], w["rotations"]
-    s_updates, s_in_out = w["s_updates"], w["s_in_out"]
-    key_bytes = 8 * ((n + 31) // 32)
-    term_bytes = key_bytes + 8
+    term_bytes = 8 * ((n + 31) // 32) + 8
     value = updates / elapsed
-    # dominant kernel: the timed kind with the largest total time.  Algorithmic bytes per
-    # launch: k_rot_write reads every input term once and writes every output term once;
-    # k_rot_count reads every input term once.
-    alg_bytes = {"k_rot_write": term_bytes * s_in_out, "k_rot_count": term_bytes * s_updates}
-    dom = max(alg_bytes, key=lambda k: prof[k][0])
-    dom_ms, dom_n = prof[dom]
-    dom_bytes = alg_bytes[dom]
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
-    log(f"timed {args.steps} steps: {elapsed:.3f} s, {rotations} rotations, {updates} updates, "
-        f"terms {start_terms} -> {w['terms_last']}")
-    for name, (ms, cnt) in prof.items():
-        log(f"  {name:16s} {cnt:7d} launches {ms:9.1f} ms total {ms / max(cnt, 1) * 1e3:8.1f} us avg")
+    log(f"timed {args.steps} steps ({done + 1}..{done + args.steps}): {elapsed:.3f} s, "
+        f"{rotations} rotations, {updates} updates, terms {start_terms} -> {w['terms_last']}")
+    for name, (ms, cnt) in w["prof"].items():
+        log(f"  {name:16s} {cnt:7d} launches {ms:9.1f} ms total "
+            f"{ms / max(cnt, 1) * 1e3:8.1f} us avg")
+    roof = roofline_of(w["prof"], term_bytes, w["s_updates"], w["s_in_out"], w["sample"])
+    roof["traffic"] = (args.traffic if args.traffic is not None
+                       else MEASURED_TRAFFIC.get((roof["kernel"], done, args.steps)))
 
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads, avail = host_threads()
+        model = cpu_model()
         try:
-            cpu = cpu_baseline(np.ascontiguousarray(keys0.reshape(-1)), np.ascontiguousarray(vals0),
-                               n, terms, dt, thr, args.cpu_seconds, threads)
+            kflat = np.ascontiguousarray(keys0.reshape(-1))
+            vflat = np.ascontiguousarray(vals0)
+            cpu = cpu_baseline(kflat, vflat, n, terms, dt, thr, args.cpu_seconds, threads)
+            one = cpu_baseline(kflat, vflat, n, terms, dt, thr, args.cpu_seconds, 1)
+            cpu.update({
+                "cpu_model": model, "cores_in_affinity": avail,
+                "single_thread": {"value": one["value"], "rotations": one["rotations"],
+                                  "seconds": one["seconds"]},
+                "gpu_over_cpu": value / cpu["value"],
+                "sample": f"{cpu['rotations']} rotations of the J1-J2 Strang sequence (from "
+                          f"its first term, continuing the evolution) on the {start_terms}-term "
+                          f"map at the start of the timed region, {cpu['seconds']:.1f} s, "
+                          f"OpenMP branch generation on {threads} threads ({model}; all cores "
+                          f"of this job's CPU share, {avail} in its affinity mask); one thread: "
+                          f"{one['rotations']} rotations in {one['seconds']:.1f} s"})
             log(f"cpu baseline: {cpu}")
         except Exception as e:  # reported, not fatal
             log(f"cpu baseline unavailable: {e}")
 
-    out = {
+    return {
         "metric": "FP64 Pauli branch-updates/s (N=16 J1-J2); batched FIDs/s + p99 latency vs plain",
         "value": value,
         "unit": "branch-updates/s",
@@ -562,26 +759,20 @@ def main():
         "data": "synthetic (seeded couplings; initial state sum_s X_s)",
         "config": {"workload": "N=16 J1-J2 ring FID, J2 from InputPCG(7), dt=0.005, "
                                "threshold 1e-6, binary64",
-                   "timed_steps": f"{args.warmup + 1}..{args.warmup + args.steps}",
+                   "operating_point": f"untimed steps 1..{start} grow the map to the window's "
+                                      f"size; --warmup {args.warmup} further untimed steps "
+                                      f"there",
+                   "untimed_steps": done, "untimed_seconds": grow_s,
+                   "timed_steps": f"{done + 1}..{done + args.steps}",
                    "terms_at_start": start_terms, "terms_at_end": w["terms_last"],
                    "rotations": rotations, "branch_updates": updates, "terms_in_out": in_out,
                    "fid_at_end": w["fid_last"],
-                   # a loaded state carries no ledger of its warmup steps
+                   # a loaded state carries no ledger of its earlier steps
                    "fid_allowance_at_end": None if args.load_state else
-                   fid_allowance(E, terms, dt, led, n, args.warmup + args.steps)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": args.traffic if args.traffic is not None
-                     else MEASURED_TRAFFIC.get((dom, args.warmup, args.steps)),
-                     "kernel": dom, "launches": dom_n,
-                     "sample": w["sample"],
-                     "avg_us": dom_ms / max(dom_n, 1) * 1e3,
-                     "algorithmic_bytes_per_launch": dom_bytes / max(dom_n, 1),
-                     "kernels": {k: {"launches": n_, "avg_us": ms / max(n_, 1) * 1e3}
-                                 for k, (ms, n_) in prof.items()}},
+                   fid_allowance(E, terms, dt, led, n, done + args.steps)},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

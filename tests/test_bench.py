@@ -46,3 +46,38 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one FID per GPU" in msg["error"]
+
+
+def _trace_csv(path, launches):
+    with open(path, "w") as f:
+        f.write('"Kind","Kernel_Name","Start_Timestamp","End_Timestamp"\n')
+        for name, s, e in launches:
+            f.write(f'"KERNEL_DISPATCH","void {name}(BatchDev, PassMap const*)",{s},{e}\n')
+
+
+def test_trace_summary_keeps_only_the_window(tmp_path):
+    # two warmup rotations, then a window of three; each rotation: count then write
+    launches, t = [], 0
+    for r in range(5):
+        dur = 1000 if r < 2 else 5000  # window launches are longer
+        for name in ("k_rot_count<1, false>", "k_rot_write<1>"):
+            launches.append((name, t, t + dur))
+            t += dur + 10
+    _trace_csv(tmp_path / "run_kernel_trace.csv", launches)
+    md = bench.summarize_trace(str(tmp_path), rotations=3, steps=1)
+    rows = {line.split("|")[1].strip(): line.split("|") for line in md.splitlines()
+            if line.startswith("| k_")}
+    assert set(rows) == {"k_rot_count<1, false>", "k_rot_write<1>"}
+    for r in rows.values():
+        assert int(r[2]) == 3 and float(r[4]) == pytest.approx(5.0)
+
+
+def test_pmc_summary_averages_the_last_launches(tmp_path):
+    with open(tmp_path / "run_counter_collection.csv", "w") as f:
+        f.write('"Dispatch_Id","Kernel_Name","Counter_Name","Counter_Value"\n')
+        for d in range(6):
+            f.write(f'{d},"void k_rot_write<1>(BatchDev)","FETCH_SIZE",{100 if d < 2 else 400}\n')
+            f.write(f'{d + 100},"void k_part_scan(BatchDev)","FETCH_SIZE",7\n')
+    md = bench.summarize_pmc([str(tmp_path)], rotations=4)
+    assert "| k_rot_write<1> | FETCH_SIZE | 4 | 400.0 |" in md
+    assert "k_part_scan" not in md
