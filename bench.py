@@ -547,9 +547,27 @@ def _rocprof_csvs(path, suffix):
     return sorted(glob.glob(os.path.join(path, "**", f"*{suffix}"), recursive=True))
 
 
+def _demangle(names):
+    """Demangle Itanium C++ names (as a kernel trace may record them) with c++filt, when
+    present; other names pass through."""
+    import shutil
+    import subprocess
+
+    mangled = sorted({n for n in names if n.startswith("_Z")})
+    tool = shutil.which("c++filt") or shutil.which("llvm-cxxfilt") or \
+        (os.path.exists("/opt/rocm/llvm/bin/llvm-cxxfilt") and "/opt/rocm/llvm/bin/llvm-cxxfilt")
+    if not mangled or not tool:
+        return {}
+    out = subprocess.run([tool], input="\n".join(mangled), capture_output=True, text=True)
+    lines = out.stdout.splitlines()
+    return dict(zip(mangled, lines)) if len(lines) == len(mangled) else {}
+
+
 def _short_name(name):
     """'void k_rot_write<1>(BatchDev, ...)' -> 'k_rot_write<1>'."""
-    name = name.strip()
+    name = name.strip().replace("(anonymous namespace)::", "")
+    if name.endswith(".kd"):
+        name = name[:-3]
     if name.startswith("void "):
         name = name[5:]
     depth = 0
@@ -577,11 +595,14 @@ def summarize_trace(path, rotations, steps, anchor="k_rot_write"):
             rows += list(csv.DictReader(fh))
     if not rows:
         raise SystemExit(f"no kernel_trace.csv under {path}")
+    dm = _demangle([r["Kernel_Name"] for r in rows])
     recs = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
-                   _short_name(r["Kernel_Name"])) for r in rows)
+                   _short_name(dm.get(r["Kernel_Name"], r["Kernel_Name"]))) for r in rows)
     idx = [k for k, r in enumerate(recs) if r[2].split("<")[0] == anchor]
     if len(idx) < rotations:
-        raise SystemExit(f"only {len(idx)} {anchor} launches in the trace, window has {rotations}")
+        names = sorted({r[2] for r in recs})[:20]
+        raise SystemExit(f"only {len(idx)} {anchor} launches in the trace, window has "
+                         f"{rotations}; kernel names seen: {names}")
     first = idx[len(idx) - rotations]
     prev_end = recs[idx[len(idx) - rotations - 1]][1] if len(idx) > rotations else -1
     win = [r for r in recs[:first] if r[0] > prev_end] + recs[first:]
@@ -613,12 +634,20 @@ def summarize_pmc(path, rotations, copy_path=None, kernels=("k_rot_write", "k_ro
     `copy_path` holds the same counters for a plain device copy, the copy's average."""
     import csv
 
+    import functools
+
+    @functools.lru_cache(maxsize=None)
+    def dm(name):
+        if name.startswith("_Z"):
+            return _demangle([name]).get(name, name)
+        return name
+
     def load(p):
         vals = {}  # (counter, kernel) -> [(dispatch, value)]
         for f in _rocprof_csvs(p, "counter_collection.csv"):
             with open(f, newline="") as fh:
                 for r in csv.DictReader(fh):
-                    key = (r["Counter_Name"], _short_name(r["Kernel_Name"]))
+                    key = (r["Counter_Name"], _short_name(dm(r["Kernel_Name"])))
                     vals.setdefault(key, []).append((int(r["Dispatch_Id"]),
                                                      float(r["Counter_Value"])))
         return vals

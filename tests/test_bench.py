@@ -52,7 +52,8 @@ def _trace_csv(path, launches):
     with open(path, "w") as f:
         f.write('"Kind","Kernel_Name","Start_Timestamp","End_Timestamp"\n')
         for name, s, e in launches:
-            f.write(f'"KERNEL_DISPATCH","void {name}(BatchDev, PassMap const*)",{s},{e}\n')
+            f.write(f'"KERNEL_DISPATCH","void (anonymous namespace)::{name}((anonymous '
+                    f'namespace)::BatchDev, (anonymous namespace)::PassMap const*)",{s},{e}\n')
 
 
 def test_trace_summary_keeps_only_the_window(tmp_path):
