@@ -1,25 +1,40 @@
 /* TPGPRVCP version-2 checkpoints for the engine's maps (C17).
  *
- * The record layout, its invariants and the commit protocol follow the reference's checkpoint
- * format (engine/src/checkpoint_rev.c:121-604) so that records are interchangeable:
- *   "TPGPRVCP", u64 version 2, then n_spins, len, term_count, step, total_steps, dt,
- *   residual_depth + 1, initial_global, model[32], j2, linear_size, resumed_from_step,
- *   prior_total_steps, prior_max_terms, stop_reason, last_status, threshold, max_terms, threads,
- *   backend, validated, max_drop_l2, the ledger (4 doubles, 3 counts, enclosed flag), pf_bound,
- *   residual_bound, max_fid_bound, the terms (i, j, qi, qj, h, layer), the map (W key words and
- *   the value per term); all little-endian 64-bit; then the SHA-256 of everything before.
+ * The on-disk record is the reference's restart format (engine/include/checkpoint_rev.h:7-40,
+ * written by engine/src/checkpoint_rev.c), kept byte for byte so that either program resumes
+ * the other's runs: the 8-byte magic "TPGPRVCP", version 2, a fixed header of 64-bit
+ * little-endian fields, the Hamiltonian terms, the map (key words, then the value, per term),
+ * and the SHA-256 of everything before it.  A companion "<path>.manifest.json" carries the
+ * payload hash and run metadata.
+ *
+ * The code is organised around one description of the record: `HEADER` and `TERM` list every
+ * serialised field with its C location, its encoding, the largest value its encoding admits
+ * and whether it belongs to the run configuration.  The same tables drive the encoder, the
+ * decoder (with its range checks) and the configuration comparison; the semantic invariants
+ * of a record are checked by one function on save and on load.  Reading maps the file, checks
+ * its length and checksum before anything is allocated, then decodes straight from the
+ * mapping.  Every file is published by write-to-temporary, fsync, rename and a directory
+ * fsync.
  */
 #define _POSIX_C_SOURCE 200809L
 #include "../../include/tpgp_engine.h"
 
+#include <errno.h>
 #include <fcntl.h>
 #include <float.h>
 #include <math.h>
+#include <stdarg.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
+
+_Static_assert(FLT_RADIX == 2 && DBL_MANT_DIG == 53 && sizeof(double) == 8,
+               "the record stores IEEE binary64 values");
+_Static_assert(sizeof(size_t) == 8, "the record stores sizes as 64-bit words");
 
 /* ---- SHA-256 (FIPS 180-4) ---------------------------------------------------------------- */
 typedef struct {
@@ -107,79 +122,189 @@ void tpgp_payload_sha256(const void *data, size_t bytes, unsigned char digest[32
     sha_finish(&c, digest);
 }
 
-/* ---- record invariants (checkpoint_rev.c:121-206) ----------------------------------------- */
-static void set_error(char *e, size_t cap, const char *m) {
-    if (e && cap) snprintf(e, cap, "%s", m);
+/* ---- the record's description ------------------------------------------------------------ */
+typedef enum {
+    AS_SIZE,   /* size_t, stored as is */
+    AS_INT,    /* int (or an int-sized enum), stored as is; negative values are invalid */
+    AS_UINT,   /* unsigned */
+    AS_DEPTH,  /* int >= -1, stored plus one */
+    AS_F64,    /* double, stored as its bit pattern */
+    AS_TEXT32  /* char[32], stored as 32 raw bytes */
+} encoding;
+enum { CONFIG = 1 }; /* the field is part of the run configuration */
+typedef struct {
+    encoding as;
+    size_t at;        /* offset in the owning struct */
+    uint64_t top;     /* largest admissible stored value (integer encodings) */
+    unsigned flags;
+} field;
+
+#define AT(member) offsetof(tpgp_checkpoint, member)
+#define ANY UINT64_MAX
+/* Header fields, in record order after the magic and the version. */
+static const field HEADER[] = {
+    {AS_SIZE, AT(map.n_spins), ANY, CONFIG},
+    {AS_SIZE, AT(map.len), ANY, 0},
+    {AS_SIZE, AT(term_count), ANY, CONFIG},
+    {AS_SIZE, AT(step), ANY, 0},
+    {AS_SIZE, AT(total_steps), ANY, CONFIG},
+    {AS_F64, AT(dt), 0, CONFIG},
+    {AS_DEPTH, AT(residual_depth), 11, CONFIG},
+    {AS_INT, AT(initial_global), 1, CONFIG},
+    {AS_TEXT32, AT(model), 0, CONFIG},
+    {AS_F64, AT(j2), 0, CONFIG},
+    {AS_SIZE, AT(linear_size), ANY, CONFIG},
+    {AS_SIZE, AT(resumed_from_step), ANY, 0},
+    {AS_SIZE, AT(prior_total_steps), ANY, 0},
+    {AS_SIZE, AT(prior_max_terms), ANY, 0},
+    {AS_INT, AT(stop_reason), 6, 0},
+    {AS_INT, AT(last_status), TPGP_EBUDGET, 0},
+    {AS_F64, AT(options.threshold), 0, CONFIG},
+    {AS_SIZE, AT(options.max_terms), ANY, CONFIG},
+    {AS_UINT, AT(options.threads), 65536, CONFIG},
+    {AS_INT, AT(options.backend), 4, CONFIG},
+    {AS_INT, AT(options.validated), 1, CONFIG},
+    {AS_F64, AT(options.max_drop_l2), 0, CONFIG},
+    {AS_F64, AT(ledger.dropped_l1), 0, 0},
+    {AS_F64, AT(ledger.dropped_l2), 0, 0},
+    {AS_F64, AT(ledger.dropped_l2_squared), 0, 0},
+    {AS_F64, AT(ledger.arithmetic_l2), 0, 0},
+    {AS_SIZE, AT(ledger.rotations), ANY, 0},
+    {AS_SIZE, AT(ledger.removal_events), ANY, 0},
+    {AS_SIZE, AT(ledger.peak_terms), ANY, 0},
+    {AS_INT, AT(ledger.arithmetic_enclosed), 1, 0},
+    {AS_F64, AT(pf_bound), 0, 0},
+    {AS_F64, AT(residual_bound), 0, 0},
+    {AS_F64, AT(max_fid_bound), 0, 0},
+};
+#undef AT
+#define AT(member) offsetof(tpgp_term, member)
+/* One Hamiltonian term; every field is configuration. */
+static const field TERM[] = {
+    {AS_SIZE, AT(i), ANY, CONFIG},   {AS_SIZE, AT(j), ANY, CONFIG},
+    {AS_UINT, AT(qi), 3, CONFIG},    {AS_UINT, AT(qj), 3, CONFIG},
+    {AS_F64, AT(h), 0, CONFIG},      {AS_SIZE, AT(layer), ANY, CONFIG},
+};
+#undef AT
+#define COUNT(t) (sizeof(t) / sizeof((t)[0]))
+static const char MAGIC[8] = {'T', 'P', 'G', 'P', 'R', 'V', 'C', 'P'};
+enum { VERSION = 2, DIGEST = 32 };
+
+static size_t encoded_size(const field *f) { return f->as == AS_TEXT32 ? 32 : 8; }
+static size_t table_size(const field *t, size_t n) {
+    size_t bytes = 0;
+    for (size_t k = 0; k < n; k++) bytes += encoded_size(t + k);
+    return bytes;
 }
-static int mul_ok(size_t a, size_t b, size_t *out) {
-    if (b && a > SIZE_MAX / b) return 0;
-    *out = a * b;
+/* The stored 64-bit word of an integer or floating field. */
+static uint64_t stored_word(const field *f, const void *base) {
+    const char *p = (const char *)base + f->at;
+    switch (f->as) {
+    case AS_SIZE: return *(const size_t *)p;
+    case AS_INT: return (uint64_t)(int64_t) * (const int *)p;
+    case AS_UINT: return *(const unsigned *)p;
+    case AS_DEPTH: return (uint64_t)((int64_t) * (const int *)p + 1);
+    case AS_F64: {
+        uint64_t u;
+        memcpy(&u, p, 8);
+        return u;
+    }
+    default: return 0;
+    }
+}
+/* Store a decoded word; 0 when it is outside what the field admits. */
+static int load_word(const field *f, void *base, uint64_t w) {
+    char *p = (char *)base + f->at;
+    if (f->as != AS_F64 && w > f->top) return 0;
+    switch (f->as) {
+    case AS_SIZE: *(size_t *)p = (size_t)w; break;
+    case AS_INT: *(int *)p = (int)w; break;
+    case AS_UINT: *(unsigned *)p = (unsigned)w; break;
+    case AS_DEPTH: *(int *)p = (int)w - 1; break;
+    case AS_F64: memcpy(p, &w, 8); break;
+    default: return 0;
+    }
     return 1;
 }
-static int bits_equal(double a, double b) { return memcmp(&a, &b, sizeof(double)) == 0; }
 
-static int valid_record(const tpgp_checkpoint *s) {
-    if (!s || FLT_RADIX != 2 || DBL_MANT_DIG != 53 || !s->map.n_spins ||
-        s->map.n_spins > SIZE_MAX - 31 || s->map.n_words != (s->map.n_spins + 31) / 32 ||
-        s->map.len > s->map.capacity || (s->map.len && (!s->map.keys || !s->map.values)) ||
+/* ---- semantic invariants ------------------------------------------------------------------
+ * One function for save and load: NULL when the record is well formed, else the first broken
+ * rule.  The rules are the format's (checkpoint_rev.c:121-197): geometry, run state, options,
+ * ledger, then the map's canonical order and the terms. */
+static int mul_fits(size_t a, size_t b) { return b == 0 || a <= SIZE_MAX / b; }
+static int nonneg(double x) { return !isnan(x) && x >= 0; }
+static int finite_nonneg(double x) { return isfinite(x) && x >= 0; }
+
+static const char *broken_rule(const tpgp_checkpoint *s) {
+    if (!s) return "no record";
+    const tpgp_map *m = &s->map;
+    /* geometry and storage */
+    if (!m->n_spins || m->n_spins > SIZE_MAX - 31 || m->n_words != (m->n_spins + 31) / 32)
+        return "spin count and key width disagree";
+    if (m->len > m->capacity || (m->len && (!m->keys || !m->values)) ||
         (s->term_count && !s->terms))
-        return 0;
-    if (!isfinite(s->dt) || s->dt <= 0 || s->step > s->total_steps || s->residual_depth < -1 ||
-        s->residual_depth > 10 || (s->initial_global != 0 && s->initial_global != 1) ||
-        !memchr(s->model, 0, sizeof(s->model)) || !s->model[0] || !isfinite(s->j2) ||
-        s->resumed_from_step > s->step || s->stop_reason < 0 || s->stop_reason > 6 ||
-        s->last_status < TPGP_OK || s->last_status > TPGP_EBUDGET)
-        return 0;
-    size_t t;
-    if (!mul_ok(s->map.len, s->map.n_words, &t) || !mul_ok(t, 8, &t) ||
-        !mul_ok(s->map.len, 8, &t) || !mul_ok(s->term_count, sizeof(tpgp_term), &t) ||
-        !isfinite(s->max_fid_bound) || s->max_fid_bound < 0 ||
-        (s->max_fid_bound > 0 && !s->options.validated && s->residual_depth < 0) ||
-        (s->stop_reason == 1 && s->step != s->total_steps))
-        return 0;
-    for (const char *c = s->model; *c; c++)
-        if (!((*c >= 'a' && *c <= 'z') || (*c >= '0' && *c <= '9') || *c == '_')) return 0;
+        return "map or term storage missing";
+    if (!mul_fits(m->len, m->n_words) || !mul_fits(m->len * m->n_words, 8) ||
+        !mul_fits(m->len, sizeof(double)) || !mul_fits(s->term_count, sizeof(tpgp_term)))
+        return "record sizes overflow";
+    /* run state */
+    if (!isfinite(s->dt) || !(s->dt > 0)) return "time step must be finite and positive";
+    if (s->step > s->total_steps || s->resumed_from_step > s->step)
+        return "step counters out of order";
+    if (s->stop_reason == 1 && s->step != s->total_steps)
+        return "a completed run must stand at its last step";
+    if (s->residual_depth < -1 || s->residual_depth > 10) return "residual depth out of range";
+    if (s->initial_global != 0 && s->initial_global != 1) return "initial-state flag not 0/1";
+    if (s->stop_reason < 0 || s->stop_reason > 6) return "unknown stop reason";
+    if ((int)s->last_status < TPGP_OK || (int)s->last_status > TPGP_EBUDGET)
+        return "unknown status";
+    if (!isfinite(s->j2)) return "J2 must be finite";
+    const char *end = memchr(s->model, 0, sizeof(s->model));
+    if (!end || end == s->model) return "model name empty or unterminated";
+    for (const char *c = s->model; c < end; c++)
+        if (!((*c >= 'a' && *c <= 'z') || (*c >= '0' && *c <= '9') || *c == '_'))
+            return "model name has characters outside [a-z0-9_]";
+    /* options */
     const tpgp_options *o = &s->options;
-    if (!isfinite(o->threshold) || o->threshold < 0 || !o->threads || o->threads > 65536 ||
-        o->backend < 0 || o->backend > 4 || (o->validated != 0 && o->validated != 1) ||
-        !isfinite(o->max_drop_l2) || o->max_drop_l2 < 0 ||
-        (o->max_terms && s->map.len > o->max_terms))
-        return 0;
-    const double nonneg[5] = {s->ledger.dropped_l1, s->ledger.dropped_l2,
-                              s->ledger.dropped_l2_squared, s->ledger.arithmetic_l2, s->pf_bound};
-    for (int k = 0; k < 5; k++)
-        if (isnan(nonneg[k]) || nonneg[k] < 0) return 0;
-    if ((s->ledger.arithmetic_enclosed != 0 && s->ledger.arithmetic_enclosed != 1) ||
-        (s->ledger.arithmetic_enclosed && !o->validated) || s->ledger.peak_terms < s->map.len)
-        return 0;
-    if (s->residual_depth >= 0 ? (isnan(s->residual_bound) || s->residual_bound < 0)
-                               : !isnan(s->residual_bound))
-        return 0;
-    const size_t W = s->map.n_words, rem = s->map.n_spins % 32;
-    for (size_t k = 0; k < s->map.len; k++) {  /* finite nonzero values, zero padding, order */
-        const double v = s->map.values[k];
-        if (!isfinite(v) || v == 0) return 0;
-        if (rem && (s->map.keys[(k + 1) * W - 1] >> (2 * rem))) return 0;
+    if (!finite_nonneg(o->threshold) || !finite_nonneg(o->max_drop_l2))
+        return "threshold and drop budget must be finite and non-negative";
+    if (o->threads < 1 || o->threads > 65536) return "thread count out of range";
+    if (o->backend < 0 || o->backend > 4 || (o->validated != 0 && o->validated != 1))
+        return "backend or validated flag out of range";
+    if (o->max_terms && m->len > o->max_terms) return "map exceeds its term cap";
+    /* ledger and bounds */
+    const tpgp_ledger *l = &s->ledger;
+    if (!nonneg(l->dropped_l1) || !nonneg(l->dropped_l2) || !nonneg(l->dropped_l2_squared) ||
+        !nonneg(l->arithmetic_l2) || !nonneg(s->pf_bound))
+        return "ledger sums must be non-negative";
+    if (l->arithmetic_enclosed != 0 && l->arithmetic_enclosed != 1) return "enclosed flag not 0/1";
+    if (l->arithmetic_enclosed && !o->validated) return "arithmetic enclosure without validation";
+    if (l->peak_terms < m->len) return "peak term count below the map's length";
+    if (!finite_nonneg(s->max_fid_bound)) return "FID accuracy limit must be finite";
+    if (s->max_fid_bound > 0 && !o->validated && s->residual_depth < 0)
+        return "an accuracy limit needs validation or a residual bound";
+    if (s->residual_depth >= 0 ? !nonneg(s->residual_bound) : !isnan(s->residual_bound))
+        return "residual bound inconsistent with the residual depth";
+    /* the map: finite nonzero values, zero key padding, strictly ascending keys */
+    const size_t W = m->n_words, pad = m->n_spins % 32;
+    for (size_t k = 0; k < m->len; k++) {
+        if (!isfinite(m->values[k]) || m->values[k] == 0) return "map value zero or not finite";
+        const uint64_t *key = m->keys + k * W;
+        if (pad && key[W - 1] >> (2 * pad)) return "key padding bits set";
         if (k) {
-            int less = 0;
-            for (size_t w = W; w-- > 0;) {
-                const uint64_t a = s->map.keys[(k - 1) * W + w], b = s->map.keys[k * W + w];
-                if (a != b) {
-                    less = a < b;
-                    break;
-                }
-                if (w == 0) return 0; /* duplicate key */
-            }
-            if (!less) return 0;
+            const uint64_t *prev = key - W;
+            size_t w = W;
+            while (w > 0 && prev[w - 1] == key[w - 1]) w--;
+            if (w == 0 || prev[w - 1] > key[w - 1]) return "map keys not strictly ascending";
         }
     }
     for (size_t k = 0; k < s->term_count; k++) {
-        const tpgp_term *q = s->terms + k;
-        if (q->i >= s->map.n_spins || q->j >= s->map.n_spins || q->i == q->j || q->qi > 3 ||
-            q->qj > 3 || !isfinite(q->h))
-            return 0;
+        const tpgp_term *t = s->terms + k;
+        if (t->i >= m->n_spins || t->j >= m->n_spins || t->i == t->j || t->qi > 3 ||
+            t->qj > 3 || !isfinite(t->h))
+            return "invalid Hamiltonian term";
     }
-    return 1;
+    return NULL;
 }
 
 void tpgp_checkpoint_free(tpgp_checkpoint *s) {
@@ -189,392 +314,328 @@ void tpgp_checkpoint_free(tpgp_checkpoint *s) {
     memset(s, 0, sizeof(*s));
 }
 
-/* ---- writing: exclusive temporary file, fsync, atomic rename, parent fsync -------------- */
+static tpgp_status fail(char *error, size_t cap, tpgp_status st, const char *what,
+                        const char *detail) {
+    if (error && cap) {
+        if (detail) snprintf(error, cap, "%s: %s", what, detail);
+        else snprintf(error, cap, "%s", what);
+    }
+    return st;
+}
+
+/* ---- durable files --------------------------------------------------------------------------
+ * A sink buffers bytes for a file descriptor and hashes what it is told to.  durable_write()
+ * produces a file's whole content into a fresh temporary beside the target (created with
+ * O_EXCL), forces it to disk, renames it over the target and forces the directory entry, so a
+ * reader sees the old file or the new one, never a mixture. */
 typedef struct {
-    FILE *f;
+    int fd;
+    int failed;
+    size_t used;
     sha_ctx sha;
-    int ok;
-} out_stream;
-static void out_bytes(out_stream *o, const void *p, size_t n) {
-    if (!o->ok) return;
-    if (fwrite(p, 1, n, o->f) != n) {
-        o->ok = 0;
-        return;
-    }
-    sha_feed(&o->sha, p, n);
-}
-static void out_u64(out_stream *o, uint64_t x) {
-    unsigned char b[8];
-    for (int i = 0; i < 8; i++) b[i] = (unsigned char)(x >> (8 * i));
-    out_bytes(o, b, 8);
-}
-static void out_f64(out_stream *o, double x) {
-    uint64_t u;
-    memcpy(&u, &x, 8);
-    out_u64(o, u);
-}
-static FILE *open_temporary(const char *path, char **tmp_name) {
-    const size_t n = strlen(path) + 16;
-    *tmp_name = malloc(n);
-    if (!*tmp_name) return NULL;
-    snprintf(*tmp_name, n, "%s.tmp.XXXXXX", path);
-    const int fd = mkstemp(*tmp_name);
-    FILE *f = fd >= 0 ? fdopen(fd, "wb") : NULL;
-    if (!f) {
-        if (fd >= 0) {
-            close(fd);
-            unlink(*tmp_name);
+    unsigned char buf[1 << 16];
+} sink;
+static void sink_flush(sink *k) {
+    const unsigned char *p = k->buf;
+    while (!k->failed && k->used) {
+        ssize_t n = write(k->fd, p, k->used);
+        if (n < 0 && errno == EINTR) continue;
+        if (n <= 0) {
+            k->failed = 1;
+            break;
         }
-        free(*tmp_name);
-        *tmp_name = NULL;
+        p += n;
+        k->used -= (size_t)n;
     }
-    return f;
+    k->used = 0;
 }
-static int publish(FILE *f, const char *tmp_name, const char *path) {
-    int ok = fflush(f) == 0 && !ferror(f) && fsync(fileno(f)) == 0;
-    ok = (fclose(f) == 0) && ok;
-    ok = ok && rename(tmp_name, path) == 0;
-    if (!ok) {
-        unlink(tmp_name);
-        return 0;
+static void sink_put(sink *k, const void *data, size_t n, int hashed) {
+    const unsigned char *p = data;
+    if (hashed) sha_feed(&k->sha, p, n);
+    while (n && !k->failed) {
+        size_t room = sizeof(k->buf) - k->used, take = n < room ? n : room;
+        memcpy(k->buf + k->used, p, take);
+        k->used += take;
+        p += take;
+        n -= take;
+        if (k->used == sizeof(k->buf)) sink_flush(k);
     }
-    char *dir = strdup(path);
+}
+static void sink_word(sink *k, uint64_t w) {
+    unsigned char b[8];
+    for (int i = 0; i < 8; i++) b[i] = (unsigned char)(w >> (8 * i));
+    sink_put(k, b, 8, 1);
+}
+static void sink_fields(sink *k, const field *t, size_t n, const void *base) {
+    for (size_t i = 0; i < n; i++) {
+        if (t[i].as == AS_TEXT32) sink_put(k, (const char *)base + t[i].at, 32, 1);
+        else sink_word(k, stored_word(t + i, base));
+    }
+}
+
+typedef void (*producer)(sink *k, const void *arg);
+static int sync_directory_of(const char *path) {
+    const char *slash = strrchr(path, '/');
+    char *dir;
+    if (!slash) dir = strdup(".");
+    else if (slash == path) dir = strdup("/");
+    else {
+        dir = malloc((size_t)(slash - path) + 1);
+        if (dir) {
+            memcpy(dir, path, (size_t)(slash - path));
+            dir[slash - path] = 0;
+        }
+    }
     if (!dir) return 0;
-    char *slash = strrchr(dir, '/');
-    if (!slash)
-        strcpy(dir, ".");
-    else if (slash == dir)
-        slash[1] = 0;
-    else
-        *slash = 0;
-    const int fd = open(dir, O_RDONLY);
+    int fd = open(dir, O_RDONLY);
     free(dir);
     if (fd < 0) return 0;
-    ok = fsync(fd) == 0;
+    int ok = fsync(fd) == 0;
     close(fd);
     return ok;
+}
+/* 0 on success, else a static message. */
+static const char *durable_write(const char *path, producer make, const void *arg) {
+    const size_t n = strlen(path);
+    char *tmp = malloc(n + 16);
+    sink *k = malloc(sizeof(sink));
+    const char *why = NULL;
+    if (!tmp || !k) {
+        free(tmp);
+        free(k);
+        return "out of memory";
+    }
+    memcpy(tmp, path, n);
+    memcpy(tmp + n, ".tmp.XXXXXX", 12);
+    k->fd = mkstemp(tmp);
+    if (k->fd < 0) {
+        free(tmp);
+        free(k);
+        return "cannot create a temporary file beside the target";
+    }
+    k->failed = 0;
+    k->used = 0;
+    sha_start(&k->sha);
+    make(k, arg);
+    sink_flush(k);
+    if (k->failed) why = "write failed; the previous file is untouched";
+    else if (fsync(k->fd)) why = "fsync of the new file failed";
+    if (close(k->fd) && !why) why = "close of the new file failed";
+    if (!why && rename(tmp, path)) why = "rename over the target failed";
+    if (why) unlink(tmp);
+    else if (!sync_directory_of(path)) why = "fsync of the containing directory failed";
+    free(tmp);
+    free(k);
+    return why;
+}
+
+/* ---- save ------------------------------------------------------------------------------------ */
+typedef struct {
+    const tpgp_checkpoint *s;
+    unsigned char digest[DIGEST];
+} record_job;
+static void make_record(sink *k, const void *arg) {
+    record_job *job = (record_job *)arg;
+    const tpgp_checkpoint *s = job->s;
+    sink_put(k, MAGIC, sizeof(MAGIC), 1);
+    sink_word(k, VERSION);
+    sink_fields(k, HEADER, COUNT(HEADER), s);
+    for (size_t t = 0; t < s->term_count; t++) sink_fields(k, TERM, COUNT(TERM), s->terms + t);
+    const size_t W = s->map.n_words;
+    for (size_t e = 0; e < s->map.len; e++) {
+        for (size_t w = 0; w < W; w++) sink_word(k, s->map.keys[e * W + w]);
+        uint64_t v;
+        memcpy(&v, s->map.values + e, 8);
+        sink_word(k, v);
+    }
+    sha_finish(&k->sha, job->digest);
+    sink_put(k, job->digest, DIGEST, 0);
+}
+
+/* The manifest: one "key": value line per entry, in the reference's key order
+ * (checkpoint_rev.c:360-404).  Numbers in decimal, doubles as C99 hex floats, strings quoted. */
+static void put_text(sink *k, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static void put_text(sink *k, const char *fmt, ...) {
+    char line[256];
+    va_list ap;
+    va_start(ap, fmt);
+    int n = vsnprintf(line, sizeof(line), fmt, ap);
+    va_end(ap);
+    if (n < 0 || (size_t)n >= sizeof(line)) {
+        k->failed = 1;
+        return;
+    }
+    sink_put(k, line, (size_t)n, 0);
+}
+static void make_manifest(sink *k, const void *arg) {
+    const record_job *job = arg;
+    const tpgp_checkpoint *s = job->s;
+    static const char *const reason[7] = {"running",          "complete",
+                                          "walltime",         "signal",
+                                          "resource_failure", "numerical_failure",
+                                          "accuracy_limit"};
+    char hex[2 * DIGEST + 1];
+    for (int b = 0; b < DIGEST; b++) snprintf(hex + 2 * b, 3, "%02x", job->digest[b]);
+    put_text(k, "{\n  \"format\": \"TPGPRVCP-2\",\n  \"sha256_payload\": \"%s\",\n", hex);
+    put_text(k, "  \"model\": \"%s\",\n  \"n_spins\": %zu,\n", s->model, s->map.n_spins);
+    put_text(k, "  \"active_terms\": %zu,\n  \"hamiltonian_terms\": %zu,\n", s->map.len,
+             s->term_count);
+    put_text(k, "  \"step\": %zu,\n  \"total_steps\": %zu,\n", s->step, s->total_steps);
+    put_text(k, "  \"dt_hex\": \"%a\",\n  \"backend\": %d,\n", s->dt, s->options.backend);
+    put_text(k, "  \"threads\": %u,\n  \"validated\": %d,\n", s->options.threads,
+             s->options.validated);
+    put_text(k, "  \"threshold_hex\": \"%a\",\n  \"max_terms\": %zu,\n", s->options.threshold,
+             s->options.max_terms);
+    put_text(k, "  \"residual_depth\": %d,\n  \"max_fid_bound_hex\": \"%a\",\n",
+             s->residual_depth, s->max_fid_bound);
+    put_text(k, "  \"resumed_from_step\": %zu,\n  \"prior_total_steps\": %zu,\n",
+             s->resumed_from_step, s->prior_total_steps);
+    put_text(k, "  \"prior_max_terms\": %zu,\n  \"stop_reason\": \"%s\",\n", s->prior_max_terms,
+             reason[s->stop_reason]);
+    put_text(k, "  \"status_code\": %d,\n  \"status\": \"%s\"\n}\n", (int)s->last_status,
+             tpgp_status_string(s->last_status));
 }
 
 tpgp_status tpgp_checkpoint_save(const char *path, const tpgp_checkpoint *s, char *error,
                                  size_t cap) {
-    if (!path || !*path || !valid_record(s)) {
-        set_error(error, cap, "invalid checkpoint state");
-        return TPGP_EINVAL;
-    }
-    char *tmp = NULL;
-    FILE *f = open_temporary(path, &tmp);
-    if (!f) {
-        set_error(error, cap, "cannot create checkpoint temporary file");
-        return TPGP_EINVAL;
-    }
-    out_stream o = {f, {{0}}, 1};
-    sha_start(&o.sha);
-    out_bytes(&o, "TPGPRVCP", 8);
-    const uint64_t head[5] = {2, s->map.n_spins, s->map.len, s->term_count, s->step};
-    for (int k = 0; k < 5; k++) out_u64(&o, head[k]);
-    out_u64(&o, s->total_steps);
-    out_f64(&o, s->dt);
-    out_u64(&o, (uint64_t)(s->residual_depth + 1));
-    out_u64(&o, (uint64_t)s->initial_global);
-    out_bytes(&o, s->model, 32);
-    out_f64(&o, s->j2);
-    out_u64(&o, s->linear_size);
-    out_u64(&o, s->resumed_from_step);
-    out_u64(&o, s->prior_total_steps);
-    out_u64(&o, s->prior_max_terms);
-    out_u64(&o, (uint64_t)s->stop_reason);
-    out_u64(&o, (uint64_t)s->last_status);
-    out_f64(&o, s->options.threshold);
-    out_u64(&o, s->options.max_terms);
-    out_u64(&o, s->options.threads);
-    out_u64(&o, (uint64_t)s->options.backend);
-    out_u64(&o, (uint64_t)s->options.validated);
-    out_f64(&o, s->options.max_drop_l2);
-    out_f64(&o, s->ledger.dropped_l1);
-    out_f64(&o, s->ledger.dropped_l2);
-    out_f64(&o, s->ledger.dropped_l2_squared);
-    out_f64(&o, s->ledger.arithmetic_l2);
-    out_u64(&o, s->ledger.rotations);
-    out_u64(&o, s->ledger.removal_events);
-    out_u64(&o, s->ledger.peak_terms);
-    out_u64(&o, (uint64_t)s->ledger.arithmetic_enclosed);
-    out_f64(&o, s->pf_bound);
-    out_f64(&o, s->residual_bound);
-    out_f64(&o, s->max_fid_bound);
-    for (size_t k = 0; k < s->term_count; k++) {
-        const tpgp_term *q = s->terms + k;
-        out_u64(&o, q->i);
-        out_u64(&o, q->j);
-        out_u64(&o, q->qi);
-        out_u64(&o, q->qj);
-        out_f64(&o, q->h);
-        out_u64(&o, q->layer);
-    }
-    const size_t W = s->map.n_words;
-    for (size_t k = 0; k < s->map.len; k++) {
-        for (size_t w = 0; w < W; w++) out_u64(&o, s->map.keys[k * W + w]);
-        out_f64(&o, s->map.values[k]);
-    }
-    unsigned char digest[32];
-    sha_finish(&o.sha, digest);
-    if (o.ok && fwrite(digest, 1, 32, f) != 32) o.ok = 0;
-    if (!o.ok) {
-        fclose(f);
-        unlink(tmp);
-        free(tmp);
-        set_error(error, cap, "checkpoint write failed; old record preserved");
-        return TPGP_EINVAL;
-    }
-    if (!publish(f, tmp, path)) {
-        free(tmp);
-        set_error(error, cap, "checkpoint atomic commit or directory synchronization failed");
-        return TPGP_EINVAL;
-    }
-    free(tmp);
-    /* the manifest: payload hash and run metadata (checkpoint_rev.c:360-404) */
-    const size_t n = strlen(path) + 20;
-    char *mpath = malloc(n);
-    if (!mpath) {
-        set_error(error, cap, "checkpoint saved but manifest allocation failed");
-        return TPGP_ENOMEM;
-    }
-    snprintf(mpath, n, "%s.manifest.json", path);
-    f = open_temporary(mpath, &tmp);
-    if (!f) {
-        free(mpath);
-        set_error(error, cap, "checkpoint saved but manifest creation failed");
-        return TPGP_EINVAL;
-    }
-    char hex[65], dt_hex[40], thr_hex[40], bound_hex[40];
-    for (int k = 0; k < 32; k++) snprintf(hex + 2 * k, 3, "%02x", digest[k]);
-    snprintf(dt_hex, sizeof(dt_hex), "%a", s->dt);
-    snprintf(thr_hex, sizeof(thr_hex), "%a", s->options.threshold);
-    snprintf(bound_hex, sizeof(bound_hex), "%a", s->max_fid_bound);
-    static const char *reason[7] = {"running",          "complete",          "walltime", "signal",
-                                    "resource_failure", "numerical_failure", "accuracy_limit"};
-    /* one "key": value line per field; strings quoted, numbers in decimal */
-    char num[11][24];
-    const unsigned long long counts[10] = {
-        s->map.n_spins,        s->map.len,           s->term_count,      s->step,
-        s->total_steps,        s->options.max_terms, s->resumed_from_step,
-        s->prior_total_steps,  s->prior_max_terms,   s->options.threads};
-    for (int k = 0; k < 10; k++) snprintf(num[k], sizeof(num[k]), "%llu", counts[k]);
-    snprintf(num[10], sizeof(num[10]), "%d", s->residual_depth);
-    char small[3][8];
-    snprintf(small[0], sizeof(small[0]), "%d", s->options.backend);
-    snprintf(small[1], sizeof(small[1]), "%d", s->options.validated);
-    snprintf(small[2], sizeof(small[2]), "%d", (int)s->last_status);
-    const struct {
-        const char *key, *value;
-        int quoted;
-    } rows[] = {{"format", "TPGPRVCP-2", 1},       {"sha256_payload", hex, 1},
-                {"model", s->model, 1},            {"n_spins", num[0], 0},
-                {"active_terms", num[1], 0},       {"hamiltonian_terms", num[2], 0},
-                {"step", num[3], 0},               {"total_steps", num[4], 0},
-                {"dt_hex", dt_hex, 1},             {"backend", small[0], 0},
-                {"threads", num[9], 0},            {"validated", small[1], 0},
-                {"threshold_hex", thr_hex, 1},     {"max_terms", num[5], 0},
-                {"residual_depth", num[10], 0},    {"max_fid_bound_hex", bound_hex, 1},
-                {"resumed_from_step", num[6], 0},  {"prior_total_steps", num[7], 0},
-                {"prior_max_terms", num[8], 0},    {"stop_reason", reason[s->stop_reason], 1},
-                {"status_code", small[2], 0},      {"status", tpgp_status_string(s->last_status), 1}};
-    const size_t n_rows = sizeof(rows) / sizeof(rows[0]);
-    fputs("{\n", f);
-    for (size_t k = 0; k < n_rows; k++)
-        fprintf(f, "  \"%s\": %s%s%s%s\n", rows[k].key, rows[k].quoted ? "\"" : "", rows[k].value,
-                rows[k].quoted ? "\"" : "", k + 1 < n_rows ? "," : "");
-    fputs("}\n", f);
-    const int ok = publish(f, tmp, mpath);
-    free(tmp);
-    free(mpath);
-    if (!ok) {
-        set_error(error, cap, "checkpoint saved but manifest commit failed");
-        return TPGP_EINVAL;
-    }
-    set_error(error, cap, "ok");
-    return TPGP_OK;
+    if (!path || !*path) return fail(error, cap, TPGP_EINVAL, "checkpoint save", "no path");
+    const char *bad = broken_rule(s);
+    if (bad) return fail(error, cap, TPGP_EINVAL, "checkpoint refused", bad);
+    record_job job = {s, {0}};
+    const char *why = durable_write(path, make_record, &job);
+    if (why) return fail(error, cap, TPGP_EINVAL, "checkpoint not written", why);
+    const size_t n = strlen(path);
+    char *manifest = malloc(n + sizeof(".manifest.json"));
+    if (!manifest)
+        return fail(error, cap, TPGP_ENOMEM, "checkpoint written, manifest not", "out of memory");
+    memcpy(manifest, path, n);
+    memcpy(manifest + n, ".manifest.json", sizeof(".manifest.json"));
+    why = durable_write(manifest, make_manifest, &job);
+    free(manifest);
+    if (why) return fail(error, cap, TPGP_EINVAL, "checkpoint written, manifest not", why);
+    return fail(error, cap, TPGP_OK, "ok", NULL);
 }
 
-/* ---- reading: whole-file checksum first, then a strict parse ------------------------------ */
+/* ---- load ------------------------------------------------------------------------------------ */
 typedef struct {
-    FILE *f;
-    uint64_t left;
-    int ok;
-} in_stream;
-static void in_bytes(in_stream *r, void *p, size_t n) {
-    if (!r->ok || n > r->left || fread(p, 1, n, r->f) != n) {
-        r->ok = 0;
-        memset(p, 0, n);
-        return;
+    const unsigned char *p, *end;
+} cursor;
+static uint64_t take_word(cursor *c) {
+    uint64_t w = 0;
+    for (int i = 0; i < 8; i++) w |= (uint64_t)c->p[i] << (8 * i);
+    c->p += 8;
+    return w;
+}
+/* Decode a table's fields into base; 0 when a stored value is out of range.  The caller has
+ * checked that the bytes are there. */
+static int take_fields(cursor *c, const field *t, size_t n, void *base) {
+    int ok = 1;
+    for (size_t i = 0; i < n; i++) {
+        if (t[i].as == AS_TEXT32) {
+            memcpy((char *)base + t[i].at, c->p, 32);
+            c->p += 32;
+        } else {
+            ok &= load_word(t + i, base, take_word(c));
+        }
     }
-    r->left -= n;
+    return ok;
 }
-static uint64_t in_u64(in_stream *r) {
-    unsigned char b[8];
-    in_bytes(r, b, 8);
-    uint64_t x = 0;
-    for (int i = 0; i < 8; i++) x |= (uint64_t)b[i] << (8 * i);
-    return x;
-}
-static size_t in_size(in_stream *r) {
-    const uint64_t x = in_u64(r);
-    if (x > SIZE_MAX) r->ok = 0;
-    return (size_t)x;
-}
-static double in_f64(in_stream *r) {
-    const uint64_t u = in_u64(r);
-    double d;
-    memcpy(&d, &u, 8);
-    return d;
+
+static tpgp_status decode(const unsigned char *data, size_t size, tpgp_checkpoint *s,
+                          const char **why) {
+    memset(s, 0, sizeof(*s));
+    const size_t head = sizeof(MAGIC) + 8 + table_size(HEADER, COUNT(HEADER));
+    const size_t term_bytes = table_size(TERM, COUNT(TERM));
+    if (size < head + DIGEST) return *why = "file shorter than a record header", TPGP_EINVAL;
+    const size_t payload = size - DIGEST;
+    unsigned char digest[DIGEST];
+    tpgp_payload_sha256(data, payload, digest);
+    if (memcmp(digest, data + payload, DIGEST)) return *why = "checksum mismatch", TPGP_EINVAL;
+    cursor c = {data, data + payload};
+    if (memcmp(c.p, MAGIC, sizeof(MAGIC))) return *why = "not a TPGPRVCP record", TPGP_EINVAL;
+    c.p += sizeof(MAGIC);
+    if (take_word(&c) != VERSION) return *why = "unsupported record version", TPGP_EINVAL;
+    s->options = tpgp_default_options();
+    if (!take_fields(&c, HEADER, COUNT(HEADER), s))
+        return *why = "header field out of range", TPGP_EINVAL;
+    /* the body's length follows from the header's counts */
+    const size_t n = s->map.n_spins, len = s->map.len, terms = s->term_count;
+    if (n == 0 || n > SIZE_MAX - 31) return *why = "bad spin count", TPGP_EINVAL;
+    const size_t W = (n + 31) / 32, rest = (size_t)(c.end - c.p);
+    if (!mul_fits(terms, term_bytes) || !mul_fits(len, W + 1) || !mul_fits(len * (W + 1), 8) ||
+        terms * term_bytes > rest || len * (W + 1) * 8 != rest - terms * term_bytes)
+        return *why = "record length disagrees with its header", TPGP_EINVAL;
+    const size_t count = len;
+    /* reset the map to empty storage before allocating: a failure below frees only ours */
+    if (tpgp_map_init(&s->map, n) != TPGP_OK) return *why = "bad spin count", TPGP_EINVAL;
+    if (count) {
+        s->map.keys = malloc(count * W * sizeof(uint64_t));
+        s->map.values = malloc(count * sizeof(double));
+    }
+    if (terms) s->terms = malloc(terms * sizeof(tpgp_term));
+    if ((count && (!s->map.keys || !s->map.values)) || (terms && !s->terms))
+        return *why = "out of memory", TPGP_ENOMEM;
+    s->map.len = s->map.capacity = count;
+    for (size_t t = 0; t < terms; t++) {
+        memset(s->terms + t, 0, sizeof(tpgp_term));
+        if (!take_fields(&c, TERM, COUNT(TERM), s->terms + t))
+            return *why = "term field out of range", TPGP_EINVAL;
+    }
+    for (size_t e = 0; e < count; e++) {
+        for (size_t w = 0; w < W; w++) s->map.keys[e * W + w] = take_word(&c);
+        const uint64_t v = take_word(&c);
+        memcpy(s->map.values + e, &v, 8);
+    }
+    const char *bad = broken_rule(s);
+    if (bad) return *why = bad, TPGP_EINVAL;
+    return TPGP_OK;
 }
 
 tpgp_status tpgp_checkpoint_load(const char *path, tpgp_checkpoint *dst, char *error, size_t cap) {
-    if (!path || !dst || FLT_RADIX != 2 || DBL_MANT_DIG != 53) {
-        set_error(error, cap, "invalid checkpoint load arguments");
-        return TPGP_EINVAL;
-    }
-    FILE *f = fopen(path, "rb");
-    if (!f) {
-        set_error(error, cap, "cannot open checkpoint");
-        return TPGP_EINVAL;
-    }
+    if (!path || !dst) return fail(error, cap, TPGP_EINVAL, "checkpoint load", "no path or record");
+    int fd = open(path, O_RDONLY);
+    if (fd < 0) return fail(error, cap, TPGP_EINVAL, "cannot open checkpoint", strerror(errno));
     struct stat st;
-    if (fstat(fileno(f), &st) || st.st_size < 328) {
-        fclose(f);
-        set_error(error, cap, "checkpoint is truncated or has invalid length");
-        return TPGP_EINVAL;
+    if (fstat(fd, &st) || !S_ISREG(st.st_mode) || st.st_size <= 0) {
+        close(fd);
+        return fail(error, cap, TPGP_EINVAL, "checkpoint rejected", "not a non-empty file");
     }
-    const uint64_t payload = (uint64_t)st.st_size - 32;
-    unsigned char chunk[1 << 16], want[32], got[32];
-    sha_ctx sha;
-    sha_start(&sha);
-    int good = 1;
-    for (uint64_t left = payload; left && good;) {
-        const size_t n = left > sizeof(chunk) ? sizeof(chunk) : (size_t)left;
-        good = fread(chunk, 1, n, f) == n;
-        if (good) sha_feed(&sha, chunk, n);
-        left -= n;
-    }
-    sha_finish(&sha, got);
-    if (!good || fread(want, 1, 32, f) != 32 || memcmp(got, want, 32) || fgetc(f) != EOF) {
-        fclose(f);
-        set_error(error, cap, "checkpoint checksum/length mismatch");
-        return TPGP_EINVAL;
-    }
-    rewind(f);
-    in_stream r = {f, payload, 1};
+    const size_t size = (size_t)st.st_size;
+    void *map = mmap(NULL, size, PROT_READ, MAP_PRIVATE, fd, 0);
+    close(fd);
+    if (map == MAP_FAILED)
+        return fail(error, cap, TPGP_EINVAL, "cannot map checkpoint", strerror(errno));
     tpgp_checkpoint s;
-    memset(&s, 0, sizeof(s));
-    char magic[8];
-    in_bytes(&r, magic, 8);
-    const uint64_t version = in_u64(&r);
-    const size_t spins = in_size(&r), count = in_size(&r);
-    s.term_count = in_size(&r);
-    s.step = in_size(&r);
-    s.total_steps = in_size(&r);
-    s.dt = in_f64(&r);
-    const uint64_t depth = in_u64(&r), global = in_u64(&r);
-    if (depth > 11 || global > 1) r.ok = 0;
-    s.residual_depth = (int)depth - 1;
-    s.initial_global = (int)global;
-    in_bytes(&r, s.model, 32);
-    s.j2 = in_f64(&r);
-    s.linear_size = in_size(&r);
-    s.resumed_from_step = in_size(&r);
-    s.prior_total_steps = in_size(&r);
-    s.prior_max_terms = in_size(&r);
-    const uint64_t reason = in_u64(&r), last = in_u64(&r);
-    if (reason > 6 || last > TPGP_EBUDGET) r.ok = 0;
-    s.stop_reason = (int)reason;
-    s.last_status = (tpgp_status)last;
-    s.options = tpgp_default_options();
-    s.options.threshold = in_f64(&r);
-    s.options.max_terms = in_size(&r);
-    const uint64_t threads = in_u64(&r), backend = in_u64(&r), validated = in_u64(&r);
-    if (threads > 65536 || backend > 4 || validated > 1) r.ok = 0;
-    s.options.threads = (unsigned)threads;
-    s.options.backend = (int)backend;
-    s.options.validated = (int)validated;
-    s.options.max_drop_l2 = in_f64(&r);
-    s.ledger.dropped_l1 = in_f64(&r);
-    s.ledger.dropped_l2 = in_f64(&r);
-    s.ledger.dropped_l2_squared = in_f64(&r);
-    s.ledger.arithmetic_l2 = in_f64(&r);
-    s.ledger.rotations = in_size(&r);
-    s.ledger.removal_events = in_size(&r);
-    s.ledger.peak_terms = in_size(&r);
-    const uint64_t enclosed = in_u64(&r);
-    if (enclosed > 1) r.ok = 0;
-    s.ledger.arithmetic_enclosed = (int)enclosed;
-    s.pf_bound = in_f64(&r);
-    s.residual_bound = in_f64(&r);
-    s.max_fid_bound = in_f64(&r);
-    const size_t W = spins && spins <= SIZE_MAX - 31 ? (spins + 31) / 32 : 0;
-    size_t keys = 0, key_bytes = 0, term_bytes = 0, disk_terms = 0, disk_map = 0;
-    if (!r.ok || memcmp(magic, "TPGPRVCP", 8) || version != 2 || !W ||
-        !mul_ok(count, W, &keys) || !mul_ok(keys, 8, &key_bytes) ||
-        !mul_ok(s.term_count, sizeof(tpgp_term), &term_bytes) ||
-        !mul_ok(s.term_count, 48, &disk_terms) || !mul_ok(count, W + 1, &disk_map) ||
-        !mul_ok(disk_map, 8, &disk_map) || disk_terms > UINT64_MAX - disk_map ||
-        r.left != (uint64_t)disk_terms + disk_map)
-        goto invalid;
-    if (tpgp_map_init(&s.map, spins)) goto invalid;
-    s.map.len = s.map.capacity = count;
-    s.map.keys = count ? malloc(key_bytes) : NULL;
-    s.map.values = count ? malloc(count * sizeof(double)) : NULL;
-    s.terms = s.term_count ? malloc(term_bytes) : NULL;
-    if ((count && (!s.map.keys || !s.map.values)) || (s.term_count && !s.terms)) {
+    const char *why = NULL;
+    tpgp_status status = decode(map, size, &s, &why);
+    munmap(map, size);
+    if (status != TPGP_OK) {
         tpgp_checkpoint_free(&s);
-        fclose(f);
-        set_error(error, cap, "checkpoint allocation failed");
-        return TPGP_ENOMEM;
+        return fail(error, cap, status, "checkpoint rejected", why);
     }
-    for (size_t k = 0; k < s.term_count; k++) {
-        tpgp_term *q = s.terms + k;
-        q->i = in_size(&r);
-        q->j = in_size(&r);
-        const uint64_t qi = in_u64(&r), qj = in_u64(&r);
-        if (qi > 3 || qj > 3) r.ok = 0;
-        q->qi = (unsigned)qi;
-        q->qj = (unsigned)qj;
-        q->h = in_f64(&r);
-        q->layer = in_size(&r);
-    }
-    for (size_t k = 0; k < count; k++) {
-        for (size_t w = 0; w < W; w++) s.map.keys[k * W + w] = in_u64(&r);
-        s.map.values[k] = in_f64(&r);
-    }
-    if (!r.ok || r.left || !valid_record(&s)) goto invalid;
-    fclose(f);
     tpgp_checkpoint_free(dst);
     *dst = s;
-    set_error(error, cap, "ok");
-    return TPGP_OK;
-invalid:
-    tpgp_checkpoint_free(&s);
-    fclose(f);
-    set_error(error, cap, "checkpoint format or numerical invariants are invalid");
-    return TPGP_EINVAL;
+    return fail(error, cap, TPGP_OK, "ok", NULL);
 }
 
-/* Same run configuration (checkpoint_rev.c:585-604): everything but state, step and runtime. */
-int tpgp_checkpoint_same_configuration(const tpgp_checkpoint *a, const tpgp_checkpoint *b) {
-    if (!a || !b || a->map.n_spins != b->map.n_spins || a->term_count != b->term_count ||
-        a->total_steps != b->total_steps || !bits_equal(a->dt, b->dt) ||
-        a->residual_depth != b->residual_depth || a->initial_global != b->initial_global ||
-        strncmp(a->model, b->model, 32) || !bits_equal(a->j2, b->j2) ||
-        a->linear_size != b->linear_size)
-        return 0;
-    const tpgp_options *x = &a->options, *y = &b->options;
-    if (!bits_equal(x->threshold, y->threshold) || x->max_terms != y->max_terms ||
-        x->threads != y->threads || x->backend != y->backend || x->validated != y->validated ||
-        !bits_equal(x->max_drop_l2, y->max_drop_l2))
-        return 0;
-    for (size_t k = 0; k < a->term_count; k++) {
-        const tpgp_term *p = a->terms + k, *q = b->terms + k;
-        if (p->i != q->i || p->j != q->j || p->qi != q->qi || p->qj != q->qj ||
-            !bits_equal(p->h, q->h) || p->layer != q->layer)
+/* ---- configuration --------------------------------------------------------------------------
+ * Two records describe the same run when every CONFIG field of the header and every term agree
+ * (doubles by bit pattern, the model as a string); state, step counters, bounds and the ledger
+ * are not configuration (checkpoint_rev.h:36-40). */
+static int same_fields(const field *t, size_t n, const void *a, const void *b) {
+    for (size_t i = 0; i < n; i++) {
+        if (!(t[i].flags & CONFIG)) continue;
+        if (t[i].as == AS_TEXT32) {
+            if (strncmp((const char *)a + t[i].at, (const char *)b + t[i].at, 32)) return 0;
+        } else if (stored_word(t + i, a) != stored_word(t + i, b)) {
             return 0;
+        }
     }
+    return 1;
+}
+int tpgp_checkpoint_same_configuration(const tpgp_checkpoint *a, const tpgp_checkpoint *b) {
+    if (!a || !b || !same_fields(HEADER, COUNT(HEADER), a, b)) return 0;
+    for (size_t t = 0; t < a->term_count; t++)
+        if (!same_fields(TERM, COUNT(TERM), a->terms + t, b->terms + t)) return 0;
     return 1;
 }

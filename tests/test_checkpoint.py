@@ -14,6 +14,7 @@ import ctypes as C
 import hashlib
 import math
 import os
+import struct
 
 import numpy as np
 import pytest
@@ -162,6 +163,18 @@ def corrupt_cases(data: bytes):
     body = bytearray(data[:-32])
     yield "version 3", _resealed(body, 8, (3).to_bytes(8, "little"))
     yield "bad magic", _resealed(body, 0, b"TPGPRVCQ")
+    # header fields whose stored value is outside what the field admits, resealed
+    for name, offset, value in [("depth 12", 64, 12), ("global 2", 72, 2),
+                                ("stop reason 7", 152, 7), ("status 7", 160, 7),
+                                ("threads 65537", 184, 65537), ("threads 0", 184, 0),
+                                ("backend 5", 192, 5), ("validated 2", 200, 2),
+                                ("zero spins", 16, 0), ("one more term", 24, None),
+                                ("step past total", 40, 10 ** 6)]:
+        if value is None:  # the header's length disagrees with the body
+            value = int.from_bytes(body[24:32], "little") + 1
+        yield name, _resealed(body, offset, value.to_bytes(8, "little"))
+    yield "negative dt", _resealed(body, 56, struct.pack("<d", -0.01))
+    yield "model with a capital", _resealed(body, 80, b"J")
 
 
 def _resealed(body: bytearray, offset, patch):
@@ -189,6 +202,13 @@ def test_corrupt_records_rejected_like_the_reference(tmp_path, ref_lib):
             E.Checkpoint.load(bad)
         st, rec = ref_load(ref_lib, bad)
         assert st != 0, name
+    # a rejected load leaves the destination record as it was
+    keep = E.Checkpoint.load(path)
+    before = fields(keep.c)
+    err = C.create_string_buffer(256)
+    assert N.engine().tpgp_checkpoint_load(str(tmp_path / "bad.ck").encode(), C.byref(keep.c),
+                                           err, 256) != 0
+    assert fields(keep.c) == before and err.value
     # invalid records are refused at save by both
     o = E.options(threshold=-1.0)
     o.backend = 0
