@@ -41,11 +41,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public specification
-# HBM bytes per launch of k_rot_write over the default window (warmup 600, 40 timed steps),
-# from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings (factors 2.00 and
-# 1.00): profiles/k_rot_write_j1j2_n16_window_pmc.md.  Used for roofline.traffic when the run
-# is that window and no --traffic is given; other windows report null.
-MEASURED_TRAFFIC = {("k_rot_write", 600, 40): 555.8e6}
+# HBM bytes per launch of k_rot_write over the default window (untimed steps 1..605, timed steps
+# 606..625), from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings (factors
+# 2.00 and 1.00): profiles/k_rot_write_j1j2_n16_headline_pmc.md.  Used for roofline.traffic when
+# the run is that window and no --traffic is given; other windows report null.
+MEASURED_TRAFFIC = {("k_rot_write", 605, 20): 533.9e6}
 
 
 def log(*a):

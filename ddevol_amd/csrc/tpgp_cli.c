@@ -99,22 +99,27 @@ static void usage(FILE *f) {
           f);
 }
 
-/* ---- argument parsing ------------------------------------------------------------------- */
-static int parse_real(const char *s, double *v) {
-    char *end;
+/* ---- argument parsing -------------------------------------------------------------------
+ * One reader for the two kinds of numeric option value: a finite double (strtod, nothing
+ * trailing), or a non-negative decimal count that fits a size_t. */
+enum { VALUE_REAL, VALUE_COUNT };
+static int read_value(const char *text, int kind, void *out) {
+    char *end = NULL;
     errno = 0;
-    *v = strtod(s, &end);
-    return !errno && end != s && !*end && isfinite(*v);
-}
-static int parse_count(const char *s, size_t *v) {
-    if (*s == '-') return 0;
-    char *end;
-    errno = 0;
-    const uintmax_t x = strtoumax(s, &end, 10);
-    if (errno || end == s || *end || x > SIZE_MAX) return 0;
-    *v = (size_t)x;
+    if (kind == VALUE_REAL) {
+        const double v = strtod(text, &end);
+        if (errno || end == text || *end || !isfinite(v)) return 0;
+        *(double *)out = v;
+        return 1;
+    }
+    if (text[0] == '-') return 0; /* strtoumax would wrap a negative count */
+    const uintmax_t v = strtoumax(text, &end, 10);
+    if (errno || end == text || *end || v > SIZE_MAX) return 0;
+    *(size_t *)out = (size_t)v;
     return 1;
 }
+#define parse_real(text, out) read_value((text), VALUE_REAL, (out))
+#define parse_count(text, out) read_value((text), VALUE_COUNT, (out))
 
 /* 0 ok, 1 --help printed, 2 usage error */
 static int parse_args(int argc, char **argv, config *c) {
@@ -197,13 +202,29 @@ static int push_term(term_list *l, size_t i, size_t j, unsigned qi, unsigned qj,
     l->len++;
     return 1;
 }
-/* One bond of coupling d: Ising d ZZ/4; XY d(XX + YY)/4; otherwise the secular dipolar form
- * d(2ZZ - XX - YY)/4 as ZZ, XX, YY in that order. */
+/* The two-site Pauli terms of one bond of coupling d, per model, in the order they are applied
+ * (cli_rev.c:120-127): Ising d ZZ/4; XY d(XX + YY)/4; otherwise the secular dipolar form
+ * d(2ZZ - XX - YY)/4 as ZZ, XX, YY.  Scaling by 1/2 and 1/4 is exact, so d * factor equals the
+ * reference's d / 2 and -d / 4 bit for bit. */
+typedef struct {
+    const char *model;
+    unsigned parts;
+    struct {
+        unsigned q;
+        double factor;
+    } part[3];
+} bond_form;
+static const bond_form kBondForms[] = {
+    {"ising", 1, {{3, 0.25}}},
+    {"xy", 2, {{1, 0.25}, {2, 0.25}}},
+    {NULL, 3, {{3, 0.5}, {1, -0.25}, {2, -0.25}}}, /* secular: every other model */
+};
 static int push_bond(term_list *l, size_t i, size_t j, double d, const char *model) {
-    if (!strcmp(model, "ising")) return push_term(l, i, j, 3, 3, d / 4);
-    if (!strcmp(model, "xy")) return push_term(l, i, j, 1, 1, d / 4) && push_term(l, i, j, 2, 2, d / 4);
-    return push_term(l, i, j, 3, 3, d / 2) && push_term(l, i, j, 1, 1, -d / 4) &&
-           push_term(l, i, j, 2, 2, -d / 4);
+    const bond_form *f = kBondForms;
+    while (f->model && strcmp(f->model, model)) f++;
+    for (unsigned k = 0; k < f->parts; k++)
+        if (!push_term(l, i, j, f->part[k].q, f->part[k].q, d * f->part[k].factor)) return 0;
+    return 1;
 }
 static long min_image(long d, size_t L) {
     if (d > (long)L / 2) d -= (long)L;
@@ -256,21 +277,34 @@ static int build_terms(term_list *l, const config *c) {
     return 1;
 }
 
-/* ---- checkpoint lock (cli_rev.c:38-62) ---------------------------------------------------- */
+/* ---- checkpoint lock (cli_rev.c:38-62) ----------------------------------------------------
+ * A run that writes checkpoints holds an exclusive advisory lock (flock, as the reference's
+ * driver takes it, so the two programs exclude each other) on "<checkpoint>.lock" for its whole
+ * life, and never replaces a record that it did not restart from. */
+static int lock_beside(const char *path) {
+    const size_t n = strlen(path);
+    char *lock_path = malloc(n + sizeof(".lock"));
+    if (!lock_path) return -1;
+    memcpy(lock_path, path, n);
+    memcpy(lock_path + n, ".lock", sizeof(".lock"));
+    int fd = open(lock_path, O_CREAT | O_RDWR, 0600);
+    free(lock_path);
+    if (fd >= 0 && flock(fd, LOCK_EX | LOCK_NB) != 0) {
+        close(fd);
+        fd = -1;
+    }
+    return fd;
+}
 static int take_lock(const char *checkpoint, const char *restart) {
-    const size_t n = strlen(checkpoint) + 6;
-    char *name = malloc(n);
-    if (!name) return -1;
-    snprintf(name, n, "%s.lock", checkpoint);
-    const int fd = open(name, O_CREAT | O_RDWR, 0600);
-    free(name);
-    if (fd < 0 || flock(fd, LOCK_EX | LOCK_NB)) {
+    const int fd = lock_beside(checkpoint);
+    if (fd < 0) {
         fprintf(stderr, "Checkpoint is locked by another run or cannot be locked.\n");
-        if (fd >= 0) close(fd);
         return -1;
     }
     struct stat st;
-    if (lstat(checkpoint, &st) == 0 && (!restart || strcmp(checkpoint, restart))) {
+    const int exists = lstat(checkpoint, &st) == 0;
+    const int resuming_it = restart && !strcmp(checkpoint, restart);
+    if (exists && !resuming_it) {
         fprintf(stderr, "Refusing to overwrite an existing checkpoint without its restart.\n");
         close(fd);
         return -1;
