@@ -2985,6 +2985,10 @@ bool load_tuning(tpgp_hip_context *ctx, const char *path, char *error, size_t ca
 }
 }  // namespace
 
+namespace {
+hipError_t resolve_scratch(int device);  // the allocation-free entry point's fixed scratch
+}  // namespace
+
 extern "C" tpgp_hip_context *tpgp_hip_create(int index, char *error, size_t cap) {
     msg(error, cap, "");
     int count = 0, previous = 0;
@@ -3027,6 +3031,14 @@ extern "C" tpgp_hip_context *tpgp_hip_create(int index, char *error, size_t cap)
     code = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (code != hipSuccess) {
         hip_fail(code, "HIP stream creation", error, cap);
+        delete ctx;
+        (void)hipSetDevice(previous);
+        return nullptr;
+    }
+    // this device's copy of the allocation-free entry point's fixed scratch
+    if ((code = resolve_scratch(index)) != hipSuccess) {
+        hip_fail(code, "HIP fixed-scratch address", error, cap);
+        (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         (void)hipSetDevice(previous);
         return nullptr;
@@ -3102,12 +3114,17 @@ extern "C" int tpgp_hip_get_tuning(tpgp_hip_context *ctx, unsigned words, unsign
     return TPGP_HIP_OK;
 }
 
-extern "C" tpgp_hip_status tpgp_hip_branches(
-    tpgp_hip_context *ctx, const uint64_t *keys, const double *values, size_t count,
-    size_t words, size_t spins, size_t i, size_t j, unsigned qi, unsigned qj, double c, double s,
-    double rc, double rs, int enclose, uint64_t *out_keys, double *out_values, double *out_radii,
-    char *error, size_t cap) {
-    msg(error, cap, "");
+namespace {
+// Validated sizes of one branch call (tpgp_hip_branches' contract).
+struct BranchSizes {
+    size_t key_bytes, value_bytes, okey_bytes, oval_bytes, total;
+};
+tpgp_hip_status check_branches(tpgp_hip_context *ctx, const uint64_t *keys,
+                               const double *values, size_t count, size_t words, size_t spins,
+                               size_t i, size_t j, unsigned qi, unsigned qj, double c, double s,
+                               double rc, double rs, int enclose, const uint64_t *out_keys,
+                               const double *out_values, const double *out_radii,
+                               BranchSizes *z, char *error, size_t cap) {
     if (!ctx || spins < 2 || words != spins / 32 + (spins % 32 != 0) || i >= spins ||
         j >= spins || i == j || qi > 3 || qj > 3 || !std::isfinite(c) || !std::isfinite(s) ||
         !std::isfinite(rc) || !std::isfinite(rs) || rc < 0 || rs < 0 ||
@@ -3117,51 +3134,44 @@ extern "C" tpgp_hip_status tpgp_hip_branches(
         return TPGP_HIP_EINVAL;
     }
     if (!count) return TPGP_HIP_OK;
-    size_t key_bytes, value_bytes, okey_bytes, oval_bytes, total, t;
+    size_t t;
     if (!keys || !values || !out_keys || !out_values || !out_radii ||
-        !mulz(count, words, &t) || !mulz(t, sizeof(uint64_t), &key_bytes) ||
-        !mulz(count, sizeof(double), &value_bytes) || !mulz(key_bytes, 2, &okey_bytes) ||
-        !mulz(value_bytes, 2, &oval_bytes) || !mulz(key_bytes, 3, &total) ||
-        !mulz(value_bytes, 5, &t) || !addz(total, t, &total) || !addz(total, 256, &total)) {
+        !mulz(count, words, &t) || !mulz(t, sizeof(uint64_t), &z->key_bytes) ||
+        !mulz(count, sizeof(double), &z->value_bytes) || !mulz(z->key_bytes, 2, &z->okey_bytes) ||
+        !mulz(z->value_bytes, 2, &z->oval_bytes) || !mulz(z->key_bytes, 3, &z->total) ||
+        !mulz(z->value_bytes, 5, &t) || !addz(z->total, t, &z->total) ||
+        !addz(z->total, 256, &z->total)) {
         msg(error, cap, "Null HIP array or packed-branch allocation size overflow.");
         return TPGP_HIP_EINVAL;
     }
     const void *arrays[] = {keys, values, out_keys, out_values, out_radii};
-    const size_t lengths[] = {key_bytes, value_bytes, okey_bytes, oval_bytes, oval_bytes};
+    const size_t lengths[] = {z->key_bytes, z->value_bytes, z->okey_bytes, z->oval_bytes,
+                              z->oval_bytes};
     for (int a = 0; a < 5; ++a)
         for (int b2 = a + 1; b2 < 5; ++b2)
             if (overlap(arrays[a], lengths[a], arrays[b2], lengths[b2])) {
                 msg(error, cap, "HIP input/output arrays must be disjoint.");
                 return TPGP_HIP_EINVAL;
             }
-    std::lock_guard<std::mutex> lock(ctx->mutex);
-    DeviceGuard guard(ctx->device);
-    if (!guard.ok) {
-        msg(error, cap, "HIP device selection failed.");
-        return TPGP_HIP_EDEVICE;
-    }
+    return TPGP_HIP_OK;
+}
+
+// One branch call on a device buffer of at least z.total bytes (the caller holds the context's
+// mutex and has made its device current): inputs in, k_branches, status word and outputs out.
+// The status word is cleared by a fill kernel, not a memset, so the sequence holds no memset
+// node if a caller captures the stream.
+tpgp_hip_status run_branches(tpgp_hip_context *ctx, unsigned char *base, const BranchSizes &z,
+                             const uint64_t *keys, const double *values, size_t count,
+                             size_t words, size_t spins, size_t i, size_t j, unsigned qi,
+                             unsigned qj, double c, double s, double rc, double rs, int enclose,
+                             uint64_t *out_keys, double *out_values, double *out_radii,
+                             char *error, size_t cap) {
     tpgp_hip_status result = TPGP_HIP_OK;
     hipError_t code;
-    if (total > ctx->capacity) {
-        // release first: avoids a transient double allocation as a trajectory grows
-        if (ctx->storage) {
-            code = hipFree(ctx->storage);
-            ctx->storage = nullptr;
-            ctx->capacity = 0;
-            if (code != hipSuccess) return hip_fail(code, "HIP scratch release", error, cap);
-        }
-        code = hipMalloc(&ctx->storage, total);
-        if (code != hipSuccess) {
-            ctx->storage = nullptr;
-            return hip_fail(code, "HIP persistent scratch allocation", error, cap);
-        }
-        ctx->capacity = total;
-    }
-    unsigned char *base = (unsigned char *)ctx->storage;
     uint64_t *dk = (uint64_t *)base;
-    double *dv = (double *)(base + key_bytes);
-    uint64_t *dok = (uint64_t *)(base + key_bytes + value_bytes);
-    double *dov = (double *)(base + key_bytes + value_bytes + okey_bytes);
+    double *dv = (double *)(base + z.key_bytes);
+    uint64_t *dok = (uint64_t *)(base + z.key_bytes + z.value_bytes);
+    double *dov = (double *)(base + z.key_bytes + z.value_bytes + z.okey_bytes);
     double *dor = dov + 2 * count;
     unsigned *dflag = (unsigned *)(dor + 2 * count);
     unsigned flag = 0;
@@ -3172,11 +3182,14 @@ extern "C" tpgp_hip_status tpgp_hip_branches(
             if (code != hipSuccess) result = hip_fail(code, what, error, cap); \
         }                                                                     \
     } while (0)
-    HIPCALL(hipMemcpyAsync(dk, keys, key_bytes, hipMemcpyHostToDevice, ctx->stream),
+    HIPCALL(hipMemcpyAsync(dk, keys, z.key_bytes, hipMemcpyHostToDevice, ctx->stream),
             "HIP input-key transfer");
-    HIPCALL(hipMemcpyAsync(dv, values, value_bytes, hipMemcpyHostToDevice, ctx->stream),
+    HIPCALL(hipMemcpyAsync(dv, values, z.value_bytes, hipMemcpyHostToDevice, ctx->stream),
             "HIP input-coefficient transfer");
-    HIPCALL(hipMemsetAsync(dflag, 0, sizeof(unsigned), ctx->stream), "HIP status reset");
+    if (result == TPGP_HIP_OK) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, dflag, 1ull, 0u);
+        HIPCALL(hipGetLastError(), "HIP status reset");
+    }
     if (result == TPGP_HIP_OK) {
         size_t want = (count + kBlock - 1) / kBlock;
         size_t maxb = (size_t)ctx->cus * 16;
@@ -3194,17 +3207,116 @@ extern "C" tpgp_hip_status tpgp_hip_branches(
         msg(error, cap, (flag & 1u) ? "HIP input keys contain nonzero padding bits."
                                     : "HIP branch arithmetic or coefficients are nonfinite.");
     }
-    HIPCALL(hipMemcpyAsync(out_keys, dok, okey_bytes, hipMemcpyDeviceToHost, ctx->stream),
+    HIPCALL(hipMemcpyAsync(out_keys, dok, z.okey_bytes, hipMemcpyDeviceToHost, ctx->stream),
             "HIP output-key transfer");
-    HIPCALL(hipMemcpyAsync(out_values, dov, oval_bytes, hipMemcpyDeviceToHost, ctx->stream),
+    HIPCALL(hipMemcpyAsync(out_values, dov, z.oval_bytes, hipMemcpyDeviceToHost, ctx->stream),
             "HIP output-coefficient transfer");
-    HIPCALL(hipMemcpyAsync(out_radii, dor, oval_bytes, hipMemcpyDeviceToHost, ctx->stream),
+    HIPCALL(hipMemcpyAsync(out_radii, dor, z.oval_bytes, hipMemcpyDeviceToHost, ctx->stream),
             "HIP output-radius transfer");
     HIPCALL(hipStreamSynchronize(ctx->stream), "HIP output completion");
 #undef HIPCALL
     // drain anything queued before an earlier failure; the caller may release its arrays
     if (result != TPGP_HIP_OK) (void)hipStreamSynchronize(ctx->stream);
     return result;
+}
+}  // namespace
+
+extern "C" tpgp_hip_status tpgp_hip_branches(
+    tpgp_hip_context *ctx, const uint64_t *keys, const double *values, size_t count,
+    size_t words, size_t spins, size_t i, size_t j, unsigned qi, unsigned qj, double c, double s,
+    double rc, double rs, int enclose, uint64_t *out_keys, double *out_values, double *out_radii,
+    char *error, size_t cap) {
+    msg(error, cap, "");
+    BranchSizes z{};
+    tpgp_hip_status st = check_branches(ctx, keys, values, count, words, spins, i, j, qi, qj, c,
+                                        s, rc, rs, enclose, out_keys, out_values, out_radii, &z,
+                                        error, cap);
+    if (st != TPGP_HIP_OK || !count) return st;
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    DeviceGuard guard(ctx->device);
+    if (!guard.ok) {
+        msg(error, cap, "HIP device selection failed.");
+        return TPGP_HIP_EDEVICE;
+    }
+    hipError_t code;
+    if (z.total > ctx->capacity) {
+        // release first: avoids a transient double allocation as a trajectory grows
+        if (ctx->storage) {
+            code = hipFree(ctx->storage);
+            ctx->storage = nullptr;
+            ctx->capacity = 0;
+            if (code != hipSuccess) return hip_fail(code, "HIP scratch release", error, cap);
+        }
+        code = hipMalloc(&ctx->storage, z.total);
+        if (code != hipSuccess) {
+            ctx->storage = nullptr;
+            return hip_fail(code, "HIP persistent scratch allocation", error, cap);
+        }
+        ctx->capacity = z.total;
+    }
+    return run_branches(ctx, (unsigned char *)ctx->storage, z, keys, values, count, words, spins,
+                        i, j, qi, qj, c, s, rc, rs, enclose, out_keys, out_values, out_radii,
+                        error, cap);
+}
+
+// ---- the allocation-free branch call ------------------------------------------------------
+// A fixed device array per GPU replaces the context's growable scratch.  In host code the name
+// `scratch` is only a stand-in for the symbol: its device address is taken once per device by
+// hipGetSymbolAddress (in tpgp_hip_create, with that device current) and only that address is
+// handed to copies and kernels.
+namespace {
+__device__ __align__(256) unsigned char scratch[TPGP_HIP_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+// With `device` current: resolve this device's copy of `scratch` once.  0 on success.
+hipError_t resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    hipError_t code = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (code == hipSuccess) g_scratch_ptr[device] = (unsigned char *)ptr;
+    return code;
+}
+unsigned char *scratch_of(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
+
+extern "C" tpgp_hip_status tpgp_hip_branches_noalloc(
+    tpgp_hip_context *ctx, const uint64_t *keys, const double *values, size_t count,
+    size_t words, size_t spins, size_t i, size_t j, unsigned qi, unsigned qj, double c, double s,
+    double rc, double rs, int enclose, uint64_t *out_keys, double *out_values, double *out_radii,
+    char *error, size_t cap) {
+    msg(error, cap, "");
+    BranchSizes z{};
+    tpgp_hip_status st = check_branches(ctx, keys, values, count, words, spins, i, j, qi, qj, c,
+                                        s, rc, rs, enclose, out_keys, out_values, out_radii, &z,
+                                        error, cap);
+    if (st != TPGP_HIP_OK || !count) return st;
+    if (z.total > (size_t)TPGP_HIP_NOALLOC_SCRATCH_BYTES) {
+        msg(error, cap,
+            "HIP allocation-free branch call needs %zu bytes of device scratch; its fixed "
+            "scratch holds %zu (at most %zu terms of %zu key words).",
+            z.total, (size_t)TPGP_HIP_NOALLOC_SCRATCH_BYTES,
+            ((size_t)TPGP_HIP_NOALLOC_SCRATCH_BYTES - 256) / (24 * words + 40), words);
+        return TPGP_HIP_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mutex);
+    DeviceGuard guard(ctx->device);
+    if (!guard.ok) {
+        msg(error, cap, "HIP device selection failed.");
+        return TPGP_HIP_EDEVICE;
+    }
+    unsigned char *base = scratch_of(ctx->device);
+    if (!base) {
+        msg(error, cap, "HIP fixed scratch of device %d was not resolved.", ctx->device);
+        return TPGP_HIP_EDEVICE;
+    }
+    return run_branches(ctx, base, z, keys, values, count, words, spins, i, j, qi, qj, c, s, rc,
+                        rs, enclose, out_keys, out_values, out_radii, error, cap);
 }
 
 // ==========================================================================================

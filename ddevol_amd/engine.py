@@ -55,18 +55,21 @@ class Context:
         except Exception:
             pass
 
-    def branches(self, keys, values, n_words, n_spins, i, j, qi, qj, c, s, rc, rs, enclose):
-        """tpgp_hip_branches on host arrays; returns (status, out_keys, out_values, out_radii, msg)."""
+    def branches(self, keys, values, n_words, n_spins, i, j, qi, qj, c, s, rc, rs, enclose,
+                 noalloc=False, fill=0):
+        """tpgp_hip_branches (or, with noalloc, tpgp_hip_branches_noalloc) on host arrays;
+        returns (status, out_keys, out_values, out_radii, msg).  `fill` is the bit pattern the
+        output arrays hold before the call."""
         keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1)
         values = np.ascontiguousarray(values, dtype=np.float64)
         count = len(values)
-        ok = np.zeros(2 * count * n_words, np.uint64)
-        ov = np.zeros(2 * count, np.float64)
-        orad = np.zeros(2 * count, np.float64)
+        ok = np.full(2 * count * n_words, fill, np.uint64)
+        ov = np.full(2 * count, fill, np.uint64).view(np.float64)
+        orad = np.full(2 * count, fill, np.uint64).view(np.float64)
         err = C.create_string_buffer(512)
-        st = self.lib.tpgp_hip_branches(self.ptr, N.as_u64p(keys), N.as_dp(values), count,
-                                        n_words, n_spins, i, j, qi, qj, c, s, rc, rs, enclose,
-                                        N.as_u64p(ok), N.as_dp(ov), N.as_dp(orad), err, 512)
+        fn = self.lib.tpgp_hip_branches_noalloc if noalloc else self.lib.tpgp_hip_branches
+        st = fn(self.ptr, N.as_u64p(keys), N.as_dp(values), count, n_words, n_spins, i, j, qi,
+                qj, c, s, rc, rs, enclose, N.as_u64p(ok), N.as_dp(ov), N.as_dp(orad), err, 512)
         return st, ok.reshape(2 * count, n_words) if count else ok, ov, orad, err.value.decode()
 
 

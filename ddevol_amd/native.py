@@ -130,7 +130,8 @@ class Checkpoint(C.Structure):  # checkpoint_rev.h:7-23 layout
 
 
 HIP_SYMBOLS = ["tpgp_hip_compiled", "tpgp_hip_create", "tpgp_hip_destroy",
-               "tpgp_hip_device_name", "tpgp_hip_branches", "tpgp_hip_batch_create",
+               "tpgp_hip_device_name", "tpgp_hip_branches", "tpgp_hip_branches_noalloc",
+               "tpgp_hip_batch_create",
                "tpgp_hip_batch_destroy", "tpgp_hip_batch_upload", "tpgp_hip_batch_download",
                "tpgp_hip_batch_rotate", "tpgp_hip_batch_rotate_many", "tpgp_hip_batch_readout", "tpgp_hip_batch_sync",
                "tpgp_hip_batch_commit", "tpgp_hip_batch_rollback", "tpgp_hip_batch_reserve",
@@ -179,6 +180,9 @@ def hip():
     f("tpgp_hip_branches", C.c_int, C.c_void_p, p_u64, p_d, sz, sz, sz, sz, sz, C.c_uint,
       C.c_uint, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, p_u64, p_d, p_d, p_c,
       sz)
+    f("tpgp_hip_branches_noalloc", C.c_int, C.c_void_p, p_u64, p_d, sz, sz, sz, sz, sz,
+      C.c_uint, C.c_uint, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, p_u64, p_d,
+      p_d, p_c, sz)
     f("tpgp_hip_batch_create", C.c_void_p, C.c_void_p, sz, sz, C.POINTER(sz), C.POINTER(sz),
       p_c, sz)
     f("tpgp_hip_batch_destroy", None, C.c_void_p)

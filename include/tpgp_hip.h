@@ -56,6 +56,25 @@ tpgp_hip_status tpgp_hip_branches(tpgp_hip_context *context, const uint64_t *key
                                   uint64_t *output_keys, double *output_values,
                                   double *output_radii, char *error, size_t error_capacity);
 
+/* No counterpart in the reference: tpgp_hip_branches for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, outputs, statuses and error convention as tpgp_hip_branches, but the device
+ * copies of inputs and outputs live in a fixed per-GPU array of TPGP_HIP_NOALLOC_SCRATCH_BYTES
+ * bytes, held from the library's load until the process ends, instead of the context's growable
+ * scratch; the call performs no hipMalloc, hipFree or other allocating routine.  A call needs
+ * count * (24 * n_words + 40) + 256 bytes (one-word keys: up to 524,284 terms); a larger call
+ * returns TPGP_HIP_EINVAL with a message.  Every call on one GPU shares that GPU's array, and
+ * calls through one context are serialised by its mutex: calls through two contexts on the
+ * same GPU that overlap in time race on the array. */
+#define TPGP_HIP_NOALLOC_SCRATCH_BYTES (32u << 20)
+tpgp_hip_status tpgp_hip_branches_noalloc(tpgp_hip_context *context, const uint64_t *keys,
+                                          const double *values, size_t count, size_t n_words,
+                                          size_t n_spins, size_t i, size_t j, unsigned qi,
+                                          unsigned qj, double c, double s, double rc, double rs,
+                                          int enclose, uint64_t *output_keys,
+                                          double *output_values, double *output_radii,
+                                          char *error, size_t error_capacity);
+
 /* ============================ Part 2: resident batch ==================================== */
 
 /* A batch holds n_maps independent Pauli maps (one per FID: a powder orientation, a coupling
