@@ -364,6 +364,8 @@ struct BatchDev {
     uint64_t max_samples, max_ftiles;
     uint32_t *counter;        // [4]
     uint32_t *overflow;       // [M]
+    uint32_t *small_resume;   // [M] first pass the global small-map kernel runs (the LDS kernel
+                              // ran the ones before it)
     tpgp_hip_record *rec;     // [M*max_rec]
     double *rd;               // [M*max_rd*64]
     LogDev *log;              // [log_cap]
@@ -2773,15 +2775,525 @@ __device__ __forceinline__ void small_partition(const BatchDev &b, const PassMap
     }
 }
 
-// One workgroup per small map (maps[blockIdx.x]) runs n_ops passes: descriptor k of map m is
-// ops[k * M + m]; descriptors without kPassSmall belong to other maps' paths.
+// ------------------------------------------------------------------------------------------
+// Small maps resident in LDS.  A map of at most kLdsCap<W> terms is loaded into LDS once per
+// launch and every pass of the chunk runs there, ping-ponging between two LDS buffers; only the
+// per-rotation records, the ledger log and the final state go to global memory.  Both passes
+// are two walks over the thread's share of the work with a scan between them: the first walk
+// counts each thread's kept outputs per output class (class-major table cnt[class][thread]),
+// the scan turns the table into output offsets, the second walk writes.
+//   Rotation: the work of a commuting class is its terms; the work of an anticommuting pair
+//   (c, c ^ q) is the merge path of the two classes (keys of c ^ q xor-ed into c's top digits).
+//   Every merge-path position gives one output key in each class of the pair -- the key with
+//   its own a-branch and, when the partner class holds the same lower digits, the partner's
+//   d-branch -- so both classes' outputs are written in key order by the same walk.
+//   Partition: a stable 16-way split by the digits at positions pa, pb, moved to the top.
+// A map that starts too large, or whose next output would outgrow LDS or its slot, leaves the
+// rest of the chunk to k_small (global memory), which the host launches right after: the LDS
+// kernel writes the input of the pass it stops at to that pass's global buffer and records the
+// pass in small_resume.
+// ------------------------------------------------------------------------------------------
+constexpr int kLdsThreads = 512;  // 8 waves: one workgroup per CU (its LDS holds the map)
+template <int W> struct LdsCap;
+template <> struct LdsCap<1> { static constexpr uint32_t value = 4096; };
+template <> struct LdsCap<2> { static constexpr uint32_t value = 2816; };
+
+template <int W> struct LdsMap {
+    static constexpr uint32_t kCap = LdsCap<W>::value;
+    uint64_t keys[2][kCap * W];
+    double vals[2][kCap];
+    uint16_t cnt[16][kLdsThreads];  // per (output class, thread): kept count, then offset
+    uint32_t reg[2][17];            // class regions of each buffer
+    uint32_t len[2];
+    uint32_t seg[17];               // rotation work: start of the work of class c (the upper
+                                    // class of an anticommuting pair has none of its own)
+    uint32_t total;
+    unsigned scratch[kLdsThreads / 64];
+    TileLedger red[kLdsThreads / 64];
+};
+
+__device__ unsigned lds_exclusive_scan(unsigned v, unsigned *scratch, unsigned *tot) {
+    const unsigned x = wave_scan_u32(v);
+    if (lane_id() == 63) scratch[wave_id()] = x;
+    __syncthreads();
+    unsigned base = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kLdsThreads / 64; ++w) {
+        const unsigned s = scratch[w];
+        if ((unsigned)w < wave_id()) base += s;
+        all += s;
+    }
+    __syncthreads();
+    *tot = all;
+    return base + x - v;
+}
+
+// Per-thread ledger partials (upward-rounded chains, so upper bounds) summed over the block for
+// thread 0: a DPP tree per wave (depth 6) and a fold across the waves in wave order (depth
+// kLdsThreads / 64 - 1 more), in round-to-nearest, then made an upper bound once.  Every
+// summand is non-negative, so a sum S' of depth d satisfies S' >= (1 - u)^d S, and
+// next_up(S' (1 + 2 d u)) >= S (u = 2^-53).  Counts are exact.  Fixed order: deterministic.
+__device__ TileLedger lds_ledger(TileLedger a, TileLedger *sh) {
+    const double l1 = wave_scan_f64(a.l1), sq = wave_scan_f64(a.sq), rsq = wave_scan_f64(a.rsq);
+    const unsigned nd = wave_scan_u32((unsigned)a.dropped), nr = wave_scan_u32((unsigned)a.radii);
+    const bool bad = __ballot(a.nonfinite != 0) != 0;
+    if (lane_id() == 63) {
+        TileLedger w;
+        w.l1 = l1;
+        w.sq = sq;
+        w.rsq = rsq;
+        w.dropped = nd;
+        w.radii = nr;
+        w.nonfinite = bad ? 1u : 0u;
+        w.pad = 0;
+        sh[wave_id()] = w;
+    }
+    __syncthreads();
+    TileLedger r = sh[0];
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kLdsThreads / 64; ++w) {
+            r.l1 += sh[w].l1;
+            r.sq += sh[w].sq;
+            r.rsq += sh[w].rsq;
+            r.dropped += sh[w].dropped;
+            r.radii += sh[w].radii;
+            r.nonfinite |= sh[w].nonfinite;
+        }
+        constexpr int kDepth = 6 + (kLdsThreads / 64 - 1);
+        constexpr double kInflate = 1.0 + 2.0 * kDepth / 9007199254740992.0;  // exact
+        r.l1 = upmul(r.l1, kInflate);
+        r.sq = upmul(r.sq, kInflate);
+        r.rsq = upmul(r.rsq, kInflate);
+    }
+    return r;
+}
+
+// cnt[c][t] (this thread's kept count per class) -> exclusive output offsets in class-major,
+// thread-minor order; reg[c] = the offset of class c, reg[16] = the total (also sh.total).
+// Each thread scans 16 consecutive entries of one row.
+template <int W>
+__device__ void lds_offsets(LdsMap<W> &sh, uint32_t *reg) {
+    constexpr unsigned kPerRow = kLdsThreads / 16;  // threads per row of the table
+    const unsigned row = threadIdx.x / kPerRow, col0 = (threadIdx.x % kPerRow) * 16;
+    unsigned sum = 0;
+    for (int k = 0; k < 16; ++k) sum += sh.cnt[row][col0 + k];
+    unsigned tot;
+    unsigned at = lds_exclusive_scan(sum, sh.scratch, &tot);
+    if (col0 == 0) reg[row] = at;
+    for (int k = 0; k < 16; ++k) {
+        const unsigned here = sh.cnt[row][col0 + k];
+        sh.cnt[row][col0 + k] = (uint16_t)at;
+        at += here;
+    }
+    if (threadIdx.x == 0) {
+        reg[16] = tot;
+        sh.total = tot;
+    }
+    __syncthreads();
+}
+
+// Arena buffer `buf` selected by comparisons: indexing the kernel-argument array with a runtime
+// value would make the compiler copy the whole argument block to scratch memory.
+__device__ __forceinline__ Arena arena_at(const BatchDev &b, uint32_t buf) {
+    return buf == 0 ? b.arena[0] : buf == 1 ? b.arena[1] : b.arena[2];
+}
+
+template <int W>
+__device__ bool lds_load(const BatchDev &b, uint32_t buf, uint32_t m, LdsMap<W> &sh) {
+    const uint64_t len = b.len[m * 3 + buf];
+    if (len > LdsMap<W>::kCap) return false;  // uniform: every thread reads the same word
+    const uint64_t off = b.slot[m].off;
+    const Arena a = arena_at(b, buf);
+    const uint64_t *gk = a.keys + off * W;
+    const double *gv = a.vals + off;
+    for (uint32_t e = threadIdx.x; e < len; e += kLdsThreads) {
+        kstore<W>(sh.keys[0] + e * W, kload<W>(gk + e * W));
+        sh.vals[0][e] = gv[e];
+    }
+    if (threadIdx.x < 17)
+        sh.reg[0][threadIdx.x] = (uint32_t)b.regions[((uint64_t)m * 3 + buf) * 17 + threadIdx.x];
+    if (threadIdx.x == 0) sh.len[0] = (uint32_t)len;
+    __syncthreads();
+    return true;
+}
+
+template <int W>
+__device__ void lds_store(const BatchDev &b, uint32_t buf, uint32_t m, const LdsMap<W> &sh,
+                          int cur) {
+    const uint32_t len = sh.len[cur];
+    const uint64_t off = b.slot[m].off;
+    const Arena a = arena_at(b, buf);
+    uint64_t *gk = a.keys + off * W;
+    double *gv = a.vals + off;
+    for (uint32_t e = threadIdx.x; e < len; e += kLdsThreads) {
+        kstore<W>(gk + e * W, kload<W>(sh.keys[cur] + e * W));
+        gv[e] = sh.vals[cur][e];
+    }
+    if (threadIdx.x < 17)
+        b.regions[((uint64_t)m * 3 + buf) * 17 + threadIdx.x] = sh.reg[cur][threadIdx.x];
+    if (threadIdx.x == 0) b.len[m * 3 + buf] = len;
+}
+
+template <int W>
+__device__ void lds_partition(const PassMap &P, uint32_t n, LdsMap<W> &sh, int cur) {
+    const uint32_t len = sh.len[cur], per = (len + kLdsThreads - 1) / kLdsThreads;
+    const uint32_t e0 = min(len, threadIdx.x * per), e1 = min(len, e0 + per);
+    const uint64_t *keys = sh.keys[cur];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) sh.cnt[c][threadIdx.x] = 0;
+    for (uint32_t e = e0; e < e1; ++e) {
+        const Key<W> k = kload<W>(keys + e * W);
+        sh.cnt[(kdigit<W>(k, P.pa) << 2) | kdigit<W>(k, P.pb)][threadIdx.x]++;
+    }
+    __syncthreads();
+    lds_offsets<W>(sh, sh.reg[cur ^ 1]);
+    uint64_t *okeys = sh.keys[cur ^ 1];
+    double *ovals = sh.vals[cur ^ 1];
+    for (uint32_t e = e0; e < e1; ++e) {
+        const Key<W> k = kload<W>(keys + e * W);
+        const unsigned cls = (kdigit<W>(k, P.pa) << 2) | kdigit<W>(k, P.pb);
+        const uint32_t at = sh.cnt[cls][threadIdx.x]++;
+        kstore<W>(okeys + at * W, move_to_top<W>(k, P.pa, P.pb, n));
+        ovals[at] = sh.vals[cur][e];
+    }
+    if (threadIdx.x == 0) sh.len[cur ^ 1] = len;
+    __syncthreads();
+}
+
+// The coefficient rules of the 16 output classes: shared factors, and per class (bit c) whether
+// it anticommutes and whether its partner's d-branch is negated.
+struct ClassCoefs {
+    Coef base;
+    unsigned anti, dneg;
+    __device__ __forceinline__ Coef of(unsigned c) const {
+        Coef k = base;
+        k.anti = (anti >> c) & 1u;
+        k.dneg = (dneg >> c) & 1u;
+        return k;
+    }
+};
+// The merged coefficient alone (merged() without radii or flags), branch-free: one product per
+// present branch and one addition.  An absent branch contributes +0, which changes nothing but
+// the sign of a zero result, and zero results are dropped.
+__device__ __forceinline__ double merged_value(const Coef &k, bool hasA, double va, bool hasD,
+                                               double vd) {
+    if (!k.anti) return va;
+    const double y = vd * k.s;
+    return (hasA ? va * k.c : 0.0) + (hasD ? (k.dneg ? -y : y) : 0.0);
+}
+
+// Lane l of every wave evaluates class l % 16; two ballots gather the masks.
+__device__ __forceinline__ ClassCoefs class_coefs(const PassMap &P) {
+    ClassCoefs r;
+    r.base.c = P.c;
+    r.base.s = P.s;
+    r.base.rc = P.rc;
+    r.base.rs = P.rs;
+    r.base.thr = P.threshold;
+    r.base.validated = P.flags & 1u;
+    r.base.anti = false;
+    r.base.dneg = false;
+    const unsigned c = lane_id() & 15u, cq = c ^ ((P.qt << 2) | P.qs);
+    const bool a = class_anti(c, P.qt, P.qs);
+    const bool neg = ((pauli_phase(P.qt, cq >> 2) + pauli_phase(P.qs, cq & 3u)) & 3u) != 1;
+    r.anti = (unsigned)__ballot(a) & 0xffffu;
+    r.dneg = (unsigned)__ballot(neg) & 0xffffu;
+    return r;
+}
+
+// Walk this thread's work [w0, w1) of a rotation: emit(c, key, hasA, va, hasD, vd) once per
+// output key, in ascending key order within every output class c.
+// Each segment the thread works on is announced first, begin(c, partner) (partner == c for a
+// commuting class), and its outputs then come as emit(side, c, key, hasA, va, hasD, vd), side 0
+// for class c and 1 for the partner.  `split` is the merge-path split of the first (partial)
+// segment, ~0u if not searched yet; the walk returns it, so the second walk need not search.
+template <int W, typename B, typename F>
+__device__ __forceinline__ uint32_t lds_rotation_walk(const LdsMap<W> &sh, int cur,
+                                                      unsigned anti, unsigned qq,
+                                                      const Key<W> &q, uint32_t w0, uint32_t w1,
+                                                      uint32_t split, B &&begin, F &&emit) {
+    const uint64_t *keys = sh.keys[cur];
+    const double *vals = sh.vals[cur];
+    const uint32_t *reg = sh.reg[cur];
+    uint32_t seg[17];
+#pragma unroll
+    for (int k = 0; k < 17; ++k) seg[k] = sh.seg[k];
+    uint32_t g = w0;
+    while (g < w1) {
+        unsigned c = 0;  // the segment holding g: the number of segment ends at or below g
+#pragma unroll
+        for (int k = 1; k < 16; ++k) c += seg[k] <= g ? 1u : 0u;
+        const uint32_t d0 = g - seg[c], d1 = min(w1, seg[c + 1]) - seg[c];
+        const uint32_t A0 = reg[c], nA = reg[c + 1] - reg[c];
+        if (!((anti >> c) & 1u)) {
+            begin(c, c);
+            for (uint32_t i = d0; i < d1; ++i)
+                emit(0, c, kload<W>(keys + (A0 + i) * W), true, vals[A0 + i], false, 0.0);
+        } else {
+            const unsigned cq = c ^ qq;
+            begin(c, cq);
+            const uint32_t B0 = reg[cq], nB = reg[cq + 1] - reg[cq];
+            auto Akey = [&](uint32_t i) { return kload<W>(keys + (A0 + i) * W); };
+            auto Bkey = [&](uint32_t j) { return kxor<W>(kload<W>(keys + (B0 + j) * W), q); };
+            // split at diagonal d0: the number of A terms among the first d0 positions of the
+            // merge (ties: A first), i.e. the first i with A[i] > T(B[d0 - i - 1])
+            uint32_t lo = d0 > nB ? d0 - nB : 0, hi = d0 < nA ? d0 : nA;
+            if (g == w0 && split != ~0u) {
+                lo = split;
+            } else {
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (klt<W>(Bkey(d0 - mid - 1), Akey(mid)))
+                        hi = mid;
+                    else
+                        lo = mid + 1;
+                }
+                if (g == w0) split = lo;
+            }
+            uint32_t i = lo, j = d0 - lo, used = 0;
+            const uint32_t limit = d1 - d0;
+            // a B term equal to the A term just before the split went out with it
+            if (i > 0 && j < nB && keq<W>(Akey(i - 1), Bkey(j))) {
+                ++j;
+                ++used;
+            }
+            // one step per output key, without branches on the step's kind: the A term (key x,
+            // its partner's d-branch if B holds the same lower digits) or the B term alone
+            // The current A and B terms stay in registers; each step loads the next of both
+            // (an exhausted run is read at a clamped index inside the LDS map and ignored)
+            // while it emits, and keeps whichever its run advanced to.
+            const uint32_t alast = nA ? nA - 1 : 0, blast = nB ? nB - 1 : 0;
+            Key<W> x = Akey(min(i, alast)), y = Bkey(min(j, blast));
+            double vx = vals[A0 + min(i, alast)], vy = vals[B0 + min(j, blast)];
+            while (used < limit) {
+                const bool hasA = i < nA, hasB = j < nB;
+                const bool takeA = hasA && (!hasB || !klt<W>(y, x));
+                const bool takeB = !takeA || (hasB && keq<W>(y, x));
+                const uint32_t in = min(i + 1, alast), jn = min(j + 1, blast);
+                const Key<W> xn = Akey(in), yn = Bkey(jn);
+                const double vxn = vals[A0 + in], vyn = vals[B0 + jn];
+                const Key<W> key = takeA ? x : y;
+                emit(0, c, key, takeA, vx, takeB, vy);
+                emit(1, cq, kxor<W>(key, q), takeB, vy, takeA, vx);
+                if (takeA) {
+                    x = xn;
+                    vx = vxn;
+                    ++i;
+                }
+                if (takeB) {
+                    y = yn;
+                    vy = vyn;
+                    ++j;
+                }
+                used += (takeA ? 1u : 0u) + (takeB ? 1u : 0u);
+            }
+        }
+        g = seg[c] + d1;
+    }
+    return split;
+}
+
+// Where a rotation in LDS reports: the ledger log and this map's records and slot capacity
+// (copied out of the kernel arguments, so the walks' closures hold a few registers).
+struct LdsSink {
+    LogDev *log;
+    uint64_t *log_count;
+    uint64_t log_cap;
+    tpgp_hip_record *rec;  // this map's records
+    uint64_t cap;          // this map's slot capacity
+    __device__ __forceinline__ void entry(uint64_t k0, uint64_t k1, double v, double aux,
+                                          uint32_t m, uint32_t rot, uint32_t kind) const {
+        unsigned long long at = atomicAdd((unsigned long long *)log_count, 1ull);
+        if (at < log_cap) {
+            LogDev &L = log[at];
+            L.key[0] = k0;
+            L.key[1] = k1;
+            L.value = v;
+            L.aux = aux;
+            L.map = m;
+            L.rotation = rot;
+            L.kind = kind;
+        }
+    }
+};
+
+// One rotation in LDS; false (nothing written anywhere, no record) when its output does not fit.
+// V: the rotation computes validated radii.
+template <int W, bool V>
+__device__ bool lds_rotation(const LdsSink &out_to, const PassMap &P, uint32_t m, uint32_t n,
+                             LdsMap<W> &sh, int cur) {
+    const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
+    const unsigned qq = (P.qt << 2) | P.qs;
+    const ClassCoefs kc = class_coefs(P);
+    const uint32_t *rin = sh.reg[cur];
+    if (threadIdx.x < 64) {  // work segments: sizes on lanes 0-15, one wave-wide scan
+        const unsigned c = threadIdx.x;
+        uint32_t size = 0;
+        if (c < 16) {
+            const uint32_t nc = rin[c + 1] - rin[c];
+            if (!((kc.anti >> c) & 1u)) size = nc;
+            else if ((c ^ qq) > c) size = nc + (rin[(c ^ qq) + 1] - rin[c ^ qq]);
+        }
+        const uint32_t incl = wave_scan_u32(size);
+        if (c < 16) sh.seg[c + 1] = incl;
+        if (c == 0) sh.seg[0] = 0;
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) sh.cnt[c][threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t work = sh.seg[16], per = (work + kLdsThreads - 1) / kLdsThreads;
+    const uint32_t w0 = min(work, threadIdx.x * per), w1 = min(work, w0 + per);
+    const bool want_log = (P.flags & 2u) != 0;
+    // The walks keep the segment's two classes' counts (then offsets) in registers: each class
+    // is in one segment, so a thread visits it in one stretch.
+    // walk 1: kept outputs per class (values only; nothing leaves the workgroup, so a rotation
+    // that turns out not to fit has no side effect)
+    unsigned ca = 16, cb = 16;  // the segment's classes (16: none yet)
+    uint32_t na = 0, nb = 0;
+    auto flush = [&]() {
+        if (ca < 16) sh.cnt[ca][threadIdx.x] = (uint16_t)na;
+        if (cb < 16 && cb != ca) sh.cnt[cb][threadIdx.x] = (uint16_t)nb;
+    };
+    const uint32_t split = lds_rotation_walk<W>(
+        sh, cur, kc.anti, qq, q, w0, w1, ~0u,
+        [&](unsigned c, unsigned partner) {
+            flush();
+            ca = c;
+            cb = partner;
+            na = nb = 0;
+        },
+        [&](unsigned side, unsigned c, const Key<W> &key, bool hasA, double va, bool hasD,
+            double vd) {
+            const double v = merged_value(kc.of(c), hasA, va, hasD, vd);
+            if (v == 0.0) return;  // exact cancellation: dropped silently
+            if (!kzero<W>(key) && __builtin_fabs(v) <= kc.base.thr) return;
+            if (side) ++nb;
+            else ++na;
+        });
+    flush();
+    __syncthreads();
+    uint32_t *rout = sh.reg[cur ^ 1];
+    lds_offsets<W>(sh, rout);
+    const uint32_t out = sh.total;
+    if (out > LdsMap<W>::kCap || out > out_to.cap) return false;  // uniform
+    // walk 2: values with radii, ledger partials and log; the kept outputs written in order
+    uint64_t *okeys = sh.keys[cur ^ 1];
+    double *ovals = sh.vals[cur ^ 1];
+    TileLedger L{};
+    bool bad = false;
+    (void)lds_rotation_walk<W>(
+        sh, cur, kc.anti, qq, q, w0, w1, split,
+        [&](unsigned c, unsigned partner) {
+            na = sh.cnt[c][threadIdx.x];
+            nb = sh.cnt[partner][threadIdx.x];
+        },
+        [&](unsigned side, unsigned c, const Key<W> &key, bool hasA, double va, bool hasD,
+            double vd) {
+            const Coef k = kc.of(c);
+            double v;
+            const uint64_t k0 = key.w[0], k1 = W > 1 ? key.w[W - 1] : 0;
+            if constexpr (V) {
+                double r;
+                merged(k, hasA, va, hasD, vd, v, r, bad);
+                if (r != 0.0) {
+                    L.radii++;
+                    L.rsq = upadd(L.rsq, upmul(r, r));
+                    if (want_log) out_to.entry(k0, k1, r, 0.0, m, P.rot_seq, 1);
+                }
+            } else {
+                // without radii merged() flags exactly a non-finite result of an
+                // anticommuting class: a non-finite product makes the sum non-finite too
+                v = merged_value(k, hasA, va, hasD, vd);
+                bad |= k.anti && !__builtin_isfinite(v);
+            }
+            if (v == 0.0) return;
+            if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) {
+                L.dropped++;
+                L.l1 = upadd(L.l1, __builtin_fabs(v));
+                L.sq = upadd(L.sq, upmul(v, v));
+                if (want_log) out_to.entry(k0, k1, __builtin_fabs(v), v, m, P.rot_seq, 0);
+                return;
+            }
+            const uint32_t at = side ? nb++ : na++;
+            kstore<W>(okeys + at * W, key);
+            ovals[at] = v;
+        });
+    L.nonfinite = bad ? 1u : 0u;
+    const TileLedger T = lds_ledger(L, sh.red);
+    if (threadIdx.x == 0) {
+        sh.len[cur ^ 1] = out;
+        tpgp_hip_record rec;
+        rec.len_in = rin[16];
+        rec.len_out = out;
+        rec.dropped = T.dropped;
+        rec.l1_up = T.l1;
+        rec.sq_up = T.sq;
+        rec.rad_sq_up = T.rsq;
+        rec.radii = T.radii;
+        rec.executed = rin[16] > 0;
+        rec.nonfinite = T.nonfinite;
+        rec.overflow = 0;
+        rec.rotation = P.rot_seq;
+        out_to.rec[P.rec_slot] = rec;
+    }
+    __syncthreads();
+    return true;
+}
+
+// One workgroup per small map (maps[blockIdx.x]) runs the chunk's passes with the map in LDS:
+// descriptor k of map m is ops[k * M + m]; descriptors without kPassSmall belong to other maps'
+// paths.  small_resume[m] receives the first pass left to k_small (n_ops when none is).
+template <int W>
+__global__ __launch_bounds__(kLdsThreads) void k_small_lds(BatchDev b, const PassMap *ops,
+                                                           uint32_t n_ops, const uint32_t *maps,
+                                                           uint32_t n_small) {
+    __shared__ LdsMap<W> sh;
+    if (blockIdx.x >= n_small) return;
+    const uint32_t m = maps[blockIdx.x], n = b.slot[m].n_spins;
+    uint32_t k = 0;
+    while (k < n_ops && !(ops[(uint64_t)k * b.n_maps + m].flags & kPassSmall)) ++k;
+    if (k == n_ops || b.overflow[m] != 0 ||
+        !lds_load<W>(b, ops[(uint64_t)k * b.n_maps + m].in_buf, m, sh)) {
+        if (threadIdx.x == 0) b.small_resume[m] = k;
+        return;
+    }
+    const LdsSink out_to{b.log, b.log_count, b.log_cap, b.rec + (uint64_t)m * b.max_rec,
+                         b.slot[m].cap};
+    int cur = 0;
+    uint32_t last_out = 0;
+    for (; k < n_ops; ++k) {
+        // a private copy: the passes read its fields inside their loops, and a reference to
+        // global memory would be reloaded there (LDS stores and barriers stop the hoisting)
+        const PassMap P = ops[(uint64_t)k * b.n_maps + m];
+        if (!(P.flags & kPassSmall)) continue;
+        bool done = true;
+        if (P.flags & kPassPartition)
+            lds_partition<W>(P, n, sh, cur);
+        else if (P.flags & 1u)
+            done = lds_rotation<W, true>(out_to, P, m, n, sh, cur);
+        else
+            done = lds_rotation<W, false>(out_to, P, m, n, sh, cur);
+        if (!done) {  // the pass's input, for the global kernel
+            lds_store<W>(b, P.in_buf, m, sh, cur);
+            if (threadIdx.x == 0) b.small_resume[m] = k;
+            return;
+        }
+        cur ^= 1;
+        last_out = P.out_buf;
+    }
+    lds_store<W>(b, last_out, m, sh, cur);
+    if (threadIdx.x == 0) b.small_resume[m] = n_ops;
+}
+
+// The global-memory small-map kernel: one workgroup per small map runs the passes from
+// small_resume[m] on (the LDS kernel, launched just before, ran the ones before it).
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_small(BatchDev b, const PassMap *ops, uint32_t n_ops,
                                                   const uint32_t *maps, uint32_t n_small) {
     __shared__ SmallShared<W> sh;
     if (blockIdx.x >= n_small) return;
     const uint32_t m = maps[blockIdx.x];
-    for (uint32_t k = 0; k < n_ops; ++k) {
+    for (uint32_t k = b.small_resume[m]; k < n_ops; ++k) {
         const PassMap &P = ops[(uint64_t)k * b.n_maps + m];
         if (!(P.flags & kPassSmall)) continue;
         if (P.flags & kPassPartition)
@@ -3371,8 +3883,10 @@ struct tpgp_hip_batch {
     // path choice (per rotate_many call): modelled costs of one rotation, or a fixed bound
     double cost_launch_us = 45.0;  // multi-kernel chain of one rotation (launches, latency)
     double cost_multi_ns = 0.03;   // multi-kernel streaming cost per term
-    double cost_small_ns = 5.0;    // one persistent workgroup, per term of its map (W=1;
-                                   // twice that for two-word keys)
+    double cost_small_ns = 5.0;    // one persistent workgroup on global memory, per term of its
+                                   // map (W=1; twice that for two-word keys)
+    double cost_lds_us = 3.0;      // one persistent workgroup with the map in LDS: per rotation
+    double cost_lds_ns = 1.0;      // and per term
     bool small_fixed = false;      // TPGP_HIP_SMALL_LIMIT given: length bound instead
     bool fuse = false;             // TPGP_HIP_FUSE=1: fused passes over runs of one bond
     uint64_t small_limit = 0;
@@ -3475,6 +3989,7 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     ALLOC(d.chunk_led, b->max_tiles / kChunkTiles + M + 1);
     ALLOC(d.counter, 4);
     ALLOC(d.overflow, M);
+    ALLOC(d.small_resume, M);
     ALLOC(d.rec, (size_t)M * b->max_rec);
     ALLOC(d.rd, (size_t)M * b->max_rd * 64);
     d.log_cap = b->log_entries;
@@ -3583,13 +4098,19 @@ void choose_paths(tpgp_hip_batch *b, const tpgp_hip_rotation *req, size_t n_rot)
               [&](uint32_t x, uint32_t y) { return b->maps[x].est_len < b->maps[y].est_len; });
     double rest = 0;  // terms left on the multi-kernel path
     for (uint32_t m : idx) rest += (double)b->maps[m].est_len;
-    const double resident = 4.0 * b->ctx->cus;
+    // one small-map workgroup per CU (its LDS holds a whole map); a map that fits the LDS
+    // capacity with room to grow runs there, larger ones on global memory
+    const double resident = (double)b->ctx->cus;
+    const double lds_cap = b->W == 1 ? LdsCap<1>::value : LdsCap<2>::value;
+    auto small_us = [&](double len) {
+        if (len <= 0.8 * lds_cap) return b->cost_lds_us + b->cost_lds_ns * len * 1e-3;
+        return b->cost_small_ns * (b->W == 2 ? 2.0 : 1.0) * len * 1e-3;
+    };
     auto cost = [&](size_t k, double large_terms) {  // first k maps small
         double t = 0;
         if (k < idx.size()) t += b->cost_launch_us + b->cost_multi_ns * large_terms * 1e-3;
         if (k > 0)
-            t += b->cost_small_ns * (b->W == 2 ? 2.0 : 1.0) *
-                 (double)b->maps[idx[k - 1]].est_len * 1e-3 * std::ceil((double)k / resident);
+            t += small_us((double)b->maps[idx[k - 1]].est_len) * std::ceil((double)k / resident);
         return t;
     };
     size_t best = 0;
@@ -4383,12 +4904,19 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
             return hip_fail(e, "HIP descriptor staging", error, cap);
         ProfileScope ps(b, TPGP_HIP_PROFILE_SMALL);
         const unsigned ns = (unsigned)small_ids.size();
-        if (b->W == 1)
+        const uint32_t n_ops = (uint32_t)(used / M);
+        // the LDS kernel first; the global one finishes what it left (small_resume)
+        if (b->W == 1) {
+            hipLaunchKernelGGL(k_small_lds<1>, dim3(ns), dim3(kLdsThreads), 0, b->ctx->stream,
+                               b->dev, (const PassMap *)d0, n_ops, d_small, ns);
             hipLaunchKernelGGL(k_small<1>, dim3(ns), dim3(kBlock), 0, b->ctx->stream, b->dev,
-                               (const PassMap *)d0, (uint32_t)(used / M), d_small, ns);
-        else
+                               (const PassMap *)d0, n_ops, d_small, ns);
+        } else {
+            hipLaunchKernelGGL(k_small_lds<2>, dim3(ns), dim3(kLdsThreads), 0, b->ctx->stream,
+                               b->dev, (const PassMap *)d0, n_ops, d_small, ns);
             hipLaunchKernelGGL(k_small<2>, dim3(ns), dim3(kBlock), 0, b->ctx->stream, b->dev,
-                               (const PassMap *)d0, (uint32_t)(used / M), d_small, ns);
+                               (const PassMap *)d0, n_ops, d_small, ns);
+        }
     }
     if ((e = hipGetLastError()) != hipSuccess)
         return hip_fail(e, "HIP rotation launch", error, cap);
