@@ -1046,6 +1046,7 @@ typedef struct {
     double *fid, *rad;
     size_t *act;
     size_t cap;
+    size_t want; /* capacity asked for by an overflowing step */
 } fleet_slot;
 
 struct tpgp_fleet {
@@ -1063,6 +1064,8 @@ struct tpgp_fleet {
     double *scratch;
     size_t scratch_cap;
     tpgp_evolution_counters c;
+    uint64_t budget; /* device bytes the slots may hold together (0: no limit) */
+    size_t cap0;     /* every slot's capacity at open, to which tpgp_fleet_trim returns */
 };
 
 static void slot_clear(fleet_slot *q) {
@@ -1111,6 +1114,7 @@ tpgp_status tpgp_fleet_open(tpgp_fleet **out, void *context, size_t n_words, siz
     }
     f->W = n_words;
     f->M = n_slots;
+    f->cap0 = slot_capacity;
     f->slot = calloc(n_slots, sizeof(*f->slot));
     size_t *spins = malloc(n_slots * sizeof(size_t));
     size_t *caps = malloc(n_slots * sizeof(size_t));
@@ -1241,6 +1245,108 @@ static int cmp_entry_map(const void *a, const void *b) {
     return cmp_entry(a, b);
 }
 
+static uint64_t slot_bytes_at(const tpgp_fleet *f, size_t cap) {
+    return 3ull * cap * (8 * f->W + 8);
+}
+
+/* Grow the slots with want[m] > their capacity, taking them in ascending order of the bytes
+ * they add while the fleet stays within its budget.  One re-layout for all the granted slots;
+ * if the device cannot hold them together, one slot at a time, smallest first.  A slot that is
+ * refused (budget) or cannot be allocated gets refused[m] = 1 and keeps its capacity and its
+ * committed state; every other slot is unaffected.  Returns the number of refused slots. */
+static size_t grow_slots(tpgp_fleet *f, const size_t *want, int *refused) {
+    const size_t M = f->M;
+    size_t *order = malloc(M * sizeof(size_t)), n = 0, nref = 0;
+    size_t *caps = calloc(M, sizeof(size_t));
+    char err[512];
+    if (!order || !caps) {
+        for (size_t m = 0; m < M; m++)
+            if (want[m] > f->slot[m].cap) refused[m] = 1, nref++;
+        free(order);
+        free(caps);
+        return nref;
+    }
+    uint64_t held = 0; /* every slot's arena share, free ones included: the batch holds it */
+    for (size_t m = 0; m < M; m++) {
+        held += slot_bytes_at(f, f->slot[m].cap);
+        if (want[m] > f->slot[m].cap) order[n++] = m;
+    }
+    for (size_t a = 1; a < n; a++) /* insertion sort by added bytes (few slots grow at once) */
+        for (size_t k = a; k > 0 && want[order[k]] - f->slot[order[k]].cap <
+                                        want[order[k - 1]] - f->slot[order[k - 1]].cap; k--) {
+            size_t t = order[k];
+            order[k] = order[k - 1];
+            order[k - 1] = t;
+        }
+    size_t granted = 0;
+    for (size_t a = 0; a < n; a++) {
+        const size_t m = order[a];
+        const uint64_t add = slot_bytes_at(f, want[m]) - slot_bytes_at(f, f->slot[m].cap);
+        if (f->budget && held + add > f->budget) {
+            refused[m] = 1;
+            nref++;
+            continue;
+        }
+        held += add;
+        caps[m] = want[m];
+        order[granted++] = m;
+    }
+    if (granted && tpgp_hip_batch_reserve_many(f->batch, caps, err, sizeof(err))) {
+        note("reserve", err);
+        /* together they do not fit: one at a time, smallest first */
+        for (size_t a = 0; a < granted; a++) {
+            const size_t m = order[a];
+            memset(caps, 0, M * sizeof(size_t));
+            caps[m] = want[m];
+            if (tpgp_hip_batch_reserve_many(f->batch, caps, err, sizeof(err))) {
+                note("reserve", err);
+                refused[m] = 1;
+                nref++;
+            } else {
+                f->slot[m].cap = want[m];
+            }
+        }
+    } else {
+        for (size_t a = 0; a < granted; a++) f->slot[order[a]].cap = want[order[a]];
+    }
+    free(order);
+    free(caps);
+    return nref;
+}
+
+tpgp_status tpgp_fleet_set_budget(tpgp_fleet *f, uint64_t bytes) {
+    if (!f) return TPGP_EINVAL;
+    f->budget = bytes;
+    return TPGP_OK;
+}
+
+tpgp_status tpgp_fleet_trim(tpgp_fleet *f) {
+    if (!f) return TPGP_EINVAL;
+    char err[512];
+    size_t *caps = calloc(f->M, sizeof(size_t));
+    if (!caps) return TPGP_ENOMEM;
+    int any = 0;
+    for (size_t m = 0; m < f->M; m++) {
+        if (f->slot[m].state != TPGP_SLOT_FREE || f->slot[m].cap <= f->cap0) continue;
+        /* a free slot still holds its last map: empty it so the smaller slot can take it */
+        int hs = tpgp_hip_batch_upload(f->batch, m, NULL, NULL, 0, err, sizeof(err));
+        if (hs) {
+            note("trim", err);
+            free(caps);
+            return map_status(hs);
+        }
+        caps[m] = f->cap0;
+        any = 1;
+    }
+    int hs = any ? tpgp_hip_batch_resize(f->batch, caps, err, sizeof(err)) : 0;
+    if (hs) note("trim", err);
+    else
+        for (size_t m = 0; m < f->M; m++)
+            if (caps[m]) f->slot[m].cap = caps[m];
+    free(caps);
+    return hs ? map_status(hs) : TPGP_OK;
+}
+
 tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
     if (!f) return TPGP_EINVAL;
     char err[512];
@@ -1258,31 +1364,27 @@ tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
     int any = 0;
     for (size_t m = 0; m < M; m++) any |= part[m] = f->slot[m].state == TPGP_SLOT_RUNNING;
     /* grow slots ahead of need (one re-layout for all of them): a slot whose committed map
-     * fills half of it gets room for three times that */
+     * fills more than half of it gets room for three times that.  Growth ahead is an
+     * optimisation: a slot it cannot serve (budget, device memory) keeps its capacity and grows
+     * when a step actually overflows it. */
     if (any) {
         size_t *caps = calloc(M, sizeof(size_t));
+        int *refused = calloc(M, sizeof(int));
         int grow_any = 0;
-        if (!caps) s = TPGP_ENOMEM;
-        for (size_t m = 0; m < M && caps; m++) {
+        if (!caps || !refused) s = TPGP_ENOMEM;
+        for (size_t m = 0; m < M && caps && refused; m++) {
             size_t len = 0;
             if (!part[m] || tpgp_hip_batch_info(f->batch, m, &len, NULL)) continue;
-            if (2 * len + 4096 > f->slot[m].cap) {
+            if (2 * len > f->slot[m].cap) {
                 size_t cap = 3 * len + 4096;
                 if (cap < 2 * f->slot[m].cap) cap = 2 * f->slot[m].cap;
                 caps[m] = cap;
                 grow_any = 1;
             }
         }
-        if (grow_any) {
-            if (tpgp_hip_batch_reserve_many(f->batch, caps, err, sizeof(err))) {
-                note("reserve", err);
-                s = TPGP_ENOMEM;
-            } else {
-                for (size_t m = 0; m < M; m++)
-                    if (caps[m]) f->slot[m].cap = caps[m];
-            }
-        }
+        if (grow_any && !s) (void)grow_slots(f, caps, refused);
         free(caps);
+        free(refused);
     }
     while (any && !s) {
         /* readouts of the committed state, one call per readout symbol */
@@ -1416,17 +1518,19 @@ tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
             uint64_t up = 0, io = 0, ex = 0;
             tpgp_status fs = fleet_fold(f, m, f->log + l0, l_at - l0, &trial, &grow, &need, &up,
                                         &io, &ex);
+            /* the sample of the committed state stands whatever becomes of this step (a slot
+             * refused its growth below fails with it as its last sample) */
+            q->fid[q->done] = mid;
+            q->rad[q->done] = rad;
+            q->act[q->done] = read_len;
             if (!fs && grow) {
                 size_t cap = need + need / 2 + 1024;
                 if (cap < 2 * q->cap) cap = 2 * q->cap;
-                q->cap = cap;
+                q->want = cap;
                 part[m] = 1; /* rerun this slot's step after growing */
                 regrow = 1;
                 continue;
             }
-            q->fid[q->done] = mid;
-            q->rad[q->done] = rad;
-            q->act[q->done] = read_len;
             if (!fs && rs) fs = rs;
             if (fs) {
                 q->status = fs;
@@ -1447,26 +1551,30 @@ tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
             break;
         }
         any = 0;
-        if (regrow) { /* every overflowing slot grows in one re-layout, then runs again */
+        if (regrow) { /* the overflowing slots grow (one re-layout), then run again; a slot
+                       * that cannot grow fails alone with ENOMEM at its last accepted step */
             size_t *caps = calloc(M, sizeof(size_t));
-            if (!caps) {
+            int *refused = calloc(M, sizeof(int));
+            if (!caps || !refused) {
+                free(caps);
+                free(refused);
                 s = TPGP_ENOMEM;
                 break;
             }
             for (size_t m = 0; m < M; m++)
-                if (part[m]) caps[m] = f->slot[m].cap;
-            int rh = tpgp_hip_batch_reserve_many(f->batch, caps, err, sizeof(err));
+                if (part[m]) caps[m] = f->slot[m].want;
+            (void)grow_slots(f, caps, refused);
             free(caps);
             for (size_t m = 0; m < M; m++) {
                 if (!part[m]) continue;
-                if (rh) {
-                    note("reserve", err);
-                    f->slot[m].status = map_status(rh);
+                if (refused[m]) {
+                    f->slot[m].status = TPGP_ENOMEM;
                     f->slot[m].state = TPGP_SLOT_FAILED;
                     part[m] = 0;
                 }
                 any |= part[m];
             }
+            free(refused);
         }
     }
     if (running) {

@@ -52,9 +52,6 @@ constexpr uint32_t kSumBlocks = 512;     // blocks of the sum-of-squares reducti
 // PassMap.flags bit: the map's commuting classes cannot lose a key to this rotation's
 // truncation (every coefficient there already exceeds the threshold), so they pass unchanged
 constexpr uint32_t kPassCommutingKept = 16u;
-constexpr int kFuseMax = 3;              // rotations fused into one pass (one bond: XX, YY, ZZ)
-constexpr int kSampleStride = 64;        // fused passes: one sample per 64 elements of a class
-constexpr int kSamplesPerTile = 12;      // fused tiles hold < (12 + 4) * 64 = 1024 elements
 constexpr int kPartRounds = 8;     // partition: 8 rounds of 256 elements (the default)
 constexpr int kPartTile = kBlock * kPartRounds;
 constexpr int kPartRoundsMin = 4;  // smallest tunable partition tile: 4 rounds
@@ -348,20 +345,6 @@ struct BatchDev {
     uint64_t *chunk_base;     // [maxC] output offset of every chunk within its map
     uint32_t *chunk_sum;      // [maxC] kept outputs of every chunk
     TileLedger *chunk_led;    // [maxC] ledger partials of every chunk
-    // fused passes
-    uint64_t *fs_start;       // [M*16+1] sample offsets per (map, class)
-    uint64_t *fg_tiles;       // [M*16+1] tile offsets per (map, group)
-    uint64_t *fg_chunks;      // [M*16+1] scan-chunk offsets per (map, group)
-    uint64_t *fsamp;          // [max_samples*W] sampled lower digits
-    uint64_t *ftpos;          // [max_ftiles*4] tile start per member (class-relative)
-    struct FusedTile *ftile;  // [max_ftiles]
-    uint64_t *ftoff;          // [max_ftiles*4] output offset per member within the chunk
-    struct FusedChunk *fchunk;  // [max_fchunks]
-    uint64_t *fchbase;        // [max_fchunks*4] output offset per member of every chunk
-    uint64_t *fcls_total;     // [M*16] outputs per class
-    TileLedger *fg_led;       // [M*16*kFuseMax] ledger per (map, group, rotation)
-    uint64_t *fg_after;       // [M*16*kFuseMax] length per (map, group, rotation)
-    uint64_t max_samples, max_ftiles;
     uint32_t *counter;        // [4]
     uint32_t *overflow;       // [M]
     uint32_t *small_resume;   // [M] first pass the global small-map kernel runs (the LDS kernel
@@ -1659,863 +1642,6 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
                 gv1[at] = v;
             }
         }
-        __syncthreads();
-    }
-}
-
-// ------------------------------------------------------------------------------------------
-// Fused pass: a run of up to kFuseMax rotations on one site pair (on the top two digits) in one
-// pass over the map.  The run's class XOR masks span a subgroup V of the 16 classes (|V| = G,
-// at most 4), so the classes fall into cosets ("groups") of V and every rotation of the run maps
-// a key to keys of its own group with the same lower digits u.  Tiles hold all members of one
-// group over a range of u; per u the rotations are applied in registers one after another,
-// each followed by the reference's truncation, so the result and every per-rotation record
-// equal those of the separate rotations.
-// ------------------------------------------------------------------------------------------
-
-struct FusedMap {  // per map, per fused pass (staged from the host)
-    uint32_t active, in_buf, out_buf, R;
-    uint32_t G, n_groups;
-    uint8_t cls_group[16], cls_member[16], members[16][4];
-    uint8_t tpair[kFuseMax], pad8[8 - kFuseMax];  // rotation r pairs member k with k ^ tpair[r]
-    uint32_t qt[kFuseMax], qs[kFuseMax], flags[kFuseMax], rec_slot[kFuseMax], rot_seq[kFuseMax];
-    double c[kFuseMax], s[kFuseMax], rc[kFuseMax], rs[kFuseMax], thr[kFuseMax];
-};
-struct FusedTile {  // per tile: kept outputs per member, lengths after each rotation, ledgers
-    uint32_t cnt[4];
-    uint32_t after[kFuseMax];
-    uint32_t nonfinite, pad;
-    TileLedger led[kFuseMax];
-};
-struct FusedChunk {
-    uint64_t cnt[4];
-    uint64_t after[kFuseMax];
-    TileLedger led[kFuseMax];
-};
-
-template <int W> __device__ __forceinline__ Key<W> top_mask(uint32_t n) {
-    return kxor<W>(kdigit_mask<W>(n - 1, 3u), kdigit_mask<W>(n - 2, 3u));
-}
-template <int W> __device__ __forceinline__ Key<W> kandnot(const Key<W> &a, const Key<W> &m) {
-    Key<W> r;
-#pragma unroll
-    for (int i = 0; i < W; ++i) r.w[i] = a.w[i] & ~m.w[i];
-    return r;
-}
-template <int W> __device__ __forceinline__ Key<W> kor(const Key<W> &a, const Key<W> &b) {
-    Key<W> r;
-#pragma unroll
-    for (int i = 0; i < W; ++i) r.w[i] = a.w[i] | b.w[i];
-    return r;
-}
-template <int W> __device__ __forceinline__ Key<W> class_bits(uint32_t n, uint32_t c) {
-    return kxor<W>(kdigit_mask<W>(n - 1, c >> 2), kdigit_mask<W>(n - 2, c & 3u));
-}
-
-// Plan (one block): sample offsets per (map, class), tiles per (map, group) and scan chunks per
-// (map, group).
-__global__ __launch_bounds__(kBlock) void k_fuse_plan(BatchDev b, const FusedMap *fm) {
-    __shared__ unsigned scratch[8];
-    const uint32_t M = b.n_maps, n = M * 16;
-    uint64_t carry = 0;
-    for (uint32_t base = 0; base < n; base += kBlock) {
-        const uint32_t t = base + threadIdx.x;
-        unsigned cnt = 0;
-        if (t < n) {
-            const FusedMap &F = fm[t >> 4];
-            if (F.active) {
-                const uint64_t *reg = b.regions + ((uint64_t)(t >> 4) * 3 + F.in_buf) * 17;
-                const uint64_t ne = reg[(t & 15) + 1] - reg[t & 15];
-                cnt = (unsigned)((ne + kSampleStride - 1) / kSampleStride);
-            }
-        }
-        unsigned tot;
-        const unsigned ex = block_exclusive_scan(cnt, scratch, &tot);
-        if (t < n) b.fs_start[t] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) b.fs_start[n] = carry;
-    carry = 0;
-    uint64_t ccarry = 0;
-    for (uint32_t base = 0; base < n; base += kBlock) {
-        const uint32_t t = base + threadIdx.x;
-        unsigned tiles = 0, chunks = 0;
-        if (t < n) {
-            const uint32_t m = t >> 4, g = t & 15;
-            const FusedMap &F = fm[m];
-            if (F.active && g < F.n_groups) {
-                const uint64_t *reg = b.regions + ((uint64_t)m * 3 + F.in_buf) * 17;
-                uint64_t ns = 0;
-                for (uint32_t k = 0; k < F.G; ++k) {
-                    const uint32_t c = F.members[g][k];
-                    ns += (reg[c + 1] - reg[c] + kSampleStride - 1) / kSampleStride;
-                }
-                tiles = (unsigned)((ns + kSamplesPerTile - 1) / kSamplesPerTile + 1);
-                chunks = (tiles + kChunkTiles - 1) / kChunkTiles;
-            }
-        }
-        unsigned tot, ctot;
-        const unsigned ex = block_exclusive_scan(tiles, scratch, &tot);
-        const unsigned cex = block_exclusive_scan(chunks, scratch, &ctot);
-        if (t < n) {
-            b.fg_tiles[t] = carry + ex;
-            b.fg_chunks[t] = ccarry + cex;
-        }
-        carry += tot;
-        ccarry += ctot;
-    }
-    if (threadIdx.x == 0) {
-        b.fg_tiles[n] = carry;
-        b.fg_chunks[n] = ccarry;
-        b.counter[2] = carry > b.max_ftiles || b.fs_start[n] > b.max_samples ? 1u : 0u;
-    }
-}
-
-// Samples: the lower digits u of every kSampleStride-th element of every class.
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_fuse_samples(BatchDev b, const FusedMap *fm) {
-    const uint32_t n16 = b.n_maps * 16;
-    if (b.counter[2]) return;
-    const uint64_t total = b.fs_start[n16];
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * kBlock) {
-        const uint32_t t = find_bucket(b.fs_start, n16, i);
-        const uint32_t m = t >> 4, c = t & 15;
-        const FusedMap &F = fm[m];
-        const uint64_t *reg = b.regions + ((uint64_t)m * 3 + F.in_buf) * 17;
-        const uint64_t e = reg[c] + (i - b.fs_start[t]) * kSampleStride;
-        const uint64_t *keys = b.arena[F.in_buf].keys + b.slot[m].off * W;
-        kstore<W>(b.fsamp + i * W,
-                  kandnot<W>(kload<W>(keys + e * W), top_mask<W>(b.slot[m].n_spins)));
-    }
-}
-
-// Boundaries: a sample whose rank in its group's merged sample order ((u, member) order) is a
-// multiple of kSamplesPerTile starts a tile at u: in every member class, the tile begins at the
-// first element with lower digits >= u.
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_fuse_bounds(BatchDev b, const FusedMap *fm) {
-    const uint32_t n16 = b.n_maps * 16;
-    if (b.counter[2]) return;
-    const uint64_t total = b.fs_start[n16];
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * kBlock) {
-        const uint32_t t = find_bucket(b.fs_start, n16, i);
-        const uint32_t m = t >> 4, c = t & 15;
-        const FusedMap &F = fm[m];
-        const uint32_t g = F.cls_group[c], k = F.cls_member[c];
-        const uint64_t ti = i - b.fs_start[t];
-        const Key<W> u = kload<W>(b.fsamp + i * W);
-        uint64_t rank = ti;
-        uint64_t lt[4] = {0, 0, 0, 0};  // samples of member j below u
-        for (uint32_t j = 0; j < F.G; ++j) {
-            if (j == k) continue;
-            const uint32_t cj = F.members[g][j];
-            const uint64_t s0 = b.fs_start[m * 16 + cj], s1 = b.fs_start[m * 16 + cj + 1];
-            uint64_t lo = s0, hi = s1;  // first sample >= u
-            while (lo < hi) {
-                const uint64_t mid = (lo + hi) >> 1;
-                if (klt<W>(kload<W>(b.fsamp + mid * W), u))
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            const uint64_t below = lo - s0;
-            const bool eq = lo < s1 && keq<W>(kload<W>(b.fsamp + lo * W), u);
-            lt[j] = below;
-            rank += below + ((j < k && eq) ? 1 : 0);  // equal u: lower member first
-        }
-        if (rank % kSamplesPerTile) continue;
-        const uint64_t T = b.fg_tiles[m * 16 + g] + rank / kSamplesPerTile + 1;
-        if (T >= b.fg_tiles[m * 16 + g + 1]) {  // inconsistent ranks: report, write nothing
-            atomicOr(&b.counter[1], 1u);
-            continue;
-        }
-        const uint64_t *reg = b.regions + ((uint64_t)m * 3 + F.in_buf) * 17;
-        const uint64_t *keys = b.arena[F.in_buf].keys + b.slot[m].off * W;
-        const Key<W> tmask = top_mask<W>(b.slot[m].n_spins);
-        for (uint32_t j = 0; j < F.G; ++j) {
-            uint64_t pos;
-            if (j == k) {
-                pos = ti * kSampleStride;
-            } else {
-                const uint32_t cj = F.members[g][j];
-                const uint64_t A0 = reg[cj], ne = reg[cj + 1] - reg[cj];
-                uint64_t lo = lt[j] ? (lt[j] - 1) * kSampleStride + 1 : 0;
-                uint64_t hi = lt[j] * kSampleStride < ne ? lt[j] * kSampleStride : ne;
-                while (lo < hi) {  // first element with u >= the boundary
-                    const uint64_t mid = (lo + hi) >> 1;
-                    if (klt<W>(kandnot<W>(kload<W>(keys + (A0 + mid) * W), tmask), u))
-                        lo = mid + 1;
-                    else
-                        hi = mid;
-                }
-                pos = lo;
-            }
-            b.ftpos[T * 4 + j] = pos;
-        }
-    }
-}
-
-// A fused tile's element ranges (per member) from the boundaries; false on an inconsistency.
-struct FuseTileGeom {
-    uint32_t m, g, G, total;
-    uint64_t T, bt;
-    uint64_t A0[4], st[4], en[4];
-    uint32_t off[5];
-};
-__device__ __forceinline__ bool fuse_geom(const BatchDev &b, const FusedMap *fm, uint64_t T,
-                                          FuseTileGeom &x) {
-    const uint32_t n16 = b.n_maps * 16;
-    const uint32_t t = find_bucket(b.fg_tiles, n16, T);
-    x.m = t >> 4;
-    x.g = t & 15;
-    x.T = T;
-    const FusedMap &F = fm[x.m];
-    x.G = F.G;
-    x.bt = T - b.fg_tiles[t];
-    const uint64_t ntg = b.fg_tiles[t + 1] - b.fg_tiles[t];
-    const uint64_t *reg = b.regions + ((uint64_t)x.m * 3 + F.in_buf) * 17;
-    bool ok = true;
-    x.off[0] = 0;
-    for (uint32_t j = 0; j < 4; ++j) {
-        if (j < x.G) {
-            const uint32_t cj = F.members[x.g][j];
-            const uint64_t ne = reg[cj + 1] - reg[cj];
-            x.A0[j] = reg[cj];
-            x.st[j] = x.bt == 0 ? 0 : b.ftpos[T * 4 + j];
-            x.en[j] = x.bt + 1 < ntg ? b.ftpos[(T + 1) * 4 + j] : ne;
-            ok = ok && x.st[j] <= x.en[j] && x.en[j] <= ne;
-        } else {
-            x.A0[j] = x.st[j] = x.en[j] = 0;
-        }
-        const uint64_t sz = x.en[j] - x.st[j];
-        ok = ok && x.off[j] + sz <= (uint64_t)kMergeTile;
-        x.off[j + 1] = ok ? x.off[j] + (uint32_t)sz : x.off[j];
-    }
-    x.total = x.off[4];  // members beyond G are empty
-    return ok;
-}
-
-// The run's coefficients and per-(rotation, member) facts of one tile's group, in LDS.
-template <int W> struct FuseTab {
-    uint32_t R, G, map, ident;  // ident: member index of class 0 in this group, 4 if absent
-    uint32_t t[kFuseMax];       // rotation r pairs member k with member k ^ t[r]
-    double c[kFuseMax], s[kFuseMax], rc[kFuseMax], rs[kFuseMax], thr[kFuseMax];
-    uint32_t flags[kFuseMax], rot_seq[kFuseMax];
-    uint8_t anti[kFuseMax][4], partner[kFuseMax][4], dneg[kFuseMax][4];
-    uint64_t cbits[4][W];  // key bits of every member's class
-};
-template <int W> struct FuseShared {
-    FuseTab<W> tab;
-    uint64_t keys[kMergeTile * W];
-    double vals[kMergeTile];
-    uint16_t ia[kMergeTile], ib[kMergeTile];
-    uint8_t tag[kMergeTile];
-    unsigned scratch[8];
-    double red[3][kBlock / 64];
-    uint64_t redn[2][kBlock / 64];
-    unsigned redc[kFuseMax + 5][kBlock / 64];
-    uint64_t cbase[4];
-    uint64_t wsum[kBlock / 64];
-};
-
-template <int W>
-__device__ __forceinline__ void fuse_table(const FusedMap &F, const FuseTileGeom &x, uint32_t n,
-                                           FuseShared<W> &sh) {
-    FuseTab<W> &t = sh.tab;
-    const uint32_t i = threadIdx.x;
-    if (i == 0) {
-        t.R = F.R;
-        t.G = F.G;
-        t.map = x.m;
-        t.ident = 4;
-        for (uint32_t k = 0; k < F.G; ++k)
-            if (F.members[x.g][k] == 0) t.ident = k;
-    }
-    if (i < (uint32_t)kFuseMax) {
-        t.t[i] = F.tpair[i];
-        t.c[i] = F.c[i];
-        t.s[i] = F.s[i];
-        t.rc[i] = F.rc[i];
-        t.rs[i] = F.rs[i];
-        t.thr[i] = F.thr[i];
-        t.flags[i] = F.flags[i];
-        t.rot_seq[i] = F.rot_seq[i];
-    }
-    if (i < (uint32_t)kFuseMax * 4) {
-        const uint32_t r = i >> 2, j = i & 3;
-        const uint32_t cj = F.members[x.g][j], qt = F.qt[r], qs = F.qs[r];
-        const uint32_t cp = cj ^ ((qt << 2) | qs);
-        t.anti[r][j] = (j < F.G && r < F.R) ? (uint8_t)class_anti(cj, qt, qs) : 0;
-        t.partner[r][j] = (uint8_t)F.cls_member[cp & 15];
-        t.dneg[r][j] =
-            (uint8_t)((((pauli_phase(qt, cp >> 2) + pauli_phase(qs, cp & 3u)) & 3u) != 1) ? 1 : 0);
-    }
-    if (i < 4) {
-        const Key<W> cb = class_bits<W>(n, F.members[x.g][i]);
-        for (int w = 0; w < W; ++w) t.cbits[i][w] = cb.w[w];
-    }
-}
-
-// Merge two sorted index lists X (nx) and Y (ny) of the tile by (u, member), X's members first
-// on equal u, into out; every thread produces its contiguous share of the output.
-template <int W>
-__device__ __forceinline__ void lds_merge(const FuseShared<W> &sh, const Key<W> &tmask,
-                                          const uint16_t *X, uint32_t nx, const uint16_t *Y,
-                                          uint32_t ny, uint16_t *out) {
-    const uint32_t total = nx + ny;
-    const uint32_t per = (total + kBlock - 1) / kBlock;
-    const uint32_t p0 = threadIdx.x * per < total ? threadIdx.x * per : total;
-    const uint32_t p1 = p0 + per < total ? p0 + per : total;
-    auto ukey = [&](uint16_t e) { return kandnot<W>(kload<W>(sh.keys + e * W), tmask); };
-    // X[i] precedes Y[j] iff u(X[i]) <= u(Y[j]) (X's members are lower)
-    auto xfirst = [&](uint16_t xe, uint16_t ye) { return !klt<W>(ukey(ye), ukey(xe)); };
-    uint32_t lo = p0 > ny ? p0 - ny : 0, hi = p0 < nx ? p0 : nx;
-    while (lo < hi) {  // first i with X[i] after Y[p0-i-1]
-        const uint32_t mid = (lo + hi) >> 1;
-        if (xfirst(X[mid], Y[p0 - mid - 1]))
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    uint32_t i = lo, j = p0 - lo;
-    for (uint32_t p = p0; p < p1; ++p) {
-        const bool takeX = j >= ny || (i < nx && xfirst(X[i], Y[j]));
-        out[p] = takeX ? X[i++] : Y[j++];
-    }
-}
-
-// Load a fused tile into LDS and bring it into (u, member) order: the order ends up in sh.ia.
-template <int W>
-__device__ __forceinline__ void fuse_load(const BatchDev &b, const FusedMap &F,
-                                          const FuseTileGeom &x, const Key<W> &tmask,
-                                          FuseShared<W> &sh) {
-    const uint64_t *ikeys = b.arena[F.in_buf].keys + b.slot[x.m].off * W;
-    const double *ivals = b.arena[F.in_buf].vals + b.slot[x.m].off;
-    const uint32_t total = x.total;
-    constexpr int R = kMergeTile / kBlock;
-    Key<W> kk[R];
-    double vv[R];
-    uint8_t tg[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const uint32_t e = threadIdx.x + r * kBlock;
-        tg[r] = 0;
-        if (e < total) {
-            const uint32_t j = (e >= x.off[1] ? 1u : 0u) + (e >= x.off[2] ? 1u : 0u) +
-                               (e >= x.off[3] ? 1u : 0u);
-            uint64_t src = 0;
-#pragma unroll
-            for (uint32_t q = 0; q < 4; ++q)
-                if (q == j) src = x.A0[q] + x.st[q] + (e - x.off[q]);
-            kk[r] = kload<W>(ikeys + src * W);
-            vv[r] = ivals[src];
-            tg[r] = (uint8_t)j;
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const uint32_t e = threadIdx.x + r * kBlock;
-        if (e < total) {
-            kstore<W>(sh.keys + e * W, kk[r]);
-            sh.vals[e] = vv[r];
-            sh.tag[e] = tg[r];
-            sh.ib[e] = (uint16_t)e;
-        }
-    }
-    __syncthreads();
-    if (x.G == 1) {
-        for (uint32_t e = threadIdx.x; e < total; e += kBlock) sh.ia[e] = (uint16_t)e;
-    } else if (x.G == 2) {
-        lds_merge<W>(sh, tmask, sh.ib, x.off[1], sh.ib + x.off[1], x.off[2] - x.off[1], sh.ia);
-    } else {
-        // members 0,1 -> ia[0 .. off2), members 2,3 -> ia[off2 .. off4), then both -> ib -> ia
-        lds_merge<W>(sh, tmask, sh.ib, x.off[1], sh.ib + x.off[1], x.off[2] - x.off[1], sh.ia);
-        lds_merge<W>(sh, tmask, sh.ib + x.off[2], x.off[3] - x.off[2], sh.ib + x.off[3],
-                     x.off[4] - x.off[3], sh.ia + x.off[2]);
-        __syncthreads();
-        lds_merge<W>(sh, tmask, sh.ia, x.off[2], sh.ia + x.off[2], x.off[4] - x.off[2], sh.ib);
-        __syncthreads();
-        for (uint32_t e = threadIdx.x; e < total; e += kBlock) sh.ia[e] = sh.ib[e];
-    }
-    __syncthreads();
-}
-
-// Per-thread accumulators of one fused tile.
-struct FuseAcc {
-    uint32_t cnt[4];
-    uint32_t after[kFuseMax];
-    double l1[kFuseMax], sq[kFuseMax], rsq[kFuseMax];
-    uint32_t nd[kFuseMax], nr[kFuseMax];
-    bool bad;
-    __device__ __forceinline__ void after_add(uint32_t r, uint32_t x) { after[r] += x; }
-};
-
-// Apply the run's rotations to the group at merged positions [p, p + len): values per member.
-// emit(member, key, value) is called for every member kept after the last rotation, in member
-// order.
-// One member's output of rotation r (a-slot: its own value, d-slot: its partner's), the radius
-// and truncation bookkeeping of the reference; `out`/`out_p` receive the kept value.
-template <int W, bool LOG>
-__device__ __forceinline__ void fuse_member(const BatchDev &b, const FuseTab<W> &t, uint32_t r,
-                                            uint32_t j, const Key<W> &u, bool ident_j,
-                                            double va, bool hasA, double vd, bool hasD,
-                                            FuseAcc &acc, double &out, bool &out_p) {
-    out = 0.0;
-    out_p = false;
-    Coef k;
-    k.anti = t.anti[r][j] != 0;
-    if (!k.anti) hasD = false;
-    if (!hasA && !hasD) return;
-    k.c = t.c[r];
-    k.s = t.s[r];
-    k.rc = t.rc[r];
-    k.rs = t.rs[r];
-    k.thr = t.thr[r];
-    k.validated = t.flags[r] & 1u;
-    k.dneg = t.dneg[r][j] != 0;
-    double val, rad;
-    merged(k, hasA, va, hasD, vd, val, rad, acc.bad);
-    if (rad != 0.0) {
-        acc.nr[r]++;
-        acc.rsq[r] = upadd(acc.rsq[r], upmul(rad, rad));
-        if (LOG && (t.flags[r] & 2u)) {
-            const uint64_t k0 = u.w[0] | t.cbits[j][0];
-            const uint64_t k1 = W > 1 ? (u.w[W - 1] | t.cbits[j][W - 1]) : 0;
-            log_entry(b, k0, k1, rad, 0.0, t.map, t.rot_seq[r], 1);
-        }
-    }
-    if (val == 0.0) return;  // exact cancellation: dropped silently
-    if (!ident_j && __builtin_fabs(val) <= k.thr) {
-        acc.nd[r]++;
-        acc.l1[r] = upadd(acc.l1[r], __builtin_fabs(val));
-        acc.sq[r] = upadd(acc.sq[r], upmul(val, val));
-        if (LOG && (t.flags[r] & 2u)) {
-            const uint64_t k0 = u.w[0] | t.cbits[j][0];
-            const uint64_t k1 = W > 1 ? (u.w[W - 1] | t.cbits[j][W - 1]) : 0;
-            log_entry(b, k0, k1, __builtin_fabs(val), val, t.map, t.rot_seq[r], 0);
-        }
-        return;
-    }
-    out = val;
-    out_p = true;
-}
-
-// Apply the run's rotations to the group at merged positions [p, p + len).  emit(member, key,
-// value) is called for every member kept after the last rotation, in member order.
-template <int W, bool LOG, typename E>
-__device__ __forceinline__ void fuse_group(const BatchDev &b, const FuseShared<W> &sh,
-                                           uint32_t p, uint32_t len, const Key<W> &u,
-                                           FuseAcc &acc, E &&emit) {
-    const FuseTab<W> &t = sh.tab;
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-    bool p0 = false, p1 = false, p2 = false, p3 = false;
-    for (uint32_t i = 0; i < len; ++i) {
-        const uint16_t e = sh.ia[p + i];
-        const uint32_t j = sh.tag[e];
-        const double x = sh.vals[e];
-        if (j == 0) { v0 = x; p0 = true; }
-        if (j == 1) { v1 = x; p1 = true; }
-        if (j == 2) { v2 = x; p2 = true; }
-        if (j == 3) { v3 = x; p3 = true; }
-    }
-    const bool uzero = kzero<W>(u);
-    const bool i0 = uzero && t.ident == 0, i1 = uzero && t.ident == 1,
-               i2 = uzero && t.ident == 2, i3 = uzero && t.ident == 3;
-    const uint32_t G = t.G, R = t.R;
-    // a pair (x, y) of members exchanged by rotation r: both outputs from the old values
-#define FUSE_PAIR(X, Y)                                                                        \
-    do {                                                                                       \
-        double nx, ny;                                                                         \
-        bool qx, qy;                                                                           \
-        fuse_member<W, LOG>(b, t, r, X, u, i##X, v##X, p##X, v##Y, p##Y, acc, nx, qx);        \
-        fuse_member<W, LOG>(b, t, r, Y, u, i##Y, v##Y, p##Y, v##X, p##X, acc, ny, qy);        \
-        v##X = nx;                                                                             \
-        p##X = qx;                                                                             \
-        v##Y = ny;                                                                             \
-        p##Y = qy;                                                                             \
-    } while (0)
-#pragma unroll
-    for (uint32_t r = 0; r < (uint32_t)kFuseMax; ++r) {
-        if (r >= R) break;
-        const uint32_t tr = t.t[r];
-        if (G == 1) {
-            double n0;
-            bool q0;
-            fuse_member<W, LOG>(b, t, r, 0, u, i0, v0, p0, 0.0, false, acc, n0, q0);
-            v0 = n0;
-            p0 = q0;
-        } else if (G == 2) {
-            FUSE_PAIR(0, 1);
-        } else if (tr == 1) {
-            FUSE_PAIR(0, 1);
-            FUSE_PAIR(2, 3);
-        } else if (tr == 2) {
-            FUSE_PAIR(0, 2);
-            FUSE_PAIR(1, 3);
-        } else {
-            FUSE_PAIR(0, 3);
-            FUSE_PAIR(1, 2);
-        }
-        acc.after_add(r, (p0 ? 1u : 0u) + (p1 ? 1u : 0u) + (p2 ? 1u : 0u) + (p3 ? 1u : 0u));
-    }
-#undef FUSE_PAIR
-    Key<W> key;
-#define FUSE_EMIT(J)                                                                           \
-    if (p##J) {                                                                                \
-        acc.cnt[J]++;                                                                          \
-        for (int w = 0; w < W; ++w) key.w[w] = u.w[w] | t.cbits[J][w];                         \
-        emit(J, key, v##J);                                                                    \
-    }
-    FUSE_EMIT(0)
-    FUSE_EMIT(1)
-    FUSE_EMIT(2)
-    FUSE_EMIT(3)
-#undef FUSE_EMIT
-}
-
-// Walk the groups whose first element lies in this thread's 4 merged positions.
-template <int W, bool LOG, typename E>
-__device__ __forceinline__ void fuse_walk(const BatchDev &b, const FuseTileGeom &x,
-                                          const FuseShared<W> &sh, const Key<W> &tmask,
-                                          FuseAcc &acc, E &&emit) {
-    const uint32_t total = x.total;
-    auto ukey = [&](uint32_t p) {
-        return kandnot<W>(kload<W>(sh.keys + sh.ia[p] * W), tmask);
-    };
-    for (uint32_t p = threadIdx.x * 4; p < threadIdx.x * 4 + 4 && p < total; ++p) {
-        const Key<W> u = ukey(p);
-        if (p > 0 && keq<W>(ukey(p - 1), u)) continue;  // not the first of its group
-        uint32_t len = 1;
-        while (len < x.G && p + len < total && keq<W>(ukey(p + len), u)) ++len;
-        fuse_group<W, LOG>(b, sh, p, len, u, acc, emit);
-    }
-}
-
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_fuse_count(BatchDev b, const FusedMap *fm) {
-    __shared__ FuseShared<W> sh;
-    if (b.counter[2]) return;
-    const uint64_t ntiles = b.fg_tiles[b.n_maps * 16];
-    for (uint64_t T = blockIdx.x; T < ntiles; T += gridDim.x) {
-        FuseTileGeom x;
-        const bool ok = fuse_geom(b, fm, T, x);
-        const FusedMap &F = fm[x.m];
-        const uint32_t n = b.slot[x.m].n_spins;
-        const Key<W> tmask = top_mask<W>(n);
-        if (!ok) {
-            if (threadIdx.x == 0) {
-                atomicOr(&b.counter[1], 1u);
-                b.ftile[T] = FusedTile{};
-            }
-            continue;
-        }
-        fuse_table<W>(F, x, n, sh);
-        fuse_load<W>(b, F, x, tmask, sh);
-        FuseAcc acc{};
-        fuse_walk<W, true>(b, x, sh, tmask, acc, [&](uint32_t, const Key<W> &, double) {});
-        // block sums: counts and lengths exactly, ledgers in a fixed upward-rounded tree
-        FusedTile out{};
-#pragma unroll
-        for (uint32_t j = 0; j < 4 + kFuseMax; ++j) {
-            unsigned val = j < 4 ? acc.cnt[j] : acc.after[j - 4];
-            for (int d = 32; d > 0; d >>= 1) val += __shfl_down(val, d, 64);
-            if (lane_id() == 0) sh.redc[j][wave_id()] = val;
-        }
-        const unsigned bad = __syncthreads_or(acc.bad ? 1 : 0);
-#pragma unroll
-        for (uint32_t j = 0; j < 4 + kFuseMax; ++j) {
-            const unsigned tot = sh.redc[j][0] + sh.redc[j][1] + sh.redc[j][2] + sh.redc[j][3];
-            if (j < 4)
-                out.cnt[j] = tot;
-            else
-                out.after[j - 4] = tot;
-        }
-#pragma unroll
-        for (uint32_t r = 0; r < (uint32_t)kFuseMax; ++r) {
-            if (r >= F.R) break;
-            double l1 = acc.l1[r], sq = acc.sq[r], rsq = acc.rsq[r];
-            uint64_t nd = acc.nd[r], nr = acc.nr[r];
-            const unsigned any = __syncthreads_or((nd | nr) ? 1 : 0);
-            if (!any) continue;
-            for (int d = 1; d < 64; d <<= 1) {
-                const double o1 = __shfl_down(l1, d, 64), o2 = __shfl_down(sq, d, 64),
-                             o3 = __shfl_down(rsq, d, 64);
-                const uint64_t c1 = __shfl_down(nd, d, 64), c2 = __shfl_down(nr, d, 64);
-                if (lane_id() + d < 64) {
-                    l1 = upadd(l1, o1);
-                    sq = upadd(sq, o2);
-                    rsq = upadd(rsq, o3);
-                    nd += c1;
-                    nr += c2;
-                }
-            }
-            if (lane_id() == 0) {
-                sh.red[0][wave_id()] = l1;
-                sh.red[1][wave_id()] = sq;
-                sh.red[2][wave_id()] = rsq;
-                sh.redn[0][wave_id()] = nd;
-                sh.redn[1][wave_id()] = nr;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                TileLedger L{};
-                for (int w = 0; w < kBlock / 64; ++w) {
-                    L.l1 = upadd(L.l1, sh.red[0][w]);
-                    L.sq = upadd(L.sq, sh.red[1][w]);
-                    L.rsq = upadd(L.rsq, sh.red[2][w]);
-                    L.dropped += sh.redn[0][w];
-                    L.radii += sh.redn[1][w];
-                }
-                out.led[r] = L;
-            }
-            __syncthreads();
-        }
-        out.nonfinite = bad;
-        if (threadIdx.x == 0) b.ftile[T] = out;
-        __syncthreads();
-    }
-}
-
-// Scan level 1, one block per chunk of kChunkTiles tiles of one (map, group): per-member tile
-// offsets within the chunk, chunk totals, lengths and ledger partials (tile order).
-__global__ __launch_bounds__(kBlock) void k_fuse_scan1(BatchDev b, const FusedMap *fm) {
-    __shared__ unsigned scratch[8];
-    __shared__ TileLedger sh[kBlock / 64];
-    __shared__ uint64_t ash[kBlock];
-    if (b.counter[2]) return;
-    const uint32_t n16 = b.n_maps * 16;
-    const uint64_t nchunks = b.fg_chunks[n16];
-    for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-        const uint32_t t = find_bucket(b.fg_chunks, n16, ch);
-        const FusedMap &F = fm[t >> 4];
-        const uint64_t T1 = b.fg_tiles[t + 1];
-        const uint64_t T0 = b.fg_tiles[t] + (ch - b.fg_chunks[t]) * kChunkTiles;
-        constexpr int per = kChunkTiles / kBlock;
-        const uint64_t mine = T0 + (uint64_t)threadIdx.x * per;
-        FusedChunk &C = b.fchunk[ch];
-        for (uint32_t j = 0; j < 4; ++j) {
-            if (j >= F.G) {
-                if (threadIdx.x == 0) C.cnt[j] = 0;
-                continue;
-            }
-            unsigned cnt[per], sum = 0;
-#pragma unroll
-            for (int i = 0; i < per; ++i) {
-                cnt[i] = mine + i < T1 ? b.ftile[mine + i].cnt[j] : 0u;
-                sum += cnt[i];
-            }
-            unsigned tot;
-            unsigned at = block_exclusive_scan(sum, scratch, &tot);
-#pragma unroll
-            for (int i = 0; i < per; ++i)
-                if (mine + i < T1) {
-                    b.ftoff[(mine + i) * 4 + j] = at;
-                    at += cnt[i];
-                }
-            if (threadIdx.x == 0) C.cnt[j] = tot;
-        }
-        unsigned bad = 0;
-        for (uint32_t r = 0; r < kFuseMax; ++r) {
-            if (r >= F.R) break;
-            TileLedger L{};
-            uint64_t after = 0;
-#pragma unroll
-            for (int i = 0; i < per; ++i)
-                if (mine + i < T1) {
-                    ledger_fold(L, b.ftile[mine + i].led[r]);
-                    after += b.ftile[mine + i].after[r];
-                    if (r == 0) bad |= b.ftile[mine + i].nonfinite;
-                }
-            const TileLedger Cl = block_ledger(L, sh);
-            ash[threadIdx.x] = after;
-            __syncthreads();
-            for (int st = kBlock / 2; st > 0; st >>= 1) {
-                if ((int)threadIdx.x < st) ash[threadIdx.x] += ash[threadIdx.x + st];
-                __syncthreads();
-            }
-            if (threadIdx.x == 0) {
-                C.led[r] = Cl;
-                C.after[r] = ash[0];
-            }
-            __syncthreads();
-        }
-        bad = __syncthreads_or(bad);
-        if (threadIdx.x == 0) C.led[0].nonfinite |= bad;
-        __syncthreads();
-    }
-}
-
-// Scan level 2, one block per (map, group): chunk bases per member, member totals, and the
-// group's lengths and ledgers per rotation (chunk order).
-__global__ __launch_bounds__(kBlock) void k_fuse_scan2(BatchDev b, const FusedMap *fm) {
-    __shared__ uint64_t runs[kBlock];
-    __shared__ TileLedger sh[kBlock / 64];
-    __shared__ uint64_t ash[kBlock];
-    if (b.counter[2]) return;
-    const uint32_t t = blockIdx.x, m = t >> 4, g = t & 15;
-    const FusedMap &F = fm[m];
-    if (!F.active || g >= F.n_groups) return;
-    const uint64_t c0 = b.fg_chunks[t], c1 = b.fg_chunks[t + 1];
-    for (uint32_t j = 0; j < F.G; ++j) {
-        uint64_t carry = 0;
-        for (uint64_t base = c0; base < c1; base += kBlock) {
-            const uint64_t ch = base + threadIdx.x;
-            const uint64_t v = ch < c1 ? b.fchunk[ch].cnt[j] : 0;
-            runs[threadIdx.x] = v;
-            __syncthreads();
-            for (int d = 1; d < kBlock; d <<= 1) {
-                const uint64_t y = (int)threadIdx.x >= d ? runs[threadIdx.x - d] : 0;
-                __syncthreads();
-                runs[threadIdx.x] += y;
-                __syncthreads();
-            }
-            if (ch < c1) b.fchbase[ch * 4 + j] = carry + runs[threadIdx.x] - v;
-            carry += runs[kBlock - 1];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) b.fcls_total[m * 16 + F.members[g][j]] = carry;
-    }
-    for (uint32_t r = 0; r < F.R; ++r) {
-        TileLedger L{};
-        uint64_t after = 0;
-        for (uint64_t ch = c0 + threadIdx.x; ch < c1; ch += kBlock) {
-            ledger_fold(L, b.fchunk[ch].led[r]);
-            after += b.fchunk[ch].after[r];
-        }
-        const TileLedger Gl = block_ledger(L, sh);
-        ash[threadIdx.x] = after;
-        __syncthreads();
-        for (int st = kBlock / 2; st > 0; st >>= 1) {
-            if ((int)threadIdx.x < st) ash[threadIdx.x] += ash[threadIdx.x + st];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            b.fg_led[(uint64_t)t * kFuseMax + r] = Gl;
-            b.fg_after[(uint64_t)t * kFuseMax + r] = ash[0];
-        }
-        __syncthreads();
-    }
-}
-
-// Scan level 3, one block per map: output regions (classes in order), the per-rotation
-// records (groups in order), overflow.
-__global__ __launch_bounds__(kBlock) void k_fuse_scan3(BatchDev b, const FusedMap *fm) {
-    const uint32_t m = blockIdx.x;
-    const FusedMap &F = fm[m];
-    if (!F.active || threadIdx.x != 0) return;
-    const uint64_t *rin = b.regions + ((uint64_t)m * 3 + F.in_buf) * 17;
-    uint64_t *rout = b.regions + ((uint64_t)m * 3 + F.out_buf) * 17;
-    if (b.counter[2]) {  // scratch too small for this pass: report, write nothing
-        b.overflow[m] = 1;
-        for (int c = 0; c < 17; ++c) rout[c] = 0;
-        b.len[m * 3 + F.out_buf] = 0;
-        for (uint32_t r = 0; r < F.R; ++r) {
-            tpgp_hip_record rec{};
-            rec.len_in = rin[16];
-            rec.executed = 1;
-            rec.overflow = 1;
-            rec.rotation = F.rot_seq[r];
-            b.rec[(uint64_t)m * b.max_rec + F.rec_slot[r]] = rec;
-        }
-        return;
-    }
-    uint64_t at = 0;
-    for (int c = 0; c < 16; ++c) {
-        rout[c] = at;
-        at += b.fcls_total[m * 16 + c];
-    }
-    const bool over = b.overflow[m] || at > b.slot[m].cap;
-    if (over) {
-        for (int c = 0; c < 17; ++c) rout[c] = 0;
-        b.overflow[m] = 1;
-    }
-    rout[16] = over ? 0 : at;
-    b.len[m * 3 + F.out_buf] = over ? 0 : at;
-    uint64_t len_in = rin[16];
-    for (uint32_t r = 0; r < F.R; ++r) {
-        TileLedger L{};
-        uint64_t after = 0;
-        for (uint32_t g = 0; g < F.n_groups; ++g) {
-            ledger_fold(L, b.fg_led[((uint64_t)m * 16 + g) * kFuseMax + r]);
-            after += b.fg_after[((uint64_t)m * 16 + g) * kFuseMax + r];
-        }
-        tpgp_hip_record rec;
-        rec.len_in = len_in;
-        rec.len_out = after;
-        rec.dropped = L.dropped;
-        rec.l1_up = L.l1;
-        rec.sq_up = L.sq;
-        rec.rad_sq_up = L.rsq;
-        rec.radii = L.radii;
-        rec.executed = len_in > 0;
-        rec.nonfinite = 0;
-        for (uint32_t g = 0; g < F.n_groups; ++g)
-            rec.nonfinite |= b.fg_led[((uint64_t)m * 16 + g) * kFuseMax + 0].nonfinite;
-        rec.overflow = (over && r + 1 == F.R) ? 1u : 0u;
-        rec.rotation = F.rot_seq[r];
-        b.rec[(uint64_t)m * b.max_rec + F.rec_slot[r]] = rec;
-        len_in = after;
-    }
-}
-
-// Write: every tile again, kept outputs to their class runs (per member: class start + chunk
-// base + tile offset + rank within the tile).
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_fuse_write(BatchDev b, const FusedMap *fm) {
-    __shared__ FuseShared<W> sh;
-    if (b.counter[2]) return;
-    const uint64_t ntiles = b.fg_tiles[b.n_maps * 16];
-    for (uint64_t T = blockIdx.x; T < ntiles; T += gridDim.x) {
-        FuseTileGeom x;
-        const bool ok = fuse_geom(b, fm, T, x);
-        const FusedMap &F = fm[x.m];
-        if (!ok || b.overflow[x.m]) continue;
-        const uint32_t n = b.slot[x.m].n_spins;
-        const Key<W> tmask = top_mask<W>(n);
-        fuse_table<W>(F, x, n, sh);
-        fuse_load<W>(b, F, x, tmask, sh);
-        // this thread's kept outputs per member, then their ranks in the tile
-        FuseAcc acc{};
-        fuse_walk<W, false>(b, x, sh, tmask, acc, [&](uint32_t, const Key<W> &, double) {});
-        uint64_t packed = 0;  // four 16-bit counts
-        for (uint32_t j = 0; j < 4; ++j) packed |= (uint64_t)acc.cnt[j] << (16 * j);
-        // exclusive scan of the packed counts (no field overflows: a tile emits < 4096 per member)
-        uint64_t xs = packed;
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint64_t y = __shfl_up(xs, d, 64);
-            if ((int)lane_id() >= d) xs += y;
-        }
-        if (lane_id() == 63) sh.wsum[wave_id()] = xs;
-        __syncthreads();
-        uint64_t base = 0;
-        for (uint32_t w = 0; w < wave_id(); ++w) base += sh.wsum[w];
-        uint64_t excl = base + xs - packed;
-        if (threadIdx.x < 4 && threadIdx.x < x.G) {
-            const uint32_t j = threadIdx.x, cj = F.members[x.g][j];
-            const uint64_t ch = b.fg_chunks[x.m * 16 + x.g] + x.bt / kChunkTiles;
-            sh.cbase[j] = b.regions[((uint64_t)x.m * 3 + F.out_buf) * 17 + cj] +
-                          b.fchbase[ch * 4 + j] + b.ftoff[T * 4 + j];
-        }
-        __syncthreads();
-        uint64_t pos[4];
-        for (uint32_t j = 0; j < 4; ++j) pos[j] = j < x.G ? sh.cbase[j] + ((excl >> (16 * j)) & 0xffffu) : 0;
-        uint64_t *gk = b.arena[F.out_buf].keys + b.slot[x.m].off * W;
-        double *gv = b.arena[F.out_buf].vals + b.slot[x.m].off;
-        const uint64_t cap = b.slot[x.m].cap;
-        FuseAcc acc2{};
-        fuse_walk<W, false>(b, x, sh, tmask, acc2,
-                     [&](uint32_t j, const Key<W> &key, double v) {
-                         uint64_t at = 0;
-#pragma unroll
-                         for (uint32_t t = 0; t < 4; ++t)
-                             if (t == j) at = pos[t]++;
-                         if (at < cap) {
-                             kstore<W>(gk + at * W, key);
-                             gv[at] = v;
-                         }
-                     });
         __syncthreads();
     }
 }
@@ -3888,7 +3014,6 @@ struct tpgp_hip_batch {
     double cost_lds_us = 3.0;      // one persistent workgroup with the map in LDS: per rotation
     double cost_lds_ns = 1.0;      // and per term
     bool small_fixed = false;      // TPGP_HIP_SMALL_LIMIT given: length bound instead
-    bool fuse = false;             // TPGP_HIP_FUSE=1: fused passes over runs of one bond
     uint64_t small_limit = 0;
     std::vector<uint8_t> small;  // per map, decided at each rotate_many call
     uint32_t pass_count = 0;
@@ -3968,21 +3093,6 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     ALLOC(d.tsplit, b->max_tiles * kBlock);
     ALLOC(d.wstart, (size_t)M * 16 + 2);
     ALLOC(d.tile_led, b->max_tiles);
-    d.max_samples = T / kSampleStride + 16ull * M + 64;
-    d.max_ftiles = d.max_samples / kSamplesPerTile + 32ull * M + 64;
-    const uint64_t max_fchunks = d.max_ftiles / kChunkTiles + 16ull * M + 16;
-    ALLOC(d.fs_start, (size_t)M * 16 + 1);
-    ALLOC(d.fg_tiles, (size_t)M * 16 + 1);
-    ALLOC(d.fg_chunks, (size_t)M * 16 + 1);
-    ALLOC(d.fsamp, d.max_samples * b->W);
-    ALLOC(d.ftpos, d.max_ftiles * 4);
-    ALLOC(d.ftile, d.max_ftiles);
-    ALLOC(d.ftoff, d.max_ftiles * 4);
-    ALLOC(d.fchunk, max_fchunks);
-    ALLOC(d.fchbase, max_fchunks * 4);
-    ALLOC(d.fcls_total, (size_t)M * 16);
-    ALLOC(d.fg_led, (size_t)M * 16 * kFuseMax);
-    ALLOC(d.fg_after, (size_t)M * 16 * kFuseMax);
     ALLOC(d.chunk_start, (size_t)M + 1);
     ALLOC(d.chunk_base, b->max_tiles / kChunkTiles + M + 1);
     ALLOC(d.chunk_sum, b->max_tiles / kChunkTiles + M + 1);
@@ -4349,8 +3459,6 @@ extern "C" tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *ctx, size_t n
     b->validate = v && *v && *v != '0';
     // small-map path: worth it while one workgroup per map beats ~10 launches per rotation;
     // with many maps the workgroups fill the GPU, so larger maps qualify
-    const char *fz = getenv("TPGP_HIP_FUSE");
-    b->fuse = fz && *fz == '1';
     const char *sl = getenv("TPGP_HIP_SMALL_LIMIT");
     b->small_fixed = sl && *sl;
     b->small_limit = b->small_fixed ? strtoull(sl, nullptr, 10) : 0;
@@ -4513,8 +3621,8 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
     const uint64_t est_tiles = rgrid > 64 ? (uint64_t)(rgrid - 64) * 16 / 17 + 1 : 1;
     {
         ProfileScope whole(b, TPGP_HIP_PROFILE_ROTATION);
-        const int fused_plan = b->M <= kPlanMaps;  // the split pass plans too
-        if (!fused_plan) {
+        const int plan_in_splits = b->M <= kPlanMaps;  // the split pass plans too
+        if (!plan_in_splits) {
             ProfileScope pp(b, TPGP_HIP_PROFILE_PLAN);
             hipLaunchKernelGGL(k_rot_plan, dim3(1), dim3(kBlock), 0, st, b->dev, d);
         }
@@ -4523,10 +3631,10 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
             const unsigned sgrid = tile_grid(est_tiles / 32 + 1);  // 8 lanes per tile
             if (b->W == 1)
                 hipLaunchKernelGGL(k_rot_splits<1>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d,
-                                   fused_plan);
+                                   plan_in_splits);
             else
                 hipLaunchKernelGGL(k_rot_splits<2>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d,
-                                   fused_plan);
+                                   plan_in_splits);
         }
         {
             ProfileScope pc(b, TPGP_HIP_PROFILE_COUNT);
@@ -4582,172 +3690,6 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
 
 }  // namespace
 
-namespace {
-// Class XOR mask of a request for a map whose active pair is on its top two digits.
-uint32_t request_mask(const HostMap &hm, const tpgp_hip_rotation &r) {
-    const bool i_top = hm.sig[hm.n_spins - 1] == r.i;
-    const uint32_t qt = i_top ? r.qi : r.qj, qs = i_top ? r.qj : r.qi;
-    return (qt << 2) | qs;
-}
-// Span of class masks (a subgroup of the 16 classes): element list, or empty when it would hold
-// more than four elements.
-bool span_add(std::vector<uint32_t> &V, uint32_t q) {
-    if (std::find(V.begin(), V.end(), q) != V.end()) return true;
-    if (V.size() >= 4) return false;
-    const size_t n = V.size();
-    for (size_t k = 0; k < n; ++k) V.push_back(V[k] ^ q);
-    return true;
-}
-// Rotations r, r+1, ... (up to kFuseMax, within [r, r1)) that every map can take in one fused
-// pass: the same maps active, each on one site pair, the class masks of each map spanning at
-// most four classes.
-size_t run_length(const tpgp_hip_batch *b, const tpgp_hip_rotation *req, size_t r, size_t r1) {
-    const uint32_t M = b->M;
-    size_t R = 1;
-    std::vector<std::vector<uint32_t>> V(M);
-    for (uint32_t m = 0; m < M; ++m)
-        if (req[r * M + m].active) {
-            V[m] = {0u};
-            span_add(V[m], (req[r * M + m].qi << 2) | req[r * M + m].qj);
-        }
-    for (; r + R < r1 && R < (size_t)kFuseMax; ++R) {
-        bool ok = true;
-        for (uint32_t m = 0; m < M && ok; ++m) {
-            const tpgp_hip_rotation &a = req[r * M + m], &c = req[(r + R) * M + m];
-            if (a.active != c.active) ok = false;
-            if (!a.active || !ok) continue;
-            const bool same = (a.i == c.i && a.j == c.j);
-            const bool swapped = (a.i == c.j && a.j == c.i);
-            if (!same && !swapped) {
-                ok = false;
-                continue;
-            }
-            // masks in (a.i, a.j) orientation
-            const uint32_t q = same ? ((c.qi << 2) | c.qj) : ((c.qj << 2) | c.qi);
-            std::vector<uint32_t> W2 = V[m];
-            if (!span_add(W2, q)) ok = false;
-        }
-        if (!ok) break;
-        for (uint32_t m = 0; m < M; ++m) {
-            const tpgp_hip_rotation &a = req[r * M + m], &c = req[(r + R) * M + m];
-            if (!a.active) continue;
-            const bool same = (a.i == c.i && a.j == c.j);
-            span_add(V[m], same ? ((c.qi << 2) | c.qj) : ((c.qj << 2) | c.qi));
-        }
-    }
-    return R;
-}
-// Fused-pass descriptors of the large maps for rotations [r, r + R), and the host-side state
-// they imply (one buffer switch, R records).  Returns false when no large map takes part.
-bool plan_fused(tpgp_hip_batch *b, const tpgp_hip_rotation *req, const uint32_t *rotations,
-                size_t r, size_t R, FusedMap *h) {
-    const uint32_t M = b->M;
-    bool any = false;
-    for (uint32_t m = 0; m < M; ++m) {
-        std::memset((void *)&h[m], 0, sizeof(FusedMap));
-        const tpgp_hip_rotation &a = req[r * M + m];
-        if (!a.active || b->small[m]) continue;
-        any = true;
-        HostMap &hm = b->maps[m];
-        FusedMap &F = h[m];
-        F.active = 1;
-        F.in_buf = (uint32_t)hm.kcur;
-        F.out_buf = (uint32_t)next_buffer(hm);
-        F.R = (uint32_t)R;
-        std::vector<uint32_t> V = {0u};
-        for (size_t k = 0; k < R; ++k) {
-            const tpgp_hip_rotation &q = req[(r + k) * M + m];
-            const uint32_t mask = request_mask(hm, q);
-            span_add(V, mask);
-            F.qt[k] = mask >> 2;
-            F.qs[k] = mask & 3u;
-            F.flags[k] = (q.validated ? 1u : 0u) | (q.log ? 2u : 0u);
-            F.rec_slot[k] = hm.nrec + (uint32_t)k;
-            F.rot_seq[k] = rotations[r + k];
-            F.c[k] = q.c;
-            F.s[k] = q.s;
-            F.rc[k] = q.validated ? q.rc : 0.0;
-            F.rs[k] = q.validated ? q.rs : 0.0;
-            F.thr[k] = q.threshold;
-            if (q.log) b->logged.push_back(LoggedLayout{m, rotations[r + k], hm.sig});
-        }
-        // span in index order: V[0] = 0, and V[i] ^ V[j] = V[i ^ j] (sorted ascending gives
-        // this for spans of 2 and 4), so a rotation with mask V[t] pairs member k with k ^ t
-        std::sort(V.begin(), V.end());
-        F.G = (uint32_t)V.size();
-        for (size_t k = 0; k < R; ++k) {
-            const uint32_t mask = (F.qt[k] << 2) | F.qs[k];
-            F.tpair[k] = (uint8_t)(std::find(V.begin(), V.end(), mask) - V.begin());
-        }
-        uint32_t g = 0;
-        for (uint32_t c = 0; c < 16; ++c) {
-            uint32_t rep = c;
-            for (uint32_t v : V) rep = std::min(rep, c ^ v);
-            if (rep != c) continue;
-            for (uint32_t k = 0; k < V.size(); ++k) {
-                const uint32_t mc = c ^ V[k];
-                F.members[g][k] = (uint8_t)mc;
-                F.cls_group[mc] = (uint8_t)g;
-                F.cls_member[mc] = (uint8_t)k;
-            }
-            ++g;
-        }
-        F.n_groups = g;
-        hm.kcur = (int)F.out_buf;
-        hm.nrec += (uint32_t)R;
-        hm.floor = std::nextafter(req[(r + R - 1) * M + m].threshold, INFINITY);
-    }
-    return any;
-}
-
-void launch_fused(tpgp_hip_batch *b, const FusedMap *d, const FusedMap *h) {
-    hipStream_t st = b->ctx->stream;
-    uint64_t est = 64;
-    for (uint32_t m = 0; m < b->M; ++m)
-        if (h[m].active) est += b->maps[m].est_len + b->maps[m].est_len / 8 + 1024;
-    const uint64_t samples = est / kSampleStride + 16ull * b->M;
-    const unsigned sgrid = (unsigned)std::min<uint64_t>(samples / kBlock + 1, 1u << 20);
-    const unsigned tgrid = tile_grid(samples / kSamplesPerTile + 16ull * b->M);
-    const unsigned cgrid = (unsigned)std::min<uint64_t>(
-        samples / kSamplesPerTile / kChunkTiles + 16ull * b->M + 1, 1u << 20);
-    ProfileScope whole(b, TPGP_HIP_PROFILE_ROTATION);
-    {
-        ProfileScope pp(b, TPGP_HIP_PROFILE_PLAN);
-        hipLaunchKernelGGL(k_fuse_plan, dim3(1), dim3(kBlock), 0, st, b->dev, d);
-    }
-    {
-        ProfileScope ps(b, TPGP_HIP_PROFILE_SPLITS);
-        if (b->W == 1) {
-            hipLaunchKernelGGL(k_fuse_samples<1>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d);
-            hipLaunchKernelGGL(k_fuse_bounds<1>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d);
-        } else {
-            hipLaunchKernelGGL(k_fuse_samples<2>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d);
-            hipLaunchKernelGGL(k_fuse_bounds<2>, dim3(sgrid), dim3(kBlock), 0, st, b->dev, d);
-        }
-    }
-    {
-        ProfileScope pc(b, TPGP_HIP_PROFILE_COUNT);
-        if (b->W == 1)
-            hipLaunchKernelGGL(k_fuse_count<1>, dim3(tgrid), dim3(kBlock), 0, st, b->dev, d);
-        else
-            hipLaunchKernelGGL(k_fuse_count<2>, dim3(tgrid), dim3(kBlock), 0, st, b->dev, d);
-    }
-    {
-        ProfileScope psn(b, TPGP_HIP_PROFILE_SCAN);
-        hipLaunchKernelGGL(k_fuse_scan1, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d);
-        hipLaunchKernelGGL(k_fuse_scan2, dim3(b->M * 16), dim3(kBlock), 0, st, b->dev, d);
-        hipLaunchKernelGGL(k_fuse_scan3, dim3(b->M), dim3(64), 0, st, b->dev, d);
-    }
-    {
-        ProfileScope pm(b, TPGP_HIP_PROFILE_MERGE);
-        if (b->W == 1)
-            hipLaunchKernelGGL(k_fuse_write<1>, dim3(tgrid), dim3(kBlock), 0, st, b->dev, d);
-        else
-            hipLaunchKernelGGL(k_fuse_write<2>, dim3(tgrid), dim3(kBlock), 0, st, b->dev, d);
-    }
-}
-}  // namespace
-
 extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rotation *req,
                                           const uint32_t *rotations, size_t n_rot, char *error,
                                           size_t cap) {
@@ -4778,13 +3720,12 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
     DeviceGuard guard(b->ctx->device);
     // rotations go out in chunks whose descriptors fill at most a quarter of the staging ring,
     // so the host stages one chunk while the device runs the previous ones
-    const bool fuse = b->fuse && !b->validate;
-    const size_t per_rot = 2 * (size_t)M * sizeof(PassMap) + (fuse ? M * sizeof(FusedMap) : 0);
+    const size_t per_rot = 2 * (size_t)M * sizeof(PassMap);
     const size_t chunk = std::max<size_t>(1, b->stage_cap / 4 / per_rot);
     std::vector<int> want(M);
     std::vector<uint32_t> pa(M), pb(M);
     struct Launch {
-        int kind;  // 0 rotation slot, 1 partition slot, 2 fused run
+        int kind;  // 0 rotation slot, 1 partition slot
         size_t at;
         unsigned grid;
         bool validated;  // rotation: some map computes radii
@@ -4796,7 +3737,6 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
         return false;
     };
     std::vector<Launch> launches;
-    std::vector<FusedMap> fused;
     // path of every map for this call (by its last known length)
     choose_paths(b, req, n_rot);
     std::vector<uint32_t> small_ids;
@@ -4807,10 +3747,9 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
     PassMap *d0;
     PassMap *h0 = stage(b, 2 * (r1 - r0) * M, &d0);
     launches.clear();
-    fused.clear();
     size_t used = 0;
     bool any_small = false;
-    for (size_t r = r0; r < r1;) {
+    for (size_t r = r0; r < r1; ++r) {
         const tpgp_hip_rotation *q = req + r * M;
         // bring each active map's pair (i, j) to the top two digits
         bool any = false;
@@ -4832,35 +3771,13 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
             launches.push_back({1, used, g, false, rounds});
             used += M;
         }
-        const size_t R = fuse ? run_length(b, req, r, r1) : 1;
-        if (fuse) {
-            // large maps: one fused pass for the run; small maps: one slot per rotation
-            const size_t at = fused.size();
-            fused.resize(at + M);
-            if (plan_fused(b, req, rotations, r, R, fused.data() + at))
-                launches.push_back({2, at, 0, false, 0});
-            else
-                fused.resize(at);
-            for (size_t k = 0; k < R; ++k) {
-                std::vector<tpgp_hip_rotation> qs(q + k * M, q + (k + 1) * M);
-                for (uint32_t m = 0; m < M; ++m)
-                    if (!b->small[m]) qs[m].active = 0;
-                launches.push_back({0, used,
-                                    plan_rotation(b, qs.data(), rotations[r + k], h0 + used),
-                                    any_validated(qs.data()), 0});
-                used += M;
-            }
-        } else {
-            launches.push_back({0, used, plan_rotation(b, q, rotations[r], h0 + used),
-                                any_validated(q), 0});
-            used += M;
-        }
-        r += R;
+        launches.push_back({0, used, plan_rotation(b, q, rotations[r], h0 + used),
+                            any_validated(q), 0});
+        used += M;
     }
     // maps on the small-map path leave the multi-kernel passes to the persistent kernel
     std::vector<uint8_t> pass_large(used / M, 0);
     for (const Launch &L : launches) {
-        if (L.kind == 2) continue;
         const size_t k = L.at / M;
         for (uint32_t m = 0; m < M; ++m) {
             PassMap &P = h0[k * M + m];
@@ -4876,18 +3793,7 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
     }
     hipError_t e = push_stage(b, h0, d0, used);
     if (e != hipSuccess) return hip_fail(e, "HIP descriptor staging", error, cap);
-    FusedMap *dF = nullptr;
-    if (!fused.empty()) {
-        FusedMap *hF = stage(b, fused.size(), &dF);
-        std::memcpy((void *)hF, fused.data(), fused.size() * sizeof(FusedMap));
-        if ((e = push_stage(b, hF, dF, fused.size())) != hipSuccess)
-            return hip_fail(e, "HIP descriptor staging", error, cap);
-    }
     for (const Launch &L : launches) {
-        if (L.kind == 2) {
-            launch_fused(b, dF + L.at, fused.data() + L.at);
-            continue;
-        }
         if (!pass_large[L.at / M]) continue;
         if (L.kind == 0)
             launch_rotation(b, d0 + L.at, L.grid, L.validated);
@@ -5245,10 +4151,11 @@ extern "C" int tpgp_hip_batch_download(tpgp_hip_batch *b, size_t map, uint64_t *
 }
 
 namespace {
-// Re-lay the arenas with new slot capacities, keeping every map's committed state (moved to
-// buffer 0 with its length, regions and digit layout).  The committed states wait in one
-// temporary device buffer (device-to-device copies); only if that allocation fails do they
-// take the way through host memory.
+// Re-lay the arenas with slot m holding new_cap[m] terms, keeping every map's committed state
+// (moved to buffer 0 with its length, regions and digit layout).  The committed states wait in
+// one temporary device buffer (device-to-device copies); only if that allocation fails do they
+// take the way through host memory.  A capacity below a map's committed length is rejected
+// before anything changes.
 int relayout(tpgp_hip_batch *b, const std::vector<uint64_t> &new_cap, char *error, size_t cap) {
     hipStream_t st = b->ctx->stream;
     hipError_t e = hipStreamSynchronize(st);
@@ -5267,6 +4174,11 @@ int relayout(tpgp_hip_batch *b, const std::vector<uint64_t> &new_cap, char *erro
         n[m] = len[(size_t)m * 3 + b->maps[m].kc];
         at[m] = total;
         total += n[m];
+        if (new_cap[m] < n[m] || !new_cap[m]) {
+            msg(error, cap, "Map %u: capacity %llu cannot hold its %llu committed terms.", m,
+                (unsigned long long)new_cap[m], (unsigned long long)n[m]);
+            return TPGP_HIP_EINVAL;
+        }
     }
     uint64_t *tk = nullptr;
     double *tv = nullptr;
@@ -5298,15 +4210,30 @@ int relayout(tpgp_hip_batch *b, const std::vector<uint64_t> &new_cap, char *erro
         return hip_fail(e, "HIP reserve", error, cap);
     }
     free_all(b);
-    uint64_t off = 0;
-    for (uint32_t m = 0; m < M; ++m) {
-        HostMap &hm = b->maps[m];
-        if (new_cap[m] > hm.cap) hm.cap = new_cap[m];
-        hm.off = off;
-        off += (hm.cap + 63) & ~63ull;
-    }
-    b->total_cap = off;
+    std::vector<uint64_t> old_cap(M);
+    auto lay_out = [&](bool grown) {
+        uint64_t off = 0;
+        for (uint32_t m = 0; m < M; ++m) {
+            HostMap &hm = b->maps[m];
+            hm.cap = grown ? new_cap[m] : old_cap[m];
+            hm.off = off;
+            off += (hm.cap + 63) & ~63ull;
+        }
+        b->total_cap = off;
+    };
+    for (uint32_t m = 0; m < M; ++m) old_cap[m] = b->maps[m].cap;
+    lay_out(true);
     int s = allocate_device(b, error, cap);
+    int result = TPGP_HIP_OK;
+    if (s == TPGP_HIP_ENOMEM) {
+        // the new layout does not fit: go back to the previous one (it fitted a moment ago,
+        // and its memory has just been released), keep every committed state, report ENOMEM
+        char again[256];
+        free_all(b);
+        lay_out(false);
+        result = s;
+        s = allocate_device(b, again, sizeof(again));
+    }
     if (s != TPGP_HIP_OK) {
         if (tk) (void)hipFree(tk);
         if (tv) (void)hipFree(tv);
@@ -5342,7 +4269,8 @@ int relayout(tpgp_hip_batch *b, const std::vector<uint64_t> &new_cap, char *erro
     if (tk) (void)hipFree(tk);
     if (tv) (void)hipFree(tv);
     b->logged.clear();
-    return e == hipSuccess ? TPGP_HIP_OK : hip_fail(e, "HIP reserve", error, cap);
+    if (e != hipSuccess) return hip_fail(e, "HIP reserve", error, cap);
+    return result;  // ENOMEM: the slots kept their previous capacities (error says why)
 }
 }  // namespace
 
@@ -5360,10 +4288,33 @@ extern "C" int tpgp_hip_batch_reserve_many(tpgp_hip_batch *b, const size_t *capa
             msg(error, cap, "Invalid reserve capacity for map %u.", m);
             return TPGP_HIP_EINVAL;
         }
-        want[m] = capacities[m];
+        want[m] = std::max<uint64_t>(capacities[m], b->maps[m].cap);
         grow |= capacities[m] > b->maps[m].cap;
     }
     if (!grow) return TPGP_HIP_OK;
+    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    return relayout(b, want, error, cap);
+}
+
+extern "C" int tpgp_hip_batch_resize(tpgp_hip_batch *b, const size_t *capacities, char *error,
+                                     size_t cap) {
+    msg(error, cap, "");
+    if (!b || !capacities) {
+        msg(error, cap, "Invalid resize arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    bool change = false;
+    std::vector<uint64_t> want(b->M);
+    for (uint32_t m = 0; m < b->M; ++m) {
+        if (capacities[m] > (1ull << 37)) {
+            msg(error, cap, "Invalid resize capacity for map %u.", m);
+            return TPGP_HIP_EINVAL;
+        }
+        want[m] = capacities[m] ? capacities[m] : b->maps[m].cap;
+        change |= want[m] != b->maps[m].cap;
+    }
+    if (!change) return TPGP_HIP_OK;
     std::lock_guard<std::mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     return relayout(b, want, error, cap);

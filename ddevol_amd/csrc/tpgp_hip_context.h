@@ -36,6 +36,7 @@ inline void msg(char *e, size_t n, const char *fmt, ...) {
 }
 inline tpgp_hip_status hip_fail(hipError_t code, const char *what, char *e, size_t n) {
     msg(e, n, "%s: %s", what, hipGetErrorString(code));
+    (void)hipGetLastError();  // reported now: a later launch check must not report it again
     return code == hipErrorOutOfMemory ? TPGP_HIP_ENOMEM : TPGP_HIP_EDEVICE;
 }
 struct DeviceGuard {  // save/restore the caller's current device

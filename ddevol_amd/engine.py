@@ -455,6 +455,18 @@ class Fleet:
     def slot_bytes(self, slot):
         return self.eng.tpgp_fleet_slot_bytes(self.ptr, slot)
 
+    def held_bytes(self):
+        """Device bytes all slots hold (free ones included: the batch keeps their share)."""
+        return sum(self.slot_bytes(s) for s in range(self.n_slots))
+
+    def set_budget(self, nbytes):
+        """Ceiling on the bytes all slots hold together (tpgp_fleet_set_budget); 0: none."""
+        return self.eng.tpgp_fleet_set_budget(self.ptr, int(nbytes))
+
+    def trim(self):
+        """Return grown FREE slots to the opening slot capacity (tpgp_fleet_trim)."""
+        return self.eng.tpgp_fleet_trim(self.ptr)
+
     def close(self):
         if self.ptr:
             self.eng.tpgp_fleet_close(self.ptr)

@@ -6,10 +6,20 @@ advances all running slots together, one Strang step per device round, so reques
 while others run join the next round as soon as a slot and memory are free.  Keys of N <= 32
 spins use one 64-bit word and N > 32 two, so the server runs one fleet per key width.
 
-Admission is by projected term-array bytes against a fixed HBM budget: a request needs its slot
-(three term buffers of `slot_terms` terms at 8*W + 8 bytes each) before it is admitted, and the
-bytes held by running slots (which grow on demand) count against the budget.  Each request's
-results are exactly those of running it alone (the fleet runs each slot's own loop).
+Admission is by projected term-array bytes against a fixed HBM budget.  A request's projection
+is the peak term count its FID is expected to reach (`projected_peak_terms`: measured growth of
+the serving rings per threshold, capped by the Pauli space the dynamics can reach), times three
+term buffers, 8W + 8 bytes a term and the slots' growth headroom.  A request whose projection
+alone (beside every other slot at its opening share) exceeds the budget is refused at once; the
+others wait until a slot is free and the bytes the fleets hold (every slot's share, free slots
+included) plus the reservations of the running requests leave room.  Free slots that grew for
+an earlier request are trimmed back to the opening share (tpgp_fleet_trim) when that is what
+keeps a request waiting.  Each round, a running request's reservation rises to the bytes its
+slot actually holds, so growth beyond the projection is charged before anything else is
+admitted, and each fleet's own ceiling (tpgp_fleet_set_budget) is what the other fleet leaves
+of the budget: a slot that cannot grow within it fails alone (TPGP_ENOMEM) and the others run
+on.  Each request's results are exactly those of running it alone (the fleet runs each slot's
+own loop).
 """
 from __future__ import annotations
 
@@ -66,6 +76,43 @@ def poisson_stream(count, rate, seed=7, n_range=(8, 48), steps_range=(100, 2000)
     return out
 
 
+# Peak term counts of the serving rings (N = 48, nearest-neighbour couplings U[0.5, 1.5],
+# next-nearest 1/8, dt 0.01, initial state sum_s X_s) after a given number of Strang steps, per
+# truncation threshold: the engine's own trajectories on MI355X, sampled every 50 steps.  Larger
+# rings hold more terms at a given step, so the N = 48 curve bounds the smaller rings.
+GROWTH = {
+    1e-4: [(50, 2509), (100, 8097), (150, 21011), (200, 43799), (300, 101616), (400, 145440),
+           (500, 175264), (700, 213004), (1000, 248964), (1500, 276011), (2000, 280936)],
+    1e-5: [(50, 11109), (100, 47087), (150, 159216), (200, 444809), (250, 1050926),
+           (300, 2104761), (400, 5451302), (500, 9266370), (600, 12505092), (800, 17333826),
+           (950, 20083609), (1000, 20877675)],
+    1e-6: [(50, 50549), (100, 323732), (150, 1484618), (200, 5539945), (250, 17242054),
+           (300, 44947444), (350, 99233726), (400, 188122001)],
+}
+SLOT_HEADROOM = 3.0  # a fleet slot whose map fills more than half of it grows to 3x the map
+
+
+def projected_peak_terms(n, threshold, steps):
+    """Term count a ring FID of n spins is expected to reach within `steps` Strang steps at this
+    truncation threshold: the measured curve of the nearest threshold at or below it (smaller
+    thresholds keep more terms), interpolated geometrically, and past its last sample extended
+    at its last growth rate (no saturation assumed); at most 4^n / 4, the part of the Pauli space
+    the dynamics reaches from sum_s X_s."""
+    keys = sorted(GROWTH)
+    thr = max([t for t in keys if t <= threshold * (1 + 1e-9)] or [keys[0]])
+    curve = GROWTH[thr]
+    if steps <= curve[0][0]:
+        peak = curve[0][1] * steps / curve[0][0]
+    elif steps >= curve[-1][0]:
+        (s0, t0), (s1, t1) = curve[-2], curve[-1]
+        peak = t1 * (t1 / t0) ** ((steps - s1) / (s1 - s0))
+    else:
+        k = next(i for i, (st, _) in enumerate(curve) if st >= steps)
+        (s0, t0), (s1, t1) = curve[k - 1], curve[k]
+        peak = t0 * (t1 / t0) ** ((steps - s0) / (s1 - s0))
+    return int(min(peak, 4.0 ** n / 4)) + n
+
+
 def initial_state(n):
     m = E.Map(n)
     for s in range(n):
@@ -82,47 +129,94 @@ class Server:
         self.slot_terms = slot_terms
         self.fleets = {w: E.Fleet(w, slots_per_width, slot_terms, context=context)
                        for w in (1, 2)}
+        for fl in self.fleets.values():
+            fl.set_budget(budget_bytes)
+        # what every slot holds at the least: the opening share, free or not
+        self.share = {w: 3 * (8 * w + 8) * slot_terms for w in (1, 2)}
+        self.floor = slots_per_width * sum(self.share.values())
         self.slot_req = {w: [None] * slots_per_width for w in (1, 2)}
+        self.reserved = {w: [0] * slots_per_width for w in (1, 2)}
+        self.refused = []
         self.rounds = 0
+        self.peak_held = 0
 
     def projected_bytes(self, req: Request):
-        return 3 * self.slot_terms * (8 * req.words + 8)
+        terms = max(projected_peak_terms(req.n, req.threshold, req.steps), self.slot_terms)
+        return int(3 * (8 * req.words + 8) * SLOT_HEADROOM * terms)
+
+    def _slot_charge(self, w, s):
+        """Bytes slot s of width w counts against the budget: what it holds, or what its running
+        request is reserved, whichever is more."""
+        return max(self.fleets[w].slot_bytes(s), self.reserved[w][s])
 
     def held_bytes(self):
-        total = 0
+        return sum(self._slot_charge(w, s) for w in (1, 2)
+                   for s in range(len(self.slot_req[w])))
+
+    def fits(self, req: Request):
+        """Whether `req` fits the budget at all: with every other slot at its opening share."""
+        w = req.words
+        return self.floor - self.share[w] + max(self.projected_bytes(req), self.share[w]) \
+            <= self.budget
+
+    def _trim(self):
+        """Give the memory of grown free slots back (tpgp_fleet_trim); True if any was."""
+        grown = False
         for w, fl in self.fleets.items():
-            for s, r in enumerate(self.slot_req[w]):
-                if r is not None:
-                    total += fl.slot_bytes(s)
-        return total
+            if any(r is None and fl.slot_bytes(s) > self.share[w]
+                   for s, r in enumerate(self.slot_req[w])):
+                st = fl.trim()
+                if st:
+                    raise RuntimeError(f"fleet trim failed: status {st}")
+                grown = True
+        return grown
 
     def _admit(self, req: Request, now):
+        """Admit `req` if a slot and the budget allow it now; True when admitted."""
         w = req.words
-        slots = self.slot_req[w]
-        try:
-            s = slots.index(None)
-        except ValueError:
+        need = self.projected_bytes(req)
+        free = [s for s, r in enumerate(self.slot_req[w]) if r is None]
+        if not free:
             return False
-        if self.held_bytes() + self.projected_bytes(req) > self.budget:
+
+        def place():  # the free slot whose share covers most of the reservation, and the bytes
+            k = min(free, key=lambda k: max(need, self.fleets[w].slot_bytes(k))
+                    - self.fleets[w].slot_bytes(k))
+            held = self.held_bytes() - self._slot_charge(w, k)
+            return k, held + max(need, self.fleets[w].slot_bytes(k))
+
+        s, total = place()
+        if total > self.budget and self._trim():
+            s, total = place()
+        if total > self.budget:
             return False
         st = self.fleets[w].admit(s, initial_state(req.n), req.terms, req.dt, req.steps,
                                   E.options(threshold=req.threshold, max_terms=10 ** 10))
         if st:
             raise RuntimeError(f"admission failed: status {st}")
-        slots[s] = req
+        self.slot_req[w][s] = req
+        self.reserved[w][s] = need
         req.admitted = now
         return True
 
     def run(self, requests, clock=time.perf_counter, log=None):
-        """Serve the stream in real time; returns the requests with results filled in."""
+        """Serve the stream in real time; returns the requests with results filled in.  Refused
+        requests (projection beyond the budget) keep status -1 and land in self.refused."""
         pending = sorted(requests, key=lambda r: r.arrival)
         ready = []
         t0 = clock()
         done = 0
-        while done < len(requests):
+        while done + len(self.refused) < len(requests):
             now = clock() - t0
             while pending and pending[0].arrival <= now:
-                ready.append(pending.pop(0))
+                r = pending.pop(0)
+                if not self.fits(r):
+                    self.refused.append(r)
+                    if log:
+                        log(f"request {r.rid} (N={r.n}, {r.steps} steps, thr {r.threshold:g}) "
+                            f"refused: projected {self.projected_bytes(r) / 2 ** 30:.1f} GiB")
+                else:
+                    ready.append(r)
             while ready and self._admit(ready[0], now):
                 ready.pop(0)
             running = any(r is not None for w in (1, 2) for r in self.slot_req[w])
@@ -131,14 +225,24 @@ class Server:
                     time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
                     continue
                 if ready:
-                    raise RuntimeError("a request does not fit the HBM budget")
+                    raise RuntimeError(
+                        f"request {ready[0].rid} fits the budget ({self.budget} bytes) but "
+                        f"cannot be admitted: {self.held_bytes()} held, "
+                        f"{self.projected_bytes(ready[0])} projected")
+                continue
             for w, fl in self.fleets.items():
                 if any(r is not None for r in self.slot_req[w]):
+                    # the device bytes this fleet's slots may grow to: what the other holds
+                    # leaves the rest of the budget
+                    fl.set_budget(max(1, self.budget - sum(
+                        o.held_bytes() for v, o in self.fleets.items() if v != w)))
                     st, _ = fl.step()
                     if st:
                         raise RuntimeError(f"fleet step failed: status {st}: "
                                            f"{E.N.engine().tpgp_hip_last_error().decode()}")
             self.rounds += 1
+            self.peak_held = max(self.peak_held,
+                                 sum(fl.held_bytes() for fl in self.fleets.values()))
             now = clock() - t0
             for w, fl in self.fleets.items():
                 for s, r in enumerate(self.slot_req[w]):
@@ -151,10 +255,14 @@ class Server:
                         r.finished = now
                         fl.release(s)
                         self.slot_req[w][s] = None
+                        self.reserved[w][s] = 0
                         done += 1
                         if log:
                             log(f"request {r.rid} (N={r.n}, {r.steps} steps, thr {r.threshold:g})"
-                                f" done at {now:.2f} s, latency {r.latency:.2f} s")
+                                f" done at {now:.2f} s, latency {r.latency:.2f} s, status "
+                                f"{status}")
+                    else:  # growth beyond the projection is charged from now on
+                        self.reserved[w][s] = max(self.reserved[w][s], fl.slot_bytes(s))
         return requests
 
     def close(self):

@@ -191,6 +191,16 @@ tpgp_status tpgp_fleet_release(tpgp_fleet *fleet, size_t slot, tpgp_map *final_m
 void tpgp_fleet_counters_get(const tpgp_fleet *fleet, tpgp_evolution_counters *counters);
 /* Bytes of device memory the fleet's slots hold (three term buffers per slot). */
 uint64_t tpgp_fleet_slot_bytes(const tpgp_fleet *fleet, size_t slot);
+/* No counterpart in the reference (its CPU engine runs one FID at a time): a ceiling on the
+ * device bytes all slots hold together (0: none).  When slots must grow, growth is granted
+ * smallest first while the fleet stays under the ceiling; a slot refused by the ceiling, or by
+ * the device's memory, fails alone with TPGP_ENOMEM at its last accepted step, and the other
+ * slots run on unaffected. */
+tpgp_status tpgp_fleet_set_budget(tpgp_fleet *fleet, uint64_t bytes);
+/* No counterpart in the reference: return every FREE slot that has grown to the fleet's
+ * opening slot capacity (one re-layout), giving its device memory back; running slots keep
+ * theirs.  On TPGP_ENOMEM every slot keeps its capacity. */
+tpgp_status tpgp_fleet_trim(tpgp_fleet *fleet);
 
 /* -i[H, A] in the Pauli basis, computed on the GPU (default context on device 0), bit for bit
  * as the reference's tpgp_liouvillian (tpgp_rev.h:89, tpgp_rev.c:810-857); replaces *out on

@@ -202,6 +202,13 @@ int tpgp_hip_batch_reserve(tpgp_hip_batch *batch, size_t map, size_t capacity, c
  * since the last commit is discarded. */
 int tpgp_hip_batch_reserve_many(tpgp_hip_batch *batch, const size_t *capacities, char *error,
                                 size_t error_capacity);
+/* No counterpart in the reference (its maps live in host memory): set slot capacities exactly
+ * (capacities[m] terms, 0 keeps the slot), shrinking as well as growing, in one re-layout.  A
+ * capacity below a map's committed length is rejected (TPGP_HIP_EINVAL) with nothing changed;
+ * on TPGP_HIP_ENOMEM every slot keeps its previous capacity.  Committed states are kept; work
+ * since the last commit is discarded. */
+int tpgp_hip_batch_resize(tpgp_hip_batch *batch, const size_t *capacities, char *error,
+                          size_t error_capacity);
 /* Grow the device log of dropped values and radii to `entries` (sync reports an overflow as
  * TPGP_HIP_ECAPACITY with the needed count in *n_log: roll back, reserve, run again). */
 int tpgp_hip_batch_reserve_log(tpgp_hip_batch *batch, size_t entries, char *error,

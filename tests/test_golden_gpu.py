@@ -88,15 +88,3 @@ def test_golden_both_paths(hip_ctx, golden, name, limit, monkeypatch):
     m, fids, rads, act, leds = gpu_traj(hip_ctx, rec["n"], config_terms(name), fromhex(rec["dt"]),
                                         rec["steps"], fromhex(rec["threshold"]))
     check(rec, m, fids, rads, act, leds)
-
-
-@pytest.mark.parametrize("name", ["ring12", "j1j2_n16", "cluster27", "ring48", "range64"])
-def test_golden_fused_passes(hip_ctx, golden, name, monkeypatch):
-    """The golden trajectories through fused passes (a bond's rotations in one pass over the
-    map), every map on the multi-kernel path."""
-    monkeypatch.setenv("TPGP_HIP_FUSE", "1")
-    monkeypatch.setenv("TPGP_HIP_SMALL_LIMIT", "0")
-    rec = golden[name]
-    m, fids, rads, act, leds = gpu_traj(hip_ctx, rec["n"], config_terms(name), fromhex(rec["dt"]),
-                                        rec["steps"], fromhex(rec["threshold"]))
-    check(rec, m, fids, rads, act, leds)
