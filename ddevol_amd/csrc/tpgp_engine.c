@@ -803,6 +803,8 @@ struct tpgp_evolution {
     tpgp_evolve_options eo;
     size_t n_spins;
     tpgp_evolution_counters c;
+    size_t graph_enq; /* rotations a recorded step enqueues */
+    uint64_t graphs_recorded, graphs_replayed;
 };
 
 tpgp_status tpgp_evolution_open(tpgp_evolution **out, const tpgp_map *m, const tpgp_term *t,
@@ -845,6 +847,53 @@ tpgp_status tpgp_evolution_open(tpgp_evolution **out, const tpgp_map *m, const t
 
 /* `steps` Strang steps with a readout before each and after the last: one host sync per step
  * (the readout of the committed state rides with the step's rotations). */
+/* Readout of the committed state and, with `more`, the step's rotations.  A step that starts
+ * from a state it has started from before is recorded as a HIP graph, and later steps from
+ * that state replay it (tpgp_hip_batch_graph_*): the same work in one launch. */
+static tpgp_status enqueue_step(tpgp_evolution *ev, int more, size_t *enq) {
+    session *S = &ev->S;
+    char err[512];
+    int replayed = 0, capture = 0, captured = 0;
+    *enq = 0;
+    if (more) {
+        int hs = tpgp_hip_batch_graph_replay(S->batch, 1, &replayed, &capture, err, sizeof(err));
+        if (hs) {
+            note("graph replay", err);
+            return map_status(hs);
+        }
+        if (replayed) {
+            *enq = ev->graph_enq;
+            ev->graphs_replayed++;
+            return TPGP_OK;
+        }
+    }
+    int recording = capture && !tpgp_hip_batch_graph_begin(S->batch, 1, err, sizeof(err));
+    for (;;) {
+        tpgp_status s = TPGP_OK;
+        int hs = tpgp_hip_batch_readout(S->batch, ev->eo.pauli, NULL, err, sizeof(err));
+        if (hs) {
+            note("readout", err);
+            s = map_status(hs);
+        } else if (more) {
+            s = enqueue_plans(S, ev->plans, ev->nplans, ev->eo.exact_ledger, &ev->o, 0, enq);
+        }
+        if (!recording) return s;
+        recording = 0;
+        hs = tpgp_hip_batch_graph_end(S->batch, !s, &captured, err, sizeof(err));
+        if (s) return s;
+        if (hs) {
+            note("graph", err);
+            return map_status(hs);
+        }
+        if (captured) {
+            ev->graph_enq = *enq;
+            ev->graphs_recorded++;
+            return TPGP_OK;
+        }
+        *enq = 0; /* not recorded: nothing ran, enqueue the step directly */
+    }
+}
+
 tpgp_status tpgp_evolution_run(tpgp_evolution *ev, size_t steps, tpgp_ledger *l, double *fid,
                                double *fid_radius, size_t *active, size_t *accepted) {
     if (accepted) *accepted = 0;
@@ -868,13 +917,8 @@ tpgp_status tpgp_evolution_run(tpgp_evolution *ev, size_t steps, tpgp_ledger *l,
             return s;
         for (;;) {
             for (size_t k = 0; k < 2 * 64; k++) rd[k] = NAN; /* filled only by a good sync */
-            int hs = tpgp_hip_batch_readout(S->batch, ev->eo.pauli, NULL, err, sizeof(err));
-            if (hs) {
-                note("readout", err);
-                return map_status(hs);
-            }
             size_t enq = 0;
-            if (more && (s = enqueue_plans(S, ev->plans, ev->nplans, exact, o, 0, &enq))) break;
+            if ((s = enqueue_step(ev, more, &enq))) break;
             trial = *l;
             up = io = ex = 0;
             s = sync_fold(S, ev->plans, more ? ev->nplans : 0, enq, exact, o, &trial, rd, &len,
@@ -919,6 +963,12 @@ tpgp_status tpgp_evolution_run(tpgp_evolution *ev, size_t steps, tpgp_ledger *l,
         ev->c.in_out += io;
         if (accepted) *accepted = step + 1;
     }
+}
+
+void tpgp_evolution_graph_counts(const tpgp_evolution *ev, uint64_t *recorded,
+                                 uint64_t *replayed) {
+    if (recorded) *recorded = ev ? ev->graphs_recorded : 0;
+    if (replayed) *replayed = ev ? ev->graphs_replayed : 0;
 }
 
 tpgp_status tpgp_evolution_close(tpgp_evolution *ev, tpgp_map *m) {
@@ -1047,6 +1097,7 @@ typedef struct {
     size_t *act;
     size_t cap;
     size_t want; /* capacity asked for by an overflowing step */
+    uint64_t serial; /* the fleet's admission count when this job was admitted */
 } fleet_slot;
 
 struct tpgp_fleet {
@@ -1066,6 +1117,8 @@ struct tpgp_fleet {
     tpgp_evolution_counters c;
     uint64_t budget; /* device bytes the slots may hold together (0: no limit) */
     size_t cap0;     /* every slot's capacity at open, to which tpgp_fleet_trim returns */
+    uint64_t serial; /* admissions so far */
+    uint64_t graphs_recorded, graphs_replayed;
 };
 
 static void slot_clear(fleet_slot *q) {
@@ -1200,6 +1253,7 @@ tpgp_status tpgp_fleet_admit(tpgp_fleet *f, size_t m, const tpgp_fid_job *job) {
     }
     q->n_spins = init->n_spins;
     q->steps = job->steps;
+    q->serial = ++f->serial;
     q->o = *o;
     q->o.hip_context = f->ctx;
     q->eo = *eo;
@@ -1347,6 +1401,97 @@ tpgp_status tpgp_fleet_trim(tpgp_fleet *f) {
     return hs ? map_status(hs) : TPGP_OK;
 }
 
+/* One round's device work: readouts of the committed states (one call per readout symbol)
+ * and the rotations of every slot with steps left (R requests at most per slot). */
+static tpgp_status fleet_enqueue_direct(tpgp_fleet *f, const int *part, int *pmask, size_t R) {
+    char err[512];
+    const size_t M = f->M;
+    for (unsigned pauli = 1; pauli <= 3; pauli++) {
+        int need_call = 0;
+        for (size_t m = 0; m < M; m++) {
+            pmask[m] = part[m] && f->slot[m].eo.pauli == pauli;
+            need_call |= pmask[m];
+        }
+        if (need_call && tpgp_hip_batch_readout(f->batch, pauli, pmask, err, sizeof(err))) {
+            note("readout", err);
+            return TPGP_EBACKEND;
+        }
+    }
+    if (!R) return TPGP_OK;
+    tpgp_hip_rotation *req = calloc(R * M, sizeof(*req));
+    uint32_t *seq = malloc(R * sizeof(*seq));
+    if (!req || !seq) {
+        free(req);
+        free(seq);
+        return TPGP_ENOMEM;
+    }
+    for (size_t r = 0; r < R; r++) seq[r] = (uint32_t)r;
+    for (size_t m = 0; m < M; m++) {
+        fleet_slot *q = &f->slot[m];
+        for (size_t r = 0; r < q->nenq; r++) {
+            const rot_plan *p = q->plans + q->enq[r];
+            tpgp_hip_rotation *t = req + r * M + m;
+            t->active = 1;
+            t->i = p->i;
+            t->j = p->j;
+            t->qi = p->qi;
+            t->qj = p->qj;
+            t->c = p->c;
+            t->s = p->s;
+            t->rc = p->rc;
+            t->rs = p->rs;
+            t->threshold = q->o.threshold;
+            t->validated = q->o.validated;
+            t->log = q->eo.exact_ledger;
+        }
+    }
+    int hs = tpgp_hip_batch_rotate_many(f->batch, req, seq, R, err, sizeof(err));
+    free(req);
+    free(seq);
+    if (hs) {
+        note("rotate", err);
+        return map_status(hs);
+    }
+    return TPGP_OK;
+}
+
+/* The round's work, replayed from a HIP graph when the same slots (same admissions, same
+ * phase: stepping or taking their final readout) start again from a state recorded before;
+ * recorded when that state has been seen before (tpgp_hip_batch_graph_*). */
+static tpgp_status fleet_enqueue_round(tpgp_fleet *f, const int *part, int *pmask, size_t R) {
+    char err[512];
+    uint64_t key = 1469598103934665603ull;  /* FNV-1a over the round's slots */
+    for (size_t m = 0; m < f->M; m++) {
+        const fleet_slot *q = &f->slot[m];
+        const uint64_t v = part[m] ? (q->serial << 2) | (q->done >= q->steps ? 2u : 1u) : 0;
+        for (int b = 0; b < 64; b += 8) key = (key ^ ((v >> b) & 0xffu)) * 1099511628211ull;
+    }
+    int replayed = 0, capture = 0, captured = 0;
+    int hs = tpgp_hip_batch_graph_replay(f->batch, key, &replayed, &capture, err, sizeof(err));
+    if (hs) {
+        note("graph replay", err);
+        return map_status(hs);
+    }
+    if (replayed) {
+        f->graphs_replayed++;
+        return TPGP_OK;
+    }
+    if (!capture || tpgp_hip_batch_graph_begin(f->batch, key, err, sizeof(err)))
+        return fleet_enqueue_direct(f, part, pmask, R);
+    tpgp_status s = fleet_enqueue_direct(f, part, pmask, R);
+    hs = tpgp_hip_batch_graph_end(f->batch, !s, &captured, err, sizeof(err));
+    if (s) return s;
+    if (hs) {
+        note("graph", err);
+        return map_status(hs);
+    }
+    if (captured) {
+        f->graphs_recorded++;
+        return TPGP_OK;
+    }
+    return fleet_enqueue_direct(f, part, pmask, R); /* not recorded: nothing of it ran */
+}
+
 tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
     if (!f) return TPGP_EINVAL;
     char err[512];
@@ -1387,19 +1532,6 @@ tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
         free(refused);
     }
     while (any && !s) {
-        /* readouts of the committed state, one call per readout symbol */
-        for (unsigned pauli = 1; pauli <= 3 && !s; pauli++) {
-            int need_call = 0;
-            for (size_t m = 0; m < M; m++) {
-                pmask[m] = part[m] && f->slot[m].eo.pauli == pauli;
-                need_call |= pmask[m];
-            }
-            if (need_call && tpgp_hip_batch_readout(f->batch, pauli, pmask, err, sizeof(err))) {
-                note("readout", err);
-                s = TPGP_EBACKEND;
-            }
-        }
-        if (s) break;
         /* this step's rotations: slot m's r-th enqueued rotation is request r */
         size_t R = 0;
         int exact_any = 0;
@@ -1414,44 +1546,7 @@ tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
             if (q->nenq > R) R = q->nenq;
             exact_any |= q->eo.exact_ledger;
         }
-        if (R) {
-            tpgp_hip_rotation *req = calloc(R * M, sizeof(*req));
-            uint32_t *seq = malloc(R * sizeof(*seq));
-            if (!req || !seq) {
-                free(req);
-                free(seq);
-                s = TPGP_ENOMEM;
-                break;
-            }
-            for (size_t r = 0; r < R; r++) seq[r] = (uint32_t)r;
-            for (size_t m = 0; m < M; m++) {
-                fleet_slot *q = &f->slot[m];
-                for (size_t r = 0; r < q->nenq; r++) {
-                    const rot_plan *p = q->plans + q->enq[r];
-                    tpgp_hip_rotation *t = req + r * M + m;
-                    t->active = 1;
-                    t->i = p->i;
-                    t->j = p->j;
-                    t->qi = p->qi;
-                    t->qj = p->qj;
-                    t->c = p->c;
-                    t->s = p->s;
-                    t->rc = p->rc;
-                    t->rs = p->rs;
-                    t->threshold = q->o.threshold;
-                    t->validated = q->o.validated;
-                    t->log = q->eo.exact_ledger;
-                }
-            }
-            int hs = tpgp_hip_batch_rotate_many(f->batch, req, seq, R, err, sizeof(err));
-            free(req);
-            free(seq);
-            if (hs) {
-                note("rotate", err);
-                s = map_status(hs);
-                break;
-            }
-        }
+        if ((s = fleet_enqueue_round(f, part, pmask, R))) break;
         if (exact_any && !f->log) {
             f->log_cap = 1u << 20;
             f->log = malloc(f->log_cap * sizeof(*f->log));
@@ -1589,6 +1684,11 @@ tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
     free(mask);
     free(pmask);
     return s;
+}
+
+void tpgp_fleet_graph_counts(const tpgp_fleet *f, uint64_t *recorded, uint64_t *replayed) {
+    if (recorded) *recorded = f ? f->graphs_recorded : 0;
+    if (replayed) *replayed = f ? f->graphs_replayed : 0;
 }
 
 tpgp_status tpgp_fleet_slot(const tpgp_fleet *f, size_t m, tpgp_slot_state *state,

@@ -380,7 +380,7 @@ extern "C" int tpgp_hip_liouvillian(tpgp_hip_context *ctx, const uint64_t *keys,
             break;
         t.push_back(GenTerm{(uint32_t)q.i, (uint32_t)q.j, q.qi, q.qj, 2 * q.h});
     }
-    std::lock_guard<std::mutex> lock(ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(ctx->mutex);
     DeviceGuard guard(ctx->device);
     if (!guard.ok) {
         msg(error, cap, "HIP device selection failed.");

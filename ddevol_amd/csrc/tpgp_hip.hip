@@ -2736,7 +2736,7 @@ extern "C" int tpgp_hip_set_tuning(tpgp_hip_context *ctx, unsigned words, unsign
     if (!ctx || words < 1 || words > 2 || param >= TPGP_HIP_TUNE_PARAMS ||
         bucket >= TPGP_HIP_TUNE_BUCKETS || !tuning_value_ok(param, value))
         return TPGP_HIP_EINVAL;
-    std::lock_guard<std::mutex> lock(ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(ctx->mutex);
     ctx->tune[words - 1][param][bucket] = (unsigned char)value;
     return TPGP_HIP_OK;
 }
@@ -2746,7 +2746,7 @@ extern "C" int tpgp_hip_get_tuning(tpgp_hip_context *ctx, unsigned words, unsign
     if (!ctx || !value || words < 1 || words > 2 || param >= TPGP_HIP_TUNE_PARAMS ||
         bucket >= TPGP_HIP_TUNE_BUCKETS)
         return TPGP_HIP_EINVAL;
-    std::lock_guard<std::mutex> lock(ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(ctx->mutex);
     const unsigned v = ctx->tune[words - 1][param][bucket];
     *value = v ? v : kTuneDefault[param];
     return TPGP_HIP_OK;
@@ -2870,7 +2870,7 @@ extern "C" tpgp_hip_status tpgp_hip_branches(
                                         s, rc, rs, enclose, out_keys, out_values, out_radii, &z,
                                         error, cap);
     if (st != TPGP_HIP_OK || !count) return st;
-    std::lock_guard<std::mutex> lock(ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(ctx->mutex);
     DeviceGuard guard(ctx->device);
     if (!guard.ok) {
         msg(error, cap, "HIP device selection failed.");
@@ -2942,7 +2942,7 @@ extern "C" tpgp_hip_status tpgp_hip_branches_noalloc(
             ((size_t)TPGP_HIP_NOALLOC_SCRATCH_BYTES - 256) / (24 * words + 40), words);
         return TPGP_HIP_EINVAL;
     }
-    std::lock_guard<std::mutex> lock(ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(ctx->mutex);
     DeviceGuard guard(ctx->device);
     if (!guard.ok) {
         msg(error, cap, "HIP device selection failed.");
@@ -2977,6 +2977,19 @@ struct HostMap {
 struct LoggedLayout {
     uint32_t map, rotation;
     std::vector<uint8_t> sigma;
+};
+// A recorded step (tpgp_hip_batch_graph_*): the HIP graph of the readouts and rotations one
+// step enqueues, with the descriptors they read in a device buffer of their own (the same on
+// every replay), and the host state of every map before and after the step's planning.  It
+// replays only from the exact state it was recorded in (same buffers, digit layouts, floors,
+// record counters) and the same device allocations (gen).
+struct StepGraph {
+    uint64_t key = 0, gen = 0, last_use = 0;
+    std::vector<uint8_t> before, after;
+    std::vector<LoggedLayout> logged;  // exact-ledger layouts the step registers
+    std::vector<uint64_t> est;         // working lengths its launch shapes were planned for
+    hipGraphExec_t exec = nullptr;
+    uint8_t *desc = nullptr;           // device descriptors
 };
 }  // namespace
 
@@ -3027,6 +3040,17 @@ struct tpgp_hip_batch {
     std::vector<Pending> pending;
     double prof_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t prof_count[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // step graphs (TPGP_HIP_GRAPHS=0 disables them)
+    bool graphs_on = true;
+    bool capturing = false, capture_spilled = false;
+    std::vector<uint8_t> cap_host;            // descriptors staged while capturing
+    std::vector<std::vector<uint8_t>> spill;  // staging beyond cap_host (capture is dropped)
+    size_t cap_used = 0, graph_desc_cap = 0;
+    StepGraph recording;
+    size_t logged_mark = 0;
+    std::vector<StepGraph> graphs;
+    std::vector<uint64_t> seen;  // fingerprints of recent pre-step states (capture on recurrence)
+    uint64_t gen = 0, use_clock = 0;
 };
 
 namespace {
@@ -3044,7 +3068,21 @@ template <typename T> hipError_t dalloc(tpgp_hip_batch *b, T **p, size_t count) 
 }
 
 void collect_profile(tpgp_hip_batch *b);
+void drop_graph(StepGraph &g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.desc) (void)hipFree(g.desc);
+    g.exec = nullptr;
+    g.desc = nullptr;
+}
+// Device buffers are about to move: recorded steps point at them.
+void drop_graphs(tpgp_hip_batch *b) {
+    for (StepGraph &g : b->graphs) drop_graph(g);
+    b->graphs.clear();
+    b->seen.clear();
+    ++b->gen;
+}
 void free_all(tpgp_hip_batch *b) {
+    drop_graphs(b);
     collect_profile(b);
     for (auto &R : b->inflight) b->event_pool.push_back(R.ev);
     b->inflight.clear();
@@ -3140,6 +3178,21 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
 void release_stage(tpgp_hip_batch *b);
 template <class T> T *stage(tpgp_hip_batch *b, size_t n, T **dev) {
     const size_t bytes = (n * sizeof(T) + 255) & ~(size_t)255;
+    if (b->capturing) {  // the step graph's own descriptor buffer, uploaded once at its end
+        uint8_t *h;
+        if (b->cap_used + bytes <= b->cap_host.size()) {
+            h = b->cap_host.data() + b->cap_used;
+            *dev = reinterpret_cast<T *>(b->recording.desc + b->cap_used);
+            b->cap_used += bytes;
+        } else {  // too large for the graph's buffer: finish planning, drop the capture
+            b->capture_spilled = true;
+            b->spill.emplace_back(bytes);
+            h = b->spill.back().data();
+            *dev = reinterpret_cast<T *>(b->recording.desc);
+        }
+        std::memset((void *)h, 0, bytes);
+        return reinterpret_cast<T *>(h);
+    }
     if (b->stage_used + bytes > b->stage_cap) b->stage_used = 0;
     const size_t lo = b->stage_used, hi = lo + bytes;
     size_t keep = 0;
@@ -3161,6 +3214,7 @@ template <class T> T *stage(tpgp_hip_batch *b, size_t n, T **dev) {
 }
 hipEvent_t pooled_event(tpgp_hip_batch *b);
 template <class T> hipError_t push_stage(tpgp_hip_batch *b, T *h, T *d, size_t n) {
+    if (b->capturing) return hipSuccess;  // uploaded with the whole graph
     const size_t bytes = n * sizeof(T);
     hipError_t e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, b->ctx->stream);
     if (e != hipSuccess) return e;
@@ -3285,7 +3339,7 @@ struct ProfileScope {
     unsigned kind;
     hipEvent_t start = nullptr;
     ProfileScope(tpgp_hip_batch *bb, unsigned k) : b(bb), kind(k) {
-        if (!(b->profiling >> kind & 1u)) return;
+        if (b->capturing || !(b->profiling >> kind & 1u)) return;
         start = pooled_event(b);
         if (start) (void)hipEventRecord(start, b->ctx->stream);
     }
@@ -3412,6 +3466,15 @@ void to_reference(const std::vector<uint8_t> &sigma, const uint64_t *stored, uin
 
 }  // namespace
 
+namespace {
+// Calls that synchronise or move device buffers cannot run inside a step being recorded.
+bool recording_step(const tpgp_hip_batch *b, char *error, size_t cap) {
+    if (!b->capturing) return false;
+    msg(error, cap, "A step graph is being recorded: end it first.");
+    return true;
+}
+}  // namespace
+
 extern "C" tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *ctx, size_t n_words,
                                                  size_t n_maps, const size_t *n_spins,
                                                  const size_t *capacities, char *error,
@@ -3463,7 +3526,9 @@ extern "C" tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *ctx, size_t n
     b->small_fixed = sl && *sl;
     b->small_limit = b->small_fixed ? strtoull(sl, nullptr, 10) : 0;
     b->small.assign(n_maps, 0);
-    std::lock_guard<std::mutex> lock(ctx->mutex);
+    const char *gr = getenv("TPGP_HIP_GRAPHS");
+    b->graphs_on = !(gr && *gr == '0');
+    std::lock_guard<std::recursive_mutex> lock(ctx->mutex);
     DeviceGuard guard(ctx->device);
     if (!guard.ok || allocate_device(b, error, cap) != TPGP_HIP_OK) {
         if (!guard.ok) msg(error, cap, "HIP device selection failed.");
@@ -3476,8 +3541,12 @@ extern "C" tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *ctx, size_t n
 
 extern "C" void tpgp_hip_batch_destroy(tpgp_hip_batch *b) {
     if (!b) return;
+    if (b->capturing) {  // a step left recording: drop it (nothing of it ran)
+        int captured = 0;
+        (void)tpgp_hip_batch_graph_end(b, 0, &captured, nullptr, 0);
+    }
     {
-        std::lock_guard<std::mutex> lock(b->ctx->mutex);
+        std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
         DeviceGuard guard(b->ctx->device);
         (void)hipStreamSynchronize(b->ctx->stream);
         free_all(b);
@@ -3504,13 +3573,14 @@ extern "C" int tpgp_hip_batch_upload(tpgp_hip_batch *b, size_t map, const uint64
         msg(error, cap, "Invalid upload arguments.");
         return TPGP_HIP_EINVAL;
     }
+    if (recording_step(b, error, cap)) return TPGP_HIP_EINVAL;
     HostMap &hm = b->maps[map];
     if (len > hm.cap) {
         msg(error, cap, "Map %zu: %zu terms exceed the slot capacity %llu.", map, len,
             (unsigned long long)hm.cap);
         return TPGP_HIP_ECAPACITY;
     }
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     hipStream_t st = b->ctx->stream;
     hipError_t e = hipStreamSynchronize(st);
@@ -3562,9 +3632,10 @@ extern "C" int tpgp_hip_batch_assign(tpgp_hip_batch *b, size_t map, size_t n_spi
         msg(error, cap, "Invalid assign arguments (map %zu, %zu spins).", map, n_spins);
         return TPGP_HIP_EINVAL;
     }
+    if (recording_step(b, error, cap)) return TPGP_HIP_EINVAL;
     HostMap &hm = b->maps[map];
     if (hm.n_spins != n_spins) {
-        std::lock_guard<std::mutex> lock(b->ctx->mutex);
+        std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
         DeviceGuard guard(b->ctx->device);
         hipError_t e = hipStreamSynchronize(b->ctx->stream);
         hm.n_spins = n_spins;
@@ -3716,7 +3787,7 @@ extern "C" int tpgp_hip_batch_rotate_many(tpgp_hip_batch *b, const tpgp_hip_rota
                 return TPGP_HIP_EINVAL;
             }
         }
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     // rotations go out in chunks whose descriptors fill at most a quarter of the staging ring,
     // so the host stages one chunk while the device runs the previous ones
@@ -3847,7 +3918,7 @@ extern "C" int tpgp_hip_batch_readout(tpgp_hip_batch *b, unsigned pauli, const i
             msg(error, cap, "Too many readouts since the last sync (map %u).", m);
             return TPGP_HIP_EINVAL;
         }
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     ReadoutMap *d;
     ReadoutMap *h = stage(b, b->M, &d);
@@ -3879,6 +3950,7 @@ extern "C" int tpgp_hip_batch_sum_squares(tpgp_hip_batch *b, size_t map, double 
         msg(error, cap, "Invalid sum-of-squares arguments.");
         return TPGP_HIP_EINVAL;
     }
+    if (recording_step(b, error, cap)) return TPGP_HIP_EINVAL;
     HostMap &hm = b->maps[map];
     if (hm.kcur != hm.kc || hm.nrec || hm.nrd) {
         msg(error, cap, "Map %zu has uncommitted work; sync and commit or roll back first.", map);
@@ -3887,7 +3959,7 @@ extern "C" int tpgp_hip_batch_sum_squares(tpgp_hip_batch *b, size_t map, double 
     out[0] = out[1] = 0.0;
     const uint64_t len = hm.committed_len;
     if (!len) return TPGP_HIP_OK;
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     const double *vals = b->dev.arena[hm.kc].vals + hm.off;
     const uint32_t grid = (uint32_t)std::min<uint64_t>(kSumBlocks, (len + 255) / 256);
@@ -3914,7 +3986,8 @@ extern "C" int tpgp_hip_batch_sync(tpgp_hip_batch *b, tpgp_hip_record *records,
                                    char *error, size_t cap) {
     msg(error, cap, "");
     if (!b) return TPGP_HIP_EINVAL;
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    if (recording_step(b, error, cap)) return TPGP_HIP_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     hipStream_t st = b->ctx->stream;
     hipError_t e = hipStreamSynchronize(st);
@@ -4038,7 +4111,8 @@ extern "C" int tpgp_hip_batch_commit(tpgp_hip_batch *b, const int *mask, char *e
                                      size_t cap) {
     msg(error, cap, "");
     if (!b) return TPGP_HIP_EINVAL;
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    if (recording_step(b, error, cap)) return TPGP_HIP_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     hipError_t e = hipStreamSynchronize(b->ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "HIP commit", error, cap);
@@ -4066,7 +4140,8 @@ extern "C" int tpgp_hip_batch_commit(tpgp_hip_batch *b, const int *mask, char *e
 extern "C" int tpgp_hip_batch_rollback(tpgp_hip_batch *b, char *error, size_t cap) {
     msg(error, cap, "");
     if (!b) return TPGP_HIP_EINVAL;
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    if (recording_step(b, error, cap)) return TPGP_HIP_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     for (auto &hm : b->maps) {
@@ -4093,6 +4168,7 @@ extern "C" int tpgp_hip_batch_download(tpgp_hip_batch *b, size_t map, uint64_t *
         msg(error, cap, "Invalid download arguments.");
         return TPGP_HIP_EINVAL;
     }
+    if (recording_step(b, error, cap)) return TPGP_HIP_EINVAL;
     HostMap &hm = b->maps[map];
     *len = hm.committed_len;
     if (!keys && !values && capacity == 0) return TPGP_HIP_OK;
@@ -4109,7 +4185,7 @@ extern "C" int tpgp_hip_batch_download(tpgp_hip_batch *b, size_t map, uint64_t *
         return TPGP_HIP_EINVAL;
     }
     {
-        std::lock_guard<std::mutex> lock(b->ctx->mutex);
+        std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
         DeviceGuard guard(b->ctx->device);
         hipError_t e = hipStreamSynchronize(b->ctx->stream);
         if (e != hipSuccess) return hip_fail(e, "HIP download", error, cap);
@@ -4281,6 +4357,7 @@ extern "C" int tpgp_hip_batch_reserve_many(tpgp_hip_batch *b, const size_t *capa
         msg(error, cap, "Invalid reserve arguments.");
         return TPGP_HIP_EINVAL;
     }
+    if (recording_step(b, error, cap)) return TPGP_HIP_EINVAL;
     bool grow = false;
     std::vector<uint64_t> want(b->M);
     for (uint32_t m = 0; m < b->M; ++m) {
@@ -4292,7 +4369,7 @@ extern "C" int tpgp_hip_batch_reserve_many(tpgp_hip_batch *b, const size_t *capa
         grow |= capacities[m] > b->maps[m].cap;
     }
     if (!grow) return TPGP_HIP_OK;
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     return relayout(b, want, error, cap);
 }
@@ -4304,6 +4381,7 @@ extern "C" int tpgp_hip_batch_resize(tpgp_hip_batch *b, const size_t *capacities
         msg(error, cap, "Invalid resize arguments.");
         return TPGP_HIP_EINVAL;
     }
+    if (recording_step(b, error, cap)) return TPGP_HIP_EINVAL;
     bool change = false;
     std::vector<uint64_t> want(b->M);
     for (uint32_t m = 0; m < b->M; ++m) {
@@ -4315,7 +4393,7 @@ extern "C" int tpgp_hip_batch_resize(tpgp_hip_batch *b, const size_t *capacities
         change |= want[m] != b->maps[m].cap;
     }
     if (!change) return TPGP_HIP_OK;
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     return relayout(b, want, error, cap);
 }
@@ -4340,8 +4418,9 @@ extern "C" int tpgp_hip_batch_reserve_log(tpgp_hip_batch *b, size_t entries, cha
         msg(error, cap, "Invalid log reserve arguments.");
         return TPGP_HIP_EINVAL;
     }
+    if (recording_step(b, error, cap)) return TPGP_HIP_EINVAL;
     if (entries <= b->dev.log_cap) return TPGP_HIP_OK;
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     hipError_t e = hipStreamSynchronize(b->ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "HIP log reserve", error, cap);
@@ -4359,12 +4438,187 @@ extern "C" int tpgp_hip_batch_reserve_log(tpgp_hip_batch *b, size_t entries, cha
     b->dev.log = fresh;
     b->dev.log_cap = entries;
     b->log_entries = entries;
+    drop_graphs(b);  // recorded steps write to the old log
+    return TPGP_HIP_OK;
+}
+
+namespace {
+// The host state of every map that a step's planning reads and advances.
+std::vector<uint8_t> host_state(const tpgp_hip_batch *b) {
+    std::vector<uint8_t> out;
+    for (const HostMap &hm : b->maps) {
+        const uint32_t f[6] = {(uint32_t)hm.n_spins, (uint32_t)hm.kc, (uint32_t)hm.kcur, hm.nrec,
+                               hm.nrd, 0};
+        uint64_t fl;
+        std::memcpy(&fl, &hm.floor, 8);
+        const uint8_t *p = reinterpret_cast<const uint8_t *>(f);
+        out.insert(out.end(), p, p + sizeof(f));
+        p = reinterpret_cast<const uint8_t *>(&fl);
+        out.insert(out.end(), p, p + 8);
+        out.insert(out.end(), hm.sig.begin(), hm.sig.end());
+    }
+    return out;
+}
+void set_host_state(tpgp_hip_batch *b, const std::vector<uint8_t> &st) {
+    size_t at = 0;
+    for (HostMap &hm : b->maps) {
+        uint32_t f[6];
+        std::memcpy(f, st.data() + at, sizeof(f));
+        at += sizeof(f);
+        std::memcpy(&hm.floor, st.data() + at, 8);
+        at += 8;
+        hm.kcur = (int)f[2];
+        hm.nrec = f[3];
+        hm.nrd = f[4];
+        std::memcpy(hm.sig.data(), st.data() + at, hm.n_spins);
+        at += hm.n_spins;
+    }
+}
+uint64_t fingerprint(uint64_t key, const std::vector<uint8_t> &st) {
+    uint64_t h = 1469598103934665603ull ^ key;  // FNV-1a
+    for (uint8_t c : st) h = (h ^ c) * 1099511628211ull;
+    return h;
+}
+bool graphs_usable(const tpgp_hip_batch *b) {
+    return b->graphs_on && !b->validate && !b->profiling && !b->poisoned;
+}
+constexpr size_t kMaxGraphs = 6;
+}  // namespace
+
+extern "C" int tpgp_hip_batch_graph_replay(tpgp_hip_batch *b, uint64_t key, int *replayed,
+                                           int *capture, char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!b || !replayed || b->capturing) {
+        msg(error, cap, "Invalid graph replay arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    *replayed = 0;
+    if (capture) *capture = 0;
+    if (!graphs_usable(b)) return TPGP_HIP_OK;
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
+    DeviceGuard guard(b->ctx->device);
+    const std::vector<uint8_t> st = host_state(b);
+    for (size_t k = 0; k < b->graphs.size(); ++k) {
+        StepGraph &g = b->graphs[k];
+        if (g.key != key || g.gen != b->gen || g.before != st) continue;
+        // launch shapes were planned for the lengths at capture: record again once a map has
+        // grown or shrunk well past them
+        bool stale = false;
+        for (uint32_t m = 0; m < b->M; ++m)
+            stale |= 2 * b->maps[m].est_len > 3 * g.est[m] + 8192 ||
+                     3 * b->maps[m].est_len + 8192 < 2 * g.est[m];
+        if (stale) {
+            drop_graph(g);
+            b->graphs.erase(b->graphs.begin() + (long)k);
+            if (capture) *capture = 1;
+            return TPGP_HIP_OK;
+        }
+        hipError_t e = hipGraphLaunch(g.exec, b->ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "HIP graph launch", error, cap);
+        set_host_state(b, g.after);
+        b->logged.insert(b->logged.end(), g.logged.begin(), g.logged.end());
+        g.last_use = ++b->use_clock;
+        *replayed = 1;
+        return TPGP_HIP_OK;
+    }
+    // record a state the step has started from before: it is likely to recur again
+    const uint64_t fp = fingerprint(key, st);
+    const bool recurs = std::find(b->seen.begin(), b->seen.end(), fp) != b->seen.end();
+    if (!recurs) {
+        b->seen.push_back(fp);
+        if (b->seen.size() > 16) b->seen.erase(b->seen.begin());
+    }
+    if (capture) *capture = recurs;
+    return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_graph_begin(tpgp_hip_batch *b, uint64_t key, char *error,
+                                          size_t cap) {
+    msg(error, cap, "");
+    if (!b || b->capturing || !graphs_usable(b)) {
+        msg(error, cap, "Invalid graph capture arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    b->ctx->mutex.lock();  // held until tpgp_hip_batch_graph_end: nothing else joins the graph
+    DeviceGuard guard(b->ctx->device);
+    if (!b->graph_desc_cap) b->graph_desc_cap = b->stage_cap;
+    StepGraph &g = b->recording;
+    g = StepGraph{};
+    hipError_t e = hipMalloc((void **)&g.desc, b->graph_desc_cap);
+    if (e == hipSuccess) e = hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+        if (g.desc) (void)hipFree(g.desc);
+        g.desc = nullptr;
+        b->ctx->mutex.unlock();
+        return hip_fail(e, "HIP graph capture", error, cap);
+    }
+    g.key = key;
+    g.gen = b->gen;
+    g.before = host_state(b);
+    for (const HostMap &hm : b->maps) g.est.push_back(hm.est_len);
+    b->cap_host.assign(b->graph_desc_cap, 0);
+    b->cap_used = 0;
+    b->spill.clear();
+    b->capture_spilled = false;
+    b->logged_mark = b->logged.size();
+    b->capturing = true;
+    return TPGP_HIP_OK;
+}
+
+extern "C" int tpgp_hip_batch_graph_end(tpgp_hip_batch *b, int keep, int *captured,
+                                        char *error, size_t cap) {
+    msg(error, cap, "");
+    if (!b || !captured || !b->capturing) {
+        msg(error, cap, "Invalid graph end arguments.");
+        return TPGP_HIP_EINVAL;
+    }
+    *captured = 0;
+    DeviceGuard guard(b->ctx->device);
+    StepGraph &g = b->recording;
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(b->ctx->stream, &graph);
+    b->capturing = false;
+    bool ok = keep && e == hipSuccess && graph && !b->capture_spilled;
+    if (ok) ok = (e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0)) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ok)
+        ok = (e = hipMemcpy(g.desc, b->cap_host.data(), b->cap_used, hipMemcpyHostToDevice)) ==
+             hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        // nothing of the step ran: return the maps to the state it was planned from
+        set_host_state(b, g.before);
+        b->logged.resize(b->logged_mark);
+        if (b->capture_spilled) b->graph_desc_cap *= 2;  // room for the next attempt
+        drop_graph(g);
+        b->spill.clear();
+        b->ctx->mutex.unlock();
+        return TPGP_HIP_OK;  // *captured = 0: enqueue the step again, without capture
+    }
+    g.after = host_state(b);
+    g.logged.assign(b->logged.begin() + (long)b->logged_mark, b->logged.end());
+    g.last_use = ++b->use_clock;
+    if (b->graphs.size() >= kMaxGraphs) {  // the least recently used goes
+        size_t lru = 0;
+        for (size_t k = 1; k < b->graphs.size(); ++k)
+            if (b->graphs[k].last_use < b->graphs[lru].last_use) lru = k;
+        drop_graph(b->graphs[lru]);
+        b->graphs.erase(b->graphs.begin() + (long)lru);
+    }
+    b->graphs.push_back(g);
+    g = StepGraph{};  // the batch's list owns the graph now
+    e = hipGraphLaunch(b->graphs.back().exec, b->ctx->stream);
+    b->cap_host.clear();
+    b->cap_host.shrink_to_fit();
+    b->ctx->mutex.unlock();
+    if (e != hipSuccess) return hip_fail(e, "HIP graph launch", error, cap);
+    *captured = 1;
     return TPGP_HIP_OK;
 }
 
 extern "C" int tpgp_hip_batch_profile(tpgp_hip_batch *b, unsigned mask) {
     if (!b) return TPGP_HIP_EINVAL;
-    std::lock_guard<std::mutex> lock(b->ctx->mutex);
+    std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     collect_profile(b);

@@ -11,14 +11,16 @@
 #include "../../include/tpgp_hip.h"
 
 // One device, its stream and its persistent scratch; calls on a context are serialised by the
-// mutex (as the reference's CUDA context, cuda_rev.cu:22).
+// mutex (as the reference's CUDA context, cuda_rev.cu:22).  It is recursive because a batch
+// that records a step graph holds it from the start of the capture to its end, across the
+// calls that enqueue the step.
 struct tpgp_hip_context {
     int device;
     int cus;
     hipStream_t stream;
     void *storage;
     size_t capacity;
-    std::mutex mutex;
+    std::recursive_mutex mutex;
     char name[256];
     // launch-shape choices (tpgp_hip_set_tuning): per key width (1 or 2 words), parameter and
     // size bucket; 0 = the default

@@ -269,6 +269,12 @@ class Evolution:
         """Enable HIP-event timing for the kinds in `mask` (bit k = kind k; 0 disables)."""
         return self.eng.tpgp_evolution_profile(self.ptr, int(mask))
 
+    def graph_counts(self):
+        """(steps recorded as HIP graphs, steps replayed from them)."""
+        rec, rep = C.c_uint64(), C.c_uint64()
+        self.eng.tpgp_evolution_graph_counts(self.ptr, C.byref(rec), C.byref(rep))
+        return rec.value, rep.value
+
     def profile_read(self, kind):
         ms, n = C.c_double(), C.c_uint64()
         self.eng.tpgp_evolution_profile_read(self.ptr, kind, C.byref(ms), C.byref(n))
@@ -466,6 +472,12 @@ class Fleet:
     def trim(self):
         """Return grown FREE slots to the opening slot capacity (tpgp_fleet_trim)."""
         return self.eng.tpgp_fleet_trim(self.ptr)
+
+    def graph_counts(self):
+        """(rounds recorded as HIP graphs, rounds replayed from them)."""
+        rec, rep = C.c_uint64(), C.c_uint64()
+        self.eng.tpgp_fleet_graph_counts(self.ptr, C.byref(rec), C.byref(rep))
+        return rec.value, rep.value
 
     def close(self):
         if self.ptr:

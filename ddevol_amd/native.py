@@ -138,7 +138,8 @@ HIP_SYMBOLS = ["tpgp_hip_compiled", "tpgp_hip_create", "tpgp_hip_destroy",
                "tpgp_hip_batch_info", "tpgp_hip_batch_stream", "tpgp_hip_batch_profile",
                "tpgp_hip_batch_profile_read", "tpgp_hip_batch_assign",
                "tpgp_hip_batch_reserve_log", "tpgp_hip_batch_reserve_many",
-               "tpgp_hip_batch_resize",
+               "tpgp_hip_batch_resize", "tpgp_hip_batch_graph_replay",
+               "tpgp_hip_batch_graph_begin", "tpgp_hip_batch_graph_end",
                "tpgp_hip_batch_sum_squares", "tpgp_hip_liouvillian", "tpgp_hip_set_tuning",
                "tpgp_hip_get_tuning"]
 ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
@@ -152,10 +153,11 @@ ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_evolution_open", "tpgp_evolution_run", "tpgp_evolution_close",
                   "tpgp_evolution_counters_get", "tpgp_evolution_profile",
                   "tpgp_evolution_profile_read", "tpgp_evolution_key_words",
+                  "tpgp_evolution_graph_counts",
                   "tpgp_fleet_open", "tpgp_fleet_close", "tpgp_fleet_admit", "tpgp_fleet_step",
                   "tpgp_fleet_slot", "tpgp_fleet_fid", "tpgp_fleet_release",
                   "tpgp_fleet_counters_get", "tpgp_fleet_slot_bytes", "tpgp_fleet_set_budget",
-                  "tpgp_fleet_trim",
+                  "tpgp_fleet_trim", "tpgp_fleet_graph_counts",
                   "tpgp_checkpoint_free", "tpgp_checkpoint_save", "tpgp_checkpoint_load",
                   "tpgp_checkpoint_same_configuration", "tpgp_payload_sha256",
                   "tpgp_evolution_snapshot", "tpgp_evolution_l2", "tpgp_liouvillian"]
@@ -205,6 +207,10 @@ def hip():
     f("tpgp_hip_batch_reserve_log", C.c_int, C.c_void_p, sz, p_c, sz)
     f("tpgp_hip_batch_reserve_many", C.c_int, C.c_void_p, C.POINTER(sz), p_c, sz)
     f("tpgp_hip_batch_resize", C.c_int, C.c_void_p, C.POINTER(sz), p_c, sz)
+    f("tpgp_hip_batch_graph_replay", C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_int),
+      C.POINTER(C.c_int), p_c, sz)
+    f("tpgp_hip_batch_graph_begin", C.c_int, C.c_void_p, C.c_uint64, p_c, sz)
+    f("tpgp_hip_batch_graph_end", C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int), p_c, sz)
     f("tpgp_hip_batch_sum_squares", C.c_int, C.c_void_p, sz, p_d, p_c, sz)
     f("tpgp_hip_liouvillian", C.c_int, C.c_void_p, p_u64, p_d, sz, sz, sz, C.POINTER(TpgpTerm),
       sz, p_u64, p_d, sz, C.POINTER(sz), p_c, sz)
@@ -274,6 +280,8 @@ def engine():
     f("tpgp_evolution_profile", C.c_int, C.c_void_p, C.c_uint)
     f("tpgp_evolution_profile_read", C.c_int, C.c_void_p, C.c_uint, pd, C.POINTER(C.c_uint64))
     f("tpgp_evolution_key_words", sz, C.c_void_p)
+    f("tpgp_evolution_graph_counts", None, C.c_void_p, C.POINTER(C.c_uint64),
+      C.POINTER(C.c_uint64))
     vp = C.c_void_p
     f("tpgp_fleet_open", C.c_int, C.POINTER(vp), vp, sz, sz, sz)
     f("tpgp_fleet_close", None, vp)
@@ -286,6 +294,7 @@ def engine():
     f("tpgp_fleet_slot_bytes", C.c_uint64, vp, sz)
     f("tpgp_fleet_set_budget", C.c_int, vp, C.c_uint64)
     f("tpgp_fleet_trim", C.c_int, vp)
+    f("tpgp_fleet_graph_counts", None, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
     cp = C.POINTER(Checkpoint)
     f("tpgp_checkpoint_free", None, cp)
     f("tpgp_checkpoint_save", C.c_int, C.c_char_p, cp, C.c_char_p, sz)

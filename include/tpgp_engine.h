@@ -147,6 +147,11 @@ int tpgp_evolution_profile(tpgp_evolution *evolution, unsigned mask);
 int tpgp_evolution_profile_read(const tpgp_evolution *evolution, unsigned kind,
                                 double *milliseconds, uint64_t *launches);
 size_t tpgp_evolution_key_words(const tpgp_evolution *evolution);
+/* No counterpart in the reference: steps recorded as HIP graphs and steps replayed from them
+ * (a step that starts from a state seen before is recorded, later ones replay it; results are
+ * those of enqueueing every step directly; TPGP_HIP_GRAPHS=0 turns this off). */
+void tpgp_evolution_graph_counts(const tpgp_evolution *evolution, uint64_t *recorded,
+                                 uint64_t *replayed);
 
 /* ---- Batched and served FIDs ------------------------------------------------------------
  * A fleet runs many independent FIDs (powder orientations, coupling draws, served requests)
@@ -201,6 +206,9 @@ tpgp_status tpgp_fleet_set_budget(tpgp_fleet *fleet, uint64_t bytes);
  * opening slot capacity (one re-layout), giving its device memory back; running slots keep
  * theirs.  On TPGP_ENOMEM every slot keeps its capacity. */
 tpgp_status tpgp_fleet_trim(tpgp_fleet *fleet);
+/* Rounds recorded as HIP graphs and rounds replayed from them, as for an evolution (the same
+ * slots with the same jobs, in the same phase, from a state seen before). */
+void tpgp_fleet_graph_counts(const tpgp_fleet *fleet, uint64_t *recorded, uint64_t *replayed);
 
 /* -i[H, A] in the Pauli basis, computed on the GPU (default context on device 0), bit for bit
  * as the reference's tpgp_liouvillian (tpgp_rev.h:89, tpgp_rev.c:810-857); replaces *out on

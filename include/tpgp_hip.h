@@ -213,6 +213,29 @@ int tpgp_hip_batch_resize(tpgp_hip_batch *batch, const size_t *capacities, char 
  * TPGP_HIP_ECAPACITY with the needed count in *n_log: roll back, reserve, run again). */
 int tpgp_hip_batch_reserve_log(tpgp_hip_batch *batch, size_t entries, char *error,
                                size_t error_capacity);
+/* No counterpart in the reference (its CPU engine launches nothing): step graphs.  A caller
+ * that enqueues the same step again and again (readouts and rotate_many calls between two
+ * syncs) records it once as a HIP graph and replays it with one launch:
+ *   tpgp_hip_batch_graph_replay: if a graph recorded under `key` starts from the batch's
+ *     present host state (every map's buffers, digit layout, floor and record counters, and the
+ *     same device allocations), launch it and advance that state as the step's planning would
+ *     (*replayed = 1).  Otherwise *replayed = 0, and *capture (optional) is 1 when this state
+ *     has been seen before under `key`, i.e. a recording is likely to be replayed.
+ *   tpgp_hip_batch_graph_begin / _end: record the readouts and rotations enqueued in between
+ *     (nothing runs while recording; the context is held by the recording thread); _end with
+ *     keep != 0 instantiates the graph and launches it once (*captured = 1).  With keep == 0
+ *     (a call failed while recording), or if the graph could not be made, the maps return to
+ *     their state at _begin and *captured = 0: nothing of the step ran, enqueue it again
+ *     without recording.  Other batch calls are refused while recording.
+ * A graph's descriptors live in its own device buffer; graphs are dropped whenever device
+ * buffers move (re-layout, log growth).  Results are those of enqueueing the step directly.
+ * TPGP_HIP_GRAPHS=0 disables recording and replay; so do validation and profiling. */
+int tpgp_hip_batch_graph_replay(tpgp_hip_batch *batch, uint64_t key, int *replayed,
+                                int *capture, char *error, size_t error_capacity);
+int tpgp_hip_batch_graph_begin(tpgp_hip_batch *batch, uint64_t key, char *error,
+                               size_t error_capacity);
+int tpgp_hip_batch_graph_end(tpgp_hip_batch *batch, int keep, int *captured, char *error,
+                             size_t error_capacity);
 /* Committed length and slot capacity of a map. */
 int tpgp_hip_batch_info(const tpgp_hip_batch *batch, size_t map, size_t *committed_len,
                         size_t *capacity);

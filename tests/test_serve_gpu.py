@@ -1,8 +1,9 @@
 """Serving on the GPU: admission by projected bytes, deferral, refusal, and per-slot failure.
 
-* Server (ddevol_amd/serve.py): a stream under a budget that holds two requests at a time runs
-  them two at a time, every result bit for bit equal to the request run alone; a request whose
-  projection exceeds the budget is refused without stopping the others.
+* Server (ddevol_amd/serve.py): a stream under a budget that holds two or three requests at a
+  time never holds more device bytes than the budget, every result bit for bit equal to the
+  request run alone; requests projected large run in their own fleet beside the small ones; a
+  request whose projection exceeds the budget is refused without stopping the others.
 * Fleet budget (tpgp_fleet_set_budget): a slot that would have to grow beyond the budget fails
   alone with ENOMEM at its last accepted step; its neighbours finish unchanged.
 * Trim (tpgp_fleet_trim): a free slot that grew for a finished job returns to the opening share,
@@ -48,32 +49,60 @@ def test_server_defers_to_the_budget_and_matches_alone(hip_ctx):
     slot_terms = 4096
     probe = S.Server(hip_ctx, slots_per_width=8, budget_bytes=1 << 40, slot_terms=slot_terms)
     need = max(probe.projected_bytes(r) for r in reqs)
-    probe.close()
+    key = (1, "small")
+    assert all(probe.fleet_of(r) == key for r in reqs)
     # every slot starts with slot_terms terms: the fleets' shares count whether used or not;
     # a running request adds what its reservation holds beyond its slot's share
-    share = 3 * 16 * slot_terms
-    base = 8 * share + 8 * 3 * 24 * slot_terms
+    floor, share = probe.floor, probe.share[key]
+    probe.close()
     extra = need - share
     assert extra > 0
-    budget = base + 2 * extra + extra // 2  # room for two of the largest, never for all six
+    budget = floor + 2 * extra + extra // 2  # room for two of the largest, never for all six
     srv = S.Server(hip_ctx, slots_per_width=8, budget_bytes=budget, slot_terms=slot_terms)
     running_max = 0
-    orig_step = srv.fleets[1].step
+    orig_step = srv.fleets[key].step
 
     def step():  # at every round: the requests in the fleet, and what they are charged
         nonlocal running_max
-        running_max = max(running_max, sum(r is not None for r in srv.slot_req[1]))
+        running_max = max(running_max, srv.running(key))
         assert srv.held_bytes() <= budget
         assert sum(fl.held_bytes() for fl in srv.fleets.values()) <= budget
         return orig_step()
 
-    srv.fleets[1].step = step
+    srv.fleets[key].step = step
     print(f"budget {budget} bytes, reservation up to {need}", flush=True)
     srv.run(reqs)
     srv.close()
     assert not srv.refused
     assert 2 <= running_max < len(reqs)
     assert any(r.admitted > r.arrival + 1e-3 for r in reqs), "nothing had to wait"
+    for r in reqs:
+        st, fid = alone_fid(hip_ctx, r)
+        assert r.status == st == 0
+        assert np.array_equal(r.fid.view(np.uint64), fid.view(np.uint64)), r.rid
+
+
+def test_server_runs_large_requests_apart(hip_ctx, monkeypatch):
+    """Requests projected past LARGE_TERMS run in the large-map fleet, the rest in the small
+    one; both fleets take rounds, and every FID equals the request's run alone."""
+    monkeypatch.setattr(S, "LARGE_TERMS", 3000)
+    reqs = small_stream()
+    big = [r for r in reqs if S.projected_peak_terms(r.n, r.threshold, r.steps) >= 3000]
+    assert 0 < len(big) < len(reqs)
+    srv = S.Server(hip_ctx, slots_per_width=8, budget_bytes=1 << 34, slot_terms=4096,
+                   large_slots=2)
+    stepped = {}
+    orig = srv._round
+
+    def round_(k, clock):
+        stepped[k] = stepped.get(k, 0) + 1
+        return orig(k, clock)
+
+    srv._round = round_
+    srv.run(reqs)
+    srv.close()
+    print(f"rounds per fleet: {stepped}", flush=True)
+    assert set(stepped) == {(1, "small"), (1, "large")}
     for r in reqs:
         st, fid = alone_fid(hip_ctx, r)
         assert r.status == st == 0
