@@ -229,7 +229,7 @@ int liouvillian(tpgp_hip_context *ctx, const uint64_t *keys, const double *value
         (e = hipMemcpyAsync(d_terms, h_terms, n_valid * sizeof(GenTerm), hipMemcpyHostToDevice,
                             st)) ||
         (e = hipMemsetAsync(flag, 0, 2 * sizeof(uint32_t), st)))
-        return hip_fail(e, "HIP generator upload", error, cap);
+        return drain_fail(st, e, "HIP generator upload", error, cap);
     hipLaunchKernelGGL(k_gen_count<W>, dim3((unsigned)nblocks), dim3(kGenBlock), 0, st, d_keys,
                        (uint64_t)count, d_terms, n_valid, counts);
     const uint64_t ncells = nblocks * n_valid;

@@ -3159,7 +3159,7 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
         slots[m] = MapSlot{b->maps[m].off, b->maps[m].cap, (uint32_t)b->maps[m].n_spins, 0};
     if ((e = hipMemcpyAsync(b->d_slot, slots.data(), M * sizeof(MapSlot), hipMemcpyHostToDevice,
                             st)) != hipSuccess)
-        return hip_fail(e, "HIP slot upload", error, cap);
+        return drain_fail(st, e, "HIP slot upload", error, cap);
     hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, st, d.len, (uint64_t)M * 3, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, st, d.regions, (uint64_t)M * 3 * 17,
                        0ull);
@@ -3605,7 +3605,7 @@ extern "C" int tpgp_hip_batch_upload(tpgp_hip_batch *b, size_t map, const uint64
         if (e == hipSuccess)
             e = hipMemcpyAsync(b->dev.arena[k].vals + hm.off, values, len * 8,
                                hipMemcpyHostToDevice, st);
-        if (e != hipSuccess) return hip_fail(e, "HIP upload transfer", error, cap);
+        if (e != hipSuccess) return drain_fail(st, e, "HIP upload transfer", error, cap);
     }
     hipLaunchKernelGGL(k_set_len, dim3(1), dim3(1), 0, st, b->dev.len, (uint64_t)map * 3 + k,
                        (uint64_t)len);
@@ -4214,7 +4214,8 @@ extern "C" int tpgp_hip_batch_download(tpgp_hip_batch *b, size_t map, uint64_t *
             if (e == hipSuccess)
                 e = hipMemcpyAsync(values, b->dev.arena[k].vals + hm.off, hm.committed_len * 8,
                                    hipMemcpyDeviceToHost, b->ctx->stream);
-            if (e != hipSuccess) return hip_fail(e, "HIP download transfer", error, cap);
+            if (e != hipSuccess)
+                return drain_fail(b->ctx->stream, e, "HIP download transfer", error, cap);
         }
         e = hipStreamSynchronize(b->ctx->stream);
         if (e != hipSuccess) return hip_fail(e, "HIP download", error, cap);
@@ -4279,7 +4280,9 @@ int relayout(tpgp_hip_batch *b, const std::vector<uint64_t> &new_cap, char *erro
         if (e == hipSuccess)
             e = hipMemcpyAsync(dv, b->dev.arena[hm.kc].vals + hm.off, n[m] * 8, kind, st);
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    // wait even after a failed enqueue: copies already queued still write into hk / hv
+    const hipError_t drained = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = drained;
     if (e != hipSuccess) {
         if (tk) (void)hipFree(tk);
         if (tv) (void)hipFree(tv);
@@ -4341,7 +4344,9 @@ int relayout(tpgp_hip_batch *b, const std::vector<uint64_t> &new_cap, char *erro
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->dev.regions, nreg.data(), nreg.size() * 8, hipMemcpyHostToDevice,
                            st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    // wait even after a failed enqueue: queued copies still read tk / tv and hk / hv
+    const hipError_t done = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = done;
     if (tk) (void)hipFree(tk);
     if (tv) (void)hipFree(tv);
     b->logged.clear();

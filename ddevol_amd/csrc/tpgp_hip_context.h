@@ -41,6 +41,13 @@ inline tpgp_hip_status hip_fail(hipError_t code, const char *what, char *e, size
     (void)hipGetLastError();  // reported now: a later launch check must not report it again
     return code == hipErrorOutOfMemory ? TPGP_HIP_ENOMEM : TPGP_HIP_EDEVICE;
 }
+// A failure after copies were queued on `st`: wait for the stream first, so that no queued copy
+// outlives the host buffer it reads or writes, then report.
+inline tpgp_hip_status drain_fail(hipStream_t st, hipError_t code, const char *what, char *e,
+                                  size_t n) {
+    (void)hipStreamSynchronize(st);
+    return hip_fail(code, what, e, n);
+}
 struct DeviceGuard {  // save/restore the caller's current device
     int previous = -1;
     bool ok = false;
