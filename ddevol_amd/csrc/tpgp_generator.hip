@@ -9,8 +9,8 @@
 //   count   per (term, block of 256 keys): the number of anticommuting keys;
 //   scan    exclusive prefix over (term, block) in term-major order: every contribution's slot;
 //   emit    each contribution at its slot (term-major, key-minor order);
-//   sort    stable LSD radix sort by key (64-bit words, lowest word first): contributions of one
-//           key stay in emission order;
+//   sort    stable LSD radix sort by key, 4 bits a pass from the lowest digit of the low word
+//           to the highest of the high word: contributions of one key stay in emission order;
 //   sum     one thread per run of equal keys adds its contributions left to right; a run that
 //           sums to exactly zero leaves no key (as the reference's removal on a zero sum; a later
 //           insert restarts from zero exactly);
@@ -19,8 +19,12 @@
 // Statuses follow the reference's term order: terms are processed up to the first invalid one;
 // a non-finite contribution or partial sum among them is ENUMERIC, otherwise the invalid term is
 // EINVAL.  Work is O(len * terms) and memory (8W + 8) * 3 bytes per contribution.
+//
+// The scans and the sort are this file's own kernels: a tiled three-kernel scan (tile sums,
+// one workgroup over the tile sums, tiles with their carry-in) and a stable scatter whose ranks
+// come from wave ballots on the digit's bits (lanes with the same digit, lower lanes first),
+// wave counts in LDS and a running per-digit base across the tile's rounds.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <cstdint>
@@ -110,25 +114,160 @@ __global__ __launch_bounds__(kGenBlock) void k_gen_emit(const uint64_t *keys, co
     if (bad) atomicOr(nonfinite, 1u);
 }
 
-// Split interleaved keys (W words each) into one array per word.
-template <int W>
-__global__ void k_gen_split(const uint64_t *keys, uint64_t n, uint64_t *w0, uint64_t *w1) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        w0[i] = keys[i * W];
-        if (W > 1) w1[i] = keys[i * W + 1];
+// ---- scans: out[i] = sum of in[0..i) (exclusive) or in[0..i] (inclusive), 64-bit sums -------
+constexpr int kScanThreads = 256, kScanItems = 8;
+constexpr uint64_t kScanTile = (uint64_t)kScanThreads * kScanItems;
+
+// Inclusive scan of one value per thread over the workgroup (wave scans by shuffles, then the
+// wave totals); *total receives the workgroup's sum.
+__device__ uint64_t block_inclusive(uint64_t x, uint64_t *total) {
+    __shared__ uint64_t wsum[kScanThreads / 64];
+    const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (unsigned d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint64_t before = 0, all = 0;
+    for (unsigned w = 0; w < kScanThreads / 64; ++w) {
+        if (w < wave) before += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();  // wsum is reused by the next call
+    *total = all;
+    return x + before;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const T *in, uint64_t n,
+                                                             uint64_t *tile_sum) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint64_t x = 0;
+    for (int r = 0; r < kScanItems; ++r) {
+        const uint64_t i = base + (uint64_t)r * kScanThreads + threadIdx.x;
+        if (i < n) x += (uint64_t)in[i];
+    }
+    uint64_t total;
+    (void)block_inclusive(x, &total);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+// One workgroup: exclusive scan of the tile sums in place.
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(uint64_t *tile_sum, uint64_t n) {
+    uint64_t carry = 0;
+    for (uint64_t base = 0; base < n; base += kScanThreads) {
+        const uint64_t i = base + threadIdx.x;
+        const uint64_t x = i < n ? tile_sum[i] : 0;
+        uint64_t total;
+        const uint64_t inc = block_inclusive(x, &total);
+        if (i < n) tile_sum[i] = carry + inc - x;
+        carry += total;
     }
 }
+// Each tile from its carry-in: element order within the tile is round-major (round r covers
+// elements base + r * 256 .. + 255), so a round's scan continues the previous round's.
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const T *in, uint64_t n,
+                                                             const uint64_t *tile_sum,
+                                                             uint64_t *out, int inclusive) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint64_t carry = tile_sum[blockIdx.x];
+    for (int r = 0; r < kScanItems; ++r) {
+        const uint64_t i = base + (uint64_t)r * kScanThreads + threadIdx.x;
+        const uint64_t x = i < n ? (uint64_t)in[i] : 0;
+        uint64_t total;
+        const uint64_t inc = block_inclusive(x, &total);
+        if (i < n) out[i] = carry + (inclusive ? inc : inc - x);
+        carry += total;
+    }
+}
+
+// ---- stable radix sort of (key, emission index) pairs, 4 bits a pass ------------------------
+constexpr int kSortThreads = 256, kSortRounds = 8;
+constexpr uint64_t kSortTile = (uint64_t)kSortThreads * kSortRounds;
+
+template <int W>
+__device__ __forceinline__ unsigned digit_of(const uint64_t *keys, uint64_t i, unsigned word,
+                                             unsigned shift) {
+    return (unsigned)(keys[i * W + word] >> shift) & 15u;
+}
+// Lanes of the wave whose digit equals this lane's (valid lanes only).
+__device__ __forceinline__ uint64_t same_digit(unsigned d, bool valid) {
+    uint64_t m = __ballot(valid);
+    for (unsigned b = 0; b < 4; ++b) {
+        const uint64_t set = __ballot(valid && (d >> b & 1u));
+        m &= (d >> b & 1u) ? set : ~set;
+    }
+    return m;
+}
+
+// Per tile: how many of its elements have each digit.  hist[d * tiles + tile] (digit-major, so
+// one exclusive scan gives every (digit, tile) its first output position).
+template <int W>
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint64_t *keys, uint64_t n,
+                                                            unsigned word, unsigned shift,
+                                                            uint32_t *hist) {
+    __shared__ uint32_t cnt[16];
+    if (threadIdx.x < 16) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    const unsigned lane = threadIdx.x & 63u;
+    for (int r = 0; r < kSortRounds; ++r) {
+        const uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
+        const bool valid = i < n;
+        const unsigned d = valid ? digit_of<W>(keys, i, word, shift) : 0u;
+        const uint64_t m = same_digit(d, valid);
+        // the lowest lane of each digit group counts the group
+        if (valid && (m & ((1ull << lane) - 1ull)) == 0) atomicAdd(&cnt[d], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) hist[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter: element i goes to first[digit][tile] + (elements of its digit before it in
+// the tile).  Within a round, lower waves and lower lanes come first.
+template <int W>
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
+    const uint64_t *keys, const uint32_t *idx, uint64_t n, unsigned word, unsigned shift,
+    const uint64_t *first, uint64_t *keys_out, uint32_t *idx_out) {
+    constexpr unsigned kWaves = kSortThreads / 64;
+    // wave counts by round parity: the digit totals of round r are read (threads 0-15) while
+    // other waves may already clear the counts of round r + 1
+    __shared__ uint32_t wcnt[2][kWaves][16];
+    __shared__ uint64_t run[16];  // elements of each digit placed so far by this tile
+    const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (threadIdx.x < 16) run[threadIdx.x] = first[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x];
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRounds; ++r) {
+        uint32_t (*cnt)[16] = wcnt[r & 1];
+        if (lane < 16) cnt[wave][lane] = 0;
+        __syncthreads();  // run[] of the previous round is final; this round's counts are clear
+        const uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
+        const bool valid = i < n;
+        const unsigned d = valid ? digit_of<W>(keys, i, word, shift) : 0u;
+        const uint64_t m = same_digit(d, valid);
+        const unsigned below = (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+        if (valid && below == 0) cnt[wave][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint64_t at = run[d] + below;
+            for (unsigned w = 0; w < wave; ++w) at += cnt[w][d];
+            for (int k = 0; k < W; ++k) keys_out[at * W + k] = keys[i * W + k];
+            idx_out[at] = idx[i];
+        }
+        __syncthreads();  // every lane has read run[] before it moves on
+        if (threadIdx.x < 16) {
+            uint64_t add = 0;
+            for (unsigned w = 0; w < kWaves; ++w) add += cnt[w][threadIdx.x];
+            run[threadIdx.x] += add;
+        }
+    }
+}
+
 __global__ void k_gen_iota(uint32_t *p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = (uint32_t)i;
-}
-__global__ void k_gen_gather_u64(const uint64_t *src, const uint32_t *idx, uint64_t *dst,
-                                 uint64_t n) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        dst[i] = src[idx[i]];
 }
 __global__ void k_gen_gather_f64(const double *src, const uint32_t *idx, double *dst,
                                  uint64_t n) {
@@ -136,13 +275,21 @@ __global__ void k_gen_gather_f64(const double *src, const uint32_t *idx, double 
          i += (uint64_t)gridDim.x * blockDim.x)
         dst[i] = src[idx[i]];
 }
-
-// Run heads of the sorted keys (word arrays w0, w1): 1 where a key differs from its predecessor.
-template <int W>
-__global__ void k_gen_heads(const uint64_t *w0, const uint64_t *w1, uint64_t n, uint32_t *head) {
+__global__ void k_gen_zero_u32(uint32_t *p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
-        head[i] = i == 0 || w0[i] != w0[i - 1] || (W > 1 && w1[i] != w1[i - 1]);
+        p[i] = 0;
+}
+
+// Run heads of the sorted keys (W words each): 1 where a key differs from its predecessor.
+template <int W>
+__global__ void k_gen_heads(const uint64_t *keys, uint64_t n, uint32_t *head) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        bool h = i == 0;
+        for (int k = 0; k < W && !h; ++k) h = keys[i * W + k] != keys[(i - 1) * W + k];
+        head[i] = h;
+    }
 }
 // Start of every run (run index = inclusive count of heads - 1).
 __global__ void k_gen_starts(const uint32_t *head, const uint64_t *run_of, uint64_t n,
@@ -171,15 +318,14 @@ __global__ void k_gen_sum(const double *v, const uint64_t *start, uint64_t runs,
 }
 // Kept runs to their output positions (exclusive count of kept runs before them).
 template <int W>
-__global__ void k_gen_compact(const uint64_t *w0, const uint64_t *w1, const uint64_t *start,
-                              const double *sum, const uint32_t *keep, const uint64_t *pos,
-                              uint64_t runs, uint64_t *out_keys, double *out_values) {
+__global__ void k_gen_compact(const uint64_t *keys, const uint64_t *start, const double *sum,
+                              const uint32_t *keep, const uint64_t *pos, uint64_t runs,
+                              uint64_t *out_keys, double *out_values) {
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < runs;
          r += (uint64_t)gridDim.x * blockDim.x) {
         if (!keep[r]) continue;
         const uint64_t at = pos[r], i = start[r];
-        out_keys[at * W] = w0[i];
-        if (W > 1) out_keys[at * W + 1] = w1[i];
+        for (int k = 0; k < W; ++k) out_keys[at * W + k] = keys[i * W + k];
         out_values[at] = sum[r];
     }
 }
@@ -205,6 +351,49 @@ unsigned grid_for(uint64_t n) {
     return (unsigned)(g < 65536 ? (g ? g : 1) : 65536);
 }
 
+// out = exclusive (or inclusive) prefix sums of in[0..n), n >= 1.
+template <typename T>
+hipError_t scan(hipStream_t st, Arena &A, const T *in, uint64_t n, uint64_t *out,
+                int inclusive) {
+    const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+    uint64_t *sums = A.get<uint64_t>(tiles);
+    if (A.err != hipSuccess) return A.err;
+    hipLaunchKernelGGL(k_scan_tiles<T>, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, in, n,
+                       sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, sums, tiles);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, in, n,
+                       sums, out, inclusive);
+    return hipGetLastError();
+}
+
+// Stable sort of n (key, index) pairs by key: 4-bit passes over the low word's lo_bits, then
+// (two words) the high word's hi_bits.  keys/idx hold the input and receive the result (the
+// passes alternate with the arena's second buffers).
+template <int W>
+hipError_t sort_pairs(hipStream_t st, Arena &A, uint64_t *&keys, uint32_t *&idx, uint64_t n,
+                      unsigned lo_bits, unsigned hi_bits) {
+    const uint64_t tiles = (n + kSortTile - 1) / kSortTile;
+    uint32_t *hist = A.get<uint32_t>(16 * tiles);
+    uint64_t *first = A.get<uint64_t>(16 * tiles);
+    uint64_t *keys2 = A.get<uint64_t>(n * W);
+    uint32_t *idx2 = A.get<uint32_t>(n);
+    if (A.err != hipSuccess) return A.err;
+    for (unsigned word = 0; word < (unsigned)W; ++word) {
+        const unsigned bits = word == 0 ? lo_bits : hi_bits;
+        for (unsigned shift = 0; shift < bits; shift += 4) {
+            hipLaunchKernelGGL(k_sort_hist<W>, dim3((unsigned)tiles), dim3(kSortThreads), 0, st,
+                               keys, n, word, shift, hist);
+            hipError_t e = scan(st, A, hist, 16 * tiles, first, 0);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_sort_scatter<W>, dim3((unsigned)tiles), dim3(kSortThreads), 0,
+                               st, keys, idx, n, word, shift, first, keys2, idx2);
+            std::swap(keys, keys2);
+            std::swap(idx, idx2);
+        }
+    }
+    return hipGetLastError();
+}
+
 template <int W>
 int liouvillian(tpgp_hip_context *ctx, const uint64_t *keys, const double *values, size_t count,
                 size_t n_spins, const GenTerm *h_terms, uint32_t n_valid, uint64_t *out_keys,
@@ -227,9 +416,9 @@ int liouvillian(tpgp_hip_context *ctx, const uint64_t *keys, const double *value
     if ((e = hipMemcpyAsync(d_keys, keys, count * W * 8, hipMemcpyHostToDevice, st)) ||
         (e = hipMemcpyAsync(d_vals, values, count * 8, hipMemcpyHostToDevice, st)) ||
         (e = hipMemcpyAsync(d_terms, h_terms, n_valid * sizeof(GenTerm), hipMemcpyHostToDevice,
-                            st)) ||
-        (e = hipMemsetAsync(flag, 0, 2 * sizeof(uint32_t), st)))
+                            st)))
         return drain_fail(st, e, "HIP generator upload", error, cap);
+    hipLaunchKernelGGL(k_gen_zero_u32, dim3(1), dim3(64), 0, st, flag, 2ull);
     hipLaunchKernelGGL(k_gen_count<W>, dim3((unsigned)nblocks), dim3(kGenBlock), 0, st, d_keys,
                        (uint64_t)count, d_terms, n_valid, counts);
     const uint64_t ncells = nblocks * n_valid;
@@ -238,16 +427,13 @@ int liouvillian(tpgp_hip_context *ctx, const uint64_t *keys, const double *value
             (unsigned long long)ncells);
         return TPGP_HIP_ENOMEM;
     }
-    size_t tmp_bytes = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, offsets, (int)ncells, st);
-    void *tmp = A.get<unsigned char>(tmp_bytes);
-    if (A.err != hipSuccess) return hip_fail(A.err, "HIP generator allocation", error, cap);
-    hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, offsets, (int)ncells, st);
+    if ((e = scan(st, A, counts, ncells, offsets, 0)) != hipSuccess)
+        return drain_fail(st, e, "HIP generator scan", error, cap);
     uint64_t last[2];
     if ((e = hipMemcpyAsync(&last[0], offsets + ncells - 1, 8, hipMemcpyDeviceToHost, st)) ||
         (e = hipMemcpyAsync(&last[1], counts + ncells - 1, 8, hipMemcpyDeviceToHost, st)) ||
         (e = hipStreamSynchronize(st)))
-        return hip_fail(e, "HIP generator count", error, cap);
+        return drain_fail(st, e, "HIP generator count", error, cap);
     const uint64_t C = last[0] + last[1];  // contributions
     if (C == 0) return TPGP_HIP_OK;
     if (C >= (1ull << 31)) {
@@ -257,58 +443,29 @@ int liouvillian(tpgp_hip_context *ctx, const uint64_t *keys, const double *value
     }
     uint64_t *ck = A.get<uint64_t>(C * W);
     double *cv = A.get<double>(C);
+    uint32_t *order = A.get<uint32_t>(C);  // emission index of every (sorted) contribution
+    double *vs = A.get<double>(C);
     if (A.err != hipSuccess) return hip_fail(A.err, "HIP generator allocation", error, cap);
     hipLaunchKernelGGL(k_gen_emit<W>, dim3((unsigned)nblocks), dim3(kGenBlock), 0, st, d_keys,
                        d_vals, (uint64_t)count, d_terms, n_valid, offsets, ck, cv, flag);
-    // stable LSD sort by key: the low word, then (two words) the high word -- contributions of
-    // one key keep their emission order
-    uint64_t *lo = A.get<uint64_t>(C), *hi = W > 1 ? A.get<uint64_t>(C) : nullptr;
-    uint64_t *keys_a = A.get<uint64_t>(C), *lo_sorted = A.get<uint64_t>(C);
-    uint64_t *hi_sorted = W > 1 ? A.get<uint64_t>(C) : nullptr;
-    uint32_t *perm0 = A.get<uint32_t>(C), *perm1 = A.get<uint32_t>(C);
-    uint32_t *perm2 = W > 1 ? A.get<uint32_t>(C) : nullptr;
-    double *vs = A.get<double>(C);
-    if (A.err != hipSuccess) return hip_fail(A.err, "HIP generator allocation", error, cap);
     const unsigned g = grid_for(C);
-    const int n = (int)C;
-    hipLaunchKernelGGL(k_gen_split<W>, dim3(g), dim3(256), 0, st, ck, C, lo, hi);
-    hipLaunchKernelGGL(k_gen_iota, dim3(g), dim3(256), 0, st, perm0, C);
-    const int lo_bits = W == 1 ? (int)(2 * n_spins) : 64;
-    const int hi_bits = W == 1 ? 0 : (int)(2 * (n_spins - 32));
-    size_t sort_bytes = 0, sort_bytes2 = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, lo, lo_sorted, perm0, perm1, n, 0,
-                                       lo_bits, st);
-    if (W > 1)
-        hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes2, keys_a, hi_sorted, perm1, perm2,
-                                           n, 0, hi_bits, st);
-    void *stmp = A.get<unsigned char>(sort_bytes > sort_bytes2 ? sort_bytes : sort_bytes2);
-    if (A.err != hipSuccess) return hip_fail(A.err, "HIP generator allocation", error, cap);
-    hipcub::DeviceRadixSort::SortPairs(stmp, sort_bytes, lo, lo_sorted, perm0, perm1, n, 0,
-                                       lo_bits, st);
-    const uint32_t *order = perm1;  // emission index of every sorted contribution
-    if (W > 1) {
-        hipLaunchKernelGGL(k_gen_gather_u64, dim3(g), dim3(256), 0, st, hi, perm1, keys_a, C);
-        hipcub::DeviceRadixSort::SortPairs(stmp, sort_bytes2, keys_a, hi_sorted, perm1, perm2, n,
-                                           0, hi_bits, st);
-        hipLaunchKernelGGL(k_gen_gather_u64, dim3(g), dim3(256), 0, st, lo, perm2, lo_sorted, C);
-        order = perm2;
-    }
+    hipLaunchKernelGGL(k_gen_iota, dim3(g), dim3(256), 0, st, order, C);
+    // stable by key: contributions of one key keep their emission (term-major) order
+    const unsigned lo_bits = W == 1 ? (unsigned)(2 * n_spins) : 64u;
+    const unsigned hi_bits = W == 1 ? 0u : (unsigned)(2 * (n_spins - 32));
+    if ((e = sort_pairs<W>(st, A, ck, order, C, lo_bits, hi_bits)) != hipSuccess)
+        return drain_fail(st, e, "HIP generator sort", error, cap);
     hipLaunchKernelGGL(k_gen_gather_f64, dim3(g), dim3(256), 0, st, cv, order, vs, C);
-    const uint64_t *w0s = lo_sorted, *w1s = hi_sorted;
-    // sorted words: w0s (low) and, for two words, w1s (high)
     uint32_t *head = A.get<uint32_t>(C);
     uint64_t *run_of = A.get<uint64_t>(C);
-    if (A.err != hipSuccess) return hip_fail(A.err, "HIP generator allocation", error, cap);
-    hipLaunchKernelGGL(k_gen_heads<W>, dim3(g), dim3(256), 0, st, w0s, w1s, C, head);
-    size_t scan_bytes = 0;
-    hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, head, run_of, n, st);
-    void *rtmp = A.get<unsigned char>(scan_bytes);
-    if (A.err != hipSuccess) return hip_fail(A.err, "HIP generator allocation", error, cap);
-    hipcub::DeviceScan::InclusiveSum(rtmp, scan_bytes, head, run_of, n, st);
+    if (A.err != hipSuccess) return drain_fail(st, A.err, "HIP generator allocation", error, cap);
+    hipLaunchKernelGGL(k_gen_heads<W>, dim3(g), dim3(256), 0, st, ck, C, head);
+    if ((e = scan(st, A, head, C, run_of, 1)) != hipSuccess)
+        return drain_fail(st, e, "HIP generator scan", error, cap);
     uint64_t runs = 0;
     if ((e = hipMemcpyAsync(&runs, run_of + C - 1, 8, hipMemcpyDeviceToHost, st)) ||
         (e = hipStreamSynchronize(st)))
-        return hip_fail(e, "HIP generator runs", error, cap);
+        return drain_fail(st, e, "HIP generator runs", error, cap);
     uint64_t *start = A.get<uint64_t>(runs), *pos = A.get<uint64_t>(runs);
     double *sum = A.get<double>(runs);
     uint32_t *keep = A.get<uint32_t>(runs);
@@ -317,18 +474,15 @@ int liouvillian(tpgp_hip_context *ctx, const uint64_t *keys, const double *value
     const unsigned gr = grid_for(runs);
     hipLaunchKernelGGL(k_gen_sum, dim3(gr), dim3(256), 0, st, vs, start, runs, C, sum, keep,
                        flag);
-    size_t keep_bytes = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, keep_bytes, keep, pos, (int)runs, st);
-    void *ktmp = A.get<unsigned char>(keep_bytes);
-    if (A.err != hipSuccess) return hip_fail(A.err, "HIP generator allocation", error, cap);
-    hipcub::DeviceScan::ExclusiveSum(ktmp, keep_bytes, keep, pos, (int)runs, st);
+    if ((e = scan(st, A, keep, runs, pos, 0)) != hipSuccess)
+        return drain_fail(st, e, "HIP generator scan", error, cap);
     uint64_t last_pos = 0;
     uint32_t last_keep = 0, bad = 0;
     if ((e = hipMemcpyAsync(&last_pos, pos + runs - 1, 8, hipMemcpyDeviceToHost, st)) ||
         (e = hipMemcpyAsync(&last_keep, keep + runs - 1, 4, hipMemcpyDeviceToHost, st)) ||
         (e = hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, st)) ||
         (e = hipStreamSynchronize(st)))
-        return hip_fail(e, "HIP generator sums", error, cap);
+        return drain_fail(st, e, "HIP generator sums", error, cap);
     if (bad) {
         *numeric = 1;
         return TPGP_HIP_OK;
@@ -343,8 +497,8 @@ int liouvillian(tpgp_hip_context *ctx, const uint64_t *keys, const double *value
     uint64_t *ok_keys = A.get<uint64_t>(kept * W);
     double *ok_vals = A.get<double>(kept);
     if (A.err != hipSuccess) return hip_fail(A.err, "HIP generator allocation", error, cap);
-    hipLaunchKernelGGL(k_gen_compact<W>, dim3(gr), dim3(256), 0, st, w0s, w1s, start, sum, keep,
-                       pos, runs, ok_keys, ok_vals);
+    hipLaunchKernelGGL(k_gen_compact<W>, dim3(gr), dim3(256), 0, st, ck, start, sum, keep, pos,
+                       runs, ok_keys, ok_vals);
     if ((e = hipGetLastError()) != hipSuccess)
         return hip_fail(e, "HIP generator launch", error, cap);
     if (kept && ((e = hipMemcpyAsync(out_keys, ok_keys, kept * W * 8, hipMemcpyDeviceToHost,
