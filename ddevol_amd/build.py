@@ -3,6 +3,8 @@
 * ``ddevol_amd/lib/libtpgp_hip.so``    -- HIP kernels + C ABI (hipcc, gfx950)
 * ``ddevol_amd/lib/libtpgp_engine.so`` -- C17 host engine (gcc), linked to the above, with MPFR
   when its headers are present (as the reference's TPGP_HAVE_MPFR build).
+* ``ddevol_amd/lib/libtpgp_cuda_shim.so`` -- the reference's five ``tpgp_cuda_*`` backend symbols
+  forwarded to the above, for programs built against the reference's CUDA header.
 * ``ddevol_amd/lib/tpgp_rev``           -- the FID run driver (command line of the reference's
   ``tpgp_rev``) on the engine above.
 
@@ -92,6 +94,11 @@ def build(verbose: bool = False) -> None:
             "-Wl,-rpath,$ORIGIN", *libs, "-lm", "-lpthread"]
     run(cmd)
     os.replace(tmp, eng_so)
+    shim = os.path.join(LIB, "libtpgp_cuda_shim.so")
+    tmp = shim + ".tmp"
+    run(["gcc", *CC_FLAGS, "-o", tmp, os.path.join(CSRC, "tpgp_cuda_shim.c"), f"-L{LIB}",
+         "-ltpgp_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, shim)
     cli = os.path.join(LIB, "tpgp_rev")
     tmp = cli + ".tmp"
     run(["gcc", *[f for f in CC_FLAGS if f not in ("-shared", "-fPIC")], "-o", tmp,

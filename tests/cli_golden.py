@@ -11,6 +11,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden", "cli")
 OURS = os.path.join(ROOT, "ddevol_amd", "lib", "tpgp_rev")
 REF_CLI = os.path.join(ROOT, "oracle", "_ref", "tpgp_rev_cli")
+# the reference's driver, unedited, linked against libtpgp_engine.so + libtpgp_cuda_shim.so
+REF_ON_ENGINE = os.path.join(ROOT, "oracle", "_ref", "tpgp_rev_cli_on_engine")
 
 
 def golden():

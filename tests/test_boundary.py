@@ -31,6 +31,23 @@ def test_exports(header, lib):
     assert not missing, missing
 
 
+def test_cuda_shim_exports():
+    """libtpgp_cuda_shim.so carries the reference's five backend symbols
+    (tpgp_cuda_rev.h:20-49), forwarding to tpgp_hip_*; without a device it reports the build
+    capability and a create failure, as the reference's backend does."""
+    path = os.path.join(N.LIB, "libtpgp_cuda_shim.so")
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (run python -m ddevol_amd.build)")
+    so = C.CDLL(path)
+    for name in ("tpgp_cuda_compiled", "tpgp_cuda_create", "tpgp_cuda_destroy",
+                 "tpgp_cuda_device_name", "tpgp_cuda_branches"):
+        assert hasattr(so, name), name
+    assert so.tpgp_cuda_compiled() == 1
+    so.tpgp_cuda_device_name.restype = C.c_char_p
+    so.tpgp_cuda_device_name.argtypes = [C.c_void_p]
+    assert so.tpgp_cuda_device_name(None) == b"unavailable"
+
+
 def test_binding_lists_cover_headers():
     assert set(declared("tpgp_hip.h")) <= set(N.HIP_SYMBOLS)
     assert set(declared("tpgp_engine.h")) <= set(N.ENGINE_SYMBOLS)

@@ -74,6 +74,31 @@ def test_driver_resumes_reference_records(tmp_path, case, first):
                                      "rb").read()
 
 
+@pytest.mark.skipif(not os.path.exists(G.REF_ON_ENGINE),
+                    reason="reference driver not linked against the engine (make -C oracle ref)")
+@pytest.mark.parametrize("case", sorted(GOLD))
+def test_reference_driver_on_engine(tmp_path, case):
+    """The drop-in of the core API: the reference's own driver (cli_rev.c, compiled from its
+    sources without edits) linked against libtpgp_engine.so and libtpgp_cuda_shim.so in place of
+    its core library and CUDA stub.  With `--backend cuda` (the slot the GPU occupies) every
+    step runs tpgp_strang_step on the MI355X; each invocation of every golden chain must give
+    the reference's exit code, header, rows and records, as this repository's driver does."""
+    ck = tmp_path / "run.ck"
+    for k, inv in enumerate(GOLD[case]):
+        args = G.argv(inv["args"], ck)
+        if "--restart" not in args:
+            args += ["--backend", "cuda"]
+        print(f"case {case}[{k}]: {' '.join(args)}", flush=True)
+        code, header, rows, err = G.run(G.REF_ON_ENGINE, args, tmp_path / f"{k}.csv")
+        assert code == inv["exit"], (case, k, code, err)
+        assert G.same_header(header, inv["header"]), (case, k, header, inv["header"])
+        check_rows(rows, inv["rows"], (case, k))
+        if inv["checkpoint"]:
+            mine = record_as_reference(ck, tmp_path / f"{k}.as_ref.ck")
+            want = os.path.join(G.GOLDEN, f"{case}.{k}.ck")
+            assert mine.read_bytes() == open(want, "rb").read(), (case, k)
+
+
 @pytest.mark.skipif(not os.path.exists(G.REF_CLI), reason="reference driver not built")
 @pytest.mark.parametrize("case", ["restart", "cap", "walltime", "residual_restart"])
 def test_reference_resumes_our_records(tmp_path, case):
