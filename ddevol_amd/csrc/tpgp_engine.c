@@ -992,6 +992,16 @@ static char default_err[512];
 static void default_create(void) {
     default_ctx = tpgp_hip_create(0, default_err, sizeof(default_err));
 }
+/* For the other units of this library (the residual bound): the same context, or NULL with the
+ * reason noted; and the error note. */
+__attribute__((visibility("hidden"))) tpgp_hip_context *tpgp_engine_default_context(void) {
+    pthread_once(&default_once, default_create);
+    if (!default_ctx) note("context", default_err);
+    return default_ctx;
+}
+__attribute__((visibility("hidden"))) void tpgp_engine_note(const char *where, const char *what) {
+    note(where, what);
+}
 
 tpgp_status tpgp_liouvillian(tpgp_map *out, const tpgp_map *in, const tpgp_term *terms,
                              size_t n) {

@@ -28,10 +28,12 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "../../include/tpgp_hip.h"
+#include "tpgp_f256.h"
 #include "tpgp_hip_context.h"
 
 namespace {
@@ -510,6 +512,398 @@ int liouvillian(tpgp_hip_context *ctx, const uint64_t *keys, const double *value
     return TPGP_HIP_OK;
 }
 
+
+// ---- the residual bound's interval vectors -----------------------------------------------
+//
+// tpgp_residual_bound (residual_rev.c:181-194) needs D1 = 6b - 6a - 4dt La - 2dt Lb - dt^2 LLa
+// and D2 = 6b - 6a - 2dt La - 4dt Lb + dt^2 LLb as sparse vectors of 256-bit intervals, each
+// entry built by a chain of directed-rounded products and sums in a fixed order:
+//   L x    (residual_rev.c:60-111): for every entry of x in its first-insertion order, for every
+//          anticommuting term in term order, z[key xor q] += s x with s = +-2h, the lower end
+//          rounded down and the upper end up (accum, residual_rev.c:55).  The output's entries
+//          are in the order the reference first creates them.
+//   D1, D2 (residual_rev.c:112-114, 187): y += s_k x_k over x_k = b, a, La, Lb, LLa (or LLb) in
+//          that order, each vector's entries in its own order; the scalars s_k are exact.
+// Both are one pattern, "emit, stable sort by key, add each run left to right":
+//   emit    every contribution (output key, source entry, scalar) at its slot, in the order the
+//           reference makes them (entry-major, term-minor; or the vectors one after the other);
+//   sort    stable radix sort by key (the Liouvillian's): a key's contributions keep that order;
+//   place   a run's output position = the number of runs whose first contribution comes
+//           earlier, i.e. an exclusive scan over slots of a flag at each run's first slot;
+//   add     one thread per run, chaining the same operations as the reference from [+0, +0].
+// The operations are correctly rounded 256-bit products and sums (tpgp_f256.h), so every entry
+// is the reference's bit for bit.  What stays on the host is what is serial in the reference:
+// the Gram sums and norms over D1 and D2 in their entry order.
+
+struct ResTerm {  // a term as the residual generator reads it
+    uint32_t i, j, qi, qj;
+    uint32_t live;  // h != 0 (terms with h = 0 add nothing in the incident-terms generator)
+    uint32_t pad;
+};
+static_assert(sizeof(f256) == sizeof(tpgp_hip_f256), "f256 is tpgp_hip_f256's layout");
+struct ResScalar {  // s and the source vector its contributions read
+    f256 s;
+    uint32_t vec, pad;
+};
+constexpr int kResSources = 5;
+struct ResSources {
+    const f256 *lo[kResSources];
+    const f256 *hi[kResSources];
+};
+
+__device__ __forceinline__ unsigned res_phase(const ResTerm &q, const uint64_t *key) {
+    return (gen_phase(q.qi, gen_symbol(key, q.i)) + gen_phase(q.qj, gen_symbol(key, q.j))) & 3u;
+}
+__device__ __forceinline__ bool res_skipped(const f256 *lo, const f256 *hi, uint64_t e,
+                                            int incident) {
+    // the incident-terms generator passes over entries whose interval is [0, 0]
+    return incident && f256_is_zero(&lo[e]) && f256_is_zero(&hi[e]);
+}
+
+__global__ void k_res_point(const double *v, uint64_t n, f256 *lo, f256 *hi) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const f256 x = f256_from_double(v[i]);
+        lo[i] = x;
+        hi[i] = x;
+    }
+}
+
+// Contributions of every entry of x (residual_rev.c:64-66, 98-100).
+template <int W>
+__global__ void k_res_count(const uint64_t *keys, const f256 *lo, const f256 *hi, uint64_t n,
+                            const ResTerm *terms, uint32_t n_terms, int incident,
+                            uint32_t *count) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t c = 0;
+        if (!res_skipped(lo, hi, e, incident)) {
+            uint64_t key[W];
+            for (int w = 0; w < W; ++w) key[w] = keys[e * W + w];
+            for (uint32_t t = 0; t < n_terms; ++t) {
+                const ResTerm q = terms[t];
+                c += (res_phase(q, key) & 1u) && (!incident || q.live);
+            }
+        }
+        count[e] = c;
+    }
+}
+
+// Every contribution at its slot: output key, source entry, scalar index 2t (+2h) or 2t+1 (-2h).
+template <int W>
+__global__ void k_res_emit(const uint64_t *keys, const f256 *lo, const f256 *hi, uint64_t n,
+                           const ResTerm *terms, uint32_t n_terms, int incident,
+                           const uint64_t *offset, uint64_t *out_keys, uint32_t *src,
+                           uint32_t *scalar) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        if (res_skipped(lo, hi, e, incident)) continue;
+        uint64_t key[W];
+        for (int w = 0; w < W; ++w) key[w] = keys[e * W + w];
+        uint64_t at = offset[e];
+        for (uint32_t t = 0; t < n_terms; ++t) {
+            const ResTerm q = terms[t];
+            const unsigned p = res_phase(q, key);
+            if (!(p & 1u) || (incident && !q.live)) continue;
+            uint64_t nk[W];
+            for (int w = 0; w < W; ++w) nk[w] = key[w];
+            nk[q.i >> 5] ^= (uint64_t)q.qi << (2 * (q.i & 31));
+            nk[q.j >> 5] ^= (uint64_t)q.qj << (2 * (q.j & 31));
+            for (int w = 0; w < W; ++w) out_keys[at * W + w] = nk[w];
+            src[at] = (uint32_t)e;
+            scalar[at] = 2 * t + (p == 3);
+            ++at;
+        }
+    }
+}
+
+// One source vector of a combination: its entries at slots base.., all with scalar k.
+template <int W>
+__global__ void k_res_concat(const uint64_t *keys, uint64_t n, uint64_t base, uint32_t k,
+                             uint64_t *out_keys, uint32_t *src, uint32_t *scalar) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        for (int w = 0; w < W; ++w) out_keys[(base + e) * W + w] = keys[e * W + w];
+        src[base + e] = (uint32_t)e;
+        scalar[base + e] = k;
+    }
+}
+
+// Flag the slot of every run's first contribution (the sort kept slots ascending in a run).
+__global__ void k_res_first(const uint32_t *order, const uint64_t *start, uint64_t runs,
+                            uint32_t *flag) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < runs;
+         r += (uint64_t)gridDim.x * blockDim.x)
+        flag[order[start[r]]] = 1;
+}
+
+// One thread per run: z = [+0, +0], then z += s x for its contributions in slot order, the
+// lower end with products and sums rounded down, the upper end up (residual_rev.c:55, 105-106);
+// the entry goes to the run's place in first-insertion order.
+template <int W>
+__global__ void k_res_add(const uint64_t *keys, const uint32_t *order, const uint64_t *start,
+                          uint64_t runs, uint64_t n, const uint32_t *src, const uint32_t *scalar,
+                          const ResScalar *scalars, ResSources from, const uint64_t *place,
+                          uint64_t *out_keys, f256 *out_lo, f256 *out_hi) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < runs;
+         r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t s0 = start[r], s1 = r + 1 < runs ? start[r + 1] : n;
+        f256 zlo = f256_zero(0), zhi = f256_zero(0);
+        for (uint64_t p = s0; p < s1; ++p) {
+            const uint32_t slot = order[p], e = src[slot];
+            const ResScalar c = scalars[scalar[slot]];
+            const f256 *lo = from.lo[c.vec], *hi = from.hi[c.vec];
+            const bool nonneg = f256_is_zero(&c.s) || !c.s.neg;  // mpfr_sgn(s) >= 0
+            zlo = f256_add(zlo, f256_mul(nonneg ? lo[e] : hi[e], c.s, F256_DOWN), F256_DOWN);
+            zhi = f256_add(zhi, f256_mul(nonneg ? hi[e] : lo[e], c.s, F256_UP), F256_UP);
+        }
+        const uint64_t q = place[order[s0]];
+        for (int w = 0; w < W; ++w) out_keys[q * W + w] = keys[s0 * W + w];
+        out_lo[q] = zlo;
+        out_hi[q] = zhi;
+    }
+}
+
+struct RVec {  // a device interval vector in first-insertion order
+    uint64_t *keys = nullptr;
+    f256 *lo = nullptr, *hi = nullptr;
+    uint64_t len = 0;
+};
+
+// Sort the n emitted contributions (keys ck, source entries src, scalar indices sc) by key,
+// then add each run into out (allocated from `keep`, which outlives this call's temporaries).
+template <int W>
+int res_reduce(hipStream_t st, Arena &keep, uint64_t *ck, const uint32_t *src,
+               const uint32_t *sc, uint64_t n, Arena &A, const ResScalar *scalars,
+               const ResSources &from, unsigned lo_bits, unsigned hi_bits, RVec *out,
+               char *error, size_t cap) {
+    hipError_t e;
+    out->len = 0;
+    if (!n) return TPGP_HIP_OK;
+    uint32_t *order = A.get<uint32_t>(n), *head = A.get<uint32_t>(n), *flag = A.get<uint32_t>(n);
+    uint64_t *run_of = A.get<uint64_t>(n), *place = A.get<uint64_t>(n);
+    if (A.err != hipSuccess) return drain_fail(st, A.err, "HIP residual allocation", error, cap);
+    const unsigned g = grid_for(n);
+    hipLaunchKernelGGL(k_gen_iota, dim3(g), dim3(256), 0, st, order, n);
+    if ((e = sort_pairs<W>(st, A, ck, order, n, lo_bits, hi_bits)) != hipSuccess)
+        return drain_fail(st, e, "HIP residual sort", error, cap);
+    hipLaunchKernelGGL(k_gen_heads<W>, dim3(g), dim3(256), 0, st, ck, n, head);
+    if ((e = scan(st, A, head, n, run_of, 1)) != hipSuccess)
+        return drain_fail(st, e, "HIP residual scan", error, cap);
+    uint64_t runs = 0;
+    if ((e = hipMemcpyAsync(&runs, run_of + n - 1, 8, hipMemcpyDeviceToHost, st)) ||
+        (e = hipStreamSynchronize(st)))
+        return drain_fail(st, e, "HIP residual runs", error, cap);
+    uint64_t *start = A.get<uint64_t>(runs);
+    out->keys = keep.get<uint64_t>(runs * W);
+    out->lo = keep.get<f256>(runs);
+    out->hi = keep.get<f256>(runs);
+    if (A.err != hipSuccess || keep.err != hipSuccess)
+        return drain_fail(st, A.err ? A.err : keep.err, "HIP residual allocation", error, cap);
+    hipLaunchKernelGGL(k_gen_starts, dim3(g), dim3(256), 0, st, head, run_of, n, start);
+    hipLaunchKernelGGL(k_gen_zero_u32, dim3(g), dim3(256), 0, st, flag, n);
+    const unsigned gr = grid_for(runs);
+    hipLaunchKernelGGL(k_res_first, dim3(gr), dim3(256), 0, st, order, start, runs, flag);
+    if ((e = scan(st, A, flag, n, place, 0)) != hipSuccess)
+        return drain_fail(st, e, "HIP residual scan", error, cap);
+    hipLaunchKernelGGL(k_res_add<W>, dim3(gr), dim3(256), 0, st, ck, order, start, runs, n, src,
+                       sc, scalars, from, place, out->keys, out->lo, out->hi);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return drain_fail(st, e, "HIP residual launch", error, cap);
+    out->len = runs;
+    return TPGP_HIP_OK;
+}
+
+constexpr uint64_t kResMaxSlots = 1ull << 31;  // sort payloads and source entries are 32-bit
+
+// out = L in (the generator of residual_rev.c:60-111; incident = the incident-terms form).
+template <int W>
+int res_generator(hipStream_t st, Arena &keep, const RVec &in, const ResTerm *d_terms,
+                  uint32_t n_terms, const ResScalar *d_scalars, int incident, unsigned lo_bits,
+                  unsigned hi_bits, RVec *out, char *error, size_t cap) {
+    hipError_t e;
+    out->len = 0;
+    if (!in.len || !n_terms) return TPGP_HIP_OK;
+    Arena A;
+    uint32_t *count = A.get<uint32_t>(in.len);
+    uint64_t *offset = A.get<uint64_t>(in.len);
+    if (A.err != hipSuccess) return drain_fail(st, A.err, "HIP residual allocation", error, cap);
+    const unsigned g = grid_for(in.len);
+    hipLaunchKernelGGL(k_res_count<W>, dim3(g), dim3(256), 0, st, in.keys, in.lo, in.hi, in.len,
+                       d_terms, n_terms, incident, count);
+    if ((e = scan(st, A, count, in.len, offset, 0)) != hipSuccess)
+        return drain_fail(st, e, "HIP residual scan", error, cap);
+    uint64_t last[2] = {0, 0};
+    uint32_t last_count = 0;
+    if ((e = hipMemcpyAsync(&last[0], offset + in.len - 1, 8, hipMemcpyDeviceToHost, st)) ||
+        (e = hipMemcpyAsync(&last_count, count + in.len - 1, 4, hipMemcpyDeviceToHost, st)) ||
+        (e = hipStreamSynchronize(st)))
+        return drain_fail(st, e, "HIP residual count", error, cap);
+    const uint64_t n = last[0] + last_count;
+    if (n >= kResMaxSlots) {
+        msg(error, cap, "Residual generator of %llu contributions exceeds 2^31.",
+            (unsigned long long)n);
+        return TPGP_HIP_ENOMEM;
+    }
+    if (!n) return TPGP_HIP_OK;
+    uint64_t *ck = A.get<uint64_t>(n * W);
+    uint32_t *src = A.get<uint32_t>(n), *sc = A.get<uint32_t>(n);
+    if (A.err != hipSuccess) return drain_fail(st, A.err, "HIP residual allocation", error, cap);
+    hipLaunchKernelGGL(k_res_emit<W>, dim3(g), dim3(256), 0, st, in.keys, in.lo, in.hi, in.len,
+                       d_terms, n_terms, incident, offset, ck, src, sc);
+    ResSources from = {};
+    for (int k = 0; k < kResSources; ++k) {
+        from.lo[k] = in.lo;
+        from.hi[k] = in.hi;
+    }
+    const int s = res_reduce<W>(st, keep, ck, src, sc, n, A, d_scalars, from, lo_bits, hi_bits,
+                                out, error, cap);
+    if (s) return s;
+    // the temporaries are freed when A goes out of scope: finish the work that reads them
+    if ((e = hipStreamSynchronize(st)) != hipSuccess)
+        return hip_fail(e, "HIP residual generator", error, cap);
+    return TPGP_HIP_OK;
+}
+
+// out = sum_k s_k x_k over the five vectors in order (residual_rev.c:57, 112-114).
+template <int W>
+int res_combine(hipStream_t st, Arena &keep, const RVec *x, const ResScalar *d_scalars,
+                unsigned lo_bits, unsigned hi_bits, RVec *out, char *error, size_t cap) {
+    hipError_t e;
+    uint64_t n = 0;
+    for (int k = 0; k < kResSources; ++k) n += x[k].len;
+    out->len = 0;
+    if (n >= kResMaxSlots) {
+        msg(error, cap, "Residual combination of %llu entries exceeds 2^31.",
+            (unsigned long long)n);
+        return TPGP_HIP_ENOMEM;
+    }
+    if (!n) return TPGP_HIP_OK;
+    Arena A;
+    uint64_t *ck = A.get<uint64_t>(n * W);
+    uint32_t *src = A.get<uint32_t>(n), *sc = A.get<uint32_t>(n);
+    if (A.err != hipSuccess) return drain_fail(st, A.err, "HIP residual allocation", error, cap);
+    ResSources from = {};
+    uint64_t base = 0;
+    for (int k = 0; k < kResSources; ++k) {
+        from.lo[k] = x[k].lo;
+        from.hi[k] = x[k].hi;
+        if (x[k].len)
+            hipLaunchKernelGGL(k_res_concat<W>, dim3(grid_for(x[k].len)), dim3(256), 0, st,
+                               x[k].keys, x[k].len, base, (uint32_t)k, ck, src, sc);
+        base += x[k].len;
+    }
+    const int s = res_reduce<W>(st, keep, ck, src, sc, n, A, d_scalars, from, lo_bits, hi_bits,
+                                out, error, cap);
+    if (s) return s;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess)
+        return hip_fail(e, "HIP residual combination", error, cap);
+    return TPGP_HIP_OK;
+}
+
+f256 res_scale(double base_power_dt, int power, int factor) {
+    // factor * dt^power exactly (at most 109 significant bits; residual_rev.c:112-114)
+    f256 s = f256_from_double(power == 0 ? 1.0 : base_power_dt);
+    if (power == 2) s = f256_mul(s, f256_from_double(base_power_dt), F256_DOWN);
+    return f256_mul(s, f256_from_double((double)factor), F256_DOWN);
+}
+
+template <int W>
+int residual_vectors(tpgp_hip_context *ctx, const uint64_t *a_keys, const double *a_values,
+                     size_t a_len, const uint64_t *b_keys, const double *b_values, size_t b_len,
+                     size_t n_spins, const tpgp_hip_term *terms, size_t n_terms, double dt,
+                     int incident, tpgp_hip_interval_vector *out, char *error, size_t cap) {
+    hipStream_t st = ctx->stream;
+    hipError_t e;
+    Arena keep;  // the vectors a, b, La, Lb, LLa, LLb, D1, D2
+    RVec v[8];
+    // generator terms and their scalars +-2h (residual_rev.c:68, 103: exact in MPFR)
+    std::vector<ResTerm> t(n_terms);
+    std::vector<ResScalar> gs(2 * n_terms + 1);
+    for (size_t k = 0; k < n_terms; ++k) {
+        t[k] = ResTerm{(uint32_t)terms[k].i, (uint32_t)terms[k].j, terms[k].qi, terms[k].qj,
+                       terms[k].h != 0.0, 0};
+        f256 s = f256_from_double(terms[k].h);
+        if (!f256_is_zero(&s)) s.e += 1;  // 2h without binary64 overflow
+        gs[2 * k] = ResScalar{s, 0, 0};
+        gs[2 * k + 1] = ResScalar{f256_neg(s), 0, 0};
+    }
+    // combination scalars: D1 over (b, a, La, Lb, LLa), D2 over (b, a, La, Lb, LLb)
+    const ResScalar cs[2][kResSources] = {
+        {{res_scale(dt, 0, 6), 0, 0}, {res_scale(dt, 0, -6), 1, 0}, {res_scale(dt, 1, -4), 2, 0},
+         {res_scale(dt, 1, -2), 3, 0}, {res_scale(dt, 2, -1), 4, 0}},
+        {{res_scale(dt, 0, 6), 0, 0}, {res_scale(dt, 0, -6), 1, 0}, {res_scale(dt, 1, -2), 2, 0},
+         {res_scale(dt, 1, -4), 3, 0}, {res_scale(dt, 2, 1), 4, 0}}};
+    ResTerm *d_terms = keep.get<ResTerm>(n_terms);
+    ResScalar *d_gs = keep.get<ResScalar>(gs.size());
+    ResScalar *d_cs = keep.get<ResScalar>(2 * kResSources);
+    v[0].keys = keep.get<uint64_t>(a_len * W);
+    v[0].lo = keep.get<f256>(a_len);
+    v[0].hi = keep.get<f256>(a_len);
+    v[1].keys = keep.get<uint64_t>(b_len * W);
+    v[1].lo = keep.get<f256>(b_len);
+    v[1].hi = keep.get<f256>(b_len);
+    double *d_av = keep.get<double>(a_len), *d_bv = keep.get<double>(b_len);
+    if (keep.err != hipSuccess) return hip_fail(keep.err, "HIP residual allocation", error, cap);
+    v[0].len = a_len;
+    v[1].len = b_len;
+    if ((n_terms && (e = hipMemcpyAsync(d_terms, t.data(), n_terms * sizeof(ResTerm),
+                                        hipMemcpyHostToDevice, st))) ||
+        (e = hipMemcpyAsync(d_gs, gs.data(), gs.size() * sizeof(ResScalar),
+                            hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(d_cs, cs, sizeof(cs), hipMemcpyHostToDevice, st)) ||
+        (a_len && (e = hipMemcpyAsync(v[0].keys, a_keys, a_len * W * 8, hipMemcpyHostToDevice,
+                                      st))) ||
+        (a_len && (e = hipMemcpyAsync(d_av, a_values, a_len * 8, hipMemcpyHostToDevice, st))) ||
+        (b_len && (e = hipMemcpyAsync(v[1].keys, b_keys, b_len * W * 8, hipMemcpyHostToDevice,
+                                      st))) ||
+        (b_len && (e = hipMemcpyAsync(d_bv, b_values, b_len * 8, hipMemcpyHostToDevice, st))))
+        return drain_fail(st, e, "HIP residual upload", error, cap);
+    if (a_len)
+        hipLaunchKernelGGL(k_res_point, dim3(grid_for(a_len)), dim3(256), 0, st, d_av,
+                           (uint64_t)a_len, v[0].lo, v[0].hi);
+    if (b_len)
+        hipLaunchKernelGGL(k_res_point, dim3(grid_for(b_len)), dim3(256), 0, st, d_bv,
+                           (uint64_t)b_len, v[1].lo, v[1].hi);
+    const unsigned lo_bits = W == 1 ? (unsigned)(2 * n_spins) : 64u;
+    const unsigned hi_bits = W == 1 ? 0u : (unsigned)(2 * (n_spins - 32));
+    int s = 0;
+    // La, Lb, LLa, LLb (residual_rev.c:186)
+    for (int k = 0; k < 4 && !s; ++k)
+        s = res_generator<W>(st, keep, v[k], d_terms, (uint32_t)n_terms, d_gs, incident, lo_bits,
+                             hi_bits, &v[k + 2], error, cap);
+    // D1, D2 (residual_rev.c:187)
+    for (int d = 0; d < 2 && !s; ++d) {
+        const RVec x[kResSources] = {v[1], v[0], v[2], v[3], v[4 + d]};
+        s = res_combine<W>(st, keep, x, d_cs + d * kResSources, lo_bits, hi_bits, &v[6 + d],
+                           error, cap);
+    }
+    if (s) return s;
+    for (int d = 0; d < 2; ++d) {
+        const RVec &r = v[6 + d];
+        tpgp_hip_interval_vector &o = out[d];
+        o.len = r.len;
+        o.keys = (uint64_t *)malloc((r.len ? r.len : 1) * W * 8);
+        o.lower = (tpgp_hip_f256 *)malloc((r.len ? r.len : 1) * sizeof(tpgp_hip_f256));
+        o.upper = (tpgp_hip_f256 *)malloc((r.len ? r.len : 1) * sizeof(tpgp_hip_f256));
+        if (!o.keys || !o.lower || !o.upper) {
+            (void)hipStreamSynchronize(st);
+            msg(error, cap, "Host allocation of the residual vectors failed.");
+            return TPGP_HIP_ENOMEM;
+        }
+        if (r.len &&
+            ((e = hipMemcpyAsync(o.keys, r.keys, r.len * W * 8, hipMemcpyDeviceToHost, st)) ||
+             (e = hipMemcpyAsync(o.lower, r.lo, r.len * sizeof(f256), hipMemcpyDeviceToHost,
+                                 st)) ||
+             (e = hipMemcpyAsync(o.upper, r.hi, r.len * sizeof(f256), hipMemcpyDeviceToHost,
+                                 st))))
+            return drain_fail(st, e, "HIP residual download", error, cap);
+    }
+    if ((e = hipStreamSynchronize(st)) != hipSuccess)
+        return hip_fail(e, "HIP residual vectors", error, cap);
+    return TPGP_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" int tpgp_hip_liouvillian(tpgp_hip_context *ctx, const uint64_t *keys,
@@ -560,4 +954,59 @@ extern "C" int tpgp_hip_liouvillian(tpgp_hip_context *ctx, const uint64_t *keys,
         return TPGP_HIP_EINVAL;
     }
     return TPGP_HIP_OK;
+}
+
+
+extern "C" void tpgp_hip_interval_vector_free(tpgp_hip_interval_vector *v) {
+    if (!v) return;
+    free(v->keys);
+    free(v->lower);
+    free(v->upper);
+    v->keys = nullptr;
+    v->lower = v->upper = nullptr;
+    v->len = 0;
+}
+
+extern "C" int tpgp_hip_residual_vectors(tpgp_hip_context *ctx, const uint64_t *a_keys,
+                                         const double *a_values, size_t a_len,
+                                         const uint64_t *b_keys, const double *b_values,
+                                         size_t b_len, size_t n_words, size_t n_spins,
+                                         const tpgp_hip_term *terms, size_t n_terms, double dt,
+                                         int incident_terms, tpgp_hip_interval_vector out[2],
+                                         char *error, size_t cap) {
+    msg(error, cap, "");
+    if (out)
+        for (int d = 0; d < 2; ++d) out[d] = tpgp_hip_interval_vector{0, nullptr, nullptr, nullptr};
+    if (!ctx || !out || !n_spins || n_spins > 64 || n_words != (n_spins + 31) / 32 ||
+        (a_len && (!a_keys || !a_values)) || (b_len && (!b_keys || !b_values)) ||
+        (n_terms && !terms) || !std::isfinite(dt) || dt <= 0 ||
+        (incident_terms != 0 && incident_terms != 1) || a_len >= kResMaxSlots ||
+        b_len >= kResMaxSlots || n_terms > (1u << 30)) {
+        msg(error, cap, "Invalid residual-vector arguments (at most 64 spins).");
+        return TPGP_HIP_EINVAL;
+    }
+    for (size_t k = 0; k < n_terms; ++k) {
+        const tpgp_hip_term &q = terms[k];
+        if (q.i >= n_spins || q.j >= n_spins || q.i == q.j || q.qi > 3 || q.qj > 3 ||
+            !std::isfinite(q.h)) {
+            msg(error, cap, "Invalid Hamiltonian term %zu.", k);
+            return TPGP_HIP_EINVAL;
+        }
+    }
+    std::lock_guard<std::recursive_mutex> lock(ctx->mutex);
+    DeviceGuard guard(ctx->device);
+    if (!guard.ok) {
+        msg(error, cap, "HIP device selection failed.");
+        return TPGP_HIP_EDEVICE;
+    }
+    const int s = n_words == 1
+                      ? residual_vectors<1>(ctx, a_keys, a_values, a_len, b_keys, b_values,
+                                            b_len, n_spins, terms, n_terms, dt, incident_terms,
+                                            out, error, cap)
+                      : residual_vectors<2>(ctx, a_keys, a_values, a_len, b_keys, b_values,
+                                            b_len, n_spins, terms, n_terms, dt, incident_terms,
+                                            out, error, cap);
+    if (s)
+        for (int d = 0; d < 2; ++d) tpgp_hip_interval_vector_free(&out[d]);
+    return s;
 }

@@ -1,4 +1,4 @@
-/* The independent endpoint (residual) bound of one Strang step, host side (C17 + MPFR).
+/* The independent endpoint (residual) bound of one Strang step (C17 + MPFR, with the GPU).
  *
  * tpgp_residual_bound(a, b, ...) bounds the normalized Hilbert-Schmidt error produced over one
  * step of length dt by the cubic Hermite path through the maps a (before) and b (after), as
@@ -9,12 +9,15 @@
  *   the bound is the integral over the Bezier control points of |u D1 + v D2|, subdivided
  *   `depth` times (residual_rev.c:175-180), each norm either from the enclosed Gram matrix of
  *   (D1, D2) (residual_rev.c:116-174) or directly over the sparse supports.
- * All arithmetic is MPFR at 256 bits with directed rounding, in the reference's order of
- * operations (vector entries in first-insertion order, generator terms in term order), so the
- * rounded-up binary64 result is the reference's bit for bit.  The reference has no GPU path
- * for it; the generator here runs on the host next to the interval arithmetic it feeds.
+ * D1 and D2 are built on the GPU (tpgp_hip_residual_vectors): four generator applications and
+ * two combinations, every entry a chain of correctly rounded 256-bit products and sums, equal to
+ * the reference's MPFR entries bit for bit and in its first-insertion order.  The rest is serial
+ * in the reference (sums over D1 and D2 in entry order, each addition rounded) and runs here in
+ * MPFR at 256 bits with the reference's rounding directions and order of operations, so the
+ * rounded-up binary64 result is the reference's bit for bit.
  */
 #include "../../include/tpgp_engine.h"
+#include "../../include/tpgp_hip.h"
 
 #include <float.h>
 #include <math.h>
@@ -93,34 +96,6 @@ static interval *ivec_find(const ivec *v, const uint64_t *k) {
     return NULL;
 }
 
-/* The entry of key k, appended as [+0, +0] when absent (residual_rev.c:44-52). */
-static interval *ivec_slot(ivec *v, const uint64_t *k) {
-    interval *x = ivec_find(v, k);
-    if (x) return x;
-    if (v->len == v->cap) {
-        const size_t cap = v->cap ? 2 * v->cap : 64;
-        uint64_t *keys = realloc(v->keys, cap * v->words * sizeof(*keys));
-        if (!keys) return NULL;
-        v->keys = keys;
-        interval *vals = realloc(v->vals, cap * sizeof(*vals));
-        if (!vals) return NULL;
-        v->vals = vals;
-        v->cap = cap;
-    }
-    if (2 * (v->len + 1) > v->slots && !ivec_reindex(v, v->slots ? 2 * v->slots : 128))
-        return NULL;
-    const size_t i = v->len++;
-    memcpy(v->keys + i * v->words, k, v->words * sizeof(*k));
-    mpfr_init2(v->vals[i].lo, kBits);
-    mpfr_init2(v->vals[i].hi, kBits);
-    mpfr_set_zero(v->vals[i].lo, 1);
-    mpfr_set_zero(v->vals[i].hi, 1);
-    size_t s = key_hash(k, v->words) & (v->slots - 1);
-    while (v->index[s]) s = (s + 1) & (v->slots - 1);
-    v->index[s] = i + 1;
-    return &v->vals[i];
-}
-
 /* y += s x with outward rounding (residual_rev.c:55). */
 static void add_scaled(interval *y, const interval *x, const mpfr_t s, mpfr_t t) {
     const int nonneg = mpfr_sgn(s) >= 0;
@@ -130,161 +105,43 @@ static void add_scaled(interval *y, const interval *x, const mpfr_t s, mpfr_t t)
     mpfr_add(y->hi, y->hi, t, MPFR_RNDU);
 }
 
-/* Point intervals of a map, in map order (residual_rev.c:56). */
-static int from_map(ivec *v, const tpgp_map *m) {
-    for (size_t i = 0; i < m->len; ++i) {
-        interval *x = ivec_slot(v, m->keys + i * m->n_words);
-        if (!x || !isfinite(m->values[i])) return 0;
-        mpfr_set_d(x->lo, m->values[i], MPFR_RNDN);
-        mpfr_set_d(x->hi, m->values[i], MPFR_RNDN);
-    }
-    return 1;
-}
+/* The engine's process-wide device context (created on first use) and its error note. */
+tpgp_hip_context *tpgp_engine_default_context(void);
+void tpgp_engine_note(const char *where, const char *what);
 
-/* y += s x over x's entries in order; nothing (not even new entries) for s = 0
- * (residual_rev.c:57). */
-static int add_vec(ivec *y, const ivec *x, const mpfr_t s) {
-    if (mpfr_zero_p(s)) return 1;
-    mpfr_t t;
-    mpfr_init2(t, kBits);
-    int ok = 1;
-    for (size_t i = 0; i < x->len && ok; ++i) {
-        interval *z = ivec_slot(y, x->keys + i * x->words);
-        if (!z)
-            ok = 0;
-        else
-            add_scaled(z, &x->vals[i], s, t);
-    }
-    mpfr_clear(t);
-    return ok;
-}
-
-/* Exponent of i in the product of Pauli symbols q p (I0 X1 Y2 Z3). */
-static unsigned phase(unsigned q, unsigned p) {
-    if (!q || !p || q == p) return 0;
-    return (p + 3 - q) % 3 == 1 ? 1u : 3u;
-}
-static unsigned symbol(const uint64_t *k, size_t site) {
-    return (unsigned)(k[site / 32] >> (2 * (site % 32))) & 3u;
-}
-
-/* The contribution of term t to the generator applied to entry (k, x): for an anticommuting
- * term, out[k ^ t] += s x with s = +-2h (residual_rev.c:60-70, 96-108). */
-static int apply_term(ivec *out, const uint64_t *k, const interval *x, const tpgp_term *t,
-                      int skip_zero_h, uint64_t *tmp_key, mpfr_t s, mpfr_t tmp) {
-    const unsigned p = (phase(t->qi, symbol(k, t->i)) + phase(t->qj, symbol(k, t->j))) & 3u;
-    if (!(p & 1u) || (skip_zero_h && t->h == 0)) return 1;
-    memcpy(tmp_key, k, out->words * sizeof(*k));
-    tmp_key[t->i / 32] ^= (uint64_t)t->qi << (2 * (t->i % 32));
-    tmp_key[t->j / 32] ^= (uint64_t)t->qj << (2 * (t->j % 32));
-    interval *z = ivec_slot(out, tmp_key);
-    if (!z) return 0;
-    mpfr_set_d(s, t->h, MPFR_RNDN);
-    mpfr_mul_2ui(s, s, 1, MPFR_RNDN);
-    if (p == 3) mpfr_neg(s, s, MPFR_RNDN);
-    add_scaled(z, x, s, tmp);
-    return 1;
-}
-
-/* Generator, diagnostic form: every entry against every term in term order
- * (residual_rev.c:60-70). */
-static int generator_all(ivec *out, const ivec *in, const tpgp_term *t, size_t n) {
-    out->words = in->words;
-    uint64_t *key = malloc(in->words * sizeof(*key));
-    if (!key) return 0;
-    mpfr_t s, tmp;
-    mpfr_inits2(kBits, s, tmp, (mpfr_ptr)0);
-    int ok = 1;
-    for (size_t i = 0; i < in->len && ok; ++i)
-        for (size_t j = 0; j < n && ok; ++j)
-            ok = apply_term(out, in->keys + i * in->words, &in->vals[i], &t[j], 0, key, s, tmp);
-    mpfr_clears(s, tmp, (mpfr_ptr)0);
-    free(key);
-    return ok;
-}
-
-static int cmp_size(const void *a, const void *b) {
-    const size_t x = *(const size_t *)a, y = *(const size_t *)b;
-    return (x > y) - (x < y);
-}
-
-/* Generator: per nonzero entry only the terms incident on one of its non-identity sites, in
- * term order (the others commute); terms with h = 0 add nothing (residual_rev.c:74-111). */
-static int generator(ivec *out, const ivec *in, const tpgp_term *t, size_t n) {
-    if (!n) return generator_all(out, in, t, n);
-    size_t sites = 0;
-    for (size_t j = 0; j < n; ++j) {
-        if (t[j].i >= sites) sites = t[j].i + 1;
-        if (t[j].j >= sites) sites = t[j].j + 1;
-    }
-    if (n > SIZE_MAX / (2 * sizeof(size_t))) return 0;
-    size_t *first = calloc(sites + 1, sizeof(size_t)), *fill = calloc(sites, sizeof(size_t));
-    size_t *incident = malloc(2 * n * sizeof(size_t)), *list = malloc(n * sizeof(size_t));
-    unsigned char *taken = calloc(n, 1);
-    uint64_t *key = malloc(in->words * sizeof(*key));
-    int ok = first && fill && incident && list && taken && key;
-    if (ok) {
-        for (size_t j = 0; j < n; ++j) {  /* terms of each site, ascending */
-            first[t[j].i + 1]++;
-            first[t[j].j + 1]++;
-        }
-        for (size_t s = 1; s <= sites; ++s) first[s] += first[s - 1];
-        memcpy(fill, first, sites * sizeof(size_t));
-        for (size_t j = 0; j < n; ++j) {
-            incident[fill[t[j].i]++] = j;
-            incident[fill[t[j].j]++] = j;
-        }
-    }
-    out->words = in->words;
-    mpfr_t s, tmp;
-    mpfr_inits2(kBits, s, tmp, (mpfr_ptr)0);
-    for (size_t e = 0; ok && e < in->len; ++e) {
-        const interval *x = &in->vals[e];
-        if (mpfr_zero_p(x->lo) && mpfr_zero_p(x->hi)) continue;
-        const uint64_t *k = in->keys + e * in->words;
-        size_t count = 0;
-        for (size_t w = 0; w < in->words; ++w) {
-            for (uint64_t bits = k[w]; bits;) {
-                const unsigned local = (unsigned)__builtin_ctzll(bits) / 2;
-                bits &= ~(3ull << (2 * local));
-                const size_t site = 32 * w + local;
-                if (site >= sites) continue;
-                for (size_t q = first[site]; q < first[site + 1]; ++q)
-                    if (!taken[incident[q]]) {
-                        taken[incident[q]] = 1;
-                        list[count++] = incident[q];
-                    }
+/* A vector built on the device, in its entry order, with an index for lookups.  The device
+ * endpoints are exact 256-bit values, so the conversion to MPFR at 256 bits is exact. */
+static int from_device(ivec *v, const tpgp_hip_interval_vector *d, size_t words) {
+    v->words = words;
+    v->len = 0;
+    v->cap = d->len ? d->len : 1;
+    v->keys = malloc(v->cap * words * sizeof(*v->keys));
+    v->vals = malloc(v->cap * sizeof(*v->vals));
+    if (!v->keys || !v->vals) return 0;
+    if (d->len) memcpy(v->keys, d->keys, d->len * words * sizeof(*v->keys));
+    mpz_t z;
+    mpz_init(z);
+    for (size_t i = 0; i < d->len; ++i) {
+        const tpgp_hip_f256 *ends[2] = {&d->lower[i], &d->upper[i]};
+        mpfr_ptr out[2] = {v->vals[i].lo, v->vals[i].hi};
+        mpfr_init2(v->vals[i].lo, kBits);
+        mpfr_init2(v->vals[i].hi, kBits);
+        v->len = i + 1;
+        for (int k = 0; k < 2; ++k) {
+            const tpgp_hip_f256 *x = ends[k];
+            if (!x->m[3]) {
+                mpfr_set_zero(out[k], x->neg ? -1 : 1);
+                continue;
             }
-        }
-        qsort(list, count, sizeof(size_t), cmp_size);
-        for (size_t q = 0; q < count; ++q) {
-            taken[list[q]] = 0;
-            if (ok) ok = apply_term(out, k, x, &t[list[q]], 1, key, s, tmp);
+            mpz_import(z, 4, -1, sizeof(uint64_t), 0, 0, x->m);
+            mpfr_set_z_2exp(out[k], z, (mpfr_exp_t)x->e - kBits, MPFR_RNDN);
+            if (x->neg) mpfr_neg(out[k], out[k], MPFR_RNDN);
         }
     }
-    mpfr_clears(s, tmp, (mpfr_ptr)0);
-    free(first);
-    free(fill);
-    free(incident);
-    free(list);
-    free(taken);
-    free(key);
-    return ok;
-}
-
-/* out += factor dt^power in; the scalar is exact at 256 bits (residual_rev.c:112-114). */
-static int add_dt(ivec *out, const ivec *in, double dt, int power, int factor) {
-    mpfr_t s;
-    mpfr_init2(s, kBits);
-    if (power == 0)
-        mpfr_set_ui(s, 1, MPFR_RNDN);
-    else
-        mpfr_set_d(s, dt, MPFR_RNDN);
-    if (power == 2) mpfr_sqr(s, s, MPFR_RNDN);
-    mpfr_mul_si(s, s, factor, MPFR_RNDN);
-    const int ok = add_vec(out, in, s);
-    mpfr_clear(s);
-    return ok;
+    mpz_clear(z);
+    size_t slots = 128;
+    while (slots < 2 * (d->len + 1)) slots *= 2;
+    return ivec_reindex(v, slots);
 }
 
 /* Upper bound of |u D1 + v D2| over the union of supports (D1's entries, then D2's that D1
@@ -480,20 +337,24 @@ static tpgp_status residual(const tpgp_map *a, const tpgp_map *b, const tpgp_ter
         if (t[j].i >= a->n_spins || t[j].j >= a->n_spins || t[j].i == t[j].j || t[j].qi > 3 ||
             t[j].qj > 3 || !isfinite(t[j].h))
             return TPGP_EINVAL;
-    /* v: a, b, L a, L b, L L a, L L b, D1, D2 */
-    ivec v[8];
+    tpgp_hip_context *ctx = tpgp_engine_default_context();
+    if (!ctx) return TPGP_EBACKEND;
+    /* D1, D2 on the device (tpgp_hip_term has tpgp_term's layout) */
+    tpgp_hip_interval_vector dv[2];
+    char err[512];
+    const int hs = tpgp_hip_residual_vectors(
+        ctx, a->keys, a->values, a->len, b->keys, b->values, b->len, a->n_words, a->n_spins,
+        (const tpgp_hip_term *)t, n, dt, with_gram ? 1 : 0, dv, err, sizeof(err));
+    if (hs) {
+        tpgp_engine_note("residual vectors", err);
+        return hs == TPGP_HIP_ENOMEM ? TPGP_ENOMEM : hs == TPGP_HIP_EINVAL ? TPGP_EINVAL
+                                                                          : TPGP_EBACKEND;
+    }
+    ivec v[2];
     memset(v, 0, sizeof(v));
-    for (int i = 0; i < 8; ++i) v[i].words = a->n_words;
-    int (*gen)(ivec *, const ivec *, const tpgp_term *, size_t) =
-        with_gram ? generator : generator_all;
-    int ok = from_map(&v[0], a) && from_map(&v[1], b) && gen(&v[2], &v[0], t, n) &&
-             gen(&v[3], &v[1], t, n) && gen(&v[4], &v[2], t, n) && gen(&v[5], &v[3], t, n);
-    ok = ok && add_dt(&v[6], &v[1], dt, 0, 6) && add_dt(&v[6], &v[0], dt, 0, -6) &&
-         add_dt(&v[6], &v[2], dt, 1, -4) && add_dt(&v[6], &v[3], dt, 1, -2) &&
-         add_dt(&v[6], &v[4], dt, 2, -1);
-    ok = ok && add_dt(&v[7], &v[1], dt, 0, 6) && add_dt(&v[7], &v[0], dt, 0, -6) &&
-         add_dt(&v[7], &v[2], dt, 1, -2) && add_dt(&v[7], &v[3], dt, 1, -4) &&
-         add_dt(&v[7], &v[5], dt, 2, 1);
+    int ok = from_device(&v[0], &dv[0], a->n_words) && from_device(&v[1], &dv[1], a->n_words);
+    tpgp_hip_interval_vector_free(&dv[0]);
+    tpgp_hip_interval_vector_free(&dv[1]);
     tpgp_status status = ok ? TPGP_OK : TPGP_ENOMEM;
     if (ok) {
         static const double c[4][2] = {{0, 0}, {1, 0}, {0, 1}, {0, 0}};
@@ -505,8 +366,8 @@ static tpgp_status residual(const tpgp_map *a, const tpgp_map *b, const tpgp_ter
         iv_init(&g.ab);
         iv_init(&g.bb);
         const int use_gram = with_gram && depth > 0;
-        if (use_gram) gram_of(&g, &v[6], &v[7]);
-        integrate(sum, &v[6], &v[7], use_gram ? &g : NULL, c, depth, depth);
+        if (use_gram) gram_of(&g, &v[0], &v[1]);
+        integrate(sum, &v[0], &v[1], use_gram ? &g : NULL, c, depth, depth);
         const double up = mpfr_get_d(sum, MPFR_RNDU);
         iv_clear(&g.aa);
         iv_clear(&g.ab);
@@ -517,7 +378,7 @@ static tpgp_status residual(const tpgp_map *a, const tpgp_map *b, const tpgp_ter
         else
             *upper = up;
     }
-    for (int i = 0; i < 8; ++i) ivec_free(&v[i]);
+    for (int i = 0; i < 2; ++i) ivec_free(&v[i]);
     return status;
 }
 

@@ -117,6 +117,15 @@ class HipLogEntry(C.Structure):
                 ("pad", C.c_uint32)]
 
 
+class HipF256(C.Structure):  # tpgp_hip_f256: (-1)^neg (m / 2^256) 2^e, m[3] most significant
+    _fields_ = [("m", C.c_uint64 * 4), ("e", C.c_int32), ("neg", C.c_uint32)]
+
+
+class HipIntervalVector(C.Structure):  # tpgp_hip_interval_vector
+    _fields_ = [("len", C.c_size_t), ("keys", C.POINTER(C.c_uint64)),
+                ("lower", C.POINTER(HipF256)), ("upper", C.POINTER(HipF256))]
+
+
 class Checkpoint(C.Structure):  # checkpoint_rev.h:7-23 layout
     _fields_ = [("map", TpgpMap), ("terms", C.POINTER(TpgpTerm)), ("term_count", C.c_size_t),
                 ("options", TpgpOptions), ("ledger", TpgpLedger), ("dt", C.c_double),
@@ -141,7 +150,8 @@ HIP_SYMBOLS = ["tpgp_hip_compiled", "tpgp_hip_create", "tpgp_hip_destroy",
                "tpgp_hip_batch_resize", "tpgp_hip_batch_graph_replay",
                "tpgp_hip_batch_graph_begin", "tpgp_hip_batch_graph_end",
                "tpgp_hip_batch_sum_squares", "tpgp_hip_liouvillian", "tpgp_hip_set_tuning",
-               "tpgp_hip_get_tuning"]
+               "tpgp_hip_get_tuning", "tpgp_hip_residual_vectors",
+               "tpgp_hip_interval_vector_free"]
 ENGINE_SYMBOLS = ["tpgp_status_string", "tpgp_has_mpfr", "tpgp_has_openmp",
                   "tpgp_default_options", "tpgp_ledger_init", "tpgp_map_init", "tpgp_map_free",
                   "tpgp_map_clone", "tpgp_key_get", "tpgp_key_set", "tpgp_map_add",
@@ -214,6 +224,9 @@ def hip():
     f("tpgp_hip_batch_sum_squares", C.c_int, C.c_void_p, sz, p_d, p_c, sz)
     f("tpgp_hip_liouvillian", C.c_int, C.c_void_p, p_u64, p_d, sz, sz, sz, C.POINTER(TpgpTerm),
       sz, p_u64, p_d, sz, C.POINTER(sz), p_c, sz)
+    f("tpgp_hip_residual_vectors", C.c_int, C.c_void_p, p_u64, p_d, sz, p_u64, p_d, sz, sz, sz,
+      C.POINTER(TpgpTerm), sz, C.c_double, C.c_int, C.POINTER(HipIntervalVector), p_c, sz)
+    f("tpgp_hip_interval_vector_free", None, C.POINTER(HipIntervalVector))
     f("tpgp_hip_batch_profile", C.c_int, C.c_void_p, C.c_uint)
     f("tpgp_hip_batch_profile_read", C.c_int, C.c_void_p, C.c_uint, C.POINTER(C.c_double),
       C.POINTER(C.c_uint64))

@@ -175,6 +175,39 @@ int tpgp_hip_liouvillian(tpgp_hip_context *context, const uint64_t *keys, const 
                          double *out_values, size_t out_capacity, size_t *out_count,
                          char *error, size_t error_capacity);
 
+/* No counterpart in the reference's GPU backend: the interval vectors of the independent endpoint
+ * (residual) bound, built on the device.  The reference builds them on its CPU in 256-bit MPFR
+ * (engine/src/residual_rev.c:60-114, 181-187):
+ *   D1 = 6 b - 6 a - 4 dt L a - 2 dt L b - dt^2 L L a,
+ *   D2 = 6 b - 6 a - 2 dt L a - 4 dt L b + dt^2 L L b,
+ * L the Liouvillian generator on sparse interval vectors (incident_terms = 1: the form of
+ * tpgp_residual_bound, which passes over [0, 0] entries and h = 0 terms; 0: every entry against
+ * every term, as tpgp_residual_bound_reference).  Every entry's ends are the reference's bits:
+ * the same correctly rounded 256-bit products and sums (down for lower ends, up for upper ends)
+ * in the same order, and the entries come in the reference's first-insertion order.
+ * a and b are maps (sorted keys, finite binary64 values) of one spin count; terms as in
+ * tpgp_hip_liouvillian; dt finite and positive.  On success out[0] = D1 and out[1] = D2, each
+ * with arrays allocated by the library (free with tpgp_hip_interval_vector_free).  EINVAL: bad
+ * arguments or an invalid term; ENOMEM: device or host memory, or more than 2^31 contributions
+ * in one generator application.  Synchronous; device memory is allocated per call. */
+typedef struct {
+    uint64_t m[4]; /* significand, m[3] most significant, top bit set unless the value is 0 */
+    int32_t e;     /* value = (-1)^neg * (m / 2^256) * 2^e */
+    uint32_t neg;
+} tpgp_hip_f256;
+typedef struct {
+    size_t len;
+    uint64_t *keys; /* len * n_words, in first-insertion order */
+    tpgp_hip_f256 *lower, *upper;
+} tpgp_hip_interval_vector;
+int tpgp_hip_residual_vectors(tpgp_hip_context *context, const uint64_t *a_keys,
+                              const double *a_values, size_t a_len, const uint64_t *b_keys,
+                              const double *b_values, size_t b_len, size_t n_words,
+                              size_t n_spins, const tpgp_hip_term *terms, size_t n_terms,
+                              double dt, int incident_terms, tpgp_hip_interval_vector out[2],
+                              char *error, size_t error_capacity);
+void tpgp_hip_interval_vector_free(tpgp_hip_interval_vector *vector);
+
 /* Sum of squares of map `map`'s committed values as an unevaluated pair out[0] + out[1]
  * (error-free double-double accumulation, relative error ~len * 2^-104).  Synchronous; the map
  * must have no uncommitted work.  Feeds the l2 norm of tpgp_map_l2 (tpgp_rev.c:266-271). */
