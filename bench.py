@@ -203,6 +203,94 @@ def bench_powder(args):
     }
 
 
+def bench_residual(args):
+    """The independent endpoint (residual) bound of one N=16 J1-J2 Strang step
+    (tpgp_residual_bound, residual_rev.c:181-194, incident-terms generator with the Gram
+    enclosure, depth --residual-depth) at the operating point --start-step: its interval vectors
+    D1, D2 built on the GPU, then the serial Gram and norm sums on the host; against the
+    reference library's CPU bound (oracle/_ref, one thread: the reference's residual code has no
+    threads) on the same two maps, which must give the same bits."""
+    import numpy as np
+
+    from ddevol_amd import engine as E
+    from ddevol_amd import hamiltonians as H
+    from ddevol_amd import native as N
+
+    ctx = E.Context(0)
+    n, dt, thr, depth = 16, 0.005, 1e-6, args.residual_depth
+    terms = H.j1j2_n16_terms(n)
+    a = E.Map(n)
+    for s in range(n):
+        a.add_single(s, 1, 1.0)
+    o = E.options(threshold=thr, max_terms=10 ** 10, context=ctx)
+    st, *_ = E.evolve(a, terms, dt, args.start_step, o, E.ledger(0))
+    assert st == 0, N_last_error()
+    keys, vals = a.arrays()
+    b = E.Map.from_sorted(n, keys, vals)
+    st, *_ = E.evolve(b, terms, dt, 1, o, E.ledger(0))
+    assert st == 0, N_last_error()
+    log(f"residual: maps of {len(a)} -> {len(b)} terms after {args.start_step} steps")
+    # the device part alone: D1 and D2
+    err = C.create_string_buffer(512)
+    out = (N.HipIntervalVector * 2)()
+    ta = N.terms_array(terms)
+    hip = N.hip()
+    t0 = time.perf_counter()
+    rc = hip.tpgp_hip_residual_vectors(ctx.ptr, a.m.keys, a.m.values, a.m.len, b.m.keys,
+                                       b.m.values, b.m.len, 1, n, ta, len(terms), dt, 1, out,
+                                       err, 512)
+    t_vec = time.perf_counter() - t0
+    assert rc == 0, err.value
+    sizes = (out[0].len, out[1].len)
+    for d in range(2):
+        hip.tpgp_hip_interval_vector_free(C.byref(out[d]))
+    # the whole bound, best of --residual-repeats
+    times = []
+    for _ in range(max(1, args.residual_repeats)):
+        t0 = time.perf_counter()
+        st, up = E.residual_bound(a, b, terms, dt, depth)
+        times.append(time.perf_counter() - t0)
+        assert st == 0, N_last_error()
+    t_gpu = min(times)
+    log(f"residual: D1 {sizes[0]} and D2 {sizes[1]} entries; vectors {t_vec:.3f} s; "
+        f"bound {up:.6e} in {t_gpu:.3f} s")
+    cpu = None
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libtpgp_ref.so")
+    if not args.no_cpu and os.path.exists(ref_so):
+        lib = C.CDLL(ref_so)
+        fn = lib.tpgp_residual_bound
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_double, C.c_uint,
+                       C.POINTER(C.c_double)]
+        rup = C.c_double(float("nan"))
+        t0 = time.perf_counter()
+        rst = fn(C.addressof(a.m), C.addressof(b.m), C.cast(ta, C.c_void_p), len(terms), dt,
+                 depth, C.byref(rup))
+        t_cpu = time.perf_counter() - t0
+        same = rst == 0 and np.float64(up).view(np.uint64) == np.float64(rup.value).view(
+            np.uint64)
+        log(f"residual: reference CPU bound {rup.value:.6e} in {t_cpu:.2f} s; same bits: {same}")
+        cpu = {"value": 1.0 / t_cpu, "unit": "bounds/s", "cores": 1, "kind": "reference",
+               "seconds": t_cpu, "same_bits": bool(same), "cpu_model": cpu_model(),
+               "sample": "one bound on the same two maps (the reference's residual code is "
+                         "single-threaded)"}
+    return {
+        "metric": "residual bounds/s (N=16 J1-J2 Strang step, tpgp_residual_bound)",
+        "value": 1.0 / t_gpu, "unit": "bounds/s", "n_gpus": 1, "higher_is_better": True,
+        "vs_baseline": None, "dtype": "256-bit intervals (MPFR-exact), f64 result",
+        "data": "synthetic (seeded couplings; initial state sum_s X_s)",
+        "config": {"workload": f"N=16 J1-J2 ring, dt 0.005, threshold 1e-6: the bound of Strang "
+                               f"step {args.start_step + 1} (maps of {len(a)} -> {len(b)} "
+                               f"terms), depth {depth}, incident-terms generator with the Gram "
+                               f"enclosure",
+                   "d1_entries": sizes[0], "d2_entries": sizes[1], "bound": up},
+        "seconds": t_gpu, "device_vectors_seconds": t_vec,
+        "host_sums_seconds": max(t_gpu - t_vec, 0.0),
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": (cpu["seconds"] / t_gpu) if cpu else None,
+    }
+
+
 def N_last_error():
     from ddevol_amd import native as N
 
@@ -413,10 +501,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="j1j2",
-                    choices=["j1j2", "powder", "serve", "large", "plain"],
+                    choices=["j1j2", "powder", "serve", "large", "plain", "residual"],
                     help="j1j2: the headline N=16 J1-J2 window (default); powder: batched "
                          "orientations vs one at a time; serve: continuous batching with "
-                         "p50/p99; large: N=64 range-4 ring; plain: N=12 ring")
+                         "p50/p99; large: N=64 range-4 ring; plain: N=12 ring; residual: the "
+                         "residual bound of one J1-J2 step against the reference's CPU bound")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="calibration only: run 5 plain device-to-device copies of this many "
                          "bytes (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -444,6 +533,10 @@ def main():
     ap.add_argument("--until-terms", type=int, default=0,
                     help="large/plain: warm up until the map holds this many terms (--warmup "
                          "then bounds the warmup steps)")
+    ap.add_argument("--residual-depth", type=int, default=1,
+                    help="residual: subdivision depth of the bound")
+    ap.add_argument("--residual-repeats", type=int, default=3,
+                    help="residual: timed bounds (the best is reported)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--breakdown", action="store_true",
@@ -474,7 +567,8 @@ def main():
             print(summarize_pmc(args.summarize_pmc, rot, args.copy_pmc))
         return
     if args.start_step is None and args.workload != "j1j2":
-        args.start_step = 475 if args.workload == "large" else HEADLINE_START_STEP
+        args.start_step = {"large": 475, "residual": RESIDUAL_START_STEP}.get(
+            args.workload, HEADLINE_START_STEP)
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one FID per GPU: only --gpus 1 is supported; "
                                    "multi-GPU sharding is a design, not part of this build"}))
@@ -497,6 +591,8 @@ def main():
             out = bench_powder(args)
         elif args.workload == "serve":
             out = bench_serve(args)
+        elif args.workload == "residual":
+            out = bench_residual(args)
         elif args.workload == "large":
             out = bench_single(args, 64, H.range_ring(64, 4), 0.01, 1e-5,
                                "N=64 range-4 ring",
@@ -668,6 +764,7 @@ def summarize_pmc(path, rotations, copy_path=None, kernels=("k_rot_write", "k_ro
 # The headline FID's operating point: Strang step 600 of the N=16 J1-J2 FID, where the map holds
 # ~1.43e7 terms (the survey's windowing: the CPU path cannot run the whole 1000-step FID).
 HEADLINE_START_STEP = 600
+RESIDUAL_START_STEP = 150  # the bound of step 151
 
 
 def bench_headline(args):
