@@ -171,7 +171,10 @@ def bench_powder(args):
     fl.close()
     log(f"powder batched: {K} FIDs x {steps} steps in {t_batch:.2f} s ({rounds} rounds), "
         f"{c.updates} updates, largest map {peak} terms")
-    # one FID at a time (tpgp_hip_evolve per orientation)
+    # one FID at a time (a resident evolution session per orientation, in chunks of 25 steps
+    # with a progress line each: the samples are those of one call)
+    import numpy as np
+
     A = min(args.alone, K)
     t0 = time.perf_counter()
     same = True
@@ -180,11 +183,18 @@ def bench_powder(args):
         for s in range(n):
             m.add_single(s, 1, 1.0)
         led = E.ledger(0)
-        st, fid, rad, act, acc = E.evolve(m, terms[k], dt, steps,
-                                          E.options(threshold=thr, max_terms=10 ** 10,
-                                                    context=ctx), led)
-        assert st == 0 and acc == steps
-        same &= same_bits(fid, fids[k])
+        ev = E.Evolution(m, terms[k], dt, E.options(threshold=thr, max_terms=10 ** 10,
+                                                    context=ctx), context=ctx)
+        parts, done = [], 0
+        while done < steps:
+            st, fid, rad, act, acc = ev.run(min(25, steps - done), led)
+            assert st == 0, N_last_error()
+            parts.append(fid[:acc + 1] if not parts else fid[1:acc + 1])
+            done += acc
+            log(f"powder one-at-a-time: orientation {k}, step {done}, {int(act[acc])} terms, "
+                f"{time.perf_counter() - t0:.1f} s")
+        ev.close()
+        same &= same_bits(np.concatenate(parts), fids[k])
     t_alone = time.perf_counter() - t0
     batched, alone = K / t_batch, A / t_alone
     log(f"powder one-at-a-time: {A} FIDs in {t_alone:.2f} s; identical FIDs: {same}")
@@ -299,7 +309,11 @@ def N_last_error():
 
 def bench_serve(args):
     """Serving stream with continuous batching (ddevol_amd.serve) against the one-FID-at-a-time
-    loop on the same requests; p50/p99 of completion - arrival, FIDs/s, identical FIDs."""
+    loop on the same requests; p50/p99 of completion - arrival over the served requests,
+    FIDs/s, identical FIDs.  Requests whose projected term-array bytes exceed the budget even
+    beside empty slots are refused by admission and counted; the one-at-a-time loop runs the
+    served ones.  --serve-phase runs one side only (served: the stream; alone: the loop over the
+    requests the stream would serve) so that long streams fit separate runs."""
     import numpy as np
 
     from ddevol_amd import engine as E
@@ -309,39 +323,55 @@ def bench_serve(args):
     lo, hi = args.serve_steps
     reqs = S.poisson_stream(args.requests, args.rate, seed=7, steps_range=(lo, hi))
     srv = S.Server(ctx, slots_per_width=args.slots, budget_bytes=int(args.budget_gb * 2 ** 30))
-    t0 = time.perf_counter()
-    srv.run(reqs, log=log)
-    t_srv = time.perf_counter() - t0
-    rounds = srv.rounds
+    t_srv = rounds = None
+    if args.serve_phase in ("both", "served"):
+        t0 = time.perf_counter()
+        srv.run(reqs, log=log)
+        t_srv = time.perf_counter() - t0
+        rounds = srv.rounds
+        refused = {r.rid for r in srv.refused}
+    else:  # the admission decision alone: the same requests as the stream would refuse
+        refused = {r.rid for r in reqs if not srv.fits(r)}
     srv.close()
-    lat = np.array([r.latency for r in reqs])
-    alone = [S.Request(r.rid, r.arrival, r.n, r.terms, r.threshold, r.steps, r.dt) for r in reqs]
-    t0 = time.perf_counter()
-    S.run_alone(ctx, alone, log=log)
-    t_alone = time.perf_counter() - t0
-    lat_a = np.array([r.latency for r in alone])
-    same = all(same_bits(a.fid, b.fid) for a, b in zip(reqs, alone))
-    log(f"served {len(reqs)} requests in {t_srv:.2f} s ({rounds} rounds); one at a time "
-        f"{t_alone:.2f} s; identical FIDs: {same}")
+    served = [r for r in reqs if r.rid not in refused]
     pct = lambda a, q: float(np.percentile(a, q))  # noqa: E731
-    return {
+    out = {
         "metric": "served FIDs/s with p50/p99 latency (continuous batching)",
-        "value": len(reqs) / t_srv, "unit": "FIDs/s", "n_gpus": 1, "higher_is_better": True,
+        "value": None, "unit": "FIDs/s", "n_gpus": 1, "higher_is_better": True,
         "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded Poisson stream, InputPCG(7))",
         "config": {"workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, N in 8..48 "
                                f"rings (NN U[0.5,1.5], NNN 1/8), threshold in "
                                f"{{1e-4,1e-5,1e-6}}, steps U{{{lo}..{hi}}}, dt 0.01",
-                   "steps_note": "the config's U{100..2000} steps scaled down: at threshold "
-                                 "1e-6 an N=48 ring passes 4e7 terms by step 300",
                    "hbm_budget_gb": args.budget_gb, "slots_per_key_width": args.slots,
-                   "rounds": rounds},
-        "latency_s": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": float(lat.max())},
-        "one_at_a_time": {"value": len(reqs) / t_alone, "unit": "FIDs/s",
-                          "latency_s": {"p50": pct(lat_a, 50), "p99": pct(lat_a, 99)},
-                          "fids_identical": same},
-        "batched_over_one_at_a_time": t_alone / t_srv,
+                   "rounds": rounds, "refused": sorted(refused), "served": len(served),
+                   "phase": args.serve_phase},
     }
+    if t_srv is not None:
+        ok = [r for r in served if r.status == 0]
+        lat = np.array([r.latency for r in served])
+        out["value"] = len(ok) / t_srv
+        out["seconds"] = t_srv
+        out["latency_s"] = {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": float(lat.max())}
+        out["failed"] = [r.rid for r in served if r.status != 0]
+        log(f"served {len(ok)} of {len(reqs)} requests in {t_srv:.2f} s ({rounds} rounds), "
+            f"{len(refused)} refused by admission")
+    if args.serve_phase in ("both", "alone"):
+        alone = [S.Request(r.rid, r.arrival, r.n, r.terms, r.threshold, r.steps, r.dt)
+                 for r in served]
+        t0 = time.perf_counter()
+        S.run_alone(ctx, alone, log=log)
+        t_alone = time.perf_counter() - t0
+        lat_a = np.array([r.latency for r in alone])
+        one = {"value": sum(r.status == 0 for r in alone) / t_alone, "unit": "FIDs/s",
+               "seconds": t_alone,
+               "latency_s": {"p50": pct(lat_a, 50), "p99": pct(lat_a, 99)}}
+        if t_srv is not None:
+            one["fids_identical"] = all(same_bits(a.fid, b.fid) for a, b in zip(served, alone))
+            out["batched_over_one_at_a_time"] = t_alone / t_srv
+        out["one_at_a_time"] = one
+        log(f"one at a time: {len(alone)} requests in {t_alone:.2f} s")
+    return out
 
 
 def fid_allowance(E, terms, dt, led, n, steps):
@@ -515,9 +545,13 @@ def main():
                     help="powder: orientations run one at a time for the comparison")
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--rate", type=float, default=16.0)
-    ap.add_argument("--serve-steps", type=int, nargs=2, default=[10, 200])
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=[100, 2000],
+                    help="serve: the range of Strang steps per request (the config's 100..2000)")
+    ap.add_argument("--serve-phase", default="both", choices=["both", "served", "alone"],
+                    help="serve: run the stream, the one-at-a-time loop, or both")
     ap.add_argument("--slots", type=int, default=64)
-    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--budget-gb", type=float, default=192.0,
+                    help="serve: the HBM budget admission charges projected term arrays to")
     ap.add_argument("--steps", type=int, default=20, help="timed Strang steps")
     ap.add_argument("--warmup", type=int, default=5,
                     help="untimed steps at the operating point, after the untimed growth to "

@@ -922,9 +922,56 @@ __device__ __forceinline__ uint32_t local_split(const TileIn<W> &in, const TileG
 
 // Walk this thread's kMergeItems merge-path positions of the tile from its split `lo`, calling
 // f(key, hasA, va, hasD, vd, i, j) once per output key in ascending order (i, j: the LDS
-// indices of its A and T(D) elements, meaningful where hasA / hasD).
+// indices of its A and T(D) elements, meaningful where hasA / hasD; an absent side's value is
+// passed as 0.0).
+// Branch-free steps: the current A and T(D) keys stay in registers (read at clamped indices
+// inside the staged tile when a run is exhausted, and then ignored); each step chooses the A
+// element (with its equal T(D) partner, which may be the element after the tile) or the T(D)
+// element alone by selects, so f is inlined once and the wave does not diverge on the kind of
+// step.
 template <int W, typename F>
 __device__ __forceinline__ void walk_from(const TileIn<W> &in, const TileGeom &t, uint32_t lo,
+                                          F &&f) {
+    const uint32_t nloc = (uint32_t)(t.d1 - t.d0);
+    const uint32_t dt = threadIdx.x * kMergeItems;
+    if (dt >= nloc) return;
+    const uint32_t na = t.na, nd = t.nd, nd_look = nd + (t.has_next ? 1u : 0u);
+    auto Akey = [&](uint32_t i) { return kload<W>(in.keys + i * W); };
+    auto Dkey = [&](uint32_t j) { return kload<W>(in.keys + (na + j) * W); };
+    const uint32_t limit = (nloc - dt) < (uint32_t)kMergeItems ? (nloc - dt) : (uint32_t)kMergeItems;
+    uint32_t i = lo, j = dt - lo, used = 0;
+    // a D element equal to the A element just before this split was consumed with it
+    if (j < nd && (i > 0 || t.has_prev)) {
+        Key<W> pa = i > 0 ? Akey(i - 1) : kload<W>(in.keys + (kMergeTile + 1) * W);
+        if (keq<W>(pa, Dkey(j))) {
+            ++j;
+            ++used;
+        }
+    }
+    const uint32_t alast = na ? na - 1 : 0, dlast = nd_look ? nd_look - 1 : 0;
+    Key<W> x = Akey(i < alast ? i : alast), y = Dkey(j < dlast ? j : dlast);
+#pragma unroll
+    for (int step = 0; step < kMergeItems; ++step) {
+        if (used >= limit) break;
+        const bool hasA = i < na, hasD = j < nd;
+        const bool takeA = hasA && (!hasD || !klt<W>(y, x));  // A[i] <= D[j]
+        const bool takeD = takeA ? (j < nd_look && keq<W>(y, x)) : true;
+        const uint32_t ia = i < alast ? i : alast, jd = j < dlast ? j : dlast;
+        const double va = takeA ? in.vals[ia] : 0.0, vd = takeD ? in.vals[na + jd] : 0.0;
+        f(takeA ? x : y, takeA, va, takeD, vd, i, j);
+        i += takeA ? 1u : 0u;
+        j += takeD ? 1u : 0u;
+        used += (takeA ? 1u : 0u) + (takeD ? 1u : 0u);
+        x = Akey(i < alast ? i : alast);
+        y = Dkey(j < dlast ? j : dlast);
+    }
+}
+
+// walk_from with a branch per kind of step (A with its partner, A alone, T(D) alone): f is
+// inlined three times.  The write pass keeps this form: it fits its seven-wave register budget
+// without spills, which the select form does not.
+template <int W, typename F>
+__device__ __forceinline__ void walk_branchy(const TileIn<W> &in, const TileGeom &t, uint32_t lo,
                                           F &&f) {
     const uint32_t nloc = (uint32_t)(t.d1 - t.d0);
     const uint32_t dt = threadIdx.x * kMergeItems;
@@ -1190,15 +1237,14 @@ __device__ __forceinline__ void account(const BatchDev &b, uint32_t m, uint32_t 
         a.rsq += upmul(r, r);
         if (want_log) log_entry(b, k0, k1, r, 0.0, m, rot, 1);
     }
-    if (v == 0.0) return;  // exact cancellation: dropped silently
-    if (!kzero<W>(key) && __builtin_fabs(v) <= k.thr) {
-        a.cnt += 1u << 8;
-        a.l1 += __builtin_fabs(v);
-        a.sq += upmul(v, v);
-        if (want_log) log_entry(b, k0, k1, __builtin_fabs(v), v, m, rot, 0);
-        return;
-    }
-    a.cnt += 1u;
+    // tallies by selects: adding +0.0 to a non-negative partial sum leaves it unchanged, so the
+    // sums are those of the dropped outputs alone
+    const bool kept_or_dropped = v != 0.0;  // an exact cancellation is dropped silently
+    const bool drop = kept_or_dropped && !kzero<W>(key) && __builtin_fabs(v) <= k.thr;
+    a.cnt += drop ? (1u << 8) : (kept_or_dropped ? 1u : 0u);
+    a.l1 += drop ? __builtin_fabs(v) : 0.0;
+    a.sq += drop ? upmul(v, v) : 0.0;
+    if (want_log && drop) log_entry(b, k0, k1, __builtin_fabs(v), v, m, rot, 0);
 }
 
 // Pass 1 of a rotation: per tile, the merge-path splits, the number of kept outputs, the
@@ -1591,7 +1637,7 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
         uint32_t code[kMergeItems];
         unsigned keep0 = 0, keep1 = 0, ncall = 0;
         bool bad = false;
-        walk_from<W>(in, t, lo, [&](const Key<W> &key, bool hasA, double va, bool hasD,
+        walk_branchy<W>(in, t, lo, [&](const Key<W> &key, bool hasA, double va, bool hasD,
                                     double vd, uint32_t i, uint32_t j) {
             double v, w = 0.0, r;
             merged<false>(k0, hasA, va, hasD, vd, v, r, bad);
