@@ -237,4 +237,67 @@ F256_FN f256 f256_neg(f256 x) {
     return x;
 }
 
+
+/* x * y rounded toward dir (mpfr_mul or mpfr_sqr at 256 bits), both operands of full 256-bit
+ * precision: the exact 512-bit product, normalised, then rounded with the low 256 bits as the
+ * remainder. */
+F256_FN f256 f256_mul_full(f256 x, f256 y, int dir) {
+    const uint32_t neg = x.neg ^ y.neg;
+    if (f256_is_zero(&x) || f256_is_zero(&y)) return f256_zero(neg);
+    uint64_t p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+    for (int j = 0; j < 4; ++j) {
+        const uint64_t yj = y.m[j];
+        uint64_t carry = 0;
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+        for (int i = 0; i < 4; ++i) {
+            const uint64_t lo = x.m[i] * yj, hi = f256_mulhi(x.m[i], yj);
+            const uint64_t t = p[i + j] + lo, c1 = t < lo;
+            const uint64_t u = t + carry, c2 = u < carry;
+            p[i + j] = u;
+            carry = hi + c1 + c2;
+        }
+        p[4 + j] = carry;
+    }
+    int32_t e = x.e + y.e;
+    if (!(p[7] >> 63)) { /* the product of two significands in [1/2, 1) is in [1/4, 1) */
+        for (int i = 7; i > 0; --i) p[i] = (p[i] << 1) | (p[i - 1] >> 63);
+        p[0] <<= 1;
+        e -= 1;
+    }
+    /* round on a 384-bit window: the top 256 bits, and any nonzero bit of the low 256 folded
+     * into the window's lowest word (the remainder only has to be known to be zero or not) */
+    const uint64_t w[6] = {(p[0] | p[1] | p[2] | p[3]) != 0, 0, p[4], p[5], p[6], p[7]};
+    return f256_round(w, e, neg, dir);
+}
+
+/* The sign of x - y (-1, 0, 1) as mpfr_cmp gives it: zeros compare equal whatever their sign. */
+F256_FN int f256_cmp(const f256 *x, const f256 *y) {
+    const int xz = f256_is_zero(x), yz = f256_is_zero(y);
+    if (xz && yz) return 0;
+    if (xz) return y->neg ? 1 : -1;
+    if (yz) return x->neg ? -1 : 1;
+    if (x->neg != y->neg) return x->neg ? -1 : 1;
+    int mag = 0; /* |x| against |y| */
+    if (x->e != y->e)
+        mag = x->e > y->e ? 1 : -1;
+    else
+        for (int i = 3; i >= 0 && !mag; --i)
+            mag = x->m[i] > y->m[i] ? 1 : x->m[i] < y->m[i] ? -1 : 0;
+    return x->neg ? -mag : mag;
+}
+
+/* |x| (exact). */
+F256_FN f256 f256_abs(f256 x) {
+    x.neg = 0;
+    return x;
+}
+
+/* The sign as mpfr_sgn gives it: 0 for either zero. */
+F256_FN int f256_sgn(const f256 *x) { return f256_is_zero(x) ? 0 : x->neg ? -1 : 1; }
+
 #endif

@@ -629,6 +629,12 @@ __global__ void k_res_concat(const uint64_t *keys, uint64_t n, uint64_t base, ui
     }
 }
 
+__global__ void k_res_none(uint32_t *p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0xffffffffu;
+}
+
 // Flag the slot of every run's first contribution (the sort kept slots ascending in a run).
 __global__ void k_res_first(const uint32_t *order, const uint64_t *start, uint64_t runs,
                             uint32_t *flag) {
@@ -801,6 +807,126 @@ int res_combine(hipStream_t st, Arena &keep, const RVec *x, const ResScalar *d_s
     return TPGP_HIP_OK;
 }
 
+// The entry of every query key among n sorted keys (their original indices in idx), or
+// UINT32_MAX: a binary search per query (keys compare from the highest word down).
+template <int W>
+__global__ void k_res_join(const uint64_t *query, uint64_t nq, const uint64_t *sorted,
+                           const uint32_t *idx, uint64_t n, uint32_t *partner) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t key[W];
+        for (int w = 0; w < W; ++w) key[w] = query[i * W + w];
+        uint64_t lo = 0, hi = n;  // first sorted key >= key
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            int cmp = 0;
+            for (int w = W - 1; w >= 0 && !cmp; --w) {
+                const uint64_t a = sorted[mid * W + w];
+                cmp = a < key[w] ? -1 : a > key[w] ? 1 : 0;
+            }
+            if (cmp < 0)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        bool found = lo < n;
+        for (int w = 0; w < W && found; ++w) found = sorted[lo * W + w] == key[w];
+        partner[i] = found ? idx[lo] : 0xffffffffu;
+    }
+}
+
+// The Gram enclosure's terms of every entry (residual_rev.c:122-153), in the reference's
+// operations: the square terms [min of the downward squares of the ends, or +0 when the
+// interval holds 0; max of the upward squares], and, with a partner y in the other vector, the
+// product terms [min of the four downward end products; max of the four upward ones].
+__global__ void k_res_gram(const f256 *lo, const f256 *hi, uint64_t n, const uint32_t *partner,
+                           const f256 *plo, const f256 *phi, f256 *sq_lo, f256 *sq_hi,
+                           f256 *pr_lo, f256 *pr_hi) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const f256 a = lo[i], b = hi[i];
+        f256 p = f256_mul_full(a, a, F256_UP), q = f256_mul_full(b, b, F256_UP);
+        if (f256_cmp(&q, &p) > 0) p = q;
+        sq_hi[i] = p;
+        if (f256_sgn(&a) <= 0 && f256_sgn(&b) >= 0) {
+            p = f256_zero(0);
+        } else {
+            p = f256_mul_full(a, a, F256_DOWN);
+            q = f256_mul_full(b, b, F256_DOWN);
+            if (f256_cmp(&q, &p) < 0) p = q;
+        }
+        sq_lo[i] = p;
+        if (!pr_lo) continue;
+        const uint32_t j = partner[i];
+        if (j == 0xffffffffu) {
+            pr_lo[i] = f256_zero(0);
+            pr_hi[i] = f256_zero(0);
+            continue;
+        }
+        const f256 c = plo[j], d = phi[j];
+        p = f256_mul_full(a, c, F256_DOWN);
+        q = f256_mul_full(a, c, F256_UP);
+        auto fold = [&](const f256 &x, const f256 &y) {  // end pairs (lo, hi), (hi, lo), (hi, hi)
+            f256 r = f256_mul_full(x, y, F256_DOWN);
+            if (f256_cmp(&r, &p) < 0) p = r;
+            r = f256_mul_full(x, y, F256_UP);
+            if (f256_cmp(&r, &q) > 0) q = r;
+        };
+        fold(a, d);
+        fold(b, c);
+        fold(b, d);
+        pr_lo[i] = p;
+        pr_hi[i] = q;
+    }
+}
+
+// D1's and D2's partners (the same key in the other vector) and their Gram terms.
+template <int W>
+int res_gram_terms(hipStream_t st, Arena &keep, const RVec *d, unsigned lo_bits,
+                   unsigned hi_bits, uint32_t *partner[2], f256 *sq[2][2], f256 *pr[2],
+                   char *error, size_t cap) {
+    hipError_t e;
+    Arena A;
+    for (int k = 0; k < 2; ++k) {
+        partner[k] = keep.get<uint32_t>(d[k].len);
+        sq[k][0] = keep.get<f256>(d[k].len);
+        sq[k][1] = keep.get<f256>(d[k].len);
+    }
+    pr[0] = keep.get<f256>(d[0].len);
+    pr[1] = keep.get<f256>(d[0].len);
+    if (keep.err != hipSuccess) return drain_fail(st, keep.err, "HIP residual allocation", error,
+                                                  cap);
+    for (int k = 0; k < 2; ++k) {  // partner[k]: entries of d[k] found in d[1 - k]
+        const RVec &q = d[k], &o = d[1 - k];
+        if (!q.len) continue;
+        if (!o.len) {
+            // no partner for any entry (a fill kernel, not a memset)
+            hipLaunchKernelGGL(k_res_none, dim3(grid_for(q.len)), dim3(256), 0, st, partner[k],
+                               q.len);
+            continue;
+        }
+        uint64_t *sk = A.get<uint64_t>(o.len * W);
+        uint32_t *si = A.get<uint32_t>(o.len);
+        if (A.err != hipSuccess) return drain_fail(st, A.err, "HIP residual allocation", error,
+                                                   cap);
+        if ((e = hipMemcpyAsync(sk, o.keys, o.len * W * 8, hipMemcpyDeviceToDevice, st)))
+            return drain_fail(st, e, "HIP residual copy", error, cap);
+        hipLaunchKernelGGL(k_gen_iota, dim3(grid_for(o.len)), dim3(256), 0, st, si, o.len);
+        if ((e = sort_pairs<W>(st, A, sk, si, o.len, lo_bits, hi_bits)) != hipSuccess)
+            return drain_fail(st, e, "HIP residual sort", error, cap);
+        hipLaunchKernelGGL(k_res_join<W>, dim3(grid_for(q.len)), dim3(256), 0, st, q.keys,
+                           q.len, sk, si, o.len, partner[k]);
+    }
+    for (int k = 0; k < 2; ++k)
+        if (d[k].len)
+            hipLaunchKernelGGL(k_res_gram, dim3(grid_for(d[k].len)), dim3(256), 0, st, d[k].lo,
+                               d[k].hi, d[k].len, partner[k], d[1].lo, d[1].hi, sq[k][0],
+                               sq[k][1], k == 0 ? pr[0] : nullptr, k == 0 ? pr[1] : nullptr);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess)
+        return hip_fail(e, "HIP residual Gram terms", error, cap);
+    return TPGP_HIP_OK;
+}
+
 f256 res_scale(double base_power_dt, int power, int factor) {
     // factor * dt^power exactly (at most 109 significant bits; residual_rev.c:112-114)
     f256 s = f256_from_double(power == 0 ? 1.0 : base_power_dt);
@@ -879,24 +1005,39 @@ int residual_vectors(tpgp_hip_context *ctx, const uint64_t *a_keys, const double
                            error, cap);
     }
     if (s) return s;
+    uint32_t *partner[2];
+    f256 *sq[2][2], *pr[2];
+    if ((s = res_gram_terms<W>(st, keep, v + 6, lo_bits, hi_bits, partner, sq, pr, error, cap)))
+        return s;
     for (int d = 0; d < 2; ++d) {
         const RVec &r = v[6 + d];
         tpgp_hip_interval_vector &o = out[d];
+        const size_t n1 = r.len ? r.len : 1;
         o.len = r.len;
-        o.keys = (uint64_t *)malloc((r.len ? r.len : 1) * W * 8);
-        o.lower = (tpgp_hip_f256 *)malloc((r.len ? r.len : 1) * sizeof(tpgp_hip_f256));
-        o.upper = (tpgp_hip_f256 *)malloc((r.len ? r.len : 1) * sizeof(tpgp_hip_f256));
-        if (!o.keys || !o.lower || !o.upper) {
+        o.keys = (uint64_t *)malloc(n1 * W * 8);
+        o.lower = (tpgp_hip_f256 *)malloc(n1 * sizeof(tpgp_hip_f256));
+        o.upper = (tpgp_hip_f256 *)malloc(n1 * sizeof(tpgp_hip_f256));
+        o.partner = (uint32_t *)malloc(n1 * sizeof(uint32_t));
+        for (int k = 0; k < 2; ++k) {
+            o.square[k] = (tpgp_hip_f256 *)malloc(n1 * sizeof(tpgp_hip_f256));
+            o.product[k] = d == 0 ? (tpgp_hip_f256 *)malloc(n1 * sizeof(tpgp_hip_f256)) : nullptr;
+        }
+        if (!o.keys || !o.lower || !o.upper || !o.partner || !o.square[0] || !o.square[1] ||
+            (d == 0 && (!o.product[0] || !o.product[1]))) {
             (void)hipStreamSynchronize(st);
             msg(error, cap, "Host allocation of the residual vectors failed.");
             return TPGP_HIP_ENOMEM;
         }
-        if (r.len &&
-            ((e = hipMemcpyAsync(o.keys, r.keys, r.len * W * 8, hipMemcpyDeviceToHost, st)) ||
-             (e = hipMemcpyAsync(o.lower, r.lo, r.len * sizeof(f256), hipMemcpyDeviceToHost,
-                                 st)) ||
-             (e = hipMemcpyAsync(o.upper, r.hi, r.len * sizeof(f256), hipMemcpyDeviceToHost,
-                                 st))))
+        if (!r.len) continue;
+        const size_t fb = r.len * sizeof(f256);
+        if ((e = hipMemcpyAsync(o.keys, r.keys, r.len * W * 8, hipMemcpyDeviceToHost, st)) ||
+            (e = hipMemcpyAsync(o.lower, r.lo, fb, hipMemcpyDeviceToHost, st)) ||
+            (e = hipMemcpyAsync(o.upper, r.hi, fb, hipMemcpyDeviceToHost, st)) ||
+            (e = hipMemcpyAsync(o.partner, partner[d], r.len * 4, hipMemcpyDeviceToHost, st)) ||
+            (e = hipMemcpyAsync(o.square[0], sq[d][0], fb, hipMemcpyDeviceToHost, st)) ||
+            (e = hipMemcpyAsync(o.square[1], sq[d][1], fb, hipMemcpyDeviceToHost, st)) ||
+            (d == 0 && (e = hipMemcpyAsync(o.product[0], pr[0], fb, hipMemcpyDeviceToHost, st))) ||
+            (d == 0 && (e = hipMemcpyAsync(o.product[1], pr[1], fb, hipMemcpyDeviceToHost, st))))
             return drain_fail(st, e, "HIP residual download", error, cap);
     }
     if ((e = hipStreamSynchronize(st)) != hipSuccess)
@@ -962,9 +1103,12 @@ extern "C" void tpgp_hip_interval_vector_free(tpgp_hip_interval_vector *v) {
     free(v->keys);
     free(v->lower);
     free(v->upper);
-    v->keys = nullptr;
-    v->lower = v->upper = nullptr;
-    v->len = 0;
+    free(v->partner);
+    for (int k = 0; k < 2; ++k) {
+        free(v->square[k]);
+        free(v->product[k]);
+    }
+    *v = tpgp_hip_interval_vector{};
 }
 
 extern "C" int tpgp_hip_residual_vectors(tpgp_hip_context *ctx, const uint64_t *a_keys,
@@ -976,7 +1120,7 @@ extern "C" int tpgp_hip_residual_vectors(tpgp_hip_context *ctx, const uint64_t *
                                          char *error, size_t cap) {
     msg(error, cap, "");
     if (out)
-        for (int d = 0; d < 2; ++d) out[d] = tpgp_hip_interval_vector{0, nullptr, nullptr, nullptr};
+        for (int d = 0; d < 2; ++d) out[d] = tpgp_hip_interval_vector{};
     if (!ctx || !out || !n_spins || n_spins > 64 || n_words != (n_spins + 31) / 32 ||
         (a_len && (!a_keys || !a_values)) || (b_len && (!b_keys || !b_values)) ||
         (n_terms && !terms) || !std::isfinite(dt) || dt <= 0 ||

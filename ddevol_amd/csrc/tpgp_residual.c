@@ -11,10 +11,13 @@
  *   (D1, D2) (residual_rev.c:116-174) or directly over the sparse supports.
  * D1 and D2 are built on the GPU (tpgp_hip_residual_vectors): four generator applications and
  * two combinations, every entry a chain of correctly rounded 256-bit products and sums, equal to
- * the reference's MPFR entries bit for bit and in its first-insertion order.  The rest is serial
- * in the reference (sums over D1 and D2 in entry order, each addition rounded) and runs here in
- * MPFR at 256 bits with the reference's rounding directions and order of operations, so the
- * rounded-up binary64 result is the reference's bit for bit.
+ * the reference's MPFR entries bit for bit and in its first-insertion order; the device also
+ * pairs the two vectors' keys and forms every entry's outward-rounded Gram terms (squares and
+ * products of its ends).  What is serial in the reference stays here: the sums over D1 and D2 in
+ * entry order, each addition rounded, in the same correctly rounded 256-bit arithmetic
+ * (tpgp_f256.h, MPFR's results at 256 bits), and the O(1) norm and integral arithmetic in MPFR,
+ * with the reference's rounding directions and order of operations, so the rounded-up binary64
+ * result is the reference's bit for bit.
  */
 #include "../../include/tpgp_engine.h"
 #include "../../include/tpgp_hip.h"
@@ -28,220 +31,113 @@
 #ifdef TPGP_HAVE_MPFR
 #include <mpfr.h>
 
+#include "tpgp_f256.h"
+
 enum { kBits = 256 };
-
-typedef struct {
-    mpfr_t lo, hi;
-} interval;
-
-/* A sparse interval vector: entries in first-insertion order, an open-addressing index. */
-typedef struct {
-    size_t words, len, cap, slots;
-    uint64_t *keys;   /* [cap * words] */
-    interval *vals;   /* [cap] */
-    size_t *index;    /* [slots]: entry + 1, 0 = empty */
-} ivec;
+#define NO_PARTNER 0xffffffffu
 
 static int arithmetic_ok(void) {
     return FLT_RADIX == 2 && DBL_MANT_DIG == 53 && sizeof(size_t) <= 8 &&
            mpfr_get_emin() <= -16384 && mpfr_get_emax() >= 16384;
 }
 
-static uint64_t mix(uint64_t x) { /* splitmix64 finaliser */
-    x ^= x >> 31;
-    x *= 0x7fb5d329728ea185ull;
-    x ^= x >> 27;
-    x *= 0x81dadef4bc2dd44dull;
-    return x ^ (x >> 33);
-}
-static uint64_t key_hash(const uint64_t *k, size_t words) {
-    uint64_t h = 0x243f6a8885a308d3ull;
-    for (size_t w = 0; w < words; ++w) h = mix(h + k[w] + w);
-    return h;
-}
-
-static void ivec_free(ivec *v) {
-    for (size_t i = 0; i < v->len; ++i) {
-        mpfr_clear(v->vals[i].lo);
-        mpfr_clear(v->vals[i].hi);
-    }
-    free(v->keys);
-    free(v->vals);
-    free(v->index);
-    memset(v, 0, sizeof(*v));
-}
-
-static int ivec_reindex(ivec *v, size_t slots) {
-    size_t *ix = calloc(slots, sizeof(*ix));
-    if (!ix) return 0;
-    for (size_t i = 0; i < v->len; ++i) {
-        size_t s = key_hash(v->keys + i * v->words, v->words) & (slots - 1);
-        while (ix[s]) s = (s + 1) & (slots - 1);
-        ix[s] = i + 1;
-    }
-    free(v->index);
-    v->index = ix;
-    v->slots = slots;
-    return 1;
-}
-
-static interval *ivec_find(const ivec *v, const uint64_t *k) {
-    if (!v->slots) return NULL;
-    size_t s = key_hash(k, v->words) & (v->slots - 1);
-    while (v->index[s]) {
-        const size_t i = v->index[s] - 1;
-        if (!memcmp(v->keys + i * v->words, k, v->words * sizeof(*k))) return &v->vals[i];
-        s = (s + 1) & (v->slots - 1);
-    }
-    return NULL;
-}
-
-/* y += s x with outward rounding (residual_rev.c:55). */
-static void add_scaled(interval *y, const interval *x, const mpfr_t s, mpfr_t t) {
-    const int nonneg = mpfr_sgn(s) >= 0;
-    mpfr_mul(t, nonneg ? x->lo : x->hi, s, MPFR_RNDD);
-    mpfr_add(y->lo, y->lo, t, MPFR_RNDD);
-    mpfr_mul(t, nonneg ? x->hi : x->lo, s, MPFR_RNDU);
-    mpfr_add(y->hi, y->hi, t, MPFR_RNDU);
-}
-
 /* The engine's process-wide device context (created on first use) and its error note. */
 tpgp_hip_context *tpgp_engine_default_context(void);
 void tpgp_engine_note(const char *where, const char *what);
 
-/* A vector built on the device, in its entry order, with an index for lookups.  The device
- * endpoints are exact 256-bit values, so the conversion to MPFR at 256 bits is exact. */
-static int from_device(ivec *v, const tpgp_hip_interval_vector *d, size_t words) {
-    v->words = words;
-    v->len = 0;
-    v->cap = d->len ? d->len : 1;
-    v->keys = malloc(v->cap * words * sizeof(*v->keys));
-    v->vals = malloc(v->cap * sizeof(*v->vals));
-    if (!v->keys || !v->vals) return 0;
-    if (d->len) memcpy(v->keys, d->keys, d->len * words * sizeof(*v->keys));
+/* tpgp_hip_f256 is tpgp_f256.h's f256. */
+static f256 at(const tpgp_hip_f256 *v, size_t i) {
+    f256 x;
+    memcpy(&x, &v[i], sizeof(x));
+    return x;
+}
+
+/* x exactly at 256 bits (the device values have 256-bit significands). */
+static void to_mpfr(mpfr_t out, f256 x) {
+    if (f256_is_zero(&x)) {
+        mpfr_set_zero(out, x.neg ? -1 : 1);
+        return;
+    }
     mpz_t z;
     mpz_init(z);
-    for (size_t i = 0; i < d->len; ++i) {
-        const tpgp_hip_f256 *ends[2] = {&d->lower[i], &d->upper[i]};
-        mpfr_ptr out[2] = {v->vals[i].lo, v->vals[i].hi};
-        mpfr_init2(v->vals[i].lo, kBits);
-        mpfr_init2(v->vals[i].hi, kBits);
-        v->len = i + 1;
-        for (int k = 0; k < 2; ++k) {
-            const tpgp_hip_f256 *x = ends[k];
-            if (!x->m[3]) {
-                mpfr_set_zero(out[k], x->neg ? -1 : 1);
-                continue;
-            }
-            mpz_import(z, 4, -1, sizeof(uint64_t), 0, 0, x->m);
-            mpfr_set_z_2exp(out[k], z, (mpfr_exp_t)x->e - kBits, MPFR_RNDN);
-            if (x->neg) mpfr_neg(out[k], out[k], MPFR_RNDN);
-        }
-    }
+    mpz_import(z, 4, -1, sizeof(uint64_t), 0, 0, x.m);
+    mpfr_set_z_2exp(out, z, (mpfr_exp_t)x.e - kBits, MPFR_RNDN);
+    if (x.neg) mpfr_neg(out, out, MPFR_RNDN);
     mpz_clear(z);
-    size_t slots = 128;
-    while (slots < 2 * (d->len + 1)) slots *= 2;
-    return ivec_reindex(v, slots);
+}
+
+/* y += s x with outward rounding (residual_rev.c:55): f256 operations are MPFR's at 256 bits. */
+static void add_scaled(f256 y[2], f256 lo, f256 hi, f256 s) {
+    const int nonneg = f256_sgn(&s) >= 0;
+    y[0] = f256_add(y[0], f256_mul(nonneg ? lo : hi, s, F256_DOWN), F256_DOWN);
+    y[1] = f256_add(y[1], f256_mul(nonneg ? hi : lo, s, F256_UP), F256_UP);
 }
 
 /* Upper bound of |u D1 + v D2| over the union of supports (D1's entries, then D2's that D1
- * lacks), each entry's enclosure squared upward (residual_rev.c:116-121). */
-static void direct_norm(mpfr_t out, const ivec *d1, const ivec *d2, double u, double v) {
-    mpfr_t su, sv, a, c, t;
-    mpfr_inits2(kBits, su, sv, a, c, t, (mpfr_ptr)0);
-    mpfr_set_d(su, u, MPFR_RNDN);
-    mpfr_set_d(sv, v, MPFR_RNDN);
-    mpfr_set_zero(out, 1);
-    interval x;
-    mpfr_init2(x.lo, kBits);
-    mpfr_init2(x.hi, kBits);
+ * lacks), each entry's enclosure squared upward and summed upward in that order
+ * (residual_rev.c:116-121); u and v are binary64 (at most 31 significant bits). */
+static void direct_norm(mpfr_t out, const tpgp_hip_interval_vector *d1,
+                        const tpgp_hip_interval_vector *d2, double u, double v) {
+    const f256 su = f256_from_double(u), sv = f256_from_double(v);
+    f256 sum = f256_zero(0);
     for (int pass = 0; pass < 2; ++pass) {
-        const ivec *src = pass ? d2 : d1;
+        const tpgp_hip_interval_vector *src = pass ? d2 : d1;
         for (size_t i = 0; i < src->len; ++i) {
-            const uint64_t *k = src->keys + i * src->words;
-            const interval *p1 = ivec_find(d1, k), *p2 = ivec_find(d2, k);
-            if (pass && p1) continue;
-            mpfr_set_zero(x.lo, 1);
-            mpfr_set_zero(x.hi, 1);
-            if (p1) add_scaled(&x, p1, su, t);
-            if (p2) add_scaled(&x, p2, sv, t);
-            mpfr_abs(a, x.lo, MPFR_RNDU);
-            mpfr_abs(c, x.hi, MPFR_RNDU);
-            if (mpfr_greater_p(c, a)) mpfr_set(a, c, MPFR_RNDU);
-            mpfr_sqr(a, a, MPFR_RNDU);
-            mpfr_add(out, out, a, MPFR_RNDU);
+            if (pass && src->partner[i] != NO_PARTNER) continue;  /* counted with D1's entry */
+            f256 x[2] = {f256_zero(0), f256_zero(0)};
+            if (!pass) {
+                add_scaled(x, at(d1->lower, i), at(d1->upper, i), su);
+                const uint32_t j = d1->partner[i];
+                if (j != NO_PARTNER) add_scaled(x, at(d2->lower, j), at(d2->upper, j), sv);
+            } else {
+                add_scaled(x, at(d2->lower, i), at(d2->upper, i), sv);
+            }
+            f256 a = f256_abs(x[0]), c = f256_abs(x[1]);
+            if (f256_cmp(&c, &a) > 0) a = c;
+            sum = f256_add(sum, f256_mul_full(a, a, F256_UP), F256_UP);
         }
     }
+    to_mpfr(out, sum);
     mpfr_sqrt(out, out, MPFR_RNDU);
-    mpfr_clear(x.lo);
-    mpfr_clear(x.hi);
-    mpfr_clears(su, sv, a, c, t, (mpfr_ptr)0);
 }
 
-/* Enclosures of <D1,D1>, <D1,D2> (signed) and <D2,D2> (residual_rev.c:122-153). */
+/* Enclosures of <D1,D1>, <D1,D2> (signed) and <D2,D2> (residual_rev.c:122-153): the device's
+ * per-entry terms summed in entry order, lower ends rounded down and upper ends up. */
 typedef struct {
-    interval aa, ab, bb;
+    mpfr_t aa[2], ab[2], bb[2];
 } gram;
 
-static void iv_init(interval *x) {
-    mpfr_init2(x->lo, kBits);
-    mpfr_init2(x->hi, kBits);
-    mpfr_set_zero(x->lo, 1);
-    mpfr_set_zero(x->hi, 1);
-}
-static void iv_clear(interval *x) {
-    mpfr_clear(x->lo);
-    mpfr_clear(x->hi);
-}
-
-static void add_square(interval *sum, const interval *x, mpfr_t p, mpfr_t q) {
-    mpfr_sqr(p, x->lo, MPFR_RNDU);
-    mpfr_sqr(q, x->hi, MPFR_RNDU);
-    if (mpfr_greater_p(q, p)) mpfr_set(p, q, MPFR_RNDU);
-    mpfr_add(sum->hi, sum->hi, p, MPFR_RNDU);
-    if (mpfr_sgn(x->lo) <= 0 && mpfr_sgn(x->hi) >= 0) {
-        mpfr_set_zero(p, 1);
-    } else {
-        mpfr_sqr(p, x->lo, MPFR_RNDD);
-        mpfr_sqr(q, x->hi, MPFR_RNDD);
-        if (mpfr_less_p(q, p)) mpfr_set(p, q, MPFR_RNDD);
-    }
-    mpfr_add(sum->lo, sum->lo, p, MPFR_RNDD);
-}
-
-static void add_product(interval *sum, const interval *x, const interval *y, mpfr_t p, mpfr_t q,
-                        mpfr_t r) {
-    mpfr_mul(p, x->lo, y->lo, MPFR_RNDD);
-    mpfr_mul(q, x->lo, y->lo, MPFR_RNDU);
-    mpfr_srcptr xs[2] = {x->lo, x->hi}, ys[2] = {y->lo, y->hi};
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) {
-            if (!i && !j) continue;
-            mpfr_mul(r, xs[i], ys[j], MPFR_RNDD);
-            if (mpfr_less_p(r, p)) mpfr_set(p, r, MPFR_RNDD);
-            mpfr_mul(r, xs[i], ys[j], MPFR_RNDU);
-            if (mpfr_greater_p(r, q)) mpfr_set(q, r, MPFR_RNDU);
-        }
-    mpfr_add(sum->lo, sum->lo, p, MPFR_RNDD);
-    mpfr_add(sum->hi, sum->hi, q, MPFR_RNDU);
-}
-
-static void gram_of(gram *g, const ivec *d1, const ivec *d2) {
-    mpfr_t p, q, r;
-    mpfr_inits2(kBits, p, q, r, (mpfr_ptr)0);
+static void gram_of(gram *g, const tpgp_hip_interval_vector *d1,
+                    const tpgp_hip_interval_vector *d2) {
+    f256 aa[2] = {f256_zero(0), f256_zero(0)}, ab[2] = {f256_zero(0), f256_zero(0)};
+    f256 bb[2] = {f256_zero(0), f256_zero(0)};
     for (size_t i = 0; i < d1->len; ++i) {
-        const interval *y = ivec_find(d2, d1->keys + i * d1->words);
-        add_square(&g->aa, &d1->vals[i], p, q);
-        if (y) add_product(&g->ab, &d1->vals[i], y, p, q, r);
+        aa[1] = f256_add(aa[1], at(d1->square[1], i), F256_UP);
+        aa[0] = f256_add(aa[0], at(d1->square[0], i), F256_DOWN);
+        if (d1->partner[i] != NO_PARTNER) {
+            ab[0] = f256_add(ab[0], at(d1->product[0], i), F256_DOWN);
+            ab[1] = f256_add(ab[1], at(d1->product[1], i), F256_UP);
+        }
     }
-    for (size_t i = 0; i < d2->len; ++i) add_square(&g->bb, &d2->vals[i], p, q);
-    mpfr_clears(p, q, r, (mpfr_ptr)0);
+    for (size_t i = 0; i < d2->len; ++i) {
+        bb[1] = f256_add(bb[1], at(d2->square[1], i), F256_UP);
+        bb[0] = f256_add(bb[0], at(d2->square[0], i), F256_DOWN);
+    }
+    for (int k = 0; k < 2; ++k) {
+        mpfr_inits2(kBits, g->aa[k], g->ab[k], g->bb[k], (mpfr_ptr)0);
+        to_mpfr(g->aa[k], aa[k]);
+        to_mpfr(g->ab[k], ab[k]);
+        to_mpfr(g->bb[k], bb[k]);
+    }
+}
+static void gram_clear(gram *g) {
+    for (int k = 0; k < 2; ++k) mpfr_clears(g->aa[k], g->ab[k], g->bb[k], (mpfr_ptr)0);
 }
 
 /* |u D1 + v D2| from the Gram enclosure, upper ends only; when more than half the working bits
  * could cancel, the direct norm instead (residual_rev.c:154-174). */
-static void gram_norm(mpfr_t out, const gram *g, const ivec *d1, const ivec *d2, double u,
-                      double v) {
+static void gram_norm(mpfr_t out, const gram *g, const tpgp_hip_interval_vector *d1,
+                      const tpgp_hip_interval_vector *d2, double u, double v) {
     if (u == 0 && v == 0) {
         mpfr_set_zero(out, 1);
         return;
@@ -254,11 +150,11 @@ static void gram_norm(mpfr_t out, const gram *g, const ivec *d1, const ivec *d2,
     mpfr_mul_2ui(uv, uv, 1, MPFR_RNDN);
     mpfr_sqr(uu, uu, MPFR_RNDN);
     mpfr_sqr(vv, vv, MPFR_RNDN);
-    mpfr_mul(out, uu, g->aa.hi, MPFR_RNDU);
-    mpfr_mul(z, vv, g->bb.hi, MPFR_RNDU);
+    mpfr_mul(out, uu, g->aa[1], MPFR_RNDU);
+    mpfr_mul(z, vv, g->bb[1], MPFR_RNDU);
     mpfr_add(out, out, z, MPFR_RNDU);
     mpfr_set(scale, out, MPFR_RNDU);
-    mpfr_mul(z, uv, g->ab.hi, MPFR_RNDU);
+    mpfr_mul(z, uv, g->ab[1], MPFR_RNDU);
     mpfr_add(out, out, z, MPFR_RNDU);
     mpfr_div_2ui(floor_, scale, kBits / 2, MPFR_RNDU);
     if (mpfr_sgn(out) < 0 || mpfr_lessequal_p(out, floor_))
@@ -271,8 +167,9 @@ static void gram_norm(mpfr_t out, const gram *g, const ivec *d1, const ivec *d2,
 /* The subdivided integral of the norm along the cubic Bezier control points c (residual_rev.c:
  * 175-180): at the leaves, the four control norms each weighted 1/(12 2^depth); otherwise the
  * de Casteljau halves, left then right. */
-static void integrate(mpfr_t sum, const ivec *d1, const ivec *d2, const gram *g,
-                      const double c[4][2], unsigned depth, unsigned top) {
+static void integrate(mpfr_t sum, const tpgp_hip_interval_vector *d1,
+                      const tpgp_hip_interval_vector *d2, const gram *g, const double c[4][2],
+                      unsigned depth, unsigned top) {
     if (!depth) {
         mpfr_t z;
         mpfr_init2(z, kBits);
@@ -350,35 +247,24 @@ static tpgp_status residual(const tpgp_map *a, const tpgp_map *b, const tpgp_ter
         return hs == TPGP_HIP_ENOMEM ? TPGP_ENOMEM : hs == TPGP_HIP_EINVAL ? TPGP_EINVAL
                                                                           : TPGP_EBACKEND;
     }
-    ivec v[2];
-    memset(v, 0, sizeof(v));
-    int ok = from_device(&v[0], &dv[0], a->n_words) && from_device(&v[1], &dv[1], a->n_words);
+    static const double c[4][2] = {{0, 0}, {1, 0}, {0, 1}, {0, 0}};
+    tpgp_status status = TPGP_OK;
+    mpfr_t sum;
+    mpfr_init2(sum, kBits);
+    mpfr_set_zero(sum, 1);
+    const int use_gram = with_gram && depth > 0;
+    gram g;
+    if (use_gram) gram_of(&g, &dv[0], &dv[1]);
+    integrate(sum, &dv[0], &dv[1], use_gram ? &g : NULL, c, depth, depth);
+    const double up = mpfr_get_d(sum, MPFR_RNDU);
+    if (use_gram) gram_clear(&g);
+    mpfr_clear(sum);
     tpgp_hip_interval_vector_free(&dv[0]);
     tpgp_hip_interval_vector_free(&dv[1]);
-    tpgp_status status = ok ? TPGP_OK : TPGP_ENOMEM;
-    if (ok) {
-        static const double c[4][2] = {{0, 0}, {1, 0}, {0, 1}, {0, 0}};
-        mpfr_t sum;
-        mpfr_init2(sum, kBits);
-        mpfr_set_zero(sum, 1);
-        gram g;
-        iv_init(&g.aa);
-        iv_init(&g.ab);
-        iv_init(&g.bb);
-        const int use_gram = with_gram && depth > 0;
-        if (use_gram) gram_of(&g, &v[0], &v[1]);
-        integrate(sum, &v[0], &v[1], use_gram ? &g : NULL, c, depth, depth);
-        const double up = mpfr_get_d(sum, MPFR_RNDU);
-        iv_clear(&g.aa);
-        iv_clear(&g.ab);
-        iv_clear(&g.bb);
-        mpfr_clear(sum);
-        if (!isfinite(up))
-            status = TPGP_ENUMERIC;
-        else
-            *upper = up;
-    }
-    for (int i = 0; i < 2; ++i) ivec_free(&v[i]);
+    if (!isfinite(up))
+        status = TPGP_ENUMERIC;
+    else
+        *upper = up;
     return status;
 }
 

@@ -123,7 +123,9 @@ class HipF256(C.Structure):  # tpgp_hip_f256: (-1)^neg (m / 2^256) 2^e, m[3] mos
 
 class HipIntervalVector(C.Structure):  # tpgp_hip_interval_vector
     _fields_ = [("len", C.c_size_t), ("keys", C.POINTER(C.c_uint64)),
-                ("lower", C.POINTER(HipF256)), ("upper", C.POINTER(HipF256))]
+                ("lower", C.POINTER(HipF256)), ("upper", C.POINTER(HipF256)),
+                ("partner", C.POINTER(C.c_uint32)), ("square", C.POINTER(HipF256) * 2),
+                ("product", C.POINTER(HipF256) * 2)]
 
 
 class Checkpoint(C.Structure):  # checkpoint_rev.h:7-23 layout

@@ -185,6 +185,12 @@ int tpgp_hip_liouvillian(tpgp_hip_context *context, const uint64_t *keys, const 
  * every term, as tpgp_residual_bound_reference).  Every entry's ends are the reference's bits:
  * the same correctly rounded 256-bit products and sums (down for lower ends, up for upper ends)
  * in the same order, and the entries come in the reference's first-insertion order.
+ * With each vector come, per entry, what the reference's Gram enclosure adds up in entry order
+ * (residual_rev.c:122-153): the index of the same key in the other vector (partner), the
+ * outward-rounded square terms of the entry (the max of the upward squares of its ends, and the
+ * min of the downward squares or +0 when the interval holds 0), and for D1 the outward-rounded
+ * product terms with its partner in D2 (min and max of the four directed end products).  Only
+ * the serial sums remain for the caller.
  * a and b are maps (sorted keys, finite binary64 values) of one spin count; terms as in
  * tpgp_hip_liouvillian; dt finite and positive.  On success out[0] = D1 and out[1] = D2, each
  * with arrays allocated by the library (free with tpgp_hip_interval_vector_free).  EINVAL: bad
@@ -197,8 +203,12 @@ typedef struct {
 } tpgp_hip_f256;
 typedef struct {
     size_t len;
-    uint64_t *keys; /* len * n_words, in first-insertion order */
-    tpgp_hip_f256 *lower, *upper;
+    uint64_t *keys;               /* len * n_words, in first-insertion order */
+    tpgp_hip_f256 *lower, *upper; /* the entries' intervals */
+    uint32_t *partner;            /* the entry of the same key in the other vector, or UINT32_MAX */
+    tpgp_hip_f256 *square[2];     /* Gram terms of the entry's square: [0] lower, [1] upper */
+    tpgp_hip_f256 *product[2];    /* D1: terms of the product with its partner (where it has
+                                     one); NULL in D2 */
 } tpgp_hip_interval_vector;
 int tpgp_hip_residual_vectors(tpgp_hip_context *context, const uint64_t *a_keys,
                               const double *a_values, size_t a_len, const uint64_t *b_keys,

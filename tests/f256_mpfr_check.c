@@ -184,6 +184,42 @@ int main(int argc, char **argv) {
         if (!check_mul(x, s, mx, my, want)) return 1;
         ++checked;
     }
+    /* full products and squares (the Gram terms), and comparisons */
+    for (long k = 0; k < cases / 2; ++k) {
+        f256 x = random_value((int32_t)(next() % 2001) - 1000), y;
+        y = next() % 4 == 0 ? x : random_value((int32_t)(next() % 2001) - 1000);
+        if (next() % 32 == 0) y = f256_zero((uint32_t)(next() & 1));
+        if (next() % 32 == 0) x = f256_zero((uint32_t)(next() & 1));
+        to_mpfr(mx, &x);
+        to_mpfr(my, &y);
+        for (int dir = 0; dir < 2; ++dir) {
+            const f256 got = f256_mul_full(x, y, dir);
+            mpfr_mul(want, mx, my, dir == F256_DOWN ? MPFR_RNDD : MPFR_RNDU);
+            if (!same(&got, want)) {
+                mpfr_printf("mul_full mismatch dir=%d\n x=%.80Rb\n y=%.80Rb\n", dir, mx, my);
+                return 1;
+            }
+            const f256 sq = f256_mul_full(x, x, dir);
+            mpfr_sqr(want, mx, dir == F256_DOWN ? MPFR_RNDD : MPFR_RNDU);
+            if (!same(&sq, want)) {
+                mpfr_printf("square mismatch dir=%d\n x=%.80Rb\n", dir, mx);
+                return 1;
+            }
+        }
+        /* comparisons, including near-equal values and signed zeros */
+        f256 z = y;
+        if (next() % 3 == 0) {
+            z = x;
+            if (next() & 1) z.m[next() % 4] ^= 1ull << (next() % 64), z.m[3] |= 1ull << 63;
+        }
+        to_mpfr(my, &z);
+        const int c = f256_cmp(&x, &z), w = mpfr_cmp(mx, my);
+        if ((c > 0) != (w > 0) || (c < 0) != (w < 0) || f256_sgn(&x) != mpfr_sgn(mx)) {
+            printf("cmp mismatch at case %ld\n", k);
+            return 1;
+        }
+        checked += 5;
+    }
     /* chains as the generator runs them: z = z + x*s, both directions, from +0 */
     for (long k = 0; k < cases / 10; ++k) {
         f256 lo = f256_zero(0), hi = f256_zero(0);

@@ -184,7 +184,13 @@ def device_vectors(a, b, terms, dt, incident):
 
             lo = [value(v.lower[i]) for i in range(v.len)]
             hi = [value(v.upper[i]) for i in range(v.len)]
-            res.append((keys.reshape(-1, W), lo, hi))
+            extra = {"partner": [v.partner[i] for i in range(v.len)],
+                     "square": [(value(v.square[0][i]), value(v.square[1][i]))
+                                for i in range(v.len)]}
+            if d == 0:
+                extra["product"] = [(value(v.product[0][i]), value(v.product[1][i]))
+                                    for i in range(v.len)]
+            res.append((keys.reshape(-1, W), lo, hi, extra))
             N.hip().tpgp_hip_interval_vector_free(C.byref(out[d]))
         return res
     finally:
@@ -234,7 +240,21 @@ def test_vectors_against_dense_generator():
           for i in range(dim)]
     for incident in (1, 0):
         got = device_vectors(a, b, terms, dt, incident)
-        for (keys, lo, hi), exact in zip(got, (d1, d2)):
+        # the key join and the Gram terms: on point intervals every term is the exact square or
+        # product of the entries
+        (k1, lo1, _, x1), (k2, lo2, _, x2) = got
+        pos2 = {int(k): j for j, k in enumerate(k2[:, 0])}
+        pos1 = {int(k): i for i, k in enumerate(k1[:, 0])}
+        for i, k in enumerate(k1[:, 0]):
+            j = pos2.get(int(k), 0xffffffff)
+            assert x1["partner"][i] == j
+            assert x1["square"][i] == (lo1[i] * lo1[i], lo1[i] * lo1[i])
+            if j != 0xffffffff:
+                assert x1["product"][i] == (lo1[i] * lo2[j], lo1[i] * lo2[j])
+        for j, k in enumerate(k2[:, 0]):
+            assert x2["partner"][j] == pos1.get(int(k), 0xffffffff)
+            assert x2["square"][j] == (lo2[j] * lo2[j], lo2[j] * lo2[j])
+        for (keys, lo, hi, _), exact in zip(got, (d1, d2)):
             assert len(set(int(k) for k in keys[:, 0])) == len(keys)  # unique entries
             seen = set()
             for k, l, h in zip(keys[:, 0], lo, hi):
