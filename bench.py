@@ -17,10 +17,12 @@ ledger.rotations counts the same rotations, engine/src/tpgp_rev.c:740).
 roofline: the dominant kernel is the leaf kernel kind (rotation write, count, splits, scans,
 partition histogram and scatter, small-map kernel) with the largest total time over the sampled
 launches of the timed region.  Algorithmic bytes per launch, for W key words (16 B per term for
-N <= 32): k_rot_write (8W + 8)(len_in + len_out) -- the input map read once and the output map
-written once; k_rot_count (8W + 8) len_in; k_small (8W + 8)(len_in + len_out) per rotation it
-runs.  Durations are measured live with HIP events recorded on the batch stream around the
-launches of a sample of the timed calls.
+N <= 32): k_rot_write (8W + 8)(len_in + len_out - 2 in_place) -- every input term of the classes
+the rotation rewrites read once and every output written once (an appending rotation leaves the
+classes it does not change in place: in_place terms neither read nor written); k_rot_count
+(8W + 8)(len_in - in_place); k_small (8W + 8)(len_in + len_out) per rotation it runs.
+Durations are measured live with HIP events recorded on the batch stream around the launches of
+a sample of the timed calls.
 
 cpu_baseline: the reference CPU engine (engine/src/tpgp_rev.c compiled from its own sources into
 oracle/_ref) -- or, where that build is absent, the oracle's C restatement -- runs the J1-J2
@@ -408,7 +410,7 @@ def timed_window(ev, led, steps, kinds, sample_every, progress=None):
 
     mask = sum(1 << k for k in kinds)
     prof = {N.PROFILE_NAMES[k]: [0.0, 0] for k in kinds}
-    w = {"s_updates": 0, "s_in_out": 0, "s_steps": 0, "call_steps": call_steps,
+    w = {"s_updates": 0, "s_in_out": 0, "s_in_place": 0, "s_steps": 0, "call_steps": call_steps,
          "sample_every": sample_every, "steps": steps}
     c0 = ev.counters()
     t0 = time.perf_counter()
@@ -430,6 +432,7 @@ def timed_window(ev, led, steps, kinds, sample_every, progress=None):
                 prof[N.PROFILE_NAMES[k]][1] += cnt
             w["s_updates"] += cb.updates - ca.updates
             w["s_in_out"] += cb.in_out - ca.in_out
+            w["s_in_place"] += cb.in_place - ca.in_place
             w["s_steps"] += acc
         if call == 0:
             w["terms_first"], w["fid_first"] = int(act[0]), float(fid[0])
@@ -443,6 +446,7 @@ def timed_window(ev, led, steps, kinds, sample_every, progress=None):
     w["prof"] = {k: tuple(v) for k, v in prof.items()}
     w["updates"] = c1.updates - c0.updates
     w["in_out"] = c1.in_out - c0.in_out
+    w["in_place"] = c1.in_place - c0.in_place
     w["rotations"] = c1.rotations - c0.rotations
     w["terms_last"], w["fid_last"] = int(act[-1]), float(fid[-1])
     w["sample"] = (f"HIP events on {w['s_steps']} of the {steps} timed steps (calls of "
@@ -505,8 +509,9 @@ def bench_single(args, n, terms, dt, thr, label, workload):
     ms_w, n_w = prof["k_rot_write"]
     roof = None
     if n_w and ms_w > 0:
-        # every input term read once, every output written once (sampled launches)
-        alg = term_bytes * w["s_in_out"] / n_w
+        # every input term of the rewritten classes read once, every output written once
+        # (sampled launches)
+        alg = term_bytes * (w["s_in_out"] - w["s_in_place"]) / n_w
         avg_s = ms_w * 1e-3 / n_w
         roof = {"bound": "hbm", "achieved": alg / avg_s / 1e9, "peak": 8000.0, "unit": "GB/s",
                 "frac": alg / avg_s / 8e12, "traffic": None, "kernel": "k_rot_write",
@@ -640,13 +645,16 @@ def main():
     print(json.dumps(bench_headline(args)), flush=True)
 
 
-def roofline_of(prof, term_bytes, s_updates, s_in_out, sample):
+def roofline_of(prof, term_bytes, s_updates, s_in_out, sample, s_in_place=0):
     """Roofline entry for the dominant kernel: the leaf kernel kind with the largest total time
     over the sampled launches, among those with an algorithmic byte model.  Algorithmic bytes
-    per launch (B = 8W + 8 bytes per term): k_rot_write B(len_in + len_out) -- every input term
-    read once, every output written once; k_rot_count B len_in; k_small B(len_in + len_out) per
-    rotation it runs (only when it ran every sampled rotation, so the counters are its own)."""
-    models = {"k_rot_write": term_bytes * s_in_out, "k_rot_count": term_bytes * s_updates}
+    per launch (B = 8W + 8 bytes per term; in_place: terms of the classes an appending rotation
+    left in place, 2 per term in s_in_place): k_rot_write B(len_in + len_out - 2 in_place) --
+    every input term of the rewritten classes read once, every output written once;
+    k_rot_count B(len_in - in_place); k_small B(len_in + len_out) per rotation it runs (only
+    when it ran every sampled rotation, so the counters are its own)."""
+    models = {"k_rot_write": term_bytes * (s_in_out - s_in_place),
+              "k_rot_count": term_bytes * (s_updates - s_in_place // 2)}
     if prof.get("k_rot_write", (0, 0))[1] == 0 and prof.get("k_small", (0, 0))[1]:
         models = {"k_small": term_bytes * s_in_out}
     leaf = {k: v for k, v in prof.items() if k not in ("rotation_pass", "partition_pass")}
@@ -876,7 +884,8 @@ def bench_headline(args):
     for name, (ms, cnt) in w["prof"].items():
         log(f"  {name:16s} {cnt:7d} launches {ms:9.1f} ms total "
             f"{ms / max(cnt, 1) * 1e3:8.1f} us avg")
-    roof = roofline_of(w["prof"], term_bytes, w["s_updates"], w["s_in_out"], w["sample"])
+    roof = roofline_of(w["prof"], term_bytes, w["s_updates"], w["s_in_out"], w["sample"],
+                       w["s_in_place"])
     roof["traffic"] = (args.traffic if args.traffic is not None
                        else MEASURED_TRAFFIC.get((roof["kernel"], done, args.steps)))
 
@@ -926,6 +935,7 @@ def bench_headline(args):
                    "timed_steps": f"{done + 1}..{done + args.steps}",
                    "terms_at_start": start_terms, "terms_at_end": w["terms_last"],
                    "rotations": rotations, "branch_updates": updates, "terms_in_out": in_out,
+                   "terms_left_in_place": w["in_place"] // 2,
                    "fid_at_end": w["fid_last"],
                    # a loaded state carries no ledger of its earlier steps
                    "fid_allowance_at_end": None if args.load_state else

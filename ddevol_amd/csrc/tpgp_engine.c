@@ -492,6 +492,7 @@ typedef struct {
     size_t log_cap;
     double *scratch;
     size_t scratch_cap;
+    int no_room; /* a growth for appending rotations was refused: not asked for again */
 } session;
 
 static void session_close(session *S) {
@@ -551,6 +552,20 @@ static tpgp_status session_grow(session *S, size_t need) {
     }
     S->cap = cap;
     return TPGP_OK;
+}
+/* Room for appending rotations (best effort): the library places the classes a rotation
+ * rewrites after the slot's used extent and leaves the others in place, which needs free space
+ * of about half the map per rotation between partitions.  A slot below three times the map
+ * grows to four times it; if the device refuses, the session keeps its slot (rotations then copy
+ * every class) and does not ask again. */
+static void session_room(session *S, size_t len) {
+    char err[512];
+    if (S->no_room || len > SIZE_MAX / 5 || 3 * len + 4096 <= S->cap) return;
+    const size_t want = 4 * len + 4096;
+    if (tpgp_hip_batch_reserve(S->batch, 0, want, err, sizeof(err)) == TPGP_HIP_OK)
+        S->cap = want;
+    else
+        S->no_room = 1;
 }
 /* Logs of dropped values and radii: device and host buffers hold `need` entries and more. */
 static tpgp_status grow_logs(tpgp_hip_batch *batch, tpgp_hip_log_entry **log, size_t *log_cap,
@@ -646,7 +661,8 @@ static tpgp_status enqueue_plans(session *S, const rot_plan *plans, size_t n, in
 static tpgp_status sync_fold(session *S, const rot_plan *plans, size_t n, size_t enq, int exact,
                              const tpgp_options *o, tpgp_ledger *trial, double *readout,
                              uint64_t *len, int *capacity_needed, size_t *need,
-                             uint64_t *updates, uint64_t *in_out, uint64_t *executed) {
+                             uint64_t *updates, uint64_t *in_out, uint64_t *in_place,
+                             uint64_t *executed) {
     char err[512];
     *capacity_needed = 0;
     size_t nrec = 0, nlog = 0, nrd = 0;
@@ -679,7 +695,7 @@ static tpgp_status sync_fold(session *S, const rot_plan *plans, size_t n, size_t
         if (!S->scratch) return TPGP_ENOMEM;
     }
     size_t r_at = 0, e_at = 0;
-    uint64_t up = 0, io = 0, ex = 0;
+    uint64_t up = 0, io = 0, kp = 0, ex = 0;
     for (size_t k = 0; k < n; k++) {
         const rot_plan *p = plans + k;
         if (!ledger_ok(trial)) return TPGP_EINVAL; /* tpgp_rev.c:543-547 */
@@ -701,10 +717,12 @@ static tpgp_status sync_fold(session *S, const rot_plan *plans, size_t n, size_t
         }
         up += r->len_in;
         io += r->len_in + r->len_out;
+        kp += 2 * r->in_place;
         ex++;
     }
     if (updates) *updates += up;
     if (in_out) *in_out += io;
+    if (in_place) *in_place += kp;
     if (executed) *executed += ex;
     return TPGP_OK;
 }
@@ -717,7 +735,7 @@ static tpgp_status run_plans(session *S, const rot_plan *plans, size_t n, int ex
     tpgp_status s = enqueue_plans(S, plans, n, exact, o, seq0, &enq);
     if (s) return s;
     return sync_fold(S, plans, n, enq, exact, o, trial, NULL, NULL, capacity_needed, need, NULL,
-                     NULL, NULL);
+                     NULL, NULL, NULL);
 }
 
 /* Transactional sequence of rotations on a host map (tpgp_rotate / tpgp_strang_step). */
@@ -909,20 +927,21 @@ tpgp_status tpgp_evolution_run(tpgp_evolution *ev, size_t steps, tpgp_ledger *l,
         tpgp_ledger trial = *l;
         int grow = 0;
         size_t need = 0;
-        uint64_t len = 0, up = 0, io = 0, ex = 0;
+        uint64_t len = 0, up = 0, io = 0, kp = 0, ex = 0;
         size_t read_len = 0; /* length of the state read out (the committed state) */
         tpgp_hip_batch_info(S->batch, 0, &read_len, NULL);
         if (more && read_len + read_len / 2 + 4096 > S->cap &&
             (s = session_grow(S, 2 * read_len + 4096))) /* room ahead of need */
             return s;
+        if (more) session_room(S, read_len);
         for (;;) {
             for (size_t k = 0; k < 2 * 64; k++) rd[k] = NAN; /* filled only by a good sync */
             size_t enq = 0;
             if ((s = enqueue_step(ev, more, &enq))) break;
             trial = *l;
-            up = io = ex = 0;
+            up = io = kp = ex = 0;
             s = sync_fold(S, ev->plans, more ? ev->nplans : 0, enq, exact, o, &trial, rd, &len,
-                          &grow, &need, &up, &io, &ex);
+                          &grow, &need, &up, &io, &kp, &ex);
             if (!s && grow) {
                 tpgp_hip_batch_rollback(S->batch, err, sizeof(err));
                 if ((s = session_regrow(S, grow, need))) break;
@@ -961,6 +980,7 @@ tpgp_status tpgp_evolution_run(tpgp_evolution *ev, size_t steps, tpgp_ledger *l,
         ev->c.rotations += ex;
         ev->c.updates += up;
         ev->c.in_out += io;
+        ev->c.in_place += kp;
         if (accepted) *accepted = step + 1;
     }
 }
@@ -1275,7 +1295,7 @@ tpgp_status tpgp_fleet_admit(tpgp_fleet *f, size_t m, const tpgp_fid_job *job) {
 /* Fold one slot's records of this step (plans in order, as sync_fold does for one map). */
 static tpgp_status fleet_fold(tpgp_fleet *f, size_t m, const tpgp_hip_log_entry *log,
                               size_t nlog, tpgp_ledger *trial, int *grow, size_t *need,
-                              uint64_t *up, uint64_t *io, uint64_t *ex) {
+                              uint64_t *up, uint64_t *io, uint64_t *kp, uint64_t *ex) {
     fleet_slot *q = &f->slot[m];
     const int exact = q->eo.exact_ledger;
     size_t r_at = 0, e_at = 0;
@@ -1298,6 +1318,7 @@ static tpgp_status fleet_fold(tpgp_fleet *f, size_t m, const tpgp_hip_log_entry 
         if (fs) return (tpgp_status)fs;
         *up += r->len_in;
         *io += r->len_in + r->len_out;
+        *kp += 2 * r->in_place;
         (*ex)++;
     }
     return TPGP_OK;
@@ -1620,9 +1641,9 @@ tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
             tpgp_ledger trial = q->ledger;
             int grow = 0;
             size_t need = 0;
-            uint64_t up = 0, io = 0, ex = 0;
+            uint64_t up = 0, io = 0, kp = 0, ex = 0;
             tpgp_status fs = fleet_fold(f, m, f->log + l0, l_at - l0, &trial, &grow, &need, &up,
-                                        &io, &ex);
+                                        &io, &kp, &ex);
             /* the sample of the committed state stands whatever becomes of this step (a slot
              * refused its growth below fails with it as its last sample) */
             q->fid[q->done] = mid;
@@ -1649,6 +1670,7 @@ tpgp_status tpgp_fleet_step(tpgp_fleet *f, size_t *running) {
             f->c.rotations += ex;
             f->c.updates += up;
             f->c.in_out += io;
+            f->c.in_place += kp;
         }
         if (tpgp_hip_batch_commit(f->batch, mask, err, sizeof(err))) {
             note("commit", err);

@@ -19,6 +19,12 @@
 // two raw branches (the key's own a-slot and its partner's d-slot), so the merged value is one
 // IEEE addition whose result does not depend on order: outputs are bit-identical to the
 // reference's stable merge sort.
+//
+// Class regions need not be adjacent in the slot.  A rotation whose commuting classes pass
+// unchanged may *append*: it leaves those classes where they are and writes the classes it
+// rewrites after the slot's used extent, in the same buffer, so that only the anticommuting
+// pairs are read and written.  Partitions, uploads and the small-map path write contiguous maps.
+// Every reader goes through the class region table (logical start, physical start, extent).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -52,6 +58,15 @@ constexpr uint32_t kSumBlocks = 512;     // blocks of the sum-of-squares reducti
 // PassMap.flags bit: the map's commuting classes cannot lose a key to this rotation's
 // truncation (every coefficient there already exceeds the threshold), so they pass unchanged
 constexpr uint32_t kPassCommutingKept = 16u;
+// PassMap.flags bit (with kPassCommutingKept, in_buf == out_buf): the rotation appends -- its
+// unchanged classes stay in place, the classes it rewrites go after the buffer's used extent
+constexpr uint32_t kPassAppend = 32u;
+// Class region table, kRegStride words per (map, buffer):
+//   [0, 17)   logical starts: class c is elements reg[c] .. reg[c+1]-1 of the map in key order;
+//   [17, 33)  physical starts: element reg[c] + i lives at slot index reg[kRegPhys + c] + i;
+//   [33]      used extent: slot indices from reg[kRegTop] on are free.
+// A contiguous map has reg[kRegPhys + c] == reg[c] and reg[kRegTop] == reg[16].
+constexpr int kRegStride = 34, kRegPhys = 17, kRegTop = 33;
 constexpr int kPartRounds = 8;     // partition: 8 rounds of 256 elements (the default)
 constexpr int kPartTile = kBlock * kPartRounds;
 constexpr int kPartRoundsMin = 4;  // smallest tunable partition tile: 4 rounds
@@ -326,7 +341,7 @@ struct BatchDev {
     Arena arena[3];
     const MapSlot *slot;      // [M]
     uint64_t *len;            // [M*3]
-    uint64_t *regions;        // [M*3*17]
+    uint64_t *regions;        // [M*3*kRegStride] class region table
     uint32_t n_maps;
     uint32_t max_rec, max_rd;
     // scratch
@@ -342,7 +357,8 @@ struct BatchDev {
                               // tiles, [M*16+1] all work tiles
     TileLedger *tile_led;     // [maxT]
     uint64_t *chunk_start;    // [M+1] first scan chunk of every map (rotation)
-    uint64_t *chunk_base;     // [maxC] output offset of every chunk within its map
+    uint64_t *chunk_base;     // [maxC] slot index of every chunk's first output (its offset
+                              // in the map's written output plus the rotation's output base)
     uint32_t *chunk_sum;      // [maxC] kept outputs of every chunk
     TileLedger *chunk_led;    // [maxC] ledger partials of every chunk
     uint32_t *counter;        // [4]
@@ -357,6 +373,44 @@ struct BatchDev {
     uint64_t *debug;          // [16] validation report (debug[0] != 0: violation found)
     double *sumsq;            // [kSumBlocks * 2] per-block double-double partial sums
 };
+
+__host__ __device__ __forceinline__ uint64_t *region_row(uint64_t *regions, uint32_t m,
+                                                         uint32_t buf) {
+    return regions + ((uint64_t)m * 3 + buf) * kRegStride;
+}
+
+// Logical element index -> slot index, for indices from e0 on: the class holding e0 is looked up
+// once; later classes (met only past the end of that class) by a short forward scan.
+struct Phys {
+    const uint64_t *reg;
+    uint64_t hi;     // logical end of the first class
+    uint64_t delta;  // slot index - logical index inside it (mod 2^64)
+    unsigned c0;
+};
+__device__ __forceinline__ Phys phys_from(const uint64_t *reg, uint64_t e0) {
+    unsigned c = 0;  // the largest class c with reg[c] <= e0 (e0 < reg[16])
+#pragma unroll
+    for (unsigned k = 1; k < 16; ++k) c = reg[k] <= e0 ? k : c;
+    Phys p;
+    p.reg = reg;
+    p.c0 = c;
+    p.hi = reg[c + 1];
+    p.delta = reg[kRegPhys + c] - reg[c];
+    return p;
+}
+__device__ __forceinline__ uint64_t phys_at(const Phys &p, uint64_t e) {
+    if (e < p.hi) return e + p.delta;
+    unsigned c = p.c0 + 1;
+    while (c < 15 && e >= p.reg[c + 1]) ++c;
+    return e - p.reg[c] + p.reg[kRegPhys + c];
+}
+// The region row of a contiguous map of `len` elements whose logical class starts are in
+// reg[0..16] (reg[16] == len): physical starts equal to them, extent len.
+__device__ __forceinline__ void contiguous_row(uint64_t *reg, unsigned c, uint64_t start) {
+    reg[c] = start;
+    if (c < 16) reg[kRegPhys + c] = start;
+    else reg[kRegTop] = start;
+}
 
 __device__ __forceinline__ uint32_t find_bucket(const uint64_t *start, uint32_t n, uint64_t g) {
     // largest b in [0, n) with start[b] <= g  (start is nondecreasing, start[0] == 0)
@@ -385,18 +439,20 @@ struct ScatterShared {
 };
 template <int W>
 __device__ __forceinline__ void scatter_tile(const uint64_t *keys, const double *vals,
-                                             uint64_t e0, uint64_t e1, uint32_t pa, uint32_t pb,
-                                             uint32_t n, uint64_t *okeys, double *ovals,
-                                             ScatterShared &sh) {
+                                             const uint64_t *reg, uint64_t e0, uint64_t e1,
+                                             uint32_t pa, uint32_t pb, uint32_t n,
+                                             uint64_t *okeys, double *ovals, ScatterShared &sh) {
     Key<W> kk[kPartRounds];
     double vv[kPartRounds];
     unsigned tag[kPartRounds];  // class (bits 0-4, 16 = none) | rank in wave (bits 8-14)
+    const Phys ph = phys_from(reg, e0);  // input element e lives at slot index phys_at(ph, e)
 #pragma unroll
     for (int r = 0; r < kPartRounds; ++r) {
         const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
         if (e < e1) {
-            kk[r] = kload<W>(keys + e * W);
-            vv[r] = vals[e];
+            const uint64_t at = phys_at(ph, e);
+            kk[r] = kload<W>(keys + at * W);
+            vv[r] = vals[at];
         }
     }
     const uint64_t lanemask = (1ull << lane_id()) - 1ull;
@@ -449,8 +505,8 @@ __global__ __launch_bounds__(kBlock) void k_part_plan(BatchDev b, const PassMap 
             uint64_t len = b.len[m * 3 + pm[m].in_buf];
             n = (unsigned)((len + tile - 1) / tile);
             if (len == 0) {  // no tile will describe the (empty) output: do it here
-                uint64_t *reg = b.regions + ((uint64_t)m * 3 + pm[m].out_buf) * 17;
-                for (int c = 0; c < 17; ++c) reg[c] = 0;
+                uint64_t *reg = region_row(b.regions, m, pm[m].out_buf);
+                for (int c = 0; c < kRegStride; ++c) reg[c] = 0;
                 b.len[m * 3 + pm[m].out_buf] = 0;
             }
         }
@@ -482,11 +538,12 @@ __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap
         uint64_t e1 = e0 + kTile < len ? e0 + kTile : len;
         const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[m].off * W;
         const uint32_t pa = P.pa, pb = P.pb;  // read once
+        const Phys ph = phys_from(region_row(b.regions, m, P.in_buf), e0);
         Key<W> k[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {  // branch-free: past the end, reload the last
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
-            k[r] = kload<W>(keys + (e < e1 ? e : e1 - 1) * W);
+            k[r] = kload<W>(keys + phys_at(ph, e < e1 ? e : e1 - 1) * W);
         }
         uint64_t cnt = 0;  // 16 bins x 4 bits
 #pragma unroll
@@ -583,12 +640,13 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b,
             class_total[threadIdx.x] = b.map_hist[m * 16 + threadIdx.x];
             tile_off = b.tile_base[g * 16 + threadIdx.x];
         }
+        const Phys ph = phys_from(region_row(b.regions, m, P.in_buf), e0);
         Key<W> kk[R];
         double vv[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {  // branch-free: past the end, reload the last
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
-            const uint64_t ec = e < e1 ? e : e1 - 1;
+            const uint64_t ec = phys_at(ph, e < e1 ? e : e1 - 1);
             kk[r] = kload<W>(keys + ec * W);
             vv[r] = vals[ec];
         }
@@ -597,12 +655,12 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b,
             uint64_t base = 0;
             for (unsigned c = 0; c < threadIdx.x; ++c) base += class_total[c];
             run[0][threadIdx.x] = base + tile_off;
-            if (g == b.tstart[m]) {
-                uint64_t *reg = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
-                reg[threadIdx.x] = base;
+            if (g == b.tstart[m]) {  // the output is contiguous
+                uint64_t *reg = region_row(b.regions, m, P.out_buf);
+                contiguous_row(reg, threadIdx.x, base);
                 if (threadIdx.x == 15) {
                     const uint64_t total = base + class_total[15];
-                    reg[16] = total;
+                    contiguous_row(reg, 16, total);
                     b.len[m * 3 + P.out_buf] = total;
                 }
             }
@@ -652,7 +710,7 @@ __global__ void k_regions(BatchDev b, uint32_t m, uint32_t buf) {
     const uint64_t *keys = b.arena[buf].keys + b.slot[m].off * W;
     unsigned c = threadIdx.x;
     if (c > 16) return;
-    uint64_t lo = 0, hi = len;  // first index whose class >= c
+    uint64_t lo = 0, hi = len;  // first index whose class >= c (the map is contiguous)
     while (lo < hi) {
         uint64_t mid = (lo + hi) >> 1;
         Key<W> k = kload<W>(keys + mid * W);
@@ -662,7 +720,7 @@ __global__ void k_regions(BatchDev b, uint32_t m, uint32_t buf) {
         else
             hi = mid;
     }
-    b.regions[((uint64_t)m * 3 + buf) * 17 + c] = c == 16 ? len : lo;
+    contiguous_row(region_row(b.regions, m, buf), c, c == 16 ? len : lo);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -678,7 +736,7 @@ __device__ __forceinline__ unsigned class_tiles(const BatchDev &b, const PassMap
     const uint32_t m = t >> 4, c = t & 15;
     const PassMap &P = pm[m];
     if (!P.active) return 0;
-    const uint64_t *reg = b.regions + ((uint64_t)m * 3 + P.in_buf) * 17;
+    const uint64_t *reg = region_row(b.regions, m, P.in_buf);
     const uint64_t na = reg[c + 1] - reg[c];
     uint64_t nd = 0;
     if (class_anti(c, P.qt, P.qs)) {
@@ -691,11 +749,13 @@ __device__ __forceinline__ unsigned class_tiles(const BatchDev &b, const PassMap
 // What the passes do with the tiles of class c: 1 = merged by the count and write passes (the
 // lower class of an anticommuting pair, or a commuting class truncation may change); 2 = copied
 // by the write pass only (a commuting class truncation cannot change: the split pass records
-// its counts); 0 = nothing (the higher class of a pair is handled with its partner's tiles).
-// The work list holds the kind-1 tiles in (map, class) order, then the kind-2 tiles.
+// its counts); 3 = left in place (such a class of an appending rotation: the split pass records
+// no output for it); 0 = nothing (the higher class of a pair is handled with its partner's
+// tiles).  The work list holds the kind-1 tiles in (map, class) order, then the kind-2 tiles.
 __device__ __forceinline__ unsigned tile_kind(const PassMap &P, uint32_t c) {
     if (class_anti(c, P.qt, P.qs)) return (c ^ ((P.qt << 2) | P.qs)) > c ? 1u : 0u;
-    return (P.flags & kPassCommutingKept) ? 2u : 1u;
+    if (!(P.flags & kPassCommutingKept)) return 1u;
+    return (P.flags & kPassAppend) ? 3u : 2u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_rot_plan(BatchDev b, const PassMap *pm) {
@@ -768,12 +828,12 @@ __device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm,
     x.m = t >> 4;
     x.c = t & 15;
     const PassMap &P = pm[x.m];
-    const uint64_t *reg = b.regions + ((uint64_t)x.m * 3 + P.in_buf) * 17;
-    x.A0 = reg[x.c];
+    const uint64_t *reg = region_row(b.regions, x.m, P.in_buf);
+    x.A0 = reg[kRegPhys + x.c];
     x.nA = reg[x.c + 1] - reg[x.c];
     x.anti = class_anti(x.c, P.qt, P.qs);
     x.cq = x.c ^ ((P.qt << 2) | P.qs);
-    x.D0 = reg[x.cq];
+    x.D0 = reg[kRegPhys + x.cq];
     x.nD = x.anti ? reg[x.cq + 1] - reg[x.cq] : 0;
     x.k = g - tstart[t];
     x.off = b.slot[x.m].off;
@@ -1179,10 +1239,11 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
             t.flags = (x.anti ? 1u : 0u) | (last ? 2u : 0u);
             b.tdesc[g] = t;
             const unsigned kind = tile_kind(pm[x.m], x.c);
-            if (kind) b.work[wstart[x.m * 16 + x.c] + x.k] = g;
-            if (kind == 2) {  // unchanged keys: the count pass does not visit the tile
+            if (kind == 1 || kind == 2) b.work[wstart[x.m * 16 + x.c] + x.k] = g;
+            if (kind >= 2) {  // unchanged keys: the count pass does not visit the tile; copied
+                              // (kind 2) or left in place with no output of its own (kind 3)
                 const uint64_t d1 = d0 + kMergeTile < x.nA ? d0 + kMergeTile : x.nA;
-                b.tile_hist[g] = (unsigned)(d1 - d0);
+                b.tile_hist[g] = kind == 2 ? (unsigned)(d1 - d0) : 0u;
                 b.tile_led[g] = TileLedger{};
             }
         }
@@ -1452,8 +1513,10 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan1(BatchDev b, const PassMap 
 
 // The end of a map's rotation scan (whole block, chunk_base and tile_base of the map written and
 // published): output regions and length, overflow, and the record with the map's ledger T.
+// base: the slot index of the first written output (the input's used extent when appending,
+// else 0), already included in chunk_base.
 __device__ void finish_map_scan(const BatchDev &b, const PassMap &P, uint32_t m, uint64_t acc,
-                                const TileLedger &T, uint64_t *cstart /*[16], LDS*/) {
+                                uint64_t base, const TileLedger &T, uint64_t *cstart /*[16]*/) {
     // output class c starts where its first tile writes; an empty class starts where the
     // next nonempty class does (or at the end)
     if (threadIdx.x < 16) {
@@ -1464,23 +1527,45 @@ __device__ void finish_map_scan(const BatchDev &b, const PassMap &P, uint32_t m,
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint64_t *rin = b.regions + ((uint64_t)m * 3 + P.in_buf) * 17;
-        uint64_t *rout = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
+        // appending: the input and output rows are the same words, so the input row is read
+        // whole before any of it is rewritten
+        const uint64_t *rin = region_row(b.regions, m, P.in_buf);
+        uint64_t *rout = region_row(b.regions, m, P.out_buf);
+        const bool append = (P.flags & kPassAppend) != 0;
+        uint64_t lin[17], pin[16];
+        for (int c = 0; c < 16; ++c) {
+            lin[c] = rin[c];
+            pin[c] = rin[kRegPhys + c];
+        }
+        lin[16] = rin[16];
+        // written output of class c: positions wr[c] .. wr[c+1] of the acc written terms
+        uint64_t wr[17];
+        uint64_t next = acc;
+        wr[16] = acc;
+        for (int c = 15; c >= 0; --c) {
+            if (cstart[c] != ~0ull) next = cstart[c] - base;
+            wr[c] = next;
+        }
+        const uint64_t end = base + acc;
         // A map that outgrew its slot is emptied: later passes never read past the slot; its
         // record carries the overflow and the host rolls the map back and grows the slot.
-        const bool over = b.overflow[m] || acc > b.slot[m].cap;
-        uint64_t next = acc;
-        for (int c = 15; c >= 0; --c) {
-            if (cstart[c] != ~0ull) next = cstart[c];
-            rout[c] = over ? 0 : next;
+        const bool over = b.overflow[m] || end > b.slot[m].cap;
+        uint64_t at = 0, kept = 0;  // kept: terms of the classes left in place
+        for (int c = 0; c < 16; ++c) {
+            const bool in_place = append && tile_kind(P, (uint32_t)c) == 3u;
+            rout[c] = over ? 0 : at;
+            rout[kRegPhys + c] = over ? 0 : in_place ? pin[c] : base + wr[c];
+            at += in_place ? lin[c + 1] - lin[c] : wr[c + 1] - wr[c];
+            kept += in_place ? lin[c + 1] - lin[c] : 0;
         }
-        rout[16] = over ? 0 : acc;
-        const uint64_t len_in = rin[16];
-        b.len[m * 3 + P.out_buf] = over ? 0 : acc;
+        rout[16] = over ? 0 : at;
+        rout[kRegTop] = over ? 0 : end;
+        const uint64_t len_in = lin[16];
+        b.len[m * 3 + P.out_buf] = over ? 0 : at;
         if (over) b.overflow[m] = 1;
         tpgp_hip_record r;
         r.len_in = len_in;
-        r.len_out = acc;
+        r.len_out = at;
         r.dropped = T.dropped;
         r.l1_up = T.l1;
         r.sq_up = T.sq;
@@ -1490,6 +1575,7 @@ __device__ void finish_map_scan(const BatchDev &b, const PassMap &P, uint32_t m,
         r.nonfinite = T.nonfinite;
         r.overflow = over ? 1u : 0u;
         r.rotation = P.rot_seq;
+        r.in_place = kept;
         b.rec[(uint64_t)m * b.max_rec + P.rec_slot] = r;
     }
     __syncthreads();
@@ -1505,6 +1591,10 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
     __shared__ TileLedger sh[kBlock / 64];
     __shared__ uint64_t cstart[16];
     const uint64_t c0 = b.chunk_start[m], c1 = b.chunk_start[m + 1];
+    // the outputs start at the input's used extent when appending (read before
+    // finish_map_scan rewrites the row, which is the input's own when appending)
+    const uint64_t obase =
+        (P.flags & kPassAppend) ? region_row(b.regions, m, P.in_buf)[kRegTop] : 0;
     uint64_t carry = 0;
     TileLedger L{};
     for (uint64_t base = c0; base < c1; base += kBlock) {
@@ -1514,11 +1604,11 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
         if (ch < c1) ledger_fold(L, b.chunk_led[ch]);
         unsigned tot;
         const unsigned ex = block_exclusive_scan(v, scratch, &tot);
-        if (ch < c1) b.chunk_base[ch] = carry + ex;
+        if (ch < c1) b.chunk_base[ch] = obase + carry + ex;
         carry += tot;
     }
     const TileLedger T = block_ledger(L, sh);  // (its barriers publish chunk_base too)
-    finish_map_scan(b, P, m, carry, T, cstart);
+    finish_map_scan(b, P, m, carry, obase, T, cstart);
 }
 
 // The whole rotation scan of small maps in one launch, one block per map: its chunks one after
@@ -1533,6 +1623,10 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan_maps(BatchDev b, const Pass
         const PassMap &P = pm[m];
         if (!P.active) continue;
         const uint64_t t0 = b.tstart[m * 16], t1 = b.tstart[m * 16 + 16];
+        // outputs start at the input's used extent when appending (read before the row is
+        // rewritten)
+        const uint64_t obase =
+            (P.flags & kPassAppend) ? region_row(b.regions, m, P.in_buf)[kRegTop] : 0;
         uint64_t carry = 0, ch = b.chunk_start[m];
         TileLedger L{};
         for (uint64_t g0 = t0; g0 < t1; g0 += kChunkTiles, ++ch) {  // one tile per thread
@@ -1544,11 +1638,11 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan_maps(BatchDev b, const Pass
                 b.tile_base[g] = ex;
                 ledger_fold(L, b.tile_led[g]);
             }
-            if (threadIdx.x == 0) b.chunk_base[ch] = carry;
+            if (threadIdx.x == 0) b.chunk_base[ch] = obase + carry;
             carry += tot;
         }
         const TileLedger T = block_ledger(L, sh);  // (its barriers publish the offsets too)
-        finish_map_scan(b, P, m, carry, T, cstart);
+        finish_map_scan(b, P, m, carry, obase, T, cstart);
     }
 }
 
@@ -1582,6 +1676,7 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
         t.d0 = x.k * kMergeTile;
         t.d1 = t.d0 + kMergeTile < total ? t.d0 + kMergeTile : total;
         const uint64_t tm0 = b.tstart[x.m * 16], c0 = b.chunk_start[x.m];
+        // slot index of the tile's first output (chunk_base includes the rotation's output base)
         const uint64_t out_pos = b.chunk_base[c0 + (g - tm0) / kChunkTiles] + b.tile_base[g];
         const uint64_t count = b.tile_hist[g];
         const uint64_t cap = b.slot[x.m].cap;
@@ -1755,8 +1850,8 @@ __device__ __forceinline__ void small_rotation(const BatchDev &b, const PassMap 
     const uint64_t *ikeys = b.arena[P.in_buf].keys + off * W;
     uint64_t *gk = b.arena[P.out_buf].keys + off * W;
     double *gv = b.arena[P.out_buf].vals + off;
-    const uint64_t *rin = b.regions + ((uint64_t)m * 3 + P.in_buf) * 17;
-    uint64_t *rout = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
+    const uint64_t *rin = region_row(b.regions, m, P.in_buf);
+    uint64_t *rout = region_row(b.regions, m, P.out_buf);
     const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
     const bool want_log = (P.flags & 2u) != 0;
     bool over = b.overflow[m] != 0;
@@ -1770,11 +1865,11 @@ __device__ __forceinline__ void small_rotation(const BatchDev &b, const PassMap 
         MergeCtx x;
         x.m = m;
         x.c = c;
-        x.A0 = rin[c];
+        x.A0 = rin[kRegPhys + c];
         x.nA = rin[c + 1] - rin[c];
         x.anti = class_anti(c, P.qt, P.qs);
         x.cq = c ^ ((P.qt << 2) | P.qs);
-        x.D0 = rin[x.cq];
+        x.D0 = rin[kRegPhys + x.cq];
         x.nD = x.anti ? rin[x.cq + 1] - rin[x.cq] : 0;
         x.k = 0;
         x.off = b.slot[m].off;
@@ -1882,8 +1977,8 @@ __device__ __forceinline__ void small_rotation(const BatchDev &b, const PassMap 
     if (threadIdx.x == 0) {
         TileLedger T = sh.red[0];
         for (int w = 1; w < kBlock / 64; ++w) ledger_fold(T, sh.red[w]);
-        for (int c = 0; c < 16; ++c) rout[c] = over ? 0 : sh.rout[c];
-        rout[16] = over ? 0 : out_at;
+        for (int c = 0; c < 16; ++c) contiguous_row(rout, c, over ? 0 : sh.rout[c]);
+        contiguous_row(rout, 16, over ? 0 : out_at);
         b.len[m * 3 + P.out_buf] = over ? 0 : out_at;
         if (over) b.overflow[m] = 1;
         tpgp_hip_record r;
@@ -1898,6 +1993,7 @@ __device__ __forceinline__ void small_rotation(const BatchDev &b, const PassMap 
         r.nonfinite = sh.nonfinite;
         r.overflow = over ? 1u : 0u;
         r.rotation = P.rot_seq;
+        r.in_place = 0;
         b.rec[(uint64_t)m * b.max_rec + P.rec_slot] = r;
     }
     __syncthreads();
@@ -1913,14 +2009,16 @@ __device__ __forceinline__ void small_partition(const BatchDev &b, const PassMap
     const double *vals = b.arena[P.in_buf].vals + off;
     uint64_t *okeys = b.arena[P.out_buf].keys + off * W;
     double *ovals = b.arena[P.out_buf].vals + off;
+    const uint64_t *rin = region_row(b.regions, m, P.in_buf);
     if (threadIdx.x < 16) sh.hist[threadIdx.x] = 0;
     __syncthreads();
     for (uint64_t e0 = 0; e0 < len; e0 += kPartTile) {  // histogram of the new classes
         Key<W> k[kPartRounds];
+        const Phys ph = phys_from(rin, e0);
 #pragma unroll
         for (int r = 0; r < kPartRounds; ++r) {
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
-            if (e < len) k[r] = kload<W>(keys + e * W);
+            if (e < len) k[r] = kload<W>(keys + phys_at(ph, e) * W);
         }
 #pragma unroll
         for (int r = 0; r < kPartRounds; ++r) {
@@ -1930,20 +2028,20 @@ __device__ __forceinline__ void small_partition(const BatchDev &b, const PassMap
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t *reg = b.regions + ((uint64_t)m * 3 + P.out_buf) * 17;
+        uint64_t *reg = region_row(b.regions, m, P.out_buf);
         uint64_t at = 0;
         for (int c = 0; c < 16; ++c) {
-            reg[c] = at;
+            contiguous_row(reg, c, at);
             sh.sc.run[c] = at;
             at += sh.hist[c];
         }
-        reg[16] = at;
+        contiguous_row(reg, 16, at);
         b.len[m * 3 + P.out_buf] = at;
     }
     __syncthreads();
     for (uint64_t e0 = 0; e0 < len; e0 += kPartTile) {  // stable scatter, tile by tile
         const uint64_t e1 = e0 + kPartTile < len ? e0 + kPartTile : len;
-        scatter_tile<W>(keys, vals, e0, e1, P.pa, P.pb, n, okeys, ovals, sh.sc);
+        scatter_tile<W>(keys, vals, rin, e0, e1, P.pa, P.pb, n, okeys, ovals, sh.sc);
     }
 }
 
@@ -2078,12 +2176,15 @@ __device__ bool lds_load(const BatchDev &b, uint32_t buf, uint32_t m, LdsMap<W> 
     const Arena a = arena_at(b, buf);
     const uint64_t *gk = a.keys + off * W;
     const double *gv = a.vals + off;
+    const uint64_t *reg = region_row(b.regions, m, buf);
+    unsigned c = 0;  // class of this thread's element e (e < len = reg[16], so c <= 15)
     for (uint32_t e = threadIdx.x; e < len; e += kLdsThreads) {
-        kstore<W>(sh.keys[0] + e * W, kload<W>(gk + e * W));
-        sh.vals[0][e] = gv[e];
+        while (e >= reg[c + 1]) ++c;
+        const uint64_t at = e - reg[c] + reg[kRegPhys + c];
+        kstore<W>(sh.keys[0] + e * W, kload<W>(gk + at * W));
+        sh.vals[0][e] = gv[at];
     }
-    if (threadIdx.x < 17)
-        sh.reg[0][threadIdx.x] = (uint32_t)b.regions[((uint64_t)m * 3 + buf) * 17 + threadIdx.x];
+    if (threadIdx.x < 17) sh.reg[0][threadIdx.x] = (uint32_t)reg[threadIdx.x];
     if (threadIdx.x == 0) sh.len[0] = (uint32_t)len;
     __syncthreads();
     return true;
@@ -2102,7 +2203,7 @@ __device__ void lds_store(const BatchDev &b, uint32_t buf, uint32_t m, const Lds
         gv[e] = sh.vals[cur][e];
     }
     if (threadIdx.x < 17)
-        b.regions[((uint64_t)m * 3 + buf) * 17 + threadIdx.x] = sh.reg[cur][threadIdx.x];
+        contiguous_row(region_row(b.regions, m, buf), threadIdx.x, sh.reg[cur][threadIdx.x]);
     if (threadIdx.x == 0) b.len[m * 3 + buf] = len;
 }
 
@@ -2406,6 +2507,7 @@ __device__ bool lds_rotation(const LdsSink &out_to, const PassMap &P, uint32_t m
         rec.nonfinite = T.nonfinite;
         rec.overflow = 0;
         rec.rotation = P.rot_seq;
+        rec.in_place = 0;
         out_to.rec[P.rec_slot] = rec;
     }
     __syncthreads();
@@ -2475,7 +2577,8 @@ __global__ __launch_bounds__(kBlock) void k_small(BatchDev b, const PassMap *ops
     }
 }
 
-// One wave per map: binary search of the n single-site keys (symbol `pauli` at site s).
+// One wave per map: binary search of the n single-site keys (symbol `pauli` at site s), each in
+// the class region its top two digits name.
 template <int W>
 __global__ void k_readout(BatchDev b, const ReadoutMap *pm, unsigned pauli) {
     const uint32_t m = blockIdx.x;
@@ -2485,10 +2588,12 @@ __global__ void k_readout(BatchDev b, const ReadoutMap *pm, unsigned pauli) {
     if (s >= 64) return;
     double out = 0.0;
     if (s < n) {
-        const uint64_t len = b.len[m * 3 + P.in_buf];
-        const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[m].off * W;
-        const double *vals = b.arena[P.in_buf].vals + b.slot[m].off;
-        Key<W> target = kdigit_mask<W>(P.pos[s], pauli);
+        const uint64_t *reg = region_row(b.regions, m, P.in_buf);
+        const Key<W> target = kdigit_mask<W>(P.pos[s], pauli);
+        const unsigned cls = (kdigit<W>(target, n - 1) << 2) | kdigit<W>(target, n - 2);
+        const uint64_t base = reg[kRegPhys + cls], len = reg[cls + 1] - reg[cls];
+        const uint64_t *keys = b.arena[P.in_buf].keys + (b.slot[m].off + base) * W;
+        const double *vals = b.arena[P.in_buf].vals + b.slot[m].off + base;
         uint64_t lo = 0, hi = len;
         while (lo < hi) {
             uint64_t mid = (lo + hi) >> 1;
@@ -2515,11 +2620,15 @@ __host__ __device__ __forceinline__ void dd_add(double &hi, double &lo, double b
     hi = s + err;
     lo = err - (hi - s);
 }
-__global__ void __launch_bounds__(256) k_sumsq(const double *vals, uint64_t len, double *out) {
+// Values in the map's key order (logical index k at its slot index through the region row).
+__global__ void __launch_bounds__(256) k_sumsq(const double *vals, const uint64_t *reg,
+                                               uint64_t len, double *out) {
     double hi = 0.0, lo = 0.0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned c = 0;  // class of this thread's k (k < len = reg[16], so c <= 15)
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < len; k += stride) {
-        const double x = vals[k];
+        while (k >= reg[c + 1]) ++c;
+        const double x = vals[k - reg[c] + reg[kRegPhys + c]];
         const double p = x * x;
         dd_add(hi, lo, p, fma(x, x, -p));
     }
@@ -2551,11 +2660,21 @@ __global__ void k_validate(BatchDev b, const PassMap *pm, uint32_t tag) {
         if (!P.active) continue;
         const uint32_t buf = P.out_buf, n = b.slot[m].n_spins;
         const uint64_t len = b.len[m * 3 + buf];
-        const uint64_t *reg = b.regions + ((uint64_t)m * 3 + buf) * 17;
+        const uint64_t *reg = region_row(b.regions, m, buf);
         const uint64_t *keys = b.arena[buf].keys + b.slot[m].off * W;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
-            bool bad = reg[0] != 0 || reg[16] != len;
-            for (int c = 0; c < 16; ++c) bad = bad || reg[c] > reg[c + 1];
+            // logical starts nondecreasing from 0 to len; every class inside the used extent,
+            // which is inside the slot; no two nonempty classes overlap
+            bool bad = reg[0] != 0 || reg[16] != len || reg[kRegTop] > b.slot[m].cap;
+            for (int c = 0; c < 16; ++c) {
+                bad = bad || reg[c] > reg[c + 1] ||
+                      reg[kRegPhys + c] + (reg[c + 1] - reg[c]) > reg[kRegTop];
+                for (int d = 0; d < c; ++d) {
+                    const uint64_t nc = reg[c + 1] - reg[c], nd = reg[d + 1] - reg[d];
+                    const uint64_t pc = reg[kRegPhys + c], pd = reg[kRegPhys + d];
+                    bad = bad || (nc && nd && pc < pd + nd && pd < pc + nc);
+                }
+            }
             if (bad && atomicCAS((unsigned long long *)&b.debug[0], 0ull,
                                  (unsigned long long)tag) == 0ull) {
                 b.debug[1] = m;
@@ -2566,10 +2685,14 @@ __global__ void k_validate(BatchDev b, const PassMap *pm, uint32_t tag) {
         }
         for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < len;
              e += (uint64_t)gridDim.x * blockDim.x) {
-            Key<W> k = kload<W>(keys + e * W);
+            const Phys ph = phys_from(reg, e);
+            Key<W> k = kload<W>(keys + phys_at(ph, e) * W);
             unsigned cls = (kdigit<W>(k, n - 1) << 2) | kdigit<W>(k, n - 2);
             bool bad = e < reg[cls] || e >= reg[cls + 1];
-            if (e > 0) bad = bad || !klt<W>(kload<W>(keys + (e - 1) * W), k);
+            if (e > 0) {
+                const Phys pp = phys_from(reg, e - 1);
+                bad = bad || !klt<W>(kload<W>(keys + phys_at(pp, e - 1) * W), k);
+            }
             if (bad && atomicCAS((unsigned long long *)&b.debug[0], 0ull,
                                  (unsigned long long)tag) == 0ull) {
                 b.debug[1] = m;
@@ -2579,7 +2702,7 @@ __global__ void k_validate(BatchDev b, const PassMap *pm, uint32_t tag) {
                 b.debug[5] = reg[cls];
                 b.debug[6] = reg[cls + 1];
                 b.debug[7] = k.w[0];
-                b.debug[8] = e > 0 ? keys[(e - 1) * W] : 0;
+                b.debug[8] = e > 0 ? keys[phys_at(phys_from(reg, e - 1), e - 1) * W] : 0;
             }
         }
     }
@@ -3019,7 +3142,21 @@ struct HostMap {
     // (0: nothing known).  A rotation whose threshold is below it cannot truncate a key it
     // leaves unchanged.
     double floor = 0.0, floor_c = 0.0;
+    // Appending rotations: the working / committed state may have classes out of key order in
+    // its slot (gap), the working buffer's used extent as planned (est_top, in terms), and the
+    // planning bound on what one rotation appends, as a fraction of the map (app_level indexes
+    // kAppendLevels; learned from the appended counts at sync).
+    bool gap = false, gap_c = false;
+    uint64_t est_top = 0;
+    uint32_t app_level = 4;
+    std::vector<uint32_t> app_recs;  // record slots of the rotations planned to append
+    // growth of the map over the last synced work (peak output / first input, 1 to 4): a
+    // growing map is planned at the size it will reach
+    double growth = 1.0;
 };
+// Fractions of the map one appending rotation is planned to write after the extent.
+constexpr double kAppendLevels[] = {0.25, 0.375, 0.5, 0.625, 0.75, 1.0, 1.25, 1.5, 2.0, 2.5};
+constexpr uint32_t kAppendTop = sizeof(kAppendLevels) / sizeof(kAppendLevels[0]) - 1;
 struct LoggedLayout {
     uint32_t map, rotation;
     std::vector<uint8_t> sigma;
@@ -3088,6 +3225,8 @@ struct tpgp_hip_batch {
     uint64_t prof_count[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     // step graphs (TPGP_HIP_GRAPHS=0 disables them)
     bool graphs_on = true;
+    // appending rotations (TPGP_HIP_APPEND=0 copies every class instead)
+    bool append_on = true;
     bool capturing = false, capture_spilled = false;
     std::vector<uint8_t> cap_host;            // descriptors staged while capturing
     std::vector<std::vector<uint8_t>> spill;  // staging beyond cap_host (capture is dropped)
@@ -3166,7 +3305,7 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
     ALLOC(b->d_slot, M);
     d.slot = b->d_slot;
     ALLOC(d.len, (size_t)M * 3);
-    ALLOC(d.regions, (size_t)M * 3 * 17);
+    ALLOC(d.regions, (size_t)M * 3 * kRegStride);
     ALLOC(d.tstart, (size_t)M * 16 + 2);
     ALLOC(d.tile_hist, b->max_tiles * 16);
     ALLOC(d.tile_base, b->max_tiles * 16);
@@ -3207,8 +3346,8 @@ int allocate_device(tpgp_hip_batch *b, char *error, size_t cap) {
                             st)) != hipSuccess)
         return drain_fail(st, e, "HIP slot upload", error, cap);
     hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, st, d.len, (uint64_t)M * 3, 0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, st, d.regions, (uint64_t)M * 3 * 17,
-                       0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, st, d.regions,
+                       (uint64_t)M * 3 * kRegStride, 0ull);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, d.overflow, (uint64_t)M, 0u);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, d.log_count, 1ull, 0ull);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, d.counter, 4ull, 0u);
@@ -3429,6 +3568,8 @@ unsigned plan_partition(tpgp_hip_batch *b, const std::vector<int> &want,
         h[m].pa = pa[m];
         h[m].pb = pb[m];
         hm.kcur = (int)h[m].out_buf;
+        hm.gap = false;  // the partition writes the map contiguously
+        hm.est_top = hm.est_len;
         // sigma: move sites at pa, pb to the top, keep the others' order
         size_t n = hm.n_spins;
         uint8_t sa = hm.sig[pa[m]], sb = hm.sig[pb[m]];
@@ -3574,6 +3715,8 @@ extern "C" tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *ctx, size_t n
     b->small.assign(n_maps, 0);
     const char *gr = getenv("TPGP_HIP_GRAPHS");
     b->graphs_on = !(gr && *gr == '0');
+    const char *ap = getenv("TPGP_HIP_APPEND");
+    b->append_on = !(ap && *ap == '0');
     std::lock_guard<std::recursive_mutex> lock(ctx->mutex);
     DeviceGuard guard(ctx->device);
     if (!guard.ok || allocate_device(b, error, cap) != TPGP_HIP_OK) {
@@ -3631,8 +3774,9 @@ extern "C" int tpgp_hip_batch_upload(tpgp_hip_batch *b, size_t map, const uint64
     hipStream_t st = b->ctx->stream;
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "HIP upload", error, cap);
-    // discard pending work of this map, install the new state as committed
+    // discard pending work of this map, install the new state as committed (contiguous)
     hm.kcur = hm.kc;
+    hm.gap = hm.gap_c = false;
     for (size_t p = 0; p < hm.n_spins; ++p) hm.sig[p] = (uint8_t)p;
     hm.sig_c = hm.sig;
     double fl = INFINITY;
@@ -3666,6 +3810,7 @@ extern "C" int tpgp_hip_batch_upload(tpgp_hip_batch *b, size_t map, const uint64
     hm.committed_len = len;
     hm.est_len = len;
     hm.nrec = hm.nrd = 0;
+    hm.app_recs.clear();
     return TPGP_HIP_OK;
 }
 
@@ -3705,13 +3850,26 @@ unsigned plan_rotation(tpgp_hip_batch *b, const tpgp_hip_rotation *req, uint32_t
         HostMap &hm = b->maps[m];
         size_t n = hm.n_spins;
         bool i_top = hm.sig[n - 1] == r.i;
+        const bool kept = r.threshold < hm.floor;
+        // Append when the unchanged classes can stay where they are: the working buffer is not
+        // the committed one (whose table and terms a rollback returns to), and its free space
+        // holds the planned bound on what the rotation writes (an underestimate is caught on
+        // the device as an overflow; the step is rolled back and the bound raised at sync).
+        const double grown = (double)hm.est_len * hm.growth;
+        const uint64_t est = (uint64_t)(grown + grown / 8) + 1024;
+        const uint64_t add = (uint64_t)(kAppendLevels[hm.app_level] * (double)est) + 1024;
+        const bool append = b->append_on && kept && !b->small[m] && hm.kcur != hm.kc &&
+                            hm.est_top + add <= hm.cap;
         h[m].active = 1;
         h[m].in_buf = (uint32_t)hm.kcur;
-        h[m].out_buf = (uint32_t)next_buffer(hm);
+        h[m].out_buf = append ? (uint32_t)hm.kcur : (uint32_t)next_buffer(hm);
         h[m].qt = i_top ? r.qi : r.qj;
         h[m].qs = i_top ? r.qj : r.qi;
         h[m].flags = (r.validated ? 1u : 0u) | (r.log ? 2u : 0u) |
-                     (r.threshold < hm.floor ? kPassCommutingKept : 0u);
+                     (kept ? kPassCommutingKept : 0u) | (append ? kPassAppend : 0u);
+        hm.gap = append;
+        hm.est_top = append ? hm.est_top + add : est;
+        if (append) hm.app_recs.push_back(hm.nrec);
         // afterwards every kept non-identity coefficient exceeds this threshold
         hm.floor = std::nextafter(r.threshold, INFINITY);
         h[m].rec_slot = hm.nrec;
@@ -3989,6 +4147,30 @@ extern "C" int tpgp_hip_batch_readout(tpgp_hip_batch *b, unsigned pauli, const i
     return TPGP_HIP_OK;
 }
 
+namespace {
+// After a step: the planning bound on what one appending rotation writes follows the largest
+// fraction a rotation of this map appended (up at once, with a margin; down one level at a time
+// when well clear), and rises two levels when the step overflowed the slot.
+void learn_append(HostMap &hm, const tpgp_hip_record *rec) {
+    double most = 0.0;
+    bool over = false;
+    for (uint32_t r = 0; r < hm.nrec; ++r) over = over || rec[r].overflow;
+    for (uint32_t r : hm.app_recs)  // wrote len_out - in_place after the extent
+        if (r < hm.nrec && rec[r].len_in)
+            most = std::max(most, (double)(rec[r].len_out - rec[r].in_place) /
+                                      (double)rec[r].len_in);
+    hm.app_recs.clear();
+    if (most == 0.0 && !over) return;  // nothing appended: nothing learned
+    uint32_t need = 0;
+    while (need < kAppendTop && kAppendLevels[need] < most * 1.15 + 0.02) ++need;
+    if (over) need = std::max(need, std::min(kAppendTop, hm.app_level + 2));
+    if (need > hm.app_level)
+        hm.app_level = need;
+    else if (hm.app_level > 0 && kAppendLevels[hm.app_level - 1] >= most * 1.6 + 0.05)
+        --hm.app_level;
+}
+}  // namespace
+
 extern "C" int tpgp_hip_batch_sum_squares(tpgp_hip_batch *b, size_t map, double out[2],
                                           char *error, size_t cap) {
     msg(error, cap, "");
@@ -4008,8 +4190,9 @@ extern "C" int tpgp_hip_batch_sum_squares(tpgp_hip_batch *b, size_t map, double 
     std::lock_guard<std::recursive_mutex> lock(b->ctx->mutex);
     DeviceGuard guard(b->ctx->device);
     const double *vals = b->dev.arena[hm.kc].vals + hm.off;
+    const uint64_t *row = region_row(b->dev.regions, (uint32_t)map, (uint32_t)hm.kc);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(kSumBlocks, (len + 255) / 256);
-    hipLaunchKernelGGL(k_sumsq, dim3(grid), dim3(256), 0, b->ctx->stream, vals, len,
+    hipLaunchKernelGGL(k_sumsq, dim3(grid), dim3(256), 0, b->ctx->stream, vals, row, len,
                        b->dev.sumsq);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "HIP sum-of-squares launch", error, cap);
@@ -4047,18 +4230,28 @@ extern "C" int tpgp_hip_batch_sync(tpgp_hip_batch *b, tpgp_hip_record *records,
     }
     if (n_records) *n_records = nrec;
     if (n_readouts) *n_readouts = nrd;
-    if (records && nrec) {
-        if (max_records < nrec) {
-            msg(error, cap, "Record buffer too small (%u needed).", nrec);
-            return TPGP_HIP_EINVAL;
-        }
+    if (records && nrec && max_records < nrec) {
+        msg(error, cap, "Record buffer too small (%u needed).", nrec);
+        return TPGP_HIP_EINVAL;
+    }
+    std::vector<uint64_t> peak(M, 0);  // largest output of each map's rotations (overflowed too)
+    if (nrec) {
         std::vector<tpgp_hip_record> tmp((size_t)M * b->max_rec);
         e = hipMemcpy(tmp.data(), b->dev.rec, tmp.size() * sizeof(tpgp_hip_record),
                       hipMemcpyDeviceToHost);
         if (e != hipSuccess) return hip_fail(e, "HIP record transfer", error, cap);
-        for (uint32_t m = 0; m < M; ++m)
+        for (uint32_t m = 0; m < M; ++m) {
+            if (records)
+                for (uint32_t r = 0; r < b->maps[m].nrec; ++r)
+                    records[(size_t)m * max_records + r] = tmp[(size_t)m * b->max_rec + r];
+            const tpgp_hip_record *rm = tmp.data() + (size_t)m * b->max_rec;
             for (uint32_t r = 0; r < b->maps[m].nrec; ++r)
-                records[(size_t)m * max_records + r] = tmp[(size_t)m * b->max_rec + r];
+                peak[m] = std::max<uint64_t>(peak[m], rm[r].len_out);
+            if (b->maps[m].nrec && rm[0].len_in)
+                b->maps[m].growth =
+                    std::min(4.0, std::max(1.0, (double)peak[m] / (double)rm[0].len_in));
+            learn_append(b->maps[m], tmp.data() + (size_t)m * b->max_rec);
+        }
     }
     if (readouts && nrd) {
         if (max_readouts < nrd) {
@@ -4080,7 +4273,9 @@ extern "C" int tpgp_hip_batch_sync(tpgp_hip_batch *b, tpgp_hip_record *records,
         if (e != hipSuccess) return hip_fail(e, "HIP length transfer", error, cap);
         for (uint32_t m = 0; m < M; ++m) {
             const uint64_t l = len[(size_t)m * 3 + b->maps[m].kcur];
-            b->maps[m].est_len = std::max<uint64_t>(l, b->maps[m].est_len / 2);
+            // the step's peak counts too: a map emptied by an overflow is rolled back and its
+            // step runs again at the size it reached
+            b->maps[m].est_len = std::max<uint64_t>({l, b->maps[m].est_len / 2, peak[m]});
             if (lengths) lengths[m] = l;
         }
     }
@@ -4148,7 +4343,10 @@ extern "C" int tpgp_hip_batch_sync(tpgp_hip_batch *b, tpgp_hip_record *records,
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, b->dev.log_count, 1ull, 0ull);
     }
     b->logged.clear();
-    for (uint32_t m = 0; m < M; ++m) b->maps[m].nrec = b->maps[m].nrd = 0;
+    for (uint32_t m = 0; m < M; ++m) {
+        b->maps[m].nrec = b->maps[m].nrd = 0;
+        b->maps[m].app_recs.clear();
+    }
     release_stage(b);
     return status;
 }
@@ -4171,11 +4369,13 @@ extern "C" int tpgp_hip_batch_commit(tpgp_hip_batch *b, const int *mask, char *e
             hm.kc = hm.kcur;
             hm.sig_c = hm.sig;
             hm.floor_c = hm.floor;
+            hm.gap_c = hm.gap;
             hm.committed_len = len[(size_t)m * 3 + hm.kc];
         } else {
             hm.kcur = hm.kc;
             hm.sig = hm.sig_c;
             hm.floor = hm.floor_c;
+            hm.gap = hm.gap_c;
         }
     }
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, b->ctx->stream, b->dev.overflow,
@@ -4194,7 +4394,9 @@ extern "C" int tpgp_hip_batch_rollback(tpgp_hip_batch *b, char *error, size_t ca
         hm.kcur = hm.kc;
         hm.sig = hm.sig_c;
         hm.floor = hm.floor_c;
+        hm.gap = hm.gap_c;
         hm.nrec = hm.nrd = 0;
+        hm.app_recs.clear();
     }
     b->logged.clear();
     release_stage(b);
@@ -4244,7 +4446,10 @@ extern "C" int tpgp_hip_batch_download(tpgp_hip_batch *b, size_t map, uint64_t *
         if (n % 2) pairs.push_back({n - 1, n - 2});
         bool identity = true;
         for (size_t p = 0; p < n; ++p) identity = identity && hm.sig[p] == p;
-        if (!identity) {
+        // a committed state with classes out of place is laid out contiguously by one
+        // partition that keeps the digit layout (the top pair moved to the top again)
+        if (identity && hm.gap_c) pairs.assign(1, {n - 1, n - 2});
+        if (!identity || hm.gap_c) {
             for (auto &pr : pairs) {
                 pa[map] = (uint32_t)position_of(hm, pr.first);
                 pb[map] = (uint32_t)position_of(hm, pr.second);
@@ -4268,6 +4473,7 @@ extern "C" int tpgp_hip_batch_download(tpgp_hip_batch *b, size_t map, uint64_t *
         // the reordered copy becomes the committed state (same map, reference layout)
         hm.kc = hm.kcur;
         hm.sig_c = hm.sig;
+        hm.gap_c = hm.gap;
         release_stage(b);
     }
     return TPGP_HIP_OK;
@@ -4285,7 +4491,7 @@ int relayout(tpgp_hip_batch *b, const std::vector<uint64_t> &new_cap, char *erro
     if (e != hipSuccess) return hip_fail(e, "HIP reserve", error, cap);
     const uint32_t M = b->M;
     const int W = b->W;
-    std::vector<uint64_t> len((size_t)M * 3), reg((size_t)M * 3 * 17);
+    std::vector<uint64_t> len((size_t)M * 3), reg((size_t)M * 3 * kRegStride);
     if ((e = hipMemcpy(len.data(), b->dev.len, len.size() * 8, hipMemcpyDeviceToHost)) !=
             hipSuccess ||
         (e = hipMemcpy(reg.data(), b->dev.regions, reg.size() * 8, hipMemcpyDeviceToHost)) !=
@@ -4322,9 +4528,21 @@ int relayout(tpgp_hip_batch *b, const std::vector<uint64_t> &new_cap, char *erro
         uint64_t *dk = on_device ? tk + at[m] * W : hk.data() + at[m] * W;
         double *dv = on_device ? tv + at[m] : hv.data() + at[m];
         const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-        e = hipMemcpyAsync(dk, b->dev.arena[hm.kc].keys + hm.off * W, n[m] * W * 8, kind, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(dv, b->dev.arena[hm.kc].vals + hm.off, n[m] * 8, kind, st);
+        // the committed map in key order: its classes, runs of adjacent ones copied together
+        const uint64_t *row = region_row(reg.data(), m, (uint32_t)hm.kc);
+        for (int c = 0; c < 16 && e == hipSuccess;) {
+            const uint64_t lo = row[c], src = row[kRegPhys + c];
+            int d = c + 1;  // extend while the next class follows in the slot too
+            while (d < 16 && row[kRegPhys + d] == src + (row[d] - lo)) ++d;
+            const uint64_t cnt = row[d] - lo;
+            c = d;
+            if (!cnt) continue;
+            e = hipMemcpyAsync(dk + lo * W, b->dev.arena[hm.kc].keys + (hm.off + src) * W,
+                               cnt * W * 8, kind, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(dv + lo, b->dev.arena[hm.kc].vals + hm.off + src, cnt * 8,
+                                   kind, st);
+        }
     }
     // wait even after a failed enqueue: copies already queued still write into hk / hv
     const hipError_t drained = hipStreamSynchronize(st);
@@ -4365,12 +4583,15 @@ int relayout(tpgp_hip_batch *b, const std::vector<uint64_t> &new_cap, char *erro
         b->poisoned = true;
         return s;
     }
-    std::vector<uint64_t> nlen((size_t)M * 3, 0), nreg((size_t)M * 3 * 17, 0);
+    std::vector<uint64_t> nlen((size_t)M * 3, 0), nreg((size_t)M * 3 * kRegStride, 0);
     for (uint32_t m = 0; m < M && e == hipSuccess; ++m) {
         HostMap &hm = b->maps[m];
         nlen[(size_t)m * 3] = n[m];
-        for (int c = 0; c < 17; ++c)
-            nreg[(size_t)m * 3 * 17 + c] = reg[((size_t)m * 3 + hm.kc) * 17 + c];
+        const uint64_t *row = region_row(reg.data(), m, (uint32_t)hm.kc);
+        uint64_t *nrow = region_row(nreg.data(), m, 0);  // buffer 0, contiguous
+        for (int c = 0; c < 17; ++c) nrow[c] = row[c];
+        for (int c = 0; c < 16; ++c) nrow[kRegPhys + c] = row[c];
+        nrow[kRegTop] = row[16];
         if (n[m]) {
             const uint64_t *sk = on_device ? tk + at[m] * W : hk.data() + at[m] * W;
             const double *sv = on_device ? tv + at[m] : hv.data() + at[m];
@@ -4382,7 +4603,9 @@ int relayout(tpgp_hip_batch *b, const std::vector<uint64_t> &new_cap, char *erro
         hm.kc = hm.kcur = 0;
         hm.sig = hm.sig_c;
         hm.floor = hm.floor_c;
+        hm.gap = hm.gap_c = false;
         hm.nrec = hm.nrd = 0;
+        hm.app_recs.clear();
         hm.est_len = n[m];
     }
     if (e == hipSuccess)
@@ -4499,12 +4722,17 @@ std::vector<uint8_t> host_state(const tpgp_hip_batch *b) {
     std::vector<uint8_t> out;
     for (const HostMap &hm : b->maps) {
         const uint32_t f[6] = {(uint32_t)hm.n_spins, (uint32_t)hm.kc, (uint32_t)hm.kcur, hm.nrec,
-                               hm.nrd, 0};
+                               hm.nrd, (hm.gap ? 1u : 0u) | (hm.app_level << 1)};
+        // the planned extent matters only while the working buffer is not the committed one
+        // (appending starts after a partition or a copying rotation resets it)
+        const uint64_t top = hm.kcur != hm.kc ? hm.est_top : 0;
         uint64_t fl;
         std::memcpy(&fl, &hm.floor, 8);
         const uint8_t *p = reinterpret_cast<const uint8_t *>(f);
         out.insert(out.end(), p, p + sizeof(f));
         p = reinterpret_cast<const uint8_t *>(&fl);
+        out.insert(out.end(), p, p + 8);
+        p = reinterpret_cast<const uint8_t *>(&top);
         out.insert(out.end(), p, p + 8);
         out.insert(out.end(), hm.sig.begin(), hm.sig.end());
     }
@@ -4518,9 +4746,13 @@ void set_host_state(tpgp_hip_batch *b, const std::vector<uint8_t> &st) {
         at += sizeof(f);
         std::memcpy(&hm.floor, st.data() + at, 8);
         at += 8;
+        std::memcpy(&hm.est_top, st.data() + at, 8);
+        at += 8;
         hm.kcur = (int)f[2];
         hm.nrec = f[3];
         hm.nrd = f[4];
+        hm.gap = (f[5] & 1u) != 0;
+        hm.app_level = f[5] >> 1;
         std::memcpy(hm.sig.data(), st.data() + at, hm.n_spins);
         at += hm.n_spins;
     }

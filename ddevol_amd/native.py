@@ -86,7 +86,7 @@ SLOT_FREE, SLOT_RUNNING, SLOT_DONE, SLOT_FAILED = range(4)
 
 class EvolutionCounters(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("rotations", C.c_uint64), ("updates", C.c_uint64),
-                ("in_out", C.c_uint64)]
+                ("in_out", C.c_uint64), ("in_place", C.c_uint64)]
 
 
 (PROFILE_MERGE, PROFILE_SCATTER, PROFILE_ROTATION, PROFILE_PARTITION, PROFILE_COUNT,
@@ -108,7 +108,7 @@ class HipRecord(C.Structure):
     _fields_ = [("len_in", C.c_uint64), ("len_out", C.c_uint64), ("dropped", C.c_uint64),
                 ("l1_up", C.c_double), ("sq_up", C.c_double), ("rad_sq_up", C.c_double),
                 ("radii", C.c_uint64), ("executed", C.c_uint32), ("nonfinite", C.c_uint32),
-                ("overflow", C.c_uint32), ("rotation", C.c_uint32)]
+                ("overflow", C.c_uint32), ("rotation", C.c_uint32), ("in_place", C.c_uint64)]
 
 
 class HipLogEntry(C.Structure):

@@ -128,10 +128,12 @@ tpgp_status tpgp_hip_evolve(tpgp_map *map, const tpgp_term *terms, size_t count,
 /* The same loop as an incremental session: open uploads the map once; run may be called
  * repeatedly (each call reads the FID before each of its steps and after its last); close
  * downloads the final map into `map` (or discards it for NULL).  Counters: executed rotations,
- * branch-updates (sum of input lengths of executed rotations) and sum of input+output lengths. */
+ * branch-updates (sum of input lengths of executed rotations), sum of input+output lengths, and
+ * twice the terms rotations left in place (classes an appending rotation neither read nor
+ * wrote: in_out - in_place terms were moved). */
 typedef struct tpgp_evolution tpgp_evolution;
 typedef struct {
-    uint64_t steps, rotations, updates, in_out;
+    uint64_t steps, rotations, updates, in_out, in_place;
 } tpgp_evolution_counters;
 tpgp_status tpgp_evolution_open(tpgp_evolution **evolution, const tpgp_map *map,
                                 const tpgp_term *terms, size_t count, double dt,

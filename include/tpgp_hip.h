@@ -113,6 +113,8 @@ typedef struct {
     uint32_t nonfinite;        /* a raw value, merged value or radius was not finite */
     uint32_t overflow;         /* output exceeded the map's slot capacity */
     uint32_t rotation;         /* caller's rotation sequence number */
+    uint64_t in_place;         /* terms of the classes the rotation left where they were in the
+                                  slot (neither read nor written; 0 unless it appended) */
 } tpgp_hip_record;
 
 /* One dropped coefficient or merged radius of a logged rotation.  key is in the reference
