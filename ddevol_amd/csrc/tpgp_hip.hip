@@ -752,6 +752,9 @@ __device__ __forceinline__ unsigned class_tiles(const BatchDev &b, const PassMap
 // its counts); 3 = left in place (such a class of an appending rotation: the split pass records
 // no output for it); 0 = nothing (the higher class of a pair is handled with its partner's
 // tiles).  The work list holds the kind-1 tiles in (map, class) order, then the kind-2 tiles.
+// Kind-3 classes keep their (empty) tiles so that the ledger partials fold in the same chunks as
+// when the class is copied: the upward-rounded ledger sums do not depend on whether a rotation
+// appends.
 __device__ __forceinline__ unsigned tile_kind(const PassMap &P, uint32_t c) {
     if (class_anti(c, P.qt, P.qs)) return (c ^ ((P.qt << 2) | P.qs)) > c ? 1u : 0u;
     if (!(P.flags & kPassCommutingKept)) return 1u;

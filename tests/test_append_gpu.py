@@ -116,6 +116,24 @@ def random_map(rng, n, count, low_digits=None):
     return keys, vals
 
 
+def first_diff(a, c, path=()):
+    """The path and values of the first place where two nested results differ (None: equal)."""
+    if isinstance(a, (list, tuple)) and isinstance(c, (list, tuple)):
+        if len(a) != len(c):
+            return path + ("len",), len(a), len(c)
+        for k, (x, y) in enumerate(zip(a, c)):
+            d = first_diff(x, y, path + (k,))
+            if d:
+                return d
+        return None
+    return None if a == c else (path, a, c)
+
+
+def same(a, c):
+    d = first_diff(a, c)
+    assert d is None, f"appending vs copying differ at {d[0]}: {d[1]!r} != {d[2]!r}"
+
+
 def both(hip_ctx, monkeypatch, n, cap, body):
     """body(batch) with appending on and off; the two result lists must be equal."""
     out = []
@@ -184,7 +202,7 @@ def test_gapped_committed_state(hip_ctx, monkeypatch):
         return out
 
     a, c = both(hip_ctx, monkeypatch, 12, 1_000_000, body)
-    assert a == c
+    same(a, c)
 
 
 def run_segment(b, ops):
@@ -232,7 +250,7 @@ def test_readouts_and_partitions_of_gapped_states(hip_ctx, monkeypatch):
         return out
 
     a, c = both(hip_ctx, monkeypatch, 10, 2_000_000, body)
-    assert a == c
+    same(a, c)
 
 
 def struct_bits(x):
@@ -274,4 +292,4 @@ def test_trajectories_append_equals_copy(hip_ctx, monkeypatch, name):
     print(f"append trajectory {name}: {TRAJ[name]}", flush=True)
     a = trajectory(hip_ctx, monkeypatch, name, True)
     c = trajectory(hip_ctx, monkeypatch, name, False)
-    assert a == c
+    same(a, c)
