@@ -47,7 +47,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public specification
 # 606..625), from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings (factors
 # 2.00 and 1.00): profiles/k_rot_write_j1j2_n16_headline_pmc.md.  Used for roofline.traffic when
 # the run is that window and no --traffic is given; other windows report null.
-MEASURED_TRAFFIC = {("k_rot_write", 605, 20): 533.9e6}
+MEASURED_TRAFFIC = {("k_rot_write", 605, 20): 251.6e6}
 
 
 def log(*a):
