@@ -404,6 +404,11 @@ __device__ __forceinline__ uint64_t phys_at(const Phys &p, uint64_t e) {
     while (c < 15 && e >= p.reg[c + 1]) ++c;
     return e - p.reg[c] + p.reg[kRegPhys + c];
 }
+// phys_at for a run of indices below e1 that `one` (uniform) says lie in the first class: the
+// common case of a tile inside one class costs one add per element.
+__device__ __forceinline__ uint64_t phys_in(const Phys &p, bool one, uint64_t e) {
+    return one ? e + p.delta : phys_at(p, e);
+}
 // The region row of a contiguous map of `len` elements whose logical class starts are in
 // reg[0..16] (reg[16] == len): physical starts equal to them, extent len.
 __device__ __forceinline__ void contiguous_row(uint64_t *reg, unsigned c, uint64_t start) {
@@ -539,11 +544,12 @@ __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap
         const uint64_t *keys = b.arena[P.in_buf].keys + b.slot[m].off * W;
         const uint32_t pa = P.pa, pb = P.pb;  // read once
         const Phys ph = phys_from(region_row(b.regions, m, P.in_buf), e0);
+        const bool one = e1 <= ph.hi;  // the tile lies in one class
         Key<W> k[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {  // branch-free: past the end, reload the last
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
-            k[r] = kload<W>(keys + phys_at(ph, e < e1 ? e : e1 - 1) * W);
+            k[r] = kload<W>(keys + phys_in(ph, one, e < e1 ? e : e1 - 1) * W);
         }
         uint64_t cnt = 0;  // 16 bins x 4 bits
 #pragma unroll
@@ -641,12 +647,13 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b,
             tile_off = b.tile_base[g * 16 + threadIdx.x];
         }
         const Phys ph = phys_from(region_row(b.regions, m, P.in_buf), e0);
+        const bool one = e1 <= ph.hi;  // the tile lies in one class
         Key<W> kk[R];
         double vv[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {  // branch-free: past the end, reload the last
             const uint64_t e = e0 + threadIdx.x + (uint64_t)r * kBlock;
-            const uint64_t ec = phys_at(ph, e < e1 ? e : e1 - 1);
+            const uint64_t ec = phys_in(ph, one, e < e1 ? e : e1 - 1);
             kk[r] = kload<W>(keys + ec * W);
             vv[r] = vals[ec];
         }
@@ -1517,69 +1524,73 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan1(BatchDev b, const PassMap 
 // The end of a map's rotation scan (whole block, chunk_base and tile_base of the map written and
 // published): output regions and length, overflow, and the record with the map's ledger T.
 // base: the slot index of the first written output (the input's used extent when appending,
-// else 0), already included in chunk_base.
+// else 0), already included in chunk_base.  Lanes 0-15 of the first wave take one class each.
 __device__ void finish_map_scan(const BatchDev &b, const PassMap &P, uint32_t m, uint64_t acc,
-                                uint64_t base, const TileLedger &T, uint64_t *cstart /*[16]*/) {
-    // output class c starts where its first tile writes; an empty class starts where the
-    // next nonempty class does (or at the end)
-    if (threadIdx.x < 16) {
-        const uint64_t tm0 = b.tstart[m * 16], c0 = b.chunk_start[m];
-        const uint64_t t = b.tstart[m * 16 + threadIdx.x], te = b.tstart[m * 16 + threadIdx.x + 1];
-        cstart[threadIdx.x] =
-            t < te ? b.chunk_base[c0 + (t - tm0) / kChunkTiles] + b.tile_base[t] : ~0ull;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        // appending: the input and output rows are the same words, so the input row is read
-        // whole before any of it is rewritten
+                                uint64_t base, const TileLedger &T) {
+    if (threadIdx.x < 64) {
+        const unsigned c = lane_id();
+        const bool mine = c < 16;
         const uint64_t *rin = region_row(b.regions, m, P.in_buf);
         uint64_t *rout = region_row(b.regions, m, P.out_buf);
         const bool append = (P.flags & kPassAppend) != 0;
-        uint64_t lin[17], pin[16];
-        for (int c = 0; c < 16; ++c) {
-            lin[c] = rin[c];
-            pin[c] = rin[kRegPhys + c];
+        // the input row, read before any of it is rewritten (appending: the same words)
+        uint64_t lo = 0, hi = 0, ph = 0, ws = ~0ull;
+        if (mine) {
+            lo = rin[c];
+            hi = rin[c + 1];
+            ph = rin[kRegPhys + c];
+            const uint64_t tm0 = b.tstart[m * 16], ch0 = b.chunk_start[m];
+            const uint64_t t = b.tstart[m * 16 + c], te = b.tstart[m * 16 + c + 1];
+            if (t < te)  // where class c's first tile writes, within the written output
+                ws = b.chunk_base[ch0 + (t - tm0) / kChunkTiles] + b.tile_base[t] - base;
         }
-        lin[16] = rin[16];
-        // written output of class c: positions wr[c] .. wr[c+1] of the acc written terms
-        uint64_t wr[17];
-        uint64_t next = acc;
-        wr[16] = acc;
-        for (int c = 15; c >= 0; --c) {
-            if (cstart[c] != ~0ull) next = cstart[c] - base;
-            wr[c] = next;
+        const uint64_t len_in = rin[16];
+        // written output of class c: positions wr .. wr_next of the acc written terms; a class
+        // without tiles starts where the next class with tiles does (offsets never decrease),
+        // so wr is the minimum over classes c..15 (acc where none)
+        uint64_t wr = ws < acc ? ws : acc;
+        for (unsigned d = 1; d < 16; d <<= 1) {
+            const uint64_t o = __shfl_down(wr, d, 64);
+            if (c + d < 16) wr = o < wr ? o : wr;
         }
-        const uint64_t end = base + acc;
+        uint64_t wr_next = __shfl_down(wr, 1, 64);
+        if (c == 15) wr_next = acc;
+        const bool in_place = mine && append && tile_kind(P, c) == 3u;
+        const uint64_t n = !mine ? 0 : in_place ? hi - lo : wr_next - wr;
+        uint64_t inc = n, kept = in_place ? n : 0;  // kept: terms of the classes left in place
+        for (unsigned d = 1; d < 16; d <<= 1) {
+            const uint64_t o = __shfl_up(inc, d, 64);
+            if (c >= d) inc += o;
+        }
+        for (unsigned d = 8; d >= 1; d >>= 1) kept += __shfl_xor(kept, d, 64);
+        const uint64_t at = inc - n, total = __shfl(inc, 15, 64), end = base + acc;
         // A map that outgrew its slot is emptied: later passes never read past the slot; its
         // record carries the overflow and the host rolls the map back and grows the slot.
         const bool over = b.overflow[m] || end > b.slot[m].cap;
-        uint64_t at = 0, kept = 0;  // kept: terms of the classes left in place
-        for (int c = 0; c < 16; ++c) {
-            const bool in_place = append && tile_kind(P, (uint32_t)c) == 3u;
+        if (mine) {
             rout[c] = over ? 0 : at;
-            rout[kRegPhys + c] = over ? 0 : in_place ? pin[c] : base + wr[c];
-            at += in_place ? lin[c + 1] - lin[c] : wr[c + 1] - wr[c];
-            kept += in_place ? lin[c + 1] - lin[c] : 0;
+            rout[kRegPhys + c] = over ? 0 : in_place ? ph : base + wr;
         }
-        rout[16] = over ? 0 : at;
-        rout[kRegTop] = over ? 0 : end;
-        const uint64_t len_in = lin[16];
-        b.len[m * 3 + P.out_buf] = over ? 0 : at;
-        if (over) b.overflow[m] = 1;
-        tpgp_hip_record r;
-        r.len_in = len_in;
-        r.len_out = at;
-        r.dropped = T.dropped;
-        r.l1_up = T.l1;
-        r.sq_up = T.sq;
-        r.rad_sq_up = T.rsq;
-        r.radii = T.radii;
-        r.executed = len_in > 0;
-        r.nonfinite = T.nonfinite;
-        r.overflow = over ? 1u : 0u;
-        r.rotation = P.rot_seq;
-        r.in_place = kept;
-        b.rec[(uint64_t)m * b.max_rec + P.rec_slot] = r;
+        if (c == 0) {
+            rout[16] = over ? 0 : total;
+            rout[kRegTop] = over ? 0 : end;
+            b.len[m * 3 + P.out_buf] = over ? 0 : total;
+            if (over) b.overflow[m] = 1;
+            tpgp_hip_record r;
+            r.len_in = len_in;
+            r.len_out = total;
+            r.dropped = T.dropped;
+            r.l1_up = T.l1;
+            r.sq_up = T.sq;
+            r.rad_sq_up = T.rsq;
+            r.radii = T.radii;
+            r.executed = len_in > 0;
+            r.nonfinite = T.nonfinite;
+            r.overflow = over ? 1u : 0u;
+            r.rotation = P.rot_seq;
+            r.in_place = kept;
+            b.rec[(uint64_t)m * b.max_rec + P.rec_slot] = r;
+        }
     }
     __syncthreads();
 }
@@ -1592,7 +1603,6 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
     if (!P.active) return;
     __shared__ unsigned scratch[8];
     __shared__ TileLedger sh[kBlock / 64];
-    __shared__ uint64_t cstart[16];
     const uint64_t c0 = b.chunk_start[m], c1 = b.chunk_start[m + 1];
     // the outputs start at the input's used extent when appending (read before
     // finish_map_scan rewrites the row, which is the input's own when appending)
@@ -1611,7 +1621,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
         carry += tot;
     }
     const TileLedger T = block_ledger(L, sh);  // (its barriers publish chunk_base too)
-    finish_map_scan(b, P, m, carry, obase, T, cstart);
+    finish_map_scan(b, P, m, carry, obase, T);
 }
 
 // The whole rotation scan of small maps in one launch, one block per map: its chunks one after
@@ -1621,7 +1631,6 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
 __global__ __launch_bounds__(kBlock) void k_rot_scan_maps(BatchDev b, const PassMap *pm) {
     __shared__ unsigned scratch[8];
     __shared__ TileLedger sh[kBlock / 64];
-    __shared__ uint64_t cstart[16];
     for (uint32_t m = blockIdx.x; m < b.n_maps; m += gridDim.x) {
         const PassMap &P = pm[m];
         if (!P.active) continue;
@@ -1645,7 +1654,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan_maps(BatchDev b, const Pass
             carry += tot;
         }
         const TileLedger T = block_ledger(L, sh);  // (its barriers publish the offsets too)
-        finish_map_scan(b, P, m, carry, obase, T, cstart);
+        finish_map_scan(b, P, m, carry, obase, T);
     }
 }
 
