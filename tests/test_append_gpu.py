@@ -253,6 +253,54 @@ def test_readouts_and_partitions_of_gapped_states(hip_ctx, monkeypatch):
     same(a, c)
 
 
+@pytest.mark.parametrize("cut,limit,lds", [(1.4, 5000, True), (1.35, 8000, False)])
+def test_gapped_state_onto_the_small_map_path(hip_ctx, monkeypatch, cut, limit, lds):
+    """A committed state left by an appending rotation is the input of the small-map kernels
+    (the LDS kernel loads it class by class when it fits the LDS capacity of 4096 one-word
+    terms, the global one runs from it when it does not): with a small-path bound of `limit`
+    terms, a call that truncates most of the map and then appends, a second call that copies and
+    appends (its length estimate falls to the bound), then calls on the small path."""
+    rng = np.random.default_rng(13)
+    keys, vals = random_map(rng, 12, 20_000)
+
+    def body(b):
+        out = []
+        b.upload(keys, vals)
+        calls = [[(0, 1, Z, Z, 0.3, cut), (0, 1, X, X, 0.3, 1e-12)],
+                 [(0, 1, Y, Y, 0.2, 1e-12), (0, 1, Z, Z, 0.2, 1e-12)],
+                 [(1, 2, X, X, 0.25, 1e-12), (1, 2, Y, Y, 0.25, 1e-12), (3, 4, Z, Z, 0.1, 1e-12)],
+                 [(2, 5, X, X, 0.25, 1e-12), (0, 2, Y, Y, 0.15, 1e-12)]]
+        for rots in calls:
+            out.append(run_segment(b, [lambda b, x=rots: b.rotate(x), lambda b: b.readout(X)]))
+            out.append(b.sum_squares())
+        k, v = b.download()
+        out.append((k.tolist(), v.tolist()))
+        return out
+
+    monkeypatch.setenv("TPGP_HIP_SMALL_LIMIT", str(limit))
+    out = []
+    for append in (True, False):
+        monkeypatch.setenv("TPGP_HIP_APPEND", "1" if append else "0")
+        monkeypatch.setenv("TPGP_HIP_VALIDATE", "1")
+        L = N.hip()
+        err = C.create_string_buffer(512)
+        b = Batch.__new__(Batch)
+        b.L, b.err, b.seq = L, err, 0
+        b.b = L.tpgp_hip_batch_create(hip_ctx.ptr, 1, 1, (C.c_size_t * 1)(12),
+                                      (C.c_size_t * 1)(400_000), err, 512)
+        assert b.b, err.value.decode()
+        try:
+            out.append(body(b))
+        finally:
+            b.close()
+    a, c = out
+    lens = [[r[0] for r in seg[0]] for seg in a[0:8:2]]
+    print(f"cut {cut}: input lengths per call {lens}", flush=True)
+    # the third call starts on the small path from the second call's appended state
+    assert lens[2][0] <= limit and (lens[2][0] <= 4096) == lds
+    same(a, c)
+
+
 def struct_bits(x):
     return int(np.float64(x).view(np.uint64))
 
