@@ -346,8 +346,8 @@ struct BatchDev {
     uint32_t max_rec, max_rd;
     // scratch
     uint64_t *tstart;         // partition [M+1] / rotation [M*16+1]
-    uint32_t *tile_hist;      // [maxT*16]
-    uint64_t *tile_base;      // [maxT*16]
+    uint32_t *tile_hist;      // [maxT*16] rotation: per tile; partition: class-major (class c
+    uint64_t *tile_base;      // [maxT*16]   of tile g at c * ntiles + g)
     uint64_t *map_hist;       // [M*16]
     uint64_t *splits;         // [maxT+1]
     struct TileDesc *tdesc;   // [maxT] rotation tile descriptors (written by the split pass)
@@ -576,7 +576,7 @@ __global__ __launch_bounds__(kBlock) void k_part_count(BatchDev b, const PassMap
             unsigned c = 0;
 #pragma unroll
             for (int w = 0; w < kBlock / 64; ++w) c += (whist[w][j] >> sh) & 0xffffu;
-            b.tile_hist[g * 16 + threadIdx.x] = c;
+            b.tile_hist[(uint64_t)threadIdx.x * ntiles + g] = c;  // class-major
         }
         __syncthreads();
     }
@@ -591,21 +591,24 @@ __global__ __launch_bounds__(kBlock) void k_part_scan(BatchDev b, const PassMap 
     const uint32_t m = blockIdx.x >> 4, cls = blockIdx.x & 15;
     const PassMap &P = pm[m];
     if (!P.active) return;
-    const uint64_t t0 = b.tstart[m], t1 = b.tstart[m + 1];
+    const uint64_t t0 = b.tstart[m], t1 = b.tstart[m + 1], ntiles = b.tstart[b.n_maps];
+    // this class's counts and offsets over all tiles (class-major: contiguous per class)
+    const unsigned *row_hist = b.tile_hist + (uint64_t)cls * ntiles;
+    uint64_t *row_base = b.tile_base + (uint64_t)cls * ntiles;
     uint64_t carry = 0;
     for (uint64_t base = t0; base < t1; base += (uint64_t)kBlock * kScanPer) {
         const uint64_t g0 = base + (uint64_t)threadIdx.x * kScanPer;
         unsigned v[kScanPer], sum = 0;
 #pragma unroll
         for (int i = 0; i < kScanPer; ++i) {
-            v[i] = g0 + i < t1 ? b.tile_hist[(g0 + i) * 16 + cls] : 0u;
+            v[i] = g0 + i < t1 ? row_hist[g0 + i] : 0u;
             sum += v[i];
         }
         unsigned tot;
         unsigned ex = block_exclusive_scan(sum, scratch, &tot);
 #pragma unroll
         for (int i = 0; i < kScanPer; ++i) {
-            if (g0 + i < t1) b.tile_base[(g0 + i) * 16 + cls] = carry + ex;
+            if (g0 + i < t1) row_base[g0 + i] = carry + ex;
             ex += v[i];
         }
         carry += tot;
@@ -644,7 +647,7 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(BatchDev b,
         uint64_t tile_off = 0;
         if (threadIdx.x < 16) {
             class_total[threadIdx.x] = b.map_hist[m * 16 + threadIdx.x];
-            tile_off = b.tile_base[g * 16 + threadIdx.x];
+            tile_off = b.tile_base[(uint64_t)threadIdx.x * ntiles + g];
         }
         const Phys ph = phys_from(region_row(b.regions, m, P.in_buf), e0);
         const bool one = e1 <= ph.hi;  // the tile lies in one class
