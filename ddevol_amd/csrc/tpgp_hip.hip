@@ -2079,6 +2079,11 @@ __device__ __forceinline__ void small_partition(const BatchDev &b, const PassMap
 // pass in small_resume.
 // ------------------------------------------------------------------------------------------
 constexpr int kLdsThreads = 512;  // 8 waves: one workgroup per CU (its LDS holds the map)
+// A rotation's work is cut into equal shares of weighted merge positions: a position of an
+// anticommuting pair (a merge step, two outputs) weighs four of a commuting class (one
+// output).  Equal shares of plain positions left the pairs' threads slower than the rest at
+// every barrier.
+constexpr uint32_t kAntiWeight = 4;
 template <int W> struct LdsCap;
 template <> struct LdsCap<1> { static constexpr uint32_t value = 4096; };
 template <> struct LdsCap<2> { static constexpr uint32_t value = 2816; };
@@ -2090,8 +2095,9 @@ template <int W> struct LdsMap {
     uint16_t cnt[16][kLdsThreads];  // per (output class, thread): kept count, then offset
     uint32_t reg[2][17];            // class regions of each buffer
     uint32_t len[2];
-    uint32_t seg[17];               // rotation work: start of the work of class c (the upper
-                                    // class of an anticommuting pair has none of its own)
+    uint32_t seg[17];               // rotation work: start of the work of class c in weighted
+                                    // positions (the upper class of an anticommuting pair has
+                                    // none of its own)
     uint32_t total;
     unsigned scratch[kLdsThreads / 64];
     TileLedger red[kLdsThreads / 64];
@@ -2311,7 +2317,11 @@ __device__ __forceinline__ uint32_t lds_rotation_walk(const LdsMap<W> &sh, int c
         unsigned c = 0;  // the segment holding g: the number of segment ends at or below g
 #pragma unroll
         for (int k = 1; k < 16; ++k) c += seg[k] <= g ? 1u : 0u;
-        const uint32_t d0 = g - seg[c], d1 = min(w1, seg[c + 1]) - seg[c];
+        // this thread's share of the segment, in weighted units, then in merge positions (an
+        // anticommuting pair's positions weigh kAntiWeight each)
+        const uint32_t wt = ((anti >> c) & 1u) ? kAntiWeight : 1u;
+        const uint32_t o1 = min(w1, seg[c + 1]) - seg[c];
+        const uint32_t d0 = (g - seg[c] + wt - 1) / wt, d1 = (o1 + wt - 1) / wt;
         const uint32_t A0 = reg[c], nA = reg[c + 1] - reg[c];
         if (!((anti >> c) & 1u)) {
             begin(c, c);
@@ -2376,7 +2386,7 @@ __device__ __forceinline__ uint32_t lds_rotation_walk(const LdsMap<W> &sh, int c
                 used += (takeA ? 1u : 0u) + (takeB ? 1u : 0u);
             }
         }
-        g = seg[c] + d1;
+        g = seg[c] + o1;
     }
     return split;
 }
@@ -2420,7 +2430,7 @@ __device__ bool lds_rotation(const LdsSink &out_to, const PassMap &P, uint32_t m
         if (c < 16) {
             const uint32_t nc = rin[c + 1] - rin[c];
             if (!((kc.anti >> c) & 1u)) size = nc;
-            else if ((c ^ qq) > c) size = nc + (rin[(c ^ qq) + 1] - rin[c ^ qq]);
+            else if ((c ^ qq) > c) size = (nc + (rin[(c ^ qq) + 1] - rin[c ^ qq])) * kAntiWeight;
         }
         const uint32_t incl = wave_scan_u32(size);
         if (c < 16) sh.seg[c + 1] = incl;
