@@ -2095,9 +2095,11 @@ template <int W> struct LdsMap {
     uint16_t cnt[16][kLdsThreads];  // per (output class, thread): kept count, then offset
     uint32_t reg[2][17];            // class regions of each buffer
     uint32_t len[2];
-    uint32_t seg[17];               // rotation work: start of the work of class c in weighted
-                                    // positions (the upper class of an anticommuting pair has
-                                    // none of its own)
+    uint32_t seg[17];               // rotation work: start of the work of the k-th segment in
+                                    // weighted positions
+    uint32_t sord[16];              // class of the k-th segment: the commuting classes, then the
+                                    // lower classes of the anticommuting pairs, then the upper
+                                    // ones (no work of their own), each in class order
     uint32_t total;
     unsigned scratch[kLdsThreads / 64];
     TileLedger red[kLdsThreads / 64];
@@ -2310,18 +2312,22 @@ __device__ __forceinline__ uint32_t lds_rotation_walk(const LdsMap<W> &sh, int c
     const double *vals = sh.vals[cur];
     const uint32_t *reg = sh.reg[cur];
     uint32_t seg[17];
+    unsigned sord[16];
 #pragma unroll
     for (int k = 0; k < 17; ++k) seg[k] = sh.seg[k];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) sord[k] = sh.sord[k];
     uint32_t g = w0;
     while (g < w1) {
-        unsigned c = 0;  // the segment holding g: the number of segment ends at or below g
+        unsigned s = 0;  // the segment holding g: the number of segment ends at or below g
 #pragma unroll
-        for (int k = 1; k < 16; ++k) c += seg[k] <= g ? 1u : 0u;
+        for (int k = 1; k < 16; ++k) s += seg[k] <= g ? 1u : 0u;
+        const unsigned c = sord[s];
         // this thread's share of the segment, in weighted units, then in merge positions (an
         // anticommuting pair's positions weigh kAntiWeight each)
         const uint32_t wt = ((anti >> c) & 1u) ? kAntiWeight : 1u;
-        const uint32_t o1 = min(w1, seg[c + 1]) - seg[c];
-        const uint32_t d0 = (g - seg[c] + wt - 1) / wt, d1 = (o1 + wt - 1) / wt;
+        const uint32_t o1 = min(w1, seg[s + 1]) - seg[s];
+        const uint32_t d0 = (g - seg[s] + wt - 1) / wt, d1 = (o1 + wt - 1) / wt;
         const uint32_t A0 = reg[c], nA = reg[c + 1] - reg[c];
         if (!((anti >> c) & 1u)) {
             begin(c, c);
@@ -2386,7 +2392,7 @@ __device__ __forceinline__ uint32_t lds_rotation_walk(const LdsMap<W> &sh, int c
                 used += (takeA ? 1u : 0u) + (takeB ? 1u : 0u);
             }
         }
-        g = seg[c] + o1;
+        g = seg[s] + o1;
     }
     return split;
 }
@@ -2424,16 +2430,39 @@ __device__ bool lds_rotation(const LdsSink &out_to, const PassMap &P, uint32_t m
     const unsigned qq = (P.qt << 2) | P.qs;
     const ClassCoefs kc = class_coefs(P);
     const uint32_t *rin = sh.reg[cur];
-    if (threadIdx.x < 64) {  // work segments: sizes on lanes 0-15, one wave-wide scan
+    if (threadIdx.x < 64) {  // work segments: sizes on lanes 0-15, ordered, one wave-wide scan
         const unsigned c = threadIdx.x;
         uint32_t size = 0;
+        unsigned kind = 3;  // 0 commuting, 1 lower class of a pair, 2 upper class, 3 no class
         if (c < 16) {
             const uint32_t nc = rin[c + 1] - rin[c];
-            if (!((kc.anti >> c) & 1u)) size = nc;
-            else if ((c ^ qq) > c) size = (nc + (rin[(c ^ qq) + 1] - rin[c ^ qq])) * kAntiWeight;
+            if (!((kc.anti >> c) & 1u)) {
+                size = nc;
+                kind = 0;
+            } else if ((c ^ qq) > c) {
+                size = (nc + (rin[(c ^ qq) + 1] - rin[c ^ qq])) * kAntiWeight;
+                kind = 1;
+            } else {
+                kind = 2;
+            }
         }
-        const uint32_t incl = wave_scan_u32(size);
-        if (c < 16) sh.seg[c + 1] = incl;
+        // a wave's lanes walking both kinds of segment run both loops one after the other, so
+        // the kinds are not interleaved: the commuting classes first, then the pairs
+        const uint64_t m0 = __ballot(kind == 0), m1 = __ballot(kind == 1), m2 = __ballot(kind == 2);
+        const uint64_t below = (1ull << c) - 1ull;
+        const unsigned n0 = (unsigned)__popcll(m0), n1 = (unsigned)__popcll(m1);
+        const unsigned rank = kind == 0 ? (unsigned)__popcll(m0 & below)
+                              : kind == 1 ? n0 + (unsigned)__popcll(m1 & below)
+                                          : n0 + n1 + (unsigned)__popcll(m2 & below);
+        // lane r receives the size and class of the segment of rank r
+        const int to = (int)(c < 16 ? rank : c) * 4;
+        const uint32_t rsize = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)size);
+        const unsigned rclass = (unsigned)__builtin_amdgcn_ds_permute(to, (int)c);
+        const uint32_t incl = wave_scan_u32(c < 16 ? rsize : 0u);
+        if (c < 16) {
+            sh.seg[c + 1] = incl;
+            sh.sord[c] = rclass;
+        }
         if (c == 0) sh.seg[0] = 0;
     }
 #pragma unroll
