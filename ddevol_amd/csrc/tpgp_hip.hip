@@ -1484,44 +1484,87 @@ __device__ TileLedger block_ledger(TileLedger a, TileLedger *sh /*[kBlock / 64]*
     return r;
 }
 
+// Sum of one uint64_t per thread over the block; every thread gets it.
+__device__ uint64_t block_sum_u64(uint64_t v, uint64_t *sh /*[kBlock / 64]*/) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (lane_id() == 0) sh[wave_id()] = v;
+    __syncthreads();
+    uint64_t s = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) s += sh[w];
+    __syncthreads();
+    return s;
+}
+
 // Scan level 1, one block per chunk of kChunkTiles tiles of one map: tile offsets within the
 // chunk (exclusive scan of the kept counts), the chunk total and the chunk's ledger partials.
-__global__ __launch_bounds__(kBlock) void k_rot_scan1(BatchDev b, const PassMap *pm) {
+// direct: the block also sums the kept counts of all the map's tiles before its chunk and writes
+// the chunk's output offset (chunk_base, with the rotation's output base) itself, so the write
+// pass can start without the map-level scan; the rest of that scan (regions, ledger, record)
+// runs inside the write pass (k_rot_write with finish).  Every block reads the counts of the
+// map's earlier tiles (4 KB per earlier chunk, in the same round of loads as its own tiles), so
+// the form is used for maps of at most kDirectScanChunks chunks.
+constexpr uint64_t kDirectScanChunks = 128;
+__global__ __launch_bounds__(kBlock) void k_rot_scan1(BatchDev b, const PassMap *pm, int direct) {
+    static_assert(kChunkTiles == kBlock, "one tile per thread");
     __shared__ unsigned scratch[8];
     __shared__ TileLedger sh[kBlock / 64];
+    __shared__ uint64_t sh64[kBlock / 64];
     const uint32_t M = b.n_maps;
     const uint64_t nchunks = b.chunk_start[M];
     for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
         const uint32_t m = find_bucket(b.chunk_start, M, ch);
-        const uint64_t t1 = b.tstart[m * 16 + 16];
-        const uint64_t g0 = b.tstart[m * 16] + (ch - b.chunk_start[m]) * kChunkTiles;
-        constexpr int per = kChunkTiles / kBlock;
-        const uint64_t mine0 = g0 + (uint64_t)threadIdx.x * per;
-        unsigned cnt[per];
-        unsigned sum = 0;
-#pragma unroll
-        for (int i = 0; i < per; ++i) {
-            cnt[i] = mine0 + i < t1 ? b.tile_hist[mine0 + i] : 0u;
-            sum += cnt[i];
-        }
-        unsigned tot;
-        unsigned at = block_exclusive_scan(sum, scratch, &tot);
+        const uint64_t t1 = b.tstart[m * 16 + 16], tm = b.tstart[m * 16];
+        const uint64_t g0 = tm + (ch - b.chunk_start[m]) * kChunkTiles;
+        const uint64_t g = g0 + threadIdx.x;
+        const unsigned cnt = g < t1 ? b.tile_hist[g] : 0u;
         TileLedger L{};
+        if (g < t1) ledger_fold(L, b.tile_led[g]);
+        uint64_t pre = 0;
+        if (direct) {
+            // counts of tiles tm .. g0 - 1 by 16-byte loads, entries outside the range masked
+            // (tile_hist holds 16 entries per tile of the allocation: a load reaching up to 3
+            // entries past g0 stays inside)
+            const uint4 *h4 = reinterpret_cast<const uint4 *>(b.tile_hist);
+            const uint64_t q1 = (g0 + 3) >> 2;
+            constexpr int R = 16;
+            for (uint64_t qb = tm >> 2; qb < q1; qb += (uint64_t)R * kBlock) {
+                uint4 v[R];
 #pragma unroll
-        for (int i = 0; i < per; ++i) {
-            if (mine0 + i < t1) {
-                b.tile_base[mine0 + i] = at;
-                at += cnt[i];
-                ledger_fold(L, b.tile_led[mine0 + i]);
+                for (int r = 0; r < R; ++r) {
+                    const uint64_t q = qb + (uint64_t)r * kBlock + threadIdx.x;
+                    v[r] = q < q1 ? h4[q] : make_uint4(0u, 0u, 0u, 0u);
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint64_t j = (qb + (uint64_t)r * kBlock + threadIdx.x) * 4;
+                    pre += (j >= tm && j < g0 ? v[r].x : 0u) +
+                           (j + 1 >= tm && j + 1 < g0 ? v[r].y : 0u) +
+                           (j + 2 >= tm && j + 2 < g0 ? v[r].z : 0u) +
+                           (j + 3 >= tm && j + 3 < g0 ? v[r].w : 0u);
+                }
             }
         }
-        TileLedger C = block_ledger(L, sh);
+        unsigned tot;
+        const unsigned at = block_exclusive_scan(cnt, scratch, &tot);
+        if (g < t1) b.tile_base[g] = at;
+        const TileLedger C = block_ledger(L, sh);
         if (threadIdx.x == 0) {
             b.chunk_sum[ch] = tot;
             b.chunk_led[ch] = C;
         }
+        if (direct) {
+            pre = block_sum_u64(pre, sh64);
+            if (threadIdx.x == 0) {
+                const PassMap &P = pm[m];
+                // outputs start at the input's used extent when appending (the row is
+                // rewritten only by the write pass, after this launch)
+                const uint64_t obase =
+                    (P.flags & kPassAppend) ? region_row(b.regions, m, P.in_buf)[kRegTop] : 0;
+                b.chunk_base[ch] = obase + pre;
+            }
+        }
     }
-    (void)pm;
 }
 
 // The end of a map's rotation scan (whole block, chunk_base and tile_base of the map written and
@@ -1627,6 +1670,33 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
     finish_map_scan(b, P, m, carry, obase, T);
 }
 
+// The map-level end of a direct rotation scan (k_rot_scan1 with direct has written every
+// chunk's offset): the written total and the ledger folded from the chunk partials in
+// k_rot_scan2's order (the same bits), then finish_map_scan.  Run by the write pass's first
+// blocks (block m for map m) before their tiles: no later block of that launch reads what it
+// writes (the region rows, lengths and records), and a map found to overflow here has its
+// tiles written only where they fit the slot, then emptied like any overflowed map.
+__device__ void finish_direct(const BatchDev &b, const PassMap *pm, uint64_t *sh64,
+                              TileLedger *sh) {
+    for (uint32_t m = blockIdx.x; m < b.n_maps; m += gridDim.x) {
+        const PassMap &P = pm[m];
+        if (!P.active) continue;
+        const uint64_t c0 = b.chunk_start[m], c1 = b.chunk_start[m + 1];
+        // read before finish_map_scan rewrites the row (the input's own when appending)
+        const uint64_t obase =
+            (P.flags & kPassAppend) ? region_row(b.regions, m, P.in_buf)[kRegTop] : 0;
+        uint64_t acc = 0;
+        TileLedger L{};
+        for (uint64_t ch = c0 + threadIdx.x; ch < c1; ch += kBlock) {
+            acc += b.chunk_sum[ch];
+            ledger_fold(L, b.chunk_led[ch]);
+        }
+        acc = block_sum_u64(acc, sh64);
+        const TileLedger T = block_ledger(L, sh);
+        finish_map_scan(b, P, m, acc, obase, T);
+    }
+}
+
 // The whole rotation scan of small maps in one launch, one block per map: its chunks one after
 // another (tile offsets within the chunk, chunk offsets), the ledger folded per thread across
 // the chunks, then the map's regions and record.  Correct for any size; used when the launch's
@@ -1672,9 +1742,14 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan_maps(BatchDev b, const Pass
 // it takes 74 and six; eight would spill); two-word keys are left to the compiler.
 template <int W>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W == 1 ? 7 : 1, 8)))
-void k_rot_write(BatchDev b, const PassMap *__restrict__ pm) {
+void k_rot_write(BatchDev b, const PassMap *__restrict__ pm, int finish) {
     __shared__ TileIn<W> in;
     __shared__ unsigned scratch[8];
+    if (finish) {  // the map-level scan of a direct k_rot_scan1, overlapped with the tiles
+        __shared__ TileLedger sh[kBlock / 64];
+        __shared__ uint64_t sh64[kBlock / 64];
+        finish_direct(b, pm, sh64, sh);
+    }
     const uint64_t nwork = b.wstart[b.n_maps * 16 + 1];  // kind-1 and kind-2 tiles
     uint64_t gn = blockIdx.x < nwork ? b.work[blockIdx.x] : 0;
     for (uint64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -3281,6 +3356,9 @@ struct tpgp_hip_batch {
     bool graphs_on = true;
     // appending rotations (TPGP_HIP_APPEND=0 copies every class instead)
     bool append_on = true;
+    // the rotation scan's form by size (0), or one form for every rotation (TPGP_HIP_SCAN =
+    // maps | levels | direct: 1, 2, 3), for tests
+    int scan_form = 0;
     bool capturing = false, capture_spilled = false;
     std::vector<uint8_t> cap_host;            // descriptors staged while capturing
     std::vector<std::vector<uint8_t>> spill;  // staging beyond cap_host (capture is dropped)
@@ -3771,6 +3849,9 @@ extern "C" tpgp_hip_batch *tpgp_hip_batch_create(tpgp_hip_context *ctx, size_t n
     b->graphs_on = !(gr && *gr == '0');
     const char *ap = getenv("TPGP_HIP_APPEND");
     b->append_on = !(ap && *ap == '0');
+    const char *sf = getenv("TPGP_HIP_SCAN");
+    b->scan_form = !sf ? 0 : !strcmp(sf, "maps") ? 1 : !strcmp(sf, "levels") ? 2
+                 : !strcmp(sf, "direct") ? 3 : 0;
     std::lock_guard<std::recursive_mutex> lock(ctx->mutex);
     DeviceGuard guard(ctx->device);
     if (!guard.ok || allocate_device(b, error, cap) != TPGP_HIP_OK) {
@@ -3985,14 +4066,25 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
                 hipLaunchKernelGGL((k_rot_count<2, false>), dim3(cgrid), dim3(kBlock), 0, st,
                                    b->dev, d);
         }
+        int finish = 0;  // the write pass ends the map-level scan (direct chunk offsets)
         {
             ProfileScope psn(b, TPGP_HIP_PROFILE_SCAN);
             const uint64_t want = est_tiles / kChunkTiles + b->M + 1;
             const unsigned cgrid = (unsigned)(want < (1u << 20) ? want : (1u << 20));
-            if (est_tiles <= (uint64_t)kSmallScanTiles * b->M) {
+            // mid-size rotations scan with direct chunk offsets (k_rot_scan1 with direct, the
+            // map-level end inside the write pass): one launch less than two levels
+            const int form =
+                b->scan_form                                               ? b->scan_form
+                : est_tiles <= (uint64_t)kSmallScanTiles * b->M            ? 1
+                : est_tiles <= kDirectScanChunks * kChunkTiles - b->M * 16 ? 3
+                                                                           : 2;
+            if (form == 1) {
                 hipLaunchKernelGGL(k_rot_scan_maps, dim3(b->M), dim3(kBlock), 0, st, b->dev, d);
+            } else if (form == 3) {
+                finish = 1;
+                hipLaunchKernelGGL(k_rot_scan1, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d, 1);
             } else {
-                hipLaunchKernelGGL(k_rot_scan1, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d);
+                hipLaunchKernelGGL(k_rot_scan1, dim3(cgrid), dim3(kBlock), 0, st, b->dev, d, 0);
                 hipLaunchKernelGGL(k_rot_scan2, dim3(b->M), dim3(kBlock), 0, st, b->dev, d);
             }
         }
@@ -4003,9 +4095,11 @@ void launch_rotation(tpgp_hip_batch *b, const PassMap *d, unsigned rgrid, bool v
             const unsigned wgrid = tw ? (unsigned)((uint64_t)rgrid * tw / 6) + 64
                                       : rgrid / 3 * 2 + 64;
             if (b->W == 1)
-                hipLaunchKernelGGL(k_rot_write<1>, dim3(wgrid), dim3(kBlock), 0, st, b->dev, d);
+                hipLaunchKernelGGL(k_rot_write<1>, dim3(wgrid), dim3(kBlock), 0, st, b->dev, d,
+                                   finish);
             else
-                hipLaunchKernelGGL(k_rot_write<2>, dim3(wgrid), dim3(kBlock), 0, st, b->dev, d);
+                hipLaunchKernelGGL(k_rot_write<2>, dim3(wgrid), dim3(kBlock), 0, st, b->dev, d,
+                                   finish);
         }
     }
     if (b->validate) {
