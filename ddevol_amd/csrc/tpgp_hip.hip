@@ -853,6 +853,27 @@ __device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm,
     x.in_buf = P.in_buf;
 }
 
+// locate, with the classes' physical starts and lengths (per (map, class): phys, clen) and the
+// tile offsets already in LDS: no region-table loads.
+__device__ __forceinline__ void locate_planned(const BatchDev &b, const PassMap *pm,
+                                               const uint64_t *tstart, const uint64_t *phys,
+                                               const uint64_t *clen, uint64_t g, MergeCtx &x) {
+    uint32_t t = find_bucket(tstart, b.n_maps * 16, g);
+    x.m = t >> 4;
+    x.c = t & 15;
+    const PassMap &P = pm[x.m];
+    x.anti = class_anti(x.c, P.qt, P.qs);
+    x.cq = x.c ^ ((P.qt << 2) | P.qs);
+    const uint32_t tq = (t & ~15u) | x.cq;
+    x.A0 = phys[t];
+    x.nA = clen[t];
+    x.D0 = phys[tq];
+    x.nD = x.anti ? clen[tq] : 0;
+    x.k = g - tstart[t];
+    x.off = b.slot[x.m].off;
+    x.in_buf = P.in_buf;
+}
+
 // Tile inputs in LDS: A at [0, na), T(D) at [na, na+nd), the D element after the tile at na+nd
 // (pairs with the tile's last A element), the A element before the tile at kMergeTile+1 (its
 // pair, if any, was emitted by the previous tile).
@@ -1177,10 +1198,17 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
                                                        int plan) {
     __shared__ uint64_t s_tstart[kBlock + 1];
     __shared__ uint64_t s_wstart[kBlock];
+    __shared__ uint64_t s_phys[kBlock], s_clen[kBlock];  // per (map, class), when planning
     __shared__ unsigned scratch[8];
     const uint64_t *tstart = b.tstart, *wstart = b.wstart;
     if (plan) {
         const uint32_t M = b.n_maps, n = M * 16;  // n <= kBlock
+        if (threadIdx.x < n) {  // what locate reads from the region table, read once here
+            const uint32_t m = threadIdx.x >> 4, c = threadIdx.x & 15;
+            const uint64_t *reg = region_row(b.regions, m, pm[m].in_buf);
+            s_phys[threadIdx.x] = reg[kRegPhys + c];
+            s_clen[threadIdx.x] = reg[c + 1] - reg[c];
+        }
         const unsigned tiles = threadIdx.x < n ? class_tiles(b, pm, threadIdx.x) : 0u;
         const unsigned kind = threadIdx.x < n ? tile_kind(pm[threadIdx.x >> 4], threadIdx.x & 15)
                                               : 0u;
@@ -1222,7 +1250,10 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
         const uint64_t g = base + threadIdx.x / 8;
         if (g >= ntiles) continue;  // whole 8-lane groups leave together
         MergeCtx x;
-        locate(b, pm, tstart, g, x);
+        if (plan)
+            locate_planned(b, pm, tstart, s_phys, s_clen, g, x);
+        else
+            locate(b, pm, tstart, g, x);
         const uint64_t d0 = x.k * kMergeTile;
         uint64_t a0 = d0;
         // the higher class of an anticommuting pair is counted and written with its partner's
@@ -1514,13 +1545,21 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan1(BatchDev b, const PassMap 
     const uint64_t nchunks = b.chunk_start[M];
     for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
         const uint32_t m = find_bucket(b.chunk_start, M, ch);
+        // Loads in two dependent rounds, no branch on a loaded value before the second: the
+        // map's tile range and pass descriptor, then the extent, the chunk's counts and ledgers
+        // and (direct) the earlier tiles' counts.  The ledger fold waits until all are issued.
         const uint64_t t1 = b.tstart[m * 16 + 16], tm = b.tstart[m * 16];
+        const uint32_t flags = pm[m].flags, in_buf = pm[m].in_buf;
         const uint64_t g0 = tm + (ch - b.chunk_start[m]) * kChunkTiles;
         const uint64_t g = g0 + threadIdx.x;
+        // outputs start at the input's used extent when appending (the row is rewritten only by
+        // the write pass, after this launch)
+        const uint64_t top = region_row(b.regions, m, in_buf)[kRegTop];
         const unsigned cnt = g < t1 ? b.tile_hist[g] : 0u;
-        TileLedger L{};
-        if (g < t1) ledger_fold(L, b.tile_led[g]);
+        TileLedger led{};
+        if (g < t1) led = b.tile_led[g];
         uint64_t pre = 0;
+        const uint64_t obase = (flags & kPassAppend) ? top : 0;
         if (direct) {
             // counts of tiles tm .. g0 - 1 by 16-byte loads, entries outside the range masked
             // (tile_hist holds 16 entries per tile of the allocation: a load reaching up to 3
@@ -1545,6 +1584,8 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan1(BatchDev b, const PassMap 
                 }
             }
         }
+        TileLedger L{};
+        if (g < t1) ledger_fold(L, led);
         unsigned tot;
         const unsigned at = block_exclusive_scan(cnt, scratch, &tot);
         if (g < t1) b.tile_base[g] = at;
@@ -1555,14 +1596,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan1(BatchDev b, const PassMap 
         }
         if (direct) {
             pre = block_sum_u64(pre, sh64);
-            if (threadIdx.x == 0) {
-                const PassMap &P = pm[m];
-                // outputs start at the input's used extent when appending (the row is
-                // rewritten only by the write pass, after this launch)
-                const uint64_t obase =
-                    (P.flags & kPassAppend) ? region_row(b.regions, m, P.in_buf)[kRegTop] : 0;
-                b.chunk_base[ch] = obase + pre;
-            }
+            if (threadIdx.x == 0) b.chunk_base[ch] = obase + pre;
         }
     }
 }
@@ -1673,9 +1707,9 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan2(BatchDev b, const PassMap 
 // The map-level end of a direct rotation scan (k_rot_scan1 with direct has written every
 // chunk's offset): the written total and the ledger folded from the chunk partials in
 // k_rot_scan2's order (the same bits), then finish_map_scan.  Run by the write pass's first
-// blocks (block m for map m) before their tiles: no later block of that launch reads what it
-// writes (the region rows, lengths and records), and a map found to overflow here has its
-// tiles written only where they fit the slot, then emptied like any overflowed map.
+// blocks (block m for map m) after their tiles: no block of that launch reads what it writes
+// (the region rows, lengths and records), and a map found to overflow here has had its tiles
+// written only where they fit the slot; it is emptied like any overflowed map.
 __device__ void finish_direct(const BatchDev &b, const PassMap *pm, uint64_t *sh64,
                               TileLedger *sh) {
     for (uint32_t m = blockIdx.x; m < b.n_maps; m += gridDim.x) {
@@ -1739,17 +1773,13 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan_maps(BatchDev b, const Pass
 // are not visited.  Each thread's kept outputs (at most kMergeItems per class, consecutive in
 // the output) are held as LDS indices and go straight to their places: no LDS output staging.
 // One-word keys: at most 72 VGPRs, so seven waves per SIMD fit without spills (unconstrained
-// it takes 74 and six; eight would spill); two-word keys are left to the compiler.
+// it takes 74 and six; eight would spill); two-word keys: at most 96 and five waves (it takes
+// 98 and four unconstrained).
 template <int W>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W == 1 ? 7 : 1, 8)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W == 1 ? 7 : 5, 8)))
 void k_rot_write(BatchDev b, const PassMap *__restrict__ pm, int finish) {
     __shared__ TileIn<W> in;
     __shared__ unsigned scratch[8];
-    if (finish) {  // the map-level scan of a direct k_rot_scan1, overlapped with the tiles
-        __shared__ TileLedger sh[kBlock / 64];
-        __shared__ uint64_t sh64[kBlock / 64];
-        finish_direct(b, pm, sh64, sh);
-    }
     const uint64_t nwork = b.wstart[b.n_maps * 16 + 1];  // kind-1 and kind-2 tiles
     uint64_t gn = blockIdx.x < nwork ? b.work[blockIdx.x] : 0;
     for (uint64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -1874,6 +1904,14 @@ void k_rot_write(BatchDev b, const PassMap *__restrict__ pm, int finish) {
             }
         }
         __syncthreads();
+    }
+    // The map-level end of a direct k_rot_scan1, by the first blocks after their tiles (the
+    // tiles do not wait for it); the staged tile's LDS is free again for its reductions.
+    if (finish) {
+        static_assert(sizeof(TileIn<W>) >= (kBlock / 64) * (sizeof(TileLedger) + 8), "room");
+        TileLedger *sh = reinterpret_cast<TileLedger *>(&in);
+        uint64_t *sh64 = reinterpret_cast<uint64_t *>(sh + kBlock / 64);
+        finish_direct(b, pm, sh64, sh);
     }
 }
 
