@@ -1561,28 +1561,34 @@ __global__ __launch_bounds__(kBlock) void k_rot_scan1(BatchDev b, const PassMap 
         uint64_t pre = 0;
         const uint64_t obase = (flags & kPassAppend) ? top : 0;
         if (direct) {
-            // counts of tiles tm .. g0 - 1 by 16-byte loads, entries outside the range masked
-            // (tile_hist holds 16 entries per tile of the allocation: a load reaching up to 3
-            // entries past g0 stays inside)
+            // counts of tiles tm .. g0 - 1: the 16-byte words qa .. qz - 1 that cover them are
+            // summed whole, and thread 0 takes off the at most 3 entries before tm and 3 from g0
+            // on (tile_hist holds 16 entries per tile of the allocation, so the words and entries
+            // up to g0 + 2 are inside it)
             const uint4 *h4 = reinterpret_cast<const uint4 *>(b.tile_hist);
-            const uint64_t q1 = (g0 + 3) >> 2;
-            constexpr int R = 16;
-            for (uint64_t qb = tm >> 2; qb < q1; qb += (uint64_t)R * kBlock) {
+            const uint64_t qa = tm >> 2, qz = (g0 + 3) >> 2;
+            unsigned edge = 0;
+            if (threadIdx.x == 0) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const uint64_t jh = qa * 4 + k, jt = g0 + k;
+                    edge += (jh < tm ? b.tile_hist[jh] : 0u) + (jt < qz * 4 ? b.tile_hist[jt] : 0u);
+                }
+            }
+            constexpr int R = 16;  // one round of loads for maps of up to 64 chunks
+            for (uint64_t qb = qa; qb < qz; qb += (uint64_t)R * kBlock) {
                 uint4 v[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const uint64_t q = qb + (uint64_t)r * kBlock + threadIdx.x;
-                    v[r] = q < q1 ? h4[q] : make_uint4(0u, 0u, 0u, 0u);
+                    v[r] = q < qz ? h4[q] : make_uint4(0u, 0u, 0u, 0u);
                 }
+                unsigned s32 = 0;  // <= 16 words of 4 counts <= 2048
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const uint64_t j = (qb + (uint64_t)r * kBlock + threadIdx.x) * 4;
-                    pre += (j >= tm && j < g0 ? v[r].x : 0u) +
-                           (j + 1 >= tm && j + 1 < g0 ? v[r].y : 0u) +
-                           (j + 2 >= tm && j + 2 < g0 ? v[r].z : 0u) +
-                           (j + 3 >= tm && j + 3 < g0 ? v[r].w : 0u);
-                }
+                for (int r = 0; r < R; ++r) s32 += v[r].x + v[r].y + v[r].z + v[r].w;
+                pre += s32;
             }
+            pre -= edge;  // modulo 2^64: the block sum is exact
         }
         TileLedger L{};
         if (g < t1) ledger_fold(L, led);
