@@ -765,10 +765,14 @@ __device__ __forceinline__ unsigned class_tiles(const BatchDev &b, const PassMap
 // Kind-3 classes keep their (empty) tiles so that the ledger partials fold in the same chunks as
 // when the class is copied: the upward-rounded ledger sums do not depend on whether a rotation
 // appends.
+__device__ __forceinline__ unsigned tile_kind(uint32_t qt, uint32_t qs, uint32_t flags,
+                                              uint32_t c) {
+    if (class_anti(c, qt, qs)) return (c ^ ((qt << 2) | qs)) > c ? 1u : 0u;
+    if (!(flags & kPassCommutingKept)) return 1u;
+    return (flags & kPassAppend) ? 3u : 2u;
+}
 __device__ __forceinline__ unsigned tile_kind(const PassMap &P, uint32_t c) {
-    if (class_anti(c, P.qt, P.qs)) return (c ^ ((P.qt << 2) | P.qs)) > c ? 1u : 0u;
-    if (!(P.flags & kPassCommutingKept)) return 1u;
-    return (P.flags & kPassAppend) ? 3u : 2u;
+    return tile_kind(P.qt, P.qs, P.flags, c);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rot_plan(BatchDev b, const PassMap *pm) {
@@ -853,25 +857,31 @@ __device__ __forceinline__ void locate(const BatchDev &b, const PassMap *pm,
     x.in_buf = P.in_buf;
 }
 
-// locate, with the classes' physical starts and lengths (per (map, class): phys, clen) and the
-// tile offsets already in LDS: no region-table loads.
-__device__ __forceinline__ void locate_planned(const BatchDev &b, const PassMap *pm,
-                                               const uint64_t *tstart, const uint64_t *phys,
-                                               const uint64_t *clen, uint64_t g, MergeCtx &x) {
+// What the split pass keeps in LDS per map while planning.
+struct PlanMap {
+    uint64_t off;
+    uint32_t in_buf, qt, qs, flags, n_spins, pad;
+};
+
+// locate from LDS alone: the tile offsets, the classes' physical starts and lengths (per (map,
+// class): phys, clen) and the maps' pass fields, all read while planning.
+__device__ __forceinline__ void locate_planned(const BatchDev &b, const uint64_t *tstart,
+                                               const uint64_t *phys, const uint64_t *clen,
+                                               const PlanMap *maps, uint64_t g, MergeCtx &x) {
     uint32_t t = find_bucket(tstart, b.n_maps * 16, g);
     x.m = t >> 4;
     x.c = t & 15;
-    const PassMap &P = pm[x.m];
-    x.anti = class_anti(x.c, P.qt, P.qs);
-    x.cq = x.c ^ ((P.qt << 2) | P.qs);
+    const PlanMap &Q = maps[x.m];
+    x.anti = class_anti(x.c, Q.qt, Q.qs);
+    x.cq = x.c ^ ((Q.qt << 2) | Q.qs);
     const uint32_t tq = (t & ~15u) | x.cq;
     x.A0 = phys[t];
     x.nA = clen[t];
     x.D0 = phys[tq];
     x.nD = x.anti ? clen[tq] : 0;
     x.k = g - tstart[t];
-    x.off = b.slot[x.m].off;
-    x.in_buf = P.in_buf;
+    x.off = Q.off;
+    x.in_buf = Q.in_buf;
 }
 
 // Tile inputs in LDS: A at [0, na), T(D) at [na, na+nd), the D element after the tile at na+nd
@@ -1199,19 +1209,33 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
     __shared__ uint64_t s_tstart[kBlock + 1];
     __shared__ uint64_t s_wstart[kBlock];
     __shared__ uint64_t s_phys[kBlock], s_clen[kBlock];  // per (map, class), when planning
+    __shared__ PlanMap s_map[kPlanMaps];
     __shared__ unsigned scratch[8];
     const uint64_t *tstart = b.tstart, *wstart = b.wstart;
     if (plan) {
         const uint32_t M = b.n_maps, n = M * 16;  // n <= kBlock
-        if (threadIdx.x < n) {  // what locate reads from the region table, read once here
+        // what class_tiles, tile_kind and locate read, in two rounds of loads (the pass fields
+        // and slots, then the region table); kept in LDS for the tiles
+        unsigned tiles = 0, kind = 0;
+        if (threadIdx.x < n) {
             const uint32_t m = threadIdx.x >> 4, c = threadIdx.x & 15;
-            const uint64_t *reg = region_row(b.regions, m, pm[m].in_buf);
+            const PassMap P = pm[m];
+            const uint64_t off = b.slot[m].off;
+            const uint32_t n_spins = b.slot[m].n_spins;
+            const unsigned cq = c ^ ((P.qt << 2) | P.qs);
+            const bool anti = class_anti(c, P.qt, P.qs);
+            const uint64_t *reg = region_row(b.regions, m, P.in_buf);
+            const uint64_t r0 = reg[c], r1 = reg[c + 1], q0 = reg[cq], q1 = reg[cq + 1];
             s_phys[threadIdx.x] = reg[kRegPhys + c];
-            s_clen[threadIdx.x] = reg[c + 1] - reg[c];
+            s_clen[threadIdx.x] = r1 - r0;
+            // the map's 16 threads store the same record (no branch for the loads to sink into)
+            s_map[m] = PlanMap{off, P.in_buf, P.qt, P.qs, P.flags, n_spins, 0};
+            // class_tiles (lengths masked, not branched on: their loads stay in the second round)
+            const uint64_t nd = (q1 - q0) & (anti ? ~0ull : 0ull);
+            const uint64_t len = (r1 - r0 + nd) & (P.active ? ~0ull : 0ull);
+            tiles = (unsigned)((len + kMergeTile - 1) / kMergeTile);
+            kind = tile_kind(P, c);
         }
-        const unsigned tiles = threadIdx.x < n ? class_tiles(b, pm, threadIdx.x) : 0u;
-        const unsigned kind = threadIdx.x < n ? tile_kind(pm[threadIdx.x >> 4], threadIdx.x & 15)
-                                              : 0u;
         unsigned tot, t1, t2;
         const unsigned ex = block_exclusive_scan(tiles, scratch, &tot);
         const unsigned x1 = block_exclusive_scan(kind == 1 ? tiles : 0u, scratch, &t1);
@@ -1250,19 +1274,29 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
         const uint64_t g = base + threadIdx.x / 8;
         if (g >= ntiles) continue;  // whole 8-lane groups leave together
         MergeCtx x;
-        if (plan)
-            locate_planned(b, pm, tstart, s_phys, s_clen, g, x);
-        else
+        uint32_t n_sp, qt, qs, flags;
+        if (plan) {
+            locate_planned(b, tstart, s_phys, s_clen, s_map, g, x);
+            const PlanMap &Q = s_map[x.m];
+            n_sp = Q.n_spins;
+            qt = Q.qt;
+            qs = Q.qs;
+            flags = Q.flags;
+        } else {
             locate(b, pm, tstart, g, x);
+            const PassMap &P = pm[x.m];
+            n_sp = b.slot[x.m].n_spins;
+            qt = P.qt;
+            qs = P.qs;
+            flags = P.flags;
+        }
         const uint64_t d0 = x.k * kMergeTile;
         uint64_t a0 = d0;
         // the higher class of an anticommuting pair is counted and written with its partner's
         // tiles and splits: its own are never read
         if (x.anti && x.c < x.cq) {
-            const PassMap &P = pm[x.m];
-            const uint32_t n = b.slot[x.m].n_spins;
-            const uint64_t *ikeys = b.arena[P.in_buf].keys + b.slot[x.m].off * W;
-            const Key<W> q = kxor<W>(kdigit_mask<W>(n - 1, P.qt), kdigit_mask<W>(n - 2, P.qs));
+            const uint64_t *ikeys = b.arena[x.in_buf].keys + x.off * W;
+            const Key<W> q = kxor<W>(kdigit_mask<W>(n_sp - 1, qt), kdigit_mask<W>(n_sp - 2, qs));
             a0 = split_search8<W>(ikeys, x, q, d0, grp, sub);
         }
         if (sub == 0) {
@@ -1282,7 +1316,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_splits(BatchDev b, const PassMap
             const bool last = g + 1 >= tstart[x.m * 16 + x.c + 1];
             t.flags = (x.anti ? 1u : 0u) | (last ? 2u : 0u);
             b.tdesc[g] = t;
-            const unsigned kind = tile_kind(pm[x.m], x.c);
+            const unsigned kind = tile_kind(qt, qs, flags, x.c);
             if (kind == 1 || kind == 2) b.work[wstart[x.m * 16 + x.c] + x.k] = g;
             if (kind >= 2) {  // unchanged keys: the count pass does not visit the tile; copied
                               // (kind 2) or left in place with no output of its own (kind 3)
